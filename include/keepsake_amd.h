@@ -1,0 +1,222 @@
+/*
+ * keepsake_amd: MI355X (gfx950) implementation of keepsake's small_pt path tracer.
+ *
+ * C ABI of the GPU library libkeepsake_amd.so. Plain pointers and sizes only; every function returns
+ * an int status (KS_OK = 0, negative = error class) and never throws or aborts across the ABI.
+ * ks_last_error() returns a thread-local message for the last failing call on the calling thread.
+ *
+ * The reference has no FFI for this path; its "operator API" is the C++ task/type surface listed next
+ * to each entry point below. The C++ wrapper (include/ks/small_pt.h) restores the reference's C++
+ * names and its failure behaviour (a non-zero status aborts, as the reference's ASSERT does).
+ *
+ * Threading: one host thread drives one HIP stream per device handle; calls on distinct handles are
+ * thread-safe, calls on one handle are not re-entrant.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KS_OK 0
+#define KS_ERR_INVALID (-1)     /* bad argument (the reference would ASSERT or std::abort) */
+#define KS_ERR_HIP (-2)         /* HIP runtime failure */
+#define KS_ERR_OOM (-3)         /* device or host allocation failed */
+#define KS_ERR_UNSUPPORTED (-4) /* feature outside this build's scope */
+#define KS_ERR_NO_DEVICE (-5)   /* no HIP device / library loaded without a GPU */
+
+typedef struct ks_device ks_device;
+typedef struct ks_scene ks_scene;
+typedef struct ks_film ks_film;
+typedef struct ks_host_scene ks_host_scene;
+
+/* Closure inputs of PrincipledBSDF (principled_bsdf.h:78-93, fields evaluated by
+ * PrincipledBSDF::eval_closure principled_bsdf.cpp:246-275) plus Material::emission (material.h:45).
+ * All fields are constant per material (ConstantField, shader_field.h:37-60). */
+typedef struct ks_material_desc {
+    float basecolor[3];
+    float roughness;
+    float metallic;
+    float ior;
+    float specular_r0_mul; /* field value in [0,1]; the closure doubles it */
+    float specular_trans;
+    float diffuse_trans;
+    float diffuse_trans_fwd;
+    float emission[3];
+    int32_t has_emission;
+} ks_material_desc;
+
+#define KS_LIGHT_SKY_AMBIENT 0 /* SkyLight(const color3 &ambient) light.cpp:52-57 */
+#define KS_LIGHT_DIRECTIONAL 1 /* DirectionalLight light.h:91-105 */
+#define KS_LIGHT_POINT 2       /* PointLight light.h:107-120 */
+
+/* Non-mesh lights as created by create_light/create_sky_light (light.cpp:237-265). */
+typedef struct ks_light_desc {
+    int32_t type;
+    float color[3]; /* ambient radiance, L, or I */
+    float vec[3];   /* direction (directional) or position (point) */
+} ks_light_desc;
+
+/* Flat scene: the geometry Scene::intersect1/occlude1 see (scene.h:49-87) with an identity instance
+ * transform and one material per geometry (SubScene::materials, scene.h:37). Triangles of one geometry
+ * are contiguous and geometries appear in increasing id order (mesh-light order, scene.cpp:315-336).
+ * All meshes are two-sided (MeshData::twosided default, geometry.h:84). */
+typedef struct ks_scene_desc {
+    const float *vertices; /* 3 * n_vertices */
+    uint32_t n_vertices;
+    const float *normals; /* optional smooth-shading normals, 3 * n_vertices, or NULL */
+    const uint32_t *indices; /* 3 * n_tris */
+    uint32_t n_tris;
+    const uint32_t *tri_geom; /* geometry id of each triangle */
+    const uint32_t *geom_material; /* material of each geometry */
+    uint32_t n_geoms;
+    const ks_material_desc *materials;
+    uint32_t n_materials;
+    const ks_light_desc *lights;
+    uint32_t n_lights;
+} ks_scene_desc;
+
+/* SBVHBuildOption (cuda/sbvh.cuh:9-16). Zeroed fields take the reference defaults
+ * (maxPrimsInNode 4, bucketCount 12, travWeight 1, alpha 1e-5). */
+typedef struct ks_bvh_options {
+    uint32_t max_prims_in_node;
+    uint32_t bucket_count;
+    float trav_weight;
+    float alpha;
+    int32_t threads; /* host build threads, 0 = all */
+} ks_bvh_options;
+
+/* create_camera (camera.cpp:172-205) perspective branch: look_at(pos, target, up) with vfov in
+ * degrees and aspect; when use_matrix != 0, to_world (row-major 4x4 camera-to-world) replaces look_at. */
+typedef struct ks_camera_desc {
+    float pos[3];
+    float target[3];
+    float up[3];
+    float vfov_deg;
+    float aspect;
+    int32_t use_matrix;
+    float to_world[16];
+} ks_camera_desc;
+
+#define KS_MIS_REFERENCE 0  /* NEE weight without light-selection probability (nee.cpp:49) */
+#define KS_MIS_CONSISTENT 1 /* both strategies weighted with the selection probability */
+
+#define KS_GUIDE_OFF 0
+#define KS_GUIDE_TRAIN_AND_SAMPLE 1 /* online-trained vMF-mixture guide, one-sample MIS with the BSDF */
+
+/* SmallPTInput (small_pt.h:17-41) plus this build's additions (mis_mode, guide_mode, queue size). */
+typedef struct ks_small_pt_params {
+    int32_t bounces;
+    int32_t render_width, render_height;
+    int32_t crop_start_x, crop_start_y, crop_width, crop_height;
+    int32_t spp;
+    int32_t rng_seed;
+    int32_t include_background;
+    float backdrop[3];
+    int32_t scale_ray_diff; /* accepted for compatibility; ray differentials are unused (no textures) */
+    float clamp_direct;
+    float clamp_indirect;
+    int32_t spp_prog_interval;
+    int32_t mis_mode;
+    int32_t guide_mode;
+    int32_t queue_size; /* wavefront path slots; 0 = automatic */
+} ks_small_pt_params;
+
+/* Kernel classes timed with HIP events on the render stream. */
+#define KS_K_RAYGEN 0
+#define KS_K_CLOSEST 1
+#define KS_K_SHADE 2
+#define KS_K_SHADOW 3
+#define KS_K_FILM 4
+#define KS_K_GUIDE 5
+#define KS_K_COUNT 6
+
+typedef struct ks_render_stats {
+    double wall_ms;                     /* host wall time of the render call */
+    double kernel_ms[KS_K_COUNT];       /* summed event time per kernel class (0 unless timing on) */
+    uint64_t launches[KS_K_COUNT];
+    uint64_t paths;                     /* camera samples traced (W*H*spp over the crop) */
+    uint64_t rays_closest;              /* primary + extension rays */
+    uint64_t rays_shadow;               /* NEE shadow rays */
+    uint64_t node_visits_closest, tri_tests_closest;
+    uint64_t node_visits_shadow, tri_tests_shadow;
+    uint64_t nonfinite;                 /* non-finite throughput events (thread_monitor_check, small_pt.cpp:97) */
+    uint64_t stack_overflows;           /* traversal stack entries spilled past LDS */
+    int32_t iterations;
+    int32_t queue_size;
+} ks_render_stats;
+
+typedef void (*ks_progress_fn)(const ks_film *film, int32_t spp_finished, void *user);
+
+/* ---- device ---------------------------------------------------------------------------------- */
+const char *ks_last_error(void);
+int ks_device_count(int32_t *count);
+int ks_device_open(int32_t ordinal, ks_device **out);
+int ks_device_close(ks_device *dev);
+/* Record HIP-event kernel timings into ks_render_stats (costs one event pair per launch). */
+int ks_device_set_timing(ks_device *dev, int32_t enable);
+
+/* ---- scene: replaces create_scene + Scene::create_rtc_scene (rtcCommitScene, scene.cpp:144-160),
+ *      Scene::build_mesh_lights (scene.cpp:315-336) and PowerLightSampler::build (light.cpp:636-677).
+ *      Builds the SBVH on the host (cuda/sbvh.cu:301-440) and uploads scene + BVH + light tables. ---- */
+int ks_scene_create(ks_device *dev, const ks_scene_desc *desc, const ks_bvh_options *opt, ks_scene **out);
+int ks_scene_free(ks_scene *scene);
+
+typedef struct ks_scene_info {
+    uint32_t n_tris, n_vertices, n_nodes, n_refs, n_lights, max_depth;
+    uint64_t device_bytes;
+    double build_ms;
+    float bound_lo[3], bound_hi[3]; /* Scene::bound (scene.cpp:162-169) */
+} ks_scene_info;
+int ks_scene_info_get(const ks_scene *scene, ks_scene_info *info);
+/* PowerLightSampler tables in flat light order: prim (-1 for non-mesh), power, cdf (n+1). */
+int ks_scene_light_table(const ks_scene *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf);
+
+/* ---- film: RenderTarget2 (render_target.h:224-293): FP64 RGB sums + FP64 (hit, miss) weights ---- */
+int ks_film_create(ks_device *dev, int32_t width, int32_t height, const float backdrop[3], ks_film **out);
+int ks_film_clear(ks_film *film);                                  /* RenderTarget2::clear */
+int ks_film_download(const ks_film *film, double *rgb, double *weights); /* W*H*3, W*H*2 */
+int ks_film_composite(const ks_film *film, float *rgb);            /* composite (render_target.cpp:187-192) */
+int ks_film_free(ks_film *film);
+
+/* ---- the render call: SmallPT::run (small_pt.cpp:15-70). Accumulates into `film` (which must be
+ *      render_width x render_height); calls progress after each spp batch of spp_prog_interval on the
+ *      calling thread (small_pt.cpp:66-68). ---- */
+int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params,
+                       ks_film *film, ks_progress_fn progress, void *user, ks_render_stats *stats);
+
+/* ---- component entry points (parity tests; same kernels as the render path) ---- */
+/* PTRenderSampler (sobol.h:235-248) streams for n (pixel, sample) pairs; kinds[c] = 1 (next) or
+ * 2 (next2d); out holds n * n_values values (uint32 before the float map, and the float). */
+int ks_sobol_stream(ks_device *dev, int32_t width, int32_t height, int32_t spp, int32_t seed, int64_t n,
+                    const int32_t *px, const int32_t *py, const int32_t *sample, int32_t n_calls,
+                    const int32_t *kinds, uint32_t *out_u32, float *out_f);
+/* Camera::spawn_ray after the pixel-filter jitter (small_pt.cpp:98-105); rays n x 6 (o, d). */
+int ks_camera_rays(ks_device *dev, const ks_camera_desc *cam, int32_t width, int32_t height, int32_t spp,
+                   int32_t seed, int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample,
+                   float *rays);
+/* Scene::intersect1 (scene.cpp:206-236): rays n x 8 (o.xyz, tmin, d.xyz, tmax) ->
+ * hits n x 4 (t, u, v, prim id as int32 bits, -1 = miss); u, v are Embree's barycentrics. */
+int ks_intersect(ks_scene *scene, const float *rays, int64_t n, float *hits);
+/* Scene::occlude1 (scene.cpp:238-244) */
+int ks_occluded(ks_scene *scene, const float *rays, int64_t n, uint8_t *occluded);
+/* PrincipledBSDF::internal::eval/pdf (principled_bsdf.cpp:308-421) on local directions. */
+int ks_bsdf_eval(ks_device *dev, const ks_material_desc *m, const float *wo, const float *wi, int64_t n,
+                 float *f, float *pdf);
+/* PrincipledBSDF::internal::sample (principled_bsdf.cpp:326-393); u: n x 3 (u_lobe, u_wi). */
+int ks_bsdf_sample(ks_device *dev, const ks_material_desc *m, const float *wo, const float *u, int64_t n,
+                   float *wi, float *beta, float *pdf);
+/* SmallPT::trace (small_pt.cpp:72-178) for a list of (pixel, sample): out n x 4 (L.rgb, hit). */
+int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
+                     const int32_t *px, const int32_t *py, const int32_t *sample, float *out);
+
+/* ---- procedural scenes of the benchmark configurations (no reference counterpart: the reference
+ *      loads meshes from files, mesh_asset.cpp). name: "cornell", "spheres", "room". ---- */
+int ks_host_scene_create(const char *name, uint64_t seed, ks_host_scene **out);
+int ks_host_scene_get(const ks_host_scene *hs, ks_scene_desc *desc, ks_camera_desc *cam);
+int ks_host_scene_free(ks_host_scene *hs);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,584 @@
+"""ctypes bindings of libkeepsake_amd (include/keepsake_amd.h).
+
+Python-side mirror of the reference's small_pt entry points for tests and benchmarks. Every call goes
+through the C ABI of the gfx950 library; there is no CPU fallback: loading fails loudly when the
+library is missing, and opening a device fails loudly when no GPU is present.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libkeepsake_amd.so"
+
+KS_OK = 0
+KS_ERR_INVALID = -1
+KS_ERR_HIP = -2
+KS_ERR_OOM = -3
+KS_ERR_UNSUPPORTED = -4
+KS_ERR_NO_DEVICE = -5
+
+KS_LIGHT_SKY_AMBIENT = 0
+KS_LIGHT_DIRECTIONAL = 1
+KS_LIGHT_POINT = 2
+KS_MIS_REFERENCE = 0
+KS_MIS_CONSISTENT = 1
+KS_GUIDE_OFF = 0
+KS_GUIDE_TRAIN_AND_SAMPLE = 1
+
+KERNEL_CLASSES = ("raygen", "closest", "shade", "shadow", "film", "guide")
+
+
+class KsError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"keepsake_amd error {status}: {message}")
+        self.status = status
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [
+        ("basecolor", C.c_float * 3),
+        ("roughness", C.c_float),
+        ("metallic", C.c_float),
+        ("ior", C.c_float),
+        ("specular_r0_mul", C.c_float),
+        ("specular_trans", C.c_float),
+        ("diffuse_trans", C.c_float),
+        ("diffuse_trans_fwd", C.c_float),
+        ("emission", C.c_float * 3),
+        ("has_emission", C.c_int32),
+    ]
+
+
+class LightDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("vec", C.c_float * 3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("vertices", C.POINTER(C.c_float)),
+        ("n_vertices", C.c_uint32),
+        ("normals", C.POINTER(C.c_float)),
+        ("indices", C.POINTER(C.c_uint32)),
+        ("n_tris", C.c_uint32),
+        ("tri_geom", C.POINTER(C.c_uint32)),
+        ("geom_material", C.POINTER(C.c_uint32)),
+        ("n_geoms", C.c_uint32),
+        ("materials", C.POINTER(MaterialDesc)),
+        ("n_materials", C.c_uint32),
+        ("lights", C.POINTER(LightDesc)),
+        ("n_lights", C.c_uint32),
+    ]
+
+
+class BvhOptions(C.Structure):
+    _fields_ = [
+        ("max_prims_in_node", C.c_uint32),
+        ("bucket_count", C.c_uint32),
+        ("trav_weight", C.c_float),
+        ("alpha", C.c_float),
+        ("threads", C.c_int32),
+    ]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("target", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("vfov_deg", C.c_float),
+        ("aspect", C.c_float),
+        ("use_matrix", C.c_int32),
+        ("to_world", C.c_float * 16),
+    ]
+
+
+class SmallPTParams(C.Structure):
+    _fields_ = [
+        ("bounces", C.c_int32),
+        ("render_width", C.c_int32),
+        ("render_height", C.c_int32),
+        ("crop_start_x", C.c_int32),
+        ("crop_start_y", C.c_int32),
+        ("crop_width", C.c_int32),
+        ("crop_height", C.c_int32),
+        ("spp", C.c_int32),
+        ("rng_seed", C.c_int32),
+        ("include_background", C.c_int32),
+        ("backdrop", C.c_float * 3),
+        ("scale_ray_diff", C.c_int32),
+        ("clamp_direct", C.c_float),
+        ("clamp_indirect", C.c_float),
+        ("spp_prog_interval", C.c_int32),
+        ("mis_mode", C.c_int32),
+        ("guide_mode", C.c_int32),
+        ("queue_size", C.c_int32),
+    ]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [
+        ("wall_ms", C.c_double),
+        ("kernel_ms", C.c_double * 6),
+        ("launches", C.c_uint64 * 6),
+        ("paths", C.c_uint64),
+        ("rays_closest", C.c_uint64),
+        ("rays_shadow", C.c_uint64),
+        ("node_visits_closest", C.c_uint64),
+        ("tri_tests_closest", C.c_uint64),
+        ("node_visits_shadow", C.c_uint64),
+        ("tri_tests_shadow", C.c_uint64),
+        ("nonfinite", C.c_uint64),
+        ("stack_overflows", C.c_uint64),
+        ("iterations", C.c_int32),
+        ("queue_size", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("kernel_ms", "launches")}
+        d["kernel_ms"] = {n: self.kernel_ms[i] for i, n in enumerate(KERNEL_CLASSES)}
+        d["launches"] = {n: int(self.launches[i]) for i, n in enumerate(KERNEL_CLASSES)}
+        return d
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("n_tris", C.c_uint32),
+        ("n_vertices", C.c_uint32),
+        ("n_nodes", C.c_uint32),
+        ("n_refs", C.c_uint32),
+        ("n_lights", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("device_bytes", C.c_uint64),
+        ("build_ms", C.c_double),
+        ("bound_lo", C.c_float * 3),
+        ("bound_hi", C.c_float * 3),
+    ]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p)
+
+_P = C.c_void_p
+_SIGS = {
+    "ks_last_error": ([], C.c_char_p),
+    "ks_device_count": ([C.POINTER(C.c_int32)], C.c_int),
+    "ks_device_open": ([C.c_int32, C.POINTER(_P)], C.c_int),
+    "ks_device_close": ([_P], C.c_int),
+    "ks_device_set_timing": ([_P, C.c_int32], C.c_int),
+    "ks_scene_create": ([_P, C.POINTER(SceneDesc), C.POINTER(BvhOptions), C.POINTER(_P)], C.c_int),
+    "ks_scene_free": ([_P], C.c_int),
+    "ks_scene_info_get": ([_P, C.POINTER(SceneInfo)], C.c_int),
+    "ks_scene_light_table": ([_P, C.POINTER(C.c_int32), _P, _P, _P], C.c_int),
+    "ks_film_create": ([_P, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(_P)], C.c_int),
+    "ks_film_clear": ([_P], C.c_int),
+    "ks_film_download": ([_P, _P, _P], C.c_int),
+    "ks_film_composite": ([_P, _P], C.c_int),
+    "ks_film_free": ([_P], C.c_int),
+    "ks_small_pt_render": (
+        [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), _P, PROGRESS_FN, _P, C.POINTER(RenderStats)],
+        C.c_int,
+    ),
+    "ks_sobol_stream": (
+        [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _P, _P, _P, C.c_int32, _P, _P, _P],
+        C.c_int,
+    ),
+    "ks_camera_rays": (
+        [_P, C.POINTER(CameraDesc), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64, _P, _P, _P, _P],
+        C.c_int,
+    ),
+    "ks_intersect": ([_P, _P, C.c_int64, _P], C.c_int),
+    "ks_occluded": ([_P, _P, C.c_int64, _P], C.c_int),
+    "ks_bsdf_eval": ([_P, C.POINTER(MaterialDesc), _P, _P, C.c_int64, _P, _P], C.c_int),
+    "ks_bsdf_sample": ([_P, C.POINTER(MaterialDesc), _P, _P, C.c_int64, _P, _P, _P], C.c_int),
+    "ks_trace_samples": (
+        [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), C.c_int64, _P, _P, _P, _P],
+        C.c_int,
+    ),
+    "ks_host_scene_create": ([C.c_char_p, C.c_uint64, C.POINTER(_P)], C.c_int),
+    "ks_host_scene_get": ([_P, C.POINTER(SceneDesc), C.POINTER(CameraDesc)], C.c_int),
+    "ks_host_scene_free": ([_P], C.c_int),
+}
+EXPORTED = tuple(_SIGS)
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Loads libkeepsake_amd.so; raises FileNotFoundError when it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is missing: build it with `python -m keepsake_amd.build`")
+    lib = C.CDLL(str(p))
+    for name, (args, res) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != KS_OK:
+        msg = _lib.ks_last_error().decode(errors="replace") if _lib else "?"
+        raise KsError(status, msg)
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def device_count() -> int:
+    lib = load_library()
+    n = C.c_int32(0)
+    _check(lib.ks_device_count(C.byref(n)))
+    return n.value
+
+
+# ------------------------------------------------------------------------------------------------
+# Scene description helpers
+# ------------------------------------------------------------------------------------------------
+MATERIAL_DTYPE = np.dtype(
+    [
+        ("basecolor", "<f4", 3),
+        ("roughness", "<f4"),
+        ("metallic", "<f4"),
+        ("ior", "<f4"),
+        ("specular_r0_mul", "<f4"),
+        ("specular_trans", "<f4"),
+        ("diffuse_trans", "<f4"),
+        ("diffuse_trans_fwd", "<f4"),
+        ("emission", "<f4", 3),
+        ("has_emission", "<i4"),
+    ]
+)
+LIGHT_DTYPE = np.dtype([("type", "<i4"), ("color", "<f4", 3), ("vec", "<f4", 3)])
+assert MATERIAL_DTYPE.itemsize == C.sizeof(MaterialDesc)
+assert LIGHT_DTYPE.itemsize == C.sizeof(LightDesc)
+
+
+@dataclass
+class HostScene:
+    """Flat scene arrays (the ks_scene_desc contents) plus the camera of the scene."""
+
+    vertices: np.ndarray  # (V, 3) float32
+    indices: np.ndarray  # (T, 3) uint32
+    tri_geom: np.ndarray  # (T,) uint32
+    geom_material: np.ndarray  # (G,) uint32
+    materials: np.ndarray  # MATERIAL_DTYPE
+    lights: np.ndarray  # LIGHT_DTYPE
+    camera: dict
+    normals: np.ndarray | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.indices.shape[0])
+
+    def desc(self) -> SceneDesc:
+        arrs = [
+            np.ascontiguousarray(self.vertices, np.float32),
+            np.ascontiguousarray(self.indices, np.uint32),
+            np.ascontiguousarray(self.tri_geom, np.uint32),
+            np.ascontiguousarray(self.geom_material, np.uint32),
+            np.ascontiguousarray(self.materials, MATERIAL_DTYPE),
+            np.ascontiguousarray(self.lights, LIGHT_DTYPE),
+        ]
+        nrm = None if self.normals is None else np.ascontiguousarray(self.normals, np.float32)
+        self._keep = arrs + [nrm]
+        v, idx, tg, gm, mats, lights = arrs
+        d = SceneDesc()
+        d.vertices = v.ctypes.data_as(C.POINTER(C.c_float))
+        d.n_vertices = v.shape[0]
+        d.normals = nrm.ctypes.data_as(C.POINTER(C.c_float)) if nrm is not None else None
+        d.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.n_tris = idx.shape[0]
+        d.tri_geom = tg.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.geom_material = gm.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.n_geoms = gm.shape[0]
+        d.materials = C.cast(mats.ctypes.data, C.POINTER(MaterialDesc))
+        d.n_materials = mats.shape[0]
+        d.lights = C.cast(lights.ctypes.data, C.POINTER(LightDesc)) if lights.shape[0] else None
+        d.n_lights = lights.shape[0]
+        return d
+
+
+def camera_desc(cam: dict) -> CameraDesc:
+    c = CameraDesc()
+    if "to_world" in cam:
+        c.use_matrix = 1
+        m = np.asarray(cam["to_world"], np.float32).reshape(16)
+        for i in range(16):
+            c.to_world[i] = float(m[i])
+    else:
+        for i in range(3):
+            c.pos[i] = float(cam["pos"][i])
+            c.target[i] = float(cam["target"][i])
+            c.up[i] = float(cam.get("up", (0.0, 1.0, 0.0))[i])
+    c.vfov_deg = float(cam["vfov"])
+    c.aspect = float(cam["aspect"])
+    return c
+
+
+def procedural_scene(name: str, seed: int = 0) -> HostScene:
+    """Builds one of the benchmark scenes ("cornell", "spheres", "spheres_small", "room")."""
+    lib = load_library()
+    h = C.c_void_p()
+    _check(lib.ks_host_scene_create(name.encode(), seed, C.byref(h)))
+    try:
+        d = SceneDesc()
+        cam = CameraDesc()
+        _check(lib.ks_host_scene_get(h, C.byref(d), C.byref(cam)))
+
+        def arr(ptr, n, dt):
+            if n == 0 or not ptr:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_byte)), shape=(n * np.dtype(dt).itemsize,)).view(
+                dt
+            ).copy()
+
+        verts = arr(d.vertices, d.n_vertices * 3, np.float32).reshape(-1, 3)
+        normals = arr(d.normals, d.n_vertices * 3, np.float32).reshape(-1, 3) if d.normals else None
+        idx = arr(d.indices, d.n_tris * 3, np.uint32).reshape(-1, 3)
+        tg = arr(d.tri_geom, d.n_tris, np.uint32)
+        gm = arr(d.geom_material, d.n_geoms, np.uint32)
+        mats = arr(d.materials, d.n_materials, MATERIAL_DTYPE)
+        lights = arr(d.lights, d.n_lights, LIGHT_DTYPE)
+        camera = {
+            "pos": tuple(cam.pos),
+            "target": tuple(cam.target),
+            "up": tuple(cam.up),
+            "vfov": cam.vfov_deg,
+            "aspect": cam.aspect,
+        }
+        return HostScene(verts, idx, tg, gm, mats, lights, camera, normals)
+    finally:
+        lib.ks_host_scene_free(h)
+
+
+def small_pt_params(**kw) -> SmallPTParams:
+    """SmallPTInput defaults (small_pt.h:17-41) with keyword overrides."""
+    p = SmallPTParams()
+    p.bounces = kw.get("bounces", 1)
+    p.render_width = kw.get("render_width", 256)
+    p.render_height = kw.get("render_height", 256)
+    p.crop_start_x = kw.get("crop_start_x", 0)
+    p.crop_start_y = kw.get("crop_start_y", 0)
+    p.crop_width = kw.get("crop_width", 0)
+    p.crop_height = kw.get("crop_height", 0)
+    p.spp = kw.get("spp", 1)
+    p.rng_seed = kw.get("rng_seed", 0)
+    p.include_background = int(kw.get("include_background", True))
+    bd = kw.get("backdrop", (0.0, 0.0, 0.0))
+    for i in range(3):
+        p.backdrop[i] = float(bd[i])
+    p.scale_ray_diff = int(kw.get("scale_ray_diff", True))
+    p.clamp_direct = kw.get("clamp_direct", 0.0)
+    p.clamp_indirect = kw.get("clamp_indirect", 10.0)
+    p.spp_prog_interval = kw.get("spp_prog_interval", 32)
+    p.mis_mode = kw.get("mis_mode", KS_MIS_REFERENCE)
+    p.guide_mode = kw.get("guide_mode", KS_GUIDE_OFF)
+    p.queue_size = kw.get("queue_size", 0)
+    return p
+
+
+# ------------------------------------------------------------------------------------------------
+# Handles
+# ------------------------------------------------------------------------------------------------
+class Device:
+    def __init__(self, ordinal: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.ks_device_open(ordinal, C.byref(h)))
+        self.h = h
+
+    def set_timing(self, enable: bool) -> None:
+        _check(self.lib.ks_device_set_timing(self.h, int(enable)))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.ks_device_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- component entry points -----------------------------------------------------------------
+    def sobol_stream(self, width, height, spp, seed, px, py, sample, kinds):
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        kinds = np.ascontiguousarray(kinds, np.int32)
+        n = px.shape[0]
+        nv = int(kinds.sum())
+        ou = np.zeros((n, nv), np.uint32)
+        of = np.zeros((n, nv), np.float32)
+        _check(
+            self.lib.ks_sobol_stream(
+                self.h, width, height, spp, seed, n, _ptr(px), _ptr(py), _ptr(sample), kinds.shape[0], _ptr(kinds),
+                _ptr(ou), _ptr(of),
+            )
+        )
+        return ou, of
+
+    def camera_rays(self, cam: dict, width, height, spp, seed, px, py, sample):
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        n = px.shape[0]
+        out = np.zeros((n, 6), np.float32)
+        c = camera_desc(cam)
+        _check(
+            self.lib.ks_camera_rays(
+                self.h, C.byref(c), width, height, spp, seed, n, _ptr(px), _ptr(py), _ptr(sample), _ptr(out)
+            )
+        )
+        return out
+
+    def bsdf_eval(self, material, wo, wi):
+        m = _material_struct(material)
+        wo = np.ascontiguousarray(wo, np.float32)
+        wi = np.ascontiguousarray(wi, np.float32)
+        n = wo.shape[0]
+        f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        _check(self.lib.ks_bsdf_eval(self.h, C.byref(m), _ptr(wo), _ptr(wi), n, _ptr(f), _ptr(pdf)))
+        return f, pdf
+
+    def bsdf_sample(self, material, wo, u):
+        m = _material_struct(material)
+        wo = np.ascontiguousarray(wo, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        n = wo.shape[0]
+        wi = np.zeros((n, 3), np.float32)
+        beta = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        _check(self.lib.ks_bsdf_sample(self.h, C.byref(m), _ptr(wo), _ptr(u), n, _ptr(wi), _ptr(beta), _ptr(pdf)))
+        return wi, beta, pdf
+
+
+def _material_struct(m) -> MaterialDesc:
+    if isinstance(m, MaterialDesc):
+        return m
+    rec = np.asarray(m, MATERIAL_DTYPE).reshape(())
+    s = MaterialDesc()
+    C.memmove(C.byref(s), rec.tobytes(), C.sizeof(s))
+    return s
+
+
+class Scene:
+    """Uploaded scene: geometry, SBVH and light tables on the device (Scene + PowerLightSampler)."""
+
+    def __init__(self, device: Device, hs: HostScene, bvh: dict | None = None):
+        self.device = device
+        self.lib = device.lib
+        self.host = hs
+        d = hs.desc()
+        o = BvhOptions()
+        for k, v in (bvh or {}).items():
+            setattr(o, k, v)
+        h = C.c_void_p()
+        _check(self.lib.ks_scene_create(device.h, C.byref(d), C.byref(o), C.byref(h)))
+        self.h = h
+
+    def info(self) -> dict:
+        i = SceneInfo()
+        _check(self.lib.ks_scene_info_get(self.h, C.byref(i)))
+        d = {k: getattr(i, k) for k, _ in i._fields_}
+        d["bound_lo"] = tuple(i.bound_lo)
+        d["bound_hi"] = tuple(i.bound_hi)
+        return d
+
+    def light_table(self):
+        n = C.c_int32(0)
+        _check(self.lib.ks_scene_light_table(self.h, C.byref(n), None, None, None))
+        prim = np.zeros(n.value, np.int32)
+        power = np.zeros(n.value, np.float32)
+        cdf = np.zeros(n.value + 1, np.float32)
+        _check(self.lib.ks_scene_light_table(self.h, C.byref(n), _ptr(prim), _ptr(power), _ptr(cdf)))
+        return prim, power, cdf
+
+    def intersect(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        _check(self.lib.ks_intersect(self.h, _ptr(rays), rays.shape[0], _ptr(out)))
+        return out
+
+    def occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], np.uint8)
+        _check(self.lib.ks_occluded(self.h, _ptr(rays), rays.shape[0], _ptr(out)))
+        return out
+
+    def trace_samples(self, cam: dict, params: SmallPTParams, px, py, sample) -> np.ndarray:
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        out = np.zeros((px.shape[0], 4), np.float32)
+        c = camera_desc(cam)
+        _check(
+            self.lib.ks_trace_samples(
+                self.h, C.byref(c), C.byref(params), px.shape[0], _ptr(px), _ptr(py), _ptr(sample), _ptr(out)
+            )
+        )
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.ks_scene_free(self.h)
+            self.h = None
+
+
+class Film:
+    """RenderTarget2: FP64 RGB sums + (hit, miss) weights on the device."""
+
+    def __init__(self, device: Device, width: int, height: int, backdrop=(0.0, 0.0, 0.0)):
+        self.device = device
+        self.lib = device.lib
+        self.width, self.height = width, height
+        bd = (C.c_float * 3)(*backdrop)
+        h = C.c_void_p()
+        _check(self.lib.ks_film_create(device.h, width, height, bd, C.byref(h)))
+        self.h = h
+
+    def clear(self) -> None:
+        _check(self.lib.ks_film_clear(self.h))
+
+    def download(self):
+        rgb = np.zeros((self.height, self.width, 3), np.float64)
+        w = np.zeros((self.height, self.width, 2), np.float64)
+        _check(self.lib.ks_film_download(self.h, _ptr(rgb), _ptr(w)))
+        return rgb, w
+
+    def composite(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        _check(self.lib.ks_film_composite(self.h, _ptr(out)))
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.ks_film_free(self.h)
+            self.h = None
+
+
+def small_pt_render(scene: Scene, cam: dict, params: SmallPTParams, film: Film, progress=None) -> dict:
+    """SmallPT::run: renders into `film`; returns the render statistics."""
+    st = RenderStats()
+    c = camera_desc(cam)
+    if progress is not None:
+        cb = PROGRESS_FN(lambda f, s, u: progress(film, s))
+    else:
+        cb = PROGRESS_FN()
+    _check(scene.lib.ks_small_pt_render(scene.h, C.byref(c), C.byref(params), film.h, cb, None, C.byref(st)))
+    return st.as_dict()

@@ -1,0 +1,1735 @@
+// libkeepsake_amd: the gfx950 wavefront small_pt integrator and its C ABI (include/keepsake_amd.h).
+//
+// Render loop (SmallPT::run, small_pt.cpp:15-70, restructured as a wavefront): path state lives in
+// double-buffered SoA arrays in HBM. Each iteration runs
+//   raygen  -> new camera paths fill the free tail of the current buffer (Sobol + camera)
+//   closest -> closest-hit traversal of every active ray
+//   shade   -> emitter/sky MIS, NEE light sample + shadow ray, BSDF sample, Russian roulette;
+//              survivors are compacted into the next buffer with a wave ballot + prefix count and
+//              one atomic per wave, shadow rays into the shadow queue the same way
+//   shadow  -> any-hit traversal; unoccluded NEE contributions are added to the path (or, for a path
+//              that ended this bounce, to its per-sample result)
+// and a film kernel folds the per-sample results of a batch into the FP64 film in sample order,
+// one thread per pixel, so the film is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "ks_device.h"
+#include "ks_internal.h"
+#include "ks_traverse.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace ksd;
+
+namespace ksa {
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+} // namespace ksa
+
+#define KS_HIP(call)                                                                                               \
+    do {                                                                                                           \
+        hipError_t e_ = (call);                                                                                    \
+        if (e_ != hipSuccess) {                                                                                    \
+            ksa::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));                              \
+            return KS_ERR_HIP;                                                                                     \
+        }                                                                                                          \
+    } while (0)
+
+#define KS_REQUIRE(cond, msg)                                                                                      \
+    do {                                                                                                           \
+        if (!(cond)) {                                                                                             \
+            ksa::set_error(msg);                                                                                   \
+            return KS_ERR_INVALID;                                                                                 \
+        }                                                                                                          \
+    } while (0)
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------------
+// Device-memory helpers
+// ---------------------------------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    hipError_t ensure(size_t n) { return n <= bytes ? hipSuccess : alloc(n); }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Ctrl {
+    uint32_t n_active; // paths in the current buffer, [0, n_active)
+    uint32_t n_next;   // survivors appended by shade into the next buffer
+    uint32_t n_shadow; // shadow rays appended by shade
+    uint32_t iteration;
+    uint64_t work_next, work_end;
+};
+
+enum StatSlot {
+    kRaysClosest = 0,
+    kRaysShadow,
+    kNodesClosest,
+    kTrisClosest,
+    kNodesShadow,
+    kTrisShadow,
+    kNonFinite,
+    kSpills,
+    kStatCount
+};
+
+struct DevScene {
+    const float4 *nodes;
+    const float4 *tris;
+    const float4 *verts;
+    const float4 *normals; // nullptr when the scene has no smooth normals
+    const uint4 *tri_idx;  // i0, i1, i2, material
+    const int32_t *tri_light;
+    const Material *mats;
+    const Light *lights;
+    const float *cdf;
+    int32_t n_lights;
+    int32_t n_sky;
+    int32_t sky[4];
+};
+
+struct RenderConsts {
+    int32_t W, H, cx, cy, cw, ch;
+    int32_t spp, seed, log2spp, ndigits, bounces, include_background, mis_mode;
+    float clamp_direct, clamp_indirect;
+    int32_t s0, nb;            // sample range of the batch: [s0, s0 + nb)
+    const int4 *work_list;     // optional explicit (x, y, sample) per work item
+    float p2w[16];
+    float cam[3];
+};
+
+struct PathBufs {
+    uint32_t *work;
+    uint32_t *state; // Sobol dimension (bits 0..15) | bounce (bits 16..23)
+    float4 *beta;    // beta.rgb, pdf_wi_bsdf
+    float4 *L;       // L.rgb
+    float4 *ro;      // origin, tmin
+    float4 *rd;      // direction, tmax
+};
+
+// ---------------------------------------------------------------------------------------------------
+// Wave-level helpers (wave64)
+// ---------------------------------------------------------------------------------------------------
+KS_DEV uint32_t wave_append(uint32_t *counter, bool pred) {
+    uint64_t mask = __ballot(pred);
+    uint32_t lane = __lane_id();
+    uint32_t prefix = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (mask != 0) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+    }
+    return base + prefix;
+}
+
+KS_DEV void wave_add_stat(unsigned long long *stats, int slot, uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v) atomicAdd(&stats[slot], (unsigned long long)v);
+}
+
+KS_DEV void pixel_of(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &y, uint32_t &s) {
+    if (rc.work_list) {
+        int4 e = rc.work_list[w];
+        x = (uint32_t)e.x;
+        y = (uint32_t)e.y;
+        s = (uint32_t)e.z;
+        return;
+    }
+    uint32_t pl = w / (uint32_t)rc.nb;
+    s = (uint32_t)rc.s0 + (w - pl * (uint32_t)rc.nb);
+    uint32_t ry = pl / (uint32_t)rc.cw;
+    x = (uint32_t)rc.cx + (pl - ry * (uint32_t)rc.cw);
+    y = (uint32_t)rc.cy + ry;
+}
+
+// Camera::spawn_ray (camera.cpp:129-170) after the box-filter jitter (small_pt.cpp:98-105)
+KS_DEV void camera_ray(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &sob, f3 &o, f3 &d) {
+    float ox = 0.0f, oy = 0.0f;
+    if (rc.spp != 1) {
+        float u0, u1;
+        sob.next2d(u0, u1);
+        ox = (1.0f - u0) * -0.5f + u0 * 0.5f; // BoxPixelFilter::sample (render_target.h:120)
+        oy = (1.0f - u1) * -0.5f + u1 * 0.5f;
+    }
+    float fx = ((float)x + 0.5f + ox) / (float)rc.W;
+    float fy = ((float)y + 0.5f + oy) / (float)rc.H;
+    float nx = fx * 2.0f - 1.0f, ny = fy * 2.0f - 1.0f;
+    const float *m = rc.p2w;
+    float hx = m[0] * nx + m[1] * ny + m[2] + m[3];
+    float hy = m[4] * nx + m[5] * ny + m[6] + m[7];
+    float hz = m[8] * nx + m[9] * ny + m[10] + m[11];
+    float hw = m[12] * nx + m[13] * ny + m[14] + m[15];
+    f3 wp = mk(hx / hw, hy / hw, hz / hw);
+    o = mk(rc.cam[0], rc.cam[1], rc.cam[2]);
+    d = normalized(wp - o);
+}
+
+KS_DEV void sobol_start(const RenderConsts &rc, Sobol &sob, uint32_t x, uint32_t y, uint32_t s, int dim) {
+    sob.start(x, y, s, dim, rc.seed, rc.log2spp, rc.ndigits);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------------------
+__global__ void k_fill_u64(unsigned long long *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_ctrl_init(Ctrl *c, uint64_t work_end) {
+    c->n_active = 0;
+    c->n_next = 0;
+    c->n_shadow = 0;
+    c->iteration = 0;
+    c->work_next = 0;
+    c->work_end = work_end;
+}
+
+__global__ __launch_bounds__(256) void k_raygen(RenderConsts rc, PathBufs cur, const Ctrl *ctrl, uint32_t Q) {
+    uint32_t n_active = ctrl->n_active;
+    uint64_t wn = ctrl->work_next, we = ctrl->work_end;
+    uint64_t room = (uint64_t)(Q - n_active);
+    uint32_t n_gen = (uint32_t)min(room, we - wn);
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_gen) return;
+    uint32_t w = (uint32_t)(wn + t);
+    uint32_t slot = n_active + t;
+    uint32_t x, y, s;
+    pixel_of(rc, w, x, y, s);
+    Sobol sob;
+    sobol_start(rc, sob, x, y, s, 0);
+    f3 o, d;
+    camera_ray(rc, x, y, sob, o, d);
+    cur.work[slot] = w;
+    cur.state[slot] = (uint32_t)sob.dim;
+    cur.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    cur.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    cur.ro[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+    cur.rd[slot] = make_float4(d.x, d.y, d.z, kInf);
+}
+
+__global__ void k_gen_done(Ctrl *c, uint32_t Q) {
+    uint64_t room = (uint64_t)(Q - c->n_active);
+    uint32_t n_gen = (uint32_t)min(room, c->work_end - c->work_next);
+    c->n_active += n_gen;
+    c->work_next += n_gen;
+    c->n_next = 0;
+    c->n_shadow = 0;
+}
+
+__global__ void k_iter_done(Ctrl *c) {
+    c->n_active = c->n_next;
+    c->iteration += 1;
+}
+
+// Closest hit for n rays (count read from *n_dev when given). Grid-stride over rays; one LDS stack
+// column per lane.
+__global__ __launch_bounds__(kTraceBlock) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
+                                                         const float4 *ro, const float4 *rd, float4 *hit,
+                                                         uint32_t *ovf, uint32_t ovf_cap,
+                                                         unsigned long long *stats) {
+    __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
+    const uint32_t n = n_dev ? *n_dev : n_fixed;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t G = gridDim.x * blockDim.x;
+    if (blockIdx.x * blockDim.x >= n) return;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.ovf = ovf + gid;
+    st.ovf_stride = G;
+    st.ovf_cap = ovf_cap;
+    st.spills = 0;
+    uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
+    for (uint32_t i = gid; i < n; i += G) {
+        float4 o4 = ro[i], d4 = rd[i];
+        TravRay r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
+        TravHit h;
+        traverse<false>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
+        hit[i] = h.prim >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim))
+                             : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
+        ++n_rays;
+    }
+    wave_add_stat(stats, kRaysClosest, n_rays);
+    wave_add_stat(stats, kNodesClosest, n_nodes);
+    wave_add_stat(stats, kTrisClosest, n_tris);
+    wave_add_stat(stats, kSpills, st.spills);
+}
+
+// Any hit. mode 0: write the occlusion flag; mode 1: add the NEE contribution of unoccluded rays to
+// their target (next path buffer slot, or the per-sample result when bit 31 is set).
+template <int MODE>
+__global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
+                                                        const float4 *ro, const float4 *rd, uint8_t *occ,
+                                                        const uint32_t *target, const float4 *contrib,
+                                                        float4 *path_L, float4 *samp, uint32_t *ovf,
+                                                        uint32_t ovf_cap, unsigned long long *stats) {
+    __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
+    const uint32_t n = n_dev ? *n_dev : n_fixed;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t G = gridDim.x * blockDim.x;
+    if (blockIdx.x * blockDim.x >= n) return;
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.ovf = ovf + gid;
+    st.ovf_stride = G;
+    st.ovf_cap = ovf_cap;
+    st.spills = 0;
+    uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
+    for (uint32_t i = gid; i < n; i += G) {
+        float4 o4 = ro[i], d4 = rd[i];
+        TravRay r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
+        TravHit h;
+        bool blocked = traverse<true>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
+        ++n_rays;
+        if (MODE == 0) {
+            occ[i] = blocked ? 1 : 0;
+        } else if (!blocked) {
+            uint32_t tg = target[i];
+            float4 c = contrib[i];
+            float4 *dst = (tg & 0x80000000u) ? &samp[tg & 0x7fffffffu] : &path_L[tg];
+            float4 v = *dst;
+            v.x += c.x;
+            v.y += c.y;
+            v.z += c.z;
+            *dst = v;
+        }
+    }
+    wave_add_stat(stats, kRaysShadow, n_rays);
+    wave_add_stat(stats, kNodesShadow, n_nodes);
+    wave_add_stat(stats, kTrisShadow, n_tris);
+    wave_add_stat(stats, kSpills, st.spills);
+}
+
+KS_DEV uint32_t cdf_sample(const float *cdf, int n, float u, float &prob) { // distrib.cpp:33-41
+    u = clampf(u, 0.0f, 0x1.fffffep-1f);
+    // upper_bound over cdf[0..n]: first index with cdf[i] > u
+    int lo = 0, hi = n + 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] > u) hi = mid;
+        else lo = mid + 1;
+    }
+    uint32_t idx = (uint32_t)(lo - 1);
+    prob = cdf[idx + 1] - cdf[idx];
+    return idx;
+}
+KS_DEV float light_probability(const float *cdf, int n, uint32_t i) { // light.cpp:680-683
+    return ((cdf[i + 1] - cdf[i]) * (float)n) / (float)n;
+}
+KS_DEV float sky_pdf(f3 wi) { // SkyLight::pdf (light.cpp:113-134) for a 1x1 map, y-up
+    float theta = acosf(clampf(wi.y, -1.0f, 1.0f));
+    float sin_theta = sinf(theta);
+    if (sin_theta == 0.0f) return 0.0f;
+    return 1.0f / (2.0f * kPi * kPi * sin_theta);
+}
+
+KS_DEV SurfHit make_hit(const DevScene &sc, uint4 tri, float u, float v, f3 rd) { // geometry.cpp:68-163
+    f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
+    f3 wo = -normalized(rd);
+    f3 ng = normalized(cross(v1 - v0, v2 - v0));
+    float w = 1.0f - u - v;
+    SurfHit it;
+    it.p = v0 * w + v1 * u + v2 * v;
+    f3 dpdu = v1 - v0;
+    f3 b = normalized(cross(ng, dpdu));
+    f3 t = normalized(cross(b, ng));
+    if (dot(wo, ng) < 0.0f) {
+        ng = -ng;
+        dpdu = -dpdu;
+        t = -t;
+    }
+    it.t = t;
+    it.b = b;
+    it.n = ng;
+    it.st = t;
+    it.sb = b;
+    it.sn = ng;
+    if (sc.normals) {
+        f3 vn = normalized(ld3(sc.normals[tri.x]) * w + ld3(sc.normals[tri.y]) * u + ld3(sc.normals[tri.z]) * v);
+        if (all_finite(vn)) {
+            if (dot(vn, ng) < 0.0f) vn = -vn;
+            f3 vb = normalized(cross(vn, dpdu));
+            it.st = normalized(cross(vb, vn));
+            it.sb = vb;
+            it.sn = vn;
+        }
+    }
+    return it;
+}
+
+// Light::sample for the flat light list; returns Le / pdf (light.h:27-29).
+KS_DEV f3 light_sample(const DevScene &sc, const Light &l, f3 p, float u0, float u1, f3 &wi, float &wi_dist,
+                       float &pdf) {
+    const f3 zero = mk(0, 0, 0);
+    f3 col = mk(l.color[0], l.color[1], l.color[2]);
+    if (l.type == 0) { // SkyLight::sample (light.cpp:88-111), 1x1 map
+        float a = clampf(u0, 0.0f, 0x1.fffffep-1f), b = clampf(u1, 0.0f, 0x1.fffffep-1f);
+        float phi = a * 2.0f * kPi, theta = b * kPi;
+        float st, ct, sp, cp;
+        sincosf(theta, &st, &ct);
+        sincosf(phi, &sp, &cp);
+        pdf = 1.0f / (2.0f * kPi * kPi * st);
+        if (st == 0.0f) {
+            pdf = 0.0f;
+            return zero;
+        }
+        wi = mk(cp * st, ct, sp * st);
+        wi_dist = kInf;
+        return col / pdf;
+    }
+    if (l.type == 1) { // DirectionalLight::sample (light.cpp:207-213)
+        wi = mk(l.vec[0], l.vec[1], l.vec[2]);
+        wi_dist = kInf;
+        pdf = 1.0f;
+        return col;
+    }
+    if (l.type == 2) { // PointLight::sample (light.cpp:222-233)
+        wi = mk(l.vec[0], l.vec[1], l.vec[2]) - p;
+        float l2 = sqnorm(wi);
+        pdf = 0.0f;
+        if (l2 == 0.0f) return zero;
+        float len = sqrtf(l2);
+        wi = wi / len;
+        wi_dist = len;
+        pdf = 1.0f;
+        return col / l2;
+    }
+    // MeshTriLight::sample (light.cpp:567-605)
+    float b0, b1;
+    if (u0 < u1) {
+        b0 = u0 / 2;
+        b1 = u1 - b0;
+    } else {
+        b1 = u1 / 2;
+        b0 = u0 - b1;
+    }
+    uint4 tri = sc.tri_idx[l.prim];
+    f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
+    f3 pos = v0 * (1.0f - b0 - b1) + v1 * b0 + v2 * b1;
+    f3 n = normalized(cross(v1 - v0, v2 - v1));
+    wi = pos - p;
+    float d2 = sqnorm(wi);
+    pdf = 0.0f;
+    wi_dist = 0.0f;
+    if (d2 == 0.0f) return zero;
+    float dist = sqrtf(d2);
+    wi = wi / dist;
+    float absdot = fabsf(dot(n, wi));
+    if (absdot == 0.0f) return zero;
+    pdf = (1.0f / l.area) * (d2 / absdot);
+    wi_dist = fmaxf(0.0f, dist - 1e-4f);
+    return col / pdf;
+}
+
+KS_DEV float mesh_light_pdf(const DevScene &sc, const Light &l, f3 wi, float wi_dist) { // light.cpp:607-628
+    if (wi_dist == 0.0f) return 0.0f;
+    uint4 tri = sc.tri_idx[l.prim];
+    f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
+    f3 n = normalized(cross(v1 - v0, v2 - v1));
+    float absdot = fabsf(dot(n, wi));
+    if (absdot == 0.0f) return 0.0f;
+    return (1.0f / l.area) * (sqr(wi_dist) / absdot);
+}
+
+// One bounce of SmallPT::trace (small_pt.cpp:72-178) for every active path.
+__global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, PathBufs cur, PathBufs nxt,
+                                               const float4 *hits, Ctrl *ctrl, float4 *sh_o, float4 *sh_d,
+                                               uint32_t *sh_tgt, float4 *sh_c, float4 *samp,
+                                               unsigned long long *stats) {
+    const uint32_t n = ctrl->n_active;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n) return; // whole block idle
+    const bool active = i < n;
+
+    bool survive = false, shadow = false;
+    uint32_t w = 0, state = 0;
+    f3 L = mk(0, 0, 0), beta = mk(1, 1, 1);
+    float pdf_wi_bsdf = 0.0f;
+    f3 new_o = mk(0, 0, 0), new_d = mk(0, 0, 0);
+    f3 sh_origin = mk(0, 0, 0), sh_dir = mk(0, 0, 0), sh_contrib = mk(0, 0, 0);
+    float sh_tmax = 0.0f;
+    uint32_t nonfinite = 0;
+    int bounce = 0;
+    Sobol sob;
+
+    if (active) {
+        w = cur.work[i];
+        state = cur.state[i];
+        float4 b4 = cur.beta[i];
+        beta = ld3(b4);
+        pdf_wi_bsdf = b4.w;
+        L = ld3(cur.L[i]);
+        float4 rd4 = cur.rd[i];
+        f3 rd = ld3(rd4);
+        float4 h = hits[i];
+        int32_t prim = __float_as_int(h.w);
+        bounce = (int)(state >> 16) & 0xff;
+        uint32_t x, y, s;
+        pixel_of(rc, w, x, y, s);
+        sobol_start(rc, sob, x, y, s, (int)(state & 0xffffu));
+        bool done = false;
+        bool miss_sample = false;
+
+        if (prim < 0) {
+            f3 wi = normalized(rd);
+            for (int k = 0; k < sc.n_sky; ++k) { // small_pt.cpp:85-109
+                const Light &l = sc.lights[sc.sky[k]];
+                f3 Le = mk(l.color[0], l.color[1], l.color[2]);
+                if (bounce == 0) {
+                    if (rc.include_background) L = L + beta * Le;
+                } else {
+                    float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)sc.sky[k]);
+                    float mis = power_heur(pdf_wi_bsdf, sky_pdf(wi) * pr);
+                    f3 bu = Le * mis;
+                    if (!all_finite(bu)) ++nonfinite;
+                    if (bounce == 1 && rc.clamp_direct > 0.0f) bu = vmin(bu, rc.clamp_direct);
+                    else if (bounce >= 2 && rc.clamp_indirect > 0.0f) bu = vmin(bu, rc.clamp_indirect);
+                    L = L + beta * bu;
+                }
+            }
+            if (bounce == 0 && !rc.include_background) miss_sample = true;
+            done = true;
+        } else {
+            uint4 tri = sc.tri_idx[prim];
+            const Material &m = sc.mats[tri.w];
+            SurfHit it = make_hit(sc, tri, h.y, h.z, rd);
+            if (m.has_emission) { // small_pt.cpp:111-138
+                int32_t li = sc.tri_light[prim];
+                if (li >= 0) {
+                    const Light &l = sc.lights[li];
+                    float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)li);
+                    f3 Le = mk(l.color[0], l.color[1], l.color[2]);
+                    if (bounce == 0) {
+                        L = L + beta * Le;
+                    } else if (pr > 0.0f) {
+                        float pdf_l = mesh_light_pdf(sc, l, normalized(rd), h.x);
+                        float mis = power_heur(pdf_wi_bsdf, pdf_l * pr);
+                        f3 bu = Le * mis;
+                        if (!all_finite(bu)) ++nonfinite;
+                        if (bounce == 1 && rc.clamp_direct > 0.0f) bu = vmin(bu, rc.clamp_direct);
+                        else if (bounce >= 2 && rc.clamp_indirect > 0.0f) bu = vmin(bu, rc.clamp_indirect);
+                        L = L + beta * bu;
+                    }
+                }
+            }
+            if (bounce == rc.bounces) {
+                done = true;
+            } else {
+                f3 wo = -normalized(rd);
+                Closure c;
+                c.basecolor = mk(m.basecolor[0], m.basecolor[1], m.basecolor[2]);
+                c.roughness = m.roughness;
+                c.ax = m.ax;
+                c.ay = m.ay;
+                c.metallic = m.metallic;
+                c.ior = m.ior;
+                c.specular_r0_mul = m.specular_r0_mul;
+                c.specular_trans = m.specular_trans;
+                c.diffuse_trans = m.diffuse_trans;
+                c.diffuse_trans_fwd = m.diffuse_trans_fwd;
+
+                // NEE (nee.cpp:11-58)
+                if (sc.n_lights > 0) {
+                    float pr_light;
+                    uint32_t li = cdf_sample(sc.cdf, sc.n_lights, sob.next(), pr_light);
+                    const Light &l = sc.lights[li];
+                    bool delta_light = l.type == 1 || l.type == 2;
+                    float u0, u1;
+                    sob.next2d(u0, u1);
+                    f3 wi;
+                    float wi_dist = 0.0f, pdf_l = 0.0f;
+                    f3 Le_beta = light_sample(sc, l, it.p, u0, u1, wi, wi_dist, pdf_l);
+                    if (!is_zero(Le_beta) && pdf_l > 0.0f) {
+                        f3 wol = sh_to_local(it, wo), wil = sh_to_local(it, wi);
+                        f3 f = bsdf_eval(wol, wil, c);
+                        float pdf_b = is_zero(f) ? 0.0f : bsdf_pdf(wol, wil, c);
+                        if (!is_zero(f) && pdf_b > 0.0f) {
+                            float mis = 1.0f;
+                            if (!delta_light)
+                                mis = rc.mis_mode == KS_MIS_CONSISTENT ? power_heur(pdf_l * pr_light, pdf_b)
+                                                                       : power_heur(pdf_l, pdf_b);
+                            f3 Ld = Le_beta * f * mis / pr_light;
+                            if (!all_finite(Ld)) ++nonfinite;
+                            if (bounce == 0 && rc.clamp_direct > 0.0f) Ld = vmin(Ld, rc.clamp_direct);
+                            else if (bounce >= 1 && rc.clamp_indirect > 0.0f) Ld = vmin(Ld, rc.clamp_indirect);
+                            shadow = true;
+                            sh_origin = spawn_origin(it.p, wi, it.n);
+                            sh_dir = wi;
+                            sh_tmax = wi_dist;
+                            sh_contrib = beta * Ld;
+                        }
+                    }
+                } else {
+                    (void)sob.next();
+                    float a, b;
+                    sob.next2d(a, b);
+                }
+                // BSDF sampling (material.cpp:157-166)
+                f3 wol = sh_to_local(it, wo);
+                float u_lobe = sob.next();
+                float u0, u1;
+                sob.next2d(u0, u1);
+                f3 wil;
+                float pdf_wi = 0.0f;
+                f3 sbeta = bsdf_sample(wol, wil, c, u_lobe, u0, u1, pdf_wi);
+                if (!all_finite(sbeta)) ++nonfinite;
+                if (maxc(sbeta) == 0.0f || pdf_wi == 0.0f) {
+                    done = true;
+                } else {
+                    f3 wi = sh_to_world(it, wil);
+                    beta = beta * sbeta;
+                    if (maxc(beta) == 0.0f) {
+                        done = true;
+                    } else {
+                        pdf_wi_bsdf = pdf_wi;
+                        new_o = spawn_origin(it.p, wi, it.n);
+                        new_d = wi;
+                        if (maxc(beta) < 1.0f && bounce >= 1) { // Russian roulette (small_pt.cpp:167-174)
+                            float q = fmaxf(0.0f, 1.0f - maxc(beta));
+                            if (sob.next() < q) done = true;
+                            else beta = beta / (1.0f - q);
+                        }
+                    }
+                }
+            }
+        }
+        survive = !done;
+        if (done) samp[w] = make_float4(L.x, L.y, L.z, miss_sample ? 0.0f : 1.0f);
+    }
+
+    uint32_t j = wave_append(&ctrl->n_next, survive);
+    if (survive) {
+        nxt.work[j] = w;
+        nxt.state[j] = (uint32_t)sob.dim | ((uint32_t)(bounce + 1) << 16);
+        nxt.beta[j] = make_float4(beta.x, beta.y, beta.z, pdf_wi_bsdf);
+        nxt.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+        nxt.ro[j] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+        nxt.rd[j] = make_float4(new_d.x, new_d.y, new_d.z, kInf);
+    }
+    uint32_t k = wave_append(&ctrl->n_shadow, shadow);
+    if (shadow) {
+        sh_o[k] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.0f);
+        sh_d[k] = make_float4(sh_dir.x, sh_dir.y, sh_dir.z, sh_tmax);
+        sh_tgt[k] = survive ? j : (0x80000000u | w);
+        sh_c[k] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, 0.0f);
+    }
+    wave_add_stat(stats, kNonFinite, nonfinite);
+}
+
+// RenderTarget2::add / add_miss (render_target.cpp:82-107) for every sample of the batch, in sample
+// order, one thread per pixel of the crop.
+__global__ __launch_bounds__(256) void k_film(RenderConsts rc, const float4 *samp, double *rgb, double *wts) {
+    uint32_t pl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pl >= (uint32_t)(rc.cw * rc.ch)) return;
+    uint32_t ry = pl / (uint32_t)rc.cw;
+    uint32_t x = (uint32_t)rc.cx + (pl - ry * (uint32_t)rc.cw), y = (uint32_t)rc.cy + ry;
+    size_t p = (size_t)y * rc.W + x;
+    double r = rgb[3 * p], g = rgb[3 * p + 1], b = rgb[3 * p + 2];
+    double wh = wts[2 * p], wm = wts[2 * p + 1];
+    const float4 *src = samp + (size_t)pl * rc.nb;
+    for (int s = 0; s < rc.nb; ++s) {
+        float4 v = src[s];
+        if (v.w != 0.0f) {
+            wh += 1.0;
+            r += (double)v.x;
+            g += (double)v.y;
+            b += (double)v.z;
+        } else {
+            wm += 1.0;
+        }
+    }
+    rgb[3 * p] = r;
+    rgb[3 * p + 1] = g;
+    rgb[3 * p + 2] = b;
+    wts[2 * p] = wh;
+    wts[2 * p + 1] = wm;
+}
+
+__global__ void k_sobol_stream(int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample, int32_t seed,
+                               int32_t log2spp, int32_t ndigits, int32_t n_calls, const int32_t *kinds,
+                               int32_t n_values, uint32_t *out_u32, float *out_f) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sobol sob;
+    sob.start((uint32_t)px[i], (uint32_t)py[i], (uint32_t)sample[i], 0, seed, log2spp, ndigits);
+    int64_t base = i * n_values;
+    int k = 0;
+    for (int c = 0; c < n_calls; ++c) {
+        if (kinds[c] == 1) {
+            uint32_t v = sob.next_u32();
+            out_u32[base + k] = v;
+            out_f[base + k] = u32_to_unit(v);
+            ++k;
+        } else {
+            uint32_t a, b;
+            sob.next2d_u32(a, b);
+            out_u32[base + k] = a;
+            out_f[base + k] = u32_to_unit(a);
+            out_u32[base + k + 1] = b;
+            out_f[base + k + 1] = u32_to_unit(b);
+            k += 2;
+        }
+    }
+}
+
+__global__ void k_camera_rays(RenderConsts rc, int64_t n, const int32_t *px, const int32_t *py,
+                              const int32_t *sample, float *rays) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Sobol sob;
+    sobol_start(rc, sob, (uint32_t)px[i], (uint32_t)py[i], (uint32_t)sample[i], 0);
+    f3 o, d;
+    camera_ray(rc, (uint32_t)px[i], (uint32_t)py[i], sob, o, d);
+    float *r = rays + 6 * i;
+    r[0] = o.x, r[1] = o.y, r[2] = o.z, r[3] = d.x, r[4] = d.y, r[5] = d.z;
+}
+
+__global__ void k_bsdf_eval(Material m, int64_t n, const float *wo, const float *wi, float *f, float *pdf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Closure c = closure_of(m);
+    f3 o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), w = mk(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+    f3 ff = bsdf_eval(o, w, c);
+    f[3 * i] = ff.x, f[3 * i + 1] = ff.y, f[3 * i + 2] = ff.z;
+    pdf[i] = bsdf_pdf(o, w, c);
+}
+
+__global__ void k_bsdf_sample(Material m, int64_t n, const float *wo, const float *u, float *wi, float *beta,
+                              float *pdf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Closure c = closure_of(m);
+    f3 o = mk(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+    f3 w = mk(0, 0, 0);
+    float pd = 0.0f;
+    f3 b = bsdf_sample(o, w, c, u[3 * i], u[3 * i + 1], u[3 * i + 2], pd);
+    wi[3 * i] = w.x, wi[3 * i + 1] = w.y, wi[3 * i + 2] = w.z;
+    beta[3 * i] = b.x, beta[3 * i + 1] = b.y, beta[3 * i + 2] = b.z;
+    pdf[i] = pd;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Host-side helpers
+// ---------------------------------------------------------------------------------------------------
+struct M4 {
+    double a[4][4];
+};
+M4 mul(const M4 &x, const M4 &y) {
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = 0;
+            for (int k = 0; k < 4; ++k) s += x.a[i][k] * y.a[k][j];
+            r.a[i][j] = s;
+        }
+    return r;
+}
+bool inverse(const M4 &m, M4 &out) {
+    double a[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) a[i][j] = j < 4 ? m.a[i][j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        if (a[p][c] == 0.0) return false;
+        for (int j = 0; j < 8; ++j) std::swap(a[c][j], a[p][j]);
+        double d = a[c][c];
+        for (int j = 0; j < 8; ++j) a[c][j] /= d;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) {
+                double f = a[r][c];
+                for (int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j];
+            }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) out.a[i][j] = a[i][j + 4];
+    return true;
+}
+
+// Camera(to_world, vfov, aspect) (camera.cpp:74-82) with look_at (camera.cpp:12-32) and
+// rev_inf_projection (camera.cpp:56-70, near_clip 0.1): proj_to_world and the camera position.
+int camera_setup(const ks_camera_desc &c, float p2w[16], float pos[3]) {
+    M4 c2w;
+    if (c.use_matrix) {
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) c2w.a[i][j] = c.to_world[4 * i + j];
+    } else {
+        float up[3] = {c.up[0], c.up[1], c.up[2]};
+        float ul = std::sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+        KS_REQUIRE(ul > 0.0f, "camera up vector is zero");
+        for (float &u : up) u /= ul; // load_vec3("up", force_normalize = true)
+        float back[3] = {c.pos[0] - c.target[0], c.pos[1] - c.target[1], c.pos[2] - c.target[2]};
+        float bl = std::sqrt(back[0] * back[0] + back[1] * back[1] + back[2] * back[2]);
+        KS_REQUIRE(bl > 0.0f, "camera position equals target");
+        for (float &b : back) b /= bl;
+        float right[3] = {up[1] * back[2] - up[2] * back[1], up[2] * back[0] - up[0] * back[2],
+                          up[0] * back[1] - up[1] * back[0]};
+        float rl2 = right[0] * right[0] + right[1] * right[1] + right[2] * right[2];
+        if (rl2 < 1e-8f) {
+            float sign = std::copysign(1.0f, back[2]);
+            const float a = -1.0f / (sign + back[2]);
+            const float b = back[0] * back[1] * a;
+            right[0] = 1.0f + sign * back[0] * back[0] * a, right[1] = sign * b, right[2] = -sign * back[0];
+            up[0] = b, up[1] = sign + back[1] * back[1] * a, up[2] = -back[1];
+        } else {
+            float rl = std::sqrt(rl2);
+            for (float &r : right) r /= rl;
+            up[0] = back[1] * right[2] - back[2] * right[1];
+            up[1] = back[2] * right[0] - back[0] * right[2];
+            up[2] = back[0] * right[1] - back[1] * right[0];
+        }
+        double m[4][4] = {{right[0], up[0], back[0], c.pos[0]},
+                          {right[1], up[1], back[1], c.pos[1]},
+                          {right[2], up[2], back[2], c.pos[2]},
+                          {0, 0, 0, 1}};
+        std::memcpy(c2w.a, m, sizeof(m));
+    }
+    KS_REQUIRE(c.vfov_deg > 0.0f && c.vfov_deg < 180.0f && c.aspect > 0.0f, "invalid camera vfov/aspect");
+    float vfov = c.vfov_deg / 180.0f * 3.14159265359f;
+    float cot = 1.0f / std::tan(vfov * 0.5f);
+    M4 proj{{{cot / c.aspect, 0, 0, 0}, {0, -cot, 0, 0}, {0, 0, 0, 0.1f}, {0, 0, -1, 0}}};
+    M4 w2c, p2w_m;
+    KS_REQUIRE(inverse(c2w, w2c), "camera to_world is singular");
+    M4 w2p = mul(proj, w2c);
+    KS_REQUIRE(inverse(w2p, p2w_m), "camera projection is singular");
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) p2w[4 * i + j] = (float)p2w_m.a[i][j];
+    pos[0] = (float)c2w.a[0][3], pos[1] = (float)c2w.a[1][3], pos[2] = (float)c2w.a[2][3];
+    return KS_OK;
+}
+
+int log2_int(uint32_t v) { return 31 - __builtin_clz(v); }
+int32_t round_up_pow2(int32_t v) {
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1;
+}
+// OwenScrambledSobol02Morton ctor (sobol.h:133-140)
+void sobol_consts(int W, int H, int spp, int &log2spp, int &ndigits) {
+    log2spp = log2_int((uint32_t)spp);
+    int res = round_up_pow2(std::max(W, H));
+    ndigits = log2_int((uint32_t)res) + (log2spp + 1) / 2;
+}
+
+Material to_device_material(const ks_material_desc &m) { // PrincipledBSDF::eval_closure clamps
+    Material d{};
+    for (int k = 0; k < 3; ++k) d.basecolor[k] = std::min(std::max(m.basecolor[k], 0.0f), 1.0f);
+    d.roughness = std::min(std::max(m.roughness, 0.0f), 1.0f);
+    d.ax = d.roughness * d.roughness;
+    d.ay = d.ax;
+    d.metallic = std::min(std::max(m.metallic, 0.0f), 1.0f);
+    d.ior = std::min(std::max(m.ior, 1.0f), 4.0f);
+    d.specular_r0_mul = std::max(m.specular_r0_mul * 2.0f, 0.0f);
+    d.specular_trans = std::min(std::max(m.specular_trans, 0.0f), 1.0f);
+    d.diffuse_trans = std::min(std::max(m.diffuse_trans, 0.0f), 1.0f);
+    d.diffuse_trans_fwd = std::min(std::max(m.diffuse_trans_fwd, 0.0f), 1.0f);
+    for (int k = 0; k < 3; ++k) d.emission[k] = m.emission[k];
+    d.has_emission = m.has_emission ? 1 : 0;
+    return d;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------------------
+// Handles
+// ---------------------------------------------------------------------------------------------------
+struct Workspace {
+    uint32_t Q = 0;
+    DevBuf work[2], state[2], beta[2], L[2], ro[2], rd[2];
+    DevBuf hit, sh_o, sh_d, sh_tgt, sh_c;
+    DevBuf samp;
+    DevBuf ctrl, stats;
+    DevBuf ovf;
+    uint32_t ovf_grid = 0, ovf_cap = 0;
+};
+
+struct ks_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int timing = 0;
+    int n_cu = 0;
+    Workspace ws;
+    Ctrl *host_ctrl = nullptr; // pinned
+};
+
+struct ks_scene {
+    ks_device *dev = nullptr;
+    DevBuf nodes, tris, verts, normals, tri_idx, tri_light, mats, lights, cdf;
+    DevScene ds{};
+    ks_scene_info info{};
+    std::vector<int32_t> light_prim;
+    std::vector<float> light_power, light_cdf;
+};
+
+struct ks_film {
+    ks_device *dev = nullptr;
+    int32_t w = 0, h = 0;
+    float backdrop[3] = {0, 0, 0};
+    DevBuf rgb, wts;
+};
+
+struct ks_host_scene {
+    ksa::HostScene s;
+};
+
+namespace {
+
+template <typename T> hipError_t upload(DevBuf &b, const std::vector<T> &v, hipStream_t s) {
+    hipError_t e = b.alloc(v.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    e = hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
+struct Timer {
+    ks_device *dev;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[KS_K_COUNT];
+    uint64_t launches[KS_K_COUNT] = {};
+    hipEvent_t begin(int k) {
+        launches[k]++;
+        if (!dev->timing) return nullptr;
+        hipEvent_t a, b;
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, dev->stream);
+        ev[k].push_back({a, b});
+        return b;
+    }
+    void end(hipEvent_t b) {
+        if (b) (void)hipEventRecord(b, dev->stream);
+    }
+    void collect(ks_render_stats *st) {
+        for (int k = 0; k < KS_K_COUNT; ++k) {
+            double ms = 0.0;
+            for (auto &p : ev[k]) {
+                float t = 0.0f;
+                if (hipEventElapsedTime(&t, p.first, p.second) == hipSuccess) ms += t;
+                (void)hipEventDestroy(p.first);
+                (void)hipEventDestroy(p.second);
+            }
+            ev[k].clear();
+            if (st) {
+                st->kernel_ms[k] += ms;
+                st->launches[k] += launches[k];
+            }
+        }
+    }
+};
+
+int trace_grid(const ks_device *dev, uint32_t n) {
+    // 8 blocks of 256 lanes per CU keep every SIMD at 8 waves; the grid-stride loop covers the rest
+    uint32_t max_blocks = (uint32_t)std::max(1, dev->n_cu) * 8u;
+    uint32_t need = (n + kTraceBlock - 1) / kTraceBlock;
+    return (int)std::max(1u, std::min(max_blocks, need));
+}
+
+int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_depth) {
+    Workspace &ws = dev->ws;
+    if (ws.Q < Q) {
+        for (int b = 0; b < 2; ++b) {
+            KS_HIP(ws.work[b].alloc(sizeof(uint32_t) * Q));
+            KS_HIP(ws.state[b].alloc(sizeof(uint32_t) * Q));
+            KS_HIP(ws.beta[b].alloc(sizeof(float4) * Q));
+            KS_HIP(ws.L[b].alloc(sizeof(float4) * Q));
+            KS_HIP(ws.ro[b].alloc(sizeof(float4) * Q));
+            KS_HIP(ws.rd[b].alloc(sizeof(float4) * Q));
+        }
+        KS_HIP(ws.hit.alloc(sizeof(float4) * Q));
+        KS_HIP(ws.sh_o.alloc(sizeof(float4) * Q));
+        KS_HIP(ws.sh_d.alloc(sizeof(float4) * Q));
+        KS_HIP(ws.sh_tgt.alloc(sizeof(uint32_t) * Q));
+        KS_HIP(ws.sh_c.alloc(sizeof(float4) * Q));
+        ws.Q = Q;
+    }
+    KS_HIP(ws.samp.ensure(sizeof(float4) * std::max<size_t>(n_samp, 1)));
+    KS_HIP(ws.ctrl.ensure(sizeof(Ctrl)));
+    KS_HIP(ws.stats.ensure(sizeof(unsigned long long) * kStatCount));
+    uint32_t grid = (uint32_t)std::max(1, dev->n_cu) * 8u * kTraceBlock;
+    uint32_t cap = max_depth + 4 > (uint32_t)kStackLds ? max_depth + 4 - kStackLds : 1u;
+    if (ws.ovf_grid < grid || ws.ovf_cap < cap) {
+        KS_HIP(ws.ovf.alloc(sizeof(uint32_t) * (size_t)grid * cap));
+        ws.ovf_grid = grid;
+        ws.ovf_cap = cap;
+    }
+    return KS_OK;
+}
+
+int fill_consts(const ks_scene *sc, const ks_camera_desc *cam, const ks_small_pt_params *p, RenderConsts &rc) {
+    KS_REQUIRE(p->render_width > 0 && p->render_height > 0, "render_width/render_height must be positive");
+    KS_REQUIRE(p->spp > 0, "spp must be positive");
+    KS_REQUIRE(p->bounces >= 0 && p->bounces < 255, "bounces must be in [0, 254]");
+    KS_REQUIRE(std::max(p->render_width, p->render_height) <= 65536, "resolution above 65536");
+    std::memset(&rc, 0, sizeof(rc));
+    rc.W = p->render_width;
+    rc.H = p->render_height;
+    // crop (small_pt.cpp:23-35)
+    rc.cx = 0, rc.cy = 0, rc.cw = rc.W, rc.ch = rc.H;
+    if (p->crop_width > 0 && p->crop_height > 0) {
+        rc.cx = std::max(p->crop_start_x, 0);
+        rc.cy = std::max(p->crop_start_y, 0);
+        rc.cw = std::min(p->crop_width, rc.W - rc.cx);
+        rc.ch = std::min(p->crop_height, rc.H - rc.cy);
+        KS_REQUIRE(rc.cw > 0 && rc.ch > 0, "crop window is empty");
+    }
+    rc.spp = p->spp;
+    rc.seed = p->rng_seed;
+    sobol_consts(rc.W, rc.H, rc.spp, rc.log2spp, rc.ndigits);
+    KS_REQUIRE(2 * rc.ndigits <= 52 && (uint64_t)1 << rc.log2spp <= (uint64_t)rc.spp,
+               "resolution x spp exceeds the sampler index range");
+    rc.bounces = p->bounces;
+    rc.include_background = p->include_background;
+    rc.mis_mode = p->mis_mode;
+    rc.clamp_direct = p->clamp_direct;
+    rc.clamp_indirect = p->clamp_indirect;
+    int e = camera_setup(*cam, rc.p2w, rc.cam);
+    (void)sc;
+    return e;
+}
+
+// Runs the wavefront loop over `total` work items (mapped by rc), leaving per-sample results in ws.samp.
+int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Timer &tm, ks_render_stats *stats) {
+    ks_device *dev = sc->dev;
+    Workspace &ws = dev->ws;
+    hipStream_t s = dev->stream;
+    Ctrl *ctrl = ws.ctrl.as<Ctrl>();
+    unsigned long long *st = ws.stats.as<unsigned long long>();
+    hipLaunchKernelGGL(k_ctrl_init, dim3(1), dim3(1), 0, s, ctrl, (uint64_t)total);
+    KS_HIP(hipGetLastError());
+    PathBufs pb[2];
+    for (int b = 0; b < 2; ++b)
+        pb[b] = PathBufs{ws.work[b].as<uint32_t>(), ws.state[b].as<uint32_t>(), ws.beta[b].as<float4>(),
+                         ws.L[b].as<float4>(),     ws.ro[b].as<float4>(),     ws.rd[b].as<float4>()};
+    int cur = 0;
+    const int tblocks = (int)((Q + 255) / 256);
+    const int tgrid = trace_grid(dev, Q);
+    int iterations = 0;
+    const int check_every = 2;
+    while (true) {
+        hipEvent_t e;
+        e = tm.begin(KS_K_RAYGEN);
+        hipLaunchKernelGGL(k_raygen, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
+        hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q);
+        tm.end(e);
+        e = tm.begin(KS_K_CLOSEST);
+        hipLaunchKernelGGL(k_closest, dim3(tgrid), dim3(kTraceBlock), 0, s, sc->ds, &ctrl->n_active, 0u,
+                           pb[cur].ro, pb[cur].rd, ws.hit.as<float4>(), ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
+        tm.end(e);
+        e = tm.begin(KS_K_SHADE);
+        hipLaunchKernelGGL(k_shade, dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
+                           ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
+                           ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st);
+        tm.end(e);
+        e = tm.begin(KS_K_SHADOW);
+        hipLaunchKernelGGL(k_anyhit<1>, dim3(tgrid), dim3(kTraceBlock), 0, s, sc->ds, &ctrl->n_shadow, 0u,
+                           ws.sh_o.as<float4>(), ws.sh_d.as<float4>(), (uint8_t *)nullptr, ws.sh_tgt.as<uint32_t>(),
+                           ws.sh_c.as<float4>(), pb[1 - cur].L, ws.samp.as<float4>(), ws.ovf.as<uint32_t>(),
+                           ws.ovf_cap, st);
+        hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
+        tm.end(e);
+        KS_HIP(hipGetLastError());
+        cur = 1 - cur;
+        ++iterations;
+        if (iterations % check_every == 0) {
+            KS_HIP(hipMemcpyAsync(dev->host_ctrl, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
+            KS_HIP(hipStreamSynchronize(s));
+            if (dev->host_ctrl->n_active == 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) break;
+            if (iterations > 1000000) {
+                ksa::set_error("wavefront loop did not terminate");
+                return KS_ERR_INVALID;
+            }
+        }
+    }
+    if (stats) stats->iterations += iterations;
+    return KS_OK;
+}
+
+int read_stats(ks_device *dev, ks_render_stats *stats) {
+    if (!stats) return KS_OK;
+    unsigned long long h[kStatCount];
+    KS_HIP(hipMemcpyAsync(h, dev->ws.stats.p, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
+    KS_HIP(hipStreamSynchronize(dev->stream));
+    stats->rays_closest += h[kRaysClosest];
+    stats->rays_shadow += h[kRaysShadow];
+    stats->node_visits_closest += h[kNodesClosest];
+    stats->tri_tests_closest += h[kTrisClosest];
+    stats->node_visits_shadow += h[kNodesShadow];
+    stats->tri_tests_shadow += h[kTrisShadow];
+    stats->nonfinite += h[kNonFinite];
+    stats->stack_overflows += h[kSpills];
+    return KS_OK;
+}
+
+uint32_t pick_queue(const ks_small_pt_params *p, uint64_t total) {
+    uint64_t q = p->queue_size > 0 ? (uint64_t)p->queue_size : (1ull << 21);
+    q = std::min<uint64_t>(q, std::max<uint64_t>(total, 256));
+    q = std::max<uint64_t>(q, 256);
+    return (uint32_t)std::min<uint64_t>(q, 1ull << 26);
+}
+
+template <typename T> struct HostToDev {
+    DevBuf b;
+    hipError_t put(const T *src, size_t n, hipStream_t s) {
+        hipError_t e = b.alloc(sizeof(T) * std::max<size_t>(n, 1));
+        if (e != hipSuccess || n == 0) return e;
+        return hipMemcpyAsync(b.p, src, sizeof(T) * n, hipMemcpyHostToDevice, s);
+    }
+    T *p() const { return b.as<T>(); }
+};
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------------
+extern "C" {
+
+const char *ks_last_error(void) { return ksa::g_error.c_str(); }
+
+int ks_device_count(int32_t *count) {
+    KS_REQUIRE(count, "count is null");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return KS_OK;
+}
+
+int ks_device_open(int32_t ordinal, ks_device **out) {
+    KS_REQUIRE(out, "out is null");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        ksa::set_error("no HIP device available");
+        return KS_ERR_NO_DEVICE;
+    }
+    KS_REQUIRE(ordinal >= 0 && ordinal < n, "device ordinal out of range");
+    KS_HIP(hipSetDevice(ordinal));
+    auto d = std::make_unique<ks_device>();
+    d->ordinal = ordinal;
+    KS_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    KS_HIP(hipGetDeviceProperties(&prop, ordinal));
+    d->n_cu = prop.multiProcessorCount;
+    KS_HIP(hipHostMalloc((void **)&d->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    *out = d.release();
+    return KS_OK;
+}
+
+int ks_device_close(ks_device *dev) {
+    if (!dev) return KS_OK;
+    (void)hipSetDevice(dev->ordinal);
+    if (dev->stream) (void)hipStreamSynchronize(dev->stream);
+    if (dev->host_ctrl) (void)hipHostFree(dev->host_ctrl);
+    if (dev->stream) (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return KS_OK;
+}
+
+int ks_device_set_timing(ks_device *dev, int32_t enable) {
+    KS_REQUIRE(dev, "device is null");
+    dev->timing = enable ? 1 : 0;
+    return KS_OK;
+}
+
+int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options *opt, ks_scene **out) {
+    KS_REQUIRE(dev && d && out, "null argument");
+    *out = nullptr;
+    KS_REQUIRE(d->n_tris == 0 || (d->vertices && d->indices && d->tri_geom), "scene arrays missing");
+    KS_REQUIRE(d->n_materials > 0 && d->materials, "scene needs at least one material");
+    KS_REQUIRE(d->n_lights <= 64, "at most 64 non-mesh lights");
+    for (uint32_t t = 0; t < d->n_tris; ++t) {
+        for (int k = 0; k < 3; ++k)
+            KS_REQUIRE(d->indices[3 * t + k] < d->n_vertices, "triangle index out of range");
+        KS_REQUIRE(d->tri_geom[t] < d->n_geoms, "triangle geometry id out of range");
+        KS_REQUIRE(t == 0 || d->tri_geom[t] >= d->tri_geom[t - 1], "triangles must be grouped by geometry");
+    }
+    for (uint32_t g = 0; g < d->n_geoms; ++g)
+        KS_REQUIRE(d->geom_material[g] < d->n_materials, "geometry material id out of range");
+    KS_HIP(hipSetDevice(dev->ordinal));
+    auto t0 = std::chrono::steady_clock::now();
+    auto sc = std::make_unique<ks_scene>();
+    sc->dev = dev;
+    ks_bvh_options o{};
+    if (opt) o = *opt;
+    ksa::BvhResult bvh;
+    try {
+        bvh = ksa::build_sbvh(d->vertices, d->indices, d->n_tris, o);
+    } catch (const std::exception &ex) {
+        ksa::set_error(std::string("BVH build failed: ") + ex.what());
+        return KS_ERR_INVALID;
+    }
+
+    std::vector<float4> verts(d->n_vertices);
+    for (uint32_t i = 0; i < d->n_vertices; ++i)
+        verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0.0f);
+    std::vector<uint4> tri_idx(d->n_tris);
+    for (uint32_t t = 0; t < d->n_tris; ++t)
+        tri_idx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2],
+                                d->geom_material[d->tri_geom[t]]);
+    std::vector<Material> mats(d->n_materials);
+    for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(d->materials[m]);
+
+    // Light list in PowerLightSampler order (light.cpp:636-677): non-mesh lights, then per emissive
+    // geometry the triangles with non-zero importance after swap compaction (light.cpp:524-558).
+    float ext[3] = {bvh.hi[0] - bvh.lo[0], bvh.hi[1] - bvh.lo[1], bvh.hi[2] - bvh.lo[2]};
+    float scene_radius = 0.5f * std::sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+    const float pi = 3.14159265359f, two_pi = 6.28318530718f;
+    std::vector<Light> lights;
+    std::vector<float> powers;
+    int n_sky = 0;
+    int sky[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < d->n_lights; ++i) {
+        const ks_light_desc &L = d->lights[i];
+        KS_REQUIRE(L.type >= 0 && L.type <= 2, "unknown light type");
+        Light l{};
+        l.type = L.type;
+        for (int k = 0; k < 3; ++k) {
+            l.color[k] = L.color[k];
+            l.vec[k] = L.vec[k];
+        }
+        float ph[3];
+        if (L.type == KS_LIGHT_SKY_AMBIENT) { // SkyLight::power (light.cpp:136-158), 1x1 map
+            KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
+            sky[n_sky++] = (int)i;
+            float sin_theta = std::sin(0.5f / 1.0f * pi);
+            for (int k = 0; k < 3; ++k) ph[k] = ((l.color[k] * sin_theta) * ((pi / 1.0f) * (two_pi / 1.0f))) *
+                                                (pi * (scene_radius * scene_radius));
+        } else if (L.type == KS_LIGHT_DIRECTIONAL) {
+            float n = std::sqrt(l.vec[0] * l.vec[0] + l.vec[1] * l.vec[1] + l.vec[2] * l.vec[2]);
+            KS_REQUIRE(n > 0.0f, "directional light without direction");
+            for (int k = 0; k < 3; ++k) l.vec[k] /= n; // load_vec3("dir", true)
+            for (int k = 0; k < 3; ++k) ph[k] = l.color[k] * (pi * (scene_radius * scene_radius));
+        } else {
+            for (int k = 0; k < 3; ++k) ph[k] = l.color[k] * (4.0f * pi);
+        }
+        lights.push_back(l);
+        powers.push_back((ph[0] + ph[1] + ph[2]) / 3.0f);
+        sc->light_prim.push_back(-1);
+    }
+    std::vector<int32_t> tri_light(d->n_tris, -1);
+    for (uint32_t t0 = 0; t0 < d->n_tris;) {
+        uint32_t t1 = t0;
+        while (t1 < d->n_tris && d->tri_geom[t1] == d->tri_geom[t0]) ++t1;
+        const ks_material_desc &m = d->materials[d->geom_material[d->tri_geom[t0]]];
+        if (m.has_emission) {
+            float lum = 0.212671f * m.emission[0] + 0.715160f * m.emission[1] + 0.072169f * m.emission[2];
+            uint32_t N = t1 - t0;
+            std::vector<uint32_t> ids(N);
+            std::vector<float> imp(N), area(N);
+            for (uint32_t i = 0; i < N; ++i) {
+                const float *a = d->vertices + 3 * d->indices[3 * (t0 + i)];
+                const float *b = d->vertices + 3 * d->indices[3 * (t0 + i) + 1];
+                const float *c = d->vertices + 3 * d->indices[3 * (t0 + i) + 2];
+                float e01[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+                float e12[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+                float cx = e01[1] * e12[2] - e01[2] * e12[1], cy = e01[2] * e12[0] - e01[0] * e12[2],
+                      cz = e01[0] * e12[1] - e01[1] * e12[0];
+                area[i] = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+                imp[i] = two_pi * area[i] * lum;
+                ids[i] = i;
+            }
+            for (uint32_t i = 0; i < N;) {
+                if (imp[i] == 0.0f) {
+                    std::swap(ids[N - 1], ids[i]);
+                    std::swap(imp[N - 1], imp[i]);
+                    std::swap(area[N - 1], area[i]);
+                    --N;
+                } else {
+                    ++i;
+                }
+            }
+            for (uint32_t i = 0; i < N; ++i) {
+                Light l{};
+                l.type = 3;
+                l.prim = t0 + ids[i];
+                l.area = area[i];
+                for (int k = 0; k < 3; ++k) l.color[k] = m.emission[k];
+                tri_light[l.prim] = (int32_t)lights.size();
+                lights.push_back(l);
+                powers.push_back((imp[i] + imp[i] + imp[i]) / 3.0f);
+                sc->light_prim.push_back((int32_t)l.prim);
+            }
+        }
+        t0 = t1;
+    }
+    std::vector<float> cdf;
+    if (!powers.empty()) { // DistribTable (distrib.cpp:9-31)
+        uint32_t n = (uint32_t)powers.size();
+        cdf.resize(n + 1);
+        cdf[0] = 0.0f;
+        float invn = 1.0f / (float)n;
+        for (uint32_t i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + powers[i - 1] * invn;
+        float acc = cdf[n];
+        if (acc == 0.0f) {
+            for (uint32_t i = 0; i <= n; ++i) cdf[i] = (float)i / (float)n;
+        } else {
+            float inv = 1.0f / acc;
+            for (uint32_t i = 0; i <= n; ++i) cdf[i] *= inv;
+        }
+        cdf[n] = 1.0f;
+    }
+    sc->light_power = powers;
+    sc->light_cdf = cdf;
+
+    hipStream_t s = dev->stream;
+    KS_HIP(upload(sc->nodes, bvh.nodes, s));
+    KS_HIP(upload(sc->tris, bvh.tris, s));
+    KS_HIP(upload(sc->verts, verts, s));
+    if (d->normals) {
+        std::vector<float4> nrm(d->n_vertices);
+        for (uint32_t i = 0; i < d->n_vertices; ++i)
+            nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
+        KS_HIP(upload(sc->normals, nrm, s));
+    }
+    KS_HIP(upload(sc->tri_idx, tri_idx, s));
+    KS_HIP(upload(sc->tri_light, tri_light, s));
+    KS_HIP(upload(sc->mats, mats, s));
+    KS_HIP(upload(sc->lights, lights, s));
+    KS_HIP(upload(sc->cdf, cdf, s));
+
+    DevScene &ds = sc->ds;
+    ds.nodes = sc->nodes.as<float4>();
+    ds.tris = sc->tris.as<float4>();
+    ds.verts = sc->verts.as<float4>();
+    ds.normals = d->normals ? sc->normals.as<float4>() : nullptr;
+    ds.tri_idx = sc->tri_idx.as<uint4>();
+    ds.tri_light = sc->tri_light.as<int32_t>();
+    ds.mats = sc->mats.as<Material>();
+    ds.lights = sc->lights.as<Light>();
+    ds.cdf = sc->cdf.as<float>();
+    ds.n_lights = (int32_t)lights.size();
+    ds.n_sky = n_sky;
+    for (int k = 0; k < 4; ++k) ds.sky[k] = sky[k];
+
+    ks_scene_info &in = sc->info;
+    in.n_tris = d->n_tris;
+    in.n_vertices = d->n_vertices;
+    in.n_nodes = (uint32_t)bvh.nodes.size();
+    in.n_refs = (uint32_t)bvh.tris.size();
+    in.n_lights = (uint32_t)lights.size();
+    in.max_depth = bvh.max_depth;
+    in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->verts.bytes + sc->normals.bytes + sc->tri_idx.bytes +
+                      sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes;
+    for (int k = 0; k < 3; ++k) {
+        in.bound_lo[k] = bvh.lo[k];
+        in.bound_hi[k] = bvh.hi[k];
+    }
+    in.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = sc.release();
+    return KS_OK;
+}
+
+int ks_scene_free(ks_scene *scene) {
+    if (scene) {
+        (void)hipSetDevice(scene->dev->ordinal);
+        (void)hipStreamSynchronize(scene->dev->stream);
+        delete scene;
+    }
+    return KS_OK;
+}
+
+int ks_scene_info_get(const ks_scene *scene, ks_scene_info *info) {
+    KS_REQUIRE(scene && info, "null argument");
+    *info = scene->info;
+    return KS_OK;
+}
+
+int ks_scene_light_table(const ks_scene *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf) {
+    KS_REQUIRE(scene, "scene is null");
+    int32_t n = (int32_t)scene->light_power.size();
+    if (n_lights) *n_lights = n;
+    for (int32_t i = 0; i < n; ++i) {
+        if (prim) prim[i] = scene->light_prim[i];
+        if (power) power[i] = scene->light_power[i];
+    }
+    if (cdf && n > 0) std::memcpy(cdf, scene->light_cdf.data(), sizeof(float) * (n + 1));
+    return KS_OK;
+}
+
+int ks_film_create(ks_device *dev, int32_t width, int32_t height, const float backdrop[3], ks_film **out) {
+    KS_REQUIRE(dev && out, "null argument");
+    KS_REQUIRE(width > 0 && height > 0, "film size must be positive");
+    *out = nullptr;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    auto f = std::make_unique<ks_film>();
+    f->dev = dev;
+    f->w = width;
+    f->h = height;
+    if (backdrop)
+        for (int k = 0; k < 3; ++k) f->backdrop[k] = backdrop[k];
+    size_t n = (size_t)width * height;
+    KS_HIP(f->rgb.alloc(sizeof(double) * 3 * n));
+    KS_HIP(f->wts.alloc(sizeof(double) * 2 * n));
+    KS_HIP(hipMemsetAsync(f->rgb.p, 0, f->rgb.bytes, dev->stream));
+    KS_HIP(hipMemsetAsync(f->wts.p, 0, f->wts.bytes, dev->stream));
+    KS_HIP(hipStreamSynchronize(dev->stream));
+    *out = f.release();
+    return KS_OK;
+}
+
+int ks_film_clear(ks_film *film) {
+    KS_REQUIRE(film, "film is null");
+    KS_HIP(hipSetDevice(film->dev->ordinal));
+    KS_HIP(hipMemsetAsync(film->rgb.p, 0, film->rgb.bytes, film->dev->stream));
+    KS_HIP(hipMemsetAsync(film->wts.p, 0, film->wts.bytes, film->dev->stream));
+    KS_HIP(hipStreamSynchronize(film->dev->stream));
+    return KS_OK;
+}
+
+int ks_film_download(const ks_film *film, double *rgb, double *weights) {
+    KS_REQUIRE(film, "film is null");
+    KS_HIP(hipSetDevice(film->dev->ordinal));
+    if (rgb) KS_HIP(hipMemcpyAsync(rgb, film->rgb.p, film->rgb.bytes, hipMemcpyDeviceToHost, film->dev->stream));
+    if (weights)
+        KS_HIP(hipMemcpyAsync(weights, film->wts.p, film->wts.bytes, hipMemcpyDeviceToHost, film->dev->stream));
+    KS_HIP(hipStreamSynchronize(film->dev->stream));
+    return KS_OK;
+}
+
+int ks_film_composite(const ks_film *film, float *out) {
+    KS_REQUIRE(film && out, "null argument");
+    size_t n = (size_t)film->w * film->h;
+    std::vector<double> rgb(3 * n), w(2 * n);
+    int e = ks_film_download(film, rgb.data(), w.data());
+    if (e) return e;
+    for (size_t i = 0; i < n; ++i) { // render_target.cpp:187-192
+        uint32_t w_sum = (uint32_t)(w[2 * i] + w[2 * i + 1]);
+        double a = w_sum == 0 ? 0.0 : w[2 * i] / (double)w_sum;
+        for (int k = 0; k < 3; ++k) {
+            double fg = w[2 * i] == 0 ? 0.0 : rgb[3 * i + k] / w[2 * i];
+            out[3 * i + k] = (float)((1.0 - a) * (double)film->backdrop[k] + a * fg);
+        }
+    }
+    return KS_OK;
+}
+
+int ks_film_free(ks_film *film) {
+    if (film) {
+        (void)hipSetDevice(film->dev->ordinal);
+        (void)hipStreamSynchronize(film->dev->stream);
+        delete film;
+    }
+    return KS_OK;
+}
+
+int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, ks_film *film,
+                       ks_progress_fn progress, void *user, ks_render_stats *stats) {
+    KS_REQUIRE(scene && cam && params && film, "null argument");
+    KS_REQUIRE(film->w == params->render_width && film->h == params->render_height,
+               "film size differs from render_width x render_height");
+    KS_REQUIRE(film->dev == scene->dev, "film and scene belong to different devices");
+    KS_REQUIRE(params->guide_mode == KS_GUIDE_OFF, "guide_mode is not available in this build");
+    ks_device *dev = scene->dev;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    auto t0 = std::chrono::steady_clock::now();
+    RenderConsts rc;
+    int e = fill_consts(scene, cam, params, rc);
+    if (e) return e;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
+    uint64_t pixels = (uint64_t)rc.cw * rc.ch;
+    // per-sample buffer budget: 1 GiB of float4 results per sub-batch
+    const uint64_t max_samp = (1ull << 30) / sizeof(float4);
+    int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
+    KS_REQUIRE(pixels <= max_samp, "crop window too large for one sub-batch");
+    uint32_t Q = pick_queue(params, pixels * (uint64_t)std::min(spp_chunk, params->spp));
+    e = ensure_workspace(dev, Q, pixels * (uint64_t)std::min(spp_chunk, params->spp), scene->info.max_depth);
+    if (e) return e;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                       (int)kStatCount);
+    Timer tm{dev};
+    for (int s_start = 0; s_start < params->spp; s_start += interval) {
+        int s_end = std::min(params->spp, s_start + interval);
+        for (int c0 = s_start; c0 < s_end; c0 += spp_chunk) {
+            rc.s0 = c0;
+            rc.nb = std::min(spp_chunk, s_end - c0);
+            rc.work_list = nullptr;
+            uint64_t total = pixels * (uint64_t)rc.nb;
+            e = run_wavefront(scene, rc, total, Q, tm, stats);
+            if (e) return e;
+            hipEvent_t ev = tm.begin(KS_K_FILM);
+            hipLaunchKernelGGL(k_film, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, dev->stream, rc,
+                               dev->ws.samp.as<float4>(), film->rgb.as<double>(), film->wts.as<double>());
+            tm.end(ev);
+            KS_HIP(hipGetLastError());
+        }
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        if (progress) progress(film, s_end, user);
+    }
+    KS_HIP(hipStreamSynchronize(dev->stream));
+    tm.collect(stats);
+    e = read_stats(dev, stats);
+    if (e) return e;
+    if (stats) {
+        stats->paths = pixels * (uint64_t)params->spp;
+        stats->queue_size = (int32_t)Q;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return KS_OK;
+}
+
+int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
+                     const int32_t *px, const int32_t *py, const int32_t *sample, float *out) {
+    KS_REQUIRE(scene && cam && params && px && py && sample && out, "null argument");
+    KS_REQUIRE(n > 0 && n < (1ll << 31), "sample count out of range");
+    ks_device *dev = scene->dev;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    RenderConsts rc;
+    int e = fill_consts(scene, cam, params, rc);
+    if (e) return e;
+    std::vector<int4> wl((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        KS_REQUIRE(px[i] >= 0 && px[i] < rc.W && py[i] >= 0 && py[i] < rc.H && sample[i] >= 0 && sample[i] < rc.spp,
+                   "pixel or sample index out of range");
+        wl[i] = make_int4(px[i], py[i], sample[i], 0);
+    }
+    HostToDev<int4> dwl;
+    KS_HIP(dwl.put(wl.data(), wl.size(), dev->stream));
+    rc.work_list = dwl.p();
+    rc.s0 = 0;
+    rc.nb = 1;
+    uint32_t Q = pick_queue(params, (uint64_t)n);
+    e = ensure_workspace(dev, Q, (size_t)n, scene->info.max_depth);
+    if (e) return e;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                       (int)kStatCount);
+    Timer tm{dev};
+    e = run_wavefront(scene, rc, (uint64_t)n, Q, tm, nullptr);
+    if (e) return e;
+    tm.collect(nullptr);
+    std::vector<float4> res((size_t)n);
+    KS_HIP(hipMemcpyAsync(res.data(), dev->ws.samp.p, sizeof(float4) * n, hipMemcpyDeviceToHost, dev->stream));
+    KS_HIP(hipStreamSynchronize(dev->stream));
+    for (int64_t i = 0; i < n; ++i) {
+        out[4 * i] = res[i].x;
+        out[4 * i + 1] = res[i].y;
+        out[4 * i + 2] = res[i].z;
+        out[4 * i + 3] = res[i].w;
+    }
+    return KS_OK;
+}
+
+int ks_sobol_stream(ks_device *dev, int32_t width, int32_t height, int32_t spp, int32_t seed, int64_t n,
+                    const int32_t *px, const int32_t *py, const int32_t *sample, int32_t n_calls, const int32_t *kinds,
+                    uint32_t *out_u32, float *out_f) {
+    KS_REQUIRE(dev && px && py && sample && kinds && out_u32 && out_f, "null argument");
+    KS_REQUIRE(n > 0 && width > 0 && height > 0 && spp > 0 && n_calls > 0, "invalid sizes");
+    int n_values = 0;
+    for (int c = 0; c < n_calls; ++c) {
+        KS_REQUIRE(kinds[c] == 1 || kinds[c] == 2, "kinds must be 1 or 2");
+        n_values += kinds[c];
+    }
+    KS_HIP(hipSetDevice(dev->ordinal));
+    int log2spp, ndigits;
+    sobol_consts(width, height, spp, log2spp, ndigits);
+    hipStream_t s = dev->stream;
+    HostToDev<int32_t> dpx, dpy, ds, dk;
+    KS_HIP(dpx.put(px, n, s));
+    KS_HIP(dpy.put(py, n, s));
+    KS_HIP(ds.put(sample, n, s));
+    KS_HIP(dk.put(kinds, n_calls, s));
+    DevBuf ou, of;
+    KS_HIP(ou.alloc(sizeof(uint32_t) * n * n_values));
+    KS_HIP(of.alloc(sizeof(float) * n * n_values));
+    hipLaunchKernelGGL(k_sobol_stream, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dpx.p(), dpy.p(),
+                       ds.p(), seed, log2spp, ndigits, n_calls, dk.p(), n_values, ou.as<uint32_t>(), of.as<float>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(out_u32, ou.p, sizeof(uint32_t) * n * n_values, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipMemcpyAsync(out_f, of.p, sizeof(float) * n * n_values, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_camera_rays(ks_device *dev, const ks_camera_desc *cam, int32_t width, int32_t height, int32_t spp,
+                   int32_t seed, int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample, float *rays) {
+    KS_REQUIRE(dev && cam && px && py && sample && rays, "null argument");
+    KS_REQUIRE(n > 0, "n must be positive");
+    KS_HIP(hipSetDevice(dev->ordinal));
+    ks_small_pt_params p{};
+    p.render_width = width;
+    p.render_height = height;
+    p.spp = spp;
+    p.rng_seed = seed;
+    p.bounces = 1;
+    RenderConsts rc;
+    int e = fill_consts(nullptr, cam, &p, rc);
+    if (e) return e;
+    hipStream_t s = dev->stream;
+    HostToDev<int32_t> dpx, dpy, ds;
+    KS_HIP(dpx.put(px, n, s));
+    KS_HIP(dpy.put(py, n, s));
+    KS_HIP(ds.put(sample, n, s));
+    DevBuf o;
+    KS_HIP(o.alloc(sizeof(float) * 6 * n));
+    hipLaunchKernelGGL(k_camera_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rc, n, dpx.p(), dpy.p(),
+                       ds.p(), o.as<float>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(rays, o.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits, uint8_t *occ) {
+    KS_REQUIRE(scene && rays, "null argument");
+    KS_REQUIRE(n > 0 && n < (1ll << 31), "ray count out of range");
+    ks_device *dev = scene->dev;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = dev->stream;
+    std::vector<float4> o((size_t)n), d((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float *r = rays + 8 * i;
+        o[i] = make_float4(r[0], r[1], r[2], r[3]);
+        d[i] = make_float4(r[4], r[5], r[6], r[7]);
+    }
+    HostToDev<float4> dO, dD;
+    KS_HIP(dO.put(o.data(), o.size(), s));
+    KS_HIP(dD.put(d.data(), d.size(), s));
+    int e = ensure_workspace(dev, 256, 1, scene->info.max_depth);
+    if (e) return e;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, dev->ws.stats.as<unsigned long long>(), (int)kStatCount);
+    int grid = trace_grid(dev, (uint32_t)n);
+    DevBuf out;
+    if (hits) {
+        KS_HIP(out.alloc(sizeof(float4) * n));
+        hipLaunchKernelGGL(k_closest, dim3(grid), dim3(kTraceBlock), 0, s, scene->ds, (const uint32_t *)nullptr,
+                           (uint32_t)n, dO.p(), dD.p(), out.as<float4>(), dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap,
+                           dev->ws.stats.as<unsigned long long>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(hits, out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
+    } else {
+        KS_HIP(out.alloc(n));
+        hipLaunchKernelGGL(k_anyhit<0>, dim3(grid), dim3(kTraceBlock), 0, s, scene->ds, (const uint32_t *)nullptr,
+                           (uint32_t)n, dO.p(), dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
+                           (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, dev->ws.ovf.as<uint32_t>(),
+                           dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(occ, out.p, n, hipMemcpyDeviceToHost, s));
+    }
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_intersect(ks_scene *scene, const float *rays, int64_t n, float *hits) {
+    KS_REQUIRE(hits, "hits is null");
+    return ray_query(scene, rays, n, hits, nullptr);
+}
+
+int ks_occluded(ks_scene *scene, const float *rays, int64_t n, uint8_t *occluded) {
+    KS_REQUIRE(occluded, "occluded is null");
+    return ray_query(scene, rays, n, nullptr, occluded);
+}
+
+int ks_bsdf_eval(ks_device *dev, const ks_material_desc *m, const float *wo, const float *wi, int64_t n, float *f,
+                 float *pdf) {
+    KS_REQUIRE(dev && m && wo && wi && f && pdf && n > 0, "invalid argument");
+    KS_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = dev->stream;
+    HostToDev<float> dwo, dwi;
+    KS_HIP(dwo.put(wo, 3 * n, s));
+    KS_HIP(dwi.put(wi, 3 * n, s));
+    DevBuf df, dp;
+    KS_HIP(df.alloc(sizeof(float) * 3 * n));
+    KS_HIP(dp.alloc(sizeof(float) * n));
+    hipLaunchKernelGGL(k_bsdf_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, to_device_material(*m), n,
+                       dwo.p(), dwi.p(), df.as<float>(), dp.as<float>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(f, df.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipMemcpyAsync(pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_bsdf_sample(ks_device *dev, const ks_material_desc *m, const float *wo, const float *u, int64_t n, float *wi,
+                   float *beta, float *pdf) {
+    KS_REQUIRE(dev && m && wo && u && wi && beta && pdf && n > 0, "invalid argument");
+    KS_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t s = dev->stream;
+    HostToDev<float> dwo, du;
+    KS_HIP(dwo.put(wo, 3 * n, s));
+    KS_HIP(du.put(u, 3 * n, s));
+    DevBuf dwi, db, dp;
+    KS_HIP(dwi.alloc(sizeof(float) * 3 * n));
+    KS_HIP(db.alloc(sizeof(float) * 3 * n));
+    KS_HIP(dp.alloc(sizeof(float) * n));
+    hipLaunchKernelGGL(k_bsdf_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, to_device_material(*m), n,
+                       dwo.p(), du.p(), dwi.as<float>(), db.as<float>(), dp.as<float>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(wi, dwi.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipMemcpyAsync(beta, db.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipMemcpyAsync(pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_host_scene_create(const char *name, uint64_t seed, ks_host_scene **out) {
+    KS_REQUIRE(name && out, "null argument");
+    auto hs = std::make_unique<ks_host_scene>();
+    if (!ksa::make_procedural_scene(name, seed, hs->s)) {
+        ksa::set_error(std::string("unknown procedural scene: ") + name);
+        return KS_ERR_INVALID;
+    }
+    *out = hs.release();
+    return KS_OK;
+}
+
+int ks_host_scene_get(const ks_host_scene *hs, ks_scene_desc *d, ks_camera_desc *cam) {
+    KS_REQUIRE(hs, "host scene is null");
+    const ksa::HostScene &s = hs->s;
+    if (d) {
+        std::memset(d, 0, sizeof(*d));
+        d->vertices = s.vertices.data();
+        d->n_vertices = (uint32_t)(s.vertices.size() / 3);
+        d->normals = s.normals.empty() ? nullptr : s.normals.data();
+        d->indices = s.indices.data();
+        d->n_tris = (uint32_t)(s.indices.size() / 3);
+        d->tri_geom = s.tri_geom.data();
+        d->geom_material = s.geom_material.data();
+        d->n_geoms = (uint32_t)s.geom_material.size();
+        d->materials = s.materials.data();
+        d->n_materials = (uint32_t)s.materials.size();
+        d->lights = s.lights.empty() ? nullptr : s.lights.data();
+        d->n_lights = (uint32_t)s.lights.size();
+    }
+    if (cam) *cam = s.camera;
+    return KS_OK;
+}
+
+int ks_host_scene_free(ks_host_scene *hs) {
+    delete hs;
+    return KS_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,64 @@
+// Host-side internals of libkeepsake_amd shared between the HIP translation unit and the host-only
+// C++ sources (BVH builder, procedural scenes). No device code here.
+#pragma once
+#include "../../include/keepsake_amd.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace ksa {
+
+// GPU BVH2 node: both children's boxes live in the parent so one 64-byte fetch tests two boxes.
+//   f[0..3]   = c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y
+//   f[4..7]   = c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y
+//   f[8..11]  = c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z
+//   child[2]  = child pointers (see kLeafBit), pad[2]
+// A child pointer without kLeafBit is the index of an interior node; with kLeafBit it is a leaf:
+// bits 4..30 = first triangle reference, bits 0..3 = count - 1 (1..16 references).
+struct alignas(16) GpuNode {
+    float f[12];
+    uint32_t child[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMaxLeafRefs = 16;
+constexpr uint32_t kMaxRefs = 1u << 27;
+
+// One triangle reference in leaf order: the three vertices, v0.w carries the original triangle id.
+struct alignas(16) GpuTri {
+    float v0[3];
+    uint32_t tri;
+    float v1[4];
+    float v2[4];
+};
+static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
+
+struct BvhResult {
+    std::vector<GpuNode> nodes;
+    std::vector<GpuTri> tris;
+    uint32_t max_depth = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+
+// SAH binned object + spatial split builder (restates cuda/sbvh.cu:301-440) followed by a flatten
+// into GpuNode/GpuTri arrays.
+BvhResult build_sbvh(const float *vertices, const uint32_t *indices, uint32_t n_tris, const ks_bvh_options &opt);
+
+struct HostScene {
+    std::vector<float> vertices, normals;
+    std::vector<uint32_t> indices, tri_geom, geom_material;
+    std::vector<ks_material_desc> materials;
+    std::vector<ks_light_desc> lights;
+    ks_camera_desc camera{};
+    std::string name;
+};
+
+// Procedural scenes of the benchmark configurations; returns false for an unknown name.
+bool make_procedural_scene(const std::string &name, uint64_t seed, HostScene &out);
+
+void set_error(const std::string &msg);
+
+} // namespace ksa

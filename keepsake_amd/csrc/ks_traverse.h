@@ -1,0 +1,203 @@
+// BVH2 traversal for gfx950: 64-byte nodes holding both child boxes, leaves pointing into a 48-byte
+// per-reference triangle array, the watertight ray/triangle test of cuda/ray_tri.cuh:46-97 and a
+// per-lane short stack that lives in LDS (kStackLds entries, stored with the block size as stride so
+// a wave's 64 lanes hit 64 distinct banks) and spills its oldest entries to a global overflow area.
+#pragma once
+#include "ks_device.h"
+
+namespace ksd {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr int kStackLds = 16;   // LDS entries per lane (power of two)
+constexpr int kTraceBlock = 256; // threads per traversal block
+
+struct TravRay {
+    f3 o, d;
+    f3 inv, oinv;
+    bool nx, ny, nz; // direction sign per axis (RayBoundHelper::dir_is_neg, cuda/aabb.cuh:155-169)
+    int kx, ky, kz;  // RayTriHelper (cuda/ray_tri.cuh:8-37)
+    float sx, sy, sz;
+    float tmin;
+};
+
+KS_DEV float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+KS_DEV TravRay make_trav_ray(f3 o, f3 d, float tmin) {
+    TravRay r;
+    r.o = o;
+    r.d = d;
+    r.tmin = tmin;
+    // zero direction components are replaced by a signed tiny value for the slab test only, so
+    // (b - o) * inv never forms 0 * inf
+    const float eps = 1e-20f;
+    float dx = fabsf(d.x) < eps ? copysignf(eps, d.x) : d.x;
+    float dy = fabsf(d.y) < eps ? copysignf(eps, d.y) : d.y;
+    float dz = fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z;
+    r.inv = mk(1.0f / dx, 1.0f / dy, 1.0f / dz);
+    r.oinv = mk(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+    r.nx = signbit(d.x);
+    r.ny = signbit(d.y);
+    r.nz = signbit(d.z);
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax >= ay && ax >= az) ? 0 : ((ay >= ax && ay >= az) ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dkz = comp(d, kz);
+    if (dkz < 0.0f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.sx = comp(d, kx) / dkz;
+    r.sy = comp(d, ky) / dkz;
+    r.sz = 1.0f / dkz;
+    return r;
+}
+
+// Watertight test (cuda/ray_tri.cuh:46-97); on a hit within (tmin, tmax) returns t and the weights of
+// v1 and v2 (Embree's u, v).
+KS_DEV bool isect_tri(const TravRay &r, float4 a4, float4 b4, float4 c4, float tmax, float &t_out, float &u_out,
+                      float &v_out) {
+    // No FMA contraction: the edge functions of two triangles sharing an edge must be exact negatives
+    // of each other for the test to stay watertight.
+#pragma clang fp contract(off)
+    f3 A0 = mk(a4.x - r.o.x, a4.y - r.o.y, a4.z - r.o.z);
+    f3 B0 = mk(b4.x - r.o.x, b4.y - r.o.y, b4.z - r.o.z);
+    f3 C0 = mk(c4.x - r.o.x, c4.y - r.o.y, c4.z - r.o.z);
+    float Ax = comp(A0, r.kx), Ay = comp(A0, r.ky), Az = comp(A0, r.kz);
+    float Bx = comp(B0, r.kx), By = comp(B0, r.ky), Bz = comp(B0, r.kz);
+    float Cx = comp(C0, r.kx), Cy = comp(C0, r.ky), Cz = comp(C0, r.kz);
+    float sxA = Ax - r.sx * Az, sxB = Bx - r.sx * Bz, sxC = Cx - r.sx * Cz;
+    float syA = Ay - r.sy * Az, syB = By - r.sy * Bz, syC = Cy - r.sy * Cz;
+    // bc = cross(sy, sx)
+    float U = syB * sxC - syC * sxB;
+    float V = syC * sxA - syA * sxC;
+    float W = syA * sxB - syB * sxA;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)sxC * (double)syB - (double)syC * (double)sxB);
+        V = (float)((double)sxA * (double)syC - (double)syA * (double)sxC);
+        W = (float)((double)sxB * (double)syA - (double)syB * (double)sxA);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    float det = U + V + W;
+    if (det == 0.0f) return false;
+    float T = U * (r.sz * Az) + V * (r.sz * Bz) + W * (r.sz * Cz);
+    if (det < 0.0f && (T >= r.tmin * det || T <= tmax * det)) return false;
+    if (det > 0.0f && (T <= r.tmin * det || T >= tmax * det)) return false;
+    float inv_det = 1.0f / det;
+    t_out = T * inv_det;
+    u_out = V * inv_det;
+    v_out = W * inv_det;
+    return true;
+}
+
+struct TravStack {
+    uint32_t *lds;  // this lane's column: entry i at lds[i * kTraceBlock]
+    uint32_t *ovf;  // this lane's spill column: entry i at ovf[i * ovf_stride]
+    uint32_t ovf_stride;
+    uint32_t ovf_cap; // spill entries available per lane
+    int sp;
+    uint32_t spills;
+
+    KS_DEV void push(uint32_t v) {
+        int slot = sp & (kStackLds - 1);
+        if (sp >= kStackLds) {
+            // the slot holds entry sp - kStackLds: move it to the spill area first
+            uint32_t a = (uint32_t)(sp - kStackLds);
+            if (a < ovf_cap) ovf[a * ovf_stride] = lds[slot * kTraceBlock];
+            ++spills;
+        }
+        lds[slot * kTraceBlock] = v;
+        ++sp;
+    }
+    KS_DEV uint32_t pop() {
+        --sp;
+        int slot = sp & (kStackLds - 1);
+        uint32_t v = lds[slot * kTraceBlock];
+        if (sp >= kStackLds) {
+            uint32_t a = (uint32_t)(sp - kStackLds);
+            lds[slot * kTraceBlock] = a < ovf_cap ? ovf[a * ovf_stride] : 0u;
+        }
+        return v;
+    }
+};
+
+struct TravHit {
+    float t, u, v;
+    int32_t prim;
+};
+
+// Closest hit (ANY = false) or any hit (ANY = true) in (tmin, tmax). Children are visited nearest
+// entry distance first. Counts node fetches and triangle tests for the byte model.
+template <bool ANY>
+KS_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, const TravRay &r, float tmax,
+                     TravStack &st, TravHit &hit, uint32_t &n_nodes, uint32_t &n_tris) {
+    const float robust = 1.0000004f; // 1 + 2*gamma(3): conservative far distance (Ize 2013)
+    hit.prim = -1;
+    st.sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        bool exhausted = false;
+        while ((node & kLeafBit) == 0) {
+            ++n_nodes;
+            const float4 *np = nodes + 4 * (size_t)node;
+            float4 a = np[0], b = np[1], c = np[2];
+            uint2 ch = *reinterpret_cast<const uint2 *>(np + 3);
+            // child 0: x (a.x, a.y) y (a.z, a.w) z (c.x, c.y); child 1: x (b.x, b.y) y (b.z, b.w) z (c.z, c.w)
+            float n0x = fmaf(r.nx ? a.y : a.x, r.inv.x, -r.oinv.x), f0x = fmaf(r.nx ? a.x : a.y, r.inv.x, -r.oinv.x);
+            float n0y = fmaf(r.ny ? a.w : a.z, r.inv.y, -r.oinv.y), f0y = fmaf(r.ny ? a.z : a.w, r.inv.y, -r.oinv.y);
+            float n0z = fmaf(r.nz ? c.y : c.x, r.inv.z, -r.oinv.z), f0z = fmaf(r.nz ? c.x : c.y, r.inv.z, -r.oinv.z);
+            float n1x = fmaf(r.nx ? b.y : b.x, r.inv.x, -r.oinv.x), f1x = fmaf(r.nx ? b.x : b.y, r.inv.x, -r.oinv.x);
+            float n1y = fmaf(r.ny ? b.w : b.z, r.inv.y, -r.oinv.y), f1y = fmaf(r.ny ? b.z : b.w, r.inv.y, -r.oinv.y);
+            float n1z = fmaf(r.nz ? c.w : c.z, r.inv.z, -r.oinv.z), f1z = fmaf(r.nz ? c.z : c.w, r.inv.z, -r.oinv.z);
+            float t0n = fmaxf(fmaxf(n0x, n0y), fmaxf(n0z, r.tmin));
+            float t0f = fminf(fminf(f0x, f0y), fminf(f0z, tmax)) * robust;
+            float t1n = fmaxf(fmaxf(n1x, n1y), fmaxf(n1z, r.tmin));
+            float t1f = fminf(fminf(f1x, f1y), fminf(f1z, tmax)) * robust;
+            bool h0 = t0n <= t0f, h1 = t1n <= t1f;
+            if (h0 && h1) {
+                bool first0 = t0n <= t1n;
+                st.push(first0 ? ch.y : ch.x);
+                node = first0 ? ch.x : ch.y;
+            } else if (h0) {
+                node = ch.x;
+            } else if (h1) {
+                node = ch.y;
+            } else {
+                if (st.sp == 0) {
+                    exhausted = true;
+                    break;
+                }
+                node = st.pop();
+            }
+        }
+        if (exhausted) break;
+        uint32_t first = (node >> 4) & 0x7ffffffu;
+        uint32_t cnt = (node & 15u) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            ++n_tris;
+            const float4 *tp = tris + 3 * (size_t)(first + k);
+            float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+            float t, u, v;
+            if (isect_tri(r, v0, v1, v2, tmax, t, u, v)) {
+                if (ANY) {
+                    hit.prim = __float_as_int(v0.w);
+                    return true;
+                }
+                tmax = t;
+                hit.t = t;
+                hit.u = u;
+                hit.v = v;
+                hit.prim = __float_as_int(v0.w);
+            }
+        }
+        if (st.sp == 0) break;
+        node = st.pop();
+    }
+    return hit.prim >= 0;
+}
+
+} // namespace ksd

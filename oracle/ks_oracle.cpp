@@ -1,0 +1,1769 @@
+// CPU oracle for the keepsake small_pt path. Test infrastructure only: see ks_oracle.h for the
+// rules on who may load it and for the pinning status of each function.
+//
+// Float evaluation follows the reference expressions term by term where that is cheap; the oracle is
+// compiled without FMA contraction (-ffp-contract=off), like an x86-64 SSE2 build of the reference.
+#include "ks_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr float kPi = 3.14159265359f;       // maths.h:58
+constexpr float kTwoPi = 6.28318530718f;    // maths.h:59
+constexpr float kHalfPi = kPi / 2.0f;       // maths.h:60
+constexpr float kInvPi = 1.0f / kPi;        // maths.h:63
+constexpr float kInf = std::numeric_limits<float>::infinity();
+const float kBeforeOne = std::nextafter(1.0f, -kInf); // maths.h:65
+
+struct V3 {
+    float x = 0, y = 0, z = 0;
+    V3() = default;
+    V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(float s, V3 a) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float sqnorm(V3 a) { return dot(a, a); }
+inline V3 normalized(V3 a) {
+    float n2 = sqnorm(a);
+    if (n2 > 0.0f) return a / std::sqrt(n2);
+    return a;
+}
+inline float maxc(V3 a) { return std::max(a.x, std::max(a.y, a.z)); }
+inline bool is_zero(V3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+inline bool all_finite(V3 a) { return std::isfinite(a.x) && std::isfinite(a.y) && std::isfinite(a.z); }
+inline V3 vmin(V3 a, float c) { return {std::min(a.x, c), std::min(a.y, c), std::min(a.z, c)}; }
+inline float sqr(float x) { return x * x; }
+template <typename T> T clampv(T x, T a, T b) { return std::min(std::max(x, a), b); }
+inline float lerpf(float a, float b, float t) { return a + t * (b - a); } // std::lerp for finite inputs
+
+inline float luminance(V3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; } // maths.h:675
+
+inline float power_heur(float pf, float pg) { // nee.h:18-24 (n = 2)
+    float pf2 = pf * pf, pg2 = pg * pg;
+    return pf2 / (pf2 + pg2);
+}
+
+// ----------------------------------------------------------------------------------------------------
+// Hashing (hash.h:140-252)
+// ----------------------------------------------------------------------------------------------------
+uint64_t murmur64a(const unsigned char *key, size_t len, uint64_t seed) { // hash.h:140-185
+    const uint64_t m = 0xc6a4a7935bd1e995ull;
+    const int r = 47;
+    uint64_t h = seed ^ (len * m);
+    const unsigned char *end = key + 8 * (len / 8);
+    while (key != end) {
+        uint64_t k;
+        std::memcpy(&k, key, 8);
+        key += 8;
+        k *= m;
+        k ^= k >> r;
+        k *= m;
+        h ^= k;
+        h *= m;
+    }
+    switch (len & 7) {
+    case 7: h ^= uint64_t(key[6]) << 48; [[fallthrough]];
+    case 6: h ^= uint64_t(key[5]) << 40; [[fallthrough]];
+    case 5: h ^= uint64_t(key[4]) << 32; [[fallthrough]];
+    case 4: h ^= uint64_t(key[3]) << 24; [[fallthrough]];
+    case 3: h ^= uint64_t(key[2]) << 16; [[fallthrough]];
+    case 2: h ^= uint64_t(key[1]) << 8; [[fallthrough]];
+    case 1:
+        h ^= uint64_t(key[0]);
+        h *= m;
+    }
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return h;
+}
+
+uint64_t mix_bits(uint64_t v) { // hash.h:206-214
+    v ^= (v >> 31);
+    v *= 0x7fb5d329728ea185ull;
+    v ^= (v >> 27);
+    v *= 0x81dadef4bc2dd44dull;
+    v ^= (v >> 33);
+    return v;
+}
+
+// hash(int dimension, int seed): 8 bytes of two int32 (hash.h:243-252)
+uint64_t hash_dim_seed(int dimension, int seed) {
+    unsigned char buf[8];
+    std::memcpy(buf, &dimension, 4);
+    std::memcpy(buf + 4, &seed, 4);
+    return murmur64a(buf, 8, 0);
+}
+
+// ----------------------------------------------------------------------------------------------------
+// Sobol / Owen / Morton sampler (sobol.h:17-248)
+// ----------------------------------------------------------------------------------------------------
+// Generator matrices of dimensions 0 and 1 (first 104 words of sobol.cpp:154), from their closed forms:
+// dim 0 is the van der Corput matrix, dim 1 satisfies v_i = v_{i-1} ^ (v_{i-1} >> 1); both 52-word
+// columns repeat with period 32 in the table (dim 0 is zero-filled past word 31).
+struct SobolMatrices {
+    uint32_t m[104];
+    SobolMatrices() {
+        for (int i = 0; i < 52; ++i) m[i] = i < 32 ? (0x80000000u >> i) : 0u;
+        uint32_t v = 0x80000000u;
+        uint32_t d1[32];
+        for (int i = 0; i < 32; ++i) {
+            d1[i] = v;
+            v = v ^ (v >> 1);
+        }
+        for (int i = 0; i < 52; ++i) m[52 + i] = d1[i % 32];
+    }
+};
+const SobolMatrices g_sobol;
+
+inline uint32_t reverse_bits_32(uint32_t bits) { // maths.h:267-275
+    bits = (bits << 16u) | (bits >> 16u);
+    bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+    bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+    bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+    bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+    return bits;
+}
+
+inline uint32_t owen_scramble(uint32_t v, uint32_t seed) { // FastOwenScrambler sobol.h:17-33
+    v = reverse_bits_32(v);
+    v ^= v * 0x3d20adeau;
+    v += seed;
+    v *= (seed >> 16) | 1u;
+    v ^= v * 0x05526c56u;
+    v ^= v * 0x53a22864u;
+    return reverse_bits_32(v);
+}
+
+inline uint32_t sobol_u32(uint64_t a, int dim, uint32_t scramble_seed) { // sobol.h:35-49
+    uint32_t v = 0;
+    for (int i = dim * 52; a != 0; a >>= 1, i++)
+        if (a & 1) v ^= g_sobol.m[i];
+    return owen_scramble(v, scramble_seed);
+}
+
+inline float u32_to_unit(uint32_t v) { return std::min(float(v) * 0x1p-32f, kBeforeOne); }
+
+inline uint32_t part_1_by_1(uint32_t x) { // maths.h:381-389 (spread 16 bits to even positions)
+    x &= 0x0000ffff;
+    x = (x ^ (x << 8)) & 0x00ff00ff;
+    x = (x ^ (x << 4)) & 0x0f0f0f0f;
+    x = (x ^ (x << 2)) & 0x33333333;
+    x = (x ^ (x << 1)) & 0x55555555;
+    return x;
+}
+inline uint32_t encode_morton_2(uint32_t x, uint32_t y) { return (part_1_by_1(y) << 1) + part_1_by_1(x); }
+
+inline int log2_int(uint32_t v) { return 31 - __builtin_clz(v); }
+inline int32_t round_up_pow2(int32_t v) { // maths.h:194-203
+    v--;
+    v |= v >> 1;
+    v |= v >> 2;
+    v |= v >> 4;
+    v |= v >> 8;
+    v |= v >> 16;
+    return v + 1;
+}
+
+// The 24 permutations of 4 digits, in the order of sobol.h:171-197 (pbrt-v4's ZSobolSampler order).
+const uint8_t kPerm4[24][4] = {{0, 1, 2, 3}, {0, 1, 3, 2}, {0, 2, 1, 3}, {0, 2, 3, 1}, {0, 3, 2, 1}, {0, 3, 1, 2},
+                               {1, 0, 2, 3}, {1, 0, 3, 2}, {1, 2, 0, 3}, {1, 2, 3, 0}, {1, 3, 2, 0}, {1, 3, 0, 2},
+                               {2, 1, 0, 3}, {2, 1, 3, 0}, {2, 0, 1, 3}, {2, 0, 3, 1}, {2, 3, 0, 1}, {2, 3, 1, 0},
+                               {3, 1, 2, 0}, {3, 1, 0, 2}, {3, 2, 1, 0}, {3, 2, 0, 1}, {3, 0, 2, 1}, {3, 0, 1, 2}};
+
+struct ZSobol { // OwenScrambledSobol02Morton (sobol.h:131-228) + PTRenderSampler (sobol.h:235-248)
+    int dimension = 0;
+    int seed = 0, log2_spp = 0, n_base4_digits = 0;
+    uint64_t morton_index = 0;
+
+    ZSobol(int spp, int full_w, int full_h, int seed_, int px, int py, int sample) : seed(seed_) {
+        log2_spp = log2_int((uint32_t)spp);
+        int res = round_up_pow2(std::max(full_w, full_h));
+        int log4_spp = (log2_spp + 1) / 2;
+        n_base4_digits = log2_int((uint32_t)res) + log4_spp;
+        dimension = 0;
+        // 32-bit shift as in the reference (encode_morton_2 returns uint32_t).
+        morton_index = (uint64_t)((encode_morton_2((uint32_t)px, (uint32_t)py) << log2_spp) | (uint32_t)sample);
+    }
+
+    uint64_t sample_index() const { // sobol.h:168-222
+        uint64_t si = 0;
+        bool pow2 = log2_spp & 1;
+        int last_digit = pow2 ? 1 : 0;
+        uint64_t dmask = (uint64_t)(0x55555555u * (uint32_t)dimension);
+        for (int i = n_base4_digits - 1; i >= last_digit; --i) {
+            int shift = 2 * i - (pow2 ? 1 : 0);
+            int digit = (int)((morton_index >> shift) & 3);
+            uint64_t higher = morton_index >> (shift + 2);
+            int p = (int)((mix_bits(higher ^ dmask) >> 24) % 24);
+            digit = kPerm4[p][digit];
+            si |= uint64_t(digit) << shift;
+        }
+        if (pow2) {
+            int digit = (int)(morton_index & 1);
+            si |= (uint64_t)(digit ^ (int)(mix_bits((morton_index >> 1) ^ dmask) & 1));
+        }
+        return si;
+    }
+
+    uint32_t next_u32() { // sobol.h:148-155
+        uint64_t si = sample_index();
+        ++dimension;
+        uint32_t h = (uint32_t)hash_dim_seed(dimension, seed);
+        return sobol_u32(si, 0, h);
+    }
+    void next2d_u32(uint32_t out[2]) { // sobol.h:157-166
+        uint64_t si = sample_index();
+        dimension += 2;
+        uint64_t bits = hash_dim_seed(dimension, seed);
+        out[0] = sobol_u32(si, 0, (uint32_t)bits);
+        out[1] = sobol_u32(si, 1, (uint32_t)(bits >> 32));
+    }
+    float next() { return u32_to_unit(next_u32()); }
+    void next2d(float &a, float &b) {
+        uint32_t u[2];
+        next2d_u32(u);
+        a = u32_to_unit(u[0]);
+        b = u32_to_unit(u[1]);
+    }
+};
+
+// ----------------------------------------------------------------------------------------------------
+// Sampling warps (rng.h:243-373, bsdf.cpp:62-91)
+// ----------------------------------------------------------------------------------------------------
+inline void sample_disk(float u0, float u1, float &dx, float &dy) { // rng.h:243-259
+    float a = 2.0f * u0 - 1.0f;
+    float b = 2.0f * u1 - 1.0f;
+    if (a == 0.0f && b == 0.0f) {
+        dx = dy = 0.0f;
+        return;
+    }
+    float r, phi;
+    if (a * a > b * b) {
+        r = a;
+        phi = 0.25f * kPi * (b / a);
+    } else {
+        r = b;
+        phi = kHalfPi - 0.25f * kPi * (a / b);
+    }
+    dx = r * std::cos(phi);
+    dy = r * std::sin(phi);
+}
+
+inline V3 sample_cosine_hemisphere(float u0, float u1) { // rng.h:261-267
+    float dx, dy;
+    sample_disk(u0, u1, dx, dy);
+    float z = std::sqrt(std::max(0.0f, 1.0f - (dx * dx + dy * dy)));
+    return {dx, dy, z};
+}
+
+inline void orthonormal_basis(V3 n, V3 &X, V3 &Y) { // maths.h:814-821
+    float sign = std::copysign(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    X = V3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    Y = V3(b, sign + n.y * n.y * a, -n.y);
+}
+
+inline float safe_pow(float a, float b) {
+    if (a == 0.0f && b == 0.0f) return 0.0f;
+    return std::pow(a, b);
+}
+inline float hg_phase(float cos_theta, float g) { // bsdf.h:45-48
+    return ((1.0f - g * g) / safe_pow(1.0f + g * g - 2.0f * g * cos_theta, 1.5f)) * (kInvPi * 0.25f);
+}
+V3 sample_hg(V3 D, float g, float randu, float randv, float *pdf) { // bsdf.cpp:62-91
+    float cos_theta;
+    bool isotropic = std::fabs(g) < 1e-3f;
+    if (isotropic) {
+        cos_theta = (1.0f - 2.0f * randu);
+        if (pdf) *pdf = kInvPi * 0.25f;
+    } else {
+        float k = (1.0f - g * g) / (1.0f - g + 2.0f * g * randu);
+        cos_theta = (1.0f + g * g - k * k) / (2.0f * g);
+        if (pdf) *pdf = hg_phase(cos_theta, g);
+    }
+    float sin_theta = std::sqrt(std::max(0.0f, 1.0f - cos_theta * cos_theta));
+    float phi = kTwoPi * randv;
+    V3 dir(sin_theta * std::cos(phi), sin_theta * std::sin(phi), cos_theta);
+    V3 T, B;
+    orthonormal_basis(D, T, B);
+    return T * dir.x + B * dir.y + D * dir.z;
+}
+
+// ----------------------------------------------------------------------------------------------------
+// Principled BSDF (principled_bsdf.cpp:246-705, microfacet.h:20-153, fresnel.h:9-37)
+// ----------------------------------------------------------------------------------------------------
+struct Closure {
+    V3 basecolor;
+    float roughness, ax, ay, metallic, ior, specular_r0_mul, specular_trans, diffuse_trans, diffuse_trans_fwd;
+};
+
+Closure make_closure(const ora_material &m) { // principled_bsdf.cpp:246-275
+    Closure c;
+    c.basecolor = V3(clampv(m.basecolor[0], 0.0f, 1.0f), clampv(m.basecolor[1], 0.0f, 1.0f),
+                     clampv(m.basecolor[2], 0.0f, 1.0f));
+    c.roughness = clampv(m.roughness, 0.0f, 1.0f);
+    c.ax = sqr(c.roughness);
+    c.ay = c.ax;
+    c.metallic = clampv(m.metallic, 0.0f, 1.0f);
+    c.ior = clampv(m.ior, 1.0f, 4.0f);
+    c.specular_r0_mul = std::max(m.specular_r0_mul * 2.0f, 0.0f);
+    c.specular_trans = clampv(m.specular_trans, 0.0f, 1.0f);
+    c.diffuse_trans = clampv(m.diffuse_trans, 0.0f, 1.0f);
+    c.diffuse_trans_fwd = clampv(m.diffuse_trans_fwd, 0.0f, 1.0f);
+    return c;
+}
+
+struct GGX { // microfacet.h:20-153
+    float alpha_x, alpha_y;
+    GGX(float ax, float ay) {
+        alpha_x = clampv(ax, 1e-3f, 1.0f);
+        alpha_y = clampv(ay, 1e-3f, 1.0f);
+    }
+    bool isotropic() const { return alpha_x == alpha_y; }
+    float D(V3 wm) const {
+        if (wm.z <= 0.0f) return 0.0f;
+        if (isotropic()) {
+            float a2 = alpha_x * alpha_x;
+            float t = 1.0f + (a2 - 1.0f) * wm.z * wm.z;
+            return a2 / (kPi * t * t);
+        }
+        float hx = wm.x / alpha_x, hy = wm.y / alpha_y;
+        float t = sqr(hx) + sqr(hy) + sqr(wm.z);
+        return 1.0f / (kPi * alpha_x * alpha_y * sqr(t));
+    }
+    float lambda(V3 w) const {
+        if (w.z >= 1.0f || w.z <= -1.0f) return 0.0f;
+        float alpha;
+        if (isotropic()) {
+            alpha = alpha_x;
+        } else {
+            float inv_sin_theta2 = 1.0f / (1.0f - w.z * w.z);
+            float cos_phi2 = w.x * w.x * inv_sin_theta2;
+            float sin_phi2 = w.y * w.y * inv_sin_theta2;
+            alpha = std::sqrt(cos_phi2 * alpha_x * alpha_x + sin_phi2 * alpha_y * alpha_y);
+        }
+        float alpha2 = alpha * alpha;
+        float NdotV2 = w.z * w.z;
+        float t = (1.0f - NdotV2) * alpha2 / NdotV2;
+        return 0.5f * (-1.0f + std::sqrt(1.0f + t));
+    }
+    float G1(V3 w) const { return 1.0f / (1.0f + lambda(w)); }
+    float G2(V3 wo, V3 wi) const {
+        if (wo.z * wi.z >= 0.0f) return 1.0f / (1.0f + lambda(wo) + lambda(wi));
+        return (float)std::beta(1.0f + lambda(wo), 1.0f + lambda(wi));
+    }
+    V3 sample(V3 wo, float u0, float u1) const { // VNDF, microfacet.h:98-118
+        V3 Vh = normalized(V3(alpha_x * wo.x, alpha_y * wo.y, wo.z));
+        float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+        V3 T1 = lensq > 0 ? V3(-Vh.y, Vh.x, 0.0f) * (1.0f / std::sqrt(lensq)) : V3(1.0f, 0.0f, 0.0f);
+        V3 T2 = cross(Vh, T1);
+        float r = std::sqrt(u0);
+        float phi = kTwoPi * u1;
+        float t1 = r * std::cos(phi);
+        float t2 = r * std::sin(phi);
+        float s = 0.5f * (1.0f + Vh.z);
+        t2 = (1.0f - s) * std::sqrt(1.0f - t1 * t1) + s * t2;
+        V3 Nh = T1 * t1 + T2 * t2 + Vh * std::sqrt(std::max(0.0f, 1.0f - t1 * t1 - t2 * t2));
+        return normalized(V3(alpha_x * Nh.x, alpha_y * Nh.y, std::max(0.0f, Nh.z)));
+    }
+};
+
+inline float fresnel_dielectric(float cos_theta_i, float eta) { // fresnel.h:9-22
+    float sin_theta_t_2 = eta * eta * (1.0f - cos_theta_i * cos_theta_i);
+    if (sin_theta_t_2 > 1.0) return 1.0f;
+    float cos_theta_t = std::sqrt(std::max(1.0f - sin_theta_t_2, 0.0f));
+    float rs = (eta * cos_theta_t - cos_theta_i) / (eta * cos_theta_t + cos_theta_i);
+    float rp = (eta * cos_theta_i - cos_theta_t) / (eta * cos_theta_i + cos_theta_t);
+    return 0.5f * (rs * rs + rp * rp);
+}
+inline float fresnel_schlick(float c) { // fresnel.h:24-30
+    float s = clampv(1.0f - c, 0.0f, 1.0f);
+    float s2 = s * s;
+    return s2 * s2 * s;
+}
+inline V3 reflect(V3 w, V3 n) { return n * (2.0f * dot(n, w)) - w; } // maths.h:1037
+inline bool refract(V3 wi, V3 n, float eta, V3 &wt) { // maths.h:1041-1049
+    float NdotI = dot(n, wi);
+    float k = 1.0f - eta * eta * (1.0f - sqr(NdotI));
+    if (k < 0.0f) return false;
+    wt = -wi * eta + n * (eta * NdotI - std::sqrt(k));
+    return true;
+}
+inline float sgnf(float x) { return float((0.0f < x) - (x < 0.0f)); }
+
+V3 eval_diffuse(V3 wo, V3 wi, const Closure &c) { // principled_bsdf.cpp:423-434
+    float lw = (1.0f - c.metallic) * (1.0f - c.specular_trans) * (1.0f - c.diffuse_trans);
+    if (lw == 0.0f) return V3();
+    V3 wh = normalized(wo + wi);
+    float Fd90 = 0.5f + 2.0f * c.roughness * wh.z;
+    float retro = lerpf(1.0f, Fd90, fresnel_schlick(wi.z)) * lerpf(1.0f, Fd90, fresnel_schlick(wo.z));
+    return c.basecolor * (lw * kInvPi * retro * wi.z);
+}
+V3 eval_diffuse_transmission(V3 wo, V3 wi, const Closure &c) { // :436-445
+    float lw = (1.0f - c.metallic) * (1.0f - c.specular_trans) * c.diffuse_trans;
+    if (lw == 0.0f) return V3();
+    float fwd = hg_phase(dot(wo, -wi), c.diffuse_trans_fwd);
+    return c.basecolor * (lw * kInvPi * (-wi.z) * fwd);
+}
+V3 eval_metallic_specular(V3 wo, V3 wi, const Closure &c) { // :447-460
+    float lw = c.metallic;
+    if (lw == 0.0f) return V3();
+    GGX g(c.ax, c.ay);
+    V3 wh = normalized(wo + wi);
+    float D = g.D(wh);
+    float G = g.G2(wo, wi);
+    float fs = fresnel_schlick(dot(wo, wh));
+    V3 Fr(lerpf(c.basecolor.x, 1.0f, fs), lerpf(c.basecolor.y, 1.0f, fs), lerpf(c.basecolor.z, 1.0f, fs));
+    return Fr * (lw * D * G / (4.0f * wo.z));
+}
+float dielectric_specular_adjust(float wo_dot_wh, const Closure &c) { // :510-516
+    float R0 = sqr((1.0f - c.ior) / (1.0f + c.ior));
+    float schlick = fresnel_schlick(wo_dot_wh);
+    return lerpf(c.specular_r0_mul * R0, 1.0f, schlick) / lerpf(R0, 1.0f, schlick);
+}
+V3 eval_dielectric_specular(V3 wo, V3 wi, const Closure &c) { // :462-508
+    float lw = 1.0f - c.metallic;
+    if (lw == 0.0f) return V3();
+    bool refl = wo.z * wi.z >= 0.0f;
+    float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
+    V3 wh;
+    if (refl) {
+        wh = normalized(wo + wi);
+    } else {
+        wh = normalized(wo + wi * eta);
+        lw *= c.specular_trans;
+    }
+    if (wh.z < 0.0f) wh = -wh;
+    GGX g(c.ax, c.ay);
+    float D = g.D(wh);
+    float G = g.G2(wo, wi);
+    float awh = std::fabs(dot(wo, wh));
+    float Fr = fresnel_dielectric(awh, 1.0f / eta);
+    float f;
+    if (refl) {
+        float adj = dielectric_specular_adjust(awh, c);
+        f = D * G * (Fr * adj) / (4.0f * std::fabs(wo.z));
+    } else {
+        float denom = sqr(dot(wo, wh) + eta * dot(wi, wh));
+        float spec = D * G * (1.0f - Fr) * dot(wo, wh) * dot(wi, wh) / (denom * wo.z);
+        f = std::fabs(spec);
+    }
+    f *= lw;
+    return V3(f, f, f);
+}
+
+void lobe_weights(V3 wo, const Closure &c, float w[4]) { // :518-540
+    float lum = luminance(c.basecolor);
+    float wd = wo.z > 0.0f ? (1.0f - c.metallic) * (1.0f - c.specular_trans) * lum * kInvPi : 0.0f;
+    w[0] = wd * (1.0f - c.diffuse_trans);
+    w[1] = wd * c.diffuse_trans;
+    w[2] = wo.z > 0.0f ? c.metallic * lerpf(lum, 1.0f, fresnel_schlick(wo.z)) : 0.0f;
+    float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
+    w[3] = (1.0f - c.metallic) * fresnel_dielectric(std::fabs(wo.z), 1.0f / eta);
+    float sum = w[0] + w[1] + w[2] + w[3];
+    if (sum == 0.0f) {
+        w[0] = w[1] = w[2] = w[3] = 0.0f;
+        return;
+    }
+    for (int i = 0; i < 4; ++i) w[i] /= sum;
+}
+
+float pdf_diffuse(V3, V3 wi, const Closure &) { return wi.z * kInvPi; }
+float pdf_diffuse_transmission(V3 wo, V3 wi, const Closure &c) {
+    if (c.diffuse_trans_fwd > 0.05f) return hg_phase(dot(wo, -wi), c.diffuse_trans_fwd);
+    return -wi.z * kInvPi;
+}
+float pdf_metallic_specular(V3 wo, V3 wi, const Closure &c) {
+    GGX g(c.ax, c.ay);
+    V3 wh = normalized(wo + wi);
+    return g.D(wh) * g.G1(wo) / (4.0f * std::fabs(wo.z));
+}
+float pdf_dielectric_specular(V3 wo, V3 wi, const Closure &c) { // :667-703
+    bool refl = wo.z * wi.z >= 0.0f;
+    float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
+    V3 wh = refl ? normalized(wo + wi) : normalized(wo + wi * eta);
+    if (wh.z < 0.0f) wh = -wh;
+    GGX g(c.ax, c.ay);
+    float D = g.D(wh), G1 = g.G1(wo);
+    float awh = std::fabs(dot(wo, wh));
+    float Fr = fresnel_dielectric(awh, 1.0f / eta);
+    float Fr_refl = Fr * dielectric_specular_adjust(awh, c);
+    float Fr_refr = (1.0f - Fr) * c.specular_trans;
+    if (Fr_refr == 0.0f) Fr_refl = 1.0f;
+    float pdf;
+    if (refl) {
+        pdf = D * G1 / (4.0f * std::fabs(wo.z));
+        pdf *= Fr_refl / (Fr_refl + Fr_refr);
+    } else {
+        float denom = sqr(dot(wo, wh) + eta * dot(wi, wh));
+        float jac = eta * eta * std::fabs(dot(wi, wh)) / denom;
+        pdf = D * G1 * std::fabs(dot(wo, wh)) / std::fabs(wo.z) * jac;
+        pdf *= Fr_refr / (Fr_refl + Fr_refr);
+    }
+    return pdf;
+}
+
+V3 bsdf_eval(V3 wo, V3 wi, const Closure &c) { // :308-324
+    if (wo.z == 0.0f || wi.z == 0.0f) return V3();
+    V3 f;
+    if (wo.z > 0.0f && wi.z > 0.0f) {
+        f = f + eval_diffuse(wo, wi, c);
+        f = f + eval_metallic_specular(wo, wi, c);
+    }
+    if (wo.z > 0.0f && wi.z < 0.0f) f = f + eval_diffuse_transmission(wo, wi, c);
+    f = f + eval_dielectric_specular(wo, wi, c);
+    return f;
+}
+float bsdf_pdf(V3 wo, V3 wi, const Closure &c) { // :395-421
+    if (wo.z == 0.0f || wi.z == 0.0f) return 0.0f;
+    float w[4];
+    lobe_weights(wo, c, w);
+    if (w[0] == 0.0f && w[1] == 0.0f && w[2] == 0.0f && w[3] == 0.0f) return 0.0f;
+    float p[4] = {0, 0, 0, 0};
+    if (wo.z > 0.0f && wi.z > 0.0f) {
+        p[0] = pdf_diffuse(wo, wi, c);
+        p[2] = pdf_metallic_specular(wo, wi, c);
+    }
+    if (wo.z > 0.0f && wi.z < 0.0f) p[1] = pdf_diffuse_transmission(wo, wi, c);
+    p[3] = pdf_dielectric_specular(wo, wi, c);
+    return p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3];
+}
+void bsdf_eval_and_pdf(V3 wo, V3 wi, const Closure &c, V3 &f, float &pdf) { // :297-306
+    f = bsdf_eval(wo, wi, c);
+    if (is_zero(f)) {
+        pdf = 0.0f;
+        return;
+    }
+    pdf = bsdf_pdf(wo, wi, c);
+}
+
+int sample_small_distrib4(const float w[4], float u, float *u_remap) { // rng.h:399-432
+    float sum_w = 0.0f;
+    int last_positive = 0;
+    for (int i = 0; i < 4; ++i) {
+        sum_w += w[i];
+        if (w[i] > 0.0f) last_positive = i;
+    }
+    float inv = 1.0f / sum_w;
+    float cdf = 0.0f;
+    int sel = -1;
+    for (int i = 0; i < 4; ++i) {
+        float cdf_next = cdf + w[i] * inv;
+        if (u < cdf_next) {
+            sel = i;
+            if (u_remap) *u_remap = (u - cdf) / (cdf_next - cdf);
+            break;
+        }
+        cdf = cdf_next;
+    }
+    if (sel == -1) sel = last_positive;
+    return sel;
+}
+
+V3 bsdf_sample(V3 wo, V3 &wi, const Closure &c, float u_lobe, float u0, float u1, float &pdf) { // :326-393
+    if (wo.z == 0.0f) {
+        pdf = 0.0f;
+        return V3();
+    }
+    float w[4];
+    lobe_weights(wo, c, w);
+    if (w[0] == 0.0f && w[1] == 0.0f && w[2] == 0.0f && w[3] == 0.0f) {
+        pdf = 0.0f;
+        return V3();
+    }
+    float u_remap = 0.0f;
+    int lobe = sample_small_distrib4(w, u_lobe, &u_remap);
+    float p[4] = {0, 0, 0, 0};
+    GGX g(c.ax, c.ay);
+    if (lobe == 0) {
+        wi = sample_cosine_hemisphere(u0, u1);
+        p[0] = wi.z * kInvPi;
+        if (is_zero(wi) || p[0] == 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        if (wo.z > 0.0f && wi.z > 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
+        p[3] = pdf_dielectric_specular(wo, wi, c);
+    } else if (lobe == 1) {
+        if (c.diffuse_trans_fwd > 0.05f) {
+            wi = sample_hg(-wo, c.diffuse_trans_fwd, u0, u1, &p[1]);
+        } else {
+            wi = sample_cosine_hemisphere(u0, u1);
+            p[1] = wi.z * kInvPi;
+            wi.z = -wi.z;
+        }
+        if (is_zero(wi) || p[1] == 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        p[3] = pdf_dielectric_specular(wo, wi, c);
+    } else if (lobe == 2) {
+        // sample_metallic_specular :569-588
+        V3 wh = g.sample(wo * sgnf(wo.z), u0, u1);
+        wi = reflect(wo, wh);
+        if (wo.z * wi.z < 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        p[2] = g.D(wh) * g.G1(wo) / (4.0f * std::fabs(wo.z));
+        if (is_zero(wi) || p[2] == 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        if (wo.z > 0.0f && wi.z > 0.0f) p[0] = pdf_diffuse(wo, wi, c);
+        p[3] = pdf_dielectric_specular(wo, wi, c);
+    } else {
+        // sample_dielectric_specular :590-643
+        V3 wh = g.sample(wo * sgnf(wo.z), u0, u1);
+        float D = g.D(wh), G1 = g.G1(wo);
+        float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
+        float awh = std::fabs(dot(wo, wh));
+        float Fr = fresnel_dielectric(std::fabs(dot(wo, wh)), 1.0f / eta);
+        float Fr_refl = Fr * dielectric_specular_adjust(awh, c);
+        float Fr_refr = (1.0f - Fr) * c.specular_trans;
+        if (Fr_refr == 0.0f) Fr_refl = 1.0f;
+        if (u_remap < Fr_refl / (Fr_refl + Fr_refr)) {
+            wi = reflect(wo, wh);
+            if (wo.z * wi.z < 0.0f) {
+                pdf = 0.0f;
+                return V3();
+            }
+            p[3] = D * G1 / (4.0f * std::fabs(wo.z));
+            p[3] *= Fr_refl / (Fr_refl + Fr_refr);
+        } else {
+            if (!refract(wo, wh * sgnf(wo.z), 1.0f / eta, wi)) {
+                pdf = 0.0f;
+                return V3();
+            }
+            if (wo.z * wi.z > 0.0f) {
+                pdf = 0.0f;
+                return V3();
+            }
+            float denom = sqr(dot(wo, wh) + eta * dot(wi, wh));
+            float jac = eta * eta * std::fabs(dot(wi, wh)) / denom;
+            p[3] = D * G1 * std::fabs(dot(wo, wh)) / std::fabs(wo.z) * jac;
+            p[3] *= Fr_refr / (Fr_refl + Fr_refr);
+        }
+        if (is_zero(wi) || p[3] == 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        if (wo.z > 0.0f && wi.z > 0.0f) {
+            p[0] = pdf_diffuse(wo, wi, c);
+            p[2] = pdf_metallic_specular(wo, wi, c);
+        }
+        if (wo.z > 0.0f && wi.z < 0.0f) p[1] = pdf_diffuse_transmission(wo, wi, c);
+    }
+    pdf = p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3];
+    return bsdf_eval(wo, wi, c) / pdf;
+}
+
+// ----------------------------------------------------------------------------------------------------
+// Camera (camera.cpp:12-170, maths.h:971-995)
+// ----------------------------------------------------------------------------------------------------
+struct M4 {
+    double a[4][4];
+};
+M4 mul(const M4 &x, const M4 &y) {
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = 0;
+            for (int k = 0; k < 4; ++k) s += x.a[i][k] * y.a[k][j];
+            r.a[i][j] = s;
+        }
+    return r;
+}
+M4 inverse(const M4 &m) { // Gauss-Jordan with partial pivoting (Eigen uses cofactors; agree to rounding)
+    double a[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 8; ++j) a[i][j] = j < 4 ? m.a[i][j] : (j - 4 == i ? 1.0 : 0.0);
+    for (int c = 0; c < 4; ++c) {
+        int p = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[p][c])) p = r;
+        for (int j = 0; j < 8; ++j) std::swap(a[c][j], a[p][j]);
+        double d = a[c][c];
+        for (int j = 0; j < 8; ++j) a[c][j] /= d;
+        for (int r = 0; r < 4; ++r)
+            if (r != c) {
+                double f = a[r][c];
+                for (int j = 0; j < 8; ++j) a[r][j] -= f * a[c][j];
+            }
+    }
+    M4 r;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) r.a[i][j] = a[i][j + 4];
+    return r;
+}
+
+struct Camera {
+    M4 proj_to_world, proj_to_camera, camera_to_world;
+    V3 position;
+
+    explicit Camera(const ora_camera &c) {
+        // create_camera: up is force-normalised on load (camera.cpp:179, config.cpp:146)
+        V3 pos(c.pos[0], c.pos[1], c.pos[2]), tgt(c.target[0], c.target[1], c.target[2]);
+        V3 up = normalized(V3(c.up[0], c.up[1], c.up[2]));
+        // look_at (camera.cpp:12-32)
+        V3 back = normalized(pos - tgt);
+        V3 right = cross(up, back);
+        if (sqnorm(right) < 1e-8f) {
+            orthonormal_basis(back, right, up);
+        } else {
+            right = normalized(right);
+            up = cross(back, right);
+        }
+        M4 c2w{{{right.x, up.x, back.x, pos.x}, {right.y, up.y, back.y, pos.y}, {right.z, up.z, back.z, pos.z}, {0, 0, 0, 1}}};
+        camera_to_world = c2w;
+        float vfov = c.vfov_deg / 180.0f * kPi; // to_radian, maths.h:454
+        // rev_inf_projection (camera.cpp:56-70), near_clip = 0.1
+        float cot = 1.0f / std::tan(vfov * 0.5f);
+        M4 proj{{{cot / c.aspect, 0, 0, 0}, {0, -cot, 0, 0}, {0, 0, 0, 0.1f}, {0, 0, -1, 0}}};
+        M4 w2c = inverse(c2w);
+        M4 w2p = mul(proj, w2c);
+        proj_to_world = inverse(w2p);
+        proj_to_camera = inverse(proj);
+        position = pos;
+    }
+    static V3 point_hdiv(const M4 &m, V3 p) {
+        double v[4] = {p.x, p.y, p.z, 1.0}, r[4];
+        for (int i = 0; i < 4; ++i) r[i] = m.a[i][0] * v[0] + m.a[i][1] * v[1] + m.a[i][2] * v[2] + m.a[i][3] * v[3];
+        return V3((float)(r[0] / r[3]), (float)(r[1] / r[3]), (float)(r[2] / r[3]));
+    }
+    // spawn_ray (camera.cpp:129-170), perspective branch; differentials are not needed (no textures).
+    void spawn_ray(float fx, float fy, V3 &o, V3 &d) const {
+        V3 ndc(fx * 2.0f - 1.0f, fy * 2.0f - 1.0f, 1.0f);
+        V3 world_pos = point_hdiv(proj_to_world, ndc);
+        o = position;
+        d = normalized(world_pos - position);
+    }
+};
+
+// ----------------------------------------------------------------------------------------------------
+// Scene, BVH (independent binned SAH) and ray queries
+// ----------------------------------------------------------------------------------------------------
+struct AABB {
+    V3 lo{kInf, kInf, kInf}, hi{-kInf, -kInf, -kInf};
+    void expand(V3 p) {
+        lo = V3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+        hi = V3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+    }
+    void expand(const AABB &b) {
+        expand(b.lo);
+        expand(b.hi);
+    }
+    float area() const {
+        V3 e = hi - lo;
+        if (e.x < 0) return 0.0f;
+        return 2.0f * (e.x * e.y + e.y * e.z + e.x * e.z);
+    }
+    V3 center() const { return (lo + hi) * 0.5f; }
+};
+
+struct Node { // reference SBVHNode semantics (cuda/sbvh.cuh:18-27) with an explicit leaf prim count
+    AABB box;
+    uint32_t axis = 0, right = 0, prim_offset = 0, prim_count = 0;
+};
+
+struct Light {
+    int type;  // 0 sky ambient, 1 directional, 2 point, 3 mesh triangle
+    V3 color;  // ambient / L / I / emission
+    V3 vec;    // dir or pos
+    uint32_t prim = 0; // global triangle index (mesh)
+    float area = 0.0f;
+};
+
+struct Scene {
+    std::vector<V3> verts, normals;
+    std::vector<uint32_t> idx, tri_geom, geom_mat;
+    std::vector<ora_material> mats;
+    std::vector<Node> nodes;
+    std::vector<uint32_t> prims;
+    std::vector<Light> lights;
+    std::vector<int32_t> tri_light; // flat light index of each emissive triangle, -1 otherwise
+    std::vector<float> cdf;         // DistribTable cdf (distrib.cpp:9-31)
+    std::vector<float> powers;
+    std::vector<uint32_t> sky_lights;
+    AABB bound;
+    bool has_normals = false;
+
+    V3 vtx(uint32_t t, int k) const { return verts[idx[3 * t + k]]; }
+};
+
+uint32_t build_rec(Scene &s, std::vector<uint32_t> &refs, std::vector<AABB> &tb, std::vector<V3> &tc, uint32_t begin,
+                   uint32_t end) {
+    uint32_t ni = (uint32_t)s.nodes.size();
+    s.nodes.emplace_back();
+    AABB b, cb;
+    for (uint32_t i = begin; i < end; ++i) {
+        b.expand(tb[refs[i]]);
+        cb.expand(tc[refs[i]]);
+    }
+    uint32_t n = end - begin;
+    auto leaf = [&]() {
+        Node &nd = s.nodes[ni];
+        nd.box = b;
+        nd.prim_offset = (uint32_t)s.prims.size();
+        nd.prim_count = n;
+        nd.right = 0;
+        for (uint32_t i = begin; i < end; ++i) s.prims.push_back(refs[i]);
+        return ni;
+    };
+    if (n <= 2) return leaf();
+    V3 ext = cb.hi - cb.lo;
+    int axis = (ext.x >= ext.y && ext.x >= ext.z) ? 0 : (ext.y >= ext.z ? 1 : 2);
+    if (ext[axis] <= 0.0f) return leaf();
+    constexpr int B = 16;
+    AABB bb[B];
+    uint32_t bc[B] = {};
+    float lo = cb.lo[axis], inv = B / ext[axis];
+    auto bin = [&](uint32_t r) { return std::min(B - 1, (int)((tc[r][axis] - lo) * inv)); };
+    for (uint32_t i = begin; i < end; ++i) {
+        int k = bin(refs[i]);
+        bc[k]++;
+        bb[k].expand(tb[refs[i]]);
+    }
+    float best = kInf;
+    int best_k = -1;
+    for (int k = 0; k < B - 1; ++k) {
+        AABB l, r;
+        uint32_t nl = 0, nr = 0;
+        for (int j = 0; j <= k; ++j) {
+            l.expand(bb[j]);
+            nl += bc[j];
+        }
+        for (int j = k + 1; j < B; ++j) {
+            r.expand(bb[j]);
+            nr += bc[j];
+        }
+        if (nl == 0 || nr == 0) continue;
+        float cost = 1.0f + (nl * l.area() + nr * r.area()) / b.area();
+        if (cost < best) {
+            best = cost;
+            best_k = k;
+        }
+    }
+    uint32_t mid;
+    if (best_k < 0 || (n <= 4 && best >= (float)n)) {
+        if (n <= 4) return leaf();
+        std::sort(refs.begin() + begin, refs.begin() + end,
+                  [&](uint32_t x, uint32_t y) { return tc[x][axis] < tc[y][axis]; });
+        mid = begin + n / 2;
+    } else {
+        auto it = std::partition(refs.begin() + begin, refs.begin() + end,
+                                 [&](uint32_t r) { return bin(r) <= best_k; });
+        mid = (uint32_t)(it - refs.begin());
+        if (mid == begin || mid == end) mid = begin + n / 2;
+    }
+    s.nodes[ni].box = b;
+    s.nodes[ni].axis = (uint32_t)axis;
+    uint32_t prim_start = (uint32_t)s.prims.size();
+    build_rec(s, refs, tb, tc, begin, mid);
+    uint32_t r = build_rec(s, refs, tb, tc, mid, end);
+    s.nodes[ni].right = r;
+    s.nodes[ni].prim_offset = prim_start;
+    s.nodes[ni].prim_count = (uint32_t)s.prims.size() - prim_start;
+    return ni;
+}
+
+struct RayQ {
+    V3 o, d;
+    float tmin, tmax;
+    V3 inv;
+    int neg[3];
+    int kx, ky, kz;
+    V3 shear;
+    void setup() {
+        // RayBoundHelper (cuda/aabb.cuh:155-169)
+        for (int i = 0; i < 3; ++i) {
+            inv[i] = 1.0f / d[i];
+            neg[i] = d[i] < 0.0f || (d[i] == 0.0f && std::signbit(d[i]));
+        }
+        // RayTriHelper (cuda/ray_tri.cuh:8-37)
+        V3 a(std::fabs(d.x), std::fabs(d.y), std::fabs(d.z));
+        if (a.x >= a.y && a.x >= a.z) kz = 0;
+        else if (a.y >= a.x && a.y >= a.z) kz = 1;
+        else kz = 2;
+        kx = (kz + 1) % 3;
+        ky = (kx + 1) % 3;
+        if (d[kz] < 0.0f) std::swap(kx, ky);
+        shear = V3(d[kx], d[ky], 1.0f) / d[kz];
+    }
+};
+
+bool isect_aabb(const RayQ &r, const AABB &b) { // cuda/aabb.cuh:172-251
+    const V3 *bb[2] = {&b.lo, &b.hi};
+    float tmin = (*bb[r.neg[0]]).x - r.o.x;
+    if (tmin != 0.0f) tmin *= r.inv.x;
+    float tmax = (*bb[1 - r.neg[0]]).x - r.o.x;
+    if (tmax != 0.0f) tmax *= r.inv.x;
+    float tymin = (*bb[r.neg[1]]).y - r.o.y;
+    if (tymin != 0.0f) tymin *= r.inv.y;
+    float tymax = (*bb[1 - r.neg[1]]).y - r.o.y;
+    if (tymax != 0.0f) tymax *= r.inv.y;
+    if (tmin > tymax || tymin > tmax) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    float tzmin = (*bb[r.neg[2]]).z - r.o.z;
+    if (tzmin != 0.0f) tzmin *= r.inv.z;
+    float tzmax = (*bb[1 - r.neg[2]]).z - r.o.z;
+    if (tzmax != 0.0f) tzmax *= r.inv.z;
+    if (tmin > tzmax || tzmin > tmax) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return (tmin < r.tmax) && (tmax > r.tmin);
+}
+
+// Watertight ray/triangle test (cuda/ray_tri.cuh:46-97). coord = weights of (v0, v1, v2).
+bool isect_tri(const RayQ &r, V3 v0, V3 v1, V3 v2, V3 &coord, float &t) {
+    V3 A0 = v0 - r.o, B0 = v1 - r.o, C0 = v2 - r.o;
+    V3 A(A0[r.kx], A0[r.ky], A0[r.kz]), B(B0[r.kx], B0[r.ky], B0[r.kz]), C(C0[r.kx], C0[r.ky], C0[r.kz]);
+    V3 sx = V3(A.x, B.x, C.x) - V3(A.z, B.z, C.z) * r.shear.x;
+    V3 sy = V3(A.y, B.y, C.y) - V3(A.z, B.z, C.z) * r.shear.y;
+    V3 bc = cross(sy, sx);
+    if (bc.x == 0.0f || bc.y == 0.0f || bc.z == 0.0f) {
+        bc.x = (float)((double)sx[2] * (double)sy[1] - (double)sy[2] * (double)sx[1]);
+        bc.y = (float)((double)sx[0] * (double)sy[2] - (double)sy[0] * (double)sx[2]);
+        bc.z = (float)((double)sx[1] * (double)sy[0] - (double)sy[1] * (double)sx[0]);
+    }
+    if ((bc.x < 0.0f || bc.y < 0.0f || bc.z < 0.0f) && (bc.x > 0.0f || bc.y > 0.0f || bc.z > 0.0f)) return false;
+    float det = bc.x + bc.y + bc.z;
+    if (det == 0.0f) return false;
+    V3 sz = V3(A.z, B.z, C.z) * r.shear.z;
+    float T = dot(bc, sz);
+    if (det < 0.0f && (T >= r.tmin * det || T <= r.tmax * det)) return false;
+    if (det > 0.0f && (T <= r.tmin * det || T >= r.tmax * det)) return false;
+    float inv_det = 1.0f / det;
+    coord = bc * inv_det;
+    t = T * inv_det;
+    return true;
+}
+
+struct Hit {
+    float t = -1.0f, u = 0.0f, v = 0.0f;
+    int32_t prim = -1;
+};
+
+// Closest hit with the reference traversal order (cuda/sbvh.cu:512-560): near child first by the sign
+// of the ray direction along the split axis, stack of pending nodes, strict t < tmax update.
+Hit closest(const Scene &s, RayQ r) {
+    Hit h;
+    if (s.nodes.empty()) return h;
+    uint32_t stack[128];
+    int sp = 0;
+    uint32_t cur = 0;
+    while (true) {
+        const Node &nd = s.nodes[cur];
+        if (!isect_aabb(r, nd.box)) {
+            if (sp == 0) break;
+            cur = stack[--sp];
+            continue;
+        }
+        if (nd.right == 0) {
+            for (uint32_t i = 0; i < nd.prim_count; ++i) {
+                uint32_t t = s.prims[nd.prim_offset + i];
+                V3 coord;
+                float th;
+                if (isect_tri(r, s.vtx(t, 0), s.vtx(t, 1), s.vtx(t, 2), coord, th) && th < r.tmax) {
+                    r.tmax = th;
+                    h.t = th;
+                    h.u = coord.y;
+                    h.v = coord.z;
+                    h.prim = (int32_t)t;
+                }
+            }
+            if (sp == 0) break;
+            cur = stack[--sp];
+        } else if (r.neg[nd.axis]) {
+            stack[sp++] = cur + 1;
+            cur = nd.right;
+        } else {
+            stack[sp++] = nd.right;
+            cur = cur + 1;
+        }
+    }
+    return h;
+}
+
+bool occluded(const Scene &s, RayQ r) { // cuda/sbvh.cu:562-603
+    if (s.nodes.empty()) return false;
+    uint32_t stack[128];
+    int sp = 0;
+    uint32_t cur = 0;
+    while (true) {
+        const Node &nd = s.nodes[cur];
+        if (!isect_aabb(r, nd.box)) {
+            if (sp == 0) break;
+            cur = stack[--sp];
+            continue;
+        }
+        if (nd.right == 0) {
+            for (uint32_t i = 0; i < nd.prim_count; ++i) {
+                uint32_t t = s.prims[nd.prim_offset + i];
+                V3 coord;
+                float th;
+                if (isect_tri(r, s.vtx(t, 0), s.vtx(t, 1), s.vtx(t, 2), coord, th)) return true;
+            }
+            if (sp == 0) break;
+            cur = stack[--sp];
+        } else if (r.neg[nd.axis]) {
+            stack[sp++] = cur + 1;
+            cur = nd.right;
+        } else {
+            stack[sp++] = nd.right;
+            cur = cur + 1;
+        }
+    }
+    return false;
+}
+
+Hit brute(const Scene &s, RayQ r) {
+    Hit h;
+    uint32_t nt = (uint32_t)s.tri_geom.size();
+    for (uint32_t t = 0; t < nt; ++t) {
+        V3 coord;
+        float th;
+        if (isect_tri(r, s.vtx(t, 0), s.vtx(t, 1), s.vtx(t, 2), coord, th) && th < r.tmax) {
+            r.tmax = th;
+            h.t = th;
+            h.u = coord.y;
+            h.v = coord.z;
+            h.prim = (int32_t)t;
+        }
+    }
+    return h;
+}
+
+// DistribTable (distrib.cpp:9-31)
+std::vector<float> make_cdf(const std::vector<float> &f) {
+    uint32_t n = (uint32_t)f.size();
+    std::vector<float> cdf(n + 1);
+    cdf[0] = 0;
+    float invn = 1.0f / (float)n;
+    for (uint32_t i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1] * invn;
+    float acc = cdf[n];
+    if (acc == 0.0f) {
+        for (uint32_t i = 0; i < n + 1; ++i) cdf[i] = (float)i / (float)n;
+    } else {
+        float invacc = 1.0f / acc;
+        for (uint32_t i = 0; i < n + 1; ++i) cdf[i] *= invacc;
+    }
+    cdf[n] = 1.0f;
+    return cdf;
+}
+
+uint32_t cdf_sample(const std::vector<float> &cdf, float u, float &prob) { // distrib.cpp:33-41
+    u = clampv(u, 0.0f, std::nextafter(1.0f, 0.0f));
+    auto it = std::upper_bound(cdf.begin(), cdf.end(), u);
+    uint32_t index = (uint32_t)std::distance(cdf.begin(), std::prev(it));
+    prob = cdf[index + 1] - cdf[index];
+    return index;
+}
+
+float light_probability(const Scene &s, uint32_t i) { // light.cpp:680-683
+    uint32_t n = (uint32_t)s.cdf.size() - 1;
+    return ((s.cdf[i + 1] - s.cdf[i]) * (float)n) / (float)n;
+}
+
+Scene *create(const ora_scene_desc *d) {
+    auto s = std::make_unique<Scene>();
+    s->verts.resize(d->n_vertices);
+    for (uint32_t i = 0; i < d->n_vertices; ++i)
+        s->verts[i] = V3(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2]);
+    if (d->normals) {
+        s->has_normals = true;
+        s->normals.resize(d->n_vertices);
+        for (uint32_t i = 0; i < d->n_vertices; ++i)
+            s->normals[i] = V3(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2]);
+    }
+    s->idx.assign(d->indices, d->indices + 3 * (size_t)d->n_tris);
+    s->tri_geom.assign(d->tri_geom, d->tri_geom + d->n_tris);
+    s->geom_mat.assign(d->geom_material, d->geom_material + d->n_geoms);
+    s->mats.assign(d->materials, d->materials + d->n_materials);
+
+    uint32_t nt = d->n_tris;
+    std::vector<AABB> tb(nt);
+    std::vector<V3> tc(nt);
+    std::vector<uint32_t> refs(nt);
+    for (uint32_t t = 0; t < nt; ++t) {
+        for (int k = 0; k < 3; ++k) tb[t].expand(s->vtx(t, k));
+        tc[t] = tb[t].center();
+        refs[t] = t;
+        s->bound.expand(tb[t]);
+    }
+    if (nt > 0) {
+        s->nodes.reserve(2 * nt);
+        build_rec(*s, refs, tb, tc, 0, nt);
+    }
+
+    // Lights: non-mesh lights first, then mesh lights per geometry (small_pt.cpp:187-205, light.cpp:636-677)
+    V3 ext = s->bound.hi - s->bound.lo;
+    float scene_radius = 0.5f * std::sqrt(sqnorm(ext));
+    for (uint32_t i = 0; i < d->n_lights; ++i) {
+        const ora_light &L = d->lights[i];
+        Light l;
+        l.type = L.type;
+        l.color = V3(L.color[0], L.color[1], L.color[2]);
+        l.vec = V3(L.vec[0], L.vec[1], L.vec[2]);
+        V3 phi;
+        if (L.type == 0) {
+            // SkyLight::power (light.cpp:136-158) for a 1x1 ambient map
+            float theta = 0.5f / 1.0f * kPi;
+            float sin_theta = std::sin(theta);
+            phi = l.color * sin_theta;
+            phi = phi * ((kPi / 1.0f) * (kTwoPi / 1.0f));
+            phi = phi * (kPi * sqr(scene_radius));
+            s->sky_lights.push_back(i);
+        } else if (L.type == 1) {
+            l.vec = normalized(l.vec);
+            phi = l.color * (kPi * sqr(scene_radius)); // light.cpp:216-220
+        } else {
+            phi = l.color * (4.0f * kPi); // light.cpp:235
+        }
+        s->lights.push_back(l);
+        s->powers.push_back((phi.x + phi.y + phi.z) / 3.0f);
+    }
+    s->tri_light.assign(nt, -1);
+    uint32_t g = 0;
+    for (uint32_t t0 = 0; t0 < nt; g++) {
+        uint32_t t1 = t0;
+        while (t1 < nt && s->tri_geom[t1] == s->tri_geom[t0]) ++t1;
+        const ora_material &m = s->mats[s->geom_mat[s->tri_geom[t0]]];
+        if (m.has_emission) {
+            V3 Le(m.emission[0], m.emission[1], m.emission[2]);
+            float lum = luminance(Le);
+            // compute_mesh_light_importance uniform branch (light.cpp:387-409) + swap compaction (:536-545)
+            uint32_t N = t1 - t0;
+            std::vector<uint32_t> ids(N);
+            std::vector<float> imp(N), area(N);
+            for (uint32_t i = 0; i < N; ++i) {
+                uint32_t t = t0 + i;
+                V3 e01 = s->vtx(t, 1) - s->vtx(t, 0);
+                V3 e12 = s->vtx(t, 2) - s->vtx(t, 1);
+                float a = 0.5f * std::sqrt(sqnorm(cross(e01, e12)));
+                ids[i] = i;
+                imp[i] = kTwoPi * a * lum;
+                area[i] = a;
+            }
+            for (uint32_t i = 0; i < N;) {
+                if (imp[i] == 0.0f) {
+                    std::swap(ids[N - 1], ids[i]);
+                    std::swap(imp[N - 1], imp[i]);
+                    std::swap(area[N - 1], area[i]);
+                    --N;
+                } else {
+                    ++i;
+                }
+            }
+            for (uint32_t i = 0; i < N; ++i) {
+                Light l;
+                l.type = 3;
+                l.color = Le;
+                l.prim = t0 + ids[i];
+                l.area = area[i];
+                s->tri_light[l.prim] = (int32_t)s->lights.size();
+                s->lights.push_back(l);
+                s->powers.push_back((imp[i] + imp[i] + imp[i]) / 3.0f);
+            }
+        }
+        t0 = t1;
+    }
+    if (!s->lights.empty()) s->cdf = make_cdf(s->powers);
+    return s.release();
+}
+
+// Hit record reconstruction (geometry.cpp:68-163 with an identity instance transform).
+struct SurfHit {
+    V3 p;
+    V3 t, b, n;       // geometric frame
+    V3 st, sb, sn;    // shading frame
+    float thit;
+    uint32_t prim;
+};
+
+SurfHit make_hit(const Scene &s, V3 ray_d, const Hit &h) {
+    SurfHit it;
+    it.thit = h.t;
+    it.prim = (uint32_t)h.prim;
+    uint32_t tr = it.prim;
+    V3 v0 = s.vtx(tr, 0), v1 = s.vtx(tr, 1), v2 = s.vtx(tr, 2);
+    V3 rd = normalized(ray_d);
+    V3 wo = -rd;
+    V3 ng = normalized(cross(v1 - v0, v2 - v0)); // Embree Ng = cross(e2, e1), e1 = v0 - v1, e2 = v2 - v0
+    float w = 1.0f - h.u - h.v;
+    it.p = v0 * w + v1 * h.u + v2 * h.v;
+    V3 dpdu = v1 - v0;
+    V3 b = normalized(cross(ng, dpdu));
+    V3 t = normalized(cross(b, ng));
+    if (dot(wo, ng) < 0.0f) { // two-sided (geometry.cpp:126-132)
+        ng = -ng;
+        dpdu = -dpdu;
+        t = -t;
+    }
+    it.t = t;
+    it.b = b;
+    it.n = ng;
+    it.st = t;
+    it.sb = b;
+    it.sn = ng;
+    if (s.has_normals) {
+        const V3 &n0 = s.normals[s.idx[3 * tr]], &n1 = s.normals[s.idx[3 * tr + 1]], &n2 = s.normals[s.idx[3 * tr + 2]];
+        V3 vn = normalized(n0 * w + n1 * h.u + n2 * h.v);
+        if (all_finite(vn)) {
+            if (dot(vn, ng) < 0.0f) vn = -vn;
+            V3 vb = normalized(cross(vn, dpdu));
+            V3 vt = normalized(cross(vb, vn));
+            it.st = vt;
+            it.sb = vb;
+            it.sn = vn;
+        }
+    }
+    return it;
+}
+
+inline V3 to_local(V3 t, V3 b, V3 n, V3 w) { return V3(dot(t, w), dot(b, w), dot(n, w)); }
+V3 sh_to_local(const SurfHit &it, V3 w) { // ray.h:108-117
+    bool hw = dot(it.n, w) >= 0.0f;
+    V3 l = to_local(it.st, it.sb, it.sn, w);
+    bool hl = l.z >= 0.0f;
+    if (hw != hl) l.z *= -1.0f;
+    return l;
+}
+V3 sh_to_world(const SurfHit &it, V3 l) { // ray.h:119-128
+    bool hl = l.z >= 0.0f;
+    V3 w = it.st * l.x + it.sb * l.y + it.sn * l.z;
+    bool hw = dot(it.n, w) >= 0.0f;
+    if (hl != hw) w = w - it.n * (2.0f * dot(it.n, w));
+    return w;
+}
+
+inline float int_as_float(int i) {
+    float f;
+    std::memcpy(&f, &i, 4);
+    return f;
+}
+inline int float_as_int(float f) {
+    int i;
+    std::memcpy(&i, &f, 4);
+    return i;
+}
+V3 offset_ray(V3 p, V3 n) { // ray.h:66-79
+    constexpr float origin = 1.0f / 32.0f, float_scale = 1.0f / 65536.0f, int_scale = 256.0f;
+    int ox = (int)(int_scale * n.x), oy = (int)(int_scale * n.y), oz = (int)(int_scale * n.z);
+    V3 pi(int_as_float(float_as_int(p.x) + ((p.x < 0) ? -ox : ox)),
+          int_as_float(float_as_int(p.y) + ((p.y < 0) ? -oy : oy)),
+          int_as_float(float_as_int(p.z) + ((p.z < 0) ? -oz : oz)));
+    return V3(std::fabs(p.x) < origin ? p.x + float_scale * n.x : pi.x,
+              std::fabs(p.y) < origin ? p.y + float_scale * n.y : pi.y,
+              std::fabs(p.z) < origin ? p.z + float_scale * n.z : pi.z);
+}
+V3 spawn_origin(V3 p, V3 dir, V3 ng) { return offset_ray(p, dot(dir, ng) > 0.0f ? ng : -ng); } // ray.h:87-96
+
+// Light sampling (light.cpp:59-134, 567-628)
+float sky_pdf(V3 wi) {
+    float theta = std::acos(clampv(wi.y, -1.0f, 1.0f)); // to_spherical_yup (maths.h:477-484)
+    float sin_theta = std::sin(theta);
+    if (sin_theta == 0.0f) return 0.0f;
+    float pdf = 1.0f; // 1x1 DistribTable2D
+    pdf /= (2.0f * kPi * kPi * sin_theta);
+    return pdf;
+}
+
+// Returns Le/pdf (light.h:27-29 convention).
+V3 light_sample(const Scene &s, const Light &l, V3 p, float u0, float u1, V3 &wi, float &wi_dist, float &pdf) {
+    if (l.type == 0) { // SkyLight::sample (light.cpp:88-111)
+        float uu0 = clampv(u0, 0.0f, std::nextafter(1.0f, 0.0f));
+        float uu1 = clampv(u1, 0.0f, std::nextafter(1.0f, 0.0f));
+        pdf = 1.0f;
+        float phi = uu0 * 2.0f * kPi;
+        float theta = uu1 * kPi;
+        float ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+        V3 wl(cp * st, ct, sp * st); // to_cartesian_yup
+        float sin_theta = std::sin(theta);
+        pdf /= (2.0f * kPi * kPi * sin_theta);
+        if (sin_theta == 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        wi = wl;
+        wi_dist = kInf;
+        return l.color / pdf;
+    }
+    if (l.type == 1) { // DirectionalLight::sample (light.cpp:207-213)
+        wi = l.vec;
+        wi_dist = kInf;
+        pdf = 1.0f;
+        return l.color;
+    }
+    if (l.type == 2) { // PointLight::sample (light.cpp:222-233)
+        wi = l.vec - p;
+        float l2 = sqnorm(wi);
+        if (l2 == 0.0f) return V3();
+        float len = std::sqrt(l2);
+        wi = wi / len;
+        wi_dist = len;
+        pdf = 1.0f;
+        return l.color / l2;
+    }
+    // MeshTriLight::sample (light.cpp:567-605)
+    float b0, b1;
+    if (u0 < u1) {
+        b0 = u0 / 2;
+        b1 = u1 - b0;
+    } else {
+        b1 = u1 / 2;
+        b0 = u0 - b1;
+    }
+    V3 v0 = s.vtx(l.prim, 0), v1 = s.vtx(l.prim, 1), v2 = s.vtx(l.prim, 2);
+    V3 pos = v0 * (1.0f - b0 - b1) + v1 * b0 + v2 * b1; // rtcInterpolate0 at (u, v) = (b0, b1)
+    V3 n = normalized(cross(v1 - v0, v2 - v1));
+    wi = pos - p;
+    float d2 = sqnorm(wi);
+    if (d2 == 0.0f) {
+        wi_dist = 0.0f;
+        pdf = 0.0f;
+        return V3();
+    }
+    wi_dist = std::sqrt(d2);
+    wi = wi / wi_dist;
+    pdf = 1.0f / l.area;
+    float absdot = std::fabs(dot(n, wi));
+    if (absdot == 0.0f) {
+        wi_dist = 0.0f;
+        pdf = 0.0f;
+        return V3();
+    }
+    pdf *= (d2 / absdot);
+    wi_dist = std::max(0.0f, wi_dist - 1e-4f);
+    return l.color / pdf;
+}
+
+float mesh_light_pdf(const Scene &s, const Light &l, V3 wi, float wi_dist) { // light.cpp:607-628
+    if (wi_dist == 0.0f) return 0.0f;
+    V3 v0 = s.vtx(l.prim, 0), v1 = s.vtx(l.prim, 1), v2 = s.vtx(l.prim, 2);
+    V3 n = normalized(cross(v1 - v0, v2 - v1));
+    float pdf = 1.0f / l.area;
+    float absdot = std::fabs(dot(n, wi));
+    if (absdot == 0.0f) return 0.0f;
+    pdf *= (sqr(wi_dist) / absdot);
+    return pdf;
+}
+
+struct PathOut {
+    bool hit;
+    V3 L;
+};
+
+// SmallPT::trace (small_pt.cpp:72-178) with Material::sample_with_nee (material.cpp:101-167) and
+// next_event_estimate (nee.cpp:11-58) inlined. mis_mode 1 adds the light selection probability to the
+// NEE weight so the two strategies' weights sum to one.
+PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob) {
+    V3 L, beta(1, 1, 1);
+    float pdf_wi_bsdf = 0.0f;
+    for (int bounce = 0; bounce < p.bounces + 1; ++bounce) {
+        RayQ q;
+        q.o = ro;
+        q.d = rd;
+        q.tmin = 0.0f;
+        q.tmax = kInf;
+        q.setup();
+        Hit h = closest(s, q);
+        if (h.prim < 0) {
+            for (uint32_t li : s.sky_lights) {
+                const Light &l = s.lights[li];
+                V3 wi = normalized(rd);
+                V3 Le = l.color; // SkyLight::eval of a 1x1 ambient map
+                if (bounce == 0) {
+                    if (p.include_background) L = L + beta * Le;
+                } else {
+                    float pr = light_probability(s, li);
+                    float pdf_l = sky_pdf(wi);
+                    float w = power_heur(pdf_wi_bsdf, pdf_l * pr);
+                    V3 bu = Le * w;
+                    if (bounce == 1 && p.clamp_direct > 0.0f) bu = vmin(bu, p.clamp_direct);
+                    else if (bounce >= 2 && p.clamp_indirect > 0.0f) bu = vmin(bu, p.clamp_indirect);
+                    L = L + beta * bu;
+                }
+            }
+            if (bounce == 0 && !p.include_background) return {false, V3()};
+            break;
+        }
+        const ora_material &m = s.mats[s.geom_mat[s.tri_geom[h.prim]]];
+        SurfHit it = make_hit(s, rd, h);
+        if (m.has_emission) {
+            int32_t li = s.tri_light[h.prim];
+            if (li >= 0) {
+                const Light &l = s.lights[li];
+                float pr = light_probability(s, (uint32_t)li);
+                V3 wi = normalized(rd);
+                V3 Le = l.color;
+                if (bounce == 0) {
+                    L = L + beta * Le;
+                } else if (pr > 0.0f) {
+                    float pdf_l = mesh_light_pdf(s, l, wi, it.thit);
+                    float w = power_heur(pdf_wi_bsdf, pdf_l * pr);
+                    V3 bu = Le * w;
+                    if (bounce == 1 && p.clamp_direct > 0.0f) bu = vmin(bu, p.clamp_direct);
+                    else if (bounce >= 2 && p.clamp_indirect > 0.0f) bu = vmin(bu, p.clamp_indirect);
+                    L = L + beta * bu;
+                }
+            }
+        }
+        if (bounce == p.bounces) break;
+        V3 wo = -normalized(rd);
+        Closure c = make_closure(m);
+
+        // NEE (nee.cpp:11-58)
+        V3 Ld;
+        if (!s.lights.empty()) {
+            float pr_light;
+            uint32_t li = cdf_sample(s.cdf, sob.next(), pr_light);
+            const Light &l = s.lights[li];
+            bool delta_light = l.type == 1 || l.type == 2;
+            float u0, u1;
+            sob.next2d(u0, u1);
+            V3 wi;
+            float wi_dist = 0.0f, pdf_l = 0.0f;
+            V3 Le_beta = light_sample(s, l, it.p, u0, u1, wi, wi_dist, pdf_l);
+            if (!is_zero(Le_beta) && pdf_l > 0.0f) {
+                V3 wol = sh_to_local(it, wo), wil = sh_to_local(it, wi);
+                V3 f;
+                float pdf_b;
+                bsdf_eval_and_pdf(wol, wil, c, f, pdf_b);
+                if (!is_zero(f) && pdf_b > 0.0f) {
+                    RayQ sq;
+                    sq.o = spawn_origin(it.p, wi, it.n);
+                    sq.d = wi;
+                    sq.tmin = 0.0f;
+                    sq.tmax = wi_dist;
+                    sq.setup();
+                    if (!occluded(s, sq)) {
+                        float w = 1.0f;
+                        if (!delta_light)
+                            w = p.mis_mode == 1 ? power_heur(pdf_l * pr_light, pdf_b) : power_heur(pdf_l, pdf_b);
+                        Ld = Le_beta * f * w / pr_light;
+                    }
+                }
+            }
+        } else {
+            sob.next();
+            float a, b;
+            sob.next2d(a, b);
+        }
+        if (bounce == 0 && p.clamp_direct > 0.0f) Ld = vmin(Ld, p.clamp_direct);
+        else if (bounce >= 1 && p.clamp_indirect > 0.0f) Ld = vmin(Ld, p.clamp_indirect);
+        L = L + beta * Ld;
+
+        // BSDF sampling (material.cpp:157-166)
+        V3 wol = sh_to_local(it, wo);
+        float u_lobe = sob.next();
+        float u0, u1;
+        sob.next2d(u0, u1);
+        V3 wil;
+        float pdf_wi = 0.0f;
+        V3 sbeta = bsdf_sample(wol, wil, c, u_lobe, u0, u1, pdf_wi);
+        if (maxc(sbeta) == 0.0f || pdf_wi == 0.0f) break;
+        V3 wi = sh_to_world(it, wil);
+        beta = beta * sbeta;
+        if (maxc(beta) == 0.0f) break;
+        pdf_wi_bsdf = pdf_wi;
+        ro = spawn_origin(it.p, wi, it.n);
+        rd = wi;
+        // Russian roulette (small_pt.cpp:167-174)
+        if (maxc(beta) < 1.0f && bounce >= 1) {
+            float qq = std::max(0.0f, 1.0f - maxc(beta));
+            if (sob.next() < qq) break;
+            beta = beta / (1.0f - qq);
+        }
+    }
+    return {true, L};
+}
+
+template <typename F> void parallel_for(int64_t n, int threads, F f) {
+    if (threads <= 1 || n < 64) {
+        for (int64_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    std::vector<std::thread> pool;
+    const int64_t chunk = 64;
+    for (int t = 0; t < threads; ++t)
+        pool.emplace_back([&]() {
+            while (true) {
+                int64_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                int64_t e = std::min(n, b + chunk);
+                for (int64_t i = b; i < e; ++i) f(i);
+            }
+        });
+    for (auto &th : pool) th.join();
+}
+
+int resolve_threads(int t) {
+    if (t > 0) return t;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+void camera_sample(const Camera &cam, const ora_params &p, int px, int py, ZSobol &sob, V3 &o, V3 &d) {
+    // small_pt.cpp:98-105 with BoxPixelFilter (render_target.h:108-125)
+    float ox = 0.0f, oy = 0.0f;
+    if (p.spp != 1) {
+        float u0, u1;
+        sob.next2d(u0, u1);
+        ox = (1.0f - u0) * -0.5f + u0 * 0.5f;
+        oy = (1.0f - u1) * -0.5f + u1 * 0.5f;
+    }
+    float fx = ((float)px + 0.5f + ox) / (float)p.render_width;
+    float fy = ((float)py + 0.5f + oy) / (float)p.render_height;
+    cam.spawn_ray(fx, fy, o, d);
+}
+
+} // namespace
+
+extern "C" {
+
+void *ora_scene_create(const ora_scene_desc *desc) { return create(desc); }
+void ora_scene_free(void *scene) { delete (Scene *)scene; }
+
+int ora_intersect(void *scene, const float *rays, int64_t n, float *out, int threads) {
+    const Scene &s = *(Scene *)scene;
+    parallel_for(n, resolve_threads(threads), [&](int64_t i) {
+        const float *r = rays + 8 * i;
+        RayQ q;
+        q.o = V3(r[0], r[1], r[2]);
+        q.tmin = r[3];
+        q.d = V3(r[4], r[5], r[6]);
+        q.tmax = r[7];
+        q.setup();
+        Hit h = closest(s, q);
+        out[4 * i] = h.t;
+        out[4 * i + 1] = h.u;
+        out[4 * i + 2] = h.v;
+        std::memcpy(&out[4 * i + 3], &h.prim, 4);
+    });
+    return 0;
+}
+
+int ora_intersect_brute(void *scene, const float *rays, int64_t n, float *out, int threads) {
+    const Scene &s = *(Scene *)scene;
+    parallel_for(n, resolve_threads(threads), [&](int64_t i) {
+        const float *r = rays + 8 * i;
+        RayQ q;
+        q.o = V3(r[0], r[1], r[2]);
+        q.tmin = r[3];
+        q.d = V3(r[4], r[5], r[6]);
+        q.tmax = r[7];
+        q.setup();
+        Hit h = brute(s, q);
+        out[4 * i] = h.t;
+        out[4 * i + 1] = h.u;
+        out[4 * i + 2] = h.v;
+        std::memcpy(&out[4 * i + 3], &h.prim, 4);
+    });
+    return 0;
+}
+
+int ora_occluded(void *scene, const float *rays, int64_t n, uint8_t *out, int threads) {
+    const Scene &s = *(Scene *)scene;
+    parallel_for(n, resolve_threads(threads), [&](int64_t i) {
+        const float *r = rays + 8 * i;
+        RayQ q;
+        q.o = V3(r[0], r[1], r[2]);
+        q.tmin = r[3];
+        q.d = V3(r[4], r[5], r[6]);
+        q.tmax = r[7];
+        q.setup();
+        out[i] = occluded(s, q) ? 1 : 0;
+    });
+    return 0;
+}
+
+int ora_light_table(void *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf) {
+    const Scene &s = *(Scene *)scene;
+    int32_t n = (int32_t)s.lights.size();
+    if (n_lights) *n_lights = n;
+    for (int32_t i = 0; i < n; ++i) {
+        if (prim) prim[i] = s.lights[i].type == 3 ? (int32_t)s.lights[i].prim : -1;
+        if (power) power[i] = s.powers[i];
+    }
+    if (cdf)
+        for (int32_t i = 0; i <= n && n > 0; ++i) cdf[i] = s.cdf[i];
+    return 0;
+}
+
+int ora_render(void *scene, const ora_camera *cam_desc, const ora_params *p, double *rgb, double *weights,
+               int threads) {
+    const Scene &s = *(Scene *)scene;
+    Camera cam(*cam_desc);
+    int W = p->render_width, H = p->render_height;
+    // crop (small_pt.cpp:23-35)
+    int cx = 0, cy = 0, cw = W, ch = H;
+    if (p->crop_width > 0 && p->crop_height > 0) {
+        cx = std::max(p->crop_start_x, 0);
+        cy = std::max(p->crop_start_y, 0);
+        cw = std::min(p->crop_width, W - cx);
+        ch = std::min(p->crop_height, H - cy);
+    }
+    std::memset(rgb, 0, sizeof(double) * 3 * (size_t)W * H);
+    std::memset(weights, 0, sizeof(double) * 2 * (size_t)W * H);
+    // parallel_tile_2d with 4x4 tiles (parallel.h:93-113)
+    int ntx = (cw + 3) / 4, nty = (ch + 3) / 4;
+    parallel_for((int64_t)ntx * nty, resolve_threads(threads), [&](int64_t tile) {
+        int tyi = (int)(tile / ntx), txi = (int)(tile - (int64_t)tyi * ntx);
+        for (int oy = tyi * 4; oy < std::min(tyi * 4 + 4, ch); ++oy)
+            for (int ox = txi * 4; ox < std::min(txi * 4 + 4, cw); ++ox) {
+                int px = cx + ox, py = cy + oy;
+                size_t pi = (size_t)py * W + px;
+                for (int smp = 0; smp < p->spp; ++smp) {
+                    ZSobol sob(p->spp, W, H, p->rng_seed, px, py, smp);
+                    V3 o, d;
+                    camera_sample(cam, *p, px, py, sob, o, d);
+                    PathOut r = trace(s, *p, o, d, sob);
+                    if (r.hit) { // RenderTarget2::add / add_miss (render_target.cpp:82-107)
+                        weights[2 * pi] += 1.0;
+                        rgb[3 * pi] += (double)r.L.x;
+                        rgb[3 * pi + 1] += (double)r.L.y;
+                        rgb[3 * pi + 2] += (double)r.L.z;
+                    } else {
+                        weights[2 * pi + 1] += 1.0;
+                    }
+                }
+            }
+    });
+    return 0;
+}
+
+int ora_trace_samples(void *scene, const ora_camera *cam_desc, const ora_params *p, const int32_t *px,
+                      const int32_t *py, const int32_t *sample, int64_t n, float *out, int threads) {
+    const Scene &s = *(Scene *)scene;
+    Camera cam(*cam_desc);
+    parallel_for(n, resolve_threads(threads), [&](int64_t i) {
+        ZSobol sob(p->spp, p->render_width, p->render_height, p->rng_seed, px[i], py[i], sample[i]);
+        V3 o, d;
+        camera_sample(cam, *p, px[i], py[i], sob, o, d);
+        PathOut r = trace(s, *p, o, d, sob);
+        out[4 * i] = r.L.x;
+        out[4 * i + 1] = r.L.y;
+        out[4 * i + 2] = r.L.z;
+        out[4 * i + 3] = r.hit ? 1.0f : 0.0f;
+    });
+    return 0;
+}
+
+int ora_sobol(int width, int height, int spp, int seed, int px, int py, int sample, int n_calls, const int32_t *kinds,
+              uint32_t *out_u32, float *out_f) {
+    ZSobol sob(spp, width, height, seed, px, py, sample);
+    int k = 0;
+    for (int c = 0; c < n_calls; ++c) {
+        if (kinds[c] == 1) {
+            uint32_t v = sob.next_u32();
+            out_u32[k] = v;
+            out_f[k] = u32_to_unit(v);
+            ++k;
+        } else {
+            uint32_t v[2];
+            sob.next2d_u32(v);
+            out_u32[k] = v[0];
+            out_f[k] = u32_to_unit(v[0]);
+            out_u32[k + 1] = v[1];
+            out_f[k + 1] = u32_to_unit(v[1]);
+            k += 2;
+        }
+    }
+    return 0;
+}
+
+uint64_t ora_murmur64a(const void *key, int len, uint64_t seed) {
+    return murmur64a((const unsigned char *)key, (size_t)len, seed);
+}
+uint64_t ora_mix_bits(uint64_t v) { return mix_bits(v); }
+uint32_t ora_owen_scramble(uint32_t v, uint32_t seed) { return owen_scramble(v, seed); }
+void ora_sobol_matrices(uint32_t *out104) { std::memcpy(out104, g_sobol.m, sizeof(g_sobol.m)); }
+
+int ora_camera_rays(const ora_camera *cam_desc, int width, int height, int spp, int seed, const int32_t *px,
+                    const int32_t *py, const int32_t *sample, int64_t n, float *rays) {
+    Camera cam(*cam_desc);
+    ora_params p{};
+    p.render_width = width;
+    p.render_height = height;
+    p.spp = spp;
+    for (int64_t i = 0; i < n; ++i) {
+        ZSobol sob(spp, width, height, seed, px[i], py[i], sample[i]);
+        V3 o, d;
+        camera_sample(cam, p, px[i], py[i], sob, o, d);
+        float *r = rays + 6 * i;
+        r[0] = o.x;
+        r[1] = o.y;
+        r[2] = o.z;
+        r[3] = d.x;
+        r[4] = d.y;
+        r[5] = d.z;
+    }
+    return 0;
+}
+
+int ora_camera_matrices(const ora_camera *cam_desc, float *p2w, float *pos) {
+    Camera cam(*cam_desc);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) p2w[4 * i + j] = (float)cam.proj_to_world.a[i][j];
+    pos[0] = cam.position.x;
+    pos[1] = cam.position.y;
+    pos[2] = cam.position.z;
+    return 0;
+}
+
+int ora_bsdf_eval(const ora_material *m, const float *wo, const float *wi, int64_t n, float *f, float *pdf) {
+    Closure c = make_closure(*m);
+    for (int64_t i = 0; i < n; ++i) {
+        V3 o(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), w(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        V3 ff = bsdf_eval(o, w, c);
+        f[3 * i] = ff.x;
+        f[3 * i + 1] = ff.y;
+        f[3 * i + 2] = ff.z;
+        pdf[i] = bsdf_pdf(o, w, c);
+    }
+    return 0;
+}
+
+int ora_bsdf_sample(const ora_material *m, const float *wo, const float *u, int64_t n, float *wi, float *beta,
+                    float *pdf) {
+    Closure c = make_closure(*m);
+    for (int64_t i = 0; i < n; ++i) {
+        V3 o(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+        V3 w;
+        float pd = 0.0f;
+        V3 b = bsdf_sample(o, w, c, u[3 * i], u[3 * i + 1], u[3 * i + 2], pd);
+        wi[3 * i] = w.x;
+        wi[3 * i + 1] = w.y;
+        wi[3 * i + 2] = w.z;
+        beta[3 * i] = b.x;
+        beta[3 * i + 1] = b.y;
+        beta[3 * i + 2] = b.z;
+        pdf[i] = pd;
+    }
+    return 0;
+}
+
+} // extern "C"

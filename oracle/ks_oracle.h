@@ -1,0 +1,133 @@
+/*
+ * CPU oracle for the keepsake small_pt path (test infrastructure only).
+ *
+ * This is a plain C++ restatement of the reference CPU integrator and the pieces it calls
+ * (small_pt.cpp, sobol.h, hash.h, rng.h, camera.cpp, ray.h, geometry.cpp, light.cpp, nee.cpp,
+ * material.cpp, principled_bsdf.cpp, microfacet.h, fresnel.h, distrib.cpp, render_target.cpp,
+ * parallel.h, cuda/ray_tri.cuh, cuda/aabb.cuh). Each function in ks_oracle.cpp cites the reference
+ * file:line it follows.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library, and
+ * only as the checker or as the timed CPU baseline -- never as the product path.
+ *
+ * Embree (the reference's closest-hit/occlusion engine) is not available; the oracle uses its own
+ * binned-SAH BVH (object splits only, independent of the product's SBVH) with the reference's
+ * watertight triangle test and traversal rules (strict t < tmax update, first-found wins on ties).
+ *
+ * Pinning status per function (what checks each one; details in the tests):
+ *   ora_sobol          - MurmurHash64A KAT, generator-matrix closed forms checked against the reference
+ *                        table text, (0,m,2)-net stratification per pixel, digit-permutation bijection.
+ *                        The split of `dimension` between digit permutation and scramble hash is pinned
+ *                        only by the restatement itself (parity partially unpinned).
+ *   ora_camera_rays    - closed-form pinhole direction (independent of the matrix inverse path).
+ *   ora_intersect      - brute force over all triangles; watertightness on shared edges.
+ *   ora_bsdf_*         - FP64 closed forms (Lambert/Disney retro, GGX normalisation), sample weight ==
+ *                        eval/pdf, pdf integrates to 1 by quadrature.
+ *   ora_render         - furnace/closed-form scenes (uniform emitter enclosure), equal-spp comparison is
+ *                        the parity check for the GPU path.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirrors PrincipledBSDF closure inputs (principled_bsdf.h:78-93) + Material::emission (material.h:45). */
+typedef struct ora_material {
+    float basecolor[3];
+    float roughness;
+    float metallic;
+    float ior;
+    float specular_r0_mul; /* raw [0,1] field value; the closure doubles it (principled_bsdf.cpp:258) */
+    float specular_trans;
+    float diffuse_trans;
+    float diffuse_trans_fwd;
+    float emission[3];
+    int32_t has_emission;
+} ora_material;
+
+/* Non-mesh lights (light.h:34-120): 0 = ambient SkyLight, 1 = DirectionalLight, 2 = PointLight. */
+typedef struct ora_light {
+    int32_t type;
+    float color[3]; /* ambient radiance / L / I */
+    float vec[3];   /* direction (directional) or position (point) */
+} ora_light;
+
+typedef struct ora_scene_desc {
+    const float *vertices; /* 3 * n_vertices */
+    uint32_t n_vertices;
+    const float *normals; /* optional per-vertex normals, 3 * n_vertices, or NULL */
+    const uint32_t *indices; /* 3 * n_tris */
+    uint32_t n_tris;
+    const uint32_t *tri_geom; /* geometry id per triangle, triangles of one geometry contiguous */
+    const uint32_t *geom_material;
+    uint32_t n_geoms;
+    const ora_material *materials;
+    uint32_t n_materials;
+    const ora_light *lights;
+    uint32_t n_lights;
+} ora_scene_desc;
+
+/* create_camera (camera.cpp:172-205): look_at(pos, target, up) + perspective vfov (degrees) / aspect. */
+typedef struct ora_camera {
+    float pos[3];
+    float target[3];
+    float up[3];
+    float vfov_deg;
+    float aspect;
+} ora_camera;
+
+/* SmallPTInput (small_pt.h:17-41) plus mis_mode (0 = reference weights, 1 = consistent). */
+typedef struct ora_params {
+    int32_t bounces;
+    int32_t render_width, render_height;
+    int32_t crop_start_x, crop_start_y, crop_width, crop_height;
+    int32_t spp;
+    int32_t rng_seed;
+    int32_t include_background;
+    float backdrop[3];
+    float clamp_direct, clamp_indirect;
+    int32_t mis_mode;
+} ora_params;
+
+void *ora_scene_create(const ora_scene_desc *desc);
+void ora_scene_free(void *scene);
+/* rays: n x 8 floats (o.xyz, tmin, d.xyz, tmax); out: n x 4 (t, u, v, prim as int bits, -1 = miss) */
+int ora_intersect(void *scene, const float *rays, int64_t n, float *out, int threads);
+int ora_intersect_brute(void *scene, const float *rays, int64_t n, float *out, int threads);
+int ora_occluded(void *scene, const float *rays, int64_t n, uint8_t *out, int threads);
+/* Light table as the sampler sees it: flat order, prim id (-1 non-mesh), cdf (n+1). */
+int ora_light_table(void *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf);
+/* rgb: W*H*3 FP64 sums, weights: W*H*2 FP64 (hit, miss) -- RenderTarget2 (render_target.h:224-293). */
+int ora_render(void *scene, const ora_camera *cam, const ora_params *p, double *rgb, double *weights,
+               int threads);
+/* Scalar per-sample radiance trace for given (pixel, sample) list: out n x 4 (rgb, hit). */
+int ora_trace_samples(void *scene, const ora_camera *cam, const ora_params *p, const int32_t *px,
+                      const int32_t *py, const int32_t *sample, int64_t n, float *out, int threads);
+
+/* PTRenderSampler stream for one (pixel, sample): kinds[i] = 1 (next) or 2 (next2d).
+ * out_u32/out_f receive 1 or 2 values per call, in order. */
+int ora_sobol(int width, int height, int spp, int seed, int px, int py, int sample, int n_calls,
+              const int32_t *kinds, uint32_t *out_u32, float *out_f);
+uint64_t ora_murmur64a(const void *key, int len, uint64_t seed);
+uint64_t ora_mix_bits(uint64_t v);
+uint32_t ora_owen_scramble(uint32_t v, uint32_t seed);
+void ora_sobol_matrices(uint32_t *out104);
+
+/* Camera rays for (pixel, sample) list with pixel-filter jitter from the sampler (small_pt.cpp:98-105);
+ * rays: n x 6 (o.xyz, d.xyz). */
+int ora_camera_rays(const ora_camera *cam, int width, int height, int spp, int seed, const int32_t *px,
+                    const int32_t *py, const int32_t *sample, int64_t n, float *rays);
+/* Camera matrices: proj_to_world (row-major 4x4) and position. */
+int ora_camera_matrices(const ora_camera *cam, float *proj_to_world16, float *position3);
+
+/* PrincipledBSDF on local directions: f (rgb, cosine-weighted), pdf. */
+int ora_bsdf_eval(const ora_material *m, const float *wo, const float *wi, int64_t n, float *f, float *pdf);
+/* u: n x 3 (u_lobe, u_wi.x, u_wi.y) -> wi (n x 3), beta (n x 3), pdf (n). */
+int ora_bsdf_sample(const ora_material *m, const float *wo, const float *u, int64_t n, float *wi, float *beta,
+                    float *pdf);
+
+#ifdef __cplusplus
+}
+#endif

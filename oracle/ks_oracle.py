@@ -1,0 +1,326 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libks_oracle.so). Test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, as the
+checker or as the timed CPU baseline. The scene arrays it consumes are the same flat arrays the GPU
+library consumes (keepsake_amd.api.HostScene).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libks_oracle.so"
+
+
+class OraMaterial(C.Structure):
+    _fields_ = [
+        ("basecolor", C.c_float * 3),
+        ("roughness", C.c_float),
+        ("metallic", C.c_float),
+        ("ior", C.c_float),
+        ("specular_r0_mul", C.c_float),
+        ("specular_trans", C.c_float),
+        ("diffuse_trans", C.c_float),
+        ("diffuse_trans_fwd", C.c_float),
+        ("emission", C.c_float * 3),
+        ("has_emission", C.c_int32),
+    ]
+
+
+class OraLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("vec", C.c_float * 3)]
+
+
+class OraSceneDesc(C.Structure):
+    _fields_ = [
+        ("vertices", C.c_void_p),
+        ("n_vertices", C.c_uint32),
+        ("normals", C.c_void_p),
+        ("indices", C.c_void_p),
+        ("n_tris", C.c_uint32),
+        ("tri_geom", C.c_void_p),
+        ("geom_material", C.c_void_p),
+        ("n_geoms", C.c_uint32),
+        ("materials", C.c_void_p),
+        ("n_materials", C.c_uint32),
+        ("lights", C.c_void_p),
+        ("n_lights", C.c_uint32),
+    ]
+
+
+class OraCamera(C.Structure):
+    _fields_ = [
+        ("pos", C.c_float * 3),
+        ("target", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("vfov_deg", C.c_float),
+        ("aspect", C.c_float),
+    ]
+
+
+class OraParams(C.Structure):
+    _fields_ = [
+        ("bounces", C.c_int32),
+        ("render_width", C.c_int32),
+        ("render_height", C.c_int32),
+        ("crop_start_x", C.c_int32),
+        ("crop_start_y", C.c_int32),
+        ("crop_width", C.c_int32),
+        ("crop_height", C.c_int32),
+        ("spp", C.c_int32),
+        ("rng_seed", C.c_int32),
+        ("include_background", C.c_int32),
+        ("backdrop", C.c_float * 3),
+        ("clamp_direct", C.c_float),
+        ("clamp_indirect", C.c_float),
+        ("mis_mode", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_SIGS = {
+    "ora_scene_create": ([C.POINTER(OraSceneDesc)], _P),
+    "ora_scene_free": ([_P], None),
+    "ora_intersect": ([_P, _P, C.c_int64, _P, C.c_int], C.c_int),
+    "ora_intersect_brute": ([_P, _P, C.c_int64, _P, C.c_int], C.c_int),
+    "ora_occluded": ([_P, _P, C.c_int64, _P, C.c_int], C.c_int),
+    "ora_light_table": ([_P, C.POINTER(C.c_int32), _P, _P, _P], C.c_int),
+    "ora_render": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, C.c_int], C.c_int),
+    "ora_trace_samples": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, _P, C.c_int64, _P, C.c_int], C.c_int),
+    "ora_sobol": ([C.c_int] * 8 + [_P, _P, _P], C.c_int),
+    "ora_murmur64a": ([_P, C.c_int, C.c_uint64], C.c_uint64),
+    "ora_mix_bits": ([C.c_uint64], C.c_uint64),
+    "ora_owen_scramble": ([C.c_uint32, C.c_uint32], C.c_uint32),
+    "ora_sobol_matrices": ([_P], None),
+    "ora_camera_rays": ([C.POINTER(OraCamera), C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_camera_matrices": ([C.POINTER(OraCamera), _P, _P], C.c_int),
+    "ora_bsdf_eval": ([C.POINTER(OraMaterial), _P, _P, C.c_int64, _P, _P], C.c_int),
+    "ora_bsdf_sample": ([C.POINTER(OraMaterial), _P, _P, C.c_int64, _P, _P, _P], C.c_int),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} missing: run `make -C oracle`")
+        lib = C.CDLL(str(LIB_PATH))
+        for n, (a, r) in _SIGS.items():
+            f = getattr(lib, n)
+            f.argtypes = a
+            f.restype = r
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+def threads_default() -> int:
+    return int(os.environ.get("KS_ORACLE_THREADS", "0"))
+
+
+def camera(cam: dict) -> OraCamera:
+    c = OraCamera()
+    for i in range(3):
+        c.pos[i] = float(cam["pos"][i])
+        c.target[i] = float(cam["target"][i])
+        c.up[i] = float(cam.get("up", (0.0, 1.0, 0.0))[i])
+    c.vfov_deg = float(cam["vfov"])
+    c.aspect = float(cam["aspect"])
+    return c
+
+
+def params(**kw) -> OraParams:
+    p = OraParams()
+    p.bounces = kw.get("bounces", 1)
+    p.render_width = kw.get("render_width", 256)
+    p.render_height = kw.get("render_height", 256)
+    p.crop_start_x = kw.get("crop_start_x", 0)
+    p.crop_start_y = kw.get("crop_start_y", 0)
+    p.crop_width = kw.get("crop_width", 0)
+    p.crop_height = kw.get("crop_height", 0)
+    p.spp = kw.get("spp", 1)
+    p.rng_seed = kw.get("rng_seed", 0)
+    p.include_background = int(kw.get("include_background", True))
+    bd = kw.get("backdrop", (0.0, 0.0, 0.0))
+    for i in range(3):
+        p.backdrop[i] = float(bd[i])
+    p.clamp_direct = kw.get("clamp_direct", 0.0)
+    p.clamp_indirect = kw.get("clamp_indirect", 10.0)
+    p.mis_mode = kw.get("mis_mode", 0)
+    return p
+
+
+def material(rec) -> OraMaterial:
+    m = OraMaterial()
+    b = np.asarray(rec).tobytes()
+    assert len(b) == C.sizeof(m)
+    C.memmove(C.byref(m), b, C.sizeof(m))
+    return m
+
+
+class OracleScene:
+    def __init__(self, vertices, indices, tri_geom, geom_material, materials, lights, normals=None):
+        lib = load()
+        self.lib = lib
+        self._arrs = [
+            np.ascontiguousarray(vertices, np.float32),
+            np.ascontiguousarray(indices, np.uint32),
+            np.ascontiguousarray(tri_geom, np.uint32),
+            np.ascontiguousarray(geom_material, np.uint32),
+            np.ascontiguousarray(materials),
+            np.ascontiguousarray(lights),
+            None if normals is None else np.ascontiguousarray(normals, np.float32),
+        ]
+        v, idx, tg, gm, mats, lts, nrm = self._arrs
+        d = OraSceneDesc()
+        d.vertices = v.ctypes.data
+        d.n_vertices = v.shape[0]
+        d.normals = nrm.ctypes.data if nrm is not None else None
+        d.indices = idx.ctypes.data
+        d.n_tris = idx.shape[0]
+        d.tri_geom = tg.ctypes.data
+        d.geom_material = gm.ctypes.data
+        d.n_geoms = gm.shape[0]
+        d.materials = mats.ctypes.data
+        d.n_materials = mats.shape[0]
+        d.lights = lts.ctypes.data if lts.shape[0] else None
+        d.n_lights = lts.shape[0]
+        self.h = lib.ora_scene_create(C.byref(d))
+
+    @classmethod
+    def from_host_scene(cls, hs) -> "OracleScene":
+        return cls(hs.vertices, hs.indices, hs.tri_geom, hs.geom_material, hs.materials, hs.lights, hs.normals)
+
+    def close(self):
+        if self.h:
+            self.lib.ora_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def intersect(self, rays, brute=False, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((rays.shape[0], 4), np.float32)
+        fn = self.lib.ora_intersect_brute if brute else self.lib.ora_intersect
+        fn(self.h, _p(rays), rays.shape[0], _p(out), threads_default() if threads is None else threads)
+        return out
+
+    def occluded(self, rays, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros(rays.shape[0], np.uint8)
+        self.lib.ora_occluded(self.h, _p(rays), rays.shape[0], _p(out), threads_default() if threads is None else threads)
+        return out
+
+    def light_table(self):
+        n = C.c_int32(0)
+        self.lib.ora_light_table(self.h, C.byref(n), None, None, None)
+        prim = np.zeros(n.value, np.int32)
+        power = np.zeros(n.value, np.float32)
+        cdf = np.zeros(n.value + 1, np.float32)
+        self.lib.ora_light_table(self.h, C.byref(n), _p(prim), _p(power), _p(cdf))
+        return prim, power, cdf
+
+    def render(self, cam: dict, prm: OraParams, threads=None):
+        W, H = prm.render_width, prm.render_height
+        rgb = np.zeros((H, W, 3), np.float64)
+        w = np.zeros((H, W, 2), np.float64)
+        c = camera(cam)
+        self.lib.ora_render(self.h, C.byref(c), C.byref(prm), _p(rgb), _p(w), threads_default() if threads is None else threads)
+        return rgb, w
+
+    def trace_samples(self, cam: dict, prm: OraParams, px, py, sample, threads=None):
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        out = np.zeros((px.shape[0], 4), np.float32)
+        c = camera(cam)
+        self.lib.ora_trace_samples(
+            self.h, C.byref(c), C.byref(prm), _p(px), _p(py), _p(sample), px.shape[0], _p(out),
+            threads_default() if threads is None else threads,
+        )
+        return out
+
+
+def sobol(width, height, spp, seed, px, py, sample, kinds):
+    lib = load()
+    kinds = np.ascontiguousarray(kinds, np.int32)
+    nv = int(kinds.sum())
+    ou = np.zeros(nv, np.uint32)
+    of = np.zeros(nv, np.float32)
+    lib.ora_sobol(width, height, spp, seed, px, py, sample, kinds.shape[0], _p(kinds), _p(ou), _p(of))
+    return ou, of
+
+
+def murmur64a(data: bytes, seed: int = 0) -> int:
+    lib = load()
+    buf = C.create_string_buffer(data, len(data))
+    return lib.ora_murmur64a(C.cast(buf, C.c_void_p), len(data), seed)
+
+
+def mix_bits(v: int) -> int:
+    return load().ora_mix_bits(v)
+
+
+def owen_scramble(v: int, seed: int) -> int:
+    return load().ora_owen_scramble(v, seed)
+
+
+def sobol_matrices() -> np.ndarray:
+    out = np.zeros(104, np.uint32)
+    load().ora_sobol_matrices(_p(out))
+    return out
+
+
+def camera_rays(cam, width, height, spp, seed, px, py, sample):
+    lib = load()
+    px = np.ascontiguousarray(px, np.int32)
+    py = np.ascontiguousarray(py, np.int32)
+    sample = np.ascontiguousarray(sample, np.int32)
+    out = np.zeros((px.shape[0], 6), np.float32)
+    c = camera(cam)
+    lib.ora_camera_rays(C.byref(c), width, height, spp, seed, _p(px), _p(py), _p(sample), px.shape[0], _p(out))
+    return out
+
+
+def camera_matrices(cam):
+    p2w = np.zeros(16, np.float32)
+    pos = np.zeros(3, np.float32)
+    c = camera(cam)
+    load().ora_camera_matrices(C.byref(c), _p(p2w), _p(pos))
+    return p2w.reshape(4, 4), pos
+
+
+def bsdf_eval(mat, wo, wi):
+    m = material(mat)
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    f = np.zeros((wo.shape[0], 3), np.float32)
+    pdf = np.zeros(wo.shape[0], np.float32)
+    load().ora_bsdf_eval(C.byref(m), _p(wo), _p(wi), wo.shape[0], _p(f), _p(pdf))
+    return f, pdf
+
+
+def bsdf_sample(mat, wo, u):
+    m = material(mat)
+    wo = np.ascontiguousarray(wo, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    n = wo.shape[0]
+    wi = np.zeros((n, 3), np.float32)
+    beta = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    load().ora_bsdf_sample(C.byref(m), _p(wo), _p(u), n, _p(wi), _p(beta), _p(pdf))
+    return wi, beta, pdf
