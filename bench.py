@@ -20,6 +20,7 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+sys.path.insert(1, str(ROOT / "oracle"))  # the CPU oracle, used only by the cpu_baseline leg
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one closest-hit query: ray in (2 x 16 B), hit out (16 B), each node fetch
@@ -70,7 +71,7 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--bounces", type=int, default=12)
     ap.add_argument("--queue", type=int, default=0)
-    ap.add_argument("--cpu-rows", type=int, default=16)
+    ap.add_argument("--cpu-rows", type=int, default=96)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

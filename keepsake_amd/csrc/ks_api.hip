@@ -119,7 +119,7 @@ struct RenderConsts {
     float clamp_direct, clamp_indirect;
     int32_t s0, nb;            // sample range of the batch: [s0, s0 + nb)
     const int4 *work_list;     // optional explicit (x, y, sample) per work item
-    float p2w[16];
+    float p2w[16]; // proj_to_world with the eye folded into rows 0..2 (see camera_ray)
     float cam[3];
 };
 
@@ -180,14 +180,15 @@ KS_DEV void camera_ray(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &so
     float fx = ((float)x + 0.5f + ox) / (float)rc.W;
     float fy = ((float)y + 0.5f + oy) / (float)rc.H;
     float nx = fx * 2.0f - 1.0f, ny = fy * 2.0f - 1.0f;
+    // rows 0..2 of rc.p2w hold proj_to_world rows minus eye * row 3 (folded on the host in FP64), so
+    // (world_pos - eye) comes out without cancelling two nearly equal float positions
     const float *m = rc.p2w;
     float hx = m[0] * nx + m[1] * ny + m[2] + m[3];
     float hy = m[4] * nx + m[5] * ny + m[6] + m[7];
     float hz = m[8] * nx + m[9] * ny + m[10] + m[11];
     float hw = m[12] * nx + m[13] * ny + m[14] + m[15];
-    f3 wp = mk(hx / hw, hy / hw, hz / hw);
     o = mk(rc.cam[0], rc.cam[1], rc.cam[2]);
-    d = normalized(wp - o);
+    d = normalized(mk(hx / hw, hy / hw, hz / hw));
 }
 
 KS_DEV void sobol_start(const RenderConsts &rc, Sobol &sob, uint32_t x, uint32_t y, uint32_t s, int dim) {
@@ -819,9 +820,12 @@ int camera_setup(const ks_camera_desc &c, float p2w[16], float pos[3]) {
     KS_REQUIRE(inverse(c2w, w2c), "camera to_world is singular");
     M4 w2p = mul(proj, w2c);
     KS_REQUIRE(inverse(w2p, p2w_m), "camera projection is singular");
-    for (int i = 0; i < 4; ++i)
-        for (int j = 0; j < 4; ++j) p2w[4 * i + j] = (float)p2w_m.a[i][j];
-    pos[0] = (float)c2w.a[0][3], pos[1] = (float)c2w.a[1][3], pos[2] = (float)c2w.a[2][3];
+    double eye[3] = {c2w.a[0][3], c2w.a[1][3], c2w.a[2][3]};
+    for (int j = 0; j < 4; ++j) {
+        for (int i = 0; i < 3; ++i) p2w[4 * i + j] = (float)(p2w_m.a[i][j] - eye[i] * p2w_m.a[3][j]);
+        p2w[12 + j] = (float)p2w_m.a[3][j];
+    }
+    pos[0] = (float)eye[0], pos[1] = (float)eye[1], pos[2] = (float)eye[2];
     return KS_OK;
 }
 

@@ -378,10 +378,12 @@ KS_DEV f3 eval_metallic_specular(f3 wo, f3 wi, const Closure &c) {
     f3 Fr = mk(lerpf(c.basecolor.x, 1.0f, fs), lerpf(c.basecolor.y, 1.0f, fs), lerpf(c.basecolor.z, 1.0f, fs));
     return Fr * (c.metallic * D * G / (4.0f * wo.z));
 }
-KS_DEV float dielectric_specular_adjust(float wo_dot_wh, const Closure &c) {
+KS_DEV float dielectric_specular_adjust(float wo_dot_wh, const Closure &c) { // principled_bsdf.cpp:510-516
     float R0 = sqr((1.0f - c.ior) / (1.0f + c.ior));
     float schlick = fresnel_schlick(wo_dot_wh);
-    return lerpf(c.specular_r0_mul * R0, 1.0f, schlick) / lerpf(R0, 1.0f, schlick);
+    float den = lerpf(R0, 1.0f, schlick);
+    // ior == 1 with |wo.wh| == 1 is 0/0 in the reference; its continuous limit is 1
+    return den > 0.0f ? lerpf(c.specular_r0_mul * R0, 1.0f, schlick) / den : 1.0f;
 }
 KS_DEV f3 eval_dielectric_specular(f3 wo, f3 wi, const Closure &c) {
     float lw = 1.0f - c.metallic;

@@ -439,7 +439,10 @@ V3 eval_metallic_specular(V3 wo, V3 wi, const Closure &c) { // :447-460
 float dielectric_specular_adjust(float wo_dot_wh, const Closure &c) { // :510-516
     float R0 = sqr((1.0f - c.ior) / (1.0f + c.ior));
     float schlick = fresnel_schlick(wo_dot_wh);
-    return lerpf(c.specular_r0_mul * R0, 1.0f, schlick) / lerpf(R0, 1.0f, schlick);
+    float den = lerpf(R0, 1.0f, schlick);
+    // ior == 1 and |wo.wh| == 1 give 0/0 in the reference (NaN, then its non-finite monitor aborts);
+    // the continuous limit there is 1.
+    return den > 0.0f ? lerpf(c.specular_r0_mul * R0, 1.0f, schlick) / den : 1.0f;
 }
 V3 eval_dielectric_specular(V3 wo, V3 wi, const Closure &c) { // :462-508
     float lw = 1.0f - c.metallic;

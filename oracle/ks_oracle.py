@@ -15,6 +15,9 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "libks_oracle.so"
+# Same sources built with FMA contraction: comparing the two shows how sensitive a reference formula is
+# to float evaluation order at a given input.
+LIB_PATH_FMA = HERE / "_build" / "libks_oracle_fma.so"
 
 
 class OraMaterial(C.Structure):
@@ -103,21 +106,22 @@ _SIGS = {
     "ora_bsdf_sample": ([C.POINTER(OraMaterial), _P, _P, C.c_int64, _P, _P, _P], C.c_int),
 }
 
-_lib = None
+_libs: dict = {}
 
 
-def load() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
-            raise FileNotFoundError(f"{LIB_PATH} missing: run `make -C oracle`")
-        lib = C.CDLL(str(LIB_PATH))
+def load(variant: str = "") -> C.CDLL:
+    """variant "" = no FMA contraction (default), "fma" = FMA-contracted build of the same sources."""
+    if variant not in _libs:
+        path = LIB_PATH_FMA if variant == "fma" else LIB_PATH
+        if not path.exists():
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        lib = C.CDLL(str(path))
         for n, (a, r) in _SIGS.items():
             f = getattr(lib, n)
             f.argtypes = a
             f.restype = r
-        _lib = lib
-    return _lib
+        _libs[variant] = lib
+    return _libs[variant]
 
 
 def _p(a: np.ndarray) -> C.c_void_p:
@@ -304,17 +308,17 @@ def camera_matrices(cam):
     return p2w.reshape(4, 4), pos
 
 
-def bsdf_eval(mat, wo, wi):
+def bsdf_eval(mat, wo, wi, variant: str = ""):
     m = material(mat)
     wo = np.ascontiguousarray(wo, np.float32)
     wi = np.ascontiguousarray(wi, np.float32)
     f = np.zeros((wo.shape[0], 3), np.float32)
     pdf = np.zeros(wo.shape[0], np.float32)
-    load().ora_bsdf_eval(C.byref(m), _p(wo), _p(wi), wo.shape[0], _p(f), _p(pdf))
+    load(variant).ora_bsdf_eval(C.byref(m), _p(wo), _p(wi), wo.shape[0], _p(f), _p(pdf))
     return f, pdf
 
 
-def bsdf_sample(mat, wo, u):
+def bsdf_sample(mat, wo, u, variant: str = ""):
     m = material(mat)
     wo = np.ascontiguousarray(wo, np.float32)
     u = np.ascontiguousarray(u, np.float32)
@@ -322,5 +326,5 @@ def bsdf_sample(mat, wo, u):
     wi = np.zeros((n, 3), np.float32)
     beta = np.zeros((n, 3), np.float32)
     pdf = np.zeros(n, np.float32)
-    load().ora_bsdf_sample(C.byref(m), _p(wo), _p(u), n, _p(wi), _p(beta), _p(pdf))
+    load(variant).ora_bsdf_sample(C.byref(m), _p(wo), _p(u), n, _p(wi), _p(beta), _p(pdf))
     return wi, beta, pdf

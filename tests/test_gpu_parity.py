@@ -1,8 +1,8 @@
 """GPU parity: the gfx950 kernels (through the C ABI) against the CPU oracle on the same inputs.
 
-Bars: Sobol points bit-exact (uint32 and float); camera rays within 2e-6 absolute; closest-hit
+Bars: Sobol points bit-exact (uint32 and float); camera rays within 1e-6 of the FP64 pinhole; closest-hit
 primitive ids equal on >= 99.999% of primary rays (equal-t coplanar ties count as agreement) with t
-within 1e-5 relative; BSDF values within 1e-4 relative; light tables equal; renders within Monte Carlo
+within 1e-5 relative; BSDF values within 1e-4 relative plus the formula's float conditioning; light tables equal; renders within Monte Carlo
 noise of the oracle at equal spp.
 """
 
@@ -37,16 +37,46 @@ def test_sobol_bit_exact(device, ora, W, H, spp, seed):
         assert np.array_equal(gf[i].view(np.uint32), of.view(np.uint32))
 
 
+def pinhole_dirs(ora, cam, W, H, spp, seed, px, py, s):
+    """Closed-form pinhole direction in FP64: R (tan(v/2) a x_ndc, -tan(v/2) y_ndc, -1), R = look_at
+    basis, with the box-filter jitter of the (bit-exact) sampler."""
+    pos, tgt, up = (np.asarray(cam[k], np.float64) for k in ("pos", "target", "up"))
+    back = (pos - tgt) / np.linalg.norm(pos - tgt)
+    up = up / np.linalg.norm(up)
+    right = np.cross(up, back)
+    right /= np.linalg.norm(right)
+    up = np.cross(back, right)
+    t = np.tan(np.deg2rad(np.float64(cam["vfov"])) / 2)
+    out = np.zeros((len(px), 3))
+    for i in range(len(px)):
+        jx = jy = 0.0
+        if spp != 1:
+            _, u = ora.sobol(W, H, spp, seed, int(px[i]), int(py[i]), int(s[i]), [2])
+            jx, jy = (1 - u[0]) * -0.5 + u[0] * 0.5, (1 - u[1]) * -0.5 + u[1] * 0.5
+        xn = 2 * (px[i] + 0.5 + jx) / W - 1
+        yn = 2 * (py[i] + 0.5 + jy) / H - 1
+        d = right * (t * cam["aspect"] * xn) - up * (t * yn) - back
+        out[i] = d / np.linalg.norm(d)
+    return out
+
+
 def test_camera_rays(device, ora, cornell, spheres_small):
     for hs, W, H, spp in ((cornell, 256, 256, 16), (spheres_small, 1920, 1080, 64), (cornell, 64, 64, 1)):
         rng = np.random.default_rng(1)
-        n = 4096
+        n = 2048
         px = rng.integers(0, W, n).astype(np.int32)
         py = rng.integers(0, H, n).astype(np.int32)
         s = rng.integers(0, spp, n).astype(np.int32)
         g = device.camera_rays(hs.camera, W, H, spp, 0, px, py, s)
         o = ora.camera_rays(hs.camera, W, H, spp, 0, px, py, s)
-        assert np.abs(g - o).max() < 2e-6, np.abs(g - o).max()
+        exact = pinhole_dirs(ora, hs.camera, W, H, spp, 0, px, py, s)
+        assert np.array_equal(g[:, :3], o[:, :3])  # origin = eye
+        # the GPU direction is accurate to float rounding ...
+        assert np.abs(g[:, 3:] - exact).max() < 1e-6, np.abs(g[:, 3:] - exact).max()
+        # ... and the oracle carries the reference's own float noise: it subtracts the float eye from a
+        # float near-plane point 0.1 away (camera.cpp:150-151), an error of up to ulp(|eye|) / 0.1
+        noise = 2 * np.spacing(np.float32(np.abs(hs.camera["pos"]).max() + 0.1)) / 0.1
+        assert np.abs(g[:, 3:] - o[:, 3:]).max() < noise + 1e-6
 
 
 def _primary_rays(ora, hs, W, H, spp, n, seed=0):
@@ -140,25 +170,78 @@ def _materials():
     return m
 
 
+def _spread(fn, args, fn_fma):
+    """How far two correct float evaluations of a reference formula may disagree at each input: the
+    larger of (a) the change of the outputs when any single input coordinate moves by one float32 ulp
+    and (b) the difference between the oracle built without and with FMA contraction."""
+    base = fn(*args)
+    alt = fn_fma(*args)
+    spread = [np.nan_to_num(np.abs(alt[j].astype(np.float64) - base[j]), nan=np.inf) for j in range(len(base))]
+    for ai in range(len(args)):
+        for k in range(args[ai].shape[1]):
+            for direction in (np.inf, -np.inf):
+                a = [x.copy() for x in args]
+                a[ai][:, k] = np.nextafter(a[ai][:, k], np.float32(direction))
+                out = fn(*a)
+                for j in range(len(base)):
+                    d = np.abs(out[j].astype(np.float64) - base[j])
+                    spread[j] = np.maximum(spread[j], np.nan_to_num(d, nan=np.inf))
+    return base, spread
+
+
+def _within(g, o, spread, rel=1e-4):
+    rel = np.asarray(rel, np.float64)
+    if rel.ndim == 1 and o.ndim == 2:
+        rel = rel[:, None]
+    tol = rel * np.abs(o) + 1e-6 + 16.0 * spread
+    ok = np.abs(g.astype(np.float64) - o) <= tol
+    return ok.reshape(ok.shape[0], -1).all(axis=1)
+
+
+def _ggx_condition(mat, wo, wi):
+    """Relative change of the GGX D term (microfacet.h:38-55) per float32 ulp of the half-vector's z
+    at the sample: D = a^2 / (pi t^2) with t = 1 + (a^2 - 1) z^2, which cancels near z = 1 for small
+    roughness. The half-vector is rounded inside the sampler (VNDF normalisation), below what input
+    perturbations reach, so pdf and throughput inherit this sensitivity directly."""
+    a2 = np.clip(np.float64(mat["roughness"]) ** 2, 1e-3, 1.0) ** 2
+    ior = np.clip(np.float64(mat["ior"]), 1.0, 4.0)
+    wo = wo.astype(np.float64)
+    wi = wi.astype(np.float64)
+    refl = wo[:, 2] * wi[:, 2] >= 0
+    eta = np.where(wo[:, 2] >= 0, ior, 1.0 / ior)
+    wh = np.where(refl[:, None], wo + wi, wo + wi * eta[:, None])
+    wh /= np.maximum(np.linalg.norm(wh, axis=1, keepdims=True), 1e-30)
+    z = np.abs(wh[:, 2])
+    t = 1.0 + (a2 - 1.0) * z * z
+    return 4.0 * z * (1.0 - a2) / np.maximum(t, 1e-30) * 2.0**-24
+
+
 def test_bsdf_eval_sample(device, ora):
     rng = np.random.default_rng(11)
     n = 20000
-    for mat in _materials():
+    for mi, mat in enumerate(_materials()):
         wo = rng.normal(size=(n, 3)).astype(np.float32)
         wo /= np.linalg.norm(wo, axis=1, keepdims=True)
         wo[: n // 2, 2] = np.abs(wo[: n // 2, 2])
         wi = rng.normal(size=(n, 3)).astype(np.float32)
         wi /= np.linalg.norm(wi, axis=1, keepdims=True)
         gf, gp = device.bsdf_eval(mat, wo, wi)
-        of, op = ora.bsdf_eval(mat, wo, wi)
-        tol = 2e-4 * np.maximum(1.0, np.abs(of))
-        assert (np.abs(gf - of) <= tol).mean() > 0.999
-        assert (np.abs(gp - op) <= 2e-4 * np.maximum(1.0, np.abs(op))).mean() > 0.999
+        (of, op), (sf, sp) = _spread(lambda a, b: ora.bsdf_eval(mat, a, b), [wo, wi], lambda a, b: ora.bsdf_eval(mat, a, b, 'fma'))
+        okf, okp = _within(gf, of, sf), _within(gp, op, sp)
+        _log("material", mi, "eval agree", okf.mean(), "pdf agree", okp.mean())
+        assert okf.mean() > 0.999 and okp.mean() > 0.999
         u = rng.uniform(0, 1, (n, 3)).astype(np.float32)
         gwi, gb, gpd = device.bsdf_sample(mat, wo, u)
-        owi, ob, opd = ora.bsdf_sample(mat, wo, u)
-        ok = (np.abs(gwi - owi).max(axis=1) < 1e-4) & (np.abs(gpd - opd) <= 1e-3 * np.maximum(1, np.abs(opd)))
-        assert ok.mean() > 0.999, ok.mean()
+        (owi, ob, opd), (swi, sb, spd) = _spread(lambda a, b: ora.bsdf_sample(mat, a, b), [wo, u], lambda a, b: ora.bsdf_sample(mat, a, b, 'fma'))
+        cond = 1e-4 + 4.0 * _ggx_condition(mat, wo, owi)
+        okw, okb, okp = _within(gwi, owi, swi), _within(gb, ob, sb, cond), _within(gpd, opd, spd, cond)
+        ok = okw & okb & okp
+        _log("material", mi, "sample agree", ok.mean(), "wi", okw.mean(), "beta", okb.mean(), "pdf", okp.mean())
+        if ok.mean() <= 0.999:
+            for i in np.nonzero(~ok)[0][:8]:
+                _log("  wo", wo[i], "u", u[i], "gpu wi", gwi[i], gb[i], gpd[i], "ora wi", owi[i], ob[i], opd[i],
+                     "spread", swi[i], sb[i], spd[i])
+        assert ok.mean() > 0.999
 
 
 def test_trace_samples_match(device, ks, ora, cornell):
