@@ -66,6 +66,19 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
+HOST_TOOL_SOURCES = ["config.cpp", "small_pt_task.cpp", "ks_render.cpp"]
+TOOL = LIBDIR / "ks_render"
+
+
+def build_tools() -> Path:
+    """The ks_render task runner (C++ host code over the C ABI), linked against the library."""
+    cmd = [CXX, *HOST_FLAGS, "-I", str(ROOT / "include")]
+    cmd += [str(PKG / "host" / s) for s in HOST_TOOL_SOURCES]
+    cmd += ["-L", str(LIBDIR), "-lkeepsake_amd", "-Wl,-rpath,$ORIGIN", "-o", str(TOOL)]
+    _run(cmd)
+    return TOOL
+
+
 def build_oracle() -> Path:
     _run(["make", "-C", str(ROOT / "oracle")])
     return ROOT / "oracle" / "_build" / "libks_oracle.so"
@@ -73,6 +86,7 @@ def build_oracle() -> Path:
 
 def main(argv: list[str]) -> int:
     build_library()
+    build_tools()
     if "--no-oracle" not in argv:
         build_oracle()
     return 0

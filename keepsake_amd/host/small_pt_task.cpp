@@ -1,0 +1,213 @@
+// The small_pt task (small_pt.cpp:180-303) on the gfx950 library, with the same TOML keys:
+// sky, light[], pixel_filter, tone_mapper, backdrop, bounces, clamp_direct, clamp_indirect,
+// render_width, render_height, crop_*, spp, scale_ray_diff, rng_seed, spp_prog_interval, camera.
+// Scenes come from this build's procedural asset type (keys `procedural`, `procedural_seed`), since the
+// mesh-asset loaders (mesh_asset.cpp) are not part of this build yet; `object` / `compound_object`
+// therefore fail with a clear message. Additional keys of this build: mis_mode ("reference" or
+// "consistent"), queue_size.
+#include "small_pt_task.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+
+namespace ks {
+
+static SceneData scene_from_host(const ks_host_scene *hs, ks_camera_desc *cam) {
+    ks_scene_desc d{};
+    check(ks_host_scene_get(hs, &d, cam), "ks_host_scene_get");
+    SceneData s;
+    s.vertices.assign(d.vertices, d.vertices + 3 * (size_t)d.n_vertices);
+    if (d.normals) s.normals.assign(d.normals, d.normals + 3 * (size_t)d.n_vertices);
+    s.indices.assign(d.indices, d.indices + 3 * (size_t)d.n_tris);
+    s.tri_geom.assign(d.tri_geom, d.tri_geom + d.n_tris);
+    s.geom_material.assign(d.geom_material, d.geom_material + d.n_geoms);
+    s.materials.assign(d.materials, d.materials + d.n_materials);
+    if (d.lights) s.lights.assign(d.lights, d.lights + d.n_lights);
+    return s;
+}
+
+// create_camera (camera.cpp:172-205), perspective only
+static Camera load_camera(const ConfigArgs &args) {
+    std::string type = args.load_string("type");
+    if (type != "perspective") throw ConfigError("camera type '" + type + "' is not supported (perspective only)");
+    float aspect = args.load_float("aspect");
+    float vfov;
+    if (args.contains("vfov")) {
+        vfov = args.load_float("vfov") / 180.0f * 3.14159265359f;
+    } else {
+        float hfov = args.load_float("hfov") / 180.0f * 3.14159265359f;
+        vfov = 2.0f * std::atan((1.0f / aspect) * std::tan(0.5f * hfov));
+    }
+    if (args.contains("to_world")) return Camera(args.load_transform("to_world"), vfov, aspect);
+    return Camera(args.load_vec3("pos"), args.load_vec3("target"), args.load_vec3("up", true), vfov, aspect);
+}
+
+SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
+    SmallPTSetup st;
+    if (args.contains("object") || args.contains("compound_object"))
+        throw ConfigError("mesh assets (object / compound_object) are not supported by this build; use "
+                          "procedural = \"cornell\" | \"spheres\" | \"room\"");
+    if (!args.contains("procedural")) throw ConfigError("the task needs a scene: procedural = \"...\"");
+    st.scene_name = args.load_string("procedural");
+    int seed = args.load_integer("procedural_seed", st.scene_name.rfind("spheres", 0) == 0 ? 1 : 0);
+    ks_host_scene *hs = nullptr;
+    check(ks_host_scene_create(st.scene_name.c_str(), (uint64_t)seed, &hs), "ks_host_scene_create");
+    ks_camera_desc scene_cam{};
+    st.scene = scene_from_host(hs, &scene_cam);
+    ks_host_scene_free(hs);
+
+    if (args.contains("sky") || args.contains("light")) {
+        st.scene.lights.clear();
+        if (args.contains("sky")) { // create_sky_light (light.cpp:247-262), ambient only
+            ConfigArgs sky = args["sky"];
+            if (!sky.contains("ambient")) throw ConfigError("sky maps are not supported; use sky.ambient");
+            ks_light_desc l{};
+            l.type = KS_LIGHT_SKY_AMBIENT;
+            auto a = sky.load_vec3("ambient");
+            for (int k = 0; k < 3; ++k) l.color[k] = a[k];
+            st.scene.lights.push_back(l);
+        }
+        if (args.contains("light")) { // create_light (light.cpp:237-245)
+            ConfigArgs la = args["light"];
+            for (size_t i = 0; i < la.array_size(); ++i) {
+                ConfigArgs li = la[(int)i];
+                std::string t = li.load_string("type");
+                ks_light_desc l{};
+                if (t == "directional") {
+                    l.type = KS_LIGHT_DIRECTIONAL;
+                    auto L = li.load_vec3("L");
+                    auto d = li.load_vec3("dir", true);
+                    for (int k = 0; k < 3; ++k) l.color[k] = L[k], l.vec[k] = d[k];
+                } else if (t == "point") {
+                    l.type = KS_LIGHT_POINT;
+                    std::array<float, 3> I;
+                    if (li.contains("I")) {
+                        I = li.load_vec3("I");
+                    } else {
+                        auto c = li.load_vec3("color");
+                        float pw = li.load_float("power");
+                        for (int k = 0; k < 3; ++k) I[k] = c[k] * pw / (4.0f * 3.14159265359f);
+                    }
+                    auto pos = li.load_vec3("pos");
+                    for (int k = 0; k < 3; ++k) l.color[k] = I[k], l.vec[k] = pos[k];
+                } else if (t == "sky") {
+                    if (!li.contains("ambient")) throw ConfigError("sky maps are not supported; use ambient");
+                    l.type = KS_LIGHT_SKY_AMBIENT;
+                    auto a = li.load_vec3("ambient");
+                    for (int k = 0; k < 3; ++k) l.color[k] = a[k];
+                } else {
+                    throw ConfigError("light type '" + t + "' is not supported");
+                }
+                st.scene.lights.push_back(l);
+            }
+        }
+    }
+    if (args.contains("pixel_filter")) {
+        ConfigArgs pf = args["pixel_filter"];
+        if (pf.load_string("type") != "box" || pf.load_float("width", 1.0f) != 1.0f)
+            throw ConfigError("only the unit-width box pixel filter is supported");
+    }
+    if (args.contains("tone_mapper")) std::printf("[small_pt] tone_mapper ignored: output is linear PFM\n");
+    if (args.contains("camera_animation")) throw ConfigError("camera_animation is not supported");
+
+    SmallPTInput &in = st.input;
+    if (args.contains("backdrop")) {
+        in.backdrop = args.load_vec3("backdrop");
+        in.include_background = false;
+    } else {
+        in.include_background = true;
+    }
+    in.bounces = args.load_integer("bounces");
+    in.clamp_direct = args.load_float("clamp_direct", 0.0f);
+    in.clamp_indirect = args.load_float("clamp_indirect", 10.0f);
+    in.render_width = args.load_integer("render_width");
+    in.render_height = args.load_integer("render_height");
+    in.crop_start_x = args.load_integer("crop_start_x", 0);
+    in.crop_start_y = args.load_integer("crop_start_y", 0);
+    in.crop_width = args.load_integer("crop_width", 0);
+    in.crop_height = args.load_integer("crop_height", 0);
+    in.spp = args.load_integer("spp");
+    in.scale_ray_diff = args.load_bool("scale_ray_diff", true);
+    in.rng_seed = args.load_integer("rng_seed", 0);
+    in.spp_prog_interval = args.load_integer("spp_prog_interval", 32);
+    std::string mis = args.load_string("mis_mode", "reference");
+    if (mis == "reference") in.mis_mode = KS_MIS_REFERENCE;
+    else if (mis == "consistent") in.mis_mode = KS_MIS_CONSISTENT;
+    else throw ConfigError("mis_mode must be reference or consistent");
+    in.queue_size = args.load_integer("queue_size", 0);
+
+    if (args.contains("camera")) {
+        st.camera = load_camera(args["camera"]);
+    } else {
+        st.camera.desc = scene_cam;
+    }
+    return st;
+}
+
+std::string describe(const SmallPTSetup &st) {
+    const SmallPTInput &in = st.input;
+    const ks_camera_desc &c = st.camera.desc;
+    char buf[2048];
+    std::snprintf(buf, sizeof(buf),
+                  "{\"scene\": \"%s\", \"n_tris\": %zu, \"n_lights\": %zu, \"bounces\": %d, \"render_width\": %d, "
+                  "\"render_height\": %d, \"crop\": [%d, %d, %d, %d], \"spp\": %d, \"rng_seed\": %d, "
+                  "\"include_background\": %s, \"backdrop\": [%g, %g, %g], \"clamp_direct\": %g, "
+                  "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
+                  "\"camera\": {\"use_matrix\": %d, \"pos\": [%g, %g, %g], \"target\": [%g, %g, %g], "
+                  "\"up\": [%g, %g, %g], \"vfov_deg\": %.6g, \"aspect\": %.6g, \"to_world\": [%g, %g, %g, %g, %g, %g, "
+                  "%g, %g, %g, %g, %g, %g, %g, %g, %g, %g]}}",
+                  st.scene_name.c_str(), st.scene.indices.size() / 3, st.scene.lights.size(), in.bounces,
+                  in.render_width, in.render_height, in.crop_start_x, in.crop_start_y, in.crop_width, in.crop_height,
+                  in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
+                  in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,
+                  in.queue_size, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
+                  c.up[0], c.up[1], c.up[2], c.vfov_deg, c.aspect, c.to_world[0], c.to_world[1], c.to_world[2],
+                  c.to_world[3], c.to_world[4], c.to_world[5], c.to_world[6], c.to_world[7], c.to_world[8],
+                  c.to_world[9], c.to_world[10], c.to_world[11], c.to_world[12], c.to_world[13], c.to_world[14],
+                  c.to_world[15]);
+    return buf;
+}
+
+static int g_device_ordinal = 0;
+void set_task_device(int ordinal) { g_device_ordinal = ordinal; }
+
+void small_pt(const ConfigArgs &args, const fs::path &task_dir, int task_id) {
+    SmallPTSetup st = resolve_small_pt(args);
+    Device dev(g_device_ordinal);
+    auto b0 = std::chrono::steady_clock::now();
+    Scene scene(dev, st.scene);
+    double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - b0).count();
+    RenderTarget2 rt(dev, (uint32_t)st.input.render_width, (uint32_t)st.input.render_height, st.input.backdrop);
+    SmallPTInput in = st.input;
+    in.scene = &scene;
+    in.camera = &st.camera;
+    in.target = &rt;
+    std::printf("Frame [1/1] | Start (task %d, scene %s, %zu triangles, build %.2f s)\n", task_id,
+                st.scene_name.c_str(), st.scene.indices.size() / 3, build_s);
+    auto render_start = std::chrono::steady_clock::now();
+    auto interval_start = render_start;
+    in.prog_interval_callback = [&](const RenderTarget2 &target, int spp_finished) { // small_pt.cpp:278-294
+        auto now = std::chrono::steady_clock::now();
+        std::printf("Frame [1/1] | [%d/%d] spp interval took: %.3f sec\n", spp_finished, in.spp,
+                    std::chrono::duration<double>(now - interval_start).count());
+        std::vector<float> rgb = target.composite();
+        write_pfm(task_dir / ("small_pt_spp" + std::to_string(spp_finished) + ".pfm"), rgb.data(), target.width,
+                  target.height);
+        interval_start = std::chrono::steady_clock::now();
+    };
+    SmallPT pt;
+    ks_render_stats s = pt.run(in);
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - render_start).count();
+    std::printf("Frame [1/1] | Rendering time: %.3f sec (%.1f Mpaths/s, %llu + %llu rays)\n", secs,
+                (double)s.paths / s.wall_ms / 1e3, (unsigned long long)s.rays_closest,
+                (unsigned long long)s.rays_shadow);
+    std::ofstream js(task_dir / "render_stats.json");
+    js << "{\"render_seconds\": " << secs << ", \"render_ms_library\": " << s.wall_ms << ", \"paths\": " << s.paths
+       << ", \"rays_closest\": " << s.rays_closest << ", \"rays_shadow\": " << s.rays_shadow
+       << ", \"iterations\": " << s.iterations << ", \"nonfinite\": " << s.nonfinite
+       << ", \"scene_build_seconds\": " << build_s << ", \"setup\": " << describe(st) << "}\n";
+}
+
+} // namespace ks

@@ -1,0 +1,24 @@
+// The small_pt task's resolved inputs: what the TOML keys of small_pt.cpp:180-255 turn into.
+#pragma once
+#include "../../include/ks/small_pt.h"
+#include "config.h"
+
+#include <string>
+
+namespace ks {
+
+struct SmallPTSetup {
+    SceneData scene;
+    Camera camera;
+    SmallPTInput input;
+    std::string scene_name;
+};
+
+// Parses one [[task]] table of type small_pt without touching the GPU.
+SmallPTSetup resolve_small_pt(const ConfigArgs &args);
+// The resolved inputs as one JSON object.
+std::string describe(const SmallPTSetup &st);
+// Device ordinal the small_pt task opens (ks_render -d).
+void set_task_device(int ordinal);
+
+} // namespace ks

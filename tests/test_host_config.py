@@ -1,0 +1,160 @@
+"""The C++ host side (ks_render, keepsake_amd/host): TOML parsing and the small_pt task's key handling,
+checked through `ks_render --print-config` (no GPU needed). Defaults follow small_pt.cpp:227-241."""
+
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+TOOL = ROOT / "keepsake_amd" / "_lib" / "ks_render"
+
+BASE = """
+output_dir = "{out}"
+[[task]]
+type = "small_pt"
+procedural = "cornell"
+bounces = 8
+render_width = 64
+render_height = 32
+spp = 4
+"""
+
+
+def resolve(tmp_path, text):
+    cfg = tmp_path / "c.toml"
+    cfg.write_text(text.replace("{out}", str(tmp_path / "out")))
+    r = subprocess.run([str(TOOL), "-c", str(cfg), "--print-config"], capture_output=True, text=True)
+    return r
+
+
+def tasks(tmp_path, text):
+    r = resolve(tmp_path, text)
+    assert r.returncode == 0, r.stderr
+    return [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+
+
+def test_defaults_match_small_pt(tmp_path):
+    (t,) = tasks(tmp_path, BASE)
+    assert t["scene"] == "cornell" and t["n_tris"] == 36
+    assert t["clamp_direct"] == 0 and t["clamp_indirect"] == 10  # small_pt.cpp:230-231
+    assert t["spp_prog_interval"] == 32 and t["rng_seed"] == 0
+    assert t["include_background"] is True and t["crop"] == [0, 0, 0, 0]
+    assert t["mis_mode"] == 0
+    # without a [task.camera] table the procedural scene's own camera is used
+    assert t["camera"]["vfov_deg"] == pytest.approx(40.0)
+
+
+def test_backdrop_disables_background(tmp_path):
+    (t,) = tasks(tmp_path, BASE + "backdrop = [0.1, 0.2, 0.3]\n")
+    assert t["include_background"] is False
+    assert t["backdrop"] == pytest.approx([0.1, 0.2, 0.3])
+
+
+def test_override_merges_tables(tmp_path):
+    text = BASE + """
+[[task]]
+override = 0
+spp = 16
+crop_start_x = 8
+crop_width = 16
+crop_height = 8
+"""
+    a, b = tasks(tmp_path, text)
+    assert a["spp"] == 4 and b["spp"] == 16
+    assert b["bounces"] == 8 and b["crop"] == [8, 0, 16, 8]
+
+
+def test_camera_keys(tmp_path):
+    text = BASE + """
+[task.camera]
+type = "perspective"
+pos = [1, 2, 3]
+target = [0, 0, 0]
+up = [0, 2, 0]   # normalised on load
+hfov = 60.0
+aspect = 2.0
+"""
+    (t,) = tasks(tmp_path, text)
+    c = t["camera"]
+    assert c["pos"] == [1, 2, 3] and c["up"] == [0, 1, 0]
+    # camera.cpp:196-197: vfov = 2 atan(tan(hfov/2) / aspect)
+    assert c["vfov_deg"] == pytest.approx(np.rad2deg(2 * np.arctan(np.tan(np.deg2rad(30.0)) / 2.0)), rel=1e-5)
+
+
+def test_to_world_transform(tmp_path):
+    text = BASE + """
+[task.camera]
+type = "perspective"
+vfov = 45
+aspect = 1.0
+to_world = { rotation = { euler = [0, 90, 0] }, translation = [1, 2, 3] }
+"""
+    (t,) = tasks(tmp_path, text)
+    m = np.array(t["camera"]["to_world"]).reshape(4, 4)
+    ry = np.array([[0, 0, 1], [0, 1, 0], [-1, 0, 0]])  # rotation of +90 degrees about y
+    assert np.allclose(m[:3, :3], ry, atol=1e-6)
+    assert np.allclose(m[:3, 3], [1, 2, 3]) and m[3, 3] == 1
+    text2 = BASE + """
+[task.camera]
+type = "perspective"
+vfov = 45
+aspect = 1.0
+to_world.matrix_cm = [1,0,0,0, 0,1,0,0, 0,0,1,0, 5,6,7,1]
+"""
+    (t2,) = tasks(tmp_path, text2)
+    m2 = np.array(t2["camera"]["to_world"]).reshape(4, 4)
+    assert np.allclose(m2[:3, 3], [5, 6, 7])
+
+
+def test_lights(tmp_path):
+    text = BASE + """
+sky = { ambient = [0.5, 0.5, 0.5] }
+light = [
+  { type = "directional", L = [1, 1, 1], dir = [0, -2, 0] },  # trailing comma and comments
+  { type = "point", color = [1, 1, 1], power = 12.566370614359172, pos = [0, 1, 0] },
+]
+"""
+    (t,) = tasks(tmp_path, text)
+    assert t["n_lights"] == 3
+
+
+def test_toml_syntax(tmp_path):
+    text = """
+# comment
+output_dir = '{out}'
+[[task]]
+"type" = "small_pt"   # quoted key
+procedural = "spheres_small"
+bounces = 1_2
+render_width = 1_920
+render_height = 1080
+spp = 64
+clamp_indirect = 2.5e1
+scale_ray_diff = false
+"""
+    (t,) = tasks(tmp_path, text)
+    assert t["bounces"] == 12 and t["render_width"] == 1920 and t["clamp_indirect"] == 25.0
+
+
+@pytest.mark.parametrize(
+    "extra,msg",
+    [
+        ('object = "teapot"\n', "mesh assets"),
+        ('camera_animation = "anim"\n', "camera_animation"),
+        ('pixel_filter = { type = "gaussian" }\n', "box pixel filter"),
+        ('mis_mode = "fancy"\n', "mis_mode"),
+        ('spp = "many"\n', "duplicate key"),
+    ],
+)
+def test_errors_are_reported(tmp_path, extra, msg):
+    r = resolve(tmp_path, BASE + extra)
+    assert r.returncode != 0
+    assert msg in r.stderr
+
+
+def test_missing_required_key(tmp_path):
+    r = resolve(tmp_path, BASE.replace("spp = 4\n", ""))
+    assert r.returncode != 0 and "spp" in r.stderr
