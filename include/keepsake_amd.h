@@ -211,6 +211,45 @@ int ks_bsdf_sample(ks_device *dev, const ks_material_desc *m, const float *wo, c
 int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
                      const int32_t *px, const int32_t *py, const int32_t *sample, float *out);
 
+/* ---- path guide: hash-grid encoding + 64-64-64-48 BF16 MLP (MFMA, FP32 accumulation) -> 8-lobe vMF
+ *      mixture, trained online with Adam. The reference has no network: the lobe math is VMF/WVMF/
+ *      sample_vmm (vmf.h:10-127) and the update rule is adam_optimizer_step (gpu/shaders/adam.slang:25-51).
+ *      Zeroed config fields take the defaults given in the comments. ---- */
+typedef struct ks_guide ks_guide;
+typedef struct ks_guide_config {
+    int32_t log2_table_size;    /* hash-table entries per level, log2 (19) */
+    float base_resolution;      /* coarsest grid resolution (16) */
+    float max_resolution;       /* finest grid resolution (2048) */
+    float learning_rate;        /* Adam lr (1e-2) */
+    float beta1, beta2, eps;    /* Adam moments (0.9, 0.999, 1e-8 as adam.slang) */
+    float guide_fraction;       /* one-sample MIS: probability of sampling the guide (0.5) */
+    int32_t train_batch;        /* samples per Adam step (2^18) */
+    int32_t max_steps_per_pass; /* Adam steps after each 1-spp pass of a guided render (4) */
+    uint64_t seed;
+} ks_guide_config;
+
+int ks_guide_create(ks_device *dev, const ks_guide_config *cfg, const float bound_lo[3], const float bound_hi[3],
+                    ks_guide **out);
+int ks_guide_free(ks_guide *guide);
+/* grid parameters (16 levels x 2^log2 entries x 2 features) and MLP weights (11264) */
+int ks_guide_param_count(const ks_guide *guide, int64_t *n_grid, int64_t *n_mlp);
+/* grid resolutions of the 16 levels and the position normalisation u = (x - lo) * inv_ext */
+int ks_guide_grid_info(const ks_guide *guide, int32_t res[16], float lo[3], float inv_ext[3]);
+/* FP32 master parameters: [grid][W1 64x64][W2 64x64][W3 64x48], weights [in][out] */
+int ks_guide_get_params(const ks_guide *guide, float *params);
+int ks_guide_set_params(ks_guide *guide, const float *params);
+/* queries n x 9 (position, normal, wo) -> mixtures n x 40: w[8], then (mu xyz, kappa) x 8 */
+int ks_guide_forward(ks_guide *guide, int64_t n, const float *queries, float *mixtures);
+/* loss = sum_i -weight_i log q(wi_i | query_i) and its gradient w.r.t. every parameter */
+int ks_guide_gradient(ks_guide *guide, int64_t n, const float *queries, const float *wi, const float *weight,
+                      double *loss, float *grad);
+/* one Adam step on that loss */
+int ks_guide_train_step(ks_guide *guide, int64_t n, const float *queries, const float *wi, const float *weight,
+                        double *loss);
+/* mixture pdf at directions / samples from mixtures (sample_vmm, vmf.h:121-127); u n x 2 */
+int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *dirs, float *pdf);
+int ks_vmm_sample(ks_device *dev, int64_t n, const float *mixtures, const float *u, float *dirs, float *pdf);
+
 /* ---- procedural scenes of the benchmark configurations (no reference counterpart: the reference
  *      loads meshes from files, mesh_asset.cpp). name: "cornell", "spheres", "room". ---- */
 int ks_host_scene_create(const char *name, uint64_t seed, ks_host_scene **out);

@@ -11,13 +11,17 @@ from .api import (  # noqa: F401
     KS_MIS_REFERENCE,
     Device,
     Film,
+    Guide,
     HostScene,
     KsError,
     Scene,
     camera_desc,
     device_count,
+    guide_config,
     load_library,
     procedural_scene,
     small_pt_params,
     small_pt_render,
+    vmm_pdf,
+    vmm_sample,
 )

@@ -161,6 +161,24 @@ class SceneInfo(C.Structure):
     ]
 
 
+class GuideConfig(C.Structure):
+    """ks_guide_config; zero fields take the library defaults."""
+
+    _fields_ = [
+        ("log2_table_size", C.c_int32),
+        ("base_resolution", C.c_float),
+        ("max_resolution", C.c_float),
+        ("learning_rate", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("guide_fraction", C.c_float),
+        ("train_batch", C.c_int32),
+        ("max_steps_per_pass", C.c_int32),
+        ("seed", C.c_uint64),
+    ]
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p)
 
 _P = C.c_void_p
@@ -199,6 +217,17 @@ _SIGS = {
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), C.c_int64, _P, _P, _P, _P],
         C.c_int,
     ),
+    "ks_guide_create": ([_P, C.POINTER(GuideConfig), _P, _P, C.POINTER(_P)], C.c_int),
+    "ks_guide_free": ([_P], C.c_int),
+    "ks_guide_param_count": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "ks_guide_grid_info": ([_P, _P, _P, _P], C.c_int),
+    "ks_guide_get_params": ([_P, _P], C.c_int),
+    "ks_guide_set_params": ([_P, _P], C.c_int),
+    "ks_guide_forward": ([_P, C.c_int64, _P, _P], C.c_int),
+    "ks_guide_gradient": ([_P, C.c_int64, _P, _P, _P, C.POINTER(C.c_double), _P], C.c_int),
+    "ks_guide_train_step": ([_P, C.c_int64, _P, _P, _P, C.POINTER(C.c_double)], C.c_int),
+    "ks_vmm_pdf": ([_P, C.c_int64, _P, _P, _P], C.c_int),
+    "ks_vmm_sample": ([_P, C.c_int64, _P, _P, _P, _P], C.c_int),
     "ks_host_scene_create": ([C.c_char_p, C.c_uint64, C.POINTER(_P)], C.c_int),
     "ks_host_scene_get": ([_P, C.POINTER(SceneDesc), C.POINTER(CameraDesc)], C.c_int),
     "ks_host_scene_free": ([_P], C.c_int),
@@ -582,3 +611,96 @@ def small_pt_render(scene: Scene, cam: dict, params: SmallPTParams, film: Film, 
         cb = PROGRESS_FN()
     _check(scene.lib.ks_small_pt_render(scene.h, C.byref(c), C.byref(params), film.h, cb, None, C.byref(st)))
     return st.as_dict()
+
+
+def guide_config(**kw) -> GuideConfig:
+    c = GuideConfig()
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Guide:
+    """The learned path guide (hash grid + MLP -> 8-lobe vMF mixture) on one device."""
+
+    def __init__(self, device: Device, bound_lo, bound_hi, config: GuideConfig | None = None):
+        self.device = device
+        self.lib = device.lib
+        lo = np.ascontiguousarray(bound_lo, np.float32)
+        hi = np.ascontiguousarray(bound_hi, np.float32)
+        h = C.c_void_p()
+        cfg = config if config is not None else GuideConfig()
+        _check(self.lib.ks_guide_create(device.h, C.byref(cfg), _ptr(lo), _ptr(hi), C.byref(h)))
+        self.h = h
+        ng, nm = C.c_int64(), C.c_int64()
+        _check(self.lib.ks_guide_param_count(self.h, C.byref(ng), C.byref(nm)))
+        self.n_grid, self.n_mlp = ng.value, nm.value
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.ks_guide_free(self.h)
+            self.h = None
+
+    def grid_info(self):
+        res = np.zeros(16, np.int32)
+        lo = np.zeros(3, np.float32)
+        inv = np.zeros(3, np.float32)
+        _check(self.lib.ks_guide_grid_info(self.h, _ptr(res), _ptr(lo), _ptr(inv)))
+        return res, lo, inv
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.n_grid + self.n_mlp, np.float32)
+        _check(self.lib.ks_guide_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != self.n_grid + self.n_mlp:
+            raise ValueError("parameter vector has the wrong size")
+        _check(self.lib.ks_guide_set_params(self.h, _ptr(p)))
+
+    def forward(self, queries) -> np.ndarray:
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        out = np.zeros((q.shape[0], 40), np.float32)
+        _check(self.lib.ks_guide_forward(self.h, q.shape[0], _ptr(q), _ptr(out)))
+        return out
+
+    def _samples(self, queries, wi, weight):
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        w = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+        c = np.ascontiguousarray(weight, np.float32).reshape(-1)
+        if not (len(q) == len(w) == len(c)):
+            raise ValueError("queries, wi and weight differ in length")
+        return q, w, c
+
+    def gradient(self, queries, wi, weight):
+        q, w, c = self._samples(queries, wi, weight)
+        g = np.zeros(self.n_grid + self.n_mlp, np.float32)
+        loss = C.c_double()
+        _check(self.lib.ks_guide_gradient(self.h, len(q), _ptr(q), _ptr(w), _ptr(c), C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train_step(self, queries, wi, weight) -> float:
+        q, w, c = self._samples(queries, wi, weight)
+        loss = C.c_double()
+        _check(self.lib.ks_guide_train_step(self.h, len(q), _ptr(q), _ptr(w), _ptr(c), C.byref(loss)))
+        return loss.value
+
+
+def vmm_pdf(device: Device, mixtures, dirs) -> np.ndarray:
+    m = np.ascontiguousarray(mixtures, np.float32).reshape(-1, 40)
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    p = np.zeros(len(m), np.float32)
+    _check(device.lib.ks_vmm_pdf(device.h, len(m), _ptr(m), _ptr(d), _ptr(p)))
+    return p
+
+
+def vmm_sample(device: Device, mixtures, u):
+    m = np.ascontiguousarray(mixtures, np.float32).reshape(-1, 40)
+    uu = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    d = np.zeros((len(m), 3), np.float32)
+    p = np.zeros(len(m), np.float32)
+    _check(device.lib.ks_vmm_sample(device.h, len(m), _ptr(m), _ptr(uu), _ptr(d), _ptr(p)))
+    return d, p

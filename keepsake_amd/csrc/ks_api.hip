@@ -14,7 +14,9 @@
 #include <hip/hip_runtime.h>
 
 #include "ks_device.h"
+#include "ks_guide_host.h"
 #include "ks_internal.h"
+#include "ks_render_types.h"
 #include "ks_traverse.h"
 
 #include <algorithm>
@@ -26,6 +28,7 @@
 #include <vector>
 
 using namespace ksd;
+using namespace ksr;
 
 namespace ksa {
 static thread_local std::string g_error;
@@ -77,81 +80,6 @@ struct DevBuf {
     hipError_t ensure(size_t n) { return n <= bytes ? hipSuccess : alloc(n); }
     template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
 };
-
-struct Ctrl {
-    uint32_t n_active; // paths in the current buffer, [0, n_active)
-    uint32_t n_next;   // survivors appended by shade into the next buffer
-    uint32_t n_shadow; // shadow rays appended by shade
-    uint32_t iteration;
-    uint64_t work_next, work_end;
-};
-
-enum StatSlot {
-    kRaysClosest = 0,
-    kRaysShadow,
-    kNodesClosest,
-    kTrisClosest,
-    kNodesShadow,
-    kTrisShadow,
-    kNonFinite,
-    kSpills,
-    kStatCount
-};
-
-struct DevScene {
-    const float4 *nodes;
-    const float4 *tris;
-    const float4 *verts;
-    const float4 *normals; // nullptr when the scene has no smooth normals
-    const uint4 *tri_idx;  // i0, i1, i2, material
-    const int32_t *tri_light;
-    const Material *mats;
-    const Light *lights;
-    const float *cdf;
-    int32_t n_lights;
-    int32_t n_sky;
-    int32_t sky[4];
-};
-
-struct RenderConsts {
-    int32_t W, H, cx, cy, cw, ch;
-    int32_t spp, seed, log2spp, ndigits, bounces, include_background, mis_mode;
-    float clamp_direct, clamp_indirect;
-    int32_t s0, nb;            // sample range of the batch: [s0, s0 + nb)
-    const int4 *work_list;     // optional explicit (x, y, sample) per work item
-    float p2w[16]; // proj_to_world with the eye folded into rows 0..2 (see camera_ray)
-    float cam[3];
-};
-
-struct PathBufs {
-    uint32_t *work;
-    uint32_t *state; // Sobol dimension (bits 0..15) | bounce (bits 16..23)
-    float4 *beta;    // beta.rgb, pdf_wi_bsdf
-    float4 *L;       // L.rgb
-    float4 *ro;      // origin, tmin
-    float4 *rd;      // direction, tmax
-};
-
-// ---------------------------------------------------------------------------------------------------
-// Wave-level helpers (wave64)
-// ---------------------------------------------------------------------------------------------------
-KS_DEV uint32_t wave_append(uint32_t *counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    uint32_t lane = __lane_id();
-    uint32_t prefix = __popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (mask != 0) {
-        int leader = __ffsll((unsigned long long)mask) - 1;
-        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-        base = __shfl(base, leader);
-    }
-    return base + prefix;
-}
-
-KS_DEV void wave_add_stat(unsigned long long *stats, int slot, uint32_t v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (__lane_id() == 0 && v) atomicAdd(&stats[slot], (unsigned long long)v);
-}
 
 KS_DEV void pixel_of(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &y, uint32_t &s) {
     if (rc.work_list) {
@@ -357,40 +285,6 @@ KS_DEV float sky_pdf(f3 wi) { // SkyLight::pdf (light.cpp:113-134) for a 1x1 map
     float sin_theta = sinf(theta);
     if (sin_theta == 0.0f) return 0.0f;
     return 1.0f / (2.0f * kPi * kPi * sin_theta);
-}
-
-KS_DEV SurfHit make_hit(const DevScene &sc, uint4 tri, float u, float v, f3 rd) { // geometry.cpp:68-163
-    f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
-    f3 wo = -normalized(rd);
-    f3 ng = normalized(cross(v1 - v0, v2 - v0));
-    float w = 1.0f - u - v;
-    SurfHit it;
-    it.p = v0 * w + v1 * u + v2 * v;
-    f3 dpdu = v1 - v0;
-    f3 b = normalized(cross(ng, dpdu));
-    f3 t = normalized(cross(b, ng));
-    if (dot(wo, ng) < 0.0f) {
-        ng = -ng;
-        dpdu = -dpdu;
-        t = -t;
-    }
-    it.t = t;
-    it.b = b;
-    it.n = ng;
-    it.st = t;
-    it.sb = b;
-    it.sn = ng;
-    if (sc.normals) {
-        f3 vn = normalized(ld3(sc.normals[tri.x]) * w + ld3(sc.normals[tri.y]) * u + ld3(sc.normals[tri.z]) * v);
-        if (all_finite(vn)) {
-            if (dot(vn, ng) < 0.0f) vn = -vn;
-            f3 vb = normalized(cross(vn, dpdu));
-            it.st = normalized(cross(vb, vn));
-            it.sb = vb;
-            it.sn = vn;
-        }
-    }
-    return it;
 }
 
 // Light::sample for the flat light list; returns Le / pdf (light.h:27-29).
@@ -1135,6 +1029,12 @@ template <typename T> struct HostToDev {
 };
 
 } // namespace
+
+namespace ksg {
+hipStream_t device_stream(ks_device *dev) { return dev->stream; }
+int device_ordinal(ks_device *dev) { return dev->ordinal; }
+int device_cu_count(ks_device *dev) { return dev->n_cu; }
+} // namespace ksg
 
 // ---------------------------------------------------------------------------------------------------
 // C ABI

@@ -1,0 +1,821 @@
+// Guide network kernels (gfx950) and their C ABI: initialisation, forward (inference), training
+// forward + backward (MFMA for all three layers and their weight gradients, atomics for the hash-grid
+// scatter), the per-wave gradient reduction and the Adam step (update rule of
+// gpu/shaders/adam.slang:25-51).
+#include <hip/hip_runtime.h>
+
+#include "ks_guide.h"
+#include "ks_guide_host.h"
+#include "ks_internal.h"
+#include "ks_render_types.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace ksd;
+using namespace ksr;
+
+#define KS_HIP(call)                                                                                               \
+    do {                                                                                                           \
+        hipError_t e_ = (call);                                                                                    \
+        if (e_ != hipSuccess) {                                                                                    \
+            ksa::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));                              \
+            return KS_ERR_HIP;                                                                                     \
+        }                                                                                                          \
+    } while (0)
+#define KS_REQUIRE(cond, msg)                                                                                      \
+    do {                                                                                                           \
+        if (!(cond)) {                                                                                             \
+            ksa::set_error(msg);                                                                                   \
+            return KS_ERR_INVALID;                                                                                 \
+        }                                                                                                          \
+    } while (0)
+
+namespace {
+
+constexpr int kLdx = 68;                     // LDS row stride (bf16 elements) of 64-wide activations
+constexpr int kActBytes = 64 * kLdx * 2;     // 8704
+constexpr int kOutBytes = 64 * kGuideOut * 4; // 12288
+constexpr int kFwdWaveBytes = kActBytes + kOutBytes;
+constexpr int kFwdWaves = 2;                 // waves per forward block
+constexpr int kTrainWaveBytes = 4 * kActBytes + kOutBytes;
+constexpr int kMlpW1 = 64 * 64, kMlpW2 = 64 * 64, kMlpW3 = 64 * kGuideOut;
+constexpr int kMlpParams = kMlpW1 + kMlpW2 + kMlpW3;
+
+__device__ __forceinline__ bf16x4 lds_a(const uint16_t *buf, int row, int col) {
+    return *reinterpret_cast<const bf16x4 *>(buf + row * kLdx + col);
+}
+__device__ __forceinline__ bf16x4 lds_strided(const uint16_t *buf, int row0, int col) {
+    bf16x4 v;
+    v.x = (short)buf[(row0 + 0) * kLdx + col];
+    v.y = (short)buf[(row0 + 1) * kLdx + col];
+    v.z = (short)buf[(row0 + 2) * kLdx + col];
+    v.w = (short)buf[(row0 + 3) * kLdx + col];
+    return v;
+}
+__device__ __forceinline__ bf16x4 glb_a(const uint16_t *p) { return *reinterpret_cast<const bf16x4 *>(p); }
+
+// out[64 x N] = act(in[64 x 64] * W) for one wave; W given transposed ([out][in], BF16, in-size 64).
+// N = 16 * nt. The input is BF16 in LDS (stride kLdx); with RELU the result goes to `out_bf` as BF16,
+// otherwise to `out_f` as FP32 with row stride ldo.
+template <int NT, bool RELU>
+__device__ __forceinline__ void wave_dense(const uint16_t *in, const uint16_t *__restrict__ wt, uint16_t *out_bf,
+                                           float *out_f, int ldo) {
+    const int lane = __lane_id();
+    const int r16 = lane & 15, q4 = (lane >> 4) * 4;
+    bf16x4 b[NT][4];
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) b[n][kk] = glb_a(wt + (n * 16 + r16) * 64 + kk * 16 + q4);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        f32x4 acc[NT];
+#pragma unroll
+        for (int n = 0; n < NT; ++n) acc[n] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            bf16x4 a = lds_a(in, m * 16 + r16, kk * 16 + q4);
+#pragma unroll
+            for (int n = 0; n < NT; ++n) acc[n] = mfma16(a, b[n][kk], acc[n]);
+        }
+#pragma unroll
+        for (int n = 0; n < NT; ++n)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int row = m * 16 + q4 + r, col = n * 16 + r16;
+                if (RELU) out_bf[row * kLdx + col] = f2bf(fmaxf(acc[n][r], 0.0f));
+                else out_f[row * ldo + col] = acc[n][r];
+            }
+    }
+}
+
+// Forward of 64 queries per wave through the MLP; the raw outputs end in O (FP32, row stride 48).
+__device__ __forceinline__ void wave_forward(const GuideNet &net, uint16_t *X, uint16_t *H, float *O) {
+    wave_dense<4, true>(X, net.wt[0], H, nullptr, 0);
+    __syncthreads();
+    wave_dense<4, true>(H, net.wt[1], X, nullptr, 0);
+    __syncthreads();
+    wave_dense<3, false>(X, net.wt[2], nullptr, O, kGuideOut);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void store_mixture(const Mixture &m, float4 *dst) {
+    dst[0] = make_float4(m.w[0], m.w[1], m.w[2], m.w[3]);
+    dst[1] = make_float4(m.w[4], m.w[5], m.w[6], m.w[7]);
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k)
+        dst[2 + k] = make_float4(m.lobe[k].mu.x, m.lobe[k].mu.y, m.lobe[k].mu.z, m.lobe[k].kappa);
+}
+
+__global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet net, int n, const float *q, float4 *mix) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint8_t *base = smem + wave * kFwdWaveBytes;
+    uint16_t *X = reinterpret_cast<uint16_t *>(base);
+    uint16_t *H = reinterpret_cast<uint16_t *>(base + kActBytes);
+    float *O = reinterpret_cast<float *>(base + kActBytes);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float *e = q + 9 * (size_t)i;
+        guide_encode(net, mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]), X + lane * kLdx);
+    } else {
+        guide_zero_row(X + lane * kLdx);
+    }
+    __syncthreads();
+    wave_forward(net, X, H, O);
+    if (i < n) {
+        float o[kGuideOut];
+#pragma unroll
+        for (int k = 0; k < kGuideOut; ++k) o[k] = O[lane * kGuideOut + k];
+        Mixture m;
+        head_forward(o, m);
+        store_mixture(m, mix + 10 * (size_t)i);
+    }
+}
+
+// Guide queries straight from the wavefront: shading position, geometric normal facing wo, wo.
+__global__ __launch_bounds__(64 * kFwdWaves) void k_guide_infer(GuideNet net, DevScene sc, const uint32_t *n_dev,
+                                                                const float4 *rd, const float4 *hits, float4 *mix) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const uint32_t n = *n_dev;
+    if (blockIdx.x * blockDim.x >= n) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint8_t *base = smem + wave * kFwdWaveBytes;
+    uint16_t *X = reinterpret_cast<uint16_t *>(base);
+    uint16_t *H = reinterpret_cast<uint16_t *>(base + kActBytes);
+    float *O = reinterpret_cast<float *>(base + kActBytes);
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool valid = false;
+    if (i < n) {
+        float4 h = hits[i];
+        int32_t prim = __float_as_int(h.w);
+        if (prim >= 0) {
+            valid = true;
+            uint4 tri = sc.tri_idx[prim];
+            SurfHit it = make_hit(sc, tri, h.y, h.z, ld3(rd[i]));
+            guide_encode(net, it.p, it.n, -normalized(ld3(rd[i])), X + lane * kLdx);
+        }
+    }
+    if (!valid) guide_zero_row(X + lane * kLdx);
+    __syncthreads();
+    wave_forward(net, X, H, O);
+    if (valid) {
+        float o[kGuideOut];
+#pragma unroll
+        for (int k = 0; k < kGuideOut; ++k) o[k] = O[lane * kGuideOut + k];
+        Mixture m;
+        head_forward(o, m);
+        store_mixture(m, mix + 10 * (size_t)i);
+    }
+}
+
+// One wave per block. Every wave loops over chunks of 64 samples, accumulates the three weight
+// gradients in registers (MFMA) and scatters hash-grid gradients with FP32 atomics; at the end it writes
+// its weight-gradient partials (one slot per wave) for k_guide_reduce.
+struct TrainSample {
+    float4 xc; // position, weight c
+    float4 n;
+    float4 wo;
+    float4 wi;
+};
+static_assert(sizeof(TrainSample) == ksg::kTrainSampleBytes, "training sample layout");
+
+__global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, const TrainSample *samples, float *grid_grad,
+                                                    float *partials, double *loss_out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const int lane = threadIdx.x;
+    const int r16 = lane & 15, q4 = (lane >> 4) * 4;
+    uint16_t *X = reinterpret_cast<uint16_t *>(smem);
+    uint16_t *H1 = reinterpret_cast<uint16_t *>(smem + kActBytes);
+    uint16_t *H2 = reinterpret_cast<uint16_t *>(smem + 2 * kActBytes);
+    uint16_t *G = reinterpret_cast<uint16_t *>(smem + 3 * kActBytes);
+    float *O = reinterpret_cast<float *>(smem + 4 * kActBytes);
+    uint16_t *D2 = reinterpret_cast<uint16_t *>(O); // dH2 (BF16) re-uses O once the head is done
+    float *DX = O;                                  // dX (FP32, stride 32) once dH2 is consumed
+
+    f32x4 g3[4][3], g2[4][4], g1[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) g3[a][b] = f32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int b = 0; b < 4; ++b) g2[a][b] = g1[a][b] = f32x4{0, 0, 0, 0};
+    }
+    double loss = 0.0;
+    const int n_chunks = (n + 63) / 64;
+    for (int chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
+        int i = chunk * 64 + lane;
+        bool valid = i < n;
+        TrainSample s{};
+        if (valid) s = samples[i];
+        if (valid) guide_encode(net, ld3(s.xc), ld3(s.n), ld3(s.wo), X + lane * kLdx);
+        else guide_zero_row(X + lane * kLdx);
+        __syncthreads();
+        wave_dense<4, true>(X, net.wt[0], H1, nullptr, 0);
+        __syncthreads();
+        wave_dense<4, true>(H1, net.wt[1], H2, nullptr, 0);
+        __syncthreads();
+        wave_dense<3, false>(H2, net.wt[2], nullptr, O, kGuideOut);
+        __syncthreads();
+        // head backward: d(-c log q)/d outputs
+        float o[kGuideOut], g[kGuideOut];
+#pragma unroll
+        for (int k = 0; k < kGuideOut; ++k) o[k] = O[lane * kGuideOut + k];
+        if (valid && s.xc.w > 0.0f) {
+            loss += (double)head_backward(o, normalized(ld3(s.wi)), s.xc.w, g);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kGuideOut; ++k) g[k] = 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kGuideOut / 2; ++k) reinterpret_cast<uint32_t *>(G + lane * kLdx)[k] = bf_pair(g[2 * k], g[2 * k + 1]);
+        __syncthreads();
+        // dW3 += H2^T G   (M = 64 hidden units, N = 48 outputs, K = 64 samples)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            bf16x4 bt[3];
+#pragma unroll
+            for (int nn = 0; nn < 3; ++nn) bt[nn] = lds_strided(G, kk * 16 + q4, nn * 16 + r16);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                bf16x4 a = lds_strided(H2, kk * 16 + q4, m * 16 + r16);
+#pragma unroll
+                for (int nn = 0; nn < 3; ++nn) g3[m][nn] = mfma16(a, bt[nn], g3[m][nn]);
+            }
+        }
+        // dH2 = (G W3^T) * relu'(H2)   (M = 64 samples, N = 64 units, K = 48 outputs); W3 is [in][out]
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f32x4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = f32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 3; ++kk) {
+                bf16x4 a = lds_a(G, m * 16 + r16, kk * 16 + q4);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[t] = mfma16(a, glb_a(net.w[2] + (t * 16 + r16) * kGuideOut + kk * 16 + q4), acc[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int row = m * 16 + q4 + r, col = t * 16 + r16;
+                    float act = bf2f(H2[row * kLdx + col]);
+                    D2[row * kLdx + col] = f2bf(act > 0.0f ? acc[t][r] : 0.0f);
+                }
+        }
+        __syncthreads();
+        // dW2 += H1^T dH2
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            bf16x4 bt[4];
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) bt[nn] = lds_strided(D2, kk * 16 + q4, nn * 16 + r16);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                bf16x4 a = lds_strided(H1, kk * 16 + q4, m * 16 + r16);
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) g2[m][nn] = mfma16(a, bt[nn], g2[m][nn]);
+            }
+        }
+        // dH1 = (dH2 W2^T) * relu'(H1) -> G
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f32x4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = f32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                bf16x4 a = lds_a(D2, m * 16 + r16, kk * 16 + q4);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[t] = mfma16(a, glb_a(net.w[1] + (t * 16 + r16) * 64 + kk * 16 + q4), acc[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    int row = m * 16 + q4 + r, col = t * 16 + r16;
+                    float act = bf2f(H1[row * kLdx + col]);
+                    G[row * kLdx + col] = f2bf(act > 0.0f ? acc[t][r] : 0.0f);
+                }
+        }
+        __syncthreads();
+        // dW1 += X^T dH1
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            bf16x4 bt[4];
+#pragma unroll
+            for (int nn = 0; nn < 4; ++nn) bt[nn] = lds_strided(G, kk * 16 + q4, nn * 16 + r16);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                bf16x4 a = lds_strided(X, kk * 16 + q4, m * 16 + r16);
+#pragma unroll
+                for (int nn = 0; nn < 4; ++nn) g1[m][nn] = mfma16(a, bt[nn], g1[m][nn]);
+            }
+        }
+        // dX[:, 0:32] = dH1 W1^T  (grid features only)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            f32x4 acc[2];
+            acc[0] = acc[1] = f32x4{0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                bf16x4 a = lds_a(G, m * 16 + r16, kk * 16 + q4);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) acc[t] = mfma16(a, glb_a(net.w[0] + (t * 16 + r16) * 64 + kk * 16 + q4), acc[t]);
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) DX[(m * 16 + q4 + r) * 32 + t * 16 + r16] = acc[t][r];
+        }
+        __syncthreads();
+        // hash-grid scatter
+        if (valid && s.xc.w > 0.0f) {
+            float u[3];
+            normalise_position(net, ld3(s.xc), u);
+            for (int l = 0; l < kGuideLevels; ++l) {
+                float d0 = DX[lane * 32 + 2 * l], d1 = DX[lane * 32 + 2 * l + 1];
+                if (d0 == 0.0f && d1 == 0.0f) continue;
+                GridCorner c;
+                grid_level_corners(net, l, u, c);
+                for (int k = 0; k < 8; ++k) {
+                    atomicAdd(&grid_grad[2 * (size_t)c.idx[k]], c.w[k] * d0);
+                    atomicAdd(&grid_grad[2 * (size_t)c.idx[k] + 1], c.w[k] * d1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // weight-gradient partials of this wave in [in][out] order
+    float *dst = partials + (size_t)blockIdx.x * kMlpParams;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int nn = 0; nn < 4; ++nn)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int row = m * 16 + q4 + r, col = nn * 16 + r16;
+                dst[row * 64 + col] = g1[m][nn][r];
+                dst[kMlpW1 + row * 64 + col] = g2[m][nn][r];
+                if (nn < 3) dst[kMlpW1 + kMlpW2 + row * kGuideOut + col] = g3[m][nn][r];
+            }
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) atomicAdd(loss_out, loss);
+}
+
+__global__ void k_guide_reduce(const float *partials, int n_waves, float *grad_mlp) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float s = 0.0f;
+    for (int w = 0; w < n_waves; ++w) s += partials[(size_t)w * kMlpParams + p];
+    grad_mlp[p] = s;
+}
+
+// Adam (gpu/shaders/adam.slang:39-50) over every parameter; refreshes the low-precision copies and
+// clears the gradient for the next step. Grid parameters are processed in FP16 pairs.
+__global__ void k_guide_adam(float *params, float *grad, float *m1, float *m2, int64_t n_grid, float lr, float beta1,
+                             float beta2, float eps, float bc1, float bc2, uint32_t *grid_h, uint16_t *w_bf,
+                             uint16_t *wt_bf) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n_pairs = n_grid / 2;
+    auto step = [&](int64_t i) {
+        float g = grad[i];
+        float m = m1[i], v = m2[i];
+        float mt = g + beta1 * (m - g);     // lerp(g, m, beta1)
+        float vt = g * g + beta2 * (v - g * g); // lerp(g*g, v, beta2)
+        float p = params[i] - lr * (mt / bc1) / (sqrtf(vt / bc2) + eps);
+        params[i] = p;
+        m1[i] = mt;
+        m2[i] = vt;
+        grad[i] = 0.0f;
+        return p;
+    };
+    if (t < n_pairs) {
+        float a = step(2 * t), b = step(2 * t + 1);
+        uint32_t ha = float_to_half_bits(a), hb = float_to_half_bits(b);
+        grid_h[t] = ha | (hb << 16);
+        return;
+    }
+    int64_t j = t - n_pairs;
+    if (j >= kMlpParams) return;
+    float p = step(n_grid + j);
+    uint16_t h = f2bf(p);
+    w_bf[j] = h;
+    // transposed copy: layer sizes [in][out] = 64x64, 64x64, 64x48
+    int64_t base = j < kMlpW1 ? 0 : (j < kMlpW1 + kMlpW2 ? kMlpW1 : kMlpW1 + kMlpW2);
+    int64_t outs = j < kMlpW1 + kMlpW2 ? 64 : kGuideOut;
+    int64_t local = j - base, in = local / outs, out = local % outs;
+    wt_bf[base + out * 64 + in] = h;
+}
+
+__global__ void k_guide_refresh(const float *params, int64_t n_grid, uint32_t *grid_h, uint16_t *w_bf, uint16_t *wt_bf) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t n_pairs = n_grid / 2;
+    if (t < n_pairs) {
+        uint32_t ha = float_to_half_bits(params[2 * t]);
+        uint32_t hb = float_to_half_bits(params[2 * t + 1]);
+        grid_h[t] = ha | (hb << 16);
+        return;
+    }
+    int64_t j = t - n_pairs;
+    if (j >= kMlpParams) return;
+    uint16_t h = f2bf(params[n_grid + j]);
+    w_bf[j] = h;
+    int64_t base = j < kMlpW1 ? 0 : (j < kMlpW1 + kMlpW2 ? kMlpW1 : kMlpW1 + kMlpW2);
+    int64_t outs = j < kMlpW1 + kMlpW2 ? 64 : kGuideOut;
+    int64_t local = j - base, in = local / outs, out = local % outs;
+    wt_bf[base + out * 64 + in] = h;
+}
+
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+__device__ __forceinline__ float hash_unit(uint64_t i, uint32_t seed) {
+    uint32_t h = pcg_hash((uint32_t)i ^ pcg_hash((uint32_t)(i >> 32) + seed));
+    return (float)(h >> 8) * 0x1p-24f;
+}
+
+// Grid features U(-1e-4, 1e-4); hidden layers He-uniform; the output layer scaled by 0.1 so the initial
+// mixture is close to uniform lobes of small kappa.
+__global__ void k_guide_init(float *params, float *m1, float *m2, float *grad, int64_t n_grid, uint32_t seed) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_grid + kMlpParams) return;
+    float u = hash_unit((uint64_t)i, seed) * 2.0f - 1.0f;
+    float v;
+    if (i < n_grid) {
+        v = 1e-4f * u;
+    } else {
+        int64_t j = i - n_grid;
+        float lim = sqrtf(6.0f / 64.0f);
+        v = u * (j < kMlpW1 + kMlpW2 ? lim : 0.1f * lim);
+    }
+    params[i] = v;
+    m1[i] = 0.0f;
+    m2[i] = 0.0f;
+    grad[i] = 0.0f;
+}
+
+__global__ void k_vmm_pdf(int64_t n, const float *mix, const float *dirs, float *pdf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mixture m;
+    const float *e = mix + 40 * i;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        m.w[k] = e[k];
+        m.lobe[k].mu = mk(e[8 + 4 * k], e[9 + 4 * k], e[10 + 4 * k]);
+        m.lobe[k].kappa = e[11 + 4 * k];
+    }
+    pdf[i] = mixture_pdf(m, mk(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+}
+
+__global__ void k_vmm_sample(int64_t n, const float *mix, const float *u, float *dirs, float *pdf) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mixture m;
+    const float *e = mix + 40 * i;
+    for (int k = 0; k < kGuideLobes; ++k) {
+        m.w[k] = e[k];
+        m.lobe[k].mu = mk(e[8 + 4 * k], e[9 + 4 * k], e[10 + 4 * k]);
+        m.lobe[k].kappa = e[11 + 4 * k];
+    }
+    float ur;
+    int sel = mixture_select(m.w, u[2 * i], &ur);
+    Lobe l{mk(e[8 + 4 * sel], e[9 + 4 * sel], e[10 + 4 * sel]), e[11 + 4 * sel]};
+    f3 w = vmf_sample(l, ur, u[2 * i + 1]);
+    dirs[3 * i] = w.x, dirs[3 * i + 1] = w.y, dirs[3 * i + 2] = w.z;
+    pdf[i] = mixture_pdf(m, w);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        release();
+        hipError_t e = hipMalloc(&p, std::max<size_t>(n, 16));
+        if (e == hipSuccess) bytes = n;
+        else p = nullptr;
+        return e;
+    }
+    hipError_t ensure(size_t n) { return n <= bytes ? hipSuccess : alloc(n); }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+} // namespace
+
+struct ks_guide {
+    ks_device *dev = nullptr;
+    hipStream_t stream = nullptr;
+    ks_guide_config cfg{};
+    int64_t n_grid = 0;
+    uint32_t log2_T = 19;
+    uint32_t res[kGuideLevels];
+    float lo[3], inv_ext[3];
+    DevBuf params, grad, m1, m2, grid_h, w_bf, wt_bf, partials, loss, samples;
+    int32_t n_partials = 0;
+    int64_t steps = 0;
+};
+
+namespace ksg {
+
+GuideNet net_of(const ks_guide *g) {
+    GuideNet n{};
+    n.grid_h = g->grid_h.as<uint32_t>();
+    const uint16_t *w = g->w_bf.as<uint16_t>(), *wt = g->wt_bf.as<uint16_t>();
+    n.w[0] = w;
+    n.w[1] = w + kMlpW1;
+    n.w[2] = w + kMlpW1 + kMlpW2;
+    n.wt[0] = wt;
+    n.wt[1] = wt + kMlpW1;
+    n.wt[2] = wt + kMlpW1 + kMlpW2;
+    n.log2_T = g->log2_T;
+    for (int l = 0; l < kGuideLevels; ++l) n.res[l] = g->res[l];
+    for (int k = 0; k < 3; ++k) {
+        n.lo[k] = g->lo[k];
+        n.inv_ext[k] = g->inv_ext[k];
+    }
+    return n;
+}
+
+float guide_fraction(const ks_guide *g) { return g->cfg.guide_fraction; }
+int32_t guide_batch(const ks_guide *g) { return g->cfg.train_batch; }
+int32_t guide_max_steps(const ks_guide *g) { return g->cfg.max_steps_per_pass; }
+
+int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
+                 const float4 *hits, float4 *mix, hipStream_t s) {
+    unsigned blocks = (n_max + 64 * kFwdWaves - 1) / (64 * kFwdWaves);
+    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(64 * kFwdWaves), kFwdWaves * kFwdWaveBytes, s, net_of(g), sc,
+                       n_dev, rd, hits, mix);
+    KS_HIP(hipGetLastError());
+    return KS_OK;
+}
+
+// One Adam step on n device-resident samples (TrainSample records).
+int train_on_device(ks_guide *g, const void *samples, int n, double *loss_host, hipStream_t s) {
+    if (n <= 0) return KS_OK;
+    int n_chunks = (n + 63) / 64;
+    int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
+    KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
+    KS_HIP(hipMemsetAsync(g->loss.p, 0, sizeof(double), s));
+    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, net_of(g), n,
+                       static_cast<const TrainSample *>(samples), g->grad.as<float>(), g->partials.as<float>(),
+                       g->loss.as<double>());
+    KS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
+                       g->grad.as<float>() + g->n_grid);
+    g->steps += 1;
+    float bc1 = 1.0f - std::pow(g->cfg.beta1, (float)g->steps), bc2 = 1.0f - std::pow(g->cfg.beta2, (float)g->steps);
+    int64_t work = g->n_grid / 2 + kMlpParams;
+    hipLaunchKernelGGL(k_guide_adam, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, g->params.as<float>(),
+                       g->grad.as<float>(), g->m1.as<float>(), g->m2.as<float>(), g->n_grid, g->cfg.learning_rate,
+                       g->cfg.beta1, g->cfg.beta2, g->cfg.eps, bc1, bc2, g->grid_h.as<uint32_t>(),
+                       g->w_bf.as<uint16_t>(), g->wt_bf.as<uint16_t>());
+    KS_HIP(hipGetLastError());
+    if (loss_host) {
+        KS_HIP(hipMemcpyAsync(loss_host, g->loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+    }
+    return KS_OK;
+}
+
+} // namespace ksg
+
+// ---------------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------------
+extern "C" {
+
+int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float bound_lo[3], const float bound_hi[3],
+                    ks_guide **out) {
+    KS_REQUIRE(dev && bound_lo && bound_hi && out, "null argument");
+    *out = nullptr;
+    auto g = std::make_unique<ks_guide>();
+    g->dev = dev;
+    g->stream = ksg::device_stream(dev);
+    ks_guide_config c{};
+    if (cfg_in) c = *cfg_in;
+    if (c.log2_table_size == 0) c.log2_table_size = 19;
+    if (c.base_resolution == 0.0f) c.base_resolution = 16.0f;
+    if (c.max_resolution == 0.0f) c.max_resolution = 2048.0f;
+    if (c.learning_rate == 0.0f) c.learning_rate = 1e-2f;
+    if (c.beta1 == 0.0f) c.beta1 = 0.9f;
+    if (c.beta2 == 0.0f) c.beta2 = 0.999f;
+    if (c.eps == 0.0f) c.eps = 1e-8f;
+    if (c.guide_fraction == 0.0f) c.guide_fraction = 0.5f;
+    if (c.train_batch == 0) c.train_batch = 1 << 18;
+    if (c.max_steps_per_pass == 0) c.max_steps_per_pass = 4;
+    KS_REQUIRE(c.log2_table_size >= 10 && c.log2_table_size <= 24, "log2_table_size must be in [10, 24]");
+    KS_REQUIRE(c.max_resolution >= c.base_resolution && c.base_resolution >= 2.0f, "bad grid resolutions");
+    KS_REQUIRE(c.guide_fraction >= 0.0f && c.guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
+    g->cfg = c;
+    g->log2_T = (uint32_t)c.log2_table_size;
+    // resolutions: floor(base * b^l), b = (max / base)^(1 / (L - 1)), in FP64
+    double growth = std::exp(std::log((double)c.max_resolution / (double)c.base_resolution) / (kGuideLevels - 1));
+    for (int l = 0; l < kGuideLevels; ++l)
+        g->res[l] = (uint32_t)std::floor((double)c.base_resolution * std::pow(growth, (double)l) + 1e-9);
+    for (int k = 0; k < 3; ++k) {
+        float ext = bound_hi[k] - bound_lo[k];
+        KS_REQUIRE(ext >= 0.0f, "bound_hi below bound_lo");
+        float pad = std::max(ext * 0.01f, 1e-4f);
+        g->lo[k] = bound_lo[k] - pad;
+        g->inv_ext[k] = 1.0f / (ext + 2.0f * pad);
+    }
+    g->n_grid = (int64_t)kGuideLevels * ((int64_t)1 << g->log2_T) * kGuideFeatures;
+    int64_t np = g->n_grid + kMlpParams;
+    KS_HIP(hipSetDevice(ksg::device_ordinal(dev)));
+    KS_HIP(g->params.alloc(sizeof(float) * np));
+    KS_HIP(g->grad.alloc(sizeof(float) * np));
+    KS_HIP(g->m1.alloc(sizeof(float) * np));
+    KS_HIP(g->m2.alloc(sizeof(float) * np));
+    KS_HIP(g->grid_h.alloc(sizeof(uint32_t) * g->n_grid / 2));
+    KS_HIP(g->w_bf.alloc(sizeof(uint16_t) * kMlpParams));
+    KS_HIP(g->wt_bf.alloc(sizeof(uint16_t) * kMlpParams));
+    KS_HIP(g->loss.alloc(sizeof(double)));
+    hipLaunchKernelGGL(k_guide_init, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, g->stream,
+                       g->params.as<float>(), g->m1.as<float>(), g->m2.as<float>(), g->grad.as<float>(), g->n_grid,
+                       (uint32_t)(c.seed ^ (c.seed >> 32)));
+    int64_t work = g->n_grid / 2 + kMlpParams;
+    hipLaunchKernelGGL(k_guide_refresh, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, g->stream,
+                       g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
+                       g->wt_bf.as<uint16_t>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipStreamSynchronize(g->stream));
+    *out = g.release();
+    return KS_OK;
+}
+
+int ks_guide_free(ks_guide *g) {
+    if (g) {
+        (void)hipStreamSynchronize(g->stream);
+        delete g;
+    }
+    return KS_OK;
+}
+
+int ks_guide_param_count(const ks_guide *g, int64_t *n_grid, int64_t *n_mlp) {
+    KS_REQUIRE(g, "guide is null");
+    if (n_grid) *n_grid = g->n_grid;
+    if (n_mlp) *n_mlp = kMlpParams;
+    return KS_OK;
+}
+
+int ks_guide_grid_info(const ks_guide *g, int32_t res[16], float lo[3], float inv_ext[3]) {
+    KS_REQUIRE(g, "guide is null");
+    for (int l = 0; l < kGuideLevels && res; ++l) res[l] = (int32_t)g->res[l];
+    for (int k = 0; k < 3; ++k) {
+        if (lo) lo[k] = g->lo[k];
+        if (inv_ext) inv_ext[k] = g->inv_ext[k];
+    }
+    return KS_OK;
+}
+
+int ks_guide_get_params(const ks_guide *g, float *params) {
+    KS_REQUIRE(g && params, "null argument");
+    KS_HIP(hipMemcpyAsync(params, g->params.p, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyDeviceToHost,
+                          g->stream));
+    KS_HIP(hipStreamSynchronize(g->stream));
+    return KS_OK;
+}
+
+int ks_guide_set_params(ks_guide *g, const float *params) {
+    KS_REQUIRE(g && params, "null argument");
+    KS_HIP(hipMemcpyAsync(g->params.p, params, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyHostToDevice,
+                          g->stream));
+    int64_t work = g->n_grid / 2 + kMlpParams;
+    hipLaunchKernelGGL(k_guide_refresh, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, g->stream,
+                       g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
+                       g->wt_bf.as<uint16_t>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipStreamSynchronize(g->stream));
+    return KS_OK;
+}
+
+int ks_guide_forward(ks_guide *g, int64_t n, const float *queries, float *mixtures) {
+    KS_REQUIRE(g && queries && mixtures && n > 0 && n < (1ll << 30), "invalid argument");
+    hipStream_t s = g->stream;
+    DevBuf q, m;
+    KS_HIP(q.alloc(sizeof(float) * 9 * n));
+    KS_HIP(m.alloc(sizeof(float4) * 10 * n));
+    KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
+    unsigned blocks = (unsigned)((n + 64 * kFwdWaves - 1) / (64 * kFwdWaves));
+    hipLaunchKernelGGL(k_guide_forward, dim3(blocks), dim3(64 * kFwdWaves), kFwdWaves * kFwdWaveBytes, s,
+                       ksg::net_of(g), (int)n, q.as<float>(), m.as<float4>());
+    KS_HIP(hipGetLastError());
+    std::vector<float4> h((size_t)n * 10);
+    KS_HIP(hipMemcpyAsync(h.data(), m.p, sizeof(float4) * 10 * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < n; ++i) { // [w0..w7, (mu xyz, kappa) x 8]
+        const float4 *e = &h[10 * i];
+        float *o = mixtures + 40 * i;
+        o[0] = e[0].x, o[1] = e[0].y, o[2] = e[0].z, o[3] = e[0].w;
+        o[4] = e[1].x, o[5] = e[1].y, o[6] = e[1].z, o[7] = e[1].w;
+        for (int k = 0; k < 8; ++k) {
+            o[8 + 4 * k] = e[2 + k].x;
+            o[9 + 4 * k] = e[2 + k].y;
+            o[10 + 4 * k] = e[2 + k].z;
+            o[11 + 4 * k] = e[2 + k].w;
+        }
+    }
+    return KS_OK;
+}
+
+static int upload_samples(ks_guide *g, int64_t n, const float *queries, const float *wi, const float *weight) {
+    std::vector<TrainSample> h((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float *q = queries + 9 * i;
+        h[i].xc = make_float4(q[0], q[1], q[2], weight[i]);
+        h[i].n = make_float4(q[3], q[4], q[5], 0.0f);
+        h[i].wo = make_float4(q[6], q[7], q[8], 0.0f);
+        h[i].wi = make_float4(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2], 0.0f);
+    }
+    KS_HIP(g->samples.ensure(sizeof(TrainSample) * n));
+    KS_HIP(hipMemcpyAsync(g->samples.p, h.data(), sizeof(TrainSample) * n, hipMemcpyHostToDevice, g->stream));
+    KS_HIP(hipStreamSynchronize(g->stream));
+    return KS_OK;
+}
+
+int ks_guide_gradient(ks_guide *g, int64_t n, const float *queries, const float *wi, const float *weight,
+                      double *loss, float *grad) {
+    KS_REQUIRE(g && queries && wi && weight && grad && n > 0 && n < (1ll << 30), "invalid argument");
+    int e = upload_samples(g, n, queries, wi, weight);
+    if (e) return e;
+    hipStream_t s = g->stream;
+    int n_chunks = (int)((n + 63) / 64);
+    int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
+    KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
+    KS_HIP(hipMemsetAsync(g->loss.p, 0, sizeof(double), s));
+    KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * g->n_grid, s));
+    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, ksg::net_of(g), (int)n,
+                       g->samples.as<TrainSample>(), g->grad.as<float>(), g->partials.as<float>(), g->loss.as<double>());
+    KS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
+                       g->grad.as<float>() + g->n_grid);
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(grad, g->grad.p, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyDeviceToHost, s));
+    if (loss) KS_HIP(hipMemcpyAsync(loss, g->loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * (g->n_grid + kMlpParams), s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_guide_train_step(ks_guide *g, int64_t n, const float *queries, const float *wi, const float *weight,
+                        double *loss) {
+    KS_REQUIRE(g && queries && wi && weight && n > 0 && n < (1ll << 30), "invalid argument");
+    int e = upload_samples(g, n, queries, wi, weight);
+    if (e) return e;
+    return ksg::train_on_device(g, g->samples.p, (int)n, loss, g->stream);
+}
+
+int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *dirs, float *pdf) {
+    KS_REQUIRE(dev && mixtures && dirs && pdf && n > 0, "invalid argument");
+    hipStream_t s = ksg::device_stream(dev);
+    DevBuf m, d, p;
+    KS_HIP(m.alloc(sizeof(float) * 40 * n));
+    KS_HIP(d.alloc(sizeof(float) * 3 * n));
+    KS_HIP(p.alloc(sizeof(float) * n));
+    KS_HIP(hipMemcpyAsync(m.p, mixtures, sizeof(float) * 40 * n, hipMemcpyHostToDevice, s));
+    KS_HIP(hipMemcpyAsync(d.p, dirs, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_vmm_pdf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, m.as<float>(), d.as<float>(),
+                       p.as<float>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(pdf, p.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_vmm_sample(ks_device *dev, int64_t n, const float *mixtures, const float *u, float *dirs, float *pdf) {
+    KS_REQUIRE(dev && mixtures && u && dirs && pdf && n > 0, "invalid argument");
+    hipStream_t s = ksg::device_stream(dev);
+    DevBuf m, uu, d, p;
+    KS_HIP(m.alloc(sizeof(float) * 40 * n));
+    KS_HIP(uu.alloc(sizeof(float) * 2 * n));
+    KS_HIP(d.alloc(sizeof(float) * 3 * n));
+    KS_HIP(p.alloc(sizeof(float) * n));
+    KS_HIP(hipMemcpyAsync(m.p, mixtures, sizeof(float) * 40 * n, hipMemcpyHostToDevice, s));
+    KS_HIP(hipMemcpyAsync(uu.p, u, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_vmm_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, m.as<float>(),
+                       uu.as<float>(), d.as<float>(), p.as<float>());
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(dirs, d.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipMemcpyAsync(pdf, p.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+} // extern "C"
