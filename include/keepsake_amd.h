@@ -145,6 +145,7 @@ typedef struct ks_render_stats {
     uint64_t stack_overflows;           /* traversal stack entries spilled past LDS */
     int32_t iterations;
     int32_t queue_size;
+    uint64_t guide_samples;             /* training samples consumed by Adam steps (guided renders) */
 } ks_render_stats;
 
 typedef void (*ks_progress_fn)(const ks_film *film, int32_t spp_finished, void *user);
@@ -179,6 +180,9 @@ int ks_film_clear(ks_film *film);                                  /* RenderTarg
 int ks_film_download(const ks_film *film, double *rgb, double *weights); /* W*H*3, W*H*2 */
 int ks_film_composite(const ks_film *film, float *rgb);            /* composite (render_target.cpp:187-192) */
 int ks_film_free(ks_film *film);
+/* relMSE of two films' composites: mean over pixels and channels of (x - r)^2 / (r^2 + 0.01), in FP64 on
+ * the device (the error metric of the guided-vs-plain comparison; no reference counterpart) */
+int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse);
 
 /* ---- the render call: SmallPT::run (small_pt.cpp:15-70). Accumulates into `film` (which must be
  *      render_width x render_height); calls progress after each spp batch of spp_prog_interval on the
@@ -226,6 +230,11 @@ typedef struct ks_guide_config {
     int32_t train_batch;        /* samples per Adam step (2^18) */
     int32_t max_steps_per_pass; /* Adam steps after each 1-spp pass of a guided render (4) */
     uint64_t seed;
+    int32_t train_spp;          /* guided renders train during their first train_spp samples per pixel, in
+                                   1-spp passes, then render the rest frozen in batched passes (0: train
+                                   throughout) */
+    float weight_clamp;         /* render training: sample weights divided by the pass mean, clamped at
+                                   this (20) */
 } ks_guide_config;
 
 int ks_guide_create(ks_device *dev, const ks_guide_config *cfg, const float bound_lo[3], const float bound_hi[3],
@@ -233,6 +242,15 @@ int ks_guide_create(ks_device *dev, const ks_guide_config *cfg, const float boun
 int ks_guide_free(ks_guide *guide);
 /* grid parameters (16 levels x 2^log2 entries x 2 features) and MLP weights (11264) */
 int ks_guide_param_count(const ks_guide *guide, int64_t *n_grid, int64_t *n_mlp);
+/* sampling fraction and Adam steps per pass of later guided renders (max_steps_per_pass <= 0: frozen) */
+int ks_guide_set_mode(ks_guide *guide, float guide_fraction, int32_t max_steps_per_pass);
+/* training progress; the last_* sums cover the most recent Adam step: sum c (-log q(wi)), sum c, and
+ * sum c (-log(cos(n, wi) / pi)) (the same cross-entropy for a cosine lobe, for comparison) */
+typedef struct ks_guide_stats {
+    int64_t steps, samples;
+    double last_loss, last_weight, last_loss_cosine;
+} ks_guide_stats;
+int ks_guide_get_stats(const ks_guide *guide, ks_guide_stats *stats);
 /* grid resolutions of the 16 levels and the position normalisation u = (x - lo) * inv_ext */
 int ks_guide_grid_info(const ks_guide *guide, int32_t res[16], float lo[3], float inv_ext[3]);
 /* FP32 master parameters: [grid][W1 64x64][W2 64x64][W3 64x48], weights [in][out] */
@@ -246,6 +264,16 @@ int ks_guide_gradient(ks_guide *guide, int64_t n, const float *queries, const fl
 /* one Adam step on that loss */
 int ks_guide_train_step(ks_guide *guide, int64_t n, const float *queries, const float *wi, const float *weight,
                         double *loss);
+/* small_pt with the guide: passes of one sample per pixel; after each pass the guide takes up to
+ * max_steps_per_pass Adam steps on training samples recorded at the first three vertices of each path
+ * (weight luminance of the radiance the continuation brought back, divided by the path throughput before
+ * the vertex). From the first trained pass on, vertices with roughness >= 0.05 pick the mixture with
+ * probability guide_fraction and the BSDF otherwise (one-sample MIS, continuation pdf
+ * alpha q + (1 - alpha) p_bsdf, also in the NEE and emitter MIS weights). Needs mis_mode consistent.
+ * No reference counterpart: the reference has no learned guide. */
+int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params,
+                              ks_guide *guide, ks_film *film, ks_progress_fn progress, void *user,
+                              ks_render_stats *stats);
 /* mixture pdf at directions / samples from mixtures (sample_vmm, vmf.h:121-127); u n x 2 */
 int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *dirs, float *pdf);
 int ks_vmm_sample(ks_device *dev, int64_t n, const float *mixtures, const float *u, float *dirs, float *pdf);

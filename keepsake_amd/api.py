@@ -137,6 +137,7 @@ class RenderStats(C.Structure):
         ("stack_overflows", C.c_uint64),
         ("iterations", C.c_int32),
         ("queue_size", C.c_int32),
+        ("guide_samples", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -176,6 +177,18 @@ class GuideConfig(C.Structure):
         ("train_batch", C.c_int32),
         ("max_steps_per_pass", C.c_int32),
         ("seed", C.c_uint64),
+        ("train_spp", C.c_int32),
+        ("weight_clamp", C.c_float),
+    ]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64),
+        ("samples", C.c_int64),
+        ("last_loss", C.c_double),
+        ("last_weight", C.c_double),
+        ("last_loss_cosine", C.c_double),
     ]
 
 
@@ -197,6 +210,7 @@ _SIGS = {
     "ks_film_download": ([_P, _P, _P], C.c_int),
     "ks_film_composite": ([_P, _P], C.c_int),
     "ks_film_free": ([_P], C.c_int),
+    "ks_film_relmse": ([_P, _P, C.POINTER(C.c_double)], C.c_int),
     "ks_small_pt_render": (
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), _P, PROGRESS_FN, _P, C.POINTER(RenderStats)],
         C.c_int,
@@ -217,8 +231,14 @@ _SIGS = {
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), C.c_int64, _P, _P, _P, _P],
         C.c_int,
     ),
+    "ks_small_pt_render_guided": (
+        [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), _P, _P, PROGRESS_FN, _P, C.POINTER(RenderStats)],
+        C.c_int,
+    ),
     "ks_guide_create": ([_P, C.POINTER(GuideConfig), _P, _P, C.POINTER(_P)], C.c_int),
     "ks_guide_free": ([_P], C.c_int),
+    "ks_guide_set_mode": ([_P, C.c_float, C.c_int32], C.c_int),
+    "ks_guide_get_stats": ([_P, C.POINTER(GuideStats)], C.c_int),
     "ks_guide_param_count": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "ks_guide_grid_info": ([_P, _P, _P, _P], C.c_int),
     "ks_guide_get_params": ([_P, _P], C.c_int),
@@ -595,6 +615,12 @@ class Film:
         _check(self.lib.ks_film_composite(self.h, _ptr(out)))
         return out
 
+    def relmse(self, reference: "Film") -> float:
+        """mean over pixels and channels of (x - r)^2 / (r^2 + 0.01) of the composites, FP64 on the device"""
+        v = C.c_double()
+        _check(self.lib.ks_film_relmse(self.h, reference.h, C.byref(v)))
+        return v.value
+
     def close(self) -> None:
         if self.h:
             self.lib.ks_film_free(self.h)
@@ -610,6 +636,17 @@ def small_pt_render(scene: Scene, cam: dict, params: SmallPTParams, film: Film, 
     else:
         cb = PROGRESS_FN()
     _check(scene.lib.ks_small_pt_render(scene.h, C.byref(c), C.byref(params), film.h, cb, None, C.byref(st)))
+    return st.as_dict()
+
+
+def small_pt_render_guided(scene: Scene, cam: dict, params: SmallPTParams, guide: "Guide", film: Film,
+                           progress=None) -> dict:
+    """small_pt with the learned guide (one-sample MIS, online training between 1-spp passes)."""
+    st = RenderStats()
+    c = camera_desc(cam)
+    cb = PROGRESS_FN(lambda f, s, u: progress(film, s)) if progress is not None else PROGRESS_FN()
+    _check(scene.lib.ks_small_pt_render_guided(scene.h, C.byref(c), C.byref(params), guide.h, film.h, cb, None,
+                                               C.byref(st)))
     return st.as_dict()
 
 
@@ -642,6 +679,18 @@ class Guide:
         if self.h:
             self.lib.ks_guide_free(self.h)
             self.h = None
+
+    def set_mode(self, guide_fraction: float, max_steps_per_pass: int) -> None:
+        _check(self.lib.ks_guide_set_mode(self.h, guide_fraction, max_steps_per_pass))
+
+    def stats(self) -> dict:
+        st = GuideStats()
+        _check(self.lib.ks_guide_get_stats(self.h, C.byref(st)))
+        d = {k: getattr(st, k) for k, _ in st._fields_}
+        w = d["last_weight"]
+        d["cross_entropy"] = d["last_loss"] / w if w > 0 else float("nan")
+        d["cross_entropy_cosine"] = d["last_loss_cosine"] / w if w > 0 else float("nan")
+        return d
 
     def grid_info(self):
         res = np.zeros(16, np.int32)

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ks_device.h"
+#include "ks_guide.h"
 #include "ks_guide_host.h"
 #include "ks_internal.h"
 #include "ks_render_types.h"
@@ -225,7 +226,7 @@ template <int MODE>
 __global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
                                                         const uint32_t *target, const float4 *contrib,
-                                                        float4 *path_L, float4 *samp, uint32_t *ovf,
+                                                        float4 *path_L, float4 *samp, float4 *recs, uint32_t *ovf,
                                                         uint32_t ovf_cap, unsigned long long *stats) {
     __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
     const uint32_t n = n_dev ? *n_dev : n_fixed;
@@ -256,6 +257,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint3
             v.y += c.y;
             v.z += c.z;
             *dst = v;
+            uint32_t slot = __float_as_uint(c.w); // training record of the vertex that cast the ray, + 1
+            if (recs && slot) recs[(size_t)(slot - 1) * kRecFloat4 + kRecNee] = make_float4(c.x, c.y, c.z, 0.0f);
         }
     }
     wave_add_stat(stats, kRaysShadow, n_rays);
@@ -362,10 +365,29 @@ KS_DEV float mesh_light_pdf(const DevScene &sc, const Light &l, f3 wi, float wi_
 }
 
 // One bounce of SmallPT::trace (small_pt.cpp:72-178) for every active path.
+// Mixture pdf at w from the 10-float4 record of one path (sum of w_k vmf_k(w), vmf.h:32-39).
+KS_DEV float mix_pdf(const float4 *m, f3 w) {
+    float4 wa = m[0], wb = m[1];
+    float wk[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+    float p = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float4 l = m[2 + k];
+        if (wk[k] > 0.0f)
+            p += wk[k] * __expf(vmf_log_norm(l.w) + l.w * fminf(0.0f, dot(ld3(l), w) - 1.0f));
+    }
+    return p;
+}
+
+// Shading of the current buffer (small_pt.cpp:79-176). GUIDED adds one-sample MIS between the BSDF and
+// the learned mixture (probability gs.alpha of the mixture at vertices whose roughness reaches
+// gs.min_roughness; the continuation pdf is alpha q + (1 - alpha) p_bsdf in the estimator and in both
+// MIS weights), and training records of the first kRecVertices vertices of each path.
+template <bool GUIDED>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, PathBufs cur, PathBufs nxt,
                                                const float4 *hits, Ctrl *ctrl, float4 *sh_o, float4 *sh_d,
                                                uint32_t *sh_tgt, float4 *sh_c, float4 *samp,
-                                               unsigned long long *stats) {
+                                               unsigned long long *stats, GuideShade gs) {
     const uint32_t n = ctrl->n_active;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n) return; // whole block idle
@@ -381,6 +403,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
     uint32_t nonfinite = 0;
     int bounce = 0;
     Sobol sob;
+    uint32_t rec_slot = 0; // record slot + 1 of this vertex (0: none)
 
     if (active) {
         w = cur.work[i];
@@ -446,6 +469,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                 done = true;
             } else {
                 f3 wo = -normalized(rd);
+                const float4 *mixp = GUIDED && gs.mix ? gs.mix + 10 * (size_t)i : nullptr;
+                const float alpha_m = GUIDED && mixp && m.roughness >= gs.min_roughness ? gs.alpha : 0.0f;
                 Closure c;
                 c.basecolor = mk(m.basecolor[0], m.basecolor[1], m.basecolor[2]);
                 c.roughness = m.roughness;
@@ -474,6 +499,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                         f3 f = bsdf_eval(wol, wil, c);
                         float pdf_b = is_zero(f) ? 0.0f : bsdf_pdf(wol, wil, c);
                         if (!is_zero(f) && pdf_b > 0.0f) {
+                            if (GUIDED && alpha_m > 0.0f) pdf_b = alpha_m * mix_pdf(mixp, wi) + (1.0f - alpha_m) * pdf_b;
                             float mis = 1.0f;
                             if (!delta_light)
                                 mis = rc.mis_mode == KS_MIS_CONSISTENT ? power_heur(pdf_l * pr_light, pdf_b)
@@ -501,12 +527,42 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                 sob.next2d(u0, u1);
                 f3 wil;
                 float pdf_wi = 0.0f;
-                f3 sbeta = bsdf_sample(wol, wil, c, u_lobe, u0, u1, pdf_wi);
+                f3 sbeta, wi_g = mk(0, 0, 0);
+                const f3 beta_k = beta;
+                if (GUIDED && alpha_m > 0.0f) {
+                    // one-sample MIS: u_lobe picks the strategy and is re-used inside it
+                    float pb = 0.0f;
+                    f3 fcos;
+                    if (u_lobe < alpha_m) {
+                        float4 wa = mixp[0], wb = mixp[1];
+                        float wk[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+                        float ur;
+                        int sel = mixture_select(wk, u0, &ur);
+                        float4 l4 = mixp[2 + sel];
+                        Lobe lobe;
+                        lobe.mu = ld3(l4);
+                        lobe.kappa = l4.w;
+                        wi_g = vmf_sample(lobe, ur, u1);
+                        wil = sh_to_local(it, wi_g);
+                        fcos = bsdf_eval(wol, wil, c);
+                        pb = is_zero(fcos) ? 0.0f : bsdf_pdf(wol, wil, c);
+                    } else {
+                        float ub = fminf((u_lobe - alpha_m) / (1.0f - alpha_m), kBeforeOne);
+                        f3 sb = bsdf_sample(wol, wil, c, ub, u0, u1, pb);
+                        fcos = sb * pb;
+                        wi_g = sh_to_world(it, wil);
+                    }
+                    float q = mix_pdf(mixp, wi_g);
+                    pdf_wi = alpha_m * q + (1.0f - alpha_m) * pb;
+                    sbeta = pdf_wi > 0.0f ? fcos / pdf_wi : mk(0, 0, 0);
+                } else {
+                    sbeta = bsdf_sample(wol, wil, c, u_lobe, u0, u1, pdf_wi);
+                }
                 if (!all_finite(sbeta)) ++nonfinite;
                 if (maxc(sbeta) == 0.0f || pdf_wi == 0.0f) {
                     done = true;
                 } else {
-                    f3 wi = sh_to_world(it, wil);
+                    f3 wi = GUIDED && alpha_m > 0.0f ? wi_g : sh_to_world(it, wil);
                     beta = beta * sbeta;
                     if (maxc(beta) == 0.0f) {
                         done = true;
@@ -518,6 +574,18 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                             float q = fmaxf(0.0f, 1.0f - maxc(beta));
                             if (sob.next() < q) done = true;
                             else beta = beta / (1.0f - q);
+                        }
+                        if (GUIDED && gs.recs && !done && bounce < kRecVertices) {
+                            uint32_t slot = w * (uint32_t)kRecVertices + (uint32_t)bounce;
+                            float4 *r = gs.recs + (size_t)slot * kRecFloat4;
+                            r[0] = make_float4(it.p.x, it.p.y, it.p.z, __uint_as_float(gs.stamp));
+                            r[1] = make_float4(it.n.x, it.n.y, it.n.z, 0.0f);
+                            r[2] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                            r[3] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+                            r[4] = make_float4(L.x, L.y, L.z, 0.0f);
+                            r[5] = make_float4(beta_k.x, beta_k.y, beta_k.z, 0.0f);
+                            r[kRecNee] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                            rec_slot = slot + 1;
                         }
                     }
                 }
@@ -541,7 +609,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         sh_o[k] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.0f);
         sh_d[k] = make_float4(sh_dir.x, sh_dir.y, sh_dir.z, sh_tmax);
         sh_tgt[k] = survive ? j : (0x80000000u | samp_index(rc, w));
-        sh_c[k] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, 0.0f);
+        sh_c[k] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, __uint_as_float(rec_slot));
     }
     wave_add_stat(stats, kNonFinite, nonfinite);
 }
@@ -573,6 +641,39 @@ __global__ __launch_bounds__(256) void k_film(RenderConsts rc, const float4 *sam
     rgb[3 * p + 2] = b;
     wts[2 * p] = wh;
     wts[2 * p + 1] = wm;
+}
+
+// relMSE of the composites of two films, mean over pixels and channels of (x - r)^2 / (r^2 + 0.01),
+// composite as RenderTarget2 (render_target.cpp:187-192), FP64 throughout; one partial sum per block.
+KS_DEV double composite_ch(const double *rgb, const double *wts, size_t p, int k, float backdrop) {
+    double wh = wts[2 * p], wm = wts[2 * p + 1];
+    double ws = (double)(uint32_t)(wh + wm);
+    double a = ws == 0.0 ? 0.0 : wh / ws;
+    double fg = wh == 0.0 ? 0.0 : rgb[3 * p + k] / wh;
+    return (1.0 - a) * (double)backdrop + a * fg;
+}
+
+__global__ __launch_bounds__(256) void k_film_relmse(size_t n, const double *rgb, const double *wts, float3 bd,
+                                                     const double *rrgb, const double *rwts, float3 rbd,
+                                                     double *partials) {
+    __shared__ double red[256];
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (p < n) {
+        const float b[3] = {bd.x, bd.y, bd.z}, rb[3] = {rbd.x, rbd.y, rbd.z};
+        for (int k = 0; k < 3; ++k) {
+            double x = composite_ch(rgb, wts, p, k, b[k]);
+            double r = composite_ch(rrgb, rwts, p, k, rb[k]);
+            v += (x - r) * (x - r) / (r * r + 0.01);
+        }
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
 __global__ void k_sobol_stream(int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample, int32_t seed,
@@ -781,13 +882,23 @@ struct Workspace {
     uint32_t ovf_grid = 0, ovf_cap = 0;
 };
 
+// Buffers of guided renders: per-path mixtures of the current wavefront buffer, per-work-item training
+// records, the extracted training samples and their counter.
+struct GuideWork {
+    DevBuf mix, recs, train, count;
+    uint32_t mix_q = 0, rec_items = 0;
+    uint32_t stamp = 0; // last record stamp handed out on this device
+};
+
 struct ks_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;
     int timing = 0;
     int n_cu = 0;
     Workspace ws;
+    GuideWork gw;
     Ctrl *host_ctrl = nullptr; // pinned
+    uint8_t *host_scratch = nullptr; // pinned, 64 bytes
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
 };
 
@@ -936,8 +1047,17 @@ int fill_consts(const ks_scene *sc, const ks_camera_desc *cam, const ks_small_pt
     return e;
 }
 
+// Guide state of one guided wavefront pass: the guide (nullptr: plain), its per-path mixture buffer and
+// the shading parameters.
+struct GuidePass {
+    ks_guide *guide = nullptr;
+    float4 *mix = nullptr;
+    GuideShade shade{};
+};
+
 // Runs the wavefront loop over `total` work items (mapped by rc), leaving per-sample results in ws.samp.
-int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Timer &tm, ks_render_stats *stats) {
+int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Timer &tm, ks_render_stats *stats,
+                  const GuidePass *gp = nullptr) {
     ks_device *dev = sc->dev;
     Workspace &ws = dev->ws;
     hipStream_t s = dev->stream;
@@ -964,16 +1084,28 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         hipLaunchKernelGGL(k_closest, dim3(tgrid), dim3(kTraceBlock), 0, s, sc->ds, &ctrl->n_active, 0u,
                            pb[cur].ro, pb[cur].rd, ws.hit.as<float4>(), ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
         tm.end(e);
+        if (gp && gp->shade.mix) {
+            e = tm.begin(KS_K_GUIDE);
+            int ge = ksg::launch_infer(gp->guide, sc->ds, &ctrl->n_active, Q, pb[cur].rd, ws.hit.as<float4>(), gp->mix,
+                                       s);
+            tm.end(e);
+            if (ge) return ge;
+        }
         e = tm.begin(KS_K_SHADE);
-        hipLaunchKernelGGL(k_shade, dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
-                           ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
-                           ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st);
+        if (gp)
+            hipLaunchKernelGGL(k_shade<true>, dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
+                               ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
+                               ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, gp->shade);
+        else
+            hipLaunchKernelGGL(k_shade<false>, dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
+                               ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
+                               ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, GuideShade{});
         tm.end(e);
         e = tm.begin(KS_K_SHADOW);
         hipLaunchKernelGGL(k_anyhit<1>, dim3(tgrid), dim3(kTraceBlock), 0, s, sc->ds, &ctrl->n_shadow, 0u,
                            ws.sh_o.as<float4>(), ws.sh_d.as<float4>(), (uint8_t *)nullptr, ws.sh_tgt.as<uint32_t>(),
-                           ws.sh_c.as<float4>(), pb[1 - cur].L, ws.samp.as<float4>(), ws.ovf.as<uint32_t>(),
-                           ws.ovf_cap, st);
+                           ws.sh_c.as<float4>(), pb[1 - cur].L, ws.samp.as<float4>(),
+                           gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
         hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
         tm.end(e);
         KS_HIP(hipGetLastError());
@@ -1069,6 +1201,7 @@ int ks_device_open(int32_t ordinal, ks_device **out) {
     KS_HIP(hipGetDeviceProperties(&prop, ordinal));
     d->n_cu = prop.multiProcessorCount;
     KS_HIP(hipHostMalloc((void **)&d->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    KS_HIP(hipHostMalloc((void **)&d->host_scratch, 64, hipHostMallocDefault));
     *out = d.release();
     return KS_OK;
 }
@@ -1078,6 +1211,7 @@ int ks_device_close(ks_device *dev) {
     (void)hipSetDevice(dev->ordinal);
     if (dev->stream) (void)hipStreamSynchronize(dev->stream);
     if (dev->host_ctrl) (void)hipHostFree(dev->host_ctrl);
+    if (dev->host_scratch) (void)hipHostFree(dev->host_scratch);
     for (hipEvent_t e : dev->events) (void)hipEventDestroy(e);
     if (dev->stream) (void)hipStreamDestroy(dev->stream);
     delete dev;
@@ -1427,6 +1561,157 @@ int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_smal
     return KS_OK;
 }
 
+int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse) {
+    KS_REQUIRE(film && reference && relmse, "null argument");
+    KS_REQUIRE(film->w == reference->w && film->h == reference->h, "films differ in size");
+    KS_REQUIRE(film->dev == reference->dev, "films belong to different devices");
+    ks_device *dev = film->dev;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    size_t n = (size_t)film->w * film->h;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    DevBuf part;
+    KS_HIP(part.alloc(sizeof(double) * blocks));
+    hipLaunchKernelGGL(k_film_relmse, dim3(blocks), dim3(256), 0, dev->stream, n, film->rgb.as<double>(),
+                       film->wts.as<double>(), make_float3(film->backdrop[0], film->backdrop[1], film->backdrop[2]),
+                       reference->rgb.as<double>(), reference->wts.as<double>(),
+                       make_float3(reference->backdrop[0], reference->backdrop[1], reference->backdrop[2]),
+                       part.as<double>());
+    KS_HIP(hipGetLastError());
+    std::vector<double> h(blocks);
+    KS_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, dev->stream));
+    KS_HIP(hipStreamSynchronize(dev->stream));
+    double sum = 0.0;
+    for (double v : h) sum += v;
+    *relmse = sum / (3.0 * (double)n);
+    return KS_OK;
+}
+
+int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params,
+                              ks_guide *guide, ks_film *film, ks_progress_fn progress, void *user,
+                              ks_render_stats *stats) {
+    KS_REQUIRE(scene && cam && params && film && guide, "null argument");
+    KS_REQUIRE(film->w == params->render_width && film->h == params->render_height,
+               "film size differs from render_width x render_height");
+    KS_REQUIRE(film->dev == scene->dev, "film and scene belong to different devices");
+    KS_REQUIRE(ksg::guide_device(guide) == scene->dev, "guide and scene belong to different devices");
+    KS_REQUIRE(params->mis_mode == KS_MIS_CONSISTENT,
+               "guided rendering needs mis_mode consistent (the reference weights do not sum to one)");
+    ks_device *dev = scene->dev;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    auto t0 = std::chrono::steady_clock::now();
+    RenderConsts rc;
+    int e = fill_consts(scene, cam, params, rc);
+    if (e) return e;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
+    const uint64_t pixels = (uint64_t)rc.cw * rc.ch;
+    KS_REQUIRE(pixels * kRecVertices < (1ull << 31), "crop window too large for guided rendering");
+    const uint64_t max_samp = (1ull << 30) / sizeof(float4);
+    const int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
+    const uint64_t max_items = pixels * (uint64_t)std::min(spp_chunk, params->spp);
+    uint32_t Q = pick_queue(params, max_items);
+    e = ensure_workspace(dev, Q, max_items, scene->info.max_depth);
+    if (e) return e;
+    GuideWork &gw = dev->gw;
+    if (gw.mix_q < Q) {
+        KS_HIP(gw.mix.alloc(sizeof(float4) * 10 * (size_t)Q));
+        gw.mix_q = Q;
+    }
+    const float fraction = ksg::guide_fraction(guide);
+    const int32_t max_steps = ksg::guide_max_steps(guide), batch = ksg::guide_batch(guide);
+    const int32_t train_spp = ksg::guide_train_spp(guide);
+    const bool any_training = max_steps > 0;
+    if (any_training && gw.rec_items < pixels) {
+        KS_HIP(gw.recs.alloc(sizeof(float4) * kRecFloat4 * kRecVertices * pixels));
+        KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
+        KS_HIP(gw.train.alloc(ksg::kTrainSampleBytes * kRecVertices * pixels));
+        gw.rec_items = (uint32_t)pixels;
+    }
+    KS_HIP(gw.count.ensure(2 * sizeof(double))); // uint32 count, then the double sum of the weights
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                       (int)kStatCount);
+    const uint32_t cap = (uint32_t)(kRecVertices * pixels);
+    Timer tm{dev};
+    int s = 0;
+    while (s < params->spp) {
+        // training passes hold one sample per pixel; frozen passes are batched like plain renders, without
+        // crossing a progress boundary
+        const bool train = any_training && (train_spp == 0 || s < train_spp);
+        int nb = train ? 1 : std::min(spp_chunk, params->spp - s);
+        nb = std::min(nb, (s / interval + 1) * interval - s);
+        rc.s0 = s;
+        rc.nb = nb;
+        rc.work_list = nullptr;
+        GuidePass gp;
+        gp.guide = guide;
+        gp.mix = gw.mix.as<float4>();
+        // the guide samples once it has been trained at least once; before that the pass is BSDF-only
+        const bool sample_guide = fraction > 0.0f && ksg::guide_steps(guide) > 0;
+        gp.shade.mix = sample_guide ? gw.mix.as<float4>() : nullptr;
+        gp.shade.recs = train ? gw.recs.as<float4>() : nullptr;
+        gp.shade.alpha = fraction;
+        gp.shade.min_roughness = kGuideMinRoughness;
+        if (gw.stamp == 0xffffffffu) { // stamps wrap: clear the records so no stale stamp can match
+            KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
+            gw.stamp = 0;
+        }
+        gp.shade.stamp = ++gw.stamp;
+        e = run_wavefront(scene, rc, pixels * (uint64_t)nb, Q, tm, stats, &gp);
+        if (e) return e;
+        hipEvent_t ev = tm.begin(KS_K_FILM);
+        hipLaunchKernelGGL(k_film, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, dev->stream, rc,
+                           dev->ws.samp.as<float4>(), film->rgb.as<double>(), film->wts.as<double>());
+        tm.end(ev);
+        KS_HIP(hipGetLastError());
+        if (train) {
+            ev = tm.begin(KS_K_GUIDE);
+            KS_HIP(hipMemsetAsync(gw.count.p, 0, 2 * sizeof(double), dev->stream));
+            e = ksg::launch_extract(gw.recs.as<float4>(), dev->ws.samp.as<float4>(), (uint32_t)pixels,
+                                    gp.shade.stamp, gw.train.p, gw.count.as<uint32_t>(), gw.count.as<double>() + 1,
+                                    cap, dev->stream);
+            tm.end(ev);
+            if (e) return e;
+            KS_HIP(hipMemcpyAsync(dev->host_scratch, gw.count.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                  dev->stream));
+            KS_HIP(hipStreamSynchronize(dev->stream));
+            uint32_t n_found;
+            double c_total;
+            std::memcpy(&n_found, dev->host_scratch, sizeof(uint32_t));
+            std::memcpy(&c_total, dev->host_scratch + sizeof(double), sizeof(double));
+            const uint32_t n_train = std::min(n_found, cap);
+            const float c_scale = c_total > 0.0 ? (float)((double)n_found / c_total) : 1.0f;
+            // spread the samples of the pass over the steps: step k takes samples k, k + S, k + 2S, ...
+            const int steps = (int)std::min<uint64_t>((uint64_t)max_steps, n_train / 64u);
+            if (steps > 0) {
+                const int per_step = (int)std::min<uint64_t>((uint64_t)batch, n_train / (uint32_t)steps);
+                ev = tm.begin(KS_K_GUIDE);
+                for (int k = 0; k < steps; ++k) {
+                    e = ksg::train_on_device(guide, static_cast<uint8_t *>(gw.train.p) + ksg::kTrainSampleBytes * k,
+                                             per_step, steps, c_scale, nullptr, dev->stream);
+                    if (e) return e;
+                }
+                tm.end(ev);
+                if (stats) stats->guide_samples += (uint64_t)per_step * steps;
+            }
+        }
+        s += nb;
+        if (progress && (s % interval == 0 || s == params->spp)) {
+            KS_HIP(hipStreamSynchronize(dev->stream));
+            progress(film, s, user);
+        }
+    }
+    KS_HIP(hipStreamSynchronize(dev->stream));
+    tm.collect(stats);
+    e = read_stats(dev, stats);
+    if (e) return e;
+    if (stats) {
+        stats->paths = pixels * (uint64_t)params->spp;
+        stats->queue_size = (int32_t)Q;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return KS_OK;
+}
+
 int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
                      const int32_t *px, const int32_t *py, const int32_t *sample, float *out) {
     KS_REQUIRE(scene && cam && params && px && py && sample && out, "null argument");
@@ -1559,7 +1844,8 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
         KS_HIP(out.alloc(n));
         hipLaunchKernelGGL(k_anyhit<0>, dim3(grid), dim3(kTraceBlock), 0, s, scene->ds, (const uint32_t *)nullptr,
                            (uint32_t)n, dO.p(), dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
-                           (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, dev->ws.ovf.as<uint32_t>(),
+                           (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr,
+                           dev->ws.ovf.as<uint32_t>(),
                            dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(occ, out.p, n, hipMemcpyDeviceToHost, s));

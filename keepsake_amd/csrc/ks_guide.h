@@ -16,6 +16,7 @@ constexpr int kGuideHidden = 64;
 constexpr int kGuideOut = 48;    // 8 x (logit, mean xyz, log kappa) = 40, padded to 48
 constexpr float kMaxKappa = 1e4f; // VMF::max_kappa (vmf.h:110)
 constexpr float kMinKappa = 1e-2f;
+constexpr float kGuideMinRoughness = 0.05f; // sharper materials sample the BSDF only
 constexpr int kWaveSize = 64;
 
 // Network parameter layout (FP32 master copy): [grid: L * T * F][W1: 64x64][W2: 64x64][W3: 64x48].

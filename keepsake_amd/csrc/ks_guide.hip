@@ -185,8 +185,9 @@ struct TrainSample {
 };
 static_assert(sizeof(TrainSample) == ksg::kTrainSampleBytes, "training sample layout");
 
-__global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, const TrainSample *samples, float *grid_grad,
-                                                    float *partials, double *loss_out) {
+__global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, int stride, const TrainSample *samples,
+                                                    float c_scale, float c_max, float *grid_grad, float *partials,
+                                                    double *loss_out) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const int r16 = lane & 15, q4 = (lane >> 4) * 4;
@@ -206,13 +207,16 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, const T
 #pragma unroll
         for (int b = 0; b < 4; ++b) g2[a][b] = g1[a][b] = f32x4{0, 0, 0, 0};
     }
-    double loss = 0.0;
+    double loss = 0.0, wsum = 0.0, loss_cos = 0.0;
     const int n_chunks = (n + 63) / 64;
     for (int chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
         int i = chunk * 64 + lane;
         bool valid = i < n;
         TrainSample s{};
-        if (valid) s = samples[i];
+        if (valid) {
+            s = samples[(size_t)i * stride];
+            s.xc.w = fminf(s.xc.w * c_scale, c_max);
+        }
         if (valid) guide_encode(net, ld3(s.xc), ld3(s.n), ld3(s.wo), X + lane * kLdx);
         else guide_zero_row(X + lane * kLdx);
         __syncthreads();
@@ -227,7 +231,11 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, const T
 #pragma unroll
         for (int k = 0; k < kGuideOut; ++k) o[k] = O[lane * kGuideOut + k];
         if (valid && s.xc.w > 0.0f) {
-            loss += (double)head_backward(o, normalized(ld3(s.wi)), s.xc.w, g);
+            f3 wi = normalized(ld3(s.wi));
+            loss += (double)head_backward(o, wi, s.xc.w, g);
+            wsum += (double)s.xc.w;
+            float cs = fmaxf(dot(wi, ld3(s.n)), 1e-4f);
+            loss_cos -= (double)s.xc.w * (double)logf(cs * kInvPi);
         } else {
 #pragma unroll
             for (int k = 0; k < kGuideOut; ++k) g[k] = 0.0f;
@@ -365,8 +373,16 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, const T
                 dst[kMlpW1 + row * 64 + col] = g2[m][nn][r];
                 if (nn < 3) dst[kMlpW1 + kMlpW2 + row * kGuideOut + col] = g3[m][nn][r];
             }
-    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-    if (lane == 0) atomicAdd(loss_out, loss);
+    for (int off = 32; off > 0; off >>= 1) {
+        loss += __shfl_xor(loss, off);
+        wsum += __shfl_xor(wsum, off);
+        loss_cos += __shfl_xor(loss_cos, off);
+    }
+    if (lane == 0) { // sum c (-log q), sum c, sum c (-log(cos / pi)) of the step
+        atomicAdd(loss_out, loss);
+        atomicAdd(loss_out + 1, wsum);
+        atomicAdd(loss_out + 2, loss_cos);
+    }
 }
 
 __global__ void k_guide_reduce(const float *partials, int n_waves, float *grad_mlp) {
@@ -494,6 +510,44 @@ __global__ void k_vmm_sample(int64_t n, const float *mix, const float *u, float 
     pdf[i] = mixture_pdf(m, w);
 }
 
+// One thread per record slot: work item w = slot / kRecVertices. The final sample value of a pass with
+// one sample per pixel sits at samp[w].
+__global__ void k_guide_extract(const float4 *recs, const float4 *samp, uint32_t n_items, uint32_t stamp,
+                                TrainSample *out, uint32_t *count, double *c_sum, uint32_t cap) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    TrainSample t{};
+    if (slot < n_items * (uint32_t)kRecVertices) {
+        const float4 *r = recs + (size_t)slot * kRecFloat4;
+        float4 x = r[0];
+        if (__float_as_uint(x.w) == stamp) {
+            f3 Lf = ld3(samp[slot / kRecVertices]);
+            f3 d = Lf - ld3(r[4]) - ld3(r[kRecNee]);
+            f3 b = ld3(r[5]);
+            f3 X = mk(b.x > 0.0f ? d.x / b.x : 0.0f, b.y > 0.0f ? d.y / b.y : 0.0f, b.z > 0.0f ? d.z / b.z : 0.0f);
+            float c = luminance(X);
+            if (c > 0.0f && isfinite(c)) {
+                take = true;
+                t.xc = make_float4(x.x, x.y, x.z, c);
+                t.n = r[1];
+                t.wo = r[2];
+                t.wi = r[3];
+            }
+        }
+    }
+    double cs = take ? (double)t.xc.w : 0.0;
+    for (int off = 32; off > 0; off >>= 1) cs += __shfl_xor(cs, off);
+    if (__lane_id() == 0 && cs != 0.0) atomicAdd(c_sum, cs);
+    uint32_t j = wave_append(count, take);
+    if (take && j < cap) {
+        TrainSample *o = out + j;
+        o->xc = t.xc;
+        o->n = t.n;
+        o->wo = t.wo;
+        o->wi = t.wi;
+    }
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -530,6 +584,7 @@ struct ks_guide {
     DevBuf params, grad, m1, m2, grid_h, w_bf, wt_bf, partials, loss, samples;
     int32_t n_partials = 0;
     int64_t steps = 0;
+    int64_t samples_used = 0; // training samples consumed by all steps
 };
 
 namespace ksg {
@@ -553,9 +608,11 @@ GuideNet net_of(const ks_guide *g) {
     return n;
 }
 
+ks_device *guide_device(const ks_guide *g) { return g->dev; }
 float guide_fraction(const ks_guide *g) { return g->cfg.guide_fraction; }
 int32_t guide_batch(const ks_guide *g) { return g->cfg.train_batch; }
 int32_t guide_max_steps(const ks_guide *g) { return g->cfg.max_steps_per_pass; }
+int32_t guide_train_spp(const ks_guide *g) { return g->cfg.train_spp; }
 
 int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
                  const float4 *hits, float4 *mix, hipStream_t s) {
@@ -566,20 +623,34 @@ int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_
     return KS_OK;
 }
 
-// One Adam step on n device-resident samples (TrainSample records).
-int train_on_device(ks_guide *g, const void *samples, int n, double *loss_host, hipStream_t s) {
+int64_t guide_steps(const ks_guide *g) { return g->steps; }
+
+int launch_extract(const float4 *recs, const float4 *samp, uint32_t n_items, uint32_t stamp, void *out,
+                   uint32_t *count, double *c_sum, uint32_t cap, hipStream_t s) {
+    uint64_t slots = (uint64_t)n_items * kRecVertices;
+    hipLaunchKernelGGL(k_guide_extract, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, recs, samp, n_items,
+                       stamp, static_cast<TrainSample *>(out), count, c_sum, cap);
+    KS_HIP(hipGetLastError());
+    return KS_OK;
+}
+
+// One Adam step on n device-resident samples (TrainSample records, every stride-th one).
+int train_on_device(ks_guide *g, const void *samples, int n, int stride, float c_scale, double *loss_host,
+                    hipStream_t s) {
     if (n <= 0) return KS_OK;
     int n_chunks = (n + 63) / 64;
     int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
     KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
-    KS_HIP(hipMemsetAsync(g->loss.p, 0, sizeof(double), s));
-    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, net_of(g), n,
-                       static_cast<const TrainSample *>(samples), g->grad.as<float>(), g->partials.as<float>(),
-                       g->loss.as<double>());
+    KS_HIP(hipMemsetAsync(g->loss.p, 0, 3 * sizeof(double), s));
+    const float c_max = c_scale > 0.0f && g->cfg.weight_clamp > 0.0f ? g->cfg.weight_clamp : INFINITY;
+    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, net_of(g), n, stride,
+                       static_cast<const TrainSample *>(samples), c_scale > 0.0f ? c_scale : 1.0f, c_max,
+                       g->grad.as<float>(), g->partials.as<float>(), g->loss.as<double>());
     KS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
                        g->grad.as<float>() + g->n_grid);
     g->steps += 1;
+    g->samples_used += n;
     float bc1 = 1.0f - std::pow(g->cfg.beta1, (float)g->steps), bc2 = 1.0f - std::pow(g->cfg.beta2, (float)g->steps);
     int64_t work = g->n_grid / 2 + kMlpParams;
     hipLaunchKernelGGL(k_guide_adam, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, s, g->params.as<float>(),
@@ -620,9 +691,11 @@ int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float b
     if (c.guide_fraction == 0.0f) c.guide_fraction = 0.5f;
     if (c.train_batch == 0) c.train_batch = 1 << 18;
     if (c.max_steps_per_pass == 0) c.max_steps_per_pass = 4;
+    if (c.weight_clamp == 0.0f) c.weight_clamp = 20.0f;
     KS_REQUIRE(c.log2_table_size >= 10 && c.log2_table_size <= 24, "log2_table_size must be in [10, 24]");
     KS_REQUIRE(c.max_resolution >= c.base_resolution && c.base_resolution >= 2.0f, "bad grid resolutions");
     KS_REQUIRE(c.guide_fraction >= 0.0f && c.guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
+    KS_REQUIRE(c.train_spp >= 0, "train_spp must be non-negative");
     g->cfg = c;
     g->log2_T = (uint32_t)c.log2_table_size;
     // resolutions: floor(base * b^l), b = (max / base)^(1 / (L - 1)), in FP64
@@ -646,7 +719,7 @@ int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float b
     KS_HIP(g->grid_h.alloc(sizeof(uint32_t) * g->n_grid / 2));
     KS_HIP(g->w_bf.alloc(sizeof(uint16_t) * kMlpParams));
     KS_HIP(g->wt_bf.alloc(sizeof(uint16_t) * kMlpParams));
-    KS_HIP(g->loss.alloc(sizeof(double)));
+    KS_HIP(g->loss.alloc(3 * sizeof(double)));
     hipLaunchKernelGGL(k_guide_init, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, g->stream,
                        g->params.as<float>(), g->m1.as<float>(), g->m2.as<float>(), g->grad.as<float>(), g->n_grid,
                        (uint32_t)(c.seed ^ (c.seed >> 32)));
@@ -665,6 +738,29 @@ int ks_guide_free(ks_guide *g) {
         (void)hipStreamSynchronize(g->stream);
         delete g;
     }
+    return KS_OK;
+}
+
+int ks_guide_set_mode(ks_guide *g, float guide_fraction, int32_t max_steps_per_pass) {
+    KS_REQUIRE(g, "guide is null");
+    KS_REQUIRE(guide_fraction >= 0.0f && guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
+    g->cfg.guide_fraction = guide_fraction;
+    g->cfg.max_steps_per_pass = max_steps_per_pass;
+    return KS_OK;
+}
+
+int ks_guide_get_stats(const ks_guide *g, ks_guide_stats *out) {
+    KS_REQUIRE(g && out, "null argument");
+    double acc[3] = {0, 0, 0};
+    if (g->steps > 0) {
+        KS_HIP(hipMemcpyAsync(acc, g->loss.p, sizeof(acc), hipMemcpyDeviceToHost, g->stream));
+        KS_HIP(hipStreamSynchronize(g->stream));
+    }
+    out->steps = g->steps;
+    out->samples = g->samples_used;
+    out->last_loss = acc[0];
+    out->last_weight = acc[1];
+    out->last_loss_cosine = acc[2];
     return KS_OK;
 }
 
@@ -759,10 +855,11 @@ int ks_guide_gradient(ks_guide *g, int64_t n, const float *queries, const float 
     int n_chunks = (int)((n + 63) / 64);
     int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
     KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
-    KS_HIP(hipMemsetAsync(g->loss.p, 0, sizeof(double), s));
+    KS_HIP(hipMemsetAsync(g->loss.p, 0, 3 * sizeof(double), s));
     KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * g->n_grid, s));
-    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, ksg::net_of(g), (int)n,
-                       g->samples.as<TrainSample>(), g->grad.as<float>(), g->partials.as<float>(), g->loss.as<double>());
+    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, ksg::net_of(g), (int)n, 1,
+                       g->samples.as<TrainSample>(), 1.0f, INFINITY, g->grad.as<float>(), g->partials.as<float>(),
+                       g->loss.as<double>());
     KS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
                        g->grad.as<float>() + g->n_grid);
@@ -779,7 +876,7 @@ int ks_guide_train_step(ks_guide *g, int64_t n, const float *queries, const floa
     KS_REQUIRE(g && queries && wi && weight && n > 0 && n < (1ll << 30), "invalid argument");
     int e = upload_samples(g, n, queries, wi, weight);
     if (e) return e;
-    return ksg::train_on_device(g, g->samples.p, (int)n, loss, g->stream);
+    return ksg::train_on_device(g, g->samples.p, (int)n, 1, 0.0f, loss, g->stream);
 }
 
 int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *dirs, float *pdf) {

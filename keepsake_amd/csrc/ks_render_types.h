@@ -60,6 +60,21 @@ struct PathBufs {
     float4 *rd;      // direction, tmax
 };
 
+// Guided shading inputs. Mixtures: 10 float4 per path slot of the current buffer, [w0..w3], [w4..w7],
+// then (mu, kappa) per lobe. Training records: kRecVertices per work item, kRecFloat4 float4 each:
+// (x, stamp bits), (n, 0), (wo, 0), (wi, 0), (L before the vertex's NEE, 0), (beta before scattering, 0),
+// (NEE added by the shadow ray, 0), unused.
+constexpr int kRecVertices = 3;
+constexpr int kRecFloat4 = 8;
+constexpr int kRecNee = 6;
+struct GuideShade {
+    const float4 *mix; // nullptr: no guide inference this pass (BSDF sampling only)
+    float4 *recs;      // nullptr: no training records
+    float alpha;       // probability of sampling the guide at a guidable vertex
+    float min_roughness; // materials below this roughness use the BSDF only
+    uint32_t stamp;
+};
+
 // ---------------------------------------------------------------------------------------------------
 // Wave-level helpers (wave64)
 // ---------------------------------------------------------------------------------------------------
