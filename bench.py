@@ -3,8 +3,9 @@
 
 Workload (one step = one full render): the 64-icosphere scene (1,310,728 triangles, seed 1), 1920x1080,
 64 spp, 12 bounces, plain BSDF + NEE sampling, FP32. Prints one JSON line with Mpaths/s, the roofline
-of the dominant kernel (closest-hit traversal) measured with HIP events on the render stream, and the
-CPU oracle timed on a bounded crop of the same workload on this host's cores.
+of the dominant kernel (closest-hit traversal) measured with HIP events on the render stream, the CPU
+oracle timed on a bounded band of the same workload on this host's cores, and the wall time the plain
+and the guided tracer need to reach relMSE 0.01 on the slit-lit room against an FP64 reference.
 """
 
 from __future__ import annotations
@@ -36,18 +37,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(hs, cam, kw, rows: int, threads: int) -> dict:
+def cpu_baseline(hs, cam, kw, rows: int, threads: int, budget_s: float) -> dict:
+    """The CPU oracle on a band of rows of the same frame; rows = 0 sizes the band from a short
+    calibration run so that the timed render takes about budget_s."""
     import ks_oracle as ko  # CPU oracle: the reported baseline, never the measured GPU path
 
     t0 = time.time()
     osc = ko.OracleScene.from_host_scene(hs)
     build_s = time.time() - t0
     W, H = kw["render_width"], kw["render_height"]
-    y0 = H // 2 - rows // 2
-    p = ko.params(**kw, crop_start_x=0, crop_start_y=y0, crop_width=W, crop_height=rows)
-    t0 = time.time()
-    osc.render(cam, p, threads=threads)
-    dt = time.time() - t0
+
+    def band(n):
+        y0 = H // 2 - n // 2
+        p = ko.params(**kw, crop_start_x=0, crop_start_y=y0, crop_width=W, crop_height=n)
+        t = time.time()
+        osc.render(cam, p, threads=threads)
+        return y0, time.time() - t
+
+    if rows <= 0:
+        _, t_cal = band(16)
+        rows = int(min(H, max(16, 16 * budget_s / max(t_cal, 1e-3))))
+    y0, dt = band(rows)
     paths = W * rows * kw["spp"]
     osc.close()
     return {
@@ -58,6 +68,67 @@ def cpu_baseline(hs, cam, kw, rows: int, threads: int) -> dict:
         "sample": f"rows {y0}..{y0 + rows - 1} of the {W}x{H} frame at {kw['spp']} spp ({paths} paths), "
         f"{dt:.1f} s render, oracle BVH build {build_s:.1f} s",
     }
+
+
+def time_to_relmse(ks, dev, args) -> dict:
+    """Wall time for the plain and the guided tracer to bring the room (1280x720, 16 bounces,
+    consistent MIS, no clamping) to relMSE <= target against an FP64 plain reference of ref_spp."""
+    hs = ks.procedural_scene("room", 3)
+    sc = ks.Scene(dev, hs)
+    info = sc.info()
+    cam = hs.camera
+    W, H = args.room_width, args.room_height
+
+    def params(spp, interval):
+        return ks.small_pt_params(bounces=16, render_width=W, render_height=H, spp=spp, spp_prog_interval=interval,
+                                  mis_mode=ks.KS_MIS_CONSISTENT, clamp_direct=0.0, clamp_indirect=0.0)
+
+    ref = ks.Film(dev, W, H)
+    t0 = time.perf_counter()
+    st = ks.small_pt_render(sc, cam, params(args.ref_spp, 256), ref)
+    ref_s = time.perf_counter() - t0
+    log(f"room reference: {args.ref_spp} spp in {ref_s:.1f} s ({st['paths'] / ref_s / 1e6:.0f} Mpaths/s)")
+    target = args.relmse_target
+    res = {"scene": f"room ({hs.n_tris} tris), {W}x{H}, 16 bounces, consistent MIS, no clamping",
+           "target": target, "reference_spp": args.ref_spp, "reference_s": round(ref_s, 1)}
+
+    # one progressive render per method; the progress callback measures relMSE every pass_spp samples
+    film = ks.Film(dev, W, H)
+    passes = []
+
+    def on_progress(f, spp):
+        passes.append((spp, time.perf_counter() - t_start, f.relmse(ref)))
+
+    t_start = time.perf_counter()
+    ks.small_pt_render(sc, cam, params(args.room_max_spp, args.plain_pass_spp), film, progress=on_progress)
+    hit = next(((s, t, r) for s, t, r in passes if r <= target), None)
+    s_last, _, r_last = passes[-1]
+    res["plain"] = {"pass_spp": args.plain_pass_spp, "seconds": round(hit[1], 3) if hit else None,
+                    "spp": hit[0] if hit else None, "relmse_at_max_spp": r_last, "max_spp": s_last}
+    log(f"plain: relMSE {target} at {hit}")
+
+    cfg = ks.guide_config(seed=1, guide_fraction=args.guide_fraction, train_spp=args.train_spp)
+    guide = ks.Guide(dev, info["bound_lo"], info["bound_hi"], cfg)
+    film.clear()
+    passes.clear()
+    t_start = time.perf_counter()
+    st = ks.small_pt_render_guided(sc, cam, params(args.room_max_spp, args.plain_pass_spp), guide, film,
+                                   progress=on_progress)
+    hit_g = next(((s, t, r) for s, t, r in passes if r <= target), None)
+    gst = guide.stats()
+    res["guided"] = {"guide_fraction": args.guide_fraction, "train_spp": args.train_spp,
+                     "seconds": round(hit_g[1], 3) if hit_g else None, "spp": hit_g[0] if hit_g else None,
+                     "relmse_at_max_spp": passes[-1][2], "max_spp": passes[-1][0], "adam_steps": gst["steps"],
+                     "training_samples": gst["samples"],
+                     "kernel_ms": {k: round(v, 1) for k, v in st["kernel_ms"].items()}}
+    log(f"guided: relMSE {target} at {hit_g}")
+    if hit and hit_g:
+        res["speedup_guided_vs_plain"] = round(hit[1] / hit_g[1], 3)
+    guide.close()
+    film.close()
+    ref.close()
+    sc.close()
+    return res
 
 
 def main() -> int:
@@ -71,10 +142,35 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--bounces", type=int, default=12)
     ap.add_argument("--queue", type=int, default=0)
-    ap.add_argument("--cpu-rows", type=int, default=96)
+    ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-relmse", action="store_true", help="skip the guided-vs-plain time-to-relMSE leg")
+    ap.add_argument("--room-width", type=int, default=1280)
+    ap.add_argument("--room-height", type=int, default=720)
+    ap.add_argument("--ref-spp", type=int, default=16384)
+    ap.add_argument("--room-max-spp", type=int, default=2048)
+    ap.add_argument("--pass-spp", dest="plain_pass_spp", type=int, default=8,
+                    help="samples per pixel between relMSE checks (plain and guided)")
+    ap.add_argument("--relmse-target", type=float, default=0.01)
+    ap.add_argument("--guide-fraction", type=float, default=0.3)
+    ap.add_argument("--train-spp", type=int, default=64)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration_mib > 0:
+        import torch
+
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_mib": args.copy_calibration_mib, "read_bytes": 4 * n, "write_bytes": 4 * n}))
+        return 0
 
     if args.gpus != 1:
         print(
@@ -162,7 +258,14 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(hs, cam, kw, args.cpu_rows, threads)
+        cpu = cpu_baseline(hs, cam, kw, args.cpu_rows, threads, args.cpu_seconds)
+
+    ttr = None
+    if not args.no_relmse:
+        try:
+            ttr = time_to_relmse(ks, dev, args)
+        except ks.KsError as e:  # reported, not hidden: the plain-throughput line stays valid
+            ttr = {"error": str(e)}
 
     n_rays = (totals["rays_closest"] + totals["rays_shadow"]) / args.steps
     out = {
@@ -191,6 +294,7 @@ def main() -> int:
         "kernel_ms_per_step": {k: round(v / args.steps, 2) for k, v in totals["kernel_ms"].items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "time_to_relmse": ttr,
     }
     print(json.dumps(out), flush=True)
     film.close()

@@ -40,8 +40,6 @@ namespace {
 constexpr int kLdx = 68;                     // LDS row stride (bf16 elements) of 64-wide activations
 constexpr int kActBytes = 64 * kLdx * 2;     // 8704
 constexpr int kOutBytes = 64 * kGuideOut * 4; // 12288
-constexpr int kFwdWaveBytes = kActBytes + kOutBytes;
-constexpr int kFwdWaves = 2;                 // waves per forward block
 constexpr int kTrainWaveBytes = 4 * kActBytes + kOutBytes;
 constexpr int kMlpW1 = 64 * 64, kMlpW2 = 64 * 64, kMlpW3 = 64 * kGuideOut;
 constexpr int kMlpParams = kMlpW1 + kMlpW2 + kMlpW3;
@@ -94,83 +92,175 @@ __device__ __forceinline__ void wave_dense(const uint16_t *in, const uint16_t *_
     }
 }
 
-// Forward of 64 queries per wave through the MLP; the raw outputs end in O (FP32, row stride 48).
-__device__ __forceinline__ void wave_forward(const GuideNet &net, uint16_t *X, uint16_t *H, float *O) {
-    wave_dense<4, true>(X, net.wt[0], H, nullptr, 0);
-    __syncthreads();
-    wave_dense<4, true>(H, net.wt[1], X, nullptr, 0);
-    __syncthreads();
-    wave_dense<3, false>(X, net.wt[2], nullptr, O, kGuideOut);
-    __syncthreads();
-}
-
-__device__ __forceinline__ void store_mixture(const Mixture &m, float4 *dst) {
-    dst[0] = make_float4(m.w[0], m.w[1], m.w[2], m.w[3]);
-    dst[1] = make_float4(m.w[4], m.w[5], m.w[6], m.w[7]);
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k)
-        dst[2 + k] = make_float4(m.lobe[k].mu.x, m.lobe[k].mu.y, m.lobe[k].mu.z, m.lobe[k].kappa);
-}
-
-__global__ __launch_bounds__(64 * kFwdWaves) void k_guide_forward(GuideNet net, int n, const float *q, float4 *mix) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint8_t *base = smem + wave * kFwdWaveBytes;
-    uint16_t *X = reinterpret_cast<uint16_t *>(base);
-    uint16_t *H = reinterpret_cast<uint16_t *>(base + kActBytes);
-    float *O = reinterpret_cast<float *>(base + kActBytes);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+// ---------------------------------------------------------------------------------------------------
+// Inference with the activations in registers. Each layer is computed transposed,
+// out[unit][query] = sum_in W[in][unit] x[in][query]: the MFMA result tile (lane l, register r: unit
+// 4 (l / 16) + r, query l % 16) is then exactly the B operand of the next layer for the k-block of those
+// units, so no activation goes through LDS. A wave handles 64 queries as four column tiles of 16; lane
+// group g = l / 16 encodes the hash levels 2g, 2g+1, 8+2g, 9+2g of query l % 16 (the B operand of
+// k-blocks 0 and 1), groups 0 and 1 supply normal, wo and the constant 1 (k-block 2); k-block 3 holds
+// only zero padding and is skipped. The head regroups each query's 48 outputs through 3 KB of LDS per
+// wave; lane group g then evaluates lobes 2g and 2g+1.
+// ---------------------------------------------------------------------------------------------------
+struct QueryArray { // queries n x 9: position, normal, wo
+    const float *q;
+    uint32_t n;
+    __device__ uint32_t count() const { return n; }
+    __device__ bool load(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
+        if (i >= n) return false;
         const float *e = q + 9 * (size_t)i;
-        guide_encode(net, mk(e[0], e[1], e[2]), mk(e[3], e[4], e[5]), mk(e[6], e[7], e[8]), X + lane * kLdx);
-    } else {
-        guide_zero_row(X + lane * kLdx);
+        p = mk(e[0], e[1], e[2]);
+        nn = mk(e[3], e[4], e[5]);
+        wo = mk(e[6], e[7], e[8]);
+        return true;
     }
-    __syncthreads();
-    wave_forward(net, X, H, O);
-    if (i < n) {
-        float o[kGuideOut];
-#pragma unroll
-        for (int k = 0; k < kGuideOut; ++k) o[k] = O[lane * kGuideOut + k];
-        Mixture m;
-        head_forward(o, m);
-        store_mixture(m, mix + 10 * (size_t)i);
-    }
-}
+};
 
-// Guide queries straight from the wavefront: shading position, geometric normal facing wo, wo.
-__global__ __launch_bounds__(64 * kFwdWaves) void k_guide_infer(GuideNet net, DevScene sc, const uint32_t *n_dev,
-                                                                const float4 *rd, const float4 *hits, float4 *mix) {
-    extern __shared__ __align__(16) uint8_t smem[];
-    const uint32_t n = *n_dev;
-    if (blockIdx.x * blockDim.x >= n) return;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint8_t *base = smem + wave * kFwdWaveBytes;
-    uint16_t *X = reinterpret_cast<uint16_t *>(base);
-    uint16_t *H = reinterpret_cast<uint16_t *>(base + kActBytes);
-    float *O = reinterpret_cast<float *>(base + kActBytes);
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = false;
-    if (i < n) {
+struct QueryHits { // wavefront paths with a surface hit: position, geometric normal facing wo, wo
+    DevScene sc;
+    const uint32_t *n_dev;
+    const float4 *rd;
+    const float4 *hits;
+    __device__ uint32_t count() const { return *n_dev; }
+    __device__ bool load(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
+        if (i >= *n_dev) return false;
         float4 h = hits[i];
         int32_t prim = __float_as_int(h.w);
-        if (prim >= 0) {
-            valid = true;
-            uint4 tri = sc.tri_idx[prim];
-            SurfHit it = make_hit(sc, tri, h.y, h.z, ld3(rd[i]));
-            guide_encode(net, it.p, it.n, -normalized(ld3(rd[i])), X + lane * kLdx);
-        }
+        if (prim < 0) return false;
+        f3 d = ld3(rd[i]);
+        SurfHit it = make_hit(sc, sc.tri_idx[prim], h.y, h.z, d);
+        p = it.p;
+        nn = it.n;
+        wo = -normalized(d);
+        return true;
     }
-    if (!valid) guide_zero_row(X + lane * kLdx);
-    __syncthreads();
-    wave_forward(net, X, H, O);
-    if (valid) {
-        float o[kGuideOut];
+};
+
+// Both features of one hash level: FP32 trilinear sum over corners 0..7.
+__device__ __forceinline__ void grid_pair(const GuideNet &net, int level, const float u[3], float &f0, float &f1) {
+    GridCorner c;
+    grid_level_corners(net, level, u, c);
+    f0 = 0.0f;
+    f1 = 0.0f;
 #pragma unroll
-        for (int k = 0; k < kGuideOut; ++k) o[k] = O[lane * kGuideOut + k];
-        Mixture m;
-        head_forward(o, m);
-        store_mixture(m, mix + 10 * (size_t)i);
+    for (int k = 0; k < 8; ++k) {
+        uint32_t v = net.grid_h[c.idx[k]];
+        f0 += c.w[k] * half_lo(v);
+        f1 += c.w[k] * half_hi(v);
+    }
+}
+
+__device__ __forceinline__ bf16x4 pack_bf(float a, float b, float c, float d) {
+    bf16x4 v;
+    v.x = (short)f2bf(a);
+    v.y = (short)f2bf(b);
+    v.z = (short)f2bf(c);
+    v.w = (short)f2bf(d);
+    return v;
+}
+__device__ __forceinline__ bf16x4 relu_bf(f32x4 d) {
+    return pack_bf(fmaxf(d[0], 0.0f), fmaxf(d[1], 0.0f), fmaxf(d[2], 0.0f), fmaxf(d[3], 0.0f));
+}
+// A operand of W^T (rows = units m*16.., k-block kk) from the [out][in] BF16 copy
+__device__ __forceinline__ bf16x4 wt_frag(const uint16_t *__restrict__ wt, int m, int kk, int q, int g) {
+    return glb_a(wt + (m * 16 + q) * 64 + kk * 16 + 4 * g);
+}
+
+constexpr int kInferBlock = 256;
+
+template <class Src>
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src src, float4 *mix) {
+    __shared__ float s_out[kInferBlock / 64][16 * kGuideOut];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, q = lane & 15;
+    const uint32_t n = src.count();
+    if (blockIdx.x * blockDim.x >= n) return; // whole block idle
+    const uint32_t base = blockIdx.x * blockDim.x + wave * 64;
+    f3 P = mk(0, 0, 0), Nn = mk(0, 0, 0), WO = mk(0, 0, 0);
+    const bool valid = src.load(base + lane, P, Nn, WO);
+    float u[3];
+    normalise_position(net, P, u);
+    float *so = s_out[wave];
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+        const int sl = 16 * t + q; // lane that fetched this tile's query
+        const bool vq = __shfl((int)valid, sl) != 0;
+        float uq[3] = {__shfl(u[0], sl), __shfl(u[1], sl), __shfl(u[2], sl)};
+        const float nx = __shfl(Nn.x, sl), ny = __shfl(Nn.y, sl), nz = __shfl(Nn.z, sl);
+        const float ox = __shfl(WO.x, sl), oy = __shfl(WO.y, sl), oz = __shfl(WO.z, sl);
+        bf16x4 b[3];
+        {
+            float f[8];
+            grid_pair(net, 2 * g, uq, f[0], f[1]);
+            grid_pair(net, 2 * g + 1, uq, f[2], f[3]);
+            grid_pair(net, 8 + 2 * g, uq, f[4], f[5]);
+            grid_pair(net, 9 + 2 * g, uq, f[6], f[7]);
+            b[0] = pack_bf(f[0], f[1], f[2], f[3]);
+            b[1] = pack_bf(f[4], f[5], f[6], f[7]);
+            b[2] = g == 0 ? pack_bf(nx, ny, nz, ox) : (g == 1 ? pack_bf(oy, oz, 1.0f, 0.0f) : pack_bf(0, 0, 0, 0));
+            if (!vq) b[0] = b[1] = b[2] = pack_bf(0, 0, 0, 0);
+        }
+        bf16x4 h[4];
+        {
+            f32x4 d[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                d[m] = f32x4{0, 0, 0, 0};
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) d[m] = mfma16(wt_frag(net.wt[0], m, kk, q, g), b[kk], d[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
+        }
+        {
+            f32x4 d[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                d[m] = f32x4{0, 0, 0, 0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(net.wt[1], m, kk, q, g), h[kk], d[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
+        }
+        {
+            f32x4 d[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                d[m] = f32x4{0, 0, 0, 0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(net.wt[2], m, kk, q, g), h[kk], d[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) so[q * kGuideOut + 16 * m + 4 * g + r] = d[m][r];
+        }
+        __syncthreads();
+        // head (vmf.h lobes): lane group g evaluates lobes 2g and 2g+1 of query q
+        float o[10];
+#pragma unroll
+        for (int j = 0; j < 10; ++j) o[j] = so[q * kGuideOut + 10 * g + j];
+        float mx = fmaxf(o[0], o[5]);
+        mx = fmaxf(mx, __shfl_xor(mx, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 32));
+        float e0 = expf(o[0] - mx), e1 = expf(o[5] - mx);
+        float sum = e0 + e1;
+        sum += __shfl_xor(sum, 16);
+        sum += __shfl_xor(sum, 32);
+        if (vq) {
+            float4 *dst = mix + 10 * (size_t)(base + 16 * t + q);
+            reinterpret_cast<float2 *>(dst)[g] = make_float2(e0 / sum, e1 / sum);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                f3 r = mk(o[5 * j + 1], o[5 * j + 2], o[5 * j + 3]);
+                float nr = sqrtf(sqnorm(r));
+                f3 mu = nr > 1e-12f ? r / nr : mk(0.0f, 0.0f, 1.0f);
+                float kap = fminf(fmaxf(expf(o[5 * j + 4]), kMinKappa), kMaxKappa);
+                dst[2 + 2 * g + j] = make_float4(mu.x, mu.y, mu.z, kap);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -616,9 +706,9 @@ int32_t guide_train_spp(const ks_guide *g) { return g->cfg.train_spp; }
 
 int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
                  const float4 *hits, float4 *mix, hipStream_t s) {
-    unsigned blocks = (n_max + 64 * kFwdWaves - 1) / (64 * kFwdWaves);
-    hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(64 * kFwdWaves), kFwdWaves * kFwdWaveBytes, s, net_of(g), sc,
-                       n_dev, rd, hits, mix);
+    unsigned blocks = (n_max + kInferBlock - 1) / kInferBlock;
+    hipLaunchKernelGGL(k_guide_infer<QueryHits>, dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
+                       QueryHits{sc, n_dev, rd, hits}, mix);
     KS_HIP(hipGetLastError());
     return KS_OK;
 }
@@ -809,9 +899,9 @@ int ks_guide_forward(ks_guide *g, int64_t n, const float *queries, float *mixtur
     KS_HIP(q.alloc(sizeof(float) * 9 * n));
     KS_HIP(m.alloc(sizeof(float4) * 10 * n));
     KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
-    unsigned blocks = (unsigned)((n + 64 * kFwdWaves - 1) / (64 * kFwdWaves));
-    hipLaunchKernelGGL(k_guide_forward, dim3(blocks), dim3(64 * kFwdWaves), kFwdWaves * kFwdWaveBytes, s,
-                       ksg::net_of(g), (int)n, q.as<float>(), m.as<float4>());
+    unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
+    hipLaunchKernelGGL(k_guide_infer<QueryArray>, dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                       QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>());
     KS_HIP(hipGetLastError());
     std::vector<float4> h((size_t)n * 10);
     KS_HIP(hipMemcpyAsync(h.data(), m.p, sizeof(float4) * 10 * n, hipMemcpyDeviceToHost, s));
