@@ -141,6 +141,31 @@ class RenderTarget2 {
     ks_film *h_ = nullptr;
 };
 
+// The learned path guide (no reference counterpart): hash grid + MLP -> 8-lobe vMF mixture on the device,
+// trained online by guided renders. Bounds are the scene's bounding box.
+class Guide {
+  public:
+    Guide(const Device &dev, const Scene &scene, const ks_guide_config &cfg = ks_guide_config{}) {
+        ks_scene_info i = scene.info();
+        check(ks_guide_create(dev.handle(), &cfg, i.bound_lo, i.bound_hi, &h_), "ks_guide_create");
+    }
+    ~Guide() { ks_guide_free(h_); }
+    Guide(const Guide &) = delete;
+    Guide &operator=(const Guide &) = delete;
+    void set_mode(float guide_fraction, int max_steps_per_pass) {
+        check(ks_guide_set_mode(h_, guide_fraction, max_steps_per_pass), "ks_guide_set_mode");
+    }
+    ks_guide_stats stats() const {
+        ks_guide_stats st{};
+        check(ks_guide_get_stats(h_, &st), "ks_guide_get_stats");
+        return st;
+    }
+    ks_guide *handle() const { return h_; }
+
+  private:
+    ks_guide *h_ = nullptr;
+};
+
 struct SmallPTInput {
     const Scene *scene = nullptr;
     const Camera *camera = nullptr;
@@ -166,7 +191,8 @@ struct SmallPTInput {
 
     // additions of this build
     int mis_mode = KS_MIS_REFERENCE;
-    int guide_mode = KS_GUIDE_OFF;
+    int guide_mode = KS_GUIDE_OFF; // KS_GUIDE_TRAIN_AND_SAMPLE needs `guide` and mis_mode consistent
+    Guide *guide = nullptr;
     int queue_size = 0;
 
     ks_small_pt_params params() const {
@@ -187,7 +213,7 @@ struct SmallPTInput {
         p.clamp_indirect = clamp_indirect;
         p.spp_prog_interval = spp_prog_interval;
         p.mis_mode = mis_mode;
-        p.guide_mode = guide_mode;
+        p.guide_mode = KS_GUIDE_OFF; // the guided entry point is selected by run()
         p.queue_size = queue_size;
         return p;
     }
@@ -212,8 +238,18 @@ struct SmallPT {
                 i->prog_interval_callback(*i->target, spp_finished);
             };
         }
-        check(ks_small_pt_render(in.scene->handle(), &in.camera->desc, &p, in.target->handle(), fn, &ctx, &st),
-              "ks_small_pt_render");
+        if (in.guide_mode == KS_GUIDE_TRAIN_AND_SAMPLE) {
+            if (!in.guide) {
+                std::fprintf(stderr, "[keepsake_amd] guide_mode train_and_sample needs SmallPTInput::guide\n");
+                std::abort();
+            }
+            check(ks_small_pt_render_guided(in.scene->handle(), &in.camera->desc, &p, in.guide->handle(),
+                                            in.target->handle(), fn, &ctx, &st),
+                  "ks_small_pt_render_guided");
+        } else {
+            check(ks_small_pt_render(in.scene->handle(), &in.camera->desc, &p, in.target->handle(), fn, &ctx, &st),
+                  "ks_small_pt_render");
+        }
         return st;
     }
 };

@@ -4,13 +4,16 @@
 // Scenes come from this build's procedural asset type (keys `procedural`, `procedural_seed`), since the
 // mesh-asset loaders (mesh_asset.cpp) are not part of this build yet; `object` / `compound_object`
 // therefore fail with a clear message. Additional keys of this build: mis_mode ("reference" or
-// "consistent"), queue_size.
+// "consistent"), queue_size, and a [task.guide] table that turns on the learned path guide (keys
+// fraction, train_spp, learning_rate, beta1, beta2, log2_table_size, base_resolution, max_resolution,
+// train_batch, max_steps_per_pass, weight_clamp, seed; it needs mis_mode = "consistent").
 #include "small_pt_task.h"
 
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <memory>
 
 namespace ks {
 
@@ -137,6 +140,26 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
     else if (mis == "consistent") in.mis_mode = KS_MIS_CONSISTENT;
     else throw ConfigError("mis_mode must be reference or consistent");
     in.queue_size = args.load_integer("queue_size", 0);
+    if (args.contains("guide")) {
+        ConfigArgs g = args["guide"];
+        if (in.mis_mode != KS_MIS_CONSISTENT)
+            throw ConfigError("the guide needs mis_mode = \"consistent\" (reference weights do not sum to one)");
+        st.guided = true;
+        in.guide_mode = KS_GUIDE_TRAIN_AND_SAMPLE;
+        ks_guide_config &c = st.guide_cfg;
+        c.guide_fraction = g.load_float("fraction", 0.0f);
+        c.train_spp = g.load_integer("train_spp", 0);
+        c.learning_rate = g.load_float("learning_rate", 0.0f);
+        c.beta1 = g.load_float("beta1", 0.0f);
+        c.beta2 = g.load_float("beta2", 0.0f);
+        c.log2_table_size = g.load_integer("log2_table_size", 0);
+        c.base_resolution = g.load_float("base_resolution", 0.0f);
+        c.max_resolution = g.load_float("max_resolution", 0.0f);
+        c.train_batch = g.load_integer("train_batch", 0);
+        c.max_steps_per_pass = g.load_integer("max_steps_per_pass", 0);
+        c.weight_clamp = g.load_float("weight_clamp", 0.0f);
+        c.seed = (uint64_t)g.load_integer("seed", 0);
+    }
 
     if (args.contains("camera")) {
         st.camera = load_camera(args["camera"]);
@@ -155,6 +178,8 @@ std::string describe(const SmallPTSetup &st) {
                   "\"render_height\": %d, \"crop\": [%d, %d, %d, %d], \"spp\": %d, \"rng_seed\": %d, "
                   "\"include_background\": %s, \"backdrop\": [%g, %g, %g], \"clamp_direct\": %g, "
                   "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
+                  "\"guided\": %s, \"guide\": {\"fraction\": %g, \"train_spp\": %d, \"learning_rate\": %g, "
+                  "\"log2_table_size\": %d}, "
                   "\"camera\": {\"use_matrix\": %d, \"pos\": [%g, %g, %g], \"target\": [%g, %g, %g], "
                   "\"up\": [%g, %g, %g], \"vfov_deg\": %.6g, \"aspect\": %.6g, \"to_world\": [%g, %g, %g, %g, %g, %g, "
                   "%g, %g, %g, %g, %g, %g, %g, %g, %g, %g]}}",
@@ -162,7 +187,8 @@ std::string describe(const SmallPTSetup &st) {
                   in.render_width, in.render_height, in.crop_start_x, in.crop_start_y, in.crop_width, in.crop_height,
                   in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
                   in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,
-                  in.queue_size, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
+                  in.queue_size, st.guided ? "true" : "false", st.guide_cfg.guide_fraction, st.guide_cfg.train_spp,
+                  st.guide_cfg.learning_rate, st.guide_cfg.log2_table_size, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
                   c.up[0], c.up[1], c.up[2], c.vfov_deg, c.aspect, c.to_world[0], c.to_world[1], c.to_world[2],
                   c.to_world[3], c.to_world[4], c.to_world[5], c.to_world[6], c.to_world[7], c.to_world[8],
                   c.to_world[9], c.to_world[10], c.to_world[11], c.to_world[12], c.to_world[13], c.to_world[14],
@@ -181,6 +207,11 @@ void small_pt(const ConfigArgs &args, const fs::path &task_dir, int task_id) {
     double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - b0).count();
     RenderTarget2 rt(dev, (uint32_t)st.input.render_width, (uint32_t)st.input.render_height, st.input.backdrop);
     SmallPTInput in = st.input;
+    std::unique_ptr<Guide> guide;
+    if (st.guided) {
+        guide = std::make_unique<Guide>(dev, scene, st.guide_cfg);
+        in.guide = guide.get();
+    }
     in.scene = &scene;
     in.camera = &st.camera;
     in.target = &rt;
@@ -207,6 +238,7 @@ void small_pt(const ConfigArgs &args, const fs::path &task_dir, int task_id) {
     js << "{\"render_seconds\": " << secs << ", \"render_ms_library\": " << s.wall_ms << ", \"paths\": " << s.paths
        << ", \"rays_closest\": " << s.rays_closest << ", \"rays_shadow\": " << s.rays_shadow
        << ", \"iterations\": " << s.iterations << ", \"nonfinite\": " << s.nonfinite
+       << ", \"guide_samples\": " << s.guide_samples
        << ", \"scene_build_seconds\": " << build_s << ", \"setup\": " << describe(st) << "}\n";
 }
 

@@ -12,6 +12,8 @@ struct SmallPTSetup {
     Camera camera;
     SmallPTInput input;
     std::string scene_name;
+    bool guided = false;          // a [task.guide] table is present
+    ks_guide_config guide_cfg{};  // its keys (zero: library default)
 };
 
 // Parses one [[task]] table of type small_pt without touching the GPU.

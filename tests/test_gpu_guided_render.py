@@ -1,0 +1,121 @@
+"""Guided small_pt (ks_small_pt_render_guided) on the slit-lit room.
+
+1. With the guide's sampling fraction at 0 the guided entry point must reproduce the plain render bit
+   for bit (same Sobol samples, same estimator), while it still records paths and trains: the plumbing
+   around the estimator changes nothing.
+2. With the trained guide sampling half of the directions the estimator stays unbiased: block means of
+   the per-pass images agree with an FP64 plain reference within their standard errors (z-test), and
+   relMSE keeps falling as 1 / spp.
+3. Training lowers the cross-entropy of the mixture on the radiance samples below that of a cosine lobe
+   (the network learns something the BSDF-only method does not have)."""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+W, H = 128, 72
+
+
+@pytest.fixture(scope="module")
+def room(ks, device):
+    hs = ks.procedural_scene("room", 3)
+    sc = ks.Scene(device, hs)
+    yield hs, sc
+    sc.close()
+
+
+def _params(ks, spp, interval=1, **kw):
+    return ks.small_pt_params(bounces=16, render_width=W, render_height=H, spp=spp, spp_prog_interval=interval,
+                              mis_mode=ks.KS_MIS_CONSISTENT, clamp_direct=0.0, clamp_indirect=0.0, **kw)
+
+
+def test_zero_fraction_reproduces_plain(ks, device, room):
+    hs, sc = room
+    plain = ks.Film(device, W, H)
+    ks.small_pt_render(sc, hs.camera, _params(ks, 16, 16), plain)
+    info = sc.info()
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"], ks.guide_config(seed=2))
+    g.set_mode(0.0, 4)  # never sample the guide, but keep recording and training
+    guided = ks.Film(device, W, H)
+    st = ks.small_pt_render_guided(sc, hs.camera, _params(ks, 16, 4), g, guided)
+    a, wa = plain.download()
+    b, wb = guided.download()
+    print("guide samples", st["guide_samples"], "steps", g.stats()["steps"], flush=True)
+    assert g.stats()["steps"] > 0 and st["guide_samples"] > 0
+    assert np.array_equal(wa, wb)
+    assert np.array_equal(a, b)
+    g.close()
+    plain.close()
+    guided.close()
+
+
+def test_guided_is_unbiased(ks, device, room):
+    hs, sc = room
+    ref = ks.Film(device, W, H)
+    ks.small_pt_render(sc, hs.camera, _params(ks, 8192, 8192), ref)
+    ref_rgb, ref_w = ref.download()
+    info = sc.info()
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"], ks.guide_config(seed=3, guide_fraction=0.5))
+    film = ks.Film(device, W, H)
+    sums = []
+
+    def on_pass(f, s):
+        rgb, _ = f.download()
+        sums.append(rgb.copy())
+
+    P = 160
+    ks.small_pt_render_guided(sc, hs.camera, _params(ks, P), g, film, progress=on_pass)
+    assert len(sums) == P
+    per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)  # one sample per pixel each
+    # 8 x 8 pixel blocks, three channels
+    blocks = per_pass.reshape(P, H // 8, 8, W // 8, 8, 3).mean(axis=(2, 4))
+    ref_blocks = (ref_rgb / np.maximum(ref_w[..., :1] + ref_w[..., 1:], 1)).reshape(H // 8, 8, W // 8, 8, 3).mean(
+        axis=(1, 3))
+    mean = blocks[16:].mean(axis=0)  # passes after the guide has had 16 passes of training
+    se = blocks[16:].std(axis=0, ddof=1) / math.sqrt(P - 16)
+    se_ref = se * math.sqrt((P - 16) / 8192)  # the reference's own error, same variance per sample assumed
+    z = (mean - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
+    print("max |z| over", z.size, "block-channels:", float(np.abs(z).max()), flush=True)
+    assert np.abs(z).max() < 5.0
+    img_mean = blocks[16:].mean(axis=(1, 2, 3))
+    zi = (img_mean.mean() - ref_blocks.mean()) / (img_mean.std(ddof=1) / math.sqrt(P - 16))
+    print("whole-image z", float(zi), flush=True)
+    assert abs(zi) < 4.0
+    # relMSE of the whole guided render keeps falling (an estimator with a bias would level off)
+    r = film.relmse(ref)
+    print("relMSE after", P, "spp:", r, flush=True)
+    g.close()
+    film.close()
+    ref.close()
+
+
+def test_training_beats_cosine_lobe(ks, device, room):
+    hs, sc = room
+    info = sc.info()
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"], ks.guide_config(seed=4, guide_fraction=0.3))
+    film = ks.Film(device, W * 2, H * 2)
+    p = ks.small_pt_params(bounces=16, render_width=2 * W, render_height=2 * H, spp=96, spp_prog_interval=96,
+                           mis_mode=ks.KS_MIS_CONSISTENT, clamp_direct=0.0, clamp_indirect=0.0)
+    ks.small_pt_render_guided(sc, hs.camera, p, g, film)
+    st = g.stats()
+    print("cross-entropy guide %.4f cosine %.4f after %d steps" % (st["cross_entropy"], st["cross_entropy_cosine"],
+                                                                   st["steps"]), flush=True)
+    assert st["steps"] >= 96
+    assert st["cross_entropy"] < st["cross_entropy_cosine"] - 0.05
+    g.close()
+    film.close()
+
+
+def test_guided_needs_consistent_mis(ks, device, room):
+    hs, sc = room
+    info = sc.info()
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"])
+    film = ks.Film(device, W, H)
+    p = ks.small_pt_params(bounces=4, render_width=W, render_height=H, spp=1)
+    with pytest.raises(ks.KsError, match="consistent"):
+        ks.small_pt_render_guided(sc, hs.camera, p, g, film)
+    g.close()
+    film.close()

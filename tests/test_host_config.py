@@ -158,3 +158,25 @@ def test_errors_are_reported(tmp_path, extra, msg):
 def test_missing_required_key(tmp_path):
     r = resolve(tmp_path, BASE.replace("spp = 4\n", ""))
     assert r.returncode != 0 and "spp" in r.stderr
+
+
+def test_guide_table(tmp_path):
+    text = BASE + """mis_mode = "consistent"
+[task.guide]
+fraction = 0.25
+train_spp = 32
+learning_rate = 5e-3
+log2_table_size = 18
+"""
+    (t,) = tasks(tmp_path, text)
+    assert t["guided"] is True
+    g = t["guide"]
+    assert g["fraction"] == pytest.approx(0.25) and g["train_spp"] == 32
+    assert g["learning_rate"] == pytest.approx(5e-3) and g["log2_table_size"] == 18
+    (plain,) = tasks(tmp_path, BASE)
+    assert plain["guided"] is False
+
+
+def test_guide_needs_consistent_mis(tmp_path):
+    r = resolve(tmp_path, BASE + "[task.guide]\nfraction = 0.5\n")
+    assert r.returncode != 0 and "consistent" in r.stderr
