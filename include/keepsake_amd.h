@@ -223,7 +223,7 @@ typedef struct ks_guide ks_guide;
 typedef struct ks_guide_config {
     int32_t log2_table_size;    /* hash-table entries per level, log2 (19) */
     float base_resolution;      /* coarsest grid resolution (16) */
-    float max_resolution;       /* finest grid resolution (2048) */
+    float max_resolution;       /* finest grid resolution, per unit of the scene bound (256) */
     float learning_rate;        /* Adam lr (1e-2) */
     float beta1, beta2, eps;    /* Adam moments (0.9, 0.999, 1e-8 as adam.slang) */
     float guide_fraction;       /* one-sample MIS: probability of sampling the guide (0.5) */

@@ -161,53 +161,69 @@ __device__ __forceinline__ bf16x4 pack_bf(float a, float b, float c, float d) {
 __device__ __forceinline__ bf16x4 relu_bf(f32x4 d) {
     return pack_bf(fmaxf(d[0], 0.0f), fmaxf(d[1], 0.0f), fmaxf(d[2], 0.0f), fmaxf(d[3], 0.0f));
 }
-// A operand of W^T (rows = units m*16.., k-block kk) from the [out][in] BF16 copy
-__device__ __forceinline__ bf16x4 wt_frag(const uint16_t *__restrict__ wt, int m, int kk, int q, int g) {
-    return glb_a(wt + (m * 16 + q) * 64 + kk * 16 + 4 * g);
+// A operand of W^T (rows = units m*16.., k-block kk) from an [out][in] BF16 copy with row stride ld
+__device__ __forceinline__ bf16x4 wt_frag(const uint16_t *wt, int ld, int m, int kk, int q, int g) {
+    return *reinterpret_cast<const bf16x4 *>(wt + (m * 16 + q) * ld + kk * 16 + 4 * g);
 }
 
 constexpr int kInferBlock = 256;
+constexpr int kWtLd = 68; // LDS row stride (BF16) of the staged [out][in] weights: 136 B rows spread banks
+constexpr int kInferLdsWeights = (64 + 64 + kGuideOut) * kWtLd; // BF16 elements
 
 template <class Src>
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src src, float4 *mix) {
+    __shared__ __align__(16) uint16_t s_w[kInferLdsWeights];
     __shared__ float s_out[kInferBlock / 64][16 * kGuideOut];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, q = lane & 15;
     const uint32_t n = src.count();
     if (blockIdx.x * blockDim.x >= n) return; // whole block idle
+    // stage W1^T, W2^T, W3^T (BF16, [out][in]) in LDS once per block
+    for (int e = threadIdx.x; e < (64 + 64 + kGuideOut) * 16; e += kInferBlock) {
+        int row = e >> 4, c4 = (e & 15) * 4; // rows 0..63 W1, 64..127 W2, 128..175 W3
+        const uint16_t *srcw = row < 64 ? net.wt[0] + row * 64 : (row < 128 ? net.wt[1] + (row - 64) * 64
+                                                                           : net.wt[2] + (row - 128) * 64);
+        *reinterpret_cast<uint2 *>(s_w + row * kWtLd + c4) = *reinterpret_cast<const uint2 *>(srcw + c4);
+    }
     const uint32_t base = blockIdx.x * blockDim.x + wave * 64;
     f3 P = mk(0, 0, 0), Nn = mk(0, 0, 0), WO = mk(0, 0, 0);
     const bool valid = src.load(base + lane, P, Nn, WO);
     float u[3];
     normalise_position(net, P, u);
-    float *so = s_out[wave];
-#pragma unroll 1
+    // encode all four column tiles first so that their 128 hash gathers per lane are in flight together
+    bf16x4 b0[4], b1[4], b2[4];
+    bool vq[4];
+#pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int sl = 16 * t + q; // lane that fetched this tile's query
-        const bool vq = __shfl((int)valid, sl) != 0;
+        vq[t] = __shfl((int)valid, sl) != 0;
         float uq[3] = {__shfl(u[0], sl), __shfl(u[1], sl), __shfl(u[2], sl)};
         const float nx = __shfl(Nn.x, sl), ny = __shfl(Nn.y, sl), nz = __shfl(Nn.z, sl);
         const float ox = __shfl(WO.x, sl), oy = __shfl(WO.y, sl), oz = __shfl(WO.z, sl);
-        bf16x4 b[3];
-        {
-            float f[8];
-            grid_pair(net, 2 * g, uq, f[0], f[1]);
-            grid_pair(net, 2 * g + 1, uq, f[2], f[3]);
-            grid_pair(net, 8 + 2 * g, uq, f[4], f[5]);
-            grid_pair(net, 9 + 2 * g, uq, f[6], f[7]);
-            b[0] = pack_bf(f[0], f[1], f[2], f[3]);
-            b[1] = pack_bf(f[4], f[5], f[6], f[7]);
-            b[2] = g == 0 ? pack_bf(nx, ny, nz, ox) : (g == 1 ? pack_bf(oy, oz, 1.0f, 0.0f) : pack_bf(0, 0, 0, 0));
-            if (!vq) b[0] = b[1] = b[2] = pack_bf(0, 0, 0, 0);
-        }
+        float f[8];
+        grid_pair(net, 2 * g, uq, f[0], f[1]);
+        grid_pair(net, 2 * g + 1, uq, f[2], f[3]);
+        grid_pair(net, 8 + 2 * g, uq, f[4], f[5]);
+        grid_pair(net, 9 + 2 * g, uq, f[6], f[7]);
+        b0[t] = pack_bf(f[0], f[1], f[2], f[3]);
+        b1[t] = pack_bf(f[4], f[5], f[6], f[7]);
+        b2[t] = g == 0 ? pack_bf(nx, ny, nz, ox) : (g == 1 ? pack_bf(oy, oz, 1.0f, 0.0f) : pack_bf(0, 0, 0, 0));
+        if (!vq[t]) b0[t] = b1[t] = b2[t] = pack_bf(0, 0, 0, 0);
+    }
+    __syncthreads(); // weights staged
+    const uint16_t *w1 = s_w, *w2 = s_w + 64 * kWtLd, *w3 = s_w + 128 * kWtLd;
+    float *so = s_out[wave];
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
         bf16x4 h[4];
         {
             f32x4 d[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 d[m] = f32x4{0, 0, 0, 0};
-#pragma unroll
-                for (int kk = 0; kk < 3; ++kk) d[m] = mfma16(wt_frag(net.wt[0], m, kk, q, g), b[kk], d[m]);
+                d[m] = mfma16(wt_frag(w1, kWtLd, m, 0, q, g), b0[t], d[m]);
+                d[m] = mfma16(wt_frag(w1, kWtLd, m, 1, q, g), b1[t], d[m]);
+                d[m] = mfma16(wt_frag(w1, kWtLd, m, 2, q, g), b2[t], d[m]);
             }
 #pragma unroll
             for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
@@ -218,7 +234,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
             for (int m = 0; m < 4; ++m) {
                 d[m] = f32x4{0, 0, 0, 0};
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(net.wt[1], m, kk, q, g), h[kk], d[m]);
+                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(w2, kWtLd, m, kk, q, g), h[kk], d[m]);
             }
 #pragma unroll
             for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
@@ -229,7 +245,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
             for (int m = 0; m < 3; ++m) {
                 d[m] = f32x4{0, 0, 0, 0};
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(net.wt[2], m, kk, q, g), h[kk], d[m]);
+                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(w3, kWtLd, m, kk, q, g), h[kk], d[m]);
             }
 #pragma unroll
             for (int m = 0; m < 3; ++m)
@@ -248,7 +264,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
         float sum = e0 + e1;
         sum += __shfl_xor(sum, 16);
         sum += __shfl_xor(sum, 32);
-        if (vq) {
+        if (vq[t]) {
             float4 *dst = mix + 10 * (size_t)(base + 16 * t + q);
             reinterpret_cast<float2 *>(dst)[g] = make_float2(e0 / sum, e1 / sum);
 #pragma unroll
@@ -773,7 +789,7 @@ int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float b
     if (cfg_in) c = *cfg_in;
     if (c.log2_table_size == 0) c.log2_table_size = 19;
     if (c.base_resolution == 0.0f) c.base_resolution = 16.0f;
-    if (c.max_resolution == 0.0f) c.max_resolution = 2048.0f;
+    if (c.max_resolution == 0.0f) c.max_resolution = 256.0f;
     if (c.learning_rate == 0.0f) c.learning_rate = 1e-2f;
     if (c.beta1 == 0.0f) c.beta1 = 0.9f;
     if (c.beta2 == 0.0f) c.beta2 = 0.999f;

@@ -62,7 +62,7 @@ def fp16(a):
 # ---------------------------------------------------------------------------------------------------
 # grid layout
 # ---------------------------------------------------------------------------------------------------
-def resolutions(base=16.0, max_res=2048.0):
+def resolutions(base=16.0, max_res=256.0):
     """floor(base * b^l) with b = (max / base)^(1 / (L - 1)), in float64."""
     g = math.exp(math.log(float(F32(max_res)) / float(F32(base))) / (LEVELS - 1))
     return [int(math.floor(float(F32(base)) * g**l + 1e-9)) for l in range(LEVELS)]
@@ -89,7 +89,7 @@ def grid_index(x, y, z, res, log2_t):
 class Layout:
     """Grid resolutions, table size and the position normalisation of one guide."""
 
-    def __init__(self, log2_t=19, base=16.0, max_res=2048.0, bound_lo=(0, 0, 0), bound_hi=(1, 1, 1)):
+    def __init__(self, log2_t=19, base=16.0, max_res=256.0, bound_lo=(0, 0, 0), bound_hi=(1, 1, 1)):
         self.log2_t = log2_t
         self.res = resolutions(base, max_res)
         self.lo, self.inv_ext = normalisation(bound_lo, bound_hi)

@@ -39,6 +39,7 @@ CONFIGS = {
     # dense coarse levels ((res+1)^3 <= T) and hashed fine ones
     "small": dict(log2_table_size=12, base_resolution=4.0, max_resolution=256.0),
     "default": dict(),
+    "fine": dict(max_resolution=2048.0),
 }
 BOUNDS = ((-1.0, 0.0, 2.0), (3.0, 1.5, 5.0))
 
@@ -48,7 +49,7 @@ def guide(request, ks, device):
     cfg = ks.guide_config(seed=7, **CONFIGS[request.param])
     g = ks.Guide(device, BOUNDS[0], BOUNDS[1], cfg)
     c = CONFIGS[request.param]
-    lay = go.Layout(c.get("log2_table_size", 19), c.get("base_resolution", 16.0), c.get("max_resolution", 2048.0),
+    lay = go.Layout(c.get("log2_table_size", 19), c.get("base_resolution", 16.0), c.get("max_resolution", 256.0),
                     BOUNDS[0], BOUNDS[1])
     yield g, lay
     g.close()
