@@ -25,9 +25,10 @@ sys.path.insert(1, str(ROOT / "oracle"))  # the CPU oracle, used only by the cpu
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one closest-hit query: ray in (2 x 16 B), hit out (16 B), each node fetch
-# (64 B: both child boxes + child pointers), each triangle test (48 B: three vertices + id).
+# (BVH4: 96 B of child boxes + 16 B of child pointers; BVH2: 64 B), each triangle test (48 B: three
+# vertices + id).
 RAY_IO_BYTES = 48
-NODE_BYTES = 64
+NODE_BYTES = {4: 112, 2: 64}
 TRI_BYTES = 48
 
 METRIC = "time-to-relMSE 0.01 guided vs plain, Mpaths/s, %HBM peak on 1 MI355X"
@@ -142,6 +143,7 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--bounces", type=int, default=12)
     ap.add_argument("--queue", type=int, default=0)
+    ap.add_argument("--branching", type=int, default=4, choices=(2, 4), help="BVH node width on the device")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -189,9 +191,9 @@ def main() -> int:
     log(f"scene {args.scene}: {hs.n_tris} triangles (generated in {gen_s:.1f} s)")
     dev = ks.Device(0)
     t0 = time.time()
-    sc = ks.Scene(dev, hs)
+    sc = ks.Scene(dev, hs, bvh={"branching": args.branching})
     info = sc.info()
-    log(f"SBVH: {info['n_nodes']} nodes, {info['n_refs']} refs, depth {info['max_depth']}, "
+    log(f"SBVH: {info['n_nodes']} BVH{info['branching']} nodes, {info['n_refs']} refs, depth {info['max_depth']}, "
         f"build {info['build_ms'] / 1e3:.1f} s")
     cam = hs.camera
     kw = dict(
@@ -235,7 +237,7 @@ def main() -> int:
     k_ms = totals["kernel_ms"]["closest"]
     alg_bytes = (
         RAY_IO_BYTES * totals["rays_closest"]
-        + NODE_BYTES * totals["node_visits_closest"]
+        + NODE_BYTES[info["branching"]] * totals["node_visits_closest"]
         + TRI_BYTES * totals["tri_tests_closest"]
     )
     achieved = alg_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
@@ -250,7 +252,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": "k_closest (closest-hit SBVH traversal)",
+        "kernel": f"k_closest (closest-hit SBVH traversal, BVH{info['branching']} nodes)",
         "bytes_per_launch": alg_bytes / max(launches, 1),
         "avg_launch_ms": k_ms / max(launches, 1),
     }
@@ -289,6 +291,7 @@ def main() -> int:
             "spp": args.spp,
             "bounces": args.bounces,
             "queue_size": st["queue_size"],
+            "bvh": f"SBVH collapsed to BVH{info['branching']}",
         },
         "mrays_per_s": round(n_rays / (elapsed / args.steps) / 1e6, 1),
         "kernel_ms_per_step": {k: round(v / args.steps, 2) for k, v in totals["kernel_ms"].items()},

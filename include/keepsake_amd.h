@@ -85,6 +85,7 @@ typedef struct ks_bvh_options {
     float trav_weight;
     float alpha;
     int32_t threads; /* host build threads, 0 = all */
+    int32_t branching; /* node width on the device: 4 (default; the SBVH collapsed to 128-byte nodes) or 2 */
 } ks_bvh_options;
 
 /* create_camera (camera.cpp:172-205) perspective branch: look_at(pos, target, up) with vfov in
@@ -169,6 +170,7 @@ typedef struct ks_scene_info {
     uint64_t device_bytes;
     double build_ms;
     float bound_lo[3], bound_hi[3]; /* Scene::bound (scene.cpp:162-169) */
+    uint32_t branching;             /* node width of the device BVH (n_nodes and max_depth refer to it) */
 } ks_scene_info;
 int ks_scene_info_get(const ks_scene *scene, ks_scene_info *info);
 /* PowerLightSampler tables in flat light order: prim (-1 for non-mesh), power, cdf (n+1). */

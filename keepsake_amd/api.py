@@ -83,6 +83,7 @@ class BvhOptions(C.Structure):
         ("trav_weight", C.c_float),
         ("alpha", C.c_float),
         ("threads", C.c_int32),
+        ("branching", C.c_int32),
     ]
 
 
@@ -159,6 +160,7 @@ class SceneInfo(C.Structure):
         ("build_ms", C.c_double),
         ("bound_lo", C.c_float * 3),
         ("bound_hi", C.c_float * 3),
+        ("branching", C.c_uint32),
     ]
 
 

@@ -189,6 +189,7 @@ __global__ void k_iter_done(Ctrl *c) {
 
 // Closest hit for n rays (count read from *n_dev when given). Grid-stride over rays; one LDS stack
 // column per lane.
+template <int W>
 __global__ __launch_bounds__(kTraceBlock) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                          const float4 *ro, const float4 *rd, float4 *hit,
                                                          uint32_t *ovf, uint32_t ovf_cap,
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest(DevScene sc, const uint
         float4 o4 = ro[i], d4 = rd[i];
         TravRay r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
         TravHit h;
-        traverse<false>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
+        traverse_w<false, W>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
         hit[i] = h.prim >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim))
                              : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
         ++n_rays;
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest(DevScene sc, const uint
 
 // Any hit. mode 0: write the occlusion flag; mode 1: add the NEE contribution of unoccluded rays to
 // their target (next path buffer slot, or the per-sample result when bit 31 is set).
-template <int MODE>
+template <int MODE, int W>
 __global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
                                                         const uint32_t *target, const float4 *contrib,
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint3
         float4 o4 = ro[i], d4 = rd[i];
         TravRay r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
         TravHit h;
-        bool blocked = traverse<true>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
+        bool blocked = traverse_w<true, W>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
         ++n_rays;
         if (MODE == 0) {
             occ[i] = blocked ? 1 : 0;
@@ -336,8 +337,7 @@ KS_DEV f3 light_sample(const DevScene &sc, const Light &l, f3 p, float u0, float
         b1 = u1 / 2;
         b0 = u0 - b1;
     }
-    uint4 tri = sc.tri_idx[l.prim];
-    f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
+    f3 v0 = mk(l.vec[0], l.vec[1], l.vec[2]), v1 = mk(l.v1[0], l.v1[1], l.v1[2]), v2 = mk(l.v2[0], l.v2[1], l.v2[2]);
     f3 pos = v0 * (1.0f - b0 - b1) + v1 * b0 + v2 * b1;
     f3 n = normalized(cross(v1 - v0, v2 - v1));
     wi = pos - p;
@@ -356,8 +356,7 @@ KS_DEV f3 light_sample(const DevScene &sc, const Light &l, f3 p, float u0, float
 
 KS_DEV float mesh_light_pdf(const DevScene &sc, const Light &l, f3 wi, float wi_dist) { // light.cpp:607-628
     if (wi_dist == 0.0f) return 0.0f;
-    uint4 tri = sc.tri_idx[l.prim];
-    f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
+    f3 v0 = mk(l.vec[0], l.vec[1], l.vec[2]), v1 = mk(l.v1[0], l.v1[1], l.v1[2]), v2 = mk(l.v2[0], l.v2[1], l.v2[2]);
     f3 n = normalized(cross(v1 - v0, v2 - v1));
     float absdot = fabsf(dot(n, wi));
     if (absdot == 0.0f) return 0.0f;
@@ -909,6 +908,7 @@ struct ks_scene {
     ks_scene_info info{};
     std::vector<int32_t> light_prim;
     std::vector<float> light_power, light_cdf;
+    uint32_t max_stack = 1; // deepest traversal stack the tree can need
 };
 
 struct ks_film {
@@ -977,6 +977,16 @@ hipEvent_t Timer::take() {
     return dev->events[used++];
 }
 
+// Traversal launches for the scene's node width (2 or 4).
+template <typename... A> void launch_closest(int width, dim3 g, hipStream_t s, A... args) {
+    if (width == 4) hipLaunchKernelGGL(k_closest<4>, g, dim3(kTraceBlock), 0, s, args...);
+    else hipLaunchKernelGGL(k_closest<2>, g, dim3(kTraceBlock), 0, s, args...);
+}
+template <int MODE, typename... A> void launch_anyhit(int width, dim3 g, hipStream_t s, A... args) {
+    if (width == 4) hipLaunchKernelGGL((k_anyhit<MODE, 4>), g, dim3(kTraceBlock), 0, s, args...);
+    else hipLaunchKernelGGL((k_anyhit<MODE, 2>), g, dim3(kTraceBlock), 0, s, args...);
+}
+
 int trace_grid(const ks_device *dev, uint32_t n) {
     // 8 blocks of 256 lanes per CU keep every SIMD at 8 waves; the grid-stride loop covers the rest
     uint32_t max_blocks = (uint32_t)std::max(1, dev->n_cu) * 8u;
@@ -984,7 +994,7 @@ int trace_grid(const ks_device *dev, uint32_t n) {
     return (int)std::max(1u, std::min(max_blocks, need));
 }
 
-int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_depth) {
+int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_stack) {
     Workspace &ws = dev->ws;
     if (ws.Q < Q) {
         for (int b = 0; b < 2; ++b) {
@@ -1006,7 +1016,7 @@ int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_dep
     KS_HIP(ws.ctrl.ensure(sizeof(Ctrl)));
     KS_HIP(ws.stats.ensure(sizeof(unsigned long long) * kStatCount));
     uint32_t grid = (uint32_t)std::max(1, dev->n_cu) * 8u * kTraceBlock;
-    uint32_t cap = max_depth + 4 > (uint32_t)kStackLds ? max_depth + 4 - kStackLds : 1u;
+    uint32_t cap = max_stack + 4 > (uint32_t)kStackLds ? max_stack + 4 - kStackLds : 1u;
     if (ws.ovf_grid < grid || ws.ovf_cap < cap) {
         KS_HIP(ws.ovf.alloc(sizeof(uint32_t) * (size_t)grid * cap));
         ws.ovf_grid = grid;
@@ -1081,8 +1091,9 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q);
         tm.end(e);
         e = tm.begin(KS_K_CLOSEST);
-        hipLaunchKernelGGL(k_closest, dim3(tgrid), dim3(kTraceBlock), 0, s, sc->ds, &ctrl->n_active, 0u,
-                           pb[cur].ro, pb[cur].rd, ws.hit.as<float4>(), ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
+        launch_closest(sc->ds.width, dim3(tgrid), s, sc->ds, (const uint32_t *)&ctrl->n_active, 0u,
+                       (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, ws.hit.as<float4>(),
+                       ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
         tm.end(e);
         if (gp && gp->shade.mix) {
             e = tm.begin(KS_K_GUIDE);
@@ -1102,10 +1113,11 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                                ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, GuideShade{});
         tm.end(e);
         e = tm.begin(KS_K_SHADOW);
-        hipLaunchKernelGGL(k_anyhit<1>, dim3(tgrid), dim3(kTraceBlock), 0, s, sc->ds, &ctrl->n_shadow, 0u,
-                           ws.sh_o.as<float4>(), ws.sh_d.as<float4>(), (uint8_t *)nullptr, ws.sh_tgt.as<uint32_t>(),
-                           ws.sh_c.as<float4>(), pb[1 - cur].L, ws.samp.as<float4>(),
-                           gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
+        launch_anyhit<1>(sc->ds.width, dim3(tgrid), s, sc->ds, (const uint32_t *)&ctrl->n_shadow, 0u,
+                         (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), (uint8_t *)nullptr,
+                         (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[1 - cur].L,
+                         ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(),
+                         ws.ovf_cap, st);
         hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
         tm.end(e);
         KS_HIP(hipGetLastError());
@@ -1336,7 +1348,12 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
                 l.type = 3;
                 l.prim = t0 + ids[i];
                 l.area = area[i];
-                for (int k = 0; k < 3; ++k) l.color[k] = m.emission[k];
+                for (int k = 0; k < 3; ++k) {
+                    l.color[k] = m.emission[k];
+                    l.vec[k] = d->vertices[3 * d->indices[3 * l.prim] + k];
+                    l.v1[k] = d->vertices[3 * d->indices[3 * l.prim + 1] + k];
+                    l.v2[k] = d->vertices[3 * d->indices[3 * l.prim + 2] + k];
+                }
                 tri_light[l.prim] = (int32_t)lights.size();
                 lights.push_back(l);
                 powers.push_back((imp[i] + imp[i] + imp[i]) / 3.0f);
@@ -1365,7 +1382,9 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     sc->light_cdf = cdf;
 
     hipStream_t s = dev->stream;
-    KS_HIP(upload(sc->nodes, bvh.nodes, s));
+    const int width = o.branching == 2 ? 2 : 4;
+    if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
+    else KS_HIP(upload(sc->nodes, bvh.nodes, s));
     KS_HIP(upload(sc->tris, bvh.tris, s));
     KS_HIP(upload(sc->verts, verts, s));
     if (d->normals) {
@@ -1382,6 +1401,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
 
     DevScene &ds = sc->ds;
     ds.nodes = sc->nodes.as<float4>();
+    ds.width = width;
     ds.tris = sc->tris.as<float4>();
     ds.verts = sc->verts.as<float4>();
     ds.normals = d->normals ? sc->normals.as<float4>() : nullptr;
@@ -1397,10 +1417,13 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     ks_scene_info &in = sc->info;
     in.n_tris = d->n_tris;
     in.n_vertices = d->n_vertices;
-    in.n_nodes = (uint32_t)bvh.nodes.size();
+    in.n_nodes = (uint32_t)(width == 4 ? bvh.nodes4.size() : bvh.nodes.size());
+    in.branching = (uint32_t)width;
     in.n_refs = (uint32_t)bvh.tris.size();
     in.n_lights = (uint32_t)lights.size();
-    in.max_depth = bvh.max_depth;
+    in.max_depth = width == 4 ? bvh.max_depth4 : bvh.max_depth;
+    // traversal stack bound: every interior node on the path leaves at most width - 1 entries behind
+    sc->max_stack = (uint32_t)(width - 1) * in.max_depth + 1;
     in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->verts.bytes + sc->normals.bytes + sc->tri_idx.bytes +
                       sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes;
     for (int k = 0; k < 3; ++k) {
@@ -1526,7 +1549,7 @@ int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_smal
     int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
     KS_REQUIRE(pixels <= max_samp, "crop window too large for one sub-batch");
     uint32_t Q = pick_queue(params, pixels * (uint64_t)std::min(spp_chunk, params->spp));
-    e = ensure_workspace(dev, Q, pixels * (uint64_t)std::min(spp_chunk, params->spp), scene->info.max_depth);
+    e = ensure_workspace(dev, Q, pixels * (uint64_t)std::min(spp_chunk, params->spp), scene->max_stack);
     if (e) return e;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
                        (int)kStatCount);
@@ -1610,7 +1633,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
     const int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
     const uint64_t max_items = pixels * (uint64_t)std::min(spp_chunk, params->spp);
     uint32_t Q = pick_queue(params, max_items);
-    e = ensure_workspace(dev, Q, max_items, scene->info.max_depth);
+    e = ensure_workspace(dev, Q, max_items, scene->max_stack);
     if (e) return e;
     GuideWork &gw = dev->gw;
     if (gw.mix_q < Q) {
@@ -1733,7 +1756,7 @@ int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_
     rc.s0 = 0;
     rc.nb = 1;
     uint32_t Q = pick_queue(params, (uint64_t)n);
-    e = ensure_workspace(dev, Q, (size_t)n, scene->info.max_depth);
+    e = ensure_workspace(dev, Q, (size_t)n, scene->max_stack);
     if (e) return e;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
                        (int)kStatCount);
@@ -1828,25 +1851,24 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
     HostToDev<float4> dO, dD;
     KS_HIP(dO.put(o.data(), o.size(), s));
     KS_HIP(dD.put(d.data(), d.size(), s));
-    int e = ensure_workspace(dev, 256, 1, scene->info.max_depth);
+    int e = ensure_workspace(dev, 256, 1, scene->max_stack);
     if (e) return e;
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, dev->ws.stats.as<unsigned long long>(), (int)kStatCount);
     int grid = trace_grid(dev, (uint32_t)n);
     DevBuf out;
     if (hits) {
         KS_HIP(out.alloc(sizeof(float4) * n));
-        hipLaunchKernelGGL(k_closest, dim3(grid), dim3(kTraceBlock), 0, s, scene->ds, (const uint32_t *)nullptr,
-                           (uint32_t)n, dO.p(), dD.p(), out.as<float4>(), dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap,
-                           dev->ws.stats.as<unsigned long long>());
+        launch_closest(scene->ds.width, dim3(grid), s, scene->ds, (const uint32_t *)nullptr, (uint32_t)n,
+                       (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<float4>(), dev->ws.ovf.as<uint32_t>(),
+                       dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(hits, out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
     } else {
         KS_HIP(out.alloc(n));
-        hipLaunchKernelGGL(k_anyhit<0>, dim3(grid), dim3(kTraceBlock), 0, s, scene->ds, (const uint32_t *)nullptr,
-                           (uint32_t)n, dO.p(), dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
-                           (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr,
-                           dev->ws.ovf.as<uint32_t>(),
-                           dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
+        launch_anyhit<0>(scene->ds.width, dim3(grid), s, scene->ds, (const uint32_t *)nullptr, (uint32_t)n,
+                         (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
+                         (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr,
+                         dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(occ, out.p, n, hipMemcpyDeviceToHost, s));
     }

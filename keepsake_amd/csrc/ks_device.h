@@ -120,13 +120,22 @@ KS_DEV uint32_t part_1_by_1(uint32_t x) {
     return x;
 }
 
-// The 24 permutations of four base-4 digits in the order of sobol.h:171-197, packed 2 bits per digit.
-__constant__ static const uint8_t kPerm4[24] = {
-    0xE4, 0xB4, 0xD8, 0x78, 0x6C, 0x9C, 0xE1, 0xB1, 0xC9, 0x39, 0x2D, 0x8D,
-    0xC6, 0x36, 0xD2, 0x72, 0x4E, 0x1E, 0x27, 0x87, 0x1B, 0x4B, 0x63, 0x93};
+// The 24 base-4 digit permutations of sobol.h:171-197, one byte each (two bits per digit, in the
+// reference's order), packed four to a word and selected with compares instead of a memory access.
+KS_DEV uint32_t perm4_byte(uint32_t p) {
+    const uint32_t W0 = 0x78D8B4E4u, W1 = 0xB1E19C6Cu, W2 = 0x8D2D39C9u, W3 = 0x72D236C6u, W4 = 0x87271E4Eu,
+                   W5 = 0x93634B1Bu;
+    uint32_t w = p < 8 ? (p < 4 ? W0 : W1) : (p < 16 ? (p < 12 ? W2 : W3) : (p < 20 ? W4 : W5));
+    return (w >> ((p & 3u) * 8u)) & 0xffu;
+}
+// (mix_bits(x) >> 24) % 24 for a 32-bit x: the 40-bit quotient split at bit 32, with 2^32 = 16 mod 24.
+KS_DEV uint32_t perm_index(uint32_t x) {
+    uint64_t t = mix_bits((uint64_t)x) >> 24;
+    return ((uint32_t)(t >> 32) * 16u + (uint32_t)t % 24u) % 24u;
+}
 
 struct Sobol {
-    uint64_t morton;
+    uint32_t morton; // the reference's morton index is built from 32-bit parts (sobol.h:142-146)
     int32_t dim;
     int32_t seed;
     int32_t log2_spp;
@@ -139,26 +148,26 @@ struct Sobol {
         log2_spp = log2spp;
         n_digits = ndigits;
         dim = dim0;
-        uint32_t m32 = (((part_1_by_1(py) << 1) + part_1_by_1(px)) << log2spp) | sample; // 32-bit as reference
-        morton = (uint64_t)m32;
+        morton = (((part_1_by_1(py) << 1) + part_1_by_1(px)) << log2spp) | sample; // 32-bit as reference
     }
 
     KS_DEV uint64_t sample_index() const { // sobol.h:168-222
         uint64_t si = 0;
         const bool pow2 = log2_spp & 1;
         const int last = pow2 ? 1 : 0;
-        const uint64_t dmask = (uint64_t)(0x55555555u * (uint32_t)dim);
+        const uint32_t dmask = 0x55555555u * (uint32_t)dim;
         for (int i = n_digits - 1; i >= last; --i) {
             int shift = 2 * i - (pow2 ? 1 : 0);
-            uint32_t digit = (uint32_t)(morton >> shift) & 3u;
-            uint64_t higher = morton >> (shift + 2);
-            uint32_t p = (uint32_t)((mix_bits(higher ^ dmask) >> 24) % 24ull);
-            digit = (kPerm4[p] >> (2 * digit)) & 3u;
+            // digits above bit 31 of the 32-bit morton index are zero
+            uint32_t digit = shift < 32 ? (morton >> shift) & 3u : 0u;
+            uint32_t higher = shift + 2 < 32 ? morton >> (shift + 2) : 0u;
+            uint32_t p = perm_index(higher ^ dmask);
+            digit = (perm4_byte(p) >> (2 * digit)) & 3u;
             si |= (uint64_t)digit << shift;
         }
         if (pow2) {
-            uint32_t digit = (uint32_t)(morton & 1u);
-            si |= (uint64_t)(digit ^ (uint32_t)(mix_bits((morton >> 1) ^ dmask) & 1u));
+            uint32_t digit = morton & 1u;
+            si |= (uint64_t)(digit ^ (uint32_t)(mix_bits((uint64_t)((morton >> 1) ^ dmask)) & 1u));
         }
         return si;
     }
@@ -204,9 +213,10 @@ struct Light {
     float area;
     float pad;
     float color[4];
-    float vec[4];
+    float vec[4]; // direction / position; mesh triangle: vertex 0
+    float v1[4], v2[4]; // mesh triangle: vertices 1 and 2 (kept here so NEE needs no index fetch)
 };
-static_assert(sizeof(Light) == 48, "Light record is 48 bytes");
+static_assert(sizeof(Light) == 80, "Light record is 80 bytes");
 
 struct Closure {
     f3 basecolor;
@@ -389,6 +399,7 @@ KS_DEV f3 eval_dielectric_specular(f3 wo, f3 wi, const Closure &c) {
     float lw = 1.0f - c.metallic;
     if (lw == 0.0f) return mk(0, 0, 0);
     bool refl = wo.z * wi.z >= 0.0f;
+    if (!refl && c.specular_trans == 0.0f) return mk(0, 0, 0); // lw * specular_trans = 0
     float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
     f3 wh;
     if (refl) {
@@ -463,6 +474,16 @@ KS_DEV float pdf_dielectric_specular(f3 wo, f3 wi, const Closure &c) {
     return pdf;
 }
 
+// A lobe's pdf is only evaluated when its weight is non-zero: its product with a zero weight adds exactly
+// nothing to the mixture pdf. (At ior 1 the dielectric weight is a Fresnel term that float rounding
+// leaves far from zero at grazing angles, so it is evaluated there as in the reference.)
+KS_DEV bool dielectric_pdf_needed(f3 wo, f3 wi, const Closure &c, float w3) {
+    (void)wo;
+    (void)wi;
+    (void)c;
+    return w3 != 0.0f;
+}
+
 KS_DEV f3 bsdf_eval(f3 wo, f3 wi, const Closure &c) {
     if (wo.z == 0.0f || wi.z == 0.0f) return mk(0, 0, 0);
     f3 f = mk(0, 0, 0);
@@ -478,10 +499,10 @@ KS_DEV float bsdf_pdf(f3 wo, f3 wi, const Closure &c) {
     float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
     if (wo.z > 0.0f && wi.z > 0.0f) {
         p0 = wi.z * kInvPi;
-        p2 = pdf_metallic_specular(wo, wi, c);
+        if (w[2] != 0.0f) p2 = pdf_metallic_specular(wo, wi, c);
     }
-    if (wo.z > 0.0f && wi.z < 0.0f) p1 = pdf_diffuse_transmission(wo, wi, c);
-    float p3 = pdf_dielectric_specular(wo, wi, c);
+    if (wo.z > 0.0f && wi.z < 0.0f && w[1] != 0.0f) p1 = pdf_diffuse_transmission(wo, wi, c);
+    float p3 = dielectric_pdf_needed(wo, wi, c, w[3]) ? pdf_dielectric_specular(wo, wi, c) : 0.0f;
     return p0 * w[0] + p1 * w[1] + p2 * w[2] + p3 * w[3];
 }
 
@@ -520,8 +541,8 @@ KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, f
         wi = sample_cosine_hemisphere(u0, u1);
         p[0] = wi.z * kInvPi;
         if (is_zero(wi) || p[0] == 0.0f) return zero;
-        if (wo.z > 0.0f && wi.z > 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
-        p[3] = pdf_dielectric_specular(wo, wi, c);
+        if (wo.z > 0.0f && wi.z > 0.0f && w[2] != 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
+        if (dielectric_pdf_needed(wo, wi, c, w[3])) p[3] = pdf_dielectric_specular(wo, wi, c);
     } else if (lobe == 1) {
         if (c.diffuse_trans_fwd > 0.05f) {
             wi = sample_hg(-wo, c.diffuse_trans_fwd, u0, u1, &p[1]);
@@ -531,7 +552,7 @@ KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, f
             wi.z = -wi.z;
         }
         if (is_zero(wi) || p[1] == 0.0f) return zero;
-        p[3] = pdf_dielectric_specular(wo, wi, c);
+        if (dielectric_pdf_needed(wo, wi, c, w[3])) p[3] = pdf_dielectric_specular(wo, wi, c);
     } else if (lobe == 2) {
         f3 wh = g.sample(wo * sgnf(wo.z), u0, u1);
         wi = reflect(wo, wh);
@@ -539,7 +560,7 @@ KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, f
         p[2] = g.D(wh) * g.G1(wo) / (4.0f * fabsf(wo.z));
         if (is_zero(wi) || p[2] == 0.0f) return zero;
         if (wo.z > 0.0f && wi.z > 0.0f) p[0] = wi.z * kInvPi;
-        p[3] = pdf_dielectric_specular(wo, wi, c);
+        if (dielectric_pdf_needed(wo, wi, c, w[3])) p[3] = pdf_dielectric_specular(wo, wi, c);
     } else {
         f3 wh = g.sample(wo * sgnf(wo.z), u0, u1);
         float D = g.D(wh), G1 = g.G1(wo);
@@ -563,9 +584,9 @@ KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, f
         if (is_zero(wi) || p[3] == 0.0f) return zero;
         if (wo.z > 0.0f && wi.z > 0.0f) {
             p[0] = wi.z * kInvPi;
-            p[2] = pdf_metallic_specular(wo, wi, c);
+            if (w[2] != 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
         }
-        if (wo.z > 0.0f && wi.z < 0.0f) p[1] = pdf_diffuse_transmission(wo, wi, c);
+        if (wo.z > 0.0f && wi.z < 0.0f && w[1] != 0.0f) p[1] = pdf_diffuse_transmission(wo, wi, c);
     }
     pdf = p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3];
     return bsdf_eval(wo, wi, c) / pdf;

@@ -23,6 +23,15 @@ struct alignas(16) GpuNode {
 };
 static_assert(sizeof(GpuNode) == 64, "GpuNode must be 64 bytes");
 
+// GPU BVH4 node (the default): four children's boxes in SoA order, so one 128-byte line tests four
+// boxes; child pointers as for GpuNode (interior: GpuNode4 index). Unused slots hold an empty box.
+struct alignas(16) GpuNode4 {
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(GpuNode4) == 128, "GpuNode4 must be 128 bytes");
+
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMaxLeafRefs = 16;
 constexpr uint32_t kMaxRefs = 1u << 27;
@@ -37,15 +46,20 @@ struct alignas(16) GpuTri {
 static_assert(sizeof(GpuTri) == 48, "GpuTri must be 48 bytes");
 
 struct BvhResult {
-    std::vector<GpuNode> nodes;
+    std::vector<GpuNode> nodes;   // BVH2
+    std::vector<GpuNode4> nodes4; // the same tree collapsed to four children per node
     std::vector<GpuTri> tris;
-    uint32_t max_depth = 0;
+    uint32_t max_depth = 0;       // leaf depth of the BVH2
+    uint32_t max_depth4 = 0;      // leaf depth of the BVH4
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 };
 
 // SAH binned object + spatial split builder (restates cuda/sbvh.cu:301-440) followed by a flatten
 // into GpuNode/GpuTri arrays.
 BvhResult build_sbvh(const float *vertices, const uint32_t *indices, uint32_t n_tris, const ks_bvh_options &opt);
+// Collapses a flattened BVH2 into BVH4 nodes (leaves and references unchanged): each node repeatedly
+// opens its interior child of largest surface area until it holds four children.
+std::vector<GpuNode4> collapse_bvh4(const std::vector<GpuNode> &nodes, uint32_t &max_depth4);
 
 struct HostScene {
     std::vector<float> vertices, normals;

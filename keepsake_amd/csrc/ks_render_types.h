@@ -39,6 +39,7 @@ struct DevScene {
     int32_t n_lights;
     int32_t n_sky;
     int32_t sky[4];
+    int32_t width; // BVH node width: 2 (GpuNode, 64 B) or 4 (GpuNode4, 128 B)
 };
 
 struct RenderConsts {

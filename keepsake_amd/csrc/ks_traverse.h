@@ -1,4 +1,5 @@
-// BVH2 traversal for gfx950: 64-byte nodes holding both child boxes, leaves pointing into a 48-byte
+// BVH2 and BVH4 traversal for gfx950: 64-byte nodes holding both child boxes (or 128-byte nodes holding
+// four), leaves pointing into a 48-byte
 // per-reference triangle array, the watertight ray/triangle test of cuda/ray_tri.cuh:46-97 and a
 // per-lane short stack that lives in LDS (kStackLds entries, stored with the block size as stride so
 // a wave's 64 lanes hit 64 distinct banks) and spills its oldest entries to a global overflow area.
@@ -198,6 +199,102 @@ KS_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict_
         node = st.pop();
     }
     return hit.prim >= 0;
+}
+
+KS_DEV float lane4(float4 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+KS_DEV uint32_t lane4(uint4 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+
+// The same query over BVH4 nodes (GpuNode4: 6 float4 of SoA child boxes, then uint4 children). The
+// boxes that are hit are ordered by entry distance with a 5-comparator network; the nearest is
+// visited next and the others are pushed farthest first. One node fetch = 112 bytes.
+template <bool ANY>
+KS_DEV bool traverse4(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, const TravRay &r, float tmax,
+                      TravStack &st, TravHit &hit, uint32_t &n_nodes, uint32_t &n_tris) {
+    const float robust = 1.0000004f; // 1 + 2*gamma(3): conservative far distance (Ize 2013)
+    hit.prim = -1;
+    st.sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        bool exhausted = false;
+        while ((node & kLeafBit) == 0) {
+            ++n_nodes;
+            const float4 *np = nodes + 8 * (size_t)node;
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+            float key[4];
+            uint32_t cp[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float nx = fmaf(r.nx ? lane4(hx, i) : lane4(lx, i), r.inv.x, -r.oinv.x);
+                float fx = fmaf(r.nx ? lane4(lx, i) : lane4(hx, i), r.inv.x, -r.oinv.x);
+                float ny = fmaf(r.ny ? lane4(hy, i) : lane4(ly, i), r.inv.y, -r.oinv.y);
+                float fy = fmaf(r.ny ? lane4(ly, i) : lane4(hy, i), r.inv.y, -r.oinv.y);
+                float nz = fmaf(r.nz ? lane4(hz, i) : lane4(lz, i), r.inv.z, -r.oinv.z);
+                float fz = fmaf(r.nz ? lane4(lz, i) : lane4(hz, i), r.inv.z, -r.oinv.z);
+                float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
+                float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * robust;
+                key[i] = tn <= tf ? tn : INFINITY;
+                cp[i] = lane4(ch, i);
+            }
+            auto cswap = [&](int a, int b) {
+                bool sw = key[b] < key[a];
+                float ka = key[a], kb = key[b];
+                uint32_t ca = cp[a], cb = cp[b];
+                key[a] = sw ? kb : ka;
+                key[b] = sw ? ka : kb;
+                cp[a] = sw ? cb : ca;
+                cp[b] = sw ? ca : cb;
+            };
+            cswap(0, 1);
+            cswap(2, 3);
+            cswap(0, 2);
+            cswap(1, 3);
+            cswap(1, 2);
+            if (key[0] == INFINITY) {
+                if (st.sp == 0) {
+                    exhausted = true;
+                    break;
+                }
+                node = st.pop();
+            } else {
+                if (key[3] != INFINITY) st.push(cp[3]);
+                if (key[2] != INFINITY) st.push(cp[2]);
+                if (key[1] != INFINITY) st.push(cp[1]);
+                node = cp[0];
+            }
+        }
+        if (exhausted) break;
+        uint32_t first = (node >> 4) & 0x7ffffffu;
+        uint32_t cnt = (node & 15u) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            ++n_tris;
+            const float4 *tp = tris + 3 * (size_t)(first + k);
+            float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+            float t, u, v;
+            if (isect_tri(r, v0, v1, v2, tmax, t, u, v)) {
+                if (ANY) {
+                    hit.prim = __float_as_int(v0.w);
+                    return true;
+                }
+                tmax = t;
+                hit.t = t;
+                hit.u = u;
+                hit.v = v;
+                hit.prim = __float_as_int(v0.w);
+            }
+        }
+        if (st.sp == 0) break;
+        node = st.pop();
+    }
+    return hit.prim >= 0;
+}
+
+// Dispatch on the node width of the scene (2 or 4).
+template <bool ANY, int W>
+KS_DEV bool traverse_w(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, const TravRay &r, float tmax,
+                       TravStack &st, TravHit &hit, uint32_t &n_nodes, uint32_t &n_tris) {
+    if (W == 4) return traverse4<ANY>(nodes, tris, r, tmax, st, hit, n_nodes, n_tris);
+    return traverse<ANY>(nodes, tris, r, tmax, st, hit, n_nodes, n_tris);
 }
 
 } // namespace ksd

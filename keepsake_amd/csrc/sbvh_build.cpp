@@ -520,7 +520,92 @@ BvhResult build_sbvh(const float *vertices, const uint32_t *indices, uint32_t n_
         f.emit(root, leaf_refs, 0);
     }
     res.max_depth = f.max_depth;
+    res.nodes4 = collapse_bvh4(res.nodes, res.max_depth4);
     return res;
+}
+
+namespace {
+
+struct Child2 {
+    float lo[3], hi[3];
+    uint32_t ptr; // BVH2 child pointer
+    bool interior() const { return (ptr & kLeafBit) == 0; }
+    float area() const {
+        float e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        if (!(e[0] >= 0.0f && e[1] >= 0.0f && e[2] >= 0.0f)) return -1.0f; // empty box
+        return e[0] * e[1] + e[1] * e[2] + e[2] * e[0];
+    }
+};
+
+Child2 child_of(const GpuNode &n, int slot) {
+    Child2 c;
+    c.lo[0] = n.f[4 * slot];
+    c.hi[0] = n.f[4 * slot + 1];
+    c.lo[1] = n.f[4 * slot + 2];
+    c.hi[1] = n.f[4 * slot + 3];
+    c.lo[2] = n.f[8 + 2 * slot];
+    c.hi[2] = n.f[9 + 2 * slot];
+    c.ptr = n.child[slot];
+    return c;
+}
+
+struct Collapser {
+    const std::vector<GpuNode> &in;
+    std::vector<GpuNode4> &out;
+    uint32_t max_depth = 0;
+
+    uint32_t emit(uint32_t n2, uint32_t depth) {
+        std::vector<Child2> ch{child_of(in[n2], 0), child_of(in[n2], 1)};
+        while (ch.size() < 4) {
+            int best = -1;
+            float best_area = -1.0f;
+            for (size_t i = 0; i < ch.size(); ++i)
+                if (ch[i].interior() && ch[i].area() > best_area) {
+                    best = (int)i;
+                    best_area = ch[i].area();
+                }
+            if (best < 0) break;
+            const GpuNode &g = in[ch[best].ptr];
+            ch[best] = child_of(g, 0);
+            ch.insert(ch.begin() + best + 1, child_of(g, 1));
+        }
+        uint32_t ni = (uint32_t)out.size();
+        out.emplace_back();
+        uint32_t ptr[4];
+        for (size_t i = 0; i < ch.size(); ++i) {
+            if (ch[i].interior()) {
+                ptr[i] = emit(ch[i].ptr, depth + 1);
+            } else {
+                ptr[i] = ch[i].ptr;
+                max_depth = std::max(max_depth, depth + 1);
+            }
+        }
+        GpuNode4 &n = out[ni];
+        const float inf = std::numeric_limits<float>::infinity();
+        for (int i = 0; i < 4; ++i) {
+            bool used = i < (int)ch.size();
+            n.lox[i] = used ? ch[i].lo[0] : inf;
+            n.hix[i] = used ? ch[i].hi[0] : -inf;
+            n.loy[i] = used ? ch[i].lo[1] : inf;
+            n.hiy[i] = used ? ch[i].hi[1] : -inf;
+            n.loz[i] = used ? ch[i].lo[2] : inf;
+            n.hiz[i] = used ? ch[i].hi[2] : -inf;
+            n.child[i] = used ? ptr[i] : kLeafBit;
+            n.pad[i] = 0;
+        }
+        return ni;
+    }
+};
+
+} // namespace
+
+std::vector<GpuNode4> collapse_bvh4(const std::vector<GpuNode> &nodes, uint32_t &max_depth4) {
+    std::vector<GpuNode4> out;
+    out.reserve(nodes.size() / 2 + 1);
+    Collapser c{nodes, out};
+    if (!nodes.empty()) c.emit(0, 0);
+    max_depth4 = c.max_depth;
+    return out;
 }
 
 } // namespace ksa

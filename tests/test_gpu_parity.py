@@ -127,8 +127,10 @@ def test_closest_hit_primary_spheres(device, ks, ora, spheres_small):
     sc.close()
 
 
-def test_random_rays_and_occlusion(device, ks, ora, spheres_small):
-    sc = ks.Scene(device, spheres_small)
+@pytest.mark.parametrize("branching", [4, 2])
+def test_random_rays_and_occlusion(device, ks, ora, spheres_small, branching):
+    sc = ks.Scene(device, spheres_small, bvh={"branching": branching})
+    assert sc.info()["branching"] == branching
     osc = ora.OracleScene.from_host_scene(spheres_small)
     rng = np.random.default_rng(7)
     n = 100000
@@ -282,3 +284,28 @@ def test_render_cornell_matches_oracle(device, ks, ora, cornell):
     assert st["nonfinite"] == 0
     film.close()
     sc.close()
+
+
+def test_bvh4_matches_bvh2(device, ks, cornell):
+    """The collapsed BVH4 and the BVH2 it comes from report the same closest hits (ties on equal t aside)
+    and the same occlusion on rays through the room's 200k triangles."""
+    hs = ks.procedural_scene("room", 3)
+    s4 = ks.Scene(device, hs)
+    s2 = ks.Scene(device, hs, bvh={"branching": 2})
+    i4, i2 = s4.info(), s2.info()
+    assert i4["branching"] == 4 and i2["branching"] == 2 and i4["n_refs"] == i2["n_refs"]
+    assert i4["max_depth"] < i2["max_depth"]
+    rng = np.random.default_rng(11)
+    n = 200000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = rng.uniform([0.1, 0.1, 0.1], [9.9, 3.9, 9.9], (n, 3))
+    d = rng.normal(size=(n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.where(rng.uniform(size=n) < 0.5, np.inf, rng.uniform(0.1, 8.0, n))
+    a, b = s4.intersect(rays), s2.intersect(rays)
+    agree, trel = _compare_hits(a, b)
+    print("BVH4 vs BVH2 agree", agree, "trel", trel, flush=True)
+    assert agree >= 0.99999 and trel == 0.0
+    assert np.array_equal(s4.occluded(rays), s2.occluded(rays))
+    s4.close()
+    s2.close()
