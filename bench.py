@@ -132,6 +132,52 @@ def time_to_relmse(ks, dev, args) -> dict:
     return res
 
 
+def batched_views(ks, dev, args) -> dict:
+    """48 views of the Cornell box at seed-drawn resolutions and spp (12 bounces) in one batched
+    wavefront, against the same views rendered one at a time."""
+    hs = ks.procedural_scene("cornell", 0)
+    sc = ks.Scene(dev, hs)
+    rng = np.random.default_rng(args.batch_seed)
+    res_choices = [(64, 64), (128, 128), (256, 144), (512, 512), (1024, 576), (1024, 1024)]
+    views = []
+    for k in range(args.batch_views):
+        w, h = res_choices[rng.integers(len(res_choices))]
+        spp = int(rng.choice([4, 16, 64]))
+        cam = dict(hs.camera)
+        cam["pos"] = (0.5 + rng.uniform(-0.2, 0.2), 0.5 + rng.uniform(-0.2, 0.2), 2.4 + rng.uniform(-0.3, 0.3))
+        cam["aspect"] = w / h
+        p = ks.small_pt_params(bounces=12, render_width=w, render_height=h, spp=spp, spp_prog_interval=spp,
+                               rng_seed=k)
+        views.append((cam, p, ks.Film(dev, w, h)))
+    paths = sum(v[1].render_width * v[1].render_height * v[1].spp for v in views)
+    st = ks.small_pt_render_batch(sc, views)  # first call autotunes (cached afterwards)
+    tune_ms = st["wall_ms"]
+    for _, _, f in views:
+        f.clear()
+    t0 = time.perf_counter()
+    st = ks.small_pt_render_batch(sc, views)
+    batch_s = time.perf_counter() - t0
+    for _, _, f in views:
+        f.clear()
+    t0 = time.perf_counter()
+    st_e = ks.small_pt_render_batch(sc, views, queue_size=st["queue_size"],
+                                    trace_blocks_per_cu=st["trace_blocks_per_cu"], graph=False)
+    eager_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for cam, p, f in views:
+        f.clear()
+        ks.small_pt_render(sc, cam, p, f)
+    single_s = time.perf_counter() - t0
+    for _, _, f in views:
+        f.close()
+    sc.close()
+    return {"views": len(views), "paths": paths, "seconds": round(batch_s, 4),
+            "mpaths_per_s": round(paths / batch_s / 1e6, 1), "queue_size": st["queue_size"],
+            "trace_blocks_per_cu": st["trace_blocks_per_cu"], "first_call_with_autotune_ms": round(tune_ms, 1),
+            "eager_launches_seconds": round(eager_s, 4), "one_render_per_view_seconds": round(single_s, 4),
+            "speedup_vs_one_render_per_view": round(single_s / batch_s, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +204,9 @@ def main() -> int:
     ap.add_argument("--relmse-target", type=float, default=0.01)
     ap.add_argument("--guide-fraction", type=float, default=0.3)
     ap.add_argument("--train-spp", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 48-view leg")
+    ap.add_argument("--batch-views", type=int, default=48)
+    ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -262,6 +311,13 @@ def main() -> int:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(hs, cam, kw, args.cpu_rows, threads, args.cpu_seconds)
 
+    batch = None
+    if not args.no_batch:
+        try:
+            batch = batched_views(ks, dev, args)
+        except ks.KsError as e:
+            batch = {"error": str(e)}
+
     ttr = None
     if not args.no_relmse:
         try:
@@ -298,6 +354,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
+        "batched_views": batch,
     }
     print(json.dumps(out), flush=True)
     film.close()

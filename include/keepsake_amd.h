@@ -147,6 +147,8 @@ typedef struct ks_render_stats {
     int32_t iterations;
     int32_t queue_size;
     uint64_t guide_samples;             /* training samples consumed by Adam steps (guided renders) */
+    int32_t trace_blocks_per_cu;        /* traversal blocks per CU of a batched render */
+    int32_t reserved;
 } ks_render_stats;
 
 typedef void (*ks_progress_fn)(const ks_film *film, int32_t spp_finished, void *user);
@@ -191,6 +193,24 @@ int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse
  *      calling thread (small_pt.cpp:66-68). ---- */
 int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params,
                        ks_film *film, ks_progress_fn progress, void *user, ks_render_stats *stats);
+
+/* ---- batched renders: many views of one scene (each its own camera, resolution, crop and spp) in one
+ *      wavefront, iterations replayed from a HIP graph, queue size and traversal occupancy chosen by an
+ *      autotuner. Each view's film equals what ks_small_pt_render gives for that view alone (the same
+ *      Sobol samples, folded in sample order). No reference counterpart: the reference renders one
+ *      frame per task. spp_prog_interval and progress callbacks do not apply. ---- */
+typedef struct ks_batch_view {
+    ks_camera_desc camera;
+    ks_small_pt_params params;
+    ks_film *film;
+} ks_batch_view;
+typedef struct ks_batch_options {
+    int32_t queue_size;          /* 0: autotune over 2^18..2^22 (cached per scene and workload size) */
+    int32_t trace_blocks_per_cu; /* with a fixed queue_size: traversal blocks per CU, 1..8 (8) */
+    int32_t disable_graph;       /* 1: launch every iteration eagerly instead of replaying a HIP graph */
+} ks_batch_options;
+int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_view *views,
+                             const ks_batch_options *opt, ks_render_stats *stats);
 
 /* ---- component entry points (parity tests; same kernels as the render path) ---- */
 /* PTRenderSampler (sobol.h:235-248) streams for n (pixel, sample) pairs; kinds[c] = 1 (next) or

@@ -22,6 +22,7 @@ from .api import (  # noqa: F401
     procedural_scene,
     small_pt_params,
     small_pt_render,
+    small_pt_render_batch,
     small_pt_render_guided,
     vmm_pdf,
     vmm_sample,

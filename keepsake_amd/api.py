@@ -139,6 +139,8 @@ class RenderStats(C.Structure):
         ("iterations", C.c_int32),
         ("queue_size", C.c_int32),
         ("guide_samples", C.c_uint64),
+        ("trace_blocks_per_cu", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -194,6 +196,14 @@ class GuideStats(C.Structure):
     ]
 
 
+class BatchView(C.Structure):
+    _fields_ = [("camera", CameraDesc), ("params", SmallPTParams), ("film", C.c_void_p)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("queue_size", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("disable_graph", C.c_int32)]
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p)
 
 _P = C.c_void_p
@@ -237,6 +247,7 @@ _SIGS = {
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), _P, _P, PROGRESS_FN, _P, C.POINTER(RenderStats)],
         C.c_int,
     ),
+    "ks_small_pt_render_batch": ([_P, C.c_int32, _P, C.POINTER(BatchOptions), C.POINTER(RenderStats)], C.c_int),
     "ks_guide_create": ([_P, C.POINTER(GuideConfig), _P, _P, C.POINTER(_P)], C.c_int),
     "ks_guide_free": ([_P], C.c_int),
     "ks_guide_set_mode": ([_P, C.c_float, C.c_int32], C.c_int),
@@ -649,6 +660,21 @@ def small_pt_render_guided(scene: Scene, cam: dict, params: SmallPTParams, guide
     cb = PROGRESS_FN(lambda f, s, u: progress(film, s)) if progress is not None else PROGRESS_FN()
     _check(scene.lib.ks_small_pt_render_guided(scene.h, C.byref(c), C.byref(params), guide.h, film.h, cb, None,
                                                C.byref(st)))
+    return st.as_dict()
+
+
+def small_pt_render_batch(scene: Scene, views, queue_size: int = 0, trace_blocks_per_cu: int = 0,
+                          graph: bool = True) -> dict:
+    """Many views of one scene in one wavefront: views is a list of (camera dict, SmallPTParams, Film)."""
+    arr = (BatchView * len(views))()
+    for i, (cam, params, film) in enumerate(views):
+        arr[i].camera = camera_desc(cam)
+        arr[i].params = params
+        arr[i].film = film.h
+    o = BatchOptions(queue_size, trace_blocks_per_cu, 0 if graph else 1)
+    st = RenderStats()
+    _check(scene.lib.ks_small_pt_render_batch(scene.h, len(views), C.cast(arr, C.c_void_p), C.byref(o),
+                                              C.byref(st)))
     return st.as_dict()
 
 

@@ -99,6 +99,9 @@ KS_DEV void pixel_of(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &
 
 // Camera::spawn_ray (camera.cpp:129-170) after the box-filter jitter (small_pt.cpp:98-105)
 KS_DEV void camera_ray(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &sob, f3 &o, f3 &d) {
+    // no FMA contraction: the rounding of the projection must not depend on whether the matrix comes
+    // from kernel arguments or from memory (plain and batched renders give the same rays)
+#pragma clang fp contract(off)
     float ox = 0.0f, oy = 0.0f;
     if (rc.spp != 1) {
         float u0, u1;
@@ -133,6 +136,27 @@ KS_DEV void sobol_start(const RenderConsts &rc, Sobol &sob, uint32_t x, uint32_t
     sob.start(x, y, s, dim, rc.seed, rc.log2spp, rc.ndigits);
 }
 
+// Batched renders: the segment holding work item w (last one whose base is <= w), its view, pixel and
+// sample. The result slot of a batched item is w itself.
+KS_DEV uint32_t locate_batch(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &y, uint32_t &s) {
+    int lo = 0, hi = rc.n_segs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rc.segs[mid].x <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    uint4 sg = rc.segs[lo];
+    uint32_t local = w - sg.x;
+    uint32_t sl = local / sg.w;
+    uint32_t p = local - sl * sg.w;
+    s = sg.z + sl;
+    const RenderConsts &v = rc.views[sg.y];
+    uint32_t ry = p / (uint32_t)v.cw;
+    x = (uint32_t)v.cx + (p - ry * (uint32_t)v.cw);
+    y = (uint32_t)v.cy + ry;
+    return sg.y;
+}
+
 // ---------------------------------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------------------------------
@@ -150,6 +174,7 @@ __global__ void k_ctrl_init(Ctrl *c, uint64_t work_end) {
     c->work_end = work_end;
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_raygen(RenderConsts rc, PathBufs cur, const Ctrl *ctrl, uint32_t Q) {
     uint32_t n_active = ctrl->n_active;
     uint64_t wn = ctrl->work_next, we = ctrl->work_end;
@@ -160,11 +185,14 @@ __global__ __launch_bounds__(256) void k_raygen(RenderConsts rc, PathBufs cur, c
     uint32_t w = (uint32_t)(wn + t);
     uint32_t slot = n_active + t;
     uint32_t x, y, s;
-    pixel_of(rc, w, x, y, s);
+    uint32_t view = 0;
+    if (BATCH) view = locate_batch(rc, w, x, y, s);
+    else pixel_of(rc, w, x, y, s);
+    const RenderConsts &vr = BATCH ? rc.views[view] : rc;
     Sobol sob;
-    sobol_start(rc, sob, x, y, s, 0);
+    sobol_start(vr, sob, x, y, s, 0);
     f3 o, d;
-    camera_ray(rc, x, y, sob, o, d);
+    camera_ray(vr, x, y, sob, o, d);
     cur.work[slot] = w;
     cur.state[slot] = (uint32_t)sob.dim;
     cur.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -382,7 +410,7 @@ KS_DEV float mix_pdf(const float4 *m, f3 w) {
 // the learned mixture (probability gs.alpha of the mixture at vertices whose roughness reaches
 // gs.min_roughness; the continuation pdf is alpha q + (1 - alpha) p_bsdf in the estimator and in both
 // MIS weights), and training records of the first kRecVertices vertices of each path.
-template <bool GUIDED>
+template <bool GUIDED, bool BATCH>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, PathBufs cur, PathBufs nxt,
                                                const float4 *hits, Ctrl *ctrl, float4 *sh_o, float4 *sh_d,
                                                uint32_t *sh_tgt, float4 *sh_c, float4 *samp,
@@ -403,6 +431,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
     int bounce = 0;
     Sobol sob;
     uint32_t rec_slot = 0; // record slot + 1 of this vertex (0: none)
+    uint32_t view = 0;
 
     if (active) {
         w = cur.work[i];
@@ -417,8 +446,10 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         int32_t prim = __float_as_int(h.w);
         bounce = (int)(state >> 16) & 0xff;
         uint32_t x, y, s;
-        pixel_of(rc, w, x, y, s);
-        sobol_start(rc, sob, x, y, s, (int)(state & 0xffffu));
+        if (BATCH) view = locate_batch(rc, w, x, y, s);
+        else pixel_of(rc, w, x, y, s);
+        const RenderConsts &vr = BATCH ? rc.views[view] : rc;
+        sobol_start(vr, sob, x, y, s, (int)(state & 0xffffu));
         bool done = false;
         bool miss_sample = false;
 
@@ -428,18 +459,18 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                 const Light &l = sc.lights[sc.sky[k]];
                 f3 Le = mk(l.color[0], l.color[1], l.color[2]);
                 if (bounce == 0) {
-                    if (rc.include_background) L = L + beta * Le;
+                    if (vr.include_background) L = L + beta * Le;
                 } else {
                     float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)sc.sky[k]);
                     float mis = power_heur(pdf_wi_bsdf, sky_pdf(wi) * pr);
                     f3 bu = Le * mis;
                     if (!all_finite(bu)) ++nonfinite;
-                    if (bounce == 1 && rc.clamp_direct > 0.0f) bu = vmin(bu, rc.clamp_direct);
-                    else if (bounce >= 2 && rc.clamp_indirect > 0.0f) bu = vmin(bu, rc.clamp_indirect);
+                    if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
+                    else if (bounce >= 2 && vr.clamp_indirect > 0.0f) bu = vmin(bu, vr.clamp_indirect);
                     L = L + beta * bu;
                 }
             }
-            if (bounce == 0 && !rc.include_background) miss_sample = true;
+            if (bounce == 0 && !vr.include_background) miss_sample = true;
             done = true;
         } else {
             uint4 tri = sc.tri_idx[prim];
@@ -458,13 +489,13 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                         float mis = power_heur(pdf_wi_bsdf, pdf_l * pr);
                         f3 bu = Le * mis;
                         if (!all_finite(bu)) ++nonfinite;
-                        if (bounce == 1 && rc.clamp_direct > 0.0f) bu = vmin(bu, rc.clamp_direct);
-                        else if (bounce >= 2 && rc.clamp_indirect > 0.0f) bu = vmin(bu, rc.clamp_indirect);
+                        if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
+                        else if (bounce >= 2 && vr.clamp_indirect > 0.0f) bu = vmin(bu, vr.clamp_indirect);
                         L = L + beta * bu;
                     }
                 }
             }
-            if (bounce == rc.bounces) {
+            if (bounce == vr.bounces) {
                 done = true;
             } else {
                 f3 wo = -normalized(rd);
@@ -501,12 +532,12 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                             if (GUIDED && alpha_m > 0.0f) pdf_b = alpha_m * mix_pdf(mixp, wi) + (1.0f - alpha_m) * pdf_b;
                             float mis = 1.0f;
                             if (!delta_light)
-                                mis = rc.mis_mode == KS_MIS_CONSISTENT ? power_heur(pdf_l * pr_light, pdf_b)
+                                mis = vr.mis_mode == KS_MIS_CONSISTENT ? power_heur(pdf_l * pr_light, pdf_b)
                                                                        : power_heur(pdf_l, pdf_b);
                             f3 Ld = Le_beta * f * mis / pr_light;
                             if (!all_finite(Ld)) ++nonfinite;
-                            if (bounce == 0 && rc.clamp_direct > 0.0f) Ld = vmin(Ld, rc.clamp_direct);
-                            else if (bounce >= 1 && rc.clamp_indirect > 0.0f) Ld = vmin(Ld, rc.clamp_indirect);
+                            if (bounce == 0 && vr.clamp_direct > 0.0f) Ld = vmin(Ld, vr.clamp_direct);
+                            else if (bounce >= 1 && vr.clamp_indirect > 0.0f) Ld = vmin(Ld, vr.clamp_indirect);
                             shadow = true;
                             sh_origin = spawn_origin(it.p, wi, it.n);
                             sh_dir = wi;
@@ -591,7 +622,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
             }
         }
         survive = !done;
-        if (done) samp[samp_index(rc, w)] = make_float4(L.x, L.y, L.z, miss_sample ? 0.0f : 1.0f);
+        if (done) samp[BATCH ? w : samp_index(rc, w)] = make_float4(L.x, L.y, L.z, miss_sample ? 0.0f : 1.0f);
     }
 
     uint32_t j = wave_append(&ctrl->n_next, survive);
@@ -607,7 +638,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
     if (shadow) {
         sh_o[k] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.0f);
         sh_d[k] = make_float4(sh_dir.x, sh_dir.y, sh_dir.z, sh_tmax);
-        sh_tgt[k] = survive ? j : (0x80000000u | samp_index(rc, w));
+        sh_tgt[k] = survive ? j : (0x80000000u | (BATCH ? w : samp_index(rc, w)));
         sh_c[k] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, __uint_as_float(rec_slot));
     }
     wave_add_stat(stats, kNonFinite, nonfinite);
@@ -640,6 +671,48 @@ __global__ __launch_bounds__(256) void k_film(RenderConsts rc, const float4 *sam
     rgb[3 * p + 2] = b;
     wts[2 * p] = wh;
     wts[2 * p + 1] = wm;
+}
+
+// Film fold of a batched chunk: one thread per (segment, pixel of its view's crop); thread_base[k] is the
+// first thread of segment k, seg_items[k] its item count. Samples are folded in sample order, so every
+// view's film is the same as a render of that view alone.
+__global__ __launch_bounds__(256) void k_film_batch(const RenderConsts *views, const uint4 *segs,
+                                                    const uint32_t *thread_base, const uint32_t *seg_items, int n_segs,
+                                                    uint32_t n_threads, const float4 *samp, double *const *rgbs,
+                                                    double *const *wtss) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_threads) return;
+    int lo = 0, hi = n_segs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (thread_base[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    uint4 sg = segs[lo];
+    uint32_t p = t - thread_base[lo];
+    uint32_t nb = seg_items[lo] / sg.w;
+    const RenderConsts &v = views[sg.y];
+    uint32_t ry = p / (uint32_t)v.cw;
+    size_t px = (size_t)((uint32_t)v.cy + ry) * (uint32_t)v.W + (uint32_t)v.cx + (p - ry * (uint32_t)v.cw);
+    double *rgb = rgbs[sg.y], *wts = wtss[sg.y];
+    double r = rgb[3 * px], g = rgb[3 * px + 1], b = rgb[3 * px + 2];
+    double wh = wts[2 * px], wm = wts[2 * px + 1];
+    for (uint32_t k = 0; k < nb; ++k) {
+        float4 v4 = samp[(size_t)sg.x + (size_t)k * sg.w + p];
+        if (v4.w != 0.0f) {
+            wh += 1.0;
+            r += (double)v4.x;
+            g += (double)v4.y;
+            b += (double)v4.z;
+        } else {
+            wm += 1.0;
+        }
+    }
+    rgb[3 * px] = r;
+    rgb[3 * px + 1] = g;
+    rgb[3 * px + 2] = b;
+    wts[2 * px] = wh;
+    wts[2 * px + 1] = wm;
 }
 
 // relMSE of the composites of two films, mean over pixels and channels of (x - r)^2 / (r^2 + 0.01),
@@ -896,6 +969,11 @@ struct ks_device {
     int n_cu = 0;
     Workspace ws;
     GuideWork gw;
+    // last batched-render autotuning result: scene, workload size -> queue size, traversal blocks per CU
+    const ks_scene *tune_scene = nullptr;
+    uint64_t tune_items = 0;
+    uint32_t tune_q = 0;
+    int tune_bpc = 0;
     Ctrl *host_ctrl = nullptr; // pinned
     uint8_t *host_scratch = nullptr; // pinned, 64 bytes
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
@@ -987,9 +1065,11 @@ template <int MODE, typename... A> void launch_anyhit(int width, dim3 g, hipStre
     else hipLaunchKernelGGL((k_anyhit<MODE, 2>), g, dim3(kTraceBlock), 0, s, args...);
 }
 
-int trace_grid(const ks_device *dev, uint32_t n) {
+constexpr int kMaxTraceBlocksPerCu = 8; // the traversal overflow area is sized for this many
+
+int trace_grid(const ks_device *dev, uint32_t n, int blocks_per_cu = kMaxTraceBlocksPerCu) {
     // 8 blocks of 256 lanes per CU keep every SIMD at 8 waves; the grid-stride loop covers the rest
-    uint32_t max_blocks = (uint32_t)std::max(1, dev->n_cu) * 8u;
+    uint32_t max_blocks = (uint32_t)std::max(1, dev->n_cu) * (uint32_t)std::min(blocks_per_cu, kMaxTraceBlocksPerCu);
     uint32_t need = (n + kTraceBlock - 1) / kTraceBlock;
     return (int)std::max(1u, std::min(max_blocks, need));
 }
@@ -1015,7 +1095,7 @@ int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_sta
     KS_HIP(ws.samp.ensure(sizeof(float4) * std::max<size_t>(n_samp, 1)));
     KS_HIP(ws.ctrl.ensure(sizeof(Ctrl)));
     KS_HIP(ws.stats.ensure(sizeof(unsigned long long) * kStatCount));
-    uint32_t grid = (uint32_t)std::max(1, dev->n_cu) * 8u * kTraceBlock;
+    uint32_t grid = (uint32_t)std::max(1, dev->n_cu) * (uint32_t)kMaxTraceBlocksPerCu * kTraceBlock;
     uint32_t cap = max_stack + 4 > (uint32_t)kStackLds ? max_stack + 4 - kStackLds : 1u;
     if (ws.ovf_grid < grid || ws.ovf_cap < cap) {
         KS_HIP(ws.ovf.alloc(sizeof(uint32_t) * (size_t)grid * cap));
@@ -1065,9 +1145,19 @@ struct GuidePass {
     GuideShade shade{};
 };
 
+// Options of one wavefront run: batched work mapping, HIP-graph replay of the iterations, traversal
+// blocks per CU.
+struct WavefrontOpts {
+    bool batch = false;
+    bool graph = false;
+    int blocks_per_cu = kMaxTraceBlocksPerCu;
+};
+
 // Runs the wavefront loop over `total` work items (mapped by rc), leaving per-sample results in ws.samp.
+// With opts.graph, two iterations (buffer 0 -> 1 and 1 -> 0) are captured once into a HIP graph and the
+// graph is replayed; the kernels read their counts from the control block, so every replay is valid.
 int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Timer &tm, ks_render_stats *stats,
-                  const GuidePass *gp = nullptr) {
+                  const GuidePass *gp = nullptr, const WavefrontOpts &wo = WavefrontOpts{}) {
     ks_device *dev = sc->dev;
     Workspace &ws = dev->ws;
     hipStream_t s = dev->stream;
@@ -1079,18 +1169,17 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
     for (int b = 0; b < 2; ++b)
         pb[b] = PathBufs{ws.work[b].as<uint32_t>(), ws.state[b].as<uint32_t>(), ws.beta[b].as<float4>(),
                          ws.L[b].as<float4>(),     ws.ro[b].as<float4>(),     ws.rd[b].as<float4>()};
-    int cur = 0;
     const int tblocks = (int)((Q + 255) / 256);
-    const int tgrid = trace_grid(dev, Q);
-    int iterations = 0;
-    const int check_every = 8; // empty trailing iterations cost only launch latency
-    while (true) {
-        hipEvent_t e;
-        e = tm.begin(KS_K_RAYGEN);
-        hipLaunchKernelGGL(k_raygen, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
+    const int tgrid = trace_grid(dev, Q, wo.blocks_per_cu);
+    const bool graph = wo.graph && !gp;
+    // one iteration reading buffer cur and writing buffer 1 - cur; no events while a graph is captured
+    auto iteration = [&](int cur) -> int {
+        hipEvent_t e = graph ? nullptr : tm.begin(KS_K_RAYGEN);
+        if (wo.batch) hipLaunchKernelGGL(k_raygen<true>, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
+        else hipLaunchKernelGGL(k_raygen<false>, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
         hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q);
         tm.end(e);
-        e = tm.begin(KS_K_CLOSEST);
+        e = graph ? nullptr : tm.begin(KS_K_CLOSEST);
         launch_closest(sc->ds.width, dim3(tgrid), s, sc->ds, (const uint32_t *)&ctrl->n_active, 0u,
                        (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, ws.hit.as<float4>(),
                        ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
@@ -1102,17 +1191,21 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             tm.end(e);
             if (ge) return ge;
         }
-        e = tm.begin(KS_K_SHADE);
+        e = graph ? nullptr : tm.begin(KS_K_SHADE);
         if (gp)
-            hipLaunchKernelGGL(k_shade<true>, dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
+            hipLaunchKernelGGL((k_shade<true, false>), dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
                                ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
                                ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, gp->shade);
-        else
-            hipLaunchKernelGGL(k_shade<false>, dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
+        else if (wo.batch)
+            hipLaunchKernelGGL((k_shade<false, true>), dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
                                ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
                                ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, GuideShade{});
+        else
+            hipLaunchKernelGGL((k_shade<false, false>), dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur],
+                               pb[1 - cur], ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
+                               ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, GuideShade{});
         tm.end(e);
-        e = tm.begin(KS_K_SHADOW);
+        e = graph ? nullptr : tm.begin(KS_K_SHADOW);
         launch_anyhit<1>(sc->ds.width, dim3(tgrid), s, sc->ds, (const uint32_t *)&ctrl->n_shadow, 0u,
                          (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), (uint8_t *)nullptr,
                          (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[1 - cur].L,
@@ -1120,19 +1213,70 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                          ws.ovf_cap, st);
         hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
         tm.end(e);
-        KS_HIP(hipGetLastError());
-        cur = 1 - cur;
-        ++iterations;
+        return KS_OK;
+    };
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t g = nullptr;
+    if (graph) {
+        KS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int e0 = iteration(0);
+        int e1 = iteration(1);
+        hipError_t ce = hipStreamEndCapture(s, &g);
+        if (e0 || e1) {
+            if (g) (void)hipGraphDestroy(g);
+            return e0 ? e0 : e1;
+        }
+        KS_HIP(ce);
+        hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            KS_HIP(ie);
+        }
+    }
+    int cur = 0;
+    int iterations = 0;
+    const int check_every = 8; // empty trailing iterations cost only launch latency
+    int status = KS_OK;
+    while (true) {
+        if (graph) {
+            hipError_t le = hipGraphLaunch(exec, s);
+            if (le != hipSuccess) {
+                ksa::set_error(std::string("hipGraphLaunch failed: ") + hipGetErrorString(le));
+                status = KS_ERR_HIP;
+                break;
+            }
+            iterations += 2;
+        } else {
+            status = iteration(cur);
+            if (status) break;
+            hipError_t le = hipGetLastError();
+            if (le != hipSuccess) {
+                ksa::set_error(std::string("wavefront launch failed: ") + hipGetErrorString(le));
+                status = KS_ERR_HIP;
+                break;
+            }
+            cur = 1 - cur;
+            ++iterations;
+        }
         if (iterations % check_every == 0) {
-            KS_HIP(hipMemcpyAsync(dev->host_ctrl, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s));
-            KS_HIP(hipStreamSynchronize(s));
+            hipError_t me = hipMemcpyAsync(dev->host_ctrl, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s);
+            if (me == hipSuccess) me = hipStreamSynchronize(s);
+            if (me != hipSuccess) {
+                ksa::set_error(std::string("wavefront control read failed: ") + hipGetErrorString(me));
+                status = KS_ERR_HIP;
+                break;
+            }
             if (dev->host_ctrl->n_active == 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) break;
             if (iterations > 1000000) {
                 ksa::set_error("wavefront loop did not terminate");
-                return KS_ERR_INVALID;
+                status = KS_ERR_INVALID;
+                break;
             }
         }
     }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+    if (status) return status;
     if (stats) stats->iterations += iterations;
     return KS_OK;
 }
@@ -1730,6 +1874,183 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
     if (stats) {
         stats->paths = pixels * (uint64_t)params->spp;
         stats->queue_size = (int32_t)Q;
+        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return KS_OK;
+}
+
+namespace {
+
+// One chunk of a batched render: segments (base item, view, first sample, pixels) covering at most the
+// per-sample result budget, plus the film fold's thread bases and per-segment item counts.
+struct BatchChunk {
+    std::vector<uint4> segs;
+    std::vector<uint32_t> thread_base, seg_items;
+    uint64_t items = 0;
+    uint32_t threads = 0;
+};
+
+std::vector<BatchChunk> schedule_batch(const std::vector<RenderConsts> &views, const std::vector<int> &spp,
+                                       uint64_t budget) {
+    std::vector<BatchChunk> chunks(1);
+    for (size_t v = 0; v < views.size(); ++v) {
+        uint64_t P = (uint64_t)views[v].cw * views[v].ch;
+        int s0 = 0;
+        while (s0 < spp[v]) {
+            BatchChunk &c = chunks.back();
+            uint64_t room = budget - c.items;
+            int nb = (int)std::min<uint64_t>((uint64_t)(spp[v] - s0), room / P);
+            if (nb == 0) {
+                chunks.emplace_back();
+                continue;
+            }
+            c.segs.push_back(make_uint4((uint32_t)c.items, (uint32_t)v, (uint32_t)s0, (uint32_t)P));
+            c.seg_items.push_back((uint32_t)(P * (uint64_t)nb));
+            c.thread_base.push_back(c.threads);
+            c.threads += (uint32_t)P;
+            c.items += P * (uint64_t)nb;
+            s0 += nb;
+        }
+    }
+    if (chunks.back().segs.empty()) chunks.pop_back();
+    return chunks;
+}
+
+} // namespace
+
+int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_view *views, const ks_batch_options *opt,
+                             ks_render_stats *stats) {
+    KS_REQUIRE(scene && views && n_views > 0 && n_views <= 65536, "invalid views");
+    ks_device *dev = scene->dev;
+    KS_HIP(hipSetDevice(dev->ordinal));
+    auto t0 = std::chrono::steady_clock::now();
+    ks_batch_options o{};
+    if (opt) o = *opt;
+    std::vector<RenderConsts> hv((size_t)n_views);
+    std::vector<int> spp((size_t)n_views);
+    std::vector<double *> rgbs((size_t)n_views), wtss((size_t)n_views);
+    uint64_t paths = 0, max_pixels = 0;
+    for (int v = 0; v < n_views; ++v) {
+        const ks_batch_view &bv = views[v];
+        KS_REQUIRE(bv.film && bv.film->dev == dev, "every view needs a film on the scene's device");
+        KS_REQUIRE(bv.film->w == bv.params.render_width && bv.film->h == bv.params.render_height,
+                   "film size differs from render_width x render_height");
+        int e = fill_consts(scene, &bv.camera, &bv.params, hv[v]);
+        if (e) return e;
+        spp[v] = bv.params.spp;
+        rgbs[v] = bv.film->rgb.as<double>();
+        wtss[v] = bv.film->wts.as<double>();
+        uint64_t P = (uint64_t)hv[v].cw * hv[v].ch;
+        paths += P * (uint64_t)spp[v];
+        max_pixels = std::max(max_pixels, P);
+    }
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const uint64_t budget = (1ull << 30) / sizeof(float4); // per-sample results of one chunk: 1 GiB
+    KS_REQUIRE(max_pixels <= budget, "a view is too large for one chunk");
+    std::vector<BatchChunk> chunks = schedule_batch(hv, spp, budget);
+    uint64_t max_items = 0;
+    for (auto &c : chunks) max_items = std::max(max_items, c.items);
+    HostToDev<RenderConsts> dviews;
+    HostToDev<double *> drgb, dwts;
+    KS_HIP(dviews.put(hv.data(), hv.size(), dev->stream));
+    KS_HIP(drgb.put(rgbs.data(), rgbs.size(), dev->stream));
+    KS_HIP(dwts.put(wtss.data(), wtss.size(), dev->stream));
+
+    WavefrontOpts wo;
+    wo.batch = true;
+    wo.graph = o.disable_graph == 0;
+    uint32_t Q = 0;
+    // autotuner: queue size 2^18..2^22 x traversal blocks per CU {4, 8}, timed on the first chunk's first
+    // items (at most 2^24), cached per scene and workload size
+    if (o.queue_size > 0) {
+        Q = (uint32_t)std::min<int64_t>(o.queue_size, 1 << 26);
+        wo.blocks_per_cu = o.trace_blocks_per_cu > 0 ? o.trace_blocks_per_cu : kMaxTraceBlocksPerCu;
+    } else if (dev->tune_scene == scene && dev->tune_items == paths) {
+        Q = dev->tune_q;
+        wo.blocks_per_cu = dev->tune_bpc;
+    } else {
+        const BatchChunk &c0 = chunks.front();
+        BatchChunk probe;
+        for (size_t k = 0; k < c0.segs.size() && probe.items < (1ull << 24); ++k) {
+            probe.segs.push_back(c0.segs[k]);
+            probe.items += c0.seg_items[k];
+        }
+        HostToDev<uint4> psegs;
+        KS_HIP(psegs.put(probe.segs.data(), probe.segs.size(), dev->stream));
+        RenderConsts prc = hv[0];
+        prc.views = dviews.p();
+        prc.segs = psegs.p();
+        prc.n_segs = (int32_t)probe.segs.size();
+        hipEvent_t ea, eb;
+        KS_HIP(hipEventCreate(&ea));
+        KS_HIP(hipEventCreate(&eb));
+        float best = 1e30f;
+        const int bpcs[2] = {4, 8};
+        for (int lq = 18; lq <= 22; ++lq) {
+            uint32_t q = (uint32_t)std::min<uint64_t>(1ull << lq, std::max<uint64_t>(probe.items, 256));
+            for (int bpc : bpcs) {
+                int e = ensure_workspace(dev, q, probe.items, scene->max_stack);
+                if (e) return e;
+                WavefrontOpts pw = wo;
+                pw.blocks_per_cu = bpc;
+                Timer ptm{dev};
+                int saved = dev->timing;
+                dev->timing = 0;
+                KS_HIP(hipEventRecord(ea, dev->stream));
+                e = run_wavefront(scene, prc, probe.items, q, ptm, nullptr, nullptr, pw);
+                KS_HIP(hipEventRecord(eb, dev->stream));
+                dev->timing = saved;
+                if (e) return e;
+                KS_HIP(hipEventSynchronize(eb));
+                float ms = 0.0f;
+                KS_HIP(hipEventElapsedTime(&ms, ea, eb));
+                if (ms < best) {
+                    best = ms;
+                    Q = q;
+                    wo.blocks_per_cu = bpc;
+                }
+            }
+            if ((1ull << lq) >= probe.items) break; // larger queues run the same work
+        }
+        (void)hipEventDestroy(ea);
+        (void)hipEventDestroy(eb);
+        dev->tune_scene = scene;
+        dev->tune_items = paths;
+        dev->tune_q = Q;
+        dev->tune_bpc = wo.blocks_per_cu;
+    }
+    int e = ensure_workspace(dev, Q, max_items, scene->max_stack);
+    if (e) return e;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                       (int)kStatCount);
+    Timer tm{dev};
+    for (const BatchChunk &c : chunks) {
+        HostToDev<uint4> dsegs;
+        HostToDev<uint32_t> dtb, dsi;
+        KS_HIP(dsegs.put(c.segs.data(), c.segs.size(), dev->stream));
+        KS_HIP(dtb.put(c.thread_base.data(), c.thread_base.size(), dev->stream));
+        KS_HIP(dsi.put(c.seg_items.data(), c.seg_items.size(), dev->stream));
+        RenderConsts rc = hv[0];
+        rc.views = dviews.p();
+        rc.segs = dsegs.p();
+        rc.n_segs = (int32_t)c.segs.size();
+        e = run_wavefront(scene, rc, c.items, Q, tm, stats, nullptr, wo);
+        if (e) return e;
+        hipEvent_t ev = tm.begin(KS_K_FILM);
+        hipLaunchKernelGGL(k_film_batch, dim3((c.threads + 255) / 256), dim3(256), 0, dev->stream, dviews.p(), dsegs.p(),
+                           dtb.p(), dsi.p(), (int)c.segs.size(), c.threads, dev->ws.samp.as<float4>(), drgb.p(),
+                           dwts.p());
+        tm.end(ev);
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipStreamSynchronize(dev->stream)); // segment tables of this chunk are freed next
+    }
+    tm.collect(stats);
+    e = read_stats(dev, stats);
+    if (e) return e;
+    if (stats) {
+        stats->paths = paths;
+        stats->queue_size = (int32_t)Q;
+        stats->trace_blocks_per_cu = wo.blocks_per_cu;
         stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return KS_OK;

@@ -48,6 +48,11 @@ struct RenderConsts {
     float clamp_direct, clamp_indirect;
     int32_t s0, nb;            // sample range of the batch: [s0, s0 + nb)
     const int4 *work_list;     // optional explicit (x, y, sample) per work item
+    // batched renders: work items are segments (base item, view, first sample, pixels of the view's
+    // crop), sample-major inside a segment; every view has its own constants
+    const struct RenderConsts *views;
+    const uint4 *segs;
+    int32_t n_segs;
     float p2w[16]; // proj_to_world with the eye folded into rows 0..2 (see camera_ray)
     float cam[3];
 };
