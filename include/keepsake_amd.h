@@ -75,7 +75,11 @@ typedef struct ks_scene_desc {
     uint32_t n_materials;
     const ks_light_desc *lights;
     uint32_t n_lights;
+    const uint8_t *geom_flags; /* per geometry (KS_GEOM_*), or NULL: two-sided, smooth normals when given */
 } ks_scene_desc;
+
+#define KS_GEOM_ONE_SIDED 1u /* MeshData::twosided = false: the frame is not flipped towards wo (geometry.cpp:126) */
+#define KS_GEOM_FLAT 2u      /* MeshData::use_smooth_normal = false: vertex normals ignored (geometry.cpp:135) */
 
 /* SBVHBuildOption (cuda/sbvh.cuh:9-16). Zeroed fields take the reference defaults
  * (maxPrimsInNode 4, bucketCount 12, travWeight 1, alpha 1e-5). */

@@ -48,6 +48,7 @@ class Device {
 struct SceneData {
     std::vector<float> vertices, normals;
     std::vector<uint32_t> indices, tri_geom, geom_material;
+    std::vector<uint8_t> geom_flags; // per geometry KS_GEOM_* bits, empty: two-sided with smooth normals
     std::vector<ks_material_desc> materials;
     std::vector<ks_light_desc> lights;
 
@@ -65,6 +66,7 @@ struct SceneData {
         d.n_materials = (uint32_t)materials.size();
         d.lights = lights.empty() ? nullptr : lights.data();
         d.n_lights = (uint32_t)lights.size();
+        d.geom_flags = geom_flags.empty() ? nullptr : geom_flags.data();
         return d;
     }
 };

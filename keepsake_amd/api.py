@@ -73,7 +73,12 @@ class SceneDesc(C.Structure):
         ("n_materials", C.c_uint32),
         ("lights", C.POINTER(LightDesc)),
         ("n_lights", C.c_uint32),
+        ("geom_flags", C.POINTER(C.c_uint8)),
     ]
+
+
+KS_GEOM_ONE_SIDED = 1
+KS_GEOM_FLAT = 2
 
 
 class BvhOptions(C.Structure):
@@ -339,6 +344,7 @@ class HostScene:
     lights: np.ndarray  # LIGHT_DTYPE
     camera: dict
     normals: np.ndarray | None = None
+    geom_flags: np.ndarray | None = None  # (G,) uint8 of KS_GEOM_* bits, None: two-sided, smooth normals
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -355,7 +361,8 @@ class HostScene:
             np.ascontiguousarray(self.lights, LIGHT_DTYPE),
         ]
         nrm = None if self.normals is None else np.ascontiguousarray(self.normals, np.float32)
-        self._keep = arrs + [nrm]
+        gf = None if self.geom_flags is None else np.ascontiguousarray(self.geom_flags, np.uint8)
+        self._keep = arrs + [nrm, gf]
         v, idx, tg, gm, mats, lights = arrs
         d = SceneDesc()
         d.vertices = v.ctypes.data_as(C.POINTER(C.c_float))
@@ -370,6 +377,7 @@ class HostScene:
         d.n_materials = mats.shape[0]
         d.lights = C.cast(lights.ctypes.data, C.POINTER(LightDesc)) if lights.shape[0] else None
         d.n_lights = lights.shape[0]
+        d.geom_flags = gf.ctypes.data_as(C.POINTER(C.c_uint8)) if gf is not None else None
         return d
 
 

@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
             done = true;
         } else {
             uint4 tri = sc.tri_idx[prim];
-            const Material &m = sc.mats[tri.w];
+            const Material &m = sc.mats[tri_material(tri)];
             SurfHit it = make_hit(sc, tri, h.y, h.z, rd);
             if (m.has_emission) { // small_pt.cpp:111-138
                 int32_t li = sc.tri_light[prim];
@@ -1385,6 +1385,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     *out = nullptr;
     KS_REQUIRE(d->n_tris == 0 || (d->vertices && d->indices && d->tri_geom), "scene arrays missing");
     KS_REQUIRE(d->n_materials > 0 && d->materials, "scene needs at least one material");
+    KS_REQUIRE(d->n_materials < (1u << 24), "at most 2^24 - 1 materials");
     KS_REQUIRE(d->n_lights <= 64, "at most 64 non-mesh lights");
     for (uint32_t t = 0; t < d->n_tris; ++t) {
         for (int k = 0; k < 3; ++k)
@@ -1414,7 +1415,8 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     std::vector<uint4> tri_idx(d->n_tris);
     for (uint32_t t = 0; t < d->n_tris; ++t)
         tri_idx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2],
-                                d->geom_material[d->tri_geom[t]]);
+                                d->geom_material[d->tri_geom[t]] |
+                                    ((uint32_t)(d->geom_flags ? d->geom_flags[d->tri_geom[t]] & 3u : 0u) << 24));
     std::vector<Material> mats(d->n_materials);
     for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(d->materials[m]);
 
@@ -1879,6 +1881,8 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
     return KS_OK;
 }
 
+} // extern "C"
+
 namespace {
 
 // One chunk of a batched render: segments (base item, view, first sample, pixels) covering at most the
@@ -1917,6 +1921,8 @@ std::vector<BatchChunk> schedule_batch(const std::vector<RenderConsts> &views, c
 }
 
 } // namespace
+
+extern "C" {
 
 int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_view *views, const ks_batch_options *opt,
                              ks_render_stats *stats) {

@@ -102,6 +102,9 @@ KS_DEV void wave_add_stat(unsigned long long *stats, int slot, uint32_t v) {
     if (__lane_id() == 0 && v) atomicAdd(&stats[slot], (unsigned long long)v);
 }
 
+// tri.w: material (bits 0..23) | KS_GEOM_* flags of the triangle's geometry (bits 24..31)
+KS_DEV uint32_t tri_material(uint4 tri) { return tri.w & 0xffffffu; }
+
 KS_DEV SurfHit make_hit(const DevScene &sc, uint4 tri, float u, float v, f3 rd) { // geometry.cpp:68-163
     f3 v0 = ld3(sc.verts[tri.x]), v1 = ld3(sc.verts[tri.y]), v2 = ld3(sc.verts[tri.z]);
     f3 wo = -normalized(rd);
@@ -112,7 +115,8 @@ KS_DEV SurfHit make_hit(const DevScene &sc, uint4 tri, float u, float v, f3 rd) 
     f3 dpdu = v1 - v0;
     f3 b = normalized(cross(ng, dpdu));
     f3 t = normalized(cross(b, ng));
-    if (dot(wo, ng) < 0.0f) {
+    const uint32_t flags = tri.w >> 24;
+    if (!(flags & 1u) && dot(wo, ng) < 0.0f) { // two-sided (geometry.cpp:126-132)
         ng = -ng;
         dpdu = -dpdu;
         t = -t;
@@ -123,7 +127,7 @@ KS_DEV SurfHit make_hit(const DevScene &sc, uint4 tri, float u, float v, f3 rd) 
     it.st = t;
     it.sb = b;
     it.sn = ng;
-    if (sc.normals) {
+    if (sc.normals && !(flags & 2u)) {
         f3 vn = normalized(ld3(sc.normals[tri.x]) * w + ld3(sc.normals[tri.y]) * u + ld3(sc.normals[tri.z]) * v);
         if (all_finite(vn)) {
             if (dot(vn, ng) < 0.0f) vn = -vn;

@@ -337,13 +337,21 @@ TomlValue parse_toml(std::string_view text, const std::string &source) {
 }
 
 ConfigArgs ConfigArgs::operator[](std::string_view key) const {
-    return ConfigArgs(&require(node, key), asset_root);
+    return ConfigArgs(&require(node, key), asset_root, doc);
+}
+ConfigArgs ConfigArgs::asset(std::string_view path) const {
+    size_t dot = path.find('.');
+    if (!doc || dot == std::string_view::npos) throw ConfigError("unknown asset '" + std::string(path) + "'");
+    const TomlValue *group = doc->find(path.substr(0, dot));
+    const TomlValue *a = group && group->is_table() ? group->find(path.substr(dot + 1)) : nullptr;
+    if (!a || !a->is_table()) throw ConfigError("unknown asset '" + std::string(path) + "'");
+    return ConfigArgs(a, asset_root, doc);
 }
 bool ConfigArgs::contains(std::string_view key) const { return node && node->find(key) != nullptr; }
 ConfigArgs ConfigArgs::operator[](int idx) const {
     if (!node || !node->is_array() || idx < 0 || (size_t)idx >= node->arr.size())
         throw ConfigError("array index " + std::to_string(idx) + " out of range");
-    return ConfigArgs(&node->arr[idx], asset_root);
+    return ConfigArgs(&node->arr[idx], asset_root, doc);
 }
 size_t ConfigArgs::array_size() const { return node && node->is_array() ? node->arr.size() : 0; }
 
@@ -553,7 +561,7 @@ void ConfigService::run_all_tasks() const { // config.cpp:262-313
         if (!type) throw ConfigError("task without type");
         auto it = tasks.find(type->str);
         if (it == tasks.end()) throw ConfigError("unknown task type '" + type->str + "'");
-        it->second(ConfigArgs(&task_table, asset_root), task_dir, task_id);
+        it->second(ConfigArgs(&task_table, asset_root, &cfg), task_dir, task_id);
         std::printf("Saving output to [%s]\n\n", task_dir.string().c_str());
     }
 }

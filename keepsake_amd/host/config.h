@@ -45,7 +45,8 @@ struct ConfigError : std::runtime_error {
 TomlValue parse_toml(std::string_view text, const std::string &source = "<string>");
 
 struct ConfigArgs {
-    ConfigArgs(const TomlValue *node, fs::path asset_root) : node(node), asset_root(std::move(asset_root)) {}
+    ConfigArgs(const TomlValue *node, fs::path asset_root, const TomlValue *doc = nullptr)
+        : node(node), asset_root(std::move(asset_root)), doc(doc) {}
 
     ConfigArgs operator[](std::string_view key) const;
     bool contains(std::string_view key) const;
@@ -60,9 +61,13 @@ struct ConfigArgs {
     bool load_bool(std::string_view name, const std::optional<bool> &default_value = {}) const;
     std::string load_string(std::string_view name, const std::optional<std::string> &default_value = {}) const;
     fs::path load_path(std::string_view name) const;
+    // A named asset table of the whole document, "prefix.name" -> [prefix.name] (the reference's
+    // ConfigurableTable keys, config.cpp:315-329).
+    ConfigArgs asset(std::string_view path) const;
 
     const TomlValue *node;
     fs::path asset_root;
+    const TomlValue *doc; // the whole configuration, for asset lookups
 };
 
 // (args, task_dir, task_id)

@@ -1,7 +1,12 @@
-// ks_render: the task-runner executable (test/main.cpp:28-84 of the reference) for the gfx950 small_pt.
+// ks_render: the task-runner executable (test/main.cpp:28-84 of the reference) for the gfx950 small_pt,
+// plus the convert_mesh_asset task (OBJ -> binary mesh asset).
 //   ks_render -c scene.toml [-d device] [-r asset_root_dir] [-n threads] [--print-config]
 // --print-config resolves every small_pt task of the file and prints the inputs as JSON lines without
-// opening a GPU.
+// opening a GPU; --dump-scene DIR also writes each resolved scene's flat arrays as raw little-endian
+// files under DIR/task_<id>/ (vertices.f32, normals.f32, indices.u32, tri_geom.u32,
+// geom_material.u32, geom_flags.u8, materials.bin of ks_material_desc records, lights.bin of
+// ks_light_desc records).
+#include "mesh_asset.h"
 #include "small_pt_task.h"
 
 #include <cstdio>
@@ -11,11 +16,30 @@
 #include <string>
 
 static void usage() {
-    std::fprintf(stderr, "usage: ks_render -c config.toml [-d device] [-r asset_root] [-n threads] [--print-config]\n");
+    std::fprintf(stderr, "usage: ks_render -c config.toml [-d device] [-r asset_root] [-n threads] [--print-config] "
+                         "[--dump-scene dir]\n");
+}
+
+template <typename T> static void dump_array(const ks::fs::path &p, const std::vector<T> &v) {
+    std::ofstream f(p, std::ios::binary);
+    f.write(reinterpret_cast<const char *>(v.data()), (std::streamsize)(v.size() * sizeof(T)));
+    if (!f) throw ks::ConfigError("cannot write " + p.string());
+}
+
+static void dump_scene(const ks::SceneData &s, const ks::fs::path &dir) {
+    ks::fs::create_directories(dir);
+    dump_array(dir / "vertices.f32", s.vertices);
+    dump_array(dir / "normals.f32", s.normals);
+    dump_array(dir / "indices.u32", s.indices);
+    dump_array(dir / "tri_geom.u32", s.tri_geom);
+    dump_array(dir / "geom_material.u32", s.geom_material);
+    dump_array(dir / "geom_flags.u8", s.geom_flags);
+    dump_array(dir / "materials.bin", s.materials);
+    dump_array(dir / "lights.bin", s.lights);
 }
 
 int main(int argc, char **argv) {
-    std::string cfg_path, root;
+    std::string cfg_path, root, dump_dir;
     int device = 0;
     bool print_only = false;
     for (int i = 1; i < argc; ++i) {
@@ -33,6 +57,7 @@ int main(int argc, char **argv) {
         else if (a == "-n" || a == "--nthreads") (void)next(); // host threads: the GPU path needs none
         else if (a == "-v" || a == "--validation") (void)next(); // Vulkan validation: not applicable
         else if (a == "--print-config") print_only = true;
+        else if (a == "--dump-scene") print_only = true, dump_dir = next();
         else {
             usage();
             return 2;
@@ -51,19 +76,27 @@ int main(int argc, char **argv) {
             std::stringstream ss;
             ss << f.rdbuf();
             cfg.cfg = ks::parse_toml(ss.str(), cfg_path);
-            for (const auto &t : cfg.task_tables()) {
+            std::vector<ks::TomlValue> tables = cfg.task_tables();
+            for (size_t id = 0; id < tables.size(); ++id) {
+                const ks::TomlValue &t = tables[id];
                 const ks::TomlValue *type = t.find("type");
-                if (type && type->str == "small_pt")
-                    std::printf("%s\n", ks::describe(ks::resolve_small_pt(ks::ConfigArgs(&t, root))).c_str());
+                if (!type || type->str != "small_pt") continue;
+                ks::SmallPTSetup st = ks::resolve_small_pt(ks::ConfigArgs(&t, root, &cfg.cfg));
+                std::printf("%s\n", ks::describe(st).c_str());
+                if (!dump_dir.empty()) dump_scene(st.scene, ks::fs::path(dump_dir) / ("task_" + std::to_string(id)));
             }
             return 0;
         }
         ks::set_task_device(device);
         cfg.parse_file(cfg_path);
         cfg.register_task("small_pt", ks::small_pt);
+        cfg.register_task("convert_mesh_asset", ks::convert_mesh_asset);
         cfg.run_all_tasks();
     } catch (const ks::ConfigError &e) {
         std::fprintf(stderr, "[ks_render] config error: %s\n", e.what());
+        return 1;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "[ks_render] error: %s\n", e.what());
         return 1;
     }
     return 0;

@@ -1,18 +1,26 @@
 // The small_pt task (small_pt.cpp:180-303) on the gfx950 library, with the same TOML keys:
-// sky, light[], pixel_filter, tone_mapper, backdrop, bounces, clamp_direct, clamp_indirect,
-// render_width, render_height, crop_*, spp, scale_ray_diff, rng_seed, spp_prog_interval, camera.
-// Scenes come from this build's procedural asset type (keys `procedural`, `procedural_seed`), since the
-// mesh-asset loaders (mesh_asset.cpp) are not part of this build yet; `object` / `compound_object`
-// therefore fail with a clear message. Additional keys of this build: mis_mode ("reference" or
-// "consistent"), queue_size, and a [task.guide] table that turns on the learned path guide (keys
-// fraction, train_spp, learning_rate, beta1, beta2, log2_table_size, base_resolution, max_resolution,
-// train_batch, max_steps_per_pass, weight_clamp, seed; it needs mis_mode = "consistent").
+// object, material, sky, light[], pixel_filter, tone_mapper, backdrop, bounces, clamp_direct,
+// clamp_indirect, render_width, render_height, crop_*, spp, scale_ray_diff, rng_seed,
+// spp_prog_interval, camera. A scene is either a mesh asset (`object = "mesh_asset.X"` plus one
+// material list, create_scene scene.cpp:345-365) or one of this build's procedural scenes (keys
+// `procedural`, `procedural_seed`). Materials are [material.*] tables whose `bsdf` names a
+// [bsdf.*] table of type principled_bsdf with constant shader fields (create_material
+// material.cpp:169-177, create_principled_bsdf principled_bsdf.cpp:707-732). `compound_object`
+// (glTF scenes) is rejected with a clear message. Additional keys of this build: `emission` on a
+// material table (a constant shader field or an RGB triple: makes its triangles mesh lights),
+// mis_mode ("reference" or "consistent"), queue_size, and a [task.guide] table that turns on the
+// learned path guide (keys fraction, train_spp, learning_rate, beta1, beta2, log2_table_size,
+// base_resolution, max_resolution, train_batch, max_steps_per_pass, weight_clamp, seed; it needs
+// mis_mode = "consistent").
 #include "small_pt_task.h"
+
+#include "mesh_asset.h"
 
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <map>
 #include <memory>
 
 namespace ks {
@@ -28,6 +36,97 @@ static SceneData scene_from_host(const ks_host_scene *hs, ks_camera_desc *cam) {
     s.geom_material.assign(d.geom_material, d.geom_material + d.n_geoms);
     s.materials.assign(d.materials, d.materials + d.n_materials);
     if (d.lights) s.lights.assign(d.lights, d.lights + d.n_lights);
+    return s;
+}
+
+// A constant shader field (create_constant_shader_field_color, shader_field.h:88-105) of 1 or 3
+// channels; texture fields have no place in the flat material record.
+static std::array<float, 3> load_constant_field(const ConfigArgs &bsdf, const char *key, int channels) {
+    if (!bsdf.contains(key)) throw ConfigError(std::string("principled_bsdf needs '") + key + "'");
+    ConfigArgs f = bsdf[key];
+    std::string type = f.load_string("type");
+    if (type == "texture") throw ConfigError(std::string("texture shader fields are not supported ('") + key + "')");
+    if (type != "constant") throw ConfigError(std::string("unknown shader field type '") + type + "'");
+    if (channels == 1) {
+        float v = f.load_float("value");
+        return {v, v, v};
+    }
+    return f.load_vec3("value");
+}
+
+// [material.X] -> bsdf = "bsdf.Y" of type principled_bsdf (the only BSDF the kernels evaluate).
+static ks_material_desc load_material(const ConfigArgs &mat) {
+    for (const char *k : {"subsurface", "normal_map", "opacity_map"})
+        if (mat.contains(k) && !mat.load_string(k).empty())
+            throw ConfigError(std::string("material key '") + k + "' is not supported");
+    ConfigArgs bsdf = mat.asset(mat.load_string("bsdf"));
+    std::string type = bsdf.load_string("type");
+    if (type != "principled_bsdf") throw ConfigError("bsdf type '" + type + "' is not supported (principled_bsdf only)");
+    if (bsdf.load_string("microfacet", "ggx") != "ggx") throw ConfigError("only the ggx microfacet is supported");
+    ks_material_desc d{};
+    auto c = load_constant_field(bsdf, "basecolor", 3);
+    for (int k = 0; k < 3; ++k) d.basecolor[k] = c[k];
+    d.roughness = load_constant_field(bsdf, "roughness", 1)[0];
+    d.metallic = load_constant_field(bsdf, "metallic", 1)[0];
+    d.ior = load_constant_field(bsdf, "ior", 1)[0];
+    d.specular_r0_mul = load_constant_field(bsdf, "specular_r0_mul", 1)[0];
+    d.specular_trans = load_constant_field(bsdf, "specular_trans", 1)[0];
+    d.diffuse_trans = load_constant_field(bsdf, "diffuse_trans", 1)[0];
+    d.diffuse_trans_fwd = load_constant_field(bsdf, "diffuse_trans_fwd", 1)[0];
+    // the bsdf's `emissive` field is read but, as in the reference's material path, does not make a light
+    if (bsdf.contains("emissive")) (void)load_constant_field(bsdf, "emissive", 3);
+    if (mat.contains("emission")) {
+        std::array<float, 3> e;
+        if (mat["emission"].node->is_table()) e = load_constant_field(mat, "emission", 3);
+        else e = mat.load_vec3("emission");
+        for (int k = 0; k < 3; ++k) d.emission[k] = e[k];
+        d.has_emission = 1;
+    }
+    return d;
+}
+
+// create_scene_from_mesh_asset + assign_material_list (mesh_asset.cpp:267-299): one subscene with one
+// geometry per mesh, an identity instance, material[0][g] for geometry g.
+static SceneData scene_from_mesh_asset(const MeshAsset &asset, const ConfigArgs &args) {
+    if (!args.contains("material"))
+        throw ConfigError("No materials. Provide a material list (material = [[\"material.a\", ...]]).");
+    ConfigArgs list = args["material"];
+    if (list.array_size() != 1) throw ConfigError("Material list mismatch dimension (one subscene).");
+    ConfigArgs names = list[0];
+    if (names.array_size() != asset.meshes.size())
+        throw ConfigError("Material list mismatch dimension: " + std::to_string(names.array_size()) +
+                          " materials for " + std::to_string(asset.meshes.size()) + " meshes.");
+    SceneData s;
+    bool any_normals = false;
+    for (const MeshData &m : asset.meshes) any_normals |= m.use_smooth_normal && !m.vertex_normals.empty();
+    std::map<std::string, uint32_t> mat_index;
+    for (size_t g = 0; g < asset.meshes.size(); ++g) {
+        const MeshData &m = asset.meshes[g];
+        uint32_t base = (uint32_t)(s.vertices.size() / 3);
+        if ((uint64_t)base + m.vertex_count() >= (1ull << 32)) throw ConfigError("mesh asset has too many vertices");
+        s.vertices.insert(s.vertices.end(), m.vertices.begin(), m.vertices.end());
+        bool smooth = m.use_smooth_normal && !m.vertex_normals.empty();
+        if (any_normals) {
+            if (smooth) s.normals.insert(s.normals.end(), m.vertex_normals.begin(), m.vertex_normals.end());
+            else s.normals.insert(s.normals.end(), m.vertices.size(), 0.0f);
+        }
+        for (uint32_t i : m.indices) s.indices.push_back(base + i);
+        s.tri_geom.insert(s.tri_geom.end(), m.tri_count(), (uint32_t)g);
+        uint8_t flags = (m.twosided ? 0u : KS_GEOM_ONE_SIDED) | (smooth ? 0u : KS_GEOM_FLAT);
+        s.geom_flags.push_back(flags);
+        const TomlValue *nv = names[(int)g].node;
+        if (nv->type != TomlValue::Type::String) throw ConfigError("material list entries must be strings");
+        auto it = mat_index.find(nv->str);
+        if (it == mat_index.end()) {
+            it = mat_index.emplace(nv->str, (uint32_t)s.materials.size()).first;
+            s.materials.push_back(load_material(args.asset(nv->str)));
+        }
+        s.geom_material.push_back(it->second);
+    }
+    bool any_flag = false;
+    for (uint8_t f : s.geom_flags) any_flag |= f != 0;
+    if (!any_flag) s.geom_flags.clear();
+    if (s.indices.empty()) throw ConfigError("mesh asset has no triangles");
     return s;
 }
 
@@ -49,17 +148,25 @@ static Camera load_camera(const ConfigArgs &args) {
 
 SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
     SmallPTSetup st;
-    if (args.contains("object") || args.contains("compound_object"))
-        throw ConfigError("mesh assets (object / compound_object) are not supported by this build; use "
-                          "procedural = \"cornell\" | \"spheres\" | \"room\"");
-    if (!args.contains("procedural")) throw ConfigError("the task needs a scene: procedural = \"...\"");
-    st.scene_name = args.load_string("procedural");
-    int seed = args.load_integer("procedural_seed", st.scene_name.rfind("spheres", 0) == 0 ? 1 : 0);
-    ks_host_scene *hs = nullptr;
-    check(ks_host_scene_create(st.scene_name.c_str(), (uint64_t)seed, &hs), "ks_host_scene_create");
     ks_camera_desc scene_cam{};
-    st.scene = scene_from_host(hs, &scene_cam);
-    ks_host_scene_free(hs);
+    bool have_scene_cam = false;
+    if (args.contains("compound_object"))
+        throw ConfigError("compound_object (glTF scenes) is not supported; use object = \"mesh_asset.X\"");
+    if (args.contains("object")) {
+        st.scene_name = args.load_string("object");
+        MeshAsset asset = create_mesh_asset(args.asset(st.scene_name));
+        st.scene = scene_from_mesh_asset(asset, args);
+    } else if (args.contains("procedural")) {
+        st.scene_name = args.load_string("procedural");
+        int seed = args.load_integer("procedural_seed", st.scene_name.rfind("spheres", 0) == 0 ? 1 : 0);
+        ks_host_scene *hs = nullptr;
+        check(ks_host_scene_create(st.scene_name.c_str(), (uint64_t)seed, &hs), "ks_host_scene_create");
+        st.scene = scene_from_host(hs, &scene_cam);
+        ks_host_scene_free(hs);
+        have_scene_cam = true;
+    } else {
+        throw ConfigError("the task needs a scene: object = \"mesh_asset.X\" or procedural = \"...\"");
+    }
 
     if (args.contains("sky") || args.contains("light")) {
         st.scene.lights.clear();
@@ -163,8 +270,10 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
 
     if (args.contains("camera")) {
         st.camera = load_camera(args["camera"]);
-    } else {
+    } else if (have_scene_cam) {
         st.camera.desc = scene_cam;
+    } else {
+        throw ConfigError("missing table 'camera'");
     }
     return st;
 }
@@ -174,7 +283,8 @@ std::string describe(const SmallPTSetup &st) {
     const ks_camera_desc &c = st.camera.desc;
     char buf[2048];
     std::snprintf(buf, sizeof(buf),
-                  "{\"scene\": \"%s\", \"n_tris\": %zu, \"n_lights\": %zu, \"bounces\": %d, \"render_width\": %d, "
+                  "{\"scene\": \"%s\", \"n_tris\": %zu, \"n_vertices\": %zu, \"n_geoms\": %zu, \"n_materials\": %zu, "
+                  "\"n_lights\": %zu, \"bounces\": %d, \"render_width\": %d, "
                   "\"render_height\": %d, \"crop\": [%d, %d, %d, %d], \"spp\": %d, \"rng_seed\": %d, "
                   "\"include_background\": %s, \"backdrop\": [%g, %g, %g], \"clamp_direct\": %g, "
                   "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
@@ -183,7 +293,8 @@ std::string describe(const SmallPTSetup &st) {
                   "\"camera\": {\"use_matrix\": %d, \"pos\": [%g, %g, %g], \"target\": [%g, %g, %g], "
                   "\"up\": [%g, %g, %g], \"vfov_deg\": %.6g, \"aspect\": %.6g, \"to_world\": [%g, %g, %g, %g, %g, %g, "
                   "%g, %g, %g, %g, %g, %g, %g, %g, %g, %g]}}",
-                  st.scene_name.c_str(), st.scene.indices.size() / 3, st.scene.lights.size(), in.bounces,
+                  st.scene_name.c_str(), st.scene.indices.size() / 3, st.scene.vertices.size() / 3,
+                  st.scene.geom_material.size(), st.scene.materials.size(), st.scene.lights.size(), in.bounces,
                   in.render_width, in.render_height, in.crop_start_x, in.crop_start_y, in.crop_width, in.crop_height,
                   in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
                   in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,

@@ -800,6 +800,7 @@ struct Light {
 struct Scene {
     std::vector<V3> verts, normals;
     std::vector<uint32_t> idx, tri_geom, geom_mat;
+    std::vector<uint8_t> geom_flags;
     std::vector<ora_material> mats;
     std::vector<Node> nodes;
     std::vector<uint32_t> prims;
@@ -1103,6 +1104,8 @@ Scene *create(const ora_scene_desc *d) {
     s->idx.assign(d->indices, d->indices + 3 * (size_t)d->n_tris);
     s->tri_geom.assign(d->tri_geom, d->tri_geom + d->n_tris);
     s->geom_mat.assign(d->geom_material, d->geom_material + d->n_geoms);
+    s->geom_flags.assign(d->n_geoms, 0);
+    if (d->geom_flags) s->geom_flags.assign(d->geom_flags, d->geom_flags + d->n_geoms);
     s->mats.assign(d->materials, d->materials + d->n_materials);
 
     uint32_t nt = d->n_tris;
@@ -1219,7 +1222,8 @@ SurfHit make_hit(const Scene &s, V3 ray_d, const Hit &h) {
     V3 dpdu = v1 - v0;
     V3 b = normalized(cross(ng, dpdu));
     V3 t = normalized(cross(b, ng));
-    if (dot(wo, ng) < 0.0f) { // two-sided (geometry.cpp:126-132)
+    const uint8_t flags = s.geom_flags[s.tri_geom[tr]];
+    if (!(flags & 1) && dot(wo, ng) < 0.0f) { // two-sided (geometry.cpp:126-132)
         ng = -ng;
         dpdu = -dpdu;
         t = -t;
@@ -1230,7 +1234,7 @@ SurfHit make_hit(const Scene &s, V3 ray_d, const Hit &h) {
     it.st = t;
     it.sb = b;
     it.sn = ng;
-    if (s.has_normals) {
+    if (s.has_normals && !(flags & 2)) { // use_smooth_normal (geometry.cpp:135)
         const V3 &n0 = s.normals[s.idx[3 * tr]], &n1 = s.normals[s.idx[3 * tr + 1]], &n2 = s.normals[s.idx[3 * tr + 2]];
         V3 vn = normalized(n0 * w + n1 * h.u + n2 * h.v);
         if (all_finite(vn)) {

@@ -67,6 +67,7 @@ typedef struct ora_scene_desc {
     uint32_t n_materials;
     const ora_light *lights;
     uint32_t n_lights;
+    const uint8_t *geom_flags; /* per geometry: 1 one-sided (MeshData::twosided = false), 2 flat normals; or NULL */
 } ora_scene_desc;
 
 /* create_camera (camera.cpp:172-205): look_at(pos, target, up) + perspective vfov (degrees) / aspect. */

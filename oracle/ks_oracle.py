@@ -53,6 +53,7 @@ class OraSceneDesc(C.Structure):
         ("n_materials", C.c_uint32),
         ("lights", C.c_void_p),
         ("n_lights", C.c_uint32),
+        ("geom_flags", C.c_void_p),
     ]
 
 
@@ -173,7 +174,7 @@ def material(rec) -> OraMaterial:
 
 
 class OracleScene:
-    def __init__(self, vertices, indices, tri_geom, geom_material, materials, lights, normals=None):
+    def __init__(self, vertices, indices, tri_geom, geom_material, materials, lights, normals=None, geom_flags=None):
         lib = load()
         self.lib = lib
         self._arrs = [
@@ -184,8 +185,9 @@ class OracleScene:
             np.ascontiguousarray(materials),
             np.ascontiguousarray(lights),
             None if normals is None else np.ascontiguousarray(normals, np.float32),
+            None if geom_flags is None else np.ascontiguousarray(geom_flags, np.uint8),
         ]
-        v, idx, tg, gm, mats, lts, nrm = self._arrs
+        v, idx, tg, gm, mats, lts, nrm, gf = self._arrs
         d = OraSceneDesc()
         d.vertices = v.ctypes.data
         d.n_vertices = v.shape[0]
@@ -199,11 +201,13 @@ class OracleScene:
         d.n_materials = mats.shape[0]
         d.lights = lts.ctypes.data if lts.shape[0] else None
         d.n_lights = lts.shape[0]
+        d.geom_flags = gf.ctypes.data if gf is not None else None
         self.h = lib.ora_scene_create(C.byref(d))
 
     @classmethod
     def from_host_scene(cls, hs) -> "OracleScene":
-        return cls(hs.vertices, hs.indices, hs.tri_geom, hs.geom_material, hs.materials, hs.lights, hs.normals)
+        return cls(hs.vertices, hs.indices, hs.tri_geom, hs.geom_material, hs.materials, hs.lights, hs.normals,
+                   getattr(hs, "geom_flags", None))
 
     def close(self):
         if self.h:

@@ -142,7 +142,8 @@ scale_ray_diff = false
 @pytest.mark.parametrize(
     "extra,msg",
     [
-        ('object = "teapot"\n', "mesh assets"),
+        ('object = "mesh_asset.teapot"\n', "unknown asset 'mesh_asset.teapot'"),
+        ('compound_object = "compound_mesh_asset.x"\n', "compound_object"),
         ('camera_animation = "anim"\n', "camera_animation"),
         ('pixel_filter = { type = "gaussian" }\n', "box pixel filter"),
         ('mis_mode = "fancy"\n', "mis_mode"),
