@@ -66,7 +66,7 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
-HOST_TOOL_SOURCES = ["config.cpp", "mesh_asset.cpp", "small_pt_task.cpp", "ks_render.cpp"]
+HOST_TOOL_SOURCES = ["config.cpp", "mesh_asset.cpp", "gltf_asset.cpp", "small_pt_task.cpp", "ks_render.cpp"]
 TOOL = LIBDIR / "ks_render"
 
 

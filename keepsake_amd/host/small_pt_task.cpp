@@ -5,8 +5,10 @@
 // material list, create_scene scene.cpp:345-365) or one of this build's procedural scenes (keys
 // `procedural`, `procedural_seed`). Materials are [material.*] tables whose `bsdf` names a
 // [bsdf.*] table of type principled_bsdf with constant shader fields (create_material
-// material.cpp:169-177, create_principled_bsdf principled_bsdf.cpp:707-732). `compound_object`
-// (glTF scenes) is rejected with a clear message. Additional keys of this build: `emission` on a
+// material.cpp:169-177, create_principled_bsdf principled_bsdf.cpp:707-732). `compound_object =
+// "compound_mesh_asset.X"` loads a glTF scene whose instances are flattened into world space; its
+// materials come from the glTF file (load_materials = true) unless the task gives a material list,
+// one inner list per glTF mesh. Additional keys of this build: `emission` on a
 // material table (a constant shader field or an RGB triple: makes its triangles mesh lights),
 // mis_mode ("reference" or "consistent"), queue_size, and a [task.guide] table that turns on the
 // learned path guide (keys fraction, train_spp, learning_rate, beta1, beta2, log2_table_size,
@@ -14,6 +16,7 @@
 // mis_mode = "consistent").
 #include "small_pt_task.h"
 
+#include "gltf_asset.h"
 #include "mesh_asset.h"
 
 #include <chrono>
@@ -85,49 +88,132 @@ static ks_material_desc load_material(const ConfigArgs &mat) {
     return d;
 }
 
+// Flat scene assembly: one geometry per (instance, mesh) in order, so that the mesh-light order is the
+// reference's (Scene::build_mesh_lights, scene.cpp:316-332). The vertex normals array is all or nothing:
+// geometries without smooth normals get zero rows and the KS_GEOM_FLAT flag.
+class FlatSceneBuilder {
+  public:
+    explicit FlatSceneBuilder(bool any_normals) : any_normals_(any_normals) {}
+    void add(const MeshData &m, uint32_t material) {
+        uint32_t base = (uint32_t)(s_.vertices.size() / 3);
+        if ((uint64_t)base + m.vertex_count() >= (1ull << 32)) throw ConfigError("scene has too many vertices");
+        s_.vertices.insert(s_.vertices.end(), m.vertices.begin(), m.vertices.end());
+        bool smooth = m.use_smooth_normal && !m.vertex_normals.empty();
+        if (any_normals_) {
+            if (smooth) s_.normals.insert(s_.normals.end(), m.vertex_normals.begin(), m.vertex_normals.end());
+            else s_.normals.insert(s_.normals.end(), m.vertices.size(), 0.0f);
+        }
+        for (uint32_t i : m.indices) s_.indices.push_back(base + i);
+        s_.tri_geom.insert(s_.tri_geom.end(), m.tri_count(), (uint32_t)s_.geom_material.size());
+        s_.geom_flags.push_back((uint8_t)((m.twosided ? 0u : KS_GEOM_ONE_SIDED) | (smooth ? 0u : KS_GEOM_FLAT)));
+        s_.geom_material.push_back(material);
+    }
+    // a [material.*] table by name, read once
+    uint32_t named_material(const ConfigArgs &args, const TomlValue *name) {
+        if (name->type != TomlValue::Type::String) throw ConfigError("material list entries must be strings");
+        auto it = named_.find(name->str);
+        if (it == named_.end()) {
+            it = named_.emplace(name->str, (uint32_t)s_.materials.size()).first;
+            s_.materials.push_back(load_material(args.asset(name->str)));
+        }
+        return it->second;
+    }
+    uint32_t add_material(const ks_material_desc &d) {
+        s_.materials.push_back(d);
+        return (uint32_t)s_.materials.size() - 1;
+    }
+    SceneData finish() {
+        bool any_flag = false;
+        for (uint8_t f : s_.geom_flags) any_flag |= f != 0;
+        if (!any_flag) s_.geom_flags.clear();
+        if (s_.indices.empty()) throw ConfigError("mesh asset has no triangles");
+        return std::move(s_);
+    }
+
+  private:
+    bool any_normals_;
+    SceneData s_;
+    std::map<std::string, uint32_t> named_;
+};
+
+// The material list of the task: one inner list of [material.*] names per subscene, one name per
+// geometry (assign_material_list, mesh_asset.cpp:285-299).
+static ConfigArgs material_list(const ConfigArgs &args, size_t subscenes) {
+    ConfigArgs list = args["material"];
+    if (list.array_size() != subscenes)
+        throw ConfigError("Material list mismatch dimension: " + std::to_string(list.array_size()) + " lists for " +
+                          std::to_string(subscenes) + " subscenes.");
+    return list;
+}
+
 // create_scene_from_mesh_asset + assign_material_list (mesh_asset.cpp:267-299): one subscene with one
 // geometry per mesh, an identity instance, material[0][g] for geometry g.
 static SceneData scene_from_mesh_asset(const MeshAsset &asset, const ConfigArgs &args) {
     if (!args.contains("material"))
         throw ConfigError("No materials. Provide a material list (material = [[\"material.a\", ...]]).");
-    ConfigArgs list = args["material"];
-    if (list.array_size() != 1) throw ConfigError("Material list mismatch dimension (one subscene).");
-    ConfigArgs names = list[0];
+    ConfigArgs names = material_list(args, 1)[0];
     if (names.array_size() != asset.meshes.size())
         throw ConfigError("Material list mismatch dimension: " + std::to_string(names.array_size()) +
                           " materials for " + std::to_string(asset.meshes.size()) + " meshes.");
-    SceneData s;
     bool any_normals = false;
     for (const MeshData &m : asset.meshes) any_normals |= m.use_smooth_normal && !m.vertex_normals.empty();
-    std::map<std::string, uint32_t> mat_index;
-    for (size_t g = 0; g < asset.meshes.size(); ++g) {
-        const MeshData &m = asset.meshes[g];
-        uint32_t base = (uint32_t)(s.vertices.size() / 3);
-        if ((uint64_t)base + m.vertex_count() >= (1ull << 32)) throw ConfigError("mesh asset has too many vertices");
-        s.vertices.insert(s.vertices.end(), m.vertices.begin(), m.vertices.end());
-        bool smooth = m.use_smooth_normal && !m.vertex_normals.empty();
-        if (any_normals) {
-            if (smooth) s.normals.insert(s.normals.end(), m.vertex_normals.begin(), m.vertex_normals.end());
-            else s.normals.insert(s.normals.end(), m.vertices.size(), 0.0f);
+    FlatSceneBuilder b(any_normals);
+    for (size_t g = 0; g < asset.meshes.size(); ++g)
+        b.add(asset.meshes[g], b.named_material(args, names[(int)g].node));
+    return b.finish();
+}
+
+// create_scene_from_compound_mesh_asset (mesh_asset.cpp:1131-1155) with the instances flattened: each
+// instance's meshes are written in world space (points by its transform, normals by the inverse
+// transpose), in instance order. Materials come from the task's material list when it has one
+// (material[prototype][primitive]), else from the glTF materials.
+static SceneData scene_from_compound(const CompoundMeshAsset &asset, const ConfigArgs &args) {
+    bool toml_materials = args.contains("material");
+    if (!toml_materials && !asset.has_materials)
+        throw ConfigError("No materials. Either load materials from file(s) (load_materials = true) or provide a "
+                          "material list.");
+    std::vector<std::vector<uint32_t>> mat(asset.prototypes.size());
+    bool any_normals = false;
+    for (const auto &inst : asset.instances)
+        for (const MeshData &m : asset.prototypes[inst.first].meshes)
+            any_normals |= m.use_smooth_normal && !m.vertex_normals.empty();
+    FlatSceneBuilder b(any_normals);
+    if (toml_materials) {
+        ConfigArgs list = material_list(args, asset.prototypes.size());
+        for (size_t p = 0; p < asset.prototypes.size(); ++p) {
+            ConfigArgs names = list[(int)p];
+            if (names.array_size() != asset.prototypes[p].meshes.size())
+                throw ConfigError("Material list mismatch dimension: " + std::to_string(names.array_size()) +
+                                  " materials for " + std::to_string(asset.prototypes[p].meshes.size()) +
+                                  " primitives of mesh " + std::to_string(p) + ".");
         }
-        for (uint32_t i : m.indices) s.indices.push_back(base + i);
-        s.tri_geom.insert(s.tri_geom.end(), m.tri_count(), (uint32_t)g);
-        uint8_t flags = (m.twosided ? 0u : KS_GEOM_ONE_SIDED) | (smooth ? 0u : KS_GEOM_FLAT);
-        s.geom_flags.push_back(flags);
-        const TomlValue *nv = names[(int)g].node;
-        if (nv->type != TomlValue::Type::String) throw ConfigError("material list entries must be strings");
-        auto it = mat_index.find(nv->str);
-        if (it == mat_index.end()) {
-            it = mat_index.emplace(nv->str, (uint32_t)s.materials.size()).first;
-            s.materials.push_back(load_material(args.asset(nv->str)));
+        // materials are read in instance order, as the geometries that use them are written
+        for (const auto &inst : asset.instances) {
+            auto &mp = mat[inst.first];
+            if (!mp.empty()) continue;
+            ConfigArgs names = list[(int)inst.first];
+            for (size_t g = 0; g < asset.prototypes[inst.first].meshes.size(); ++g)
+                mp.push_back(b.named_material(args, names[(int)g].node));
         }
-        s.geom_material.push_back(it->second);
     }
-    bool any_flag = false;
-    for (uint8_t f : s.geom_flags) any_flag |= f != 0;
-    if (!any_flag) s.geom_flags.clear();
-    if (s.indices.empty()) throw ConfigError("mesh asset has no triangles");
-    return s;
+    for (const auto &inst : asset.instances) {
+        const MeshAsset &proto = asset.prototypes[inst.first];
+        auto &mp = mat[inst.first];
+        if (mp.empty())
+            for (size_t g = 0; g < proto.meshes.size(); ++g) mp.push_back(b.add_material(asset.materials[inst.first][g]));
+        bool identity = true;
+        for (int k = 0; k < 16; ++k) identity &= inst.second[k] == ((k % 5 == 0) ? 1.0f : 0.0f);
+        for (size_t g = 0; g < proto.meshes.size(); ++g) {
+            if (identity) {
+                b.add(proto.meshes[g], mp[g]);
+            } else {
+                MeshData m = proto.meshes[g];
+                m.transform(inst.second);
+                b.add(m, mp[g]);
+            }
+        }
+    }
+    return b.finish();
 }
 
 // create_camera (camera.cpp:172-205), perspective only
@@ -150,12 +236,14 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
     SmallPTSetup st;
     ks_camera_desc scene_cam{};
     bool have_scene_cam = false;
-    if (args.contains("compound_object"))
-        throw ConfigError("compound_object (glTF scenes) is not supported; use object = \"mesh_asset.X\"");
     if (args.contains("object")) {
         st.scene_name = args.load_string("object");
         MeshAsset asset = create_mesh_asset(args.asset(st.scene_name));
         st.scene = scene_from_mesh_asset(asset, args);
+    } else if (args.contains("compound_object")) {
+        st.scene_name = args.load_string("compound_object");
+        CompoundMeshAsset asset = create_compound_mesh_asset(args.asset(st.scene_name));
+        st.scene = scene_from_compound(asset, args);
     } else if (args.contains("procedural")) {
         st.scene_name = args.load_string("procedural");
         int seed = args.load_integer("procedural_seed", st.scene_name.rfind("spheres", 0) == 0 ? 1 : 0);
@@ -165,7 +253,8 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
         ks_host_scene_free(hs);
         have_scene_cam = true;
     } else {
-        throw ConfigError("the task needs a scene: object = \"mesh_asset.X\" or procedural = \"...\"");
+        throw ConfigError("the task needs a scene: object = \"mesh_asset.X\", compound_object = "
+                          "\"compound_mesh_asset.X\" or procedural = \"...\"");
     }
 
     if (args.contains("sky") || args.contains("light")) {

@@ -175,3 +175,39 @@ def test_geometry_flags_match_oracle(device, ks, ora, cornell):
     d = rel_mse(pimg, gimg)
     print("flags vs no flags relMSE", d, flush=True)
     assert d > 20 * max(e, 1e-5)
+
+
+def test_gltf_cornell_renders_like_procedural(tmp_path, ks, cornell):
+    """The Cornell box as one glTF mesh (a primitive per geometry, glTF materials with factors only, the
+    light's radiance as emissiveFactor) renders the procedural scene's image bit for bit."""
+    from test_gltf_asset import cornell_gltf
+
+    cornell_gltf(cornell).write_glb(tmp_path / "cornell.glb")
+    cam = cornell.camera
+    camera = (f'camera = {{ type = "perspective", pos = {_vec(cam["pos"])}, target = {_vec(cam["target"])}, '
+              f'up = {_vec(cam["up"])}, vfov = {_f(cam["vfov"])}, aspect = {_f(cam["aspect"])} }}')
+    common = f"bounces = 8\nrender_width = 96\nrender_height = 64\nspp = 16\nspp_prog_interval = 16\n{camera}\n"
+    cfg = tmp_path / "c.toml"
+    cfg.write_text(f"""output_dir = "{tmp_path / 'out'}"
+[compound_mesh_asset.box]
+path = "{tmp_path / 'cornell.glb'}"
+load_materials = true
+[[task]]
+type = "small_pt"
+task_dir = "procedural"
+procedural = "cornell"
+{common}
+[[task]]
+type = "small_pt"
+task_dir = "gltf"
+compound_object = "compound_mesh_asset.box"
+{common}
+""")
+    print("ks_render on", cfg, flush=True)
+    r = subprocess.run([str(TOOL), "-c", str(cfg)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    a = read_pfm(tmp_path / "out" / "procedural" / "small_pt_spp16.pfm")
+    b = read_pfm(tmp_path / "out" / "gltf" / "small_pt_spp16.pfm")
+    assert a.mean() > 0.01
+    assert np.array_equal(a, b)

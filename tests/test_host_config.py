@@ -143,7 +143,7 @@ scale_ray_diff = false
     "extra,msg",
     [
         ('object = "mesh_asset.teapot"\n', "unknown asset 'mesh_asset.teapot'"),
-        ('compound_object = "compound_mesh_asset.x"\n', "compound_object"),
+        ('compound_object = "compound_mesh_asset.x"\n', "unknown asset 'compound_mesh_asset.x'"),
         ('camera_animation = "anim"\n', "camera_animation"),
         ('pixel_filter = { type = "gaussian" }\n', "box pixel filter"),
         ('mis_mode = "fancy"\n', "mis_mode"),
