@@ -66,7 +66,8 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
-HOST_TOOL_SOURCES = ["config.cpp", "mesh_asset.cpp", "gltf_asset.cpp", "small_pt_task.cpp", "ks_render.cpp"]
+HOST_TOOL_SOURCES = ["config.cpp", "mesh_asset.cpp", "gltf_asset.cpp", "image_io.cpp", "small_pt_task.cpp",
+                     "ks_render.cpp"]
 TOOL = LIBDIR / "ks_render"
 
 
@@ -74,7 +75,7 @@ def build_tools() -> Path:
     """The ks_render task runner (C++ host code over the C ABI), linked against the library."""
     cmd = [CXX, *HOST_FLAGS, "-I", str(ROOT / "include")]
     cmd += [str(PKG / "host" / s) for s in HOST_TOOL_SOURCES]
-    cmd += ["-L", str(LIBDIR), "-lkeepsake_amd", "-Wl,-rpath,$ORIGIN", "-o", str(TOOL)]
+    cmd += ["-L", str(LIBDIR), "-lkeepsake_amd", "-lz", "-Wl,-rpath,$ORIGIN", "-o", str(TOOL)]
     _run(cmd)
     return TOOL
 

@@ -1654,12 +1654,12 @@ int ks_film_composite(const ks_film *film, float *out) {
     std::vector<double> rgb(3 * n), w(2 * n);
     int e = ks_film_download(film, rgb.data(), w.data());
     if (e) return e;
-    for (size_t i = 0; i < n; ++i) { // render_target.cpp:187-192
+    for (size_t i = 0; i < n; ++i) { // render_target.cpp:187-192; lerp takes a float t (maths.h:686-690)
         uint32_t w_sum = (uint32_t)(w[2 * i] + w[2 * i + 1]);
-        double a = w_sum == 0 ? 0.0 : w[2 * i] / (double)w_sum;
+        float t = (float)(w_sum == 0 ? 0.0 : w[2 * i] / (double)w_sum);
         for (int k = 0; k < 3; ++k) {
             double fg = w[2 * i] == 0 ? 0.0 : rgb[3 * i + k] / w[2 * i];
-            out[3 * i + k] = (float)((1.0 - a) * (double)film->backdrop[k] + a * fg);
+            out[3 * i + k] = (float)((double)(1.0f - t) * (double)film->backdrop[k] + (double)t * fg);
         }
     }
     return KS_OK;

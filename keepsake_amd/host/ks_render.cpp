@@ -1,11 +1,12 @@
 // ks_render: the task-runner executable (test/main.cpp:28-84 of the reference) for the gfx950 small_pt,
-// plus the convert_mesh_asset task (OBJ -> binary mesh asset).
+// plus the convert_mesh_asset (OBJ -> binary mesh asset) and tonemap (EXR / HDR -> tone-mapped EXR) tasks.
 //   ks_render -c scene.toml [-d device] [-r asset_root_dir] [-n threads] [--print-config]
 // --print-config resolves every small_pt task of the file and prints the inputs as JSON lines without
 // opening a GPU; --dump-scene DIR also writes each resolved scene's flat arrays as raw little-endian
 // files under DIR/task_<id>/ (vertices.f32, normals.f32, indices.u32, tri_geom.u32,
 // geom_material.u32, geom_flags.u8, materials.bin of ks_material_desc records, lights.bin of
 // ks_light_desc records).
+#include "image_io.h"
 #include "mesh_asset.h"
 #include "small_pt_task.h"
 
@@ -91,6 +92,7 @@ int main(int argc, char **argv) {
         cfg.parse_file(cfg_path);
         cfg.register_task("small_pt", ks::small_pt);
         cfg.register_task("convert_mesh_asset", ks::convert_mesh_asset);
+        cfg.register_task("tonemap", ks::tonemap_task);
         cfg.run_all_tasks();
     } catch (const ks::ConfigError &e) {
         std::fprintf(stderr, "[ks_render] config error: %s\n", e.what());

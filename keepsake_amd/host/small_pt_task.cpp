@@ -10,7 +10,8 @@
 // materials come from the glTF file (load_materials = true) unless the task gives a material list,
 // one inner list per glTF mesh. Additional keys of this build: `emission` on a
 // material table (a constant shader field or an RGB triple: makes its triangles mesh lights),
-// mis_mode ("reference" or "consistent"), queue_size, and a [task.guide] table that turns on the
+// output_pfm (also write PFM files beside the EXR / PNG ones), mis_mode ("reference" or
+// "consistent"), queue_size, and a [task.guide] table that turns on the
 // learned path guide (keys fraction, train_spp, learning_rate, beta1, beta2, log2_table_size,
 // base_resolution, max_resolution, train_batch, max_steps_per_pass, weight_clamp, seed; it needs
 // mis_mode = "consistent").
@@ -308,7 +309,8 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
         if (pf.load_string("type") != "box" || pf.load_float("width", 1.0f) != 1.0f)
             throw ConfigError("only the unit-width box pixel filter is supported");
     }
-    if (args.contains("tone_mapper")) std::printf("[small_pt] tone_mapper ignored: output is linear PFM\n");
+    if (args.contains("tone_mapper")) st.tone_mapper = create_tone_mapper(args["tone_mapper"]);
+    st.output_pfm = args.load_bool("output_pfm", false);
     if (args.contains("camera_animation")) throw ConfigError("camera_animation is not supported");
 
     SmallPTInput &in = st.input;
@@ -423,9 +425,20 @@ void small_pt(const ConfigArgs &args, const fs::path &task_dir, int task_id) {
         auto now = std::chrono::steady_clock::now();
         std::printf("Frame [1/1] | [%d/%d] spp interval took: %.3f sec\n", spp_finished, in.spp,
                     std::chrono::duration<double>(now - interval_start).count());
+        // composite_and_save_to_exr (FLOAT) and, with a tone mapper, composite_and_save_to_png
+        std::string prefix = "small_pt_spp" + std::to_string(spp_finished);
         std::vector<float> rgb = target.composite();
-        write_pfm(task_dir / ("small_pt_spp" + std::to_string(spp_finished) + ".pfm"), rgb.data(), target.width,
-                  target.height);
+        write_exr(task_dir / (prefix + ".exr"), rgb.data(), (int)target.width, (int)target.height, 3, false);
+        if (st.tone_mapper) {
+            std::vector<double> sums, weights;
+            target.download(sums, weights);
+            float bd[3] = {in.backdrop[0], in.backdrop[1], in.backdrop[2]};
+            std::vector<uint8_t> png =
+                composite_png(sums.data(), weights.data(), weights.size() / 2, bd, st.tone_mapper.get());
+            write_png(task_dir / (prefix + ".png"), png.data(), (int)target.width, (int)target.height, 3);
+        }
+        if (st.output_pfm)
+            write_pfm(task_dir / (prefix + ".pfm"), rgb.data(), target.width, target.height);
         interval_start = std::chrono::steady_clock::now();
     };
     SmallPT pt;

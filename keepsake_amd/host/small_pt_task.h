@@ -2,7 +2,9 @@
 #pragma once
 #include "../../include/ks/small_pt.h"
 #include "config.h"
+#include "image_io.h"
 
+#include <memory>
 #include <string>
 
 namespace ks {
@@ -14,6 +16,8 @@ struct SmallPTSetup {
     std::string scene_name;
     bool guided = false;          // a [task.guide] table is present
     ks_guide_config guide_cfg{};  // its keys (zero: library default)
+    std::shared_ptr<ToneMapper> tone_mapper; // [task.tone_mapper]: also write tone-mapped PNGs
+    bool output_pfm = false;                 // also write PFM files (this build's key)
 };
 
 // Parses one [[task]] table of type small_pt without touching the GPU.

@@ -2,7 +2,7 @@
 
 1. The procedural Cornell box written out as an OBJ with one shape per geometry, loaded back through
    the TOML path (object + material list + [material.*]/[bsdf.*] tables, mesh-light emission), renders
-   the same PFM bit for bit as the procedural scene: loading changes the vertex numbering, not the
+   the same EXR bit for bit as the procedural scene: loading changes the vertex numbering, not the
    triangles, so the BVH, the light table and every sample agree.
 2. Per-geometry flags (one-sided shading frames, geometric instead of vertex normals, geometry.cpp:68-163)
    match the CPU oracle at equal spp, and visibly change the image, so the flags are not ignored."""
@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_mse
+from image_files import read_exr_rgb
 
 pytestmark = pytest.mark.gpu
 
@@ -22,13 +23,8 @@ ROOT = Path(__file__).resolve().parents[1]
 TOOL = ROOT / "keepsake_amd" / "_lib" / "ks_render"
 
 
-def read_pfm(path):
-    data = Path(path).read_bytes()
-    parts = data.split(b"\n", 3)
-    assert parts[0] == b"PF"
-    w, h = (int(x) for x in parts[1].split())
-    img = np.frombuffer(parts[3], dtype="<f4" if float(parts[2]) < 0 else ">f4").reshape(h, w, 3)
-    return img[::-1]
+def read_image(path):
+    return read_exr_rgb(path)[0]
 
 
 def _f(x):
@@ -112,8 +108,8 @@ material = [[{mats}]]
     r = subprocess.run([str(TOOL), "-c", str(cfg)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
     assert r.returncode == 0
-    a = read_pfm(tmp_path / "out" / "procedural" / "small_pt_spp16.pfm")
-    b = read_pfm(tmp_path / "out" / "obj" / "small_pt_spp16.pfm")
+    a = read_image(tmp_path / "out" / "procedural" / "small_pt_spp16.exr")
+    b = read_image(tmp_path / "out" / "obj" / "small_pt_spp16.exr")
     assert a.mean() > 0.01  # the mesh light lit the box
     assert np.array_equal(a, b)
 
@@ -207,7 +203,7 @@ compound_object = "compound_mesh_asset.box"
     r = subprocess.run([str(TOOL), "-c", str(cfg)], capture_output=True, text=True, timeout=120)
     print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
     assert r.returncode == 0
-    a = read_pfm(tmp_path / "out" / "procedural" / "small_pt_spp16.pfm")
-    b = read_pfm(tmp_path / "out" / "gltf" / "small_pt_spp16.pfm")
+    a = read_image(tmp_path / "out" / "procedural" / "small_pt_spp16.exr")
+    b = read_image(tmp_path / "out" / "gltf" / "small_pt_spp16.exr")
     assert a.mean() > 0.01
     assert np.array_equal(a, b)
