@@ -218,7 +218,7 @@ __global__ void k_iter_done(Ctrl *c) {
 // Closest hit for n rays (count read from *n_dev when given). Grid-stride over rays; one LDS stack
 // column per lane.
 template <int W>
-__global__ __launch_bounds__(kTraceBlock) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                          const float4 *ro, const float4 *rd, float4 *hit,
                                                          uint32_t *ovf, uint32_t ovf_cap,
                                                          unsigned long long *stats) {
@@ -234,15 +234,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest(DevScene sc, const uint
     st.ovf_cap = ovf_cap;
     st.spills = 0;
     uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    for (uint32_t i = gid; i < n; i += G) {
-        float4 o4 = ro[i], d4 = rd[i];
-        TravRay r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
-        TravHit h;
-        traverse_w<false, W>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
-        hit[i] = h.prim >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim))
-                             : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
-        ++n_rays;
-    }
+    trace_rays<false, W>(
+        sc.nodes, sc.tris, gid, n, G, st,
+        [&](uint32_t i, TravRay &r) {
+            float4 o4 = ro[i], d4 = rd[i];
+            r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
+            return d4.w;
+        },
+        [&](uint32_t i, const TravHit &h) {
+            hit[i] = h.prim >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim))
+                                 : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
+        },
+        n_nodes, n_tris, n_rays);
     wave_add_stat(stats, kRaysClosest, n_rays);
     wave_add_stat(stats, kNodesClosest, n_nodes);
     wave_add_stat(stats, kTrisClosest, n_tris);
@@ -269,12 +272,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint3
     st.ovf_cap = ovf_cap;
     st.spills = 0;
     uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    for (uint32_t i = gid; i < n; i += G) {
+    auto load = [&](uint32_t i, TravRay &r) {
         float4 o4 = ro[i], d4 = rd[i];
-        TravRay r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
-        TravHit h;
-        bool blocked = traverse_w<true, W>(sc.nodes, sc.tris, r, d4.w, st, h, n_nodes, n_tris);
-        ++n_rays;
+        r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
+        return d4.w;
+    };
+    auto store = [&](uint32_t i, const TravHit &h) {
+        const bool blocked = h.prim >= 0;
         if (MODE == 0) {
             occ[i] = blocked ? 1 : 0;
         } else if (!blocked) {
@@ -289,7 +293,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint3
             uint32_t slot = __float_as_uint(c.w); // training record of the vertex that cast the ray, + 1
             if (recs && slot) recs[(size_t)(slot - 1) * kRecFloat4 + kRecNee] = make_float4(c.x, c.y, c.z, 0.0f);
         }
-    }
+    };
+    trace_rays<true, W>(sc.nodes, sc.tris, gid, n, G, st, load, store, n_nodes, n_tris, n_rays);
     wave_add_stat(stats, kRaysShadow, n_rays);
     wave_add_stat(stats, kNodesShadow, n_nodes);
     wave_add_stat(stats, kTrisShadow, n_tris);

@@ -289,6 +289,143 @@ KS_DEV bool traverse4(const float4 *__restrict__ nodes, const float4 *__restrict
     return hit.prim >= 0;
 }
 
+// One visit of an inner node of a BVH of width W (2 or 4), with the same child test and order as
+// traverse / traverse4: returns the next node to visit; when no child is hit it pops the stack, and
+// sets `done` when the stack is empty.
+template <int W>
+KS_DEV uint32_t visit_node(const float4 *__restrict__ nodes, const TravRay &r, float tmax, TravStack &st,
+                           uint32_t node, bool &done) {
+    const float robust = 1.0000004f;
+    if (W == 2) {
+        const float4 *np = nodes + 4 * (size_t)node;
+        float4 a = np[0], b = np[1], c = np[2];
+        uint2 ch = *reinterpret_cast<const uint2 *>(np + 3);
+        float n0x = fmaf(r.nx ? a.y : a.x, r.inv.x, -r.oinv.x), f0x = fmaf(r.nx ? a.x : a.y, r.inv.x, -r.oinv.x);
+        float n0y = fmaf(r.ny ? a.w : a.z, r.inv.y, -r.oinv.y), f0y = fmaf(r.ny ? a.z : a.w, r.inv.y, -r.oinv.y);
+        float n0z = fmaf(r.nz ? c.y : c.x, r.inv.z, -r.oinv.z), f0z = fmaf(r.nz ? c.x : c.y, r.inv.z, -r.oinv.z);
+        float n1x = fmaf(r.nx ? b.y : b.x, r.inv.x, -r.oinv.x), f1x = fmaf(r.nx ? b.x : b.y, r.inv.x, -r.oinv.x);
+        float n1y = fmaf(r.ny ? b.w : b.z, r.inv.y, -r.oinv.y), f1y = fmaf(r.ny ? b.z : b.w, r.inv.y, -r.oinv.y);
+        float n1z = fmaf(r.nz ? c.w : c.z, r.inv.z, -r.oinv.z), f1z = fmaf(r.nz ? c.z : c.w, r.inv.z, -r.oinv.z);
+        float t0n = fmaxf(fmaxf(n0x, n0y), fmaxf(n0z, r.tmin));
+        float t0f = fminf(fminf(f0x, f0y), fminf(f0z, tmax)) * robust;
+        float t1n = fmaxf(fmaxf(n1x, n1y), fmaxf(n1z, r.tmin));
+        float t1f = fminf(fminf(f1x, f1y), fminf(f1z, tmax)) * robust;
+        bool h0 = t0n <= t0f, h1 = t1n <= t1f;
+        if (h0 && h1) {
+            bool first0 = t0n <= t1n;
+            st.push(first0 ? ch.y : ch.x);
+            return first0 ? ch.x : ch.y;
+        }
+        if (h0) return ch.x;
+        if (h1) return ch.y;
+    } else {
+        const float4 *np = nodes + 8 * (size_t)node;
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+        float key[4];
+        uint32_t cp[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float nx = fmaf(r.nx ? lane4(hx, i) : lane4(lx, i), r.inv.x, -r.oinv.x);
+            float fx = fmaf(r.nx ? lane4(lx, i) : lane4(hx, i), r.inv.x, -r.oinv.x);
+            float ny = fmaf(r.ny ? lane4(hy, i) : lane4(ly, i), r.inv.y, -r.oinv.y);
+            float fy = fmaf(r.ny ? lane4(ly, i) : lane4(hy, i), r.inv.y, -r.oinv.y);
+            float nz = fmaf(r.nz ? lane4(hz, i) : lane4(lz, i), r.inv.z, -r.oinv.z);
+            float fz = fmaf(r.nz ? lane4(lz, i) : lane4(hz, i), r.inv.z, -r.oinv.z);
+            float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
+            float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * robust;
+            key[i] = tn <= tf ? tn : INFINITY;
+            cp[i] = lane4(ch, i);
+        }
+        auto cswap = [&](int a, int b) {
+            bool sw = key[b] < key[a];
+            float ka = key[a], kb = key[b];
+            uint32_t ca = cp[a], cb = cp[b];
+            key[a] = sw ? kb : ka;
+            key[b] = sw ? ka : kb;
+            cp[a] = sw ? cb : ca;
+            cp[b] = sw ? ca : cb;
+        };
+        cswap(0, 1);
+        cswap(2, 3);
+        cswap(0, 2);
+        cswap(1, 3);
+        cswap(1, 2);
+        if (key[0] != INFINITY) {
+            if (key[3] != INFINITY) st.push(cp[3]);
+            if (key[2] != INFINITY) st.push(cp[2]);
+            if (key[1] != INFINITY) st.push(cp[1]);
+            return cp[0];
+        }
+    }
+    if (st.sp == 0) {
+        done = true;
+        return 0;
+    }
+    return st.pop();
+}
+
+// Rays first, first + stride, ... < n on one lane, with replacement: a lane whose ray has finished
+// stores its result and loads its next ray at once, while the other lanes of the wave go on with
+// theirs. The wave then never waits for its slowest ray before fetching more work (the persistent
+// while-while loop of Aila and Laine, HPG 2009). Each ray's result is the one traverse_w gives.
+// load(i, ray) fills the ray and returns its tmax; store(i, hit) takes the result (prim < 0: miss).
+template <bool ANY, int W, typename Load, typename Store>
+KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t i, uint32_t n,
+                       uint32_t stride, TravStack &st, Load &&load, Store &&store, uint32_t &n_nodes,
+                       uint32_t &n_tris, uint32_t &n_rays) {
+    if (i >= n) return;
+    TravRay r;
+    TravHit hit;
+    float tmax = load(i, r);
+    hit.prim = -1;
+    st.sp = 0;
+    uint32_t node = 0;
+    while (true) {
+        bool done = false;
+        while ((node & kLeafBit) == 0) {
+            ++n_nodes;
+            node = visit_node<W>(nodes, r, tmax, st, node, done);
+            if (done) break;
+        }
+        if (!done) {
+            uint32_t first = (node >> 4) & 0x7ffffffu;
+            uint32_t cnt = (node & 15u) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                ++n_tris;
+                const float4 *tp = tris + 3 * (size_t)(first + k);
+                float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                float t, u, v;
+                if (isect_tri(r, v0, v1, v2, tmax, t, u, v)) {
+                    hit.prim = __float_as_int(v0.w);
+                    if (ANY) {
+                        done = true;
+                        break;
+                    }
+                    tmax = t;
+                    hit.t = t;
+                    hit.u = u;
+                    hit.v = v;
+                }
+            }
+            if (!done) {
+                if (st.sp == 0) done = true;
+                else node = st.pop();
+            }
+        }
+        if (done) {
+            store(i, hit);
+            ++n_rays;
+            i += stride;
+            if (i >= n) break;
+            tmax = load(i, r);
+            hit.prim = -1;
+            st.sp = 0;
+            node = 0;
+        }
+    }
+}
+
 // Dispatch on the node width of the scene (2 or 4).
 template <bool ANY, int W>
 KS_DEV bool traverse_w(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, const TravRay &r, float tmax,
