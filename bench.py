@@ -351,6 +351,12 @@ def main() -> int:
         },
         "mrays_per_s": round(n_rays / (elapsed / args.steps) / 1e6, 1),
         "kernel_ms_per_step": {k: round(v / args.steps, 2) for k, v in totals["kernel_ms"].items()},
+        "traversal": {
+            "nodes_per_closest_ray": round(totals["node_visits_closest"] / max(totals["rays_closest"], 1), 2),
+            "tris_per_closest_ray": round(totals["tri_tests_closest"] / max(totals["rays_closest"], 1), 2),
+            "closest_rays_per_step": totals["rays_closest"] // args.steps,
+            "shadow_rays_per_step": totals["rays_shadow"] // args.steps,
+        },
         "roofline": roofline,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
 // Any hit. mode 0: write the occlusion flag; mode 1: add the NEE contribution of unoccluded rays to
 // their target (next path buffer slot, or the per-sample result when bit 31 is set).
 template <int MODE, int W>
-__global__ __launch_bounds__(kTraceBlock) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
                                                         const uint32_t *target, const float4 *contrib,
                                                         float4 *path_L, float4 *samp, float4 *recs, uint32_t *ovf,
@@ -1534,6 +1534,9 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
 
     hipStream_t s = dev->stream;
     const int width = o.branching == 2 ? 2 : 4;
+    // BVH4 visits address nodes with 32-bit byte offsets
+    KS_REQUIRE(width != 4 || bvh.nodes4.size() < (1u << 25),
+               "BVH4 node array exceeds 4 GiB; build this scene with branching = 2");
     if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
     else KS_HIP(upload(sc->nodes, bvh.nodes, s));
     KS_HIP(upload(sc->tris, bvh.tris, s));

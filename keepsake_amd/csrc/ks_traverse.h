@@ -319,19 +319,25 @@ KS_DEV uint32_t visit_node(const float4 *__restrict__ nodes, const TravRay &r, f
         if (h0) return ch.x;
         if (h1) return ch.y;
     } else {
-        const float4 *np = nodes + 8 * (size_t)node;
-        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-        const uint4 ch = *reinterpret_cast<const uint4 *>(np + 6);
+        // near / far planes picked by address from the ray's direction signs: (lo, hi) of an axis are
+        // adjacent float4s, so the sign selects the load and no per-child select is needed
+        const char *base = reinterpret_cast<const char *>(nodes);
+        const uint32_t off = node * 128u; // node arrays stay below 4 GiB (2^25 nodes)
+        const uint32_t ox = 16u * (uint32_t)r.nx, oy = 16u * (uint32_t)r.ny, oz = 16u * (uint32_t)r.nz;
+        auto ld = [&](uint32_t o) { return *reinterpret_cast<const float4 *>(base + o); };
+        const float4 nxv = ld(off + ox), fxv = ld(off + 16u - ox), nyv = ld(off + 32u + oy),
+                     fyv = ld(off + 48u - oy), nzv = ld(off + 64u + oz), fzv = ld(off + 80u - oz);
+        const uint4 ch = *reinterpret_cast<const uint4 *>(base + off + 96u);
         float key[4];
         uint32_t cp[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            float nx = fmaf(r.nx ? lane4(hx, i) : lane4(lx, i), r.inv.x, -r.oinv.x);
-            float fx = fmaf(r.nx ? lane4(lx, i) : lane4(hx, i), r.inv.x, -r.oinv.x);
-            float ny = fmaf(r.ny ? lane4(hy, i) : lane4(ly, i), r.inv.y, -r.oinv.y);
-            float fy = fmaf(r.ny ? lane4(ly, i) : lane4(hy, i), r.inv.y, -r.oinv.y);
-            float nz = fmaf(r.nz ? lane4(hz, i) : lane4(lz, i), r.inv.z, -r.oinv.z);
-            float fz = fmaf(r.nz ? lane4(lz, i) : lane4(hz, i), r.inv.z, -r.oinv.z);
+            float nx = fmaf(lane4(nxv, i), r.inv.x, -r.oinv.x);
+            float fx = fmaf(lane4(fxv, i), r.inv.x, -r.oinv.x);
+            float ny = fmaf(lane4(nyv, i), r.inv.y, -r.oinv.y);
+            float fy = fmaf(lane4(fyv, i), r.inv.y, -r.oinv.y);
+            float nz = fmaf(lane4(nzv, i), r.inv.z, -r.oinv.z);
+            float fz = fmaf(lane4(fzv, i), r.inv.z, -r.oinv.z);
             float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
             float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * robust;
             key[i] = tn <= tf ? tn : INFINITY;
@@ -367,8 +373,11 @@ KS_DEV uint32_t visit_node(const float4 *__restrict__ nodes, const TravRay &r, f
 
 // Rays first, first + stride, ... < n on one lane, with replacement: a lane whose ray has finished
 // stores its result and loads its next ray at once, while the other lanes of the wave go on with
-// theirs. The wave then never waits for its slowest ray before fetching more work (the persistent
-// while-while loop of Aila and Laine, HPG 2009). Each ray's result is the one traverse_w gives.
+// theirs. The wave then never waits for its slowest ray before fetching more work. Leaves are found
+// speculatively: a lane that reaches a leaf sets it aside and keeps visiting inner nodes until every
+// lane of the wave holds a leaf (or has none left), and then all lanes test their leaves together
+// (the persistent speculative while-while loop of Aila and Laine, HPG 2009). Each ray's closest hit is
+// the one traverse_w gives; among triangles hit at exactly the same t, the first one tested wins.
 // load(i, ray) fills the ray and returns its tmax; store(i, hit) takes the result (prim < 0: miss).
 template <bool ANY, int W, typename Load, typename Store>
 KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t i, uint32_t n,
@@ -380,17 +389,30 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
     float tmax = load(i, r);
     hit.prim = -1;
     st.sp = 0;
-    uint32_t node = 0;
+    uint32_t node = 0;       // next node to visit (valid while !exhausted)
+    uint32_t leaf = 0;       // leaf set aside for testing (0: none; leaves carry kLeafBit)
+    bool exhausted = false;  // no node left to visit for this ray
     while (true) {
-        bool done = false;
-        while ((node & kLeafBit) == 0) {
-            ++n_nodes;
-            node = visit_node<W>(nodes, r, tmax, st, node, done);
-            if (done) break;
+        // inner nodes, until every lane of the wave holds a leaf or has nothing left to visit
+        while (true) {
+            if (!exhausted && (node & kLeafBit) == 0) {
+                ++n_nodes;
+                bool empty = false;
+                node = visit_node<W>(nodes, r, tmax, st, node, empty);
+                exhausted = empty;
+            }
+            if (!exhausted && (node & kLeafBit) != 0 && leaf == 0) {
+                leaf = node;
+                if (st.sp == 0) exhausted = true;
+                else node = st.pop();
+            }
+            if (!__any(leaf == 0 && !exhausted)) break;
         }
-        if (!done) {
-            uint32_t first = (node >> 4) & 0x7ffffffu;
-            uint32_t cnt = (node & 15u) + 1u;
+        bool done = false;
+        if (leaf != 0) {
+            uint32_t first = (leaf >> 4) & 0x7ffffffu;
+            uint32_t cnt = (leaf & 15u) + 1u;
+            leaf = 0;
             for (uint32_t k = 0; k < cnt; ++k) {
                 ++n_tris;
                 const float4 *tp = tris + 3 * (size_t)(first + k);
@@ -408,12 +430,8 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
                     hit.v = v;
                 }
             }
-            if (!done) {
-                if (st.sp == 0) done = true;
-                else node = st.pop();
-            }
         }
-        if (done) {
+        if (done || exhausted) {
             store(i, hit);
             ++n_rays;
             i += stride;
@@ -422,6 +440,8 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
             hit.prim = -1;
             st.sp = 0;
             node = 0;
+            leaf = 0;
+            exhausted = false;
         }
     }
 }
