@@ -261,6 +261,8 @@ typedef struct ks_guide_config {
                                    throughout) */
     float weight_clamp;         /* render training: sample weights divided by the pass mean, clamped at
                                    this (20) */
+    int32_t inference_fp8;      /* 1: sample the guide with FP8 (OCP e4m3) MLP inference, per-tensor power-of-two
+                                   scales; training stays BF16 with FP32 master weights (0: BF16) */
 } ks_guide_config;
 
 int ks_guide_create(ks_device *dev, const ks_guide_config *cfg, const float bound_lo[3], const float bound_hi[3],
@@ -277,6 +279,17 @@ typedef struct ks_guide_stats {
     double last_loss, last_weight, last_loss_cosine;
 } ks_guide_stats;
 int ks_guide_get_stats(const ks_guide *guide, ks_guide_stats *stats);
+/* inference precision of later forwards and guided renders: 1 FP8 (e4m3), 0 BF16. Switching to FP8 builds
+ * the FP8 copy of the network from the current weights and activation maxima. */
+int ks_guide_set_inference(ks_guide *guide, int32_t fp8);
+/* FP8 calibration: records the largest |input| of each MLP layer over n queries (position, normal, wo;
+ * BF16 forward) and rebuilds the FP8 copy with input scales from them. Guided renders calibrate by
+ * themselves: every training step records the maxima over its batch. */
+int ks_guide_calibrate_fp8(ks_guide *guide, int64_t n, const float *queries);
+/* the current FP8 scales: 1/s_x, 1/s_h1, 1/s_h2, s_x s_w1, s_h1 s_w2, s_h2 s_w3, s_w1, s_w2, s_w3, s_x, s_h1,
+ * s_h2 (all powers of two; s_w = the smallest power of two with max|W| / s_w <= 448, s_in = twice that for
+ * the recorded input maximum) */
+int ks_guide_fp8_scales(const ks_guide *guide, float scales[12]);
 /* grid resolutions of the 16 levels and the position normalisation u = (x - lo) * inv_ext */
 int ks_guide_grid_info(const ks_guide *guide, int32_t res[16], float lo[3], float inv_ext[3]);
 /* FP32 master parameters: [grid][W1 64x64][W2 64x64][W3 64x48], weights [in][out] */

@@ -188,6 +188,7 @@ class GuideConfig(C.Structure):
         ("seed", C.c_uint64),
         ("train_spp", C.c_int32),
         ("weight_clamp", C.c_float),
+        ("inference_fp8", C.c_int32),
     ]
 
 
@@ -257,6 +258,9 @@ _SIGS = {
     "ks_guide_free": ([_P], C.c_int),
     "ks_guide_set_mode": ([_P, C.c_float, C.c_int32], C.c_int),
     "ks_guide_get_stats": ([_P, C.POINTER(GuideStats)], C.c_int),
+    "ks_guide_set_inference": ([_P, C.c_int32], C.c_int),
+    "ks_guide_calibrate_fp8": ([_P, C.c_int64, _P], C.c_int),
+    "ks_guide_fp8_scales": ([_P, _P], C.c_int),
     "ks_guide_param_count": ([_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "ks_guide_grid_info": ([_P, _P, _P, _P], C.c_int),
     "ks_guide_get_params": ([_P, _P], C.c_int),
@@ -718,6 +722,21 @@ class Guide:
 
     def set_mode(self, guide_fraction: float, max_steps_per_pass: int) -> None:
         _check(self.lib.ks_guide_set_mode(self.h, guide_fraction, max_steps_per_pass))
+
+    def set_inference(self, fp8: bool) -> None:
+        """FP8 (e4m3) or BF16 MLP inference for later forwards and guided renders."""
+        _check(self.lib.ks_guide_set_inference(self.h, 1 if fp8 else 0))
+
+    def calibrate_fp8(self, queries) -> None:
+        """Record each layer's largest |input| over the queries and rebuild the FP8 scales from them."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 9)
+        _check(self.lib.ks_guide_calibrate_fp8(self.h, q.shape[0], _ptr(q)))
+
+    def fp8_scales(self) -> dict:
+        v = np.zeros(12, np.float32)
+        _check(self.lib.ks_guide_fp8_scales(self.h, _ptr(v)))
+        names = ["inv_sx", "inv_sh1", "inv_sh2", "c1", "c2", "c3", "sw1", "sw2", "sw3", "sx", "sh1", "sh2"]
+        return {k: float(x) for k, x in zip(names, v)}
 
     def stats(self) -> dict:
         st = GuideStats()

@@ -22,10 +22,25 @@ constexpr int kWaveSize = 64;
 // Network parameter layout (FP32 master copy): [grid: L * T * F][W1: 64x64][W2: 64x64][W3: 64x48].
 // Wk is stored [in][out] (row = input unit) for the backward pass and transposed [out][in] (BF16) for
 // the forward MFMA B operand.
+// FP8 (OCP e4m3) inference: per-tensor power-of-two scales for the three layer inputs and the three
+// weight matrices, and the weights pre-arranged as MFMA 16x16x32 A fragments (one 8-byte word per lane
+// per m-tile and 32-wide k-block).
+constexpr int kFp8MTiles = 4 + 4 + 3;
+constexpr int kFp8Frags = kFp8MTiles * 2 * 64; // 64-bit words, 11264 bytes
+enum Fp8Scale {
+    kInvSx = 0, kInvSh1, kInvSh2, // 1 / input scale of layers 1..3
+    kC1, kC2, kC3,                // input scale x weight scale of layers 1..3 (dequantisation)
+    kSw1, kSw2, kSw3,             // weight scales
+    kSx, kSh1, kSh2,              // input scales
+    kFp8ScaleCount
+};
+
 struct GuideNet {
     const uint32_t *grid_h;   // FP16 feature pairs, L * T words
     const uint16_t *wt[3];    // BF16 [out][in] (forward)
     const uint16_t *w[3];     // BF16 [in][out] (backward)
+    const uint64_t *w8;       // FP8 A fragments (kFp8Frags words)
+    const float *fp8;         // kFp8ScaleCount scales
     uint32_t log2_T;
     uint32_t res[kGuideLevels]; // level resolutions floor(base * growth^l), computed on the host in FP64
     float lo[3], inv_ext[3];    // scene bound normalisation

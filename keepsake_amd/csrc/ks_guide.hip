@@ -57,6 +57,38 @@ __device__ __forceinline__ bf16x4 lds_strided(const uint16_t *buf, int row0, int
 }
 __device__ __forceinline__ bf16x4 glb_a(const uint16_t *p) { return *reinterpret_cast<const bf16x4 *>(p); }
 
+// ---- FP8 (OCP e4m3fn) ------------------------------------------------------------------------------
+// The conversion rounds to nearest even and turns magnitudes beyond 464 into NaN, so operands are
+// clamped to the largest finite value, 448, first (saturation).
+__device__ __forceinline__ uint32_t fp8x4(float a, float b, float c, float d) {
+    a = fminf(fmaxf(a, -448.0f), 448.0f);
+    b = fminf(fmaxf(b, -448.0f), 448.0f);
+    c = fminf(fmaxf(c, -448.0f), 448.0f);
+    d = fminf(fmaxf(d, -448.0f), 448.0f);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+    return (uint32_t)w;
+}
+__device__ __forceinline__ uint64_t fp8x8(const float v[8], float scale) {
+    uint32_t lo = fp8x4(v[0] * scale, v[1] * scale, v[2] * scale, v[3] * scale);
+    uint32_t hi = fp8x4(v[4] * scale, v[5] * scale, v[6] * scale, v[7] * scale);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+// D += A (16 x 32, lane l: row l % 16, k 8 (l / 16) .. + 7) x B (32 x 16, lane l: column l % 16, same k)
+__device__ __forceinline__ f32x4 mfma8(uint64_t a, uint64_t b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a, (long)b, c, 0, 0, 0);
+}
+// Input unit of k position 8 g + j of a 32-wide k-block whose lane group g holds the units the previous
+// layer's two result tiles leave in that lane: 4 g + j (first tile) and 16 + 4 g + j - 4 (second tile).
+__host__ __device__ __forceinline__ int fp8_k_unit(int g, int j) { return j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4); }
+// Smallest power of two s with v / s <= 448 (1 for v <= 0).
+__device__ __forceinline__ float fp8_pow2_scale(float v) {
+    if (!(v > 0.0f)) return 1.0f;
+    int e;
+    float m = frexpf(v / 448.0f, &e); // v / 448 = m 2^e, m in [0.5, 1)
+    return ldexpf(1.0f, m == 0.5f ? e - 1 : e);
+}
+
 // out[64 x N] = act(in[64 x 64] * W) for one wave; W given transposed ([out][in], BF16, in-size 64).
 // N = 16 * nt. The input is BF16 in LDS (stride kLdx); with RELU the result goes to `out_bf` as BF16,
 // otherwise to `out_f` as FP32 with row stride ldo.
@@ -170,21 +202,48 @@ constexpr int kInferBlock = 256;
 constexpr int kWtLd = 68; // LDS row stride (BF16) of the staged [out][in] weights: 136 B rows spread banks
 constexpr int kInferLdsWeights = (64 + 64 + kGuideOut) * kWtLd; // BF16 elements
 
-template <class Src>
-__global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src src, float4 *mix) {
-    __shared__ __align__(16) uint16_t s_w[kInferLdsWeights];
+// Largest |value| of the wave's lanes, added to *dst with an atomic max (float bits of non-negative
+// values order like unsigned integers).
+__device__ __forceinline__ void wave_amax(float v, uint32_t *dst) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    if (__lane_id() == 0) atomicMax(dst, __float_as_uint(v));
+}
+__device__ __forceinline__ float amax4(bf16x4 v) {
+    return fmaxf(fmaxf(fabsf(bf2f((uint16_t)v.x)), fabsf(bf2f((uint16_t)v.y))),
+                 fmaxf(fabsf(bf2f((uint16_t)v.z)), fabsf(bf2f((uint16_t)v.w))));
+}
+
+// MODE 0: BF16 operands. MODE 1: BF16, and the largest |input| of each layer over all queries goes to
+// amax[0..2] (calibration of the FP8 scales). MODE 2: FP8 e4m3 operands with the per-tensor scales of
+// net.fp8: each layer's input is multiplied by its inverse scale and rounded to FP8, the FP32
+// accumulation is multiplied by (input scale x weight scale). The k positions of each 32-wide block
+// are permuted by fp8_k_unit so that a layer's result tiles are the next layer's B operand in place.
+template <class Src, int MODE>
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src src, float4 *mix, uint32_t *amax) {
+    constexpr bool FP8 = MODE == 2;
+    __shared__ __align__(16) uint16_t s_w[FP8 ? 4 : kInferLdsWeights];
+    __shared__ __align__(16) uint64_t s_w8[FP8 ? kFp8Frags : 1];
     __shared__ float s_out[kInferBlock / 64][16 * kGuideOut];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, q = lane & 15;
     const uint32_t n = src.count();
     if (blockIdx.x * blockDim.x >= n) return; // whole block idle
-    // stage W1^T, W2^T, W3^T (BF16, [out][in]) in LDS once per block
-    for (int e = threadIdx.x; e < (64 + 64 + kGuideOut) * 16; e += kInferBlock) {
-        int row = e >> 4, c4 = (e & 15) * 4; // rows 0..63 W1, 64..127 W2, 128..175 W3
-        const uint16_t *srcw = row < 64 ? net.wt[0] + row * 64 : (row < 128 ? net.wt[1] + (row - 64) * 64
-                                                                           : net.wt[2] + (row - 128) * 64);
-        *reinterpret_cast<uint2 *>(s_w + row * kWtLd + c4) = *reinterpret_cast<const uint2 *>(srcw + c4);
+    if (FP8) {
+        for (int e = threadIdx.x; e < kFp8Frags; e += kInferBlock) s_w8[e] = net.w8[e];
+    } else {
+        // stage W1^T, W2^T, W3^T (BF16, [out][in]) in LDS once per block
+        for (int e = threadIdx.x; e < (64 + 64 + kGuideOut) * 16; e += kInferBlock) {
+            int row = e >> 4, c4 = (e & 15) * 4; // rows 0..63 W1, 64..127 W2, 128..175 W3
+            const uint16_t *srcw = row < 64 ? net.wt[0] + row * 64 : (row < 128 ? net.wt[1] + (row - 64) * 64
+                                                                               : net.wt[2] + (row - 128) * 64);
+            *reinterpret_cast<uint2 *>(s_w + row * kWtLd + c4) = *reinterpret_cast<const uint2 *>(srcw + c4);
+        }
     }
+    float sc[6] = {1, 1, 1, 1, 1, 1};
+    if (FP8)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sc[k] = net.fp8[k];
     const uint32_t base = blockIdx.x * blockDim.x + wave * 64;
     f3 P = mk(0, 0, 0), Nn = mk(0, 0, 0), WO = mk(0, 0, 0);
     const bool valid = src.load(base + lane, P, Nn, WO);
@@ -192,7 +251,9 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
     normalise_position(net, P, u);
     // encode all four column tiles first so that their 128 hash gathers per lane are in flight together
     bf16x4 b0[4], b1[4], b2[4];
+    uint64_t e0[4], e1[4];
     bool vq[4];
+    float am_x = 0.0f, am_h1 = 0.0f, am_h2 = 0.0f;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int sl = 16 * t + q; // lane that fetched this tile's query
@@ -205,52 +266,116 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
         grid_pair(net, 2 * g + 1, uq, f[2], f[3]);
         grid_pair(net, 8 + 2 * g, uq, f[4], f[5]);
         grid_pair(net, 9 + 2 * g, uq, f[6], f[7]);
-        b0[t] = pack_bf(f[0], f[1], f[2], f[3]);
-        b1[t] = pack_bf(f[4], f[5], f[6], f[7]);
-        b2[t] = g == 0 ? pack_bf(nx, ny, nz, ox) : (g == 1 ? pack_bf(oy, oz, 1.0f, 0.0f) : pack_bf(0, 0, 0, 0));
-        if (!vq[t]) b0[t] = b1[t] = b2[t] = pack_bf(0, 0, 0, 0);
+        if (FP8) {
+            const float rest[8] = {nx, ny, nz, ox, oy, oz, 1.0f, 0.0f};
+            e0[t] = vq[t] ? fp8x8(f, sc[kInvSx]) : 0;
+            e1[t] = vq[t] && g == 0 ? fp8x8(rest, sc[kInvSx]) : 0;
+        } else {
+            b0[t] = pack_bf(f[0], f[1], f[2], f[3]);
+            b1[t] = pack_bf(f[4], f[5], f[6], f[7]);
+            b2[t] = g == 0 ? pack_bf(nx, ny, nz, ox) : (g == 1 ? pack_bf(oy, oz, 1.0f, 0.0f) : pack_bf(0, 0, 0, 0));
+            if (!vq[t]) b0[t] = b1[t] = b2[t] = pack_bf(0, 0, 0, 0);
+            if (MODE == 1) am_x = fmaxf(am_x, fmaxf(fmaxf(amax4(b0[t]), amax4(b1[t])), amax4(b2[t])));
+        }
     }
     __syncthreads(); // weights staged
     const uint16_t *w1 = s_w, *w2 = s_w + 64 * kWtLd, *w3 = s_w + 128 * kWtLd;
     float *so = s_out[wave];
 #pragma unroll 1
     for (int t = 0; t < 4; ++t) {
-        bf16x4 h[4];
-        {
-            f32x4 d[4];
+        if (FP8) {
+            auto frag = [&](int mt, int kb) { return s_w8[(mt * 2 + kb) * 64 + lane]; };
+            // result tiles m = 0..3 -> FP8 B operand: k-block 0 from tiles 0, 1; k-block 1 from tiles 2, 3
+            auto requant = [&](const f32x4 d[4], float c, float inv_s, uint64_t hb[2]) {
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                d[m] = f32x4{0, 0, 0, 0};
-                d[m] = mfma16(wt_frag(w1, kWtLd, m, 0, q, g), b0[t], d[m]);
-                d[m] = mfma16(wt_frag(w1, kWtLd, m, 1, q, g), b1[t], d[m]);
-                d[m] = mfma16(wt_frag(w1, kWtLd, m, 2, q, g), b2[t], d[m]);
+                for (int kb = 0; kb < 2; ++kb) {
+                    float v[8];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        v[r] = fmaxf(d[2 * kb][r] * c, 0.0f);
+                        v[4 + r] = fmaxf(d[2 * kb + 1][r] * c, 0.0f);
+                    }
+                    hb[kb] = fp8x8(v, inv_s);
+                }
+            };
+            uint64_t hb[2];
+            {
+                f32x4 d[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    d[m] = f32x4{0, 0, 0, 0};
+                    d[m] = mfma8(frag(m, 0), e0[t], d[m]);
+                    d[m] = mfma8(frag(m, 1), e1[t], d[m]);
+                }
+                requant(d, sc[kC1], sc[kInvSh1], hb);
             }
+            {
+                f32x4 d[4];
 #pragma unroll
-            for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
-        }
-        {
-            f32x4 d[4];
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                d[m] = f32x4{0, 0, 0, 0};
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(w2, kWtLd, m, kk, q, g), h[kk], d[m]);
+                for (int m = 0; m < 4; ++m) {
+                    d[m] = f32x4{0, 0, 0, 0};
+                    d[m] = mfma8(frag(4 + m, 0), hb[0], d[m]);
+                    d[m] = mfma8(frag(4 + m, 1), hb[1], d[m]);
+                }
+                requant(d, sc[kC2], sc[kInvSh2], hb);
             }
+            {
+                f32x4 d[3];
 #pragma unroll
-            for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
-        }
-        {
-            f32x4 d[3];
+                for (int m = 0; m < 3; ++m) {
+                    d[m] = f32x4{0, 0, 0, 0};
+                    d[m] = mfma8(frag(8 + m, 0), hb[0], d[m]);
+                    d[m] = mfma8(frag(8 + m, 1), hb[1], d[m]);
+                }
 #pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                d[m] = f32x4{0, 0, 0, 0};
+                for (int m = 0; m < 3; ++m)
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(w3, kWtLd, m, kk, q, g), h[kk], d[m]);
+                    for (int r = 0; r < 4; ++r) so[q * kGuideOut + 16 * m + 4 * g + r] = d[m][r] * sc[kC3];
             }
+        } else {
+            bf16x4 h[4];
+            {
+                f32x4 d[4];
 #pragma unroll
-            for (int m = 0; m < 3; ++m)
+                for (int m = 0; m < 4; ++m) {
+                    d[m] = f32x4{0, 0, 0, 0};
+                    d[m] = mfma16(wt_frag(w1, kWtLd, m, 0, q, g), b0[t], d[m]);
+                    d[m] = mfma16(wt_frag(w1, kWtLd, m, 1, q, g), b1[t], d[m]);
+                    d[m] = mfma16(wt_frag(w1, kWtLd, m, 2, q, g), b2[t], d[m]);
+                }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) so[q * kGuideOut + 16 * m + 4 * g + r] = d[m][r];
+                for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
+                if (MODE == 1 && vq[t])
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) am_h1 = fmaxf(am_h1, amax4(h[m]));
+            }
+            {
+                f32x4 d[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    d[m] = f32x4{0, 0, 0, 0};
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(w2, kWtLd, m, kk, q, g), h[kk], d[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) h[m] = relu_bf(d[m]);
+                if (MODE == 1 && vq[t])
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) am_h2 = fmaxf(am_h2, amax4(h[m]));
+            }
+            {
+                f32x4 d[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) {
+                    d[m] = f32x4{0, 0, 0, 0};
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) d[m] = mfma16(wt_frag(w3, kWtLd, m, kk, q, g), h[kk], d[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) so[q * kGuideOut + 16 * m + 4 * g + r] = d[m][r];
+            }
         }
         __syncthreads();
         // head (vmf.h lobes): lane group g evaluates lobes 2g and 2g+1 of query q
@@ -260,13 +385,13 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
         float mx = fmaxf(o[0], o[5]);
         mx = fmaxf(mx, __shfl_xor(mx, 16));
         mx = fmaxf(mx, __shfl_xor(mx, 32));
-        float e0 = expf(o[0] - mx), e1 = expf(o[5] - mx);
-        float sum = e0 + e1;
+        float ex0 = expf(o[0] - mx), ex1 = expf(o[5] - mx);
+        float sum = ex0 + ex1;
         sum += __shfl_xor(sum, 16);
         sum += __shfl_xor(sum, 32);
         if (vq[t]) {
             float4 *dst = mix + 10 * (size_t)(base + 16 * t + q);
-            reinterpret_cast<float2 *>(dst)[g] = make_float2(e0 / sum, e1 / sum);
+            reinterpret_cast<float2 *>(dst)[g] = make_float2(ex0 / sum, ex1 / sum);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
                 f3 r = mk(o[5 * j + 1], o[5 * j + 2], o[5 * j + 3]);
@@ -277,6 +402,60 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
             }
         }
         __syncthreads();
+    }
+    if (MODE == 1) {
+        wave_amax(am_x, amax + 0);
+        wave_amax(am_h1, amax + 1);
+        wave_amax(am_h2, amax + 2);
+    }
+}
+
+// FP8 copy of the network: per-tensor power-of-two weight scales from the FP32 master weights, input
+// scales from the recorded activation maxima (with one power of two of headroom), and the weights as
+// A fragments: word (m-tile, k-block, lane (g, q)) holds W^T[16 m + q][unit of k position 8 g + j],
+// j = 0..7, as FP8 of W / s_w. Layer 1's k-block 0 holds the 32 grid features in the order the encoder
+// produces them, k-block 1 holds normal, wo and the constant 1 in lane group 0. One block of 256.
+__global__ __launch_bounds__(256) void k_guide_fp8_pack(const float *mlp, const uint32_t *act_amax, uint64_t *w8,
+                                                        float *scales) {
+    __shared__ float red[256];
+    __shared__ float s_w[3];
+    const float *wl[3] = {mlp, mlp + kMlpW1, mlp + kMlpW1 + kMlpW2};
+    const int outs[3] = {64, 64, kGuideOut};
+    for (int L = 0; L < 3; ++L) {
+        float m = 0.0f;
+        for (int i = threadIdx.x; i < 64 * outs[L]; i += 256) m = fmaxf(m, fabsf(wl[L][i]));
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) s_w[L] = fp8_pow2_scale(red[0]);
+        __syncthreads();
+    }
+    float s_in[3];
+    for (int L = 0; L < 3; ++L) s_in[L] = 2.0f * fp8_pow2_scale(__uint_as_float(act_amax[L]));
+    if (threadIdx.x == 0) {
+        for (int L = 0; L < 3; ++L) {
+            scales[kInvSx + L] = 1.0f / s_in[L];
+            scales[kC1 + L] = s_in[L] * s_w[L];
+            scales[kSw1 + L] = s_w[L];
+            scales[kSx + L] = s_in[L];
+        }
+    }
+    for (int e = threadIdx.x; e < kFp8Frags; e += 256) {
+        const int lane = e & 63, f = e >> 6, mt = f >> 1, kb = f & 1;
+        const int L = mt < 4 ? 0 : (mt < 8 ? 1 : 2);
+        const int m = mt - (L == 0 ? 0 : (L == 1 ? 4 : 8));
+        const int g = lane >> 4, q = lane & 15, out = 16 * m + q;
+        float v[8];
+        for (int j = 0; j < 8; ++j) {
+            int in;
+            if (L == 0) in = kb == 0 ? fp8_k_unit(g, j) : (g == 0 ? 32 + j : -1);
+            else in = 32 * kb + fp8_k_unit(g, j);
+            v[j] = in >= 0 ? wl[L][in * outs[L] + out] : 0.0f;
+        }
+        w8[e] = fp8x8(v, 1.0f / s_w[L]);
     }
 }
 
@@ -293,7 +472,7 @@ static_assert(sizeof(TrainSample) == ksg::kTrainSampleBytes, "training sample la
 
 __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, int stride, const TrainSample *samples,
                                                     float c_scale, float c_max, float *grid_grad, float *partials,
-                                                    double *loss_out) {
+                                                    double *loss_out, uint32_t *act_amax) {
     extern __shared__ __align__(16) uint8_t smem[];
     const int lane = threadIdx.x;
     const int r16 = lane & 15, q4 = (lane >> 4) * 4;
@@ -314,6 +493,7 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, int str
         for (int b = 0; b < 4; ++b) g2[a][b] = g1[a][b] = f32x4{0, 0, 0, 0};
     }
     double loss = 0.0, wsum = 0.0, loss_cos = 0.0;
+    float am[3] = {0.0f, 0.0f, 0.0f};
     const int n_chunks = (n + 63) / 64;
     for (int chunk = blockIdx.x; chunk < n_chunks; chunk += gridDim.x) {
         int i = chunk * 64 + lane;
@@ -332,6 +512,13 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, int str
         __syncthreads();
         wave_dense<3, false>(H2, net.wt[2], nullptr, O, kGuideOut);
         __syncthreads();
+        if (act_amax && valid) { // largest |input| of each layer, for the FP8 scales
+            for (int k = 0; k < 64; ++k) {
+                am[0] = fmaxf(am[0], fabsf(bf2f(X[lane * kLdx + k])));
+                am[1] = fmaxf(am[1], fabsf(bf2f(H1[lane * kLdx + k])));
+                am[2] = fmaxf(am[2], fabsf(bf2f(H2[lane * kLdx + k])));
+            }
+        }
         // head backward: d(-c log q)/d outputs
         float o[kGuideOut], g[kGuideOut];
 #pragma unroll
@@ -488,6 +675,11 @@ __global__ __launch_bounds__(64) void k_guide_train(GuideNet net, int n, int str
         atomicAdd(loss_out, loss);
         atomicAdd(loss_out + 1, wsum);
         atomicAdd(loss_out + 2, loss_cos);
+    }
+    if (act_amax) {
+        wave_amax(am[0], act_amax + 0);
+        wave_amax(am[1], act_amax + 1);
+        wave_amax(am[2], act_amax + 2);
     }
 }
 
@@ -688,6 +880,8 @@ struct ks_guide {
     uint32_t res[kGuideLevels];
     float lo[3], inv_ext[3];
     DevBuf params, grad, m1, m2, grid_h, w_bf, wt_bf, partials, loss, samples;
+    DevBuf w8, fp8s, act_amax; // FP8 copy of the MLP, its scales, recorded activation maxima
+    bool fp8 = false;          // inference in FP8
     int32_t n_partials = 0;
     int64_t steps = 0;
     int64_t samples_used = 0; // training samples consumed by all steps
@@ -705,6 +899,8 @@ GuideNet net_of(const ks_guide *g) {
     n.wt[0] = wt;
     n.wt[1] = wt + kMlpW1;
     n.wt[2] = wt + kMlpW1 + kMlpW2;
+    n.w8 = g->w8.as<uint64_t>();
+    n.fp8 = g->fp8s.as<float>();
     n.log2_T = g->log2_T;
     for (int l = 0; l < kGuideLevels; ++l) n.res[l] = g->res[l];
     for (int k = 0; k < 3; ++k) {
@@ -723,8 +919,20 @@ int32_t guide_train_spp(const ks_guide *g) { return g->cfg.train_spp; }
 int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
                  const float4 *hits, float4 *mix, hipStream_t s) {
     unsigned blocks = (n_max + kInferBlock - 1) / kInferBlock;
-    hipLaunchKernelGGL(k_guide_infer<QueryHits>, dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
-                       QueryHits{sc, n_dev, rd, hits}, mix);
+    if (g->fp8)
+        hipLaunchKernelGGL((k_guide_infer<QueryHits, 2>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
+                           QueryHits{sc, n_dev, rd, hits}, mix, nullptr);
+    else
+        hipLaunchKernelGGL((k_guide_infer<QueryHits, 0>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
+                           QueryHits{sc, n_dev, rd, hits}, mix, nullptr);
+    KS_HIP(hipGetLastError());
+    return KS_OK;
+}
+
+// The FP8 copy of the network from the current master weights and recorded activation maxima.
+int fp8_pack(ks_guide *g, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_fp8_pack, dim3(1), dim3(256), 0, s, g->params.as<float>() + g->n_grid,
+                       g->act_amax.as<uint32_t>(), g->w8.as<uint64_t>(), g->fp8s.as<float>());
     KS_HIP(hipGetLastError());
     return KS_OK;
 }
@@ -748,10 +956,13 @@ int train_on_device(ks_guide *g, const void *samples, int n, int stride, float c
     int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
     KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
     KS_HIP(hipMemsetAsync(g->loss.p, 0, 3 * sizeof(double), s));
+    // with FP8 inference the step's forward records the activation maxima that scale the next FP8 copy
+    if (g->fp8) KS_HIP(hipMemsetAsync(g->act_amax.p, 0, 3 * sizeof(uint32_t), s));
     const float c_max = c_scale > 0.0f && g->cfg.weight_clamp > 0.0f ? g->cfg.weight_clamp : INFINITY;
     hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, net_of(g), n, stride,
                        static_cast<const TrainSample *>(samples), c_scale > 0.0f ? c_scale : 1.0f, c_max,
-                       g->grad.as<float>(), g->partials.as<float>(), g->loss.as<double>());
+                       g->grad.as<float>(), g->partials.as<float>(), g->loss.as<double>(),
+                       g->fp8 ? g->act_amax.as<uint32_t>() : nullptr);
     KS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
                        g->grad.as<float>() + g->n_grid);
@@ -764,6 +975,10 @@ int train_on_device(ks_guide *g, const void *samples, int n, int stride, float c
                        g->cfg.beta1, g->cfg.beta2, g->cfg.eps, bc1, bc2, g->grid_h.as<uint32_t>(),
                        g->w_bf.as<uint16_t>(), g->wt_bf.as<uint16_t>());
     KS_HIP(hipGetLastError());
+    if (g->fp8) {
+        int e = fp8_pack(g, s);
+        if (e) return e;
+    }
     if (loss_host) {
         KS_HIP(hipMemcpyAsync(loss_host, g->loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
         KS_HIP(hipStreamSynchronize(s));
@@ -826,6 +1041,18 @@ int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float b
     KS_HIP(g->w_bf.alloc(sizeof(uint16_t) * kMlpParams));
     KS_HIP(g->wt_bf.alloc(sizeof(uint16_t) * kMlpParams));
     KS_HIP(g->loss.alloc(3 * sizeof(double)));
+    KS_HIP(g->w8.alloc(sizeof(uint64_t) * kFp8Frags));
+    KS_HIP(g->fp8s.alloc(sizeof(float) * kFp8ScaleCount));
+    KS_HIP(g->act_amax.alloc(sizeof(uint32_t) * 3));
+    {
+        const float one = 1.0f; // activation maxima before any training or calibration
+        uint32_t bits[3];
+        std::memcpy(&bits[0], &one, 4);
+        bits[1] = bits[2] = bits[0];
+        KS_HIP(hipMemcpyAsync(g->act_amax.p, bits, sizeof(bits), hipMemcpyHostToDevice, g->stream));
+        KS_HIP(hipStreamSynchronize(g->stream));
+    }
+    g->fp8 = c.inference_fp8 != 0;
     hipLaunchKernelGGL(k_guide_init, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, g->stream,
                        g->params.as<float>(), g->m1.as<float>(), g->m2.as<float>(), g->grad.as<float>(), g->n_grid,
                        (uint32_t)(c.seed ^ (c.seed >> 32)));
@@ -834,6 +1061,10 @@ int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float b
                        g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
                        g->wt_bf.as<uint16_t>());
     KS_HIP(hipGetLastError());
+    {
+        int e = ksg::fp8_pack(g.get(), g->stream);
+        if (e) return e;
+    }
     KS_HIP(hipStreamSynchronize(g->stream));
     *out = g.release();
     return KS_OK;
@@ -904,6 +1135,8 @@ int ks_guide_set_params(ks_guide *g, const float *params) {
                        g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
                        g->wt_bf.as<uint16_t>());
     KS_HIP(hipGetLastError());
+    int e = ksg::fp8_pack(g, g->stream);
+    if (e) return e;
     KS_HIP(hipStreamSynchronize(g->stream));
     return KS_OK;
 }
@@ -916,8 +1149,12 @@ int ks_guide_forward(ks_guide *g, int64_t n, const float *queries, float *mixtur
     KS_HIP(m.alloc(sizeof(float4) * 10 * n));
     KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
     unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
-    hipLaunchKernelGGL(k_guide_infer<QueryArray>, dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
-                       QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>());
+    if (g->fp8)
+        hipLaunchKernelGGL((k_guide_infer<QueryArray, 2>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                           QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), nullptr);
+    else
+        hipLaunchKernelGGL((k_guide_infer<QueryArray, 0>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                           QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), nullptr);
     KS_HIP(hipGetLastError());
     std::vector<float4> h((size_t)n * 10);
     KS_HIP(hipMemcpyAsync(h.data(), m.p, sizeof(float4) * 10 * n, hipMemcpyDeviceToHost, s));
@@ -965,7 +1202,7 @@ int ks_guide_gradient(ks_guide *g, int64_t n, const float *queries, const float 
     KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * g->n_grid, s));
     hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, ksg::net_of(g), (int)n, 1,
                        g->samples.as<TrainSample>(), 1.0f, INFINITY, g->grad.as<float>(), g->partials.as<float>(),
-                       g->loss.as<double>());
+                       g->loss.as<double>(), nullptr);
     KS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
                        g->grad.as<float>() + g->n_grid);
@@ -983,6 +1220,41 @@ int ks_guide_train_step(ks_guide *g, int64_t n, const float *queries, const floa
     int e = upload_samples(g, n, queries, wi, weight);
     if (e) return e;
     return ksg::train_on_device(g, g->samples.p, (int)n, 1, 0.0f, loss, g->stream);
+}
+
+int ks_guide_set_inference(ks_guide *g, int32_t fp8) {
+    KS_REQUIRE(g, "guide is null");
+    g->fp8 = fp8 != 0;
+    g->cfg.inference_fp8 = fp8 != 0;
+    int e = ksg::fp8_pack(g, g->stream);
+    if (e) return e;
+    KS_HIP(hipStreamSynchronize(g->stream));
+    return KS_OK;
+}
+
+int ks_guide_calibrate_fp8(ks_guide *g, int64_t n, const float *queries) {
+    KS_REQUIRE(g && queries && n > 0 && n < (1ll << 30), "invalid argument");
+    hipStream_t s = g->stream;
+    DevBuf q, m;
+    KS_HIP(q.alloc(sizeof(float) * 9 * n));
+    KS_HIP(m.alloc(sizeof(float4) * 10 * n));
+    KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
+    KS_HIP(hipMemsetAsync(g->act_amax.p, 0, 3 * sizeof(uint32_t), s));
+    unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
+    hipLaunchKernelGGL((k_guide_infer<QueryArray, 1>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                       QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), g->act_amax.as<uint32_t>());
+    KS_HIP(hipGetLastError());
+    int e = ksg::fp8_pack(g, s);
+    if (e) return e;
+    KS_HIP(hipStreamSynchronize(s));
+    return KS_OK;
+}
+
+int ks_guide_fp8_scales(const ks_guide *g, float scales[12]) {
+    KS_REQUIRE(g && scales, "null argument");
+    KS_HIP(hipMemcpyAsync(scales, g->fp8s.p, sizeof(float) * kFp8ScaleCount, hipMemcpyDeviceToHost, g->stream));
+    KS_HIP(hipStreamSynchronize(g->stream));
+    return KS_OK;
 }
 
 int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *dirs, float *pdf) {

@@ -19,7 +19,10 @@ What it restates:
       - input row [32 features, normal, wo, 1, 0 x 25] -> ReLU(W1) -> ReLU(W2) -> W3 -> 48 raw outputs;
       - head: lobe k uses outputs 5k..5k+4 = (logit, mean xyz, log kappa); softmax weights, normalised
         means, kappa = clamp(exp(.), 1e-2, 1e4);
-      - loss sum_i -c_i log q(wi_i).
+      - loss sum_i -c_i log q(wi_i);
+      - FP8 inference (forward_fp8): each layer input x -> e4m3(x / s_in) with s_in a per-tensor power
+        of two from the recorded activation maximum (one power of two of headroom), W -> e4m3(W / s_w)
+        with s_w the smallest power of two bringing max |W| to 448, FP32 sums scaled back by s_in s_w.
 """
 
 import math
@@ -205,6 +208,111 @@ def forward(net, queries, with_bound=False):
     if with_bound:
         return out, acts, dout
     return out, acts
+
+
+# ---------------------------------------------------------------------------------------------------
+# FP8 (OCP e4m3fn) inference: per-tensor power-of-two scales, operands rounded to nearest even after
+# saturation at 448, exact products, FP32 accumulation.
+# ---------------------------------------------------------------------------------------------------
+FP8_MAX = 448.0
+
+
+def fp8_e4m3(a):
+    """Round to OCP FP8 e4m3fn (nearest even) after clamping to +-448; returns float64 values."""
+    v = np.clip(np.asarray(a, dtype=np.float64), -FP8_MAX, FP8_MAX)
+    m = np.abs(v)
+    _, ex = np.frexp(np.where(m > 0, m, 1.0))  # m = f 2^ex, f in [0.5, 1): floor(log2 m) = ex - 1
+    e = np.maximum(ex - 1, -6)  # subnormals share the quantum of the smallest normal binade, 2^-9
+    q = np.ldexp(1.0, e - 3)
+    return np.round(v / q) * q  # half to even
+
+
+def fp8_pow2_scale(v):
+    """Smallest power of two s with v / s <= 448 (v / 448 evaluated in FP32); 1 for v <= 0."""
+    v = F32(v)
+    if not v > 0:
+        return 1.0
+    f, e = np.frexp(F32(v / F32(FP8_MAX)))
+    return float(np.ldexp(1.0, int(e) - 1 if f == 0.5 else int(e)))
+
+
+class Fp8Net:
+    """The FP8 copy of a guide: FP16 grid as the GPU reads it, weights W / s_w rounded to e4m3 from the
+    FP32 master values ([in][out] float64), and the per-tensor weight scales."""
+
+    def __init__(self, params, layout):
+        g, w1, w2, w3 = split_params(params, layout)
+        self.layout = layout
+        self.grid = fp16(g)
+        self.sw = [fp8_pow2_scale(np.abs(w).max()) for w in (w1, w2, w3)]
+        self.w = [fp8_e4m3(w.astype(np.float64) / s) for w, s in zip((w1, w2, w3), self.sw)]
+
+
+def fp8_input_scales(amax):
+    """Input scales of the three layers from recorded activation maxima: twice the smallest power of two
+    that brings the maximum to 448 or below (one power of two of headroom)."""
+    return [2.0 * fp8_pow2_scale(a) for a in amax]
+
+
+def fp8_k_unit(g, j):
+    """Input unit of k position 8 g + j in a 32-wide k-block (the GPU's permutation of the units)."""
+    return 4 * g + j if j < 4 else 16 + 4 * g + (j - 4)
+
+
+def fp8_groups(layer):
+    """The input units summed by one lane group of one FP8 MFMA (8 products each), per layer."""
+    if layer == 0:
+        groups = [[fp8_k_unit(g, j) for j in range(8)] for g in range(4)]
+        return groups + [list(range(32, 40))]  # normal, wo, 1 (lane group 0 of k-block 1)
+    return [[32 * kb + fp8_k_unit(g, j) for j in range(8)] for kb in range(2) for g in range(4)]
+
+
+# The FP8 MFMA's accumulation, measured on gfx950 with an in-bounds probe: each lane group's 8 products
+# are added in a fixed-point adder aligned to the group's largest product, 2^E <= |p| < 2^(E+1), with
+# LSB 2^(E - 14); smaller products lose the bits below it (truncation). Group sums and the accumulator
+# input are added in FP32. The bound below charges 2^(E - 13) (twice the LSB) per product of a group
+# besides its largest, plus FP32 rounding of every group sum.
+FP8_GROUP_LSB_BITS = 13
+
+
+def _group_error(mag_x, w, layer):
+    """Bound on the MFMA's deviation from the exact dot product, per (query, output)."""
+    err = np.zeros((mag_x.shape[0], w.shape[1]))
+    total = np.abs(mag_x) @ np.abs(w)
+    for grp in fp8_groups(layer):
+        p = np.abs(mag_x[:, grp])[:, :, None] * np.abs(w[grp, :])[None, :, :]  # (n, 8, out)
+        m = p.max(axis=1)
+        nz = (p > 0).sum(axis=1)
+        _, ex = np.frexp(np.where(m > 0, m, 1.0))
+        err += np.where(m > 0, np.maximum(nz - 1, 0) * np.ldexp(1.0, ex - 1 - FP8_GROUP_LSB_BITS), 0.0)
+    return err + 2.0 * 10 * 2.0**-24 * total
+
+
+def forward_fp8(net, queries, s_in):
+    """FP8 forward: raw outputs (n, 48) and a bound on |GPU - oracle| per output. Every layer input is
+    carried as an interval [lo, hi]: the FP32 error of the grid sums, the MFMA's group-aligned
+    accumulation (_group_error) and the FP8 rounding of both ends, which is monotone. The nominal value
+    rounds the oracle's own exact sums."""
+    x, ex = encode(net, queries)
+    nom = fp8_e4m3(x / s_in[0])
+    lo, hi = fp8_e4m3((x - ex) / s_in[0]), fp8_e4m3((x + ex) / s_in[0])
+    for layer in range(3):
+        w = net.w[layer]
+        wp, wn = np.maximum(w, 0.0), np.minimum(w, 0.0)
+        a_nom = nom @ w
+        acc_err = _group_error(np.maximum(np.abs(lo), np.abs(hi)), w, layer)
+        a_lo = lo @ wp + hi @ wn - acc_err
+        a_hi = hi @ wp + lo @ wn + acc_err
+        c = s_in[layer] * net.sw[layer]
+        if layer < 2:
+            nxt = s_in[layer + 1]
+            nom = fp8_e4m3(np.maximum(a_nom * c, 0.0) / nxt)
+            lo = fp8_e4m3(np.maximum(a_lo * c, 0.0) / nxt)
+            hi = fp8_e4m3(np.maximum(a_hi * c, 0.0) / nxt)
+        else:
+            out = a_nom * c
+            bound = np.maximum(a_hi * c - out, out - a_lo * c)
+    return out, bound
 
 
 def head(o):

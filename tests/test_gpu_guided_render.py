@@ -3,7 +3,8 @@
 1. With the guide's sampling fraction at 0 the guided entry point must reproduce the plain render bit
    for bit (same Sobol samples, same estimator), while it still records paths and trains: the plumbing
    around the estimator changes nothing.
-2. With the trained guide sampling half of the directions the estimator stays unbiased: block means of
+2. With the trained guide sampling half of the directions the estimator stays unbiased, with BF16 and
+   with FP8 (e4m3) inference of the network: block means of
    the per-pass images agree with an FP64 plain reference within their standard errors (z-test), and
    relMSE keeps falling as 1 / spp.
 3. Training lowers the cross-entropy of the mixture on the radiance samples below that of a cosine lobe
@@ -32,12 +33,13 @@ def _params(ks, spp, interval=1, **kw):
                               mis_mode=ks.KS_MIS_CONSISTENT, clamp_direct=0.0, clamp_indirect=0.0, **kw)
 
 
-def test_zero_fraction_reproduces_plain(ks, device, room):
+@pytest.mark.parametrize("fp8", [0, 1])
+def test_zero_fraction_reproduces_plain(ks, device, room, fp8):
     hs, sc = room
     plain = ks.Film(device, W, H)
     ks.small_pt_render(sc, hs.camera, _params(ks, 16, 16), plain)
     info = sc.info()
-    g = ks.Guide(device, info["bound_lo"], info["bound_hi"], ks.guide_config(seed=2))
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"], ks.guide_config(seed=2, inference_fp8=fp8))
     g.set_mode(0.0, 4)  # never sample the guide, but keep recording and training
     guided = ks.Film(device, W, H)
     st = ks.small_pt_render_guided(sc, hs.camera, _params(ks, 16, 4), g, guided)
@@ -52,13 +54,15 @@ def test_zero_fraction_reproduces_plain(ks, device, room):
     guided.close()
 
 
-def test_guided_is_unbiased(ks, device, room):
+@pytest.mark.parametrize("fp8", [0, 1])
+def test_guided_is_unbiased(ks, device, room, fp8):
     hs, sc = room
     ref = ks.Film(device, W, H)
     ks.small_pt_render(sc, hs.camera, _params(ks, 8192, 8192), ref)
     ref_rgb, ref_w = ref.download()
     info = sc.info()
-    g = ks.Guide(device, info["bound_lo"], info["bound_hi"], ks.guide_config(seed=3, guide_fraction=0.5))
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"],
+                 ks.guide_config(seed=3, guide_fraction=0.5, inference_fp8=fp8))
     film = ks.Film(device, W, H)
     sums = []
 
