@@ -71,14 +71,22 @@ def cpu_baseline(hs, cam, kw, rows: int, threads: int, budget_s: float) -> dict:
     }
 
 
-def time_to_relmse(ks, dev, args) -> dict:
-    """Wall time for the plain and the guided tracer to bring the room (1280x720, 16 bounces,
-    consistent MIS, no clamping) to relMSE <= target against an FP64 plain reference of ref_spp."""
-    hs = ks.procedural_scene("room", 3)
+def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, max_spp=None, pass_spp=None,
+                   methods=(("guided", 0),)) -> dict:
+    """Wall time for the plain and the guided tracer to bring a scene (16 bounces, consistent MIS, no
+    clamping) to relMSE <= target against an FP64 plain reference of ref_spp. methods: (name, fp8) of the
+    guided runs."""
+    hs = ks.procedural_scene(*scene)
+    t0 = time.perf_counter()
     sc = ks.Scene(dev, hs)
+    build_s = time.perf_counter() - t0
     info = sc.info()
-    cam = hs.camera
-    W, H = args.room_width, args.room_height
+    cam = dict(hs.camera)
+    W, H = size or (args.room_width, args.room_height)
+    cam["aspect"] = W / H
+    ref_spp = ref_spp or args.ref_spp
+    max_spp = max_spp or args.room_max_spp
+    pass_spp = pass_spp or args.plain_pass_spp
 
     def params(spp, interval):
         return ks.small_pt_params(bounces=16, render_width=W, render_height=H, spp=spp, spp_prog_interval=interval,
@@ -86,12 +94,13 @@ def time_to_relmse(ks, dev, args) -> dict:
 
     ref = ks.Film(dev, W, H)
     t0 = time.perf_counter()
-    st = ks.small_pt_render(sc, cam, params(args.ref_spp, 256), ref)
+    st = ks.small_pt_render(sc, cam, params(ref_spp, 256), ref)
     ref_s = time.perf_counter() - t0
-    log(f"room reference: {args.ref_spp} spp in {ref_s:.1f} s ({st['paths'] / ref_s / 1e6:.0f} Mpaths/s)")
+    log(f"{scene[0]} reference: {ref_spp} spp in {ref_s:.1f} s ({st['paths'] / ref_s / 1e6:.0f} Mpaths/s)")
     target = args.relmse_target
-    res = {"scene": f"room ({hs.n_tris} tris), {W}x{H}, 16 bounces, consistent MIS, no clamping",
-           "target": target, "reference_spp": args.ref_spp, "reference_s": round(ref_s, 1)}
+    res = {"scene": f"{scene[0]} ({hs.n_tris} tris), {W}x{H}, 16 bounces, consistent MIS, no clamping",
+           "target": target, "reference_spp": ref_spp, "reference_s": round(ref_s, 1),
+           "reference_mpaths_per_s": round(st["paths"] / ref_s / 1e6, 1), "scene_build_s": round(build_s, 2)}
 
     # one progressive render per method; the progress callback measures relMSE every pass_spp samples
     film = ks.Film(dev, W, H)
@@ -101,31 +110,33 @@ def time_to_relmse(ks, dev, args) -> dict:
         passes.append((spp, time.perf_counter() - t_start, f.relmse(ref)))
 
     t_start = time.perf_counter()
-    ks.small_pt_render(sc, cam, params(args.room_max_spp, args.plain_pass_spp), film, progress=on_progress)
+    ks.small_pt_render(sc, cam, params(max_spp, pass_spp), film, progress=on_progress)
     hit = next(((s, t, r) for s, t, r in passes if r <= target), None)
     s_last, _, r_last = passes[-1]
-    res["plain"] = {"pass_spp": args.plain_pass_spp, "seconds": round(hit[1], 3) if hit else None,
+    res["plain"] = {"pass_spp": pass_spp, "seconds": round(hit[1], 3) if hit else None,
                     "spp": hit[0] if hit else None, "relmse_at_max_spp": r_last, "max_spp": s_last}
     log(f"plain: relMSE {target} at {hit}")
 
-    cfg = ks.guide_config(seed=1, guide_fraction=args.guide_fraction, train_spp=args.train_spp)
-    guide = ks.Guide(dev, info["bound_lo"], info["bound_hi"], cfg)
-    film.clear()
-    passes.clear()
-    t_start = time.perf_counter()
-    st = ks.small_pt_render_guided(sc, cam, params(args.room_max_spp, args.plain_pass_spp), guide, film,
-                                   progress=on_progress)
-    hit_g = next(((s, t, r) for s, t, r in passes if r <= target), None)
-    gst = guide.stats()
-    res["guided"] = {"guide_fraction": args.guide_fraction, "train_spp": args.train_spp,
+    for name, fp8 in methods:
+        cfg = ks.guide_config(seed=1, guide_fraction=args.guide_fraction, train_spp=args.train_spp, inference_fp8=fp8)
+        guide = ks.Guide(dev, info["bound_lo"], info["bound_hi"], cfg)
+        film.clear()
+        passes.clear()
+        t_start = time.perf_counter()
+        ks.small_pt_render_guided(sc, cam, params(max_spp, pass_spp), guide, film, progress=on_progress)
+        hit_g = next(((s, t, r) for s, t, r in passes if r <= target), None)
+        gst = guide.stats()
+        res[name] = {"guide_fraction": args.guide_fraction, "train_spp": args.train_spp,
+                     "inference": "fp8" if fp8 else "bf16",
                      "seconds": round(hit_g[1], 3) if hit_g else None, "spp": hit_g[0] if hit_g else None,
                      "relmse_at_max_spp": passes[-1][2], "max_spp": passes[-1][0], "adam_steps": gst["steps"],
-                     "training_samples": gst["samples"],
-                     "kernel_ms": {k: round(v, 1) for k, v in st["kernel_ms"].items()}}
-    log(f"guided: relMSE {target} at {hit_g}")
-    if hit and hit_g:
-        res["speedup_guided_vs_plain"] = round(hit[1] / hit_g[1], 3)
-    guide.close()
+                     "training_samples": gst["samples"]}
+        log(f"{name}: relMSE {target} at {hit_g}")
+        if hit and hit_g:
+            res[name]["speedup_vs_plain"] = round(hit[1] / hit_g[1], 3)
+        guide.close()
+    if "guided" in res and "speedup_vs_plain" in res["guided"]:
+        res["speedup_guided_vs_plain"] = res["guided"]["speedup_vs_plain"]
     film.close()
     ref.close()
     sc.close()
@@ -205,6 +216,15 @@ def main() -> int:
     ap.add_argument("--guide-fraction", type=float, default=0.3)
     ap.add_argument("--train-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 48-view leg")
+    ap.add_argument("--fp8-relmse", action="store_true",
+                    help="also time the guided tracer with FP8 inference in the room's time-to-relMSE leg")
+    ap.add_argument("--scale-leg", action="store_true",
+                    help="time-to-relMSE at scale: the room with 30 icospheres (10M triangles) at "
+                    "--scale-width x --scale-height, plain vs guided with FP8 inference")
+    ap.add_argument("--scale-width", type=int, default=3840)
+    ap.add_argument("--scale-height", type=int, default=2160)
+    ap.add_argument("--scale-ref-spp", type=int, default=1024)
+    ap.add_argument("--scale-max-spp", type=int, default=512)
     ap.add_argument("--batch-views", type=int, default=48)
     ap.add_argument("--batch-seed", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -321,9 +341,18 @@ def main() -> int:
     ttr = None
     if not args.no_relmse:
         try:
-            ttr = time_to_relmse(ks, dev, args)
+            methods = (("guided", 0), ("guided_fp8", 1)) if args.fp8_relmse else (("guided", 0),)
+            ttr = time_to_relmse(ks, dev, args, methods=methods)
         except ks.KsError as e:  # reported, not hidden: the plain-throughput line stays valid
             ttr = {"error": str(e)}
+    scale = None
+    if args.scale_leg:
+        try:
+            scale = time_to_relmse(ks, dev, args, scene=("room_spheres", 4), size=(args.scale_width, args.scale_height),
+                                   ref_spp=args.scale_ref_spp, max_spp=args.scale_max_spp,
+                                   methods=(("guided_fp8", 1),))
+        except ks.KsError as e:
+            scale = {"error": str(e)}
 
     n_rays = (totals["rays_closest"] + totals["rays_shadow"]) / args.steps
     out = {
@@ -360,6 +389,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
+        "time_to_relmse_at_scale": scale,
         "batched_views": batch,
     }
     print(json.dumps(out), flush=True)

@@ -318,7 +318,8 @@ int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *di
 int ks_vmm_sample(ks_device *dev, int64_t n, const float *mixtures, const float *u, float *dirs, float *pdf);
 
 /* ---- procedural scenes of the benchmark configurations (no reference counterpart: the reference
- *      loads meshes from files, mesh_asset.cpp). name: "cornell", "spheres", "room". ---- */
+ *      loads meshes from files, mesh_asset.cpp). name: "cornell", "spheres", "spheres_small", "room",
+ *      "room_spheres" (the room plus 30 icospheres, 10M triangles). ---- */
 int ks_host_scene_create(const char *name, uint64_t seed, ks_host_scene **out);
 int ks_host_scene_get(const ks_host_scene *hs, ks_scene_desc *desc, ks_camera_desc *cam);
 int ks_host_scene_free(ks_host_scene *hs);

@@ -162,6 +162,8 @@ class Guide {
         check(ks_guide_get_stats(h_, &st), "ks_guide_get_stats");
         return st;
     }
+    // FP8 (e4m3) or BF16 MLP inference for later renders
+    void set_inference(bool fp8) { check(ks_guide_set_inference(h_, fp8 ? 1 : 0), "ks_guide_set_inference"); }
     ks_guide *handle() const { return h_; }
 
   private:

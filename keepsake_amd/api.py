@@ -403,7 +403,7 @@ def camera_desc(cam: dict) -> CameraDesc:
 
 
 def procedural_scene(name: str, seed: int = 0) -> HostScene:
-    """Builds one of the benchmark scenes ("cornell", "spheres", "spheres_small", "room")."""
+    """Builds one of the benchmark scenes ("cornell", "spheres", "spheres_small", "room", "room_spheres")."""
     lib = load_library()
     h = C.c_void_p()
     _check(lib.ks_host_scene_create(name.encode(), seed, C.byref(h)))

@@ -290,6 +290,40 @@ void room(HostScene &s, uint64_t seed) {
     set_camera(s, {0.6f, 2.2f, 1.2f}, {6.0f, 1.0f, 6.0f}, {0, 1, 0}, 60.0f, 16.0f / 9.0f);
 }
 
+// The room of room() (boxes from seed 3) plus 30 displaced icospheres of 327,680 triangles each
+// (subdivision level 7 of the 20-face icosahedron), seed 4: 10,032,258 triangles. Sphere centres are
+// uniform in the room's interior, radii U[0.15, 0.45], displacement 0.01 N(0, 1) along the normal;
+// per-sphere principled BSDF as in spheres().
+void room_spheres(HostScene &s, uint64_t seed) {
+    room(s, 3);
+    Pcg32 rng(seed);
+    Builder b(s);
+    b.geom = (uint32_t)s.geom_material.size();
+    std::vector<V> uv;
+    std::vector<uint32_t> ui;
+    icosphere(7, uv, ui);
+    for (int k = 0; k < 30; ++k) {
+        V c{rng.uniform(0.6f, 9.4f), rng.uniform(0.6f, 3.4f), rng.uniform(0.6f, 9.4f)};
+        float r = rng.uniform(0.15f, 0.45f);
+        ks_material_desc m{};
+        m.basecolor[0] = rng.uniform(0.1f, 0.9f);
+        m.basecolor[1] = rng.uniform(0.1f, 0.9f);
+        m.basecolor[2] = rng.uniform(0.1f, 0.9f);
+        m.roughness = rng.uniform(0.05f, 1.0f);
+        m.metallic = rng.uniform() < 0.3f ? 1.0f : 0.0f;
+        m.ior = 1.5f;
+        m.specular_r0_mul = 0.5f;
+        s.materials.push_back(m);
+        uint32_t base = (uint32_t)(s.vertices.size() / 3);
+        for (const V &n : uv) {
+            float disp = 0.01f * rng.normal();
+            b.vert(c + n * (r + disp));
+        }
+        for (size_t f = 0; f < ui.size(); f += 3) b.tri(base + ui[f], base + ui[f + 1], base + ui[f + 2]);
+        b.end_geom((uint32_t)s.materials.size() - 1);
+    }
+}
+
 } // namespace
 
 bool make_procedural_scene(const std::string &name, uint64_t seed, HostScene &out) {
@@ -303,6 +337,8 @@ bool make_procedural_scene(const std::string &name, uint64_t seed, HostScene &ou
         spheres(out, seed, 8, 3);
     } else if (name == "room") {
         room(out, seed);
+    } else if (name == "room_spheres") {
+        room_spheres(out, seed);
     } else {
         return false;
     }

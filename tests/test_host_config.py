@@ -181,3 +181,14 @@ log2_table_size = 18
 def test_guide_needs_consistent_mis(tmp_path):
     r = resolve(tmp_path, BASE + "[task.guide]\nfraction = 0.5\n")
     assert r.returncode != 0 and "consistent" in r.stderr
+
+
+def test_guide_inference_precision(tmp_path):
+    text = BASE + """mis_mode = "consistent"
+[task.guide]
+inference = "fp8"
+"""
+    (t,) = tasks(tmp_path, text)
+    assert t["guide"]["inference_fp8"] == 1
+    r = resolve(tmp_path, text.replace('"fp8"', '"fp4"'))
+    assert r.returncode != 0 and "bf16" in r.stderr
