@@ -50,6 +50,8 @@ typedef struct ks_material_desc {
 #define KS_LIGHT_SKY_AMBIENT 0 /* SkyLight(const color3 &ambient) light.cpp:52-57 */
 #define KS_LIGHT_DIRECTIONAL 1 /* DirectionalLight light.h:91-105 */
 #define KS_LIGHT_POINT 2       /* PointLight light.h:107-120 */
+#define KS_LIGHT_SKY_MAP 3     /* SkyLight(map, l2w, transform_y_up, strength) light.cpp:19-49: the scene's sky_map;
+                                  color scales nothing (the map's strength does) */
 
 /* Non-mesh lights as created by create_light/create_sky_light (light.cpp:237-265). */
 typedef struct ks_light_desc {
@@ -57,6 +59,16 @@ typedef struct ks_light_desc {
     float color[3]; /* ambient radiance, L, or I */
     float vec[3];   /* direction (directional) or position (point) */
 } ks_light_desc;
+
+/* An equirectangular HDR sky (SkyLight from a .hdr / .exr map, light.cpp:19-49; create_sky_light
+ * light.cpp:258-272). The library builds the thresholded luminance distribution itself. */
+typedef struct ks_sky_map_desc {
+    const float *rgb;        /* width * height * 3 linear RGB, row 0 at theta = 0 (the map's up pole) */
+    int32_t width, height;
+    float to_world[9];       /* row-major 3x3 direction part of the map's l2w transform */
+    int32_t transform_y_up;  /* 1: the map's up axis is +y (to_spherical_yup), 0: +z (to_spherical) */
+    float strength;          /* radiance scale */
+} ks_sky_map_desc;
 
 /* Flat scene: the geometry Scene::intersect1/occlude1 see (scene.h:49-87) with an identity instance
  * transform and one material per geometry (SubScene::materials, scene.h:37). Triangles of one geometry
@@ -76,6 +88,7 @@ typedef struct ks_scene_desc {
     const ks_light_desc *lights;
     uint32_t n_lights;
     const uint8_t *geom_flags; /* per geometry (KS_GEOM_*), or NULL: two-sided, smooth normals when given */
+    const ks_sky_map_desc *sky_map; /* the map of the (one) KS_LIGHT_SKY_MAP light, or NULL */
 } ks_scene_desc;
 
 #define KS_GEOM_ONE_SIDED 1u /* MeshData::twosided = false: the frame is not flipped towards wo (geometry.cpp:126) */

@@ -44,13 +44,19 @@ class Device {
 };
 
 // Host-side scene description: flat triangles grouped by geometry, one material per geometry,
-// non-mesh lights (sky ambient, directional, point) in the order the light sampler sees them.
+// non-mesh lights (sky ambient, sky map, directional, point) in the order the light sampler sees them.
 struct SceneData {
     std::vector<float> vertices, normals;
     std::vector<uint32_t> indices, tri_geom, geom_material;
     std::vector<uint8_t> geom_flags; // per geometry KS_GEOM_* bits, empty: two-sided with smooth normals
     std::vector<ks_material_desc> materials;
     std::vector<ks_light_desc> lights;
+    // environment map of the KS_LIGHT_SKY_MAP light (rows of RGB texels, row 0 at theta = 0)
+    std::vector<float> sky_rgb;
+    int sky_width = 0, sky_height = 0;
+    std::array<float, 9> sky_to_world{1, 0, 0, 0, 1, 0, 0, 0, 1}; // row-major
+    bool sky_y_up = true;
+    float sky_strength = 1.0f;
 
     ks_scene_desc desc() const {
         ks_scene_desc d{};
@@ -67,8 +73,20 @@ struct SceneData {
         d.lights = lights.empty() ? nullptr : lights.data();
         d.n_lights = (uint32_t)lights.size();
         d.geom_flags = geom_flags.empty() ? nullptr : geom_flags.data();
+        if (!sky_rgb.empty()) {
+            sky_desc_.rgb = sky_rgb.data();
+            sky_desc_.width = sky_width;
+            sky_desc_.height = sky_height;
+            for (int k = 0; k < 9; ++k) sky_desc_.to_world[k] = sky_to_world[k];
+            sky_desc_.transform_y_up = sky_y_up ? 1 : 0;
+            sky_desc_.strength = sky_strength;
+            d.sky_map = &sky_desc_;
+        }
         return d;
     }
+
+  private:
+    mutable ks_sky_map_desc sky_desc_{}; // points into sky_rgb; refreshed by every desc()
 };
 
 // Scene: create_scene + rtcCommitScene + build_mesh_lights + PowerLightSampler::build in one step.

@@ -26,6 +26,7 @@ KS_ERR_NO_DEVICE = -5
 KS_LIGHT_SKY_AMBIENT = 0
 KS_LIGHT_DIRECTIONAL = 1
 KS_LIGHT_POINT = 2
+KS_LIGHT_SKY_MAP = 3
 KS_MIS_REFERENCE = 0
 KS_MIS_CONSISTENT = 1
 KS_GUIDE_OFF = 0
@@ -59,6 +60,17 @@ class LightDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("color", C.c_float * 3), ("vec", C.c_float * 3)]
 
 
+class SkyMapDesc(C.Structure):
+    _fields_ = [
+        ("rgb", C.POINTER(C.c_float)),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("to_world", C.c_float * 9),
+        ("transform_y_up", C.c_int32),
+        ("strength", C.c_float),
+    ]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("vertices", C.POINTER(C.c_float)),
@@ -74,6 +86,7 @@ class SceneDesc(C.Structure):
         ("lights", C.POINTER(LightDesc)),
         ("n_lights", C.c_uint32),
         ("geom_flags", C.POINTER(C.c_uint8)),
+        ("sky_map", C.POINTER(SkyMapDesc)),
     ]
 
 
@@ -349,6 +362,9 @@ class HostScene:
     camera: dict
     normals: np.ndarray | None = None
     geom_flags: np.ndarray | None = None  # (G,) uint8 of KS_GEOM_* bits, None: two-sided, smooth normals
+    # environment map of a KS_LIGHT_SKY_MAP light: dict(rgb=(H, W, 3) float32, to_world=3x3, y_up=bool,
+    # strength=float), or None
+    sky_map: dict | None = None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -382,6 +398,18 @@ class HostScene:
         d.lights = C.cast(lights.ctypes.data, C.POINTER(LightDesc)) if lights.shape[0] else None
         d.n_lights = lights.shape[0]
         d.geom_flags = gf.ctypes.data_as(C.POINTER(C.c_uint8)) if gf is not None else None
+        if self.sky_map is not None:
+            rgb = np.ascontiguousarray(self.sky_map["rgb"], np.float32)
+            sm = SkyMapDesc()
+            sm.rgb = rgb.ctypes.data_as(C.POINTER(C.c_float))
+            sm.height, sm.width = int(rgb.shape[0]), int(rgb.shape[1])
+            m = np.asarray(self.sky_map.get("to_world", np.eye(3)), np.float32).reshape(9)
+            for k in range(9):
+                sm.to_world[k] = float(m[k])
+            sm.transform_y_up = 1 if self.sky_map.get("y_up", True) else 0
+            sm.strength = float(self.sky_map.get("strength", 1.0))
+            self._keep += [rgb, sm]
+            d.sky_map = C.pointer(sm)
         return d
 
 

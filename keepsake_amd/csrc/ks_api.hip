@@ -344,6 +344,12 @@ KS_DEV f3 light_sample(const DevScene &sc, const Light &l, f3 p, float u0, float
         wi_dist = kInf;
         return col / pdf;
     }
+    if (l.type == 4) { // SkyLight::sample (light.cpp:88-111), environment map
+        float e[3];
+        sky_map_sample(sc.sky_map, u0, u1, wi.x, wi.y, wi.z, pdf, e);
+        wi_dist = kInf;
+        return mk(e[0], e[1], e[2]);
+    }
     if (l.type == 1) { // DirectionalLight::sample (light.cpp:207-213)
         wi = mk(l.vec[0], l.vec[1], l.vec[2]);
         wi_dist = kInf;
@@ -462,12 +468,19 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
             f3 wi = normalized(rd);
             for (int k = 0; k < sc.n_sky; ++k) { // small_pt.cpp:85-109
                 const Light &l = sc.lights[sc.sky[k]];
+                const bool map = l.type == 4;
                 f3 Le = mk(l.color[0], l.color[1], l.color[2]);
+                if (map) {
+                    float e[3];
+                    sky_eval(sc.sky_map, wi.x, wi.y, wi.z, e);
+                    Le = mk(e[0], e[1], e[2]);
+                }
                 if (bounce == 0) {
                     if (vr.include_background) L = L + beta * Le;
                 } else {
                     float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)sc.sky[k]);
-                    float mis = power_heur(pdf_wi_bsdf, sky_pdf(wi) * pr);
+                    float lpdf = map ? sky_map_pdf(sc.sky_map, wi.x, wi.y, wi.z) : sky_pdf(wi);
+                    float mis = power_heur(pdf_wi_bsdf, lpdf * pr);
                     f3 bu = Le * mis;
                     if (!all_finite(bu)) ++nonfinite;
                     if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
@@ -987,6 +1000,7 @@ struct ks_device {
 struct ks_scene {
     ks_device *dev = nullptr;
     DevBuf nodes, tris, verts, normals, tri_idx, tri_light, mats, lights, cdf;
+    DevBuf sky_rgb, sky_cond, sky_marg; // environment map and its sampling tables
     DevScene ds{};
     ks_scene_info info{};
     std::vector<int32_t> light_prim;
@@ -1385,6 +1399,104 @@ int ks_device_set_timing(ks_device *dev, int32_t enable) {
     return KS_OK;
 }
 
+} // extern "C"
+
+namespace {
+
+// DistribTable (distrib.cpp:9-29): running sum of f / n, normalised, last entry forced to 1; returns acc.
+float build_cdf(const float *f, uint32_t n, float *cdf) {
+    cdf[0] = 0.0f;
+    float invn = 1.0f / (float)n;
+    for (uint32_t i = 1; i < n + 1; ++i) cdf[i] = cdf[i - 1] + f[i - 1] * invn;
+    float acc = cdf[n];
+    if (acc == 0.0f) {
+        for (uint32_t i = 0; i < n + 1; ++i) cdf[i] = (float)i / (float)n;
+    } else {
+        float invacc = 1.0f / acc;
+        for (uint32_t i = 0; i < n + 1; ++i) cdf[i] *= invacc;
+    }
+    cdf[n] = 1.0f;
+    return acc;
+}
+
+// SkyLight(path, ...) (light.cpp:19-49): texels, and the 2D distribution of the luminance weighted by
+// sin(theta) minus its mean, clamped at 0 (the thresholded table of the reference). Also the light's
+// power (SkyLight::power, light.cpp:136-158).
+struct HostSky {
+    std::vector<float4> rgb;
+    std::vector<float> cond, marg;
+    SkyMap map{};
+    float power[3] = {0, 0, 0};
+};
+
+int build_sky(const ks_sky_map_desc &d, float scene_radius, HostSky &out) {
+    KS_REQUIRE(d.rgb && d.width > 0 && d.height > 0, "sky map without texels");
+    KS_REQUIRE((int64_t)d.width * d.height < (1ll << 31), "sky map too large");
+    const uint32_t w = (uint32_t)d.width, h = (uint32_t)d.height;
+    const float pi = 3.14159265359f, two_pi = 6.28318530718f;
+    out.rgb.resize((size_t)w * h);
+    std::vector<float> lum((size_t)w * h);
+    float lum_sum = 0.0f;
+    for (uint32_t i = 0; i < w * h; ++i) {
+        const float *c = d.rgb + 3 * (size_t)i;
+        out.rgb[i] = make_float4(c[0], c[1], c[2], 0.0f);
+        uint32_t y = i / w;
+        float theta = ((float)y + 0.5f) / (float)h * pi;
+        float sin_theta = std::sin(theta);
+        lum[i] = sin_theta * (0.212671f * c[0] + 0.715160f * c[1] + 0.072169f * c[2]); // luminance, maths.h:675
+        lum_sum += lum[i];
+    }
+    float average = lum_sum / (float)lum.size();
+    for (float &l : lum) l = std::max(l - average, 0.0f);
+    out.cond.resize((size_t)h * (w + 1));
+    out.marg.resize(h + 1);
+    std::vector<float> accs(h);
+    for (uint32_t y = 0; y < h; ++y) accs[y] = build_cdf(lum.data() + (size_t)y * w, w, out.cond.data() + (size_t)y * (w + 1));
+    build_cdf(accs.data(), h, out.marg.data());
+    SkyMap &m = out.map;
+    m.w = (int32_t)w;
+    m.h = (int32_t)h;
+    m.yup = d.transform_y_up ? 1 : 0;
+    m.strength = d.strength;
+    double a[9], det;
+    for (int k = 0; k < 9; ++k) a[k] = d.to_world[k];
+    det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+    KS_REQUIRE(det != 0.0, "sky map to_world is singular");
+    const double inv[9] = {(a[4] * a[8] - a[5] * a[7]) / det, (a[2] * a[7] - a[1] * a[8]) / det,
+                           (a[1] * a[5] - a[2] * a[4]) / det, (a[5] * a[6] - a[3] * a[8]) / det,
+                           (a[0] * a[8] - a[2] * a[6]) / det, (a[2] * a[3] - a[0] * a[5]) / det,
+                           (a[3] * a[7] - a[4] * a[6]) / det, (a[1] * a[6] - a[0] * a[7]) / det,
+                           (a[0] * a[4] - a[1] * a[3]) / det};
+    for (int k = 0; k < 9; ++k) {
+        m.l2w[k] = d.to_world[k];
+        m.w2l[k] = (float)inv[k];
+    }
+    // power: the map integrated over a z-up grid of texel centres (light.cpp:136-158)
+    m.rgb = out.rgb.data();
+    float sum[3] = {0, 0, 0};
+    for (uint32_t row = 0; row < h; ++row) {
+        float theta = ((float)row + 0.5f) / (float)h * pi;
+        float sin_theta = std::sin(theta);
+        float row_sum[3] = {0, 0, 0};
+        for (uint32_t col = 0; col < w; ++col) {
+            float phi = ((float)col + 0.5f) / (float)w * two_pi;
+            float ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+            float L[3];
+            sky_eval(m, cp * st, sp * st, ct, L);
+            for (int k = 0; k < 3; ++k) row_sum[k] += L[k] * sin_theta;
+        }
+        for (int k = 0; k < 3; ++k) sum[k] += row_sum[k];
+    }
+    for (int k = 0; k < 3; ++k)
+        out.power[k] = ((sum[k] * ((pi / (float)h) * (two_pi / (float)w))) * (pi * (scene_radius * scene_radius)));
+    m.rgb = nullptr; // device pointers are set after upload
+    return KS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
 int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options *opt, ks_scene **out) {
     KS_REQUIRE(dev && d && out, "null argument");
     *out = nullptr;
@@ -1434,17 +1546,28 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     std::vector<float> powers;
     int n_sky = 0;
     int sky[4] = {0, 0, 0, 0};
+    HostSky host_sky;
+    bool have_map = false;
     for (uint32_t i = 0; i < d->n_lights; ++i) {
         const ks_light_desc &L = d->lights[i];
-        KS_REQUIRE(L.type >= 0 && L.type <= 2, "unknown light type");
+        KS_REQUIRE(L.type >= 0 && L.type <= 3, "unknown light type");
         Light l{};
-        l.type = L.type;
+        l.type = L.type == KS_LIGHT_SKY_MAP ? 4 : L.type;
         for (int k = 0; k < 3; ++k) {
             l.color[k] = L.color[k];
             l.vec[k] = L.vec[k];
         }
         float ph[3];
-        if (L.type == KS_LIGHT_SKY_AMBIENT) { // SkyLight::power (light.cpp:136-158), 1x1 map
+        if (L.type == KS_LIGHT_SKY_MAP) {
+            KS_REQUIRE(d->sky_map, "a KS_LIGHT_SKY_MAP light needs ks_scene_desc.sky_map");
+            KS_REQUIRE(!have_map, "at most one environment-map sky per scene");
+            KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
+            int e = build_sky(*d->sky_map, scene_radius, host_sky);
+            if (e) return e;
+            have_map = true;
+            sky[n_sky++] = (int)i;
+            for (int k = 0; k < 3; ++k) ph[k] = host_sky.power[k];
+        } else if (L.type == KS_LIGHT_SKY_AMBIENT) { // SkyLight::power (light.cpp:136-158), 1x1 map
             KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
             sky[n_sky++] = (int)i;
             float sin_theta = std::sin(0.5f / 1.0f * pi);
@@ -1552,6 +1675,14 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     KS_HIP(upload(sc->mats, mats, s));
     KS_HIP(upload(sc->lights, lights, s));
     KS_HIP(upload(sc->cdf, cdf, s));
+    if (have_map) {
+        KS_HIP(upload(sc->sky_rgb, host_sky.rgb, s));
+        KS_HIP(upload(sc->sky_cond, host_sky.cond, s));
+        KS_HIP(upload(sc->sky_marg, host_sky.marg, s));
+        host_sky.map.rgb = sc->sky_rgb.as<float4>();
+        host_sky.map.cond = sc->sky_cond.as<float>();
+        host_sky.map.marg = sc->sky_marg.as<float>();
+    }
 
     DevScene &ds = sc->ds;
     ds.nodes = sc->nodes.as<float4>();
@@ -1567,6 +1698,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     ds.n_lights = (int32_t)lights.size();
     ds.n_sky = n_sky;
     for (int k = 0; k < 4; ++k) ds.sky[k] = sky[k];
+    ds.sky_map = host_sky.map;
 
     ks_scene_info &in = sc->info;
     in.n_tris = d->n_tris;

@@ -208,7 +208,7 @@ struct Material { // PrincipledBSDF closure after eval_closure clamps (principle
 static_assert(sizeof(Material) == 64, "Material record is 64 bytes");
 
 struct Light {
-    int32_t type; // 0 ambient sky, 1 directional, 2 point, 3 mesh triangle
+    int32_t type; // 0 ambient sky, 1 directional, 2 point, 3 mesh triangle, 4 environment-map sky
     uint32_t prim;
     float area;
     float pad;

@@ -2,6 +2,7 @@
 // the scene view, the wavefront control block, wave-level append/reduce helpers and hit reconstruction.
 #pragma once
 #include "ks_device.h"
+#include "ks_sky.h"
 
 namespace ksr {
 using namespace ksd;
@@ -40,6 +41,7 @@ struct DevScene {
     int32_t n_sky;
     int32_t sky[4];
     int32_t width; // BVH node width: 2 (GpuNode, 64 B) or 4 (GpuNode4, 128 B)
+    ksd::SkyMap sky_map; // the map of a light of type 4 (environment-map sky), when the scene has one
 };
 
 struct RenderConsts {

@@ -20,6 +20,7 @@
 #include "gltf_asset.h"
 #include "mesh_asset.h"
 
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -233,6 +234,41 @@ static Camera load_camera(const ConfigArgs &args) {
     return Camera(args.load_vec3("pos"), args.load_vec3("target"), args.load_vec3("up", true), vfov, aspect);
 }
 
+// create_sky_light (light.cpp:258-272): `ambient = [r, g, b]`, or an equirectangular HDR map
+// (`map` = path to .hdr or .exr, `to_world` transform of which the linear part rotates the map,
+// `transform_y_up` (default true: theta measured from +y), `strength` (default 1)). A scene holds at
+// most one map; its texels go to `scene`.
+static ks_light_desc load_sky_light(const ConfigArgs &sky, SceneData &scene) {
+    ks_light_desc l{};
+    if (sky.contains("ambient")) {
+        l.type = KS_LIGHT_SKY_AMBIENT;
+        auto a = sky.load_vec3("ambient");
+        for (int k = 0; k < 3; ++k) l.color[k] = a[k];
+        return l;
+    }
+    if (!scene.sky_rgb.empty()) throw ConfigError("at most one sky map per scene");
+    fs::path map = sky.load_path("map");
+    std::string ext = map.extension().string();
+    for (char &c : ext) c = (char)std::tolower((unsigned char)c);
+    if (ext == ".hdr") {
+        scene.sky_rgb = read_hdr(map, scene.sky_width, scene.sky_height);
+    } else if (ext == ".exr") {
+        scene.sky_rgb = read_exr(map, scene.sky_width, scene.sky_height);
+    } else {
+        throw ConfigError("unsupported sky map format '" + map.string() + "' (only .hdr or .exr)");
+    }
+    scene.sky_to_world = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (sky.contains("to_world")) {
+        std::array<float, 16> m = sky.load_transform("to_world");
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) scene.sky_to_world[3 * r + c] = m[4 * r + c];
+    }
+    scene.sky_y_up = sky.load_bool("transform_y_up", true);
+    scene.sky_strength = sky.load_float("strength", 1.0f);
+    l.type = KS_LIGHT_SKY_MAP;
+    return l;
+}
+
 SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
     SmallPTSetup st;
     ks_camera_desc scene_cam{};
@@ -260,15 +296,8 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
 
     if (args.contains("sky") || args.contains("light")) {
         st.scene.lights.clear();
-        if (args.contains("sky")) { // create_sky_light (light.cpp:247-262), ambient only
-            ConfigArgs sky = args["sky"];
-            if (!sky.contains("ambient")) throw ConfigError("sky maps are not supported; use sky.ambient");
-            ks_light_desc l{};
-            l.type = KS_LIGHT_SKY_AMBIENT;
-            auto a = sky.load_vec3("ambient");
-            for (int k = 0; k < 3; ++k) l.color[k] = a[k];
-            st.scene.lights.push_back(l);
-        }
+        st.scene.sky_rgb.clear();
+        if (args.contains("sky")) st.scene.lights.push_back(load_sky_light(args["sky"], st.scene));
         if (args.contains("light")) { // create_light (light.cpp:237-245)
             ConfigArgs la = args["light"];
             for (size_t i = 0; i < la.array_size(); ++i) {
@@ -293,10 +322,7 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
                     auto pos = li.load_vec3("pos");
                     for (int k = 0; k < 3; ++k) l.color[k] = I[k], l.vec[k] = pos[k];
                 } else if (t == "sky") {
-                    if (!li.contains("ambient")) throw ConfigError("sky maps are not supported; use ambient");
-                    l.type = KS_LIGHT_SKY_AMBIENT;
-                    auto a = li.load_vec3("ambient");
-                    for (int k = 0; k < 3; ++k) l.color[k] = a[k];
+                    l = load_sky_light(li, st.scene);
                 } else {
                     throw ConfigError("light type '" + t + "' is not supported");
                 }
@@ -378,7 +404,7 @@ std::string describe(const SmallPTSetup &st) {
     char buf[2048];
     std::snprintf(buf, sizeof(buf),
                   "{\"scene\": \"%s\", \"n_tris\": %zu, \"n_vertices\": %zu, \"n_geoms\": %zu, \"n_materials\": %zu, "
-                  "\"n_lights\": %zu, \"bounces\": %d, \"render_width\": %d, "
+                  "\"n_lights\": %zu, \"sky_map\": [%d, %d, %g, %d], \"bounces\": %d, \"render_width\": %d, "
                   "\"render_height\": %d, \"crop\": [%d, %d, %d, %d], \"spp\": %d, \"rng_seed\": %d, "
                   "\"include_background\": %s, \"backdrop\": [%g, %g, %g], \"clamp_direct\": %g, "
                   "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
@@ -388,7 +414,8 @@ std::string describe(const SmallPTSetup &st) {
                   "\"up\": [%g, %g, %g], \"vfov_deg\": %.6g, \"aspect\": %.6g, \"to_world\": [%g, %g, %g, %g, %g, %g, "
                   "%g, %g, %g, %g, %g, %g, %g, %g, %g, %g]}}",
                   st.scene_name.c_str(), st.scene.indices.size() / 3, st.scene.vertices.size() / 3,
-                  st.scene.geom_material.size(), st.scene.materials.size(), st.scene.lights.size(), in.bounces,
+                  st.scene.geom_material.size(), st.scene.materials.size(), st.scene.lights.size(), st.scene.sky_width,
+                  st.scene.sky_height, st.scene.sky_strength, st.scene.sky_y_up ? 1 : 0, in.bounces,
                   in.render_width, in.render_height, in.crop_start_x, in.crop_start_y, in.crop_width, in.crop_height,
                   in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
                   in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,

@@ -789,8 +789,184 @@ struct Node { // reference SBVHNode semantics (cuda/sbvh.cuh:18-27) with an expl
     uint32_t axis = 0, right = 0, prim_offset = 0, prim_count = 0;
 };
 
+// DistribTable (distrib.cpp:9-65): piecewise-constant distribution over [0, 1) from n weights.
+struct Distrib1D {
+    std::vector<float> cdf;
+    float acc = 0.0f;
+
+    void build(const float *f, uint32_t n) {
+        cdf.assign(n + 1, 0.0f);
+        const float invn = 1.0f / (float)n;
+        for (uint32_t i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + f[i - 1] * invn;
+        acc = cdf[n];
+        for (uint32_t i = 0; i <= n; ++i) cdf[i] = acc == 0.0f ? (float)i / (float)n : cdf[i] * (1.0f / acc);
+        cdf[n] = 1.0f;
+    }
+    uint32_t size() const { return (uint32_t)cdf.size() - 1; }
+    // sample_linear (distrib.cpp:43-55): continuous sample in [0, 1), density, bin
+    float sample_linear(float u, float &pdf, uint32_t &bin) const {
+        u = clampv(u, 0.0f, std::nextafter(1.0f, 0.0f));
+        bin = (uint32_t)(std::upper_bound(cdf.begin(), cdf.end(), u) - cdf.begin()) - 1;
+        const float lo = cdf[bin], hi = cdf[bin + 1];
+        pdf = (hi - lo) * (float)size();
+        return ((float)bin + (u - lo) / (hi - lo)) / (float)size();
+    }
+    float pdf(uint32_t bin) const { return (cdf[bin + 1] - cdf[bin]) * (float)size(); } // distrib.cpp:57-62
+};
+
+// DistribTable2D (distrib.cpp:78-107): one conditional table per row v, a marginal table over rows.
+struct Distrib2D {
+    std::vector<Distrib1D> rows;
+    Distrib1D margin;
+
+    void build(const std::vector<float> &f, uint32_t nu, uint32_t nv) {
+        rows.assign(nv, Distrib1D());
+        std::vector<float> row_weight(nv);
+        for (uint32_t v = 0; v < nv; ++v) {
+            rows[v].build(f.data() + (size_t)v * nu, nu);
+            row_weight[v] = rows[v].acc;
+        }
+        margin.build(row_weight.data(), nv);
+    }
+    void sample_linear(float u0, float u1, float &su, float &sv, float &pdf) const {
+        float pm, pc;
+        uint32_t bv, bu;
+        sv = margin.sample_linear(u1, pm, bv);
+        su = rows[bv].sample_linear(u0, pc, bu);
+        pdf = pm * pc;
+    }
+    float pdf(float u, float v) const {
+        const int nu = (int)rows[0].size(), nv = (int)rows.size();
+        int x = std::clamp((int)(u * (float)nu), 0, nu - 1);
+        int y = std::clamp((int)(v * (float)nv), 0, nv - 1);
+        return margin.pdf((uint32_t)y) * rows[y].pdf((uint32_t)x);
+    }
+};
+
+// SkyLight from an equirectangular HDR map (light.cpp:19-49 and :59-158). Texel (x, y) sits at
+// phi = 2 pi (x + 0.5) / w, theta = pi (y + 0.5) / h of the map frame.
+struct SkyMapLight {
+    int w = 0, h = 0;
+    std::vector<V3> texels;
+    float l2w[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, w2l[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    bool yup = true;
+    float strength = 1.0f;
+    Distrib2D distrib;
+
+    static V3 apply(const float m[9], V3 d) {
+        return V3(m[0] * d.x + m[1] * d.y + m[2] * d.z, m[3] * d.x + m[4] * d.y + m[5] * d.z,
+                  m[6] * d.x + m[7] * d.y + m[8] * d.z);
+    }
+    void init(const ora_sky_map &d) {
+        w = d.width;
+        h = d.height;
+        texels.resize((size_t)w * h);
+        std::vector<float> lum((size_t)w * h);
+        float total = 0.0f;
+        for (size_t i = 0; i < texels.size(); ++i) {
+            texels[i] = V3(d.rgb[3 * i], d.rgb[3 * i + 1], d.rgb[3 * i + 2]);
+            const float theta = ((float)(i / (size_t)w) + 0.5f) / (float)h * kPi;
+            lum[i] = std::sin(theta) * luminance(texels[i]);
+            total += lum[i];
+        }
+        // thresholded table: texels below the mean weighted luminance are left to BSDF sampling
+        const float mean = total / (float)lum.size();
+        for (float &l : lum) l = std::max(l - mean, 0.0f);
+        distrib.build(lum, (uint32_t)w, (uint32_t)h);
+        for (int k = 0; k < 9; ++k) l2w[k] = d.to_world[k];
+        invert3(l2w, w2l);
+        yup = d.transform_y_up != 0;
+        strength = d.strength;
+    }
+    // Gauss-Jordan inverse with partial pivoting in double
+    static void invert3(const float in[9], float out[9]) {
+        double a[3][6];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 6; ++c) a[r][c] = c < 3 ? in[3 * r + c] : (c - 3 == r ? 1.0 : 0.0);
+        for (int c = 0; c < 3; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < 3; ++r)
+                if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+            for (int k = 0; k < 6; ++k) std::swap(a[c][k], a[piv][k]);
+            const double inv = 1.0 / a[c][c];
+            for (int k = 0; k < 6; ++k) a[c][k] *= inv;
+            for (int r = 0; r < 3; ++r) {
+                if (r == c) continue;
+                const double f = a[r][c];
+                for (int k = 0; k < 6; ++k) a[r][k] -= f * a[c][k];
+            }
+        }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) out[3 * r + c] = (float)a[r][3 + c];
+    }
+    // to_spherical / to_spherical_yup (maths.h:458-484) of the map-frame direction
+    void spherical(V3 wi, float &phi, float &theta) const {
+        V3 l = apply(w2l, wi);
+        const float up = yup ? l.y : l.z, a = yup ? l.z : l.y;
+        theta = std::acos(clampv(up, -1.0f, 1.0f));
+        phi = std::atan2(a, l.x);
+        if (phi < 0.0f) phi += kTwoPi;
+    }
+    V3 texel(int x, int y) const { return texels[(size_t)y * w + x]; }
+    // SkyLight::eval (light.cpp:59-86): bilinear over texel centres, u wraps, v clamps (maths.h:760-780)
+    V3 eval(V3 wi) const {
+        float phi, theta;
+        spherical(wi, phi, theta);
+        const float us = phi * kInvPi * 0.5f * (float)w - 0.5f, vs = theta * kInvPi * (float)h - 0.5f;
+        const int x0 = (int)std::floor(us), y0 = (int)std::floor(vs);
+        const float tu = us - (float)x0, tv = vs - (float)y0;
+        const int xa = (x0 + w) % w, xb = (x0 + 1 + w) % w;
+        const int ya = std::clamp(y0, 0, h - 1), yb = std::clamp(y0 + 1, 0, h - 1);
+        V3 L;
+        L = L + texel(xa, ya) * ((1.0f - tu) * (1.0f - tv));
+        L = L + texel(xa, yb) * ((1.0f - tu) * tv);
+        L = L + texel(xb, ya) * (tu * (1.0f - tv));
+        L = L + texel(xb, yb) * (tu * tv);
+        return L * strength;
+    }
+    // SkyLight::pdf (light.cpp:113-134), solid-angle density
+    float pdf(V3 wi) const {
+        float phi, theta;
+        spherical(wi, phi, theta);
+        const float st = std::sin(theta);
+        if (st == 0.0f) return 0.0f;
+        return distrib.pdf(phi * kInvPi * 0.5f, theta * kInvPi) / (2.0f * kPi * kPi * st);
+    }
+    // SkyLight::sample (light.cpp:88-111): Le / pdf
+    V3 sample(float u0, float u1, V3 &wi, float &pdf) const {
+        float su, sv;
+        distrib.sample_linear(u0, u1, su, sv, pdf);
+        if (pdf == 0.0f) return V3();
+        const float phi = su * 2.0f * kPi, theta = sv * kPi;
+        const float ct = std::cos(theta), st = std::sin(theta), cp = std::cos(phi), sp = std::sin(phi);
+        const V3 l = yup ? V3(cp * st, ct, sp * st) : V3(cp * st, sp * st, ct); // maths.h:467-493
+        pdf /= (2.0f * kPi * kPi * st);
+        if (st == 0.0f) {
+            pdf = 0.0f;
+            return V3();
+        }
+        wi = apply(l2w, l);
+        return eval(wi) / pdf;
+    }
+    // SkyLight::power (light.cpp:136-158): midpoint rule over a z-up (theta, phi) grid of the map's size
+    V3 power(float scene_radius) const {
+        V3 sum;
+        for (int row = 0; row < h; ++row) {
+            const float theta = ((float)row + 0.5f) / (float)h * kPi, st = std::sin(theta);
+            V3 row_sum;
+            for (int col = 0; col < w; ++col) {
+                const float phi = ((float)col + 0.5f) / (float)w * kTwoPi;
+                const V3 dir(std::cos(phi) * st, std::sin(phi) * st, std::cos(theta));
+                row_sum = row_sum + eval(dir) * st;
+            }
+            sum = sum + row_sum;
+        }
+        return sum * ((kPi / (float)h) * (kTwoPi / (float)w)) * (kPi * sqr(scene_radius));
+    }
+};
+
 struct Light {
-    int type;  // 0 sky ambient, 1 directional, 2 point, 3 mesh triangle
+    int type;  // 0 sky ambient, 1 directional, 2 point, 3 mesh triangle, 4 sky map
     V3 color;  // ambient / L / I / emission
     V3 vec;    // dir or pos
     uint32_t prim = 0; // global triangle index (mesh)
@@ -809,6 +985,7 @@ struct Scene {
     std::vector<float> cdf;         // DistribTable cdf (distrib.cpp:9-31)
     std::vector<float> powers;
     std::vector<uint32_t> sky_lights;
+    SkyMapLight sky_map; // texels of the type-4 light, when there is one
     AABB bound;
     bool has_normals = false;
 
@@ -1141,6 +1318,12 @@ Scene *create(const ora_scene_desc *d) {
             phi = phi * ((kPi / 1.0f) * (kTwoPi / 1.0f));
             phi = phi * (kPi * sqr(scene_radius));
             s->sky_lights.push_back(i);
+        } else if (L.type == 3) {
+            if (!d->sky_map || !s->sky_map.texels.empty()) return nullptr; // one map, and it must be given
+            s->sky_map.init(*d->sky_map);
+            l.type = 4;
+            phi = s->sky_map.power(scene_radius);
+            s->sky_lights.push_back(i);
         } else if (L.type == 1) {
             l.vec = normalized(l.vec);
             phi = l.color * (kPi * sqr(scene_radius)); // light.cpp:216-220
@@ -1317,6 +1500,10 @@ V3 light_sample(const Scene &s, const Light &l, V3 p, float u0, float u1, V3 &wi
         wi_dist = kInf;
         return l.color / pdf;
     }
+    if (l.type == 4) {
+        wi_dist = kInf;
+        return s.sky_map.sample(u0, u1, wi, pdf);
+    }
     if (l.type == 1) { // DirectionalLight::sample (light.cpp:207-213)
         wi = l.vec;
         wi_dist = kInf;
@@ -1400,12 +1587,12 @@ PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob) {
             for (uint32_t li : s.sky_lights) {
                 const Light &l = s.lights[li];
                 V3 wi = normalized(rd);
-                V3 Le = l.color; // SkyLight::eval of a 1x1 ambient map
+                V3 Le = l.type == 4 ? s.sky_map.eval(wi) : l.color; // a 1x1 ambient map is constant
                 if (bounce == 0) {
                     if (p.include_background) L = L + beta * Le;
                 } else {
                     float pr = light_probability(s, li);
-                    float pdf_l = sky_pdf(wi);
+                    float pdf_l = l.type == 4 ? s.sky_map.pdf(wi) : sky_pdf(wi);
                     float w = power_heur(pdf_wi_bsdf, pdf_l * pr);
                     V3 bu = Le * w;
                     if (bounce == 1 && p.clamp_direct > 0.0f) bu = vmin(bu, p.clamp_direct);
@@ -1604,6 +1791,32 @@ int ora_occluded(void *scene, const float *rays, int64_t n, uint8_t *out, int th
         q.setup();
         out[i] = occluded(s, q) ? 1 : 0;
     });
+    return 0;
+}
+
+int ora_sky_eval(void *scene, const float *dirs, int64_t n, float *le, float *pdf) {
+    const Scene &s = *(const Scene *)scene;
+    if (s.sky_map.texels.empty()) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        V3 d(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        V3 L = s.sky_map.eval(d);
+        le[3 * i] = L.x, le[3 * i + 1] = L.y, le[3 * i + 2] = L.z;
+        pdf[i] = s.sky_map.pdf(d);
+    }
+    return 0;
+}
+
+int ora_sky_sample(void *scene, const float *u, int64_t n, float *wi, float *le_over_pdf, float *pdf) {
+    const Scene &s = *(const Scene *)scene;
+    if (s.sky_map.texels.empty()) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        V3 d;
+        float pd = 0.0f;
+        V3 b = s.sky_map.sample(u[2 * i], u[2 * i + 1], d, pd);
+        wi[3 * i] = d.x, wi[3 * i + 1] = d.y, wi[3 * i + 2] = d.z;
+        le_over_pdf[3 * i] = b.x, le_over_pdf[3 * i + 1] = b.y, le_over_pdf[3 * i + 2] = b.z;
+        pdf[i] = pd;
+    }
     return 0;
 }
 

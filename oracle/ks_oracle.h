@@ -47,12 +47,23 @@ typedef struct ora_material {
     int32_t has_emission;
 } ora_material;
 
-/* Non-mesh lights (light.h:34-120): 0 = ambient SkyLight, 1 = DirectionalLight, 2 = PointLight. */
+/* Non-mesh lights (light.h:34-120): 0 = ambient SkyLight, 1 = DirectionalLight, 2 = PointLight,
+ * 3 = SkyLight from the scene's equirectangular map (ora_scene_desc.sky_map). */
 typedef struct ora_light {
     int32_t type;
     float color[3]; /* ambient radiance / L / I */
     float vec[3];   /* direction (directional) or position (point) */
 } ora_light;
+
+/* SkyLight(path, to_world, transform_y_up, strength) (light.cpp:19-49): h rows of w RGB texels, row 0
+ * at theta = 0; to_world is the row-major linear part of the light's transform. */
+typedef struct ora_sky_map {
+    const float *rgb;
+    int32_t width, height;
+    float to_world[9];
+    int32_t transform_y_up;
+    float strength;
+} ora_sky_map;
 
 typedef struct ora_scene_desc {
     const float *vertices; /* 3 * n_vertices */
@@ -68,6 +79,7 @@ typedef struct ora_scene_desc {
     const ora_light *lights;
     uint32_t n_lights;
     const uint8_t *geom_flags; /* per geometry: 1 one-sided (MeshData::twosided = false), 2 flat normals; or NULL */
+    const ora_sky_map *sky_map; /* texels of the type-3 light, or NULL */
 } ora_scene_desc;
 
 /* create_camera (camera.cpp:172-205): look_at(pos, target, up) + perspective vfov (degrees) / aspect. */
@@ -100,6 +112,10 @@ int ora_intersect_brute(void *scene, const float *rays, int64_t n, float *out, i
 int ora_occluded(void *scene, const float *rays, int64_t n, uint8_t *out, int threads);
 /* Light table as the sampler sees it: flat order, prim id (-1 non-mesh), cdf (n+1). */
 int ora_light_table(void *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf);
+/* The scene's sky-map light: radiance (n x 3) and solid-angle pdf (n) at world directions (n x 3). */
+int ora_sky_eval(void *scene, const float *dirs, int64_t n, float *le, float *pdf);
+/* Light samples from u (n x 2): direction (n x 3), Le / pdf (n x 3), pdf (n). */
+int ora_sky_sample(void *scene, const float *u, int64_t n, float *wi, float *le_over_pdf, float *pdf);
 /* rgb: W*H*3 FP64 sums, weights: W*H*2 FP64 (hit, miss) -- RenderTarget2 (render_target.h:224-293). */
 int ora_render(void *scene, const ora_camera *cam, const ora_params *p, double *rgb, double *weights,
                int threads);

@@ -54,6 +54,18 @@ class OraSceneDesc(C.Structure):
         ("lights", C.c_void_p),
         ("n_lights", C.c_uint32),
         ("geom_flags", C.c_void_p),
+        ("sky_map", C.c_void_p),
+    ]
+
+
+class OraSkyMap(C.Structure):
+    _fields_ = [
+        ("rgb", C.c_void_p),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("to_world", C.c_float * 9),
+        ("transform_y_up", C.c_int32),
+        ("strength", C.c_float),
     ]
 
 
@@ -94,6 +106,8 @@ _SIGS = {
     "ora_intersect_brute": ([_P, _P, C.c_int64, _P, C.c_int], C.c_int),
     "ora_occluded": ([_P, _P, C.c_int64, _P, C.c_int], C.c_int),
     "ora_light_table": ([_P, C.POINTER(C.c_int32), _P, _P, _P], C.c_int),
+    "ora_sky_eval": ([_P, _P, C.c_int64, _P, _P], C.c_int),
+    "ora_sky_sample": ([_P, _P, C.c_int64, _P, _P, _P], C.c_int),
     "ora_render": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, C.c_int], C.c_int),
     "ora_trace_samples": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, _P, C.c_int64, _P, C.c_int], C.c_int),
     "ora_sobol": ([C.c_int] * 8 + [_P, _P, _P], C.c_int),
@@ -174,7 +188,9 @@ def material(rec) -> OraMaterial:
 
 
 class OracleScene:
-    def __init__(self, vertices, indices, tri_geom, geom_material, materials, lights, normals=None, geom_flags=None):
+    def __init__(self, vertices, indices, tri_geom, geom_material, materials, lights, normals=None, geom_flags=None,
+                 sky_map=None):
+        """sky_map: dict(rgb=(H, W, 3), to_world=3x3, y_up=bool, strength=float) for a type-3 light."""
         lib = load()
         self.lib = lib
         self._arrs = [
@@ -202,12 +218,26 @@ class OracleScene:
         d.lights = lts.ctypes.data if lts.shape[0] else None
         d.n_lights = lts.shape[0]
         d.geom_flags = gf.ctypes.data if gf is not None else None
+        if sky_map is not None:
+            rgb = np.ascontiguousarray(sky_map["rgb"], np.float32)
+            sm = OraSkyMap()
+            sm.rgb = rgb.ctypes.data
+            sm.height, sm.width = rgb.shape[0], rgb.shape[1]
+            m = np.asarray(sky_map.get("to_world", np.eye(3)), np.float32).reshape(9)
+            for k in range(9):
+                sm.to_world[k] = float(m[k])
+            sm.transform_y_up = 1 if sky_map.get("y_up", True) else 0
+            sm.strength = float(sky_map.get("strength", 1.0))
+            self._arrs += [rgb, sm]
+            d.sky_map = C.addressof(sm)
         self.h = lib.ora_scene_create(C.byref(d))
+        if not self.h:
+            raise ValueError("ora_scene_create rejected the scene")
 
     @classmethod
     def from_host_scene(cls, hs) -> "OracleScene":
         return cls(hs.vertices, hs.indices, hs.tri_geom, hs.geom_material, hs.materials, hs.lights, hs.normals,
-                   getattr(hs, "geom_flags", None))
+                   getattr(hs, "geom_flags", None), getattr(hs, "sky_map", None))
 
     def close(self):
         if self.h:
@@ -241,6 +271,23 @@ class OracleScene:
         cdf = np.zeros(n.value + 1, np.float32)
         self.lib.ora_light_table(self.h, C.byref(n), _p(prim), _p(power), _p(cdf))
         return prim, power, cdf
+
+    def sky_eval(self, dirs):
+        """The sky-map light's radiance (n, 3) and solid-angle pdf (n) at world directions (n, 3)."""
+        dirs = np.ascontiguousarray(dirs, np.float32)
+        le = np.zeros((dirs.shape[0], 3), np.float32)
+        pdf = np.zeros(dirs.shape[0], np.float32)
+        assert self.lib.ora_sky_eval(self.h, _p(dirs), dirs.shape[0], _p(le), _p(pdf)) == 0
+        return le, pdf
+
+    def sky_sample(self, u):
+        """Sky-map light samples from u (n, 2): directions (n, 3), Le / pdf (n, 3), pdf (n)."""
+        u = np.ascontiguousarray(u, np.float32)
+        wi = np.zeros((u.shape[0], 3), np.float32)
+        b = np.zeros((u.shape[0], 3), np.float32)
+        pdf = np.zeros(u.shape[0], np.float32)
+        assert self.lib.ora_sky_sample(self.h, _p(u), u.shape[0], _p(wi), _p(b), _p(pdf)) == 0
+        return wi, b, pdf
 
     def render(self, cam: dict, prm: OraParams, threads=None):
         W, H = prm.render_width, prm.render_height

@@ -192,3 +192,22 @@ inference = "fp8"
     assert t["guide"]["inference_fp8"] == 1
     r = resolve(tmp_path, text.replace('"fp8"', '"fp4"'))
     assert r.returncode != 0 and "bf16" in r.stderr
+
+
+def test_sky_map(tmp_path):
+    import image_files
+
+    rgb = np.random.default_rng(0).uniform(0.1, 4.0, (6, 10, 3)).astype(np.float32)
+    image_files.write_hdr(tmp_path / "sky.hdr", image_files.to_rgbe(rgb))
+    image_files.write_exr(tmp_path / "sky.exr", rgb)
+    text = BASE + f'sky = {{ map = "{tmp_path}/sky.hdr", strength = 2.0, transform_y_up = false, to_world = {{ rotation = {{ euler = [0, 90, 0] }} }} }}\n'
+    (t,) = tasks(tmp_path, text)
+    assert t["n_lights"] == 1 and t["sky_map"] == [10, 6, 2, 0]
+    text = BASE + f'light = [ {{ type = "sky", map = "{tmp_path}/sky.exr" }} ]\n'
+    (t,) = tasks(tmp_path, text)
+    assert t["n_lights"] == 1 and t["sky_map"] == [10, 6, 1, 1]  # defaults: strength 1, y up
+    r = resolve(tmp_path, BASE + f'sky = {{ map = "{tmp_path}/sky.png" }}\n')
+    assert r.returncode != 0 and "only .hdr or .exr" in r.stderr
+    r = resolve(tmp_path, BASE + f'sky = {{ map = "{tmp_path}/sky.hdr" }}\n'
+                f'light = [ {{ type = "sky", map = "{tmp_path}/sky.exr" }} ]\n')
+    assert r.returncode != 0 and "at most one sky map" in r.stderr
