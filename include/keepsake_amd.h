@@ -70,6 +70,72 @@ typedef struct ks_sky_map_desc {
     float strength;          /* radiance scale */
 } ks_sky_map_desc;
 
+/* A texture (Texture, texture.h:46-66; create_texture texture.cpp:1068-1084): texels row by row, each
+ * `channels` values of `data_type`, levels back to back (each level max(1, (n + 1) / 2) of the one
+ * before in both directions, as the reference's NPOT chain). Values are stored as given and converted
+ * on fetch: u8 / 255, u16 / 65535, f32 as is, then sRGB -> linear when color_space is sRGB
+ * (Texture::fetch_as_float, texture.cpp:403-460). */
+#define KS_TEX_U8 0
+#define KS_TEX_U16 1
+#define KS_TEX_F32 2
+#define KS_COLORSPACE_LINEAR 0
+#define KS_COLORSPACE_SRGB 1
+typedef struct ks_texture_desc {
+    const void *data;
+    int32_t width, height;
+    int32_t channels;      /* 1..4 */
+    int32_t data_type;     /* KS_TEX_* */
+    int32_t color_space;   /* KS_COLORSPACE_* */
+    int32_t levels;        /* levels present in data (1 .. 1 + ceil(log2(max(width, height)))) */
+    int32_t build_mipmaps; /* with levels == 1: build the whole chain as Texture(bytes, ..., true) does */
+} ks_texture_desc;
+
+/* A shader field (ShaderField<color<N>>, shader_field.h:11-86): a constant, or a texture lookup through
+ * a sampler (create_texture_sampler texture.cpp:1086-1121) with the TextureField's flip_v, uv_scale,
+ * uv_offset, swizzle and scale. */
+#define KS_SAMPLER_NEAREST 0
+#define KS_SAMPLER_LINEAR 1 /* bilinear, trilinear across levels by the primary ray's uv footprint */
+#define KS_SAMPLER_CUBIC 2
+#define KS_SAMPLER_EWA 3
+#define KS_WRAP_REPEAT 0
+#define KS_WRAP_CLAMP 1
+#define KS_CUBIC_MITCHELL 0
+#define KS_CUBIC_BSPLINE 1
+#define KS_CUBIC_CATMULL_ROM 2
+typedef struct ks_field_desc {
+    int32_t texture;       /* index into ks_scene_desc.textures, or -1: the constant `value` */
+    int32_t sampler;       /* KS_SAMPLER_* */
+    int32_t cubic_kernel;  /* KS_CUBIC_* */
+    float anisotropy;      /* EWA, clamped to [1, 16] (EWASampler ctor, texture.h:127) */
+    int32_t wrap_u, wrap_v; /* KS_WRAP_* */
+    int32_t flip_v;
+    int32_t swizzle[4];    /* texture channel of each component, -1: 0 */
+    float uv_scale[2], uv_offset[2];
+    float scale[4];
+    float value[4];        /* the constant (texture == -1) */
+} ks_field_desc;
+
+/* A material whose closure inputs come from shader fields (PrincipledBSDF::eval_closure,
+ * principled_bsdf.cpp:246-275, with Material::normal_map / opacity_map / emission, material.h:39-43).
+ * The clamps of eval_closure apply to the field values at each hit. */
+#define KS_NORMAL_MAP_TANGENT 0
+#define KS_NORMAL_MAP_WORLD 1
+typedef struct ks_material_fields {
+    ks_field_desc basecolor, roughness, metallic, ior, specular_r0_mul, specular_trans, diffuse_trans,
+        diffuse_trans_fwd;
+    int32_t has_emission; /* the emission field makes the geometry's triangles mesh lights */
+    ks_field_desc emission;
+    int32_t has_normal_map; /* NormalMap (normal_map.h:11-30, normal_map.cpp:8-43) */
+    ks_field_desc normal_map;
+    int32_t normal_space;  /* KS_NORMAL_MAP_* */
+    int32_t normal_range01;
+    float normal_strength;
+    float normal_to_world[9]; /* world-space maps: row-major linear part of the map's to_world (normals go
+                                 through its inverse transpose, Transform::normal maths.h:987) */
+    int32_t has_opacity_map; /* OpacityMap (opacity_map.h:11-21): stochastic alpha test in traversal */
+    ks_field_desc opacity;
+} ks_material_fields;
+
 /* Flat scene: the geometry Scene::intersect1/occlude1 see (scene.h:49-87) with an identity instance
  * transform and one material per geometry (SubScene::materials, scene.h:37). Triangles of one geometry
  * are contiguous and geometries appear in increasing id order (mesh-light order, scene.cpp:315-336).
@@ -89,10 +155,17 @@ typedef struct ks_scene_desc {
     uint32_t n_lights;
     const uint8_t *geom_flags; /* per geometry (KS_GEOM_*), or NULL: two-sided, smooth normals when given */
     const ks_sky_map_desc *sky_map; /* the map of the (one) KS_LIGHT_SKY_MAP light, or NULL */
+    const float *texcoords; /* 2 * n_vertices texture coordinates, or NULL: no geometry has them */
+    const ks_texture_desc *textures;
+    uint32_t n_textures;
+    /* n_materials entries: a textured material replaces materials[m], NULL keeps it; or NULL */
+    const ks_material_fields *const *material_fields;
 } ks_scene_desc;
 
 #define KS_GEOM_ONE_SIDED 1u /* MeshData::twosided = false: the frame is not flipped towards wo (geometry.cpp:126) */
 #define KS_GEOM_FLAT 2u      /* MeshData::use_smooth_normal = false: vertex normals ignored (geometry.cpp:135) */
+#define KS_GEOM_NO_TEXCOORD 4u /* the geometry has no texture coordinates: uv is the barycentric (u, v) and dpdu
+                                  the edge v1 - v0 (geometry.cpp:75-116) even when ks_scene_desc.texcoords is set */
 
 /* SBVHBuildOption (cuda/sbvh.cuh:9-16). Zeroed fields take the reference defaults
  * (maxPrimsInNode 4, bucketCount 12, travWeight 1, alpha 1e-5). */
@@ -132,7 +205,7 @@ typedef struct ks_small_pt_params {
     int32_t rng_seed;
     int32_t include_background;
     float backdrop[3];
-    int32_t scale_ray_diff; /* accepted for compatibility; ray differentials are unused (no textures) */
+    int32_t scale_ray_diff; /* camera ray differentials scaled by 1/sqrt(spp) (Camera::spawn_ray, camera.cpp:162-167) */
     float clamp_direct;
     float clamp_indirect;
     int32_t spp_prog_interval;

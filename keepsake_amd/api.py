@@ -71,6 +71,131 @@ class SkyMapDesc(C.Structure):
     ]
 
 
+KS_TEX_U8, KS_TEX_U16, KS_TEX_F32 = 0, 1, 2
+KS_COLORSPACE_LINEAR, KS_COLORSPACE_SRGB = 0, 1
+KS_SAMPLER_NEAREST, KS_SAMPLER_LINEAR, KS_SAMPLER_CUBIC, KS_SAMPLER_EWA = 0, 1, 2, 3
+KS_WRAP_REPEAT, KS_WRAP_CLAMP = 0, 1
+KS_CUBIC_MITCHELL, KS_CUBIC_BSPLINE, KS_CUBIC_CATMULL_ROM = 0, 1, 2
+KS_NORMAL_MAP_TANGENT, KS_NORMAL_MAP_WORLD = 0, 1
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p),
+        ("width", C.c_int32),
+        ("height", C.c_int32),
+        ("channels", C.c_int32),
+        ("data_type", C.c_int32),
+        ("color_space", C.c_int32),
+        ("levels", C.c_int32),
+        ("build_mipmaps", C.c_int32),
+    ]
+
+
+class FieldDesc(C.Structure):
+    _fields_ = [
+        ("texture", C.c_int32),
+        ("sampler", C.c_int32),
+        ("cubic_kernel", C.c_int32),
+        ("anisotropy", C.c_float),
+        ("wrap_u", C.c_int32),
+        ("wrap_v", C.c_int32),
+        ("flip_v", C.c_int32),
+        ("swizzle", C.c_int32 * 4),
+        ("uv_scale", C.c_float * 2),
+        ("uv_offset", C.c_float * 2),
+        ("scale", C.c_float * 4),
+        ("value", C.c_float * 4),
+    ]
+
+
+class MaterialFields(C.Structure):
+    _fields_ = [
+        ("basecolor", FieldDesc),
+        ("roughness", FieldDesc),
+        ("metallic", FieldDesc),
+        ("ior", FieldDesc),
+        ("specular_r0_mul", FieldDesc),
+        ("specular_trans", FieldDesc),
+        ("diffuse_trans", FieldDesc),
+        ("diffuse_trans_fwd", FieldDesc),
+        ("has_emission", C.c_int32),
+        ("emission", FieldDesc),
+        ("has_normal_map", C.c_int32),
+        ("normal_map", FieldDesc),
+        ("normal_space", C.c_int32),
+        ("normal_range01", C.c_int32),
+        ("normal_strength", C.c_float),
+        ("normal_to_world", C.c_float * 9),
+        ("has_opacity_map", C.c_int32),
+        ("opacity", FieldDesc),
+    ]
+
+
+_TEX_DTYPES = {np.dtype(np.uint8): KS_TEX_U8, np.dtype(np.uint16): KS_TEX_U16, np.dtype(np.float32): KS_TEX_F32}
+
+
+def field_desc(value=None, texture=-1, sampler=KS_SAMPLER_LINEAR, cubic_kernel=KS_CUBIC_MITCHELL, anisotropy=8.0,
+               wrap_u=KS_WRAP_REPEAT, wrap_v=KS_WRAP_REPEAT, flip_v=True, swizzle=None, uv_scale=(1.0, 1.0),
+               uv_offset=(0.0, 0.0), scale=(1.0, 1.0, 1.0, 1.0)) -> FieldDesc:
+    """A ShaderField: the constant `value` (texture = -1) or a texture lookup (TextureField defaults:
+    flip_v on, identity swizzle, unit scale)."""
+    f = FieldDesc()
+    f.texture = texture
+    f.sampler = sampler
+    f.cubic_kernel = cubic_kernel
+    f.anisotropy = anisotropy
+    f.wrap_u, f.wrap_v = wrap_u, wrap_v
+    f.flip_v = 1 if flip_v else 0
+    sw = list(swizzle) if swizzle is not None else [0, 1, 2, 3]
+    for k in range(4):
+        f.swizzle[k] = sw[k] if k < len(sw) else -1
+    f.uv_scale[0], f.uv_scale[1] = uv_scale
+    f.uv_offset[0], f.uv_offset[1] = uv_offset
+    sc = list(np.broadcast_to(np.asarray(scale, np.float32), (4,))) if np.ndim(scale) == 0 else list(scale) + [1.0] * (4 - len(scale))
+    for k in range(4):
+        f.scale[k] = float(sc[k])
+    if value is not None:
+        v = np.atleast_1d(np.asarray(value, np.float32))
+        for k in range(min(4, v.size)):
+            f.value[k] = float(v[k])
+    return f
+
+
+def material_fields(base: np.void | None = None, **fields) -> MaterialFields:
+    """Textured material: every closure input a field. Fields not given take the constant of `base` (a
+    MATERIAL_DTYPE record); keyword arguments are FieldDesc values or dicts for field_desc().
+    normal_map / opacity (FieldDesc) turn the maps on; normal_space, normal_range01, normal_strength,
+    normal_to_world set the normal map's options."""
+    mf = MaterialFields()
+    names = ["basecolor", "roughness", "metallic", "ior", "specular_r0_mul", "specular_trans", "diffuse_trans",
+             "diffuse_trans_fwd"]
+    for n in names + ["emission"]:
+        f = fields.get(n)
+        if f is None:
+            val = base[n] if base is not None else (0.0 if n != "ior" else 1.5)
+            f = field_desc(value=val)
+        elif isinstance(f, dict):
+            f = field_desc(**f)
+        setattr(mf, n, f)
+    mf.has_emission = 1 if (fields.get("emission") is not None or (base is not None and base["has_emission"])) else 0
+    if fields.get("normal_map") is not None:
+        f = fields["normal_map"]
+        mf.normal_map = field_desc(**f) if isinstance(f, dict) else f
+        mf.has_normal_map = 1
+        mf.normal_space = fields.get("normal_space", KS_NORMAL_MAP_TANGENT)
+        mf.normal_range01 = 1 if fields.get("normal_range01", True) else 0
+        mf.normal_strength = fields.get("normal_strength", 1.0)
+        m = np.asarray(fields.get("normal_to_world", np.eye(3)), np.float32).reshape(9)
+        for k in range(9):
+            mf.normal_to_world[k] = float(m[k])
+    if fields.get("opacity") is not None:
+        f = fields["opacity"]
+        mf.opacity = field_desc(**f) if isinstance(f, dict) else f
+        mf.has_opacity_map = 1
+    return mf
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("vertices", C.POINTER(C.c_float)),
@@ -87,11 +212,16 @@ class SceneDesc(C.Structure):
         ("n_lights", C.c_uint32),
         ("geom_flags", C.POINTER(C.c_uint8)),
         ("sky_map", C.POINTER(SkyMapDesc)),
+        ("texcoords", C.POINTER(C.c_float)),
+        ("textures", C.POINTER(TextureDesc)),
+        ("n_textures", C.c_uint32),
+        ("material_fields", C.POINTER(C.POINTER(MaterialFields))),
     ]
 
 
 KS_GEOM_ONE_SIDED = 1
 KS_GEOM_FLAT = 2
+KS_GEOM_NO_TEXCOORD = 4
 
 
 class BvhOptions(C.Structure):
@@ -365,6 +495,11 @@ class HostScene:
     # environment map of a KS_LIGHT_SKY_MAP light: dict(rgb=(H, W, 3) float32, to_world=3x3, y_up=bool,
     # strength=float), or None
     sky_map: dict | None = None
+    texcoords: np.ndarray | None = None  # (V, 2) float32 texture coordinates
+    # textures: dicts with "data" ((H, W, C) uint8 / uint16 / float32, or a list of such arrays, one per
+    # level), "srgb" (bool) and "mipmaps" (bool: build the chain from level 0)
+    textures: list | None = None
+    material_fields: dict | None = None  # material index -> MaterialFields
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -410,6 +545,35 @@ class HostScene:
             sm.strength = float(self.sky_map.get("strength", 1.0))
             self._keep += [rgb, sm]
             d.sky_map = C.pointer(sm)
+        if self.texcoords is not None:
+            tc = np.ascontiguousarray(self.texcoords, np.float32)
+            assert tc.shape == (v.shape[0], 2)
+            self._keep.append(tc)
+            d.texcoords = tc.ctypes.data_as(C.POINTER(C.c_float))
+        if self.textures:
+            arr = (TextureDesc * len(self.textures))()
+            for i, t in enumerate(self.textures):
+                levels = t["data"] if isinstance(t["data"], (list, tuple)) else [t["data"]]
+                levels = [np.asarray(x) for x in levels]
+                lv0 = levels[0] if levels[0].ndim == 3 else levels[0][..., None]
+                blob = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(x).reshape(-1) for x in levels]))
+                self._keep.append(blob)
+                arr[i].data = blob.ctypes.data
+                arr[i].height, arr[i].width, arr[i].channels = lv0.shape
+                arr[i].data_type = _TEX_DTYPES[lv0.dtype]
+                arr[i].color_space = KS_COLORSPACE_SRGB if t.get("srgb", False) else KS_COLORSPACE_LINEAR
+                arr[i].levels = len(levels)
+                arr[i].build_mipmaps = 1 if t.get("mipmaps", True) else 0
+            self._keep.append(arr)
+            d.textures = arr
+            d.n_textures = len(self.textures)
+        if self.material_fields:
+            ptrs = (C.POINTER(MaterialFields) * mats.shape[0])()
+            for m, mf in self.material_fields.items():
+                self._keep.append(mf)
+                ptrs[m] = C.pointer(mf)
+            self._keep.append(ptrs)
+            d.material_fields = ptrs
         return d
 
 

@@ -18,6 +18,8 @@
 #include "ks_guide_host.h"
 #include "ks_internal.h"
 #include "ks_render_types.h"
+#include "ks_surface.h"
+#include "ks_texture_build.h"
 #include "ks_traverse.h"
 
 #include <algorithm>
@@ -97,8 +99,9 @@ KS_DEV void pixel_of(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &
     y = (uint32_t)rc.cy + ry;
 }
 
-// Camera::spawn_ray (camera.cpp:129-170) after the box-filter jitter (small_pt.cpp:98-105)
-KS_DEV void camera_ray(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &sob, f3 &o, f3 &d) {
+// Camera::spawn_ray (camera.cpp:129-170) after the box-filter jitter (small_pt.cpp:98-105); q is
+// world_pos - eye before normalisation.
+KS_DEV void camera_ray_q(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &sob, f3 &o, f3 &q) {
     // no FMA contraction: the rounding of the projection must not depend on whether the matrix comes
     // from kernel arguments or from memory (plain and batched renders give the same rays)
 #pragma clang fp contract(off)
@@ -120,8 +123,15 @@ KS_DEV void camera_ray(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &so
     float hz = m[8] * nx + m[9] * ny + m[10] + m[11];
     float hw = m[12] * nx + m[13] * ny + m[14] + m[15];
     o = mk(rc.cam[0], rc.cam[1], rc.cam[2]);
-    d = normalized(mk(hx / hw, hy / hw, hz / hw));
+    q = mk(hx / hw, hy / hw, hz / hw);
 }
+
+KS_DEV void camera_ray(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &sob, f3 &o, f3 &d) {
+    f3 q;
+    camera_ray_q(rc, x, y, sob, o, q);
+    d = normalized(q);
+}
+
 
 // Per-sample result slot of work item w: sample-major within the batch, so the film kernel's reads of
 // one sample across consecutive pixels are contiguous.
@@ -134,6 +144,22 @@ KS_DEV uint32_t samp_index(const RenderConsts &rc, uint32_t w) {
 
 KS_DEV void sobol_start(const RenderConsts &rc, Sobol &sob, uint32_t x, uint32_t y, uint32_t s, int dim) {
     sob.start(x, y, s, dim, rc.seed, rc.log2spp, rc.ndigits);
+}
+
+// The camera ray's differential directions (camera.cpp:146-167): one pixel step along x and y on the
+// film plane, scaled towards the ray by rc.diff_scale. The sample's jitter is drawn again from the
+// start of its Sobol stream.
+KS_DEV void camera_ray_diffs(const RenderConsts &rc, uint32_t x, uint32_t y, uint32_t s, f3 &o, f3 &rxd, f3 &ryd) {
+#pragma clang fp contract(off)
+    Sobol sob;
+    sobol_start(rc, sob, x, y, s, 0);
+    f3 q;
+    camera_ray_q(rc, x, y, sob, o, q);
+    const f3 d = normalized(q);
+    rxd = normalized(q + mk(rc.cam_dx[0], rc.cam_dx[1], rc.cam_dx[2]));
+    ryd = normalized(q + mk(rc.cam_dy[0], rc.cam_dy[1], rc.cam_dy[2]));
+    rxd = d + (rxd - d) * rc.diff_scale;
+    ryd = d + (ryd - d) * rc.diff_scale;
 }
 
 // Batched renders: the segment holding work item w (last one whose base is <= w), its view, pixel and
@@ -217,7 +243,7 @@ __global__ void k_iter_done(Ctrl *c) {
 
 // Closest hit for n rays (count read from *n_dev when given). Grid-stride over rays; one LDS stack
 // column per lane.
-template <int W>
+template <int W, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                          const float4 *ro, const float4 *rd, float4 *hit,
                                                          uint32_t *ovf, uint32_t ovf_cap,
@@ -234,7 +260,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     st.ovf_cap = ovf_cap;
     st.spills = 0;
     uint32_t n_nodes = 0, n_tris = 0, n_rays = 0;
-    trace_rays<false, W>(
+    trace_rays<false, W, ALPHA>(
         sc.nodes, sc.tris, gid, n, G, st,
         [&](uint32_t i, TravRay &r) {
             float4 o4 = ro[i], d4 = rd[i];
@@ -245,7 +271,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
             hit[i] = h.prim >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim))
                                  : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
         },
-        n_nodes, n_tris, n_rays);
+        n_nodes, n_tris, n_rays,
+        [&](int32_t prim, float u, float v, const TravRay &r) { return opacity_pass(sc, prim, u, v, r.o, r.d); });
     wave_add_stat(stats, kRaysClosest, n_rays);
     wave_add_stat(stats, kNodesClosest, n_nodes);
     wave_add_stat(stats, kTrisClosest, n_tris);
@@ -254,7 +281,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
 
 // Any hit. mode 0: write the occlusion flag; mode 1: add the NEE contribution of unoccluded rays to
 // their target (next path buffer slot, or the per-sample result when bit 31 is set).
-template <int MODE, int W>
+template <int MODE, int W, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
                                                         const uint32_t *target, const float4 *contrib,
@@ -294,7 +321,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
             if (recs && slot) recs[(size_t)(slot - 1) * kRecFloat4 + kRecNee] = make_float4(c.x, c.y, c.z, 0.0f);
         }
     };
-    trace_rays<true, W>(sc.nodes, sc.tris, gid, n, G, st, load, store, n_nodes, n_tris, n_rays);
+    trace_rays<true, W, ALPHA>(
+        sc.nodes, sc.tris, gid, n, G, st, load, store, n_nodes, n_tris, n_rays,
+        [&](int32_t prim, float u, float v, const TravRay &r) { return opacity_pass(sc, prim, u, v, r.o, r.d); });
     wave_add_stat(stats, kRaysShadow, n_rays);
     wave_add_stat(stats, kNodesShadow, n_nodes);
     wave_add_stat(stats, kTrisShadow, n_tris);
@@ -324,7 +353,8 @@ KS_DEV float sky_pdf(f3 wi) { // SkyLight::pdf (light.cpp:113-134) for a 1x1 map
     return 1.0f / (2.0f * kPi * kPi * sin_theta);
 }
 
-// Light::sample for the flat light list; returns Le / pdf (light.h:27-29).
+// Light::sample for the flat light list; returns Le / pdf (light.h:27-29). TEX reads textured emission.
+template <bool TEX = false>
 KS_DEV f3 light_sample(const DevScene &sc, const Light &l, f3 p, float u0, float u1, f3 &wi, float &wi_dist,
                        float &pdf) {
     const f3 zero = mk(0, 0, 0);
@@ -390,6 +420,15 @@ KS_DEV f3 light_sample(const DevScene &sc, const Light &l, f3 p, float u0, float
     if (absdot == 0.0f) return zero;
     pdf = (1.0f / l.area) * (d2 / absdot);
     wi_dist = fmaxf(0.0f, dist - 1e-4f);
+    if (TEX && l.color[3] != 0.0f) { // textured emission at the sampled point (MeshTriLight::eval, light.cpp:560-565)
+        const uint4 tri = sc.tri_idx[l.prim];
+        const DevTexMaterial &tm = sc.tex_mats[sc.tex_mat_of[tri_material(tri)]];
+        float tu = 0.0f, tv = 0.0f; // interpolate_texcoord: zero without texcoords
+        if (geom_has_uv(sc, tri)) interp_uv(sc, tri, b0, b1, tu, tv);
+        float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        field_eval0(tex_context(sc), tm.f[kFieldEmission], tu, tv, 3, e);
+        col = mk(e[0], e[1], e[2]);
+    }
     return col / pdf;
 }
 
@@ -421,7 +460,9 @@ KS_DEV float mix_pdf(const float4 *m, f3 w) {
 // the learned mixture (probability gs.alpha of the mixture at vertices whose roughness reaches
 // gs.min_roughness; the continuation pdf is alpha q + (1 - alpha) p_bsdf in the estimator and in both
 // MIS weights), and training records of the first kRecVertices vertices of each path.
-template <bool GUIDED, bool BATCH>
+// TEX evaluates hits of textured scenes: texture coordinates and uv-based frames, uv partials of camera
+// rays, normal maps, closures and emission from shader fields.
+template <bool GUIDED, bool BATCH, bool TEX>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, PathBufs cur, PathBufs nxt,
                                                const float4 *hits, Ctrl *ctrl, float4 *sh_o, float4 *sh_d,
                                                uint32_t *sh_tgt, float4 *sh_c, float4 *samp,
@@ -492,14 +533,38 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
             done = true;
         } else {
             uint4 tri = sc.tri_idx[prim];
-            const Material &m = sc.mats[tri_material(tri)];
-            SurfHit it = make_hit(sc, tri, h.y, h.z, rd);
+            const uint32_t mid = tri_material(tri);
+            const Material &m = sc.mats[mid];
+            SurfHit it;
+            SurfUV su;
+            const DevTexMaterial *tm = nullptr;
+            if (TEX) {
+                // compute_intersection with uv partials, then the normal map (Scene::intersect1, scene.cpp:222-233)
+                it = make_hit_uv(sc, tri, h.y, h.z, rd, su);
+                const int32_t tmi = sc.tex_mat_of ? sc.tex_mat_of[mid] : -1;
+                if (tmi >= 0) {
+                    tm = &sc.tex_mats[tmi];
+                    if (bounce == 0) {
+                        f3 co, rxd, ryd;
+                        camera_ray_diffs(vr, x, y, s, co, rxd, ryd);
+                        uv_partials(it, su, co, rxd, ryd);
+                    }
+                    if (tm->has_normal) apply_normal_map(tex_context(sc), *tm, su.u, su.v, it);
+                }
+            } else {
+                it = make_hit(sc, tri, h.y, h.z, rd);
+            }
             if (m.has_emission) { // small_pt.cpp:111-138
                 int32_t li = sc.tri_light[prim];
                 if (li >= 0) {
                     const Light &l = sc.lights[li];
                     float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)li);
                     f3 Le = mk(l.color[0], l.color[1], l.color[2]);
+                    if (TEX && tm && l.color[3] != 0.0f) { // MeshTriLight::eval at the hit (light.cpp:560-565)
+                        float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                        field_eval(tex_context(sc), tm->f[kFieldEmission], su.u, su.v, su.d, 3, e);
+                        Le = mk(e[0], e[1], e[2]);
+                    }
                     if (bounce == 0) {
                         L = L + beta * Le;
                     } else if (pr > 0.0f) {
@@ -517,19 +582,23 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                 done = true;
             } else {
                 f3 wo = -normalized(rd);
-                const float4 *mixp = GUIDED && gs.mix ? gs.mix + 10 * (size_t)i : nullptr;
-                const float alpha_m = GUIDED && mixp && m.roughness >= gs.min_roughness ? gs.alpha : 0.0f;
                 Closure c;
-                c.basecolor = mk(m.basecolor[0], m.basecolor[1], m.basecolor[2]);
-                c.roughness = m.roughness;
-                c.ax = m.ax;
-                c.ay = m.ay;
-                c.metallic = m.metallic;
-                c.ior = m.ior;
-                c.specular_r0_mul = m.specular_r0_mul;
-                c.specular_trans = m.specular_trans;
-                c.diffuse_trans = m.diffuse_trans;
-                c.diffuse_trans_fwd = m.diffuse_trans_fwd;
+                if (TEX && tm) {
+                    c = closure_from_fields(tex_context(sc), *tm, su.u, su.v, su.d);
+                } else {
+                    c.basecolor = mk(m.basecolor[0], m.basecolor[1], m.basecolor[2]);
+                    c.roughness = m.roughness;
+                    c.ax = m.ax;
+                    c.ay = m.ay;
+                    c.metallic = m.metallic;
+                    c.ior = m.ior;
+                    c.specular_r0_mul = m.specular_r0_mul;
+                    c.specular_trans = m.specular_trans;
+                    c.diffuse_trans = m.diffuse_trans;
+                    c.diffuse_trans_fwd = m.diffuse_trans_fwd;
+                }
+                const float4 *mixp = GUIDED && gs.mix ? gs.mix + 10 * (size_t)i : nullptr;
+                const float alpha_m = GUIDED && mixp && c.roughness >= gs.min_roughness ? gs.alpha : 0.0f;
 
                 // NEE (nee.cpp:11-58)
                 if (sc.n_lights > 0) {
@@ -541,7 +610,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                     sob.next2d(u0, u1);
                     f3 wi;
                     float wi_dist = 0.0f, pdf_l = 0.0f;
-                    f3 Le_beta = light_sample(sc, l, it.p, u0, u1, wi, wi_dist, pdf_l);
+                    f3 Le_beta = light_sample<TEX>(sc, l, it.p, u0, u1, wi, wi_dist, pdf_l);
                     if (!is_zero(Le_beta) && pdf_l > 0.0f) {
                         f3 wol = sh_to_local(it, wo), wil = sh_to_local(it, wi);
                         f3 f = bsdf_eval(wol, wil, c);
@@ -870,7 +939,11 @@ bool inverse(const M4 &m, M4 &out) {
 
 // Camera(to_world, vfov, aspect) (camera.cpp:74-82) with look_at (camera.cpp:12-32) and
 // rev_inf_projection (camera.cpp:56-70, near_clip 0.1): proj_to_world and the camera position.
-int camera_setup(const ks_camera_desc &c, float p2w[16], float pos[3]) {
+// Camera matrices (camera.cpp:12-127) folded for camera_ray, and the camera ray differentials' film steps
+// for a film of W x H (camera.cpp:133-140): proj_to_camera maps (2/W, 0, 1) and (0, 2/H, 1) next to the
+// film centre, and camera_to_world turns the differences into world directions.
+int camera_setup(const ks_camera_desc &c, int W, int H, float p2w[16], float pos[3], float cam_dx[3],
+                 float cam_dy[3]) {
     M4 c2w;
     if (c.use_matrix) {
         for (int i = 0; i < 4; ++i)
@@ -920,6 +993,26 @@ int camera_setup(const ks_camera_desc &c, float p2w[16], float pos[3]) {
         p2w[12 + j] = (float)p2w_m.a[3][j];
     }
     pos[0] = (float)eye[0], pos[1] = (float)eye[1], pos[2] = (float)eye[2];
+    M4 p2c;
+    KS_REQUIRE(inverse(proj, p2c), "camera projection is singular");
+    auto hdiv = [&](double x, double y, double out[3]) {
+        double h[4];
+        for (int i = 0; i < 4; ++i) h[i] = p2c.a[i][0] * x + p2c.a[i][1] * y + p2c.a[i][2] * 1.0 + p2c.a[i][3];
+        for (int i = 0; i < 3; ++i) out[i] = h[i] / h[3];
+    };
+    double mid[3], right[3], up[3];
+    hdiv(0.0, 0.0, mid);
+    hdiv((double)(2.0f / (float)W), 0.0, right);
+    hdiv(0.0, (double)(2.0f / (float)H), up);
+    for (int i = 0; i < 3; ++i) {
+        double dx = 0.0, dy = 0.0;
+        for (int j = 0; j < 3; ++j) {
+            dx += c2w.a[i][j] * (right[j] - mid[j]);
+            dy += c2w.a[i][j] * (up[j] - mid[j]);
+        }
+        cam_dx[i] = (float)dx;
+        cam_dy[i] = (float)dy;
+    }
     return KS_OK;
 }
 
@@ -1001,6 +1094,7 @@ struct ks_scene {
     ks_device *dev = nullptr;
     DevBuf nodes, tris, verts, normals, tri_idx, tri_light, mats, lights, cdf;
     DevBuf sky_rgb, sky_cond, sky_marg; // environment map and its sampling tables
+    DevBuf tex_pool, tex_meta, tex_mats, tex_mat_of, uvs, tex_luts; // textures and textured materials
     DevScene ds{};
     ks_scene_info info{};
     std::vector<int32_t> light_prim;
@@ -1074,14 +1168,42 @@ hipEvent_t Timer::take() {
     return dev->events[used++];
 }
 
-// Traversal launches for the scene's node width (2 or 4).
-template <typename... A> void launch_closest(int width, dim3 g, hipStream_t s, A... args) {
-    if (width == 4) hipLaunchKernelGGL(k_closest<4>, g, dim3(kTraceBlock), 0, s, args...);
-    else hipLaunchKernelGGL(k_closest<2>, g, dim3(kTraceBlock), 0, s, args...);
+// Traversal launches for the scene's node width (2 or 4), with the opacity filter when the scene has
+// alpha-tested triangles.
+template <typename... A> void launch_closest(const DevScene &ds, dim3 g, hipStream_t s, A... args) {
+    const dim3 b(kTraceBlock);
+    if (ds.alpha) {
+        if (ds.width == 4) hipLaunchKernelGGL((k_closest<4, true>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_closest<2, true>), g, b, 0, s, ds, args...);
+    } else {
+        if (ds.width == 4) hipLaunchKernelGGL((k_closest<4, false>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_closest<2, false>), g, b, 0, s, ds, args...);
+    }
 }
-template <int MODE, typename... A> void launch_anyhit(int width, dim3 g, hipStream_t s, A... args) {
-    if (width == 4) hipLaunchKernelGGL((k_anyhit<MODE, 4>), g, dim3(kTraceBlock), 0, s, args...);
-    else hipLaunchKernelGGL((k_anyhit<MODE, 2>), g, dim3(kTraceBlock), 0, s, args...);
+template <int MODE, typename... A> void launch_anyhit(const DevScene &ds, dim3 g, hipStream_t s, A... args) {
+    const dim3 b(kTraceBlock);
+    if (ds.alpha) {
+        if (ds.width == 4) hipLaunchKernelGGL((k_anyhit<MODE, 4, true>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_anyhit<MODE, 2, true>), g, b, 0, s, ds, args...);
+    } else {
+        if (ds.width == 4) hipLaunchKernelGGL((k_anyhit<MODE, 4, false>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_anyhit<MODE, 2, false>), g, b, 0, s, ds, args...);
+    }
+}
+
+// Shading launch: guided (plain renders only), batched or plain, each with or without texture support.
+template <typename... A> void launch_shade(bool tex, bool guided, bool batch, dim3 g, hipStream_t s, A... args) {
+    const dim3 b(256);
+    if (guided) {
+        if (tex) hipLaunchKernelGGL((k_shade<true, false, true>), g, b, 0, s, args...);
+        else hipLaunchKernelGGL((k_shade<true, false, false>), g, b, 0, s, args...);
+    } else if (batch) {
+        if (tex) hipLaunchKernelGGL((k_shade<false, true, true>), g, b, 0, s, args...);
+        else hipLaunchKernelGGL((k_shade<false, true, false>), g, b, 0, s, args...);
+    } else {
+        if (tex) hipLaunchKernelGGL((k_shade<false, false, true>), g, b, 0, s, args...);
+        else hipLaunchKernelGGL((k_shade<false, false, false>), g, b, 0, s, args...);
+    }
 }
 
 constexpr int kMaxTraceBlocksPerCu = 8; // the traversal overflow area is sized for this many
@@ -1151,7 +1273,9 @@ int fill_consts(const ks_scene *sc, const ks_camera_desc *cam, const ks_small_pt
     rc.mis_mode = p->mis_mode;
     rc.clamp_direct = p->clamp_direct;
     rc.clamp_indirect = p->clamp_indirect;
-    int e = camera_setup(*cam, rc.p2w, rc.cam);
+    int e = camera_setup(*cam, rc.W, rc.H, rc.p2w, rc.cam, rc.cam_dx, rc.cam_dy);
+    // Camera::spawn_ray(film_pos, film_res, scale_ray_diff ? spp : 1) scales by 1 / sqrt(that count)
+    rc.diff_scale = 1.0f / std::sqrt((float)(p->scale_ray_diff ? p->spp : 1));
     (void)sc;
     return e;
 }
@@ -1199,7 +1323,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q);
         tm.end(e);
         e = graph ? nullptr : tm.begin(KS_K_CLOSEST);
-        launch_closest(sc->ds.width, dim3(tgrid), s, sc->ds, (const uint32_t *)&ctrl->n_active, 0u,
+        launch_closest(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, 0u,
                        (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, ws.hit.as<float4>(),
                        ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
         tm.end(e);
@@ -1211,21 +1335,12 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             if (ge) return ge;
         }
         e = graph ? nullptr : tm.begin(KS_K_SHADE);
-        if (gp)
-            hipLaunchKernelGGL((k_shade<true, false>), dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
-                               ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
-                               ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, gp->shade);
-        else if (wo.batch)
-            hipLaunchKernelGGL((k_shade<false, true>), dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur], pb[1 - cur],
-                               ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
-                               ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, GuideShade{});
-        else
-            hipLaunchKernelGGL((k_shade<false, false>), dim3(tblocks), dim3(256), 0, s, sc->ds, rc, pb[cur],
-                               pb[1 - cur], ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
-                               ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, GuideShade{});
+        launch_shade(sc->ds.textured != 0, gp != nullptr, wo.batch, dim3(tblocks), s, sc->ds, rc, pb[cur], pb[1 - cur],
+                     ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(), ws.sh_tgt.as<uint32_t>(),
+                     ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, gp ? gp->shade : GuideShade{});
         tm.end(e);
         e = graph ? nullptr : tm.begin(KS_K_SHADOW);
-        launch_anyhit<1>(sc->ds.width, dim3(tgrid), s, sc->ds, (const uint32_t *)&ctrl->n_shadow, 0u,
+        launch_anyhit<1>(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_shadow, 0u,
                          (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), (uint8_t *)nullptr,
                          (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[1 - cur].L,
                          ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(),
@@ -1403,6 +1518,289 @@ int ks_device_set_timing(ks_device *dev, int32_t enable) {
 
 namespace {
 
+// Textures, textured materials and texture coordinates of a scene, built on the host. metas point into
+// `pool` on the host (for host-side field evaluation); the device copies point into the uploaded pool.
+struct HostTextures {
+    TexLuts luts{};
+    std::vector<DevTexture> metas;
+    std::vector<uint8_t> pool;
+    std::vector<DevTexMaterial> mats;
+    std::vector<int32_t> mat_of; // per material: index into mats, or -1
+    std::vector<float2> uvs;
+    bool textured = false, alpha = false;
+};
+
+int build_textures(const ks_scene_desc *d, HostTextures &ht) {
+    ht.textured = d->texcoords != nullptr || d->n_textures > 0 || d->material_fields != nullptr;
+    if (!ht.textured) return KS_OK;
+    KS_REQUIRE(d->n_textures == 0 || d->textures, "textures missing");
+    ksa::fill_tex_luts(ht.luts);
+    std::vector<size_t> offs;
+    try {
+        for (uint32_t i = 0; i < d->n_textures; ++i) {
+            ksa::SourceTexture src = ksa::load_source_texture(d->textures[i], ht.luts);
+            DevTexture meta;
+            std::vector<uint8_t> blob = ksa::pack_texture(src, meta);
+            offs.push_back(ht.pool.size());
+            ht.pool.insert(ht.pool.end(), blob.begin(), blob.end());
+            ht.metas.push_back(meta);
+        }
+        ht.mat_of.assign(d->n_materials, -1);
+        if (d->material_fields) {
+            for (uint32_t m = 0; m < d->n_materials; ++m) {
+                const ks_material_fields *f = d->material_fields[m];
+                if (!f) continue;
+                DevTexMaterial t{};
+                const ks_field_desc *fields[9] = {&f->basecolor, &f->roughness, &f->metallic, &f->ior,
+                                                  &f->specular_r0_mul, &f->specular_trans, &f->diffuse_trans,
+                                                  &f->diffuse_trans_fwd, &f->emission};
+                static const char *names[9] = {"basecolor", "roughness", "metallic", "ior", "specular_r0_mul",
+                                               "specular_trans", "diffuse_trans", "diffuse_trans_fwd", "emission"};
+                for (int k = 0; k < 9; ++k)
+                    t.f[k] = ksa::to_dev_field(*fields[k], ht.metas, (k == 0 || k == 8) ? 3 : 1, names[k]);
+                t.has_emission = f->has_emission ? 1 : 0;
+                t.has_normal = f->has_normal_map ? 1 : 0;
+                if (t.has_normal) {
+                    t.normal = ksa::to_dev_field(f->normal_map, ht.metas, 3, "normal_map");
+                    KS_REQUIRE(f->normal_space == KS_NORMAL_MAP_TANGENT || f->normal_space == KS_NORMAL_MAP_WORLD,
+                               "unknown normal map space");
+                    t.normal_world = f->normal_space == KS_NORMAL_MAP_WORLD;
+                    t.normal_range01 = f->normal_range01 ? 1 : 0;
+                    t.normal_strength = f->normal_strength;
+                    // inverse transpose of the map's to_world (Transform::normal, maths.h:987)
+                    const float *a = f->normal_to_world;
+                    const double det = (double)a[0] * ((double)a[4] * a[8] - (double)a[5] * a[7]) -
+                                       (double)a[1] * ((double)a[3] * a[8] - (double)a[5] * a[6]) +
+                                       (double)a[2] * ((double)a[3] * a[7] - (double)a[4] * a[6]);
+                    KS_REQUIRE(!t.normal_world || det != 0.0, "normal map to_world is singular");
+                    if (t.normal_world) {
+                        const double cof[9] = {(double)a[4] * a[8] - (double)a[5] * a[7],
+                                               (double)a[5] * a[6] - (double)a[3] * a[8],
+                                               (double)a[3] * a[7] - (double)a[4] * a[6],
+                                               (double)a[2] * a[7] - (double)a[1] * a[8],
+                                               (double)a[0] * a[8] - (double)a[2] * a[6],
+                                               (double)a[1] * a[6] - (double)a[0] * a[7],
+                                               (double)a[1] * a[5] - (double)a[2] * a[4],
+                                               (double)a[2] * a[3] - (double)a[0] * a[5],
+                                               (double)a[0] * a[4] - (double)a[1] * a[3]};
+                        for (int k = 0; k < 9; ++k) t.normal_it[k] = (float)(cof[k] / det); // (A^-1)^T = cof / det
+                    }
+                }
+                t.has_opacity = f->has_opacity_map ? 1 : 0;
+                if (t.has_opacity) {
+                    t.opacity = ksa::to_dev_field(f->opacity, ht.metas, 1, "opacity");
+                    ht.alpha = true;
+                }
+                ht.mat_of[m] = (int32_t)ht.mats.size();
+                ht.mats.push_back(t);
+            }
+        }
+    } catch (const std::exception &ex) {
+        ksa::set_error(ex.what());
+        return KS_ERR_INVALID;
+    }
+    for (size_t i = 0; i < ht.metas.size(); ++i) ht.metas[i].data = ht.pool.data() + offs[i];
+    if (d->texcoords) {
+        ht.uvs.resize(d->n_vertices);
+        for (uint32_t v = 0; v < d->n_vertices; ++v) ht.uvs[v] = make_float2(d->texcoords[2 * v], d->texcoords[2 * v + 1]);
+    }
+    return KS_OK;
+}
+
+// The closure constants of a textured material for host code that reads ks_material_desc (constant
+// field values; textured fields read as their constant value, which the kernels never use).
+ks_material_desc material_constants(const DevTexMaterial &t) {
+    ks_material_desc m{};
+    for (int k = 0; k < 3; ++k) m.basecolor[k] = t.f[kFieldBasecolor].value[k];
+    m.roughness = t.f[kFieldRoughness].value[0];
+    m.metallic = t.f[kFieldMetallic].value[0];
+    m.ior = t.f[kFieldIor].value[0];
+    m.specular_r0_mul = t.f[kFieldR0Mul].value[0];
+    m.specular_trans = t.f[kFieldSpecTrans].value[0];
+    m.diffuse_trans = t.f[kFieldDiffTrans].value[0];
+    m.diffuse_trans_fwd = t.f[kFieldDiffTransFwd].value[0];
+    for (int k = 0; k < 3; ++k) m.emission[k] = t.f[kFieldEmission].value[k];
+    m.has_emission = t.has_emission;
+    return m;
+}
+
+// SummedAreaTable (sat.h:9-61): FP64 prefix sums with a one-texel offset and clamped bilinear lookups.
+struct Sat {
+    std::vector<double> table;
+    int nx = 0, ny = 0;
+    Sat(const std::vector<float> &v, int nx_, int ny_) : table(v.size()), nx(nx_), ny(ny_) {
+        table[0] = v[0];
+        for (int x = 1; x < nx; ++x) table[x] = v[x] + table[x - 1];
+        for (int y = 1; y < ny; ++y) table[(size_t)y * nx] = v[(size_t)y * nx] + table[(size_t)(y - 1) * nx];
+        for (int y = 1; y < ny; ++y)
+            for (int x = 1; x < nx; ++x)
+                table[(size_t)y * nx + x] = v[(size_t)y * nx + x] + table[(size_t)y * nx + (x - 1)] +
+                                            table[(size_t)(y - 1) * nx + x] - table[(size_t)(y - 1) * nx + (x - 1)];
+    }
+    double fetch(int x, int y) const {
+        if (x == 0 || y == 0) return 0.0;
+        x = std::min(x - 1, nx - 1);
+        y = std::min(y - 1, ny - 1);
+        return table[(size_t)y * nx + x];
+    }
+    static void boundary(float u, int res, int &i0, int &i1, float &t) { // lerp_helper, Boundary tick, Clamp
+        const float us = u * (float)(res - 1);
+        i0 = (int)std::floor(us);
+        i1 = i0 + 1;
+        t = us - (float)i0;
+        i0 = std::min(std::max(i0, 0), res - 1);
+        i1 = std::min(std::max(i1, 0), res - 1);
+    }
+    double lookup(float px, float py) const {
+        int x0, x1, y0, y1;
+        float wx, wy;
+        boundary(px, nx + 1, x0, x1, wx);
+        boundary(py, ny + 1, y0, y1, wy);
+        return fetch(x0, y0) * (1.0 - wx) * (1.0 - wy) + fetch(x1, y0) * (wx) * (1.0 - wy) +
+               fetch(x0, y1) * (1.0 - wx) * (wy) + fetch(x1, y1) * (wx) * (wy);
+    }
+    float sum(float x0, float y0, float x1, float y1) const {
+        const double s = (lookup(x1, y1) - lookup(x0, y1)) + (lookup(x0, y0) - lookup(x1, y0));
+        return std::max((float)s, 0.0f);
+    }
+};
+
+struct Box2 { // AABB2 (aabb.h:11-135): empty as (+inf, -inf), zero extents when empty
+    float lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    void expand(float x, float y) {
+        lo[0] = std::min(lo[0], x), lo[1] = std::min(lo[1], y);
+        hi[0] = std::max(hi[0], x), hi[1] = std::max(hi[1], y);
+    }
+    bool empty() const { return lo[0] > hi[0] || lo[1] > hi[1]; }
+    float ext(int k) const { return empty() ? 0.0f : hi[k] - lo[k]; }
+    float area() const { return ext(0) * ext(1); }
+};
+
+float wrap_mod1(float a) { // mod(a, 1.0f) (maths.h:133-138)
+    const float r = a - std::floor(a / 1.0f) * 1.0f;
+    return r < 0 ? r + 1.0f : r;
+}
+
+// compute_mesh_light_importance (light.cpp:307-523) for the triangles [t0, t1) of one emissive geometry:
+// uniform fields give 2 pi area times the (constant) luminance x opacity; textured emission / opacity
+// average their texel importance under each triangle's uv bounds through a summed-area table.
+void mesh_light_importance(const ks_scene_desc *d, uint32_t t0, uint32_t t1, const ks_material_desc &m,
+                           const DevTexMaterial *tm, const TexContext &hctx, std::vector<float> &imp,
+                           std::vector<float> &area) {
+    const float two_pi = 6.28318530718f;
+    const uint32_t N = t1 - t0;
+    imp.assign(N, 0.0f);
+    area.assign(N, 0.0f);
+    const bool em_tex = tm && tm->f[kFieldEmission].tex >= 0;
+    const bool has_op = tm && tm->has_opacity;
+    const bool op_tex = has_op && tm->opacity.tex >= 0;
+    bool fallback = !em_tex && !op_tex;
+    int res[2] = {1, 1};
+    bool set = false;
+    int wu = 0, wv = 0, flip = 0;
+    float scl[2] = {1, 1}, off[2] = {0, 0};
+    auto take = [&](const DevField &f) {
+        const DevTexture &t = hctx.tex[f.tex];
+        res[0] = std::max(res[0], t.width);
+        res[1] = std::max(res[1], t.height);
+        if (!set) {
+            wu = f.wrap_u, wv = f.wrap_v, flip = f.flip_v;
+            scl[0] = f.uv_scale[0], scl[1] = f.uv_scale[1], off[0] = f.uv_offset[0], off[1] = f.uv_offset[1];
+            set = true;
+        } else if (wu != f.wrap_u || wv != f.wrap_v || flip != f.flip_v || scl[0] != f.uv_scale[0] ||
+                   scl[1] != f.uv_scale[1] || off[0] != f.uv_offset[0] || off[1] != f.uv_offset[1]) {
+            fallback = true; // textured fields with different settings
+        }
+    };
+    if (em_tex) take(tm->f[kFieldEmission]);
+    if (op_tex) take(tm->opacity);
+    std::vector<float> iuv((size_t)res[0] * res[1]);
+    float sum = 0.0f;
+    const float zero[4] = {0, 0, 0, 0};
+    for (int y = 0; y < res[1]; ++y)
+        for (int x = 0; x < res[0]; ++x) {
+            const float u = ((float)x + 0.5f) / (float)res[0], v = ((float)y + 0.5f) / (float)res[1];
+            float e[4] = {m.emission[0], m.emission[1], m.emission[2], 0.0f};
+            if (tm) field_eval0(hctx, tm->f[kFieldEmission], u, v, 3, e);
+            float importance = 0.212671f * e[0] + 0.715160f * e[1] + 0.072169f * e[2];
+            if (has_op) {
+                float o[4] = {0, 0, 0, 0};
+                field_eval0(hctx, tm->opacity, u, v, 1, o);
+                importance *= o[0];
+            }
+            iuv[(size_t)y * res[0] + x] = importance;
+            sum += importance;
+        }
+    const float avg = sum / (float)iuv.size();
+    auto tri_area = [&](uint32_t t) {
+        const float *a = d->vertices + 3 * d->indices[3 * t];
+        const float *b = d->vertices + 3 * d->indices[3 * t + 1];
+        const float *c = d->vertices + 3 * d->indices[3 * t + 2];
+        float e01[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        float e12[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
+        float cx = e01[1] * e12[2] - e01[2] * e12[1], cy = e01[2] * e12[0] - e01[0] * e12[2],
+              cz = e01[0] * e12[1] - e01[1] * e12[0];
+        return 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+    };
+    if (fallback) {
+        for (uint32_t i = 0; i < N; ++i) {
+            area[i] = tri_area(t0 + i);
+            imp[i] = two_pi * area[i] * avg;
+        }
+        return;
+    }
+    const Sat sat(iuv, res[0], res[1]);
+    const float before_one = 0x1.fffffep-1f;
+    const bool has_uv = d->texcoords && !(d->geom_flags && (d->geom_flags[d->tri_geom[t0]] & KS_GEOM_NO_TEXCOORD));
+    for (uint32_t i = 0; i < N; ++i) {
+        const uint32_t t = t0 + i;
+        const float pa = tri_area(t);
+        Box2 bound;
+        for (int k = 0; k < 3; ++k) {
+            float tc[2] = {0.0f, 0.0f};
+            if (has_uv) {
+                tc[0] = d->texcoords[2 * d->indices[3 * t + k]];
+                tc[1] = d->texcoords[2 * d->indices[3 * t + k] + 1];
+            }
+            if (flip) tc[1] = 1.0f - tc[1];
+            bound.expand(tc[0] * scl[0] + off[0], tc[1] * scl[1] + off[1]);
+        }
+        const int txmin = (int)std::floor(bound.lo[0]), txmax = (int)std::ceil(bound.hi[0]);
+        const int tymin = (int)std::floor(bound.lo[1]), tymax = (int)std::ceil(bound.hi[1]);
+        float acc = 0.0f, sum_b = 0.0f;
+        for (int ty = tymin; ty < tymax; ++ty)
+            for (int tx = txmin; tx < txmax; ++tx) {
+                Box2 b; // intersect(bound, tile)
+                b.lo[0] = std::max(bound.lo[0], (float)tx), b.lo[1] = std::max(bound.lo[1], (float)ty);
+                b.hi[0] = std::min(bound.hi[0], (float)tx + before_one);
+                b.hi[1] = std::min(bound.hi[1], (float)ty + before_one);
+                if (b.empty()) b = Box2();
+                const float b_area = b.area();
+                if (b_area == 0.0f) continue;
+                for (int k = 0; k < 2; ++k) {
+                    const int mode = k == 0 ? wu : wv;
+                    if (mode == KS_WRAP_REPEAT) {
+                        b.lo[k] = wrap_mod1(b.lo[k]);
+                        b.hi[k] = wrap_mod1(b.hi[k]);
+                    } else {
+                        b.lo[k] = std::min(std::max(b.lo[k], 0.0f), 1.0f);
+                        b.hi[k] = std::min(std::max(b.hi[k], 0.0f), 1.0f);
+                    }
+                    if (b.lo[k] == 0.0f && b.hi[k] == 0.0f) b.hi[k] = 1.0f / (float)(res[k] + 1);
+                    if (b.lo[k] == 1.0f && b.hi[k] == 1.0f) b.lo[k] = 1.0f - 1.0f / (float)(res[k] + 1);
+                }
+                const float mean = sat.sum(b.lo[0], b.lo[1], b.hi[0], b.hi[1]) /
+                                   ((b.ext(0) * (float)res[0]) * (b.ext(1) * (float)res[1]));
+                acc += mean * b_area;
+                sum_b += b_area;
+            }
+        if (sum_b > 0.0f) acc /= sum_b;
+        acc *= two_pi * pa;
+        imp[i] = acc;
+        area[i] = pa;
+    }
+}
+
 // DistribTable (distrib.cpp:9-29): running sum of f / n, normalised, last entry forced to 1; returns acc.
 float build_cdf(const float *f, uint32_t n, float *cdf) {
     cdf[0] = 0.0f;
@@ -1526,6 +1924,15 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
         return KS_ERR_INVALID;
     }
 
+    HostTextures ht;
+    if (int te = build_textures(d, ht)) return te;
+    // the material constants each triangle sees (textured materials: their constant field values)
+    auto material_of = [&](uint32_t m) -> ks_material_desc {
+        if (!ht.mat_of.empty() && ht.mat_of[m] >= 0) return material_constants(ht.mats[ht.mat_of[m]]);
+        return d->materials[m];
+    };
+    const TexContext hctx{ht.metas.data(), &ht.luts};
+
     std::vector<float4> verts(d->n_vertices);
     for (uint32_t i = 0; i < d->n_vertices; ++i)
         verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0.0f);
@@ -1533,9 +1940,9 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     for (uint32_t t = 0; t < d->n_tris; ++t)
         tri_idx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2],
                                 d->geom_material[d->tri_geom[t]] |
-                                    ((uint32_t)(d->geom_flags ? d->geom_flags[d->tri_geom[t]] & 3u : 0u) << 24));
+                                    ((uint32_t)(d->geom_flags ? d->geom_flags[d->tri_geom[t]] & 7u : 0u) << 24));
     std::vector<Material> mats(d->n_materials);
-    for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(d->materials[m]);
+    for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(material_of(m));
 
     // Light list in PowerLightSampler order (light.cpp:636-677): non-mesh lights, then per emissive
     // geometry the triangles with non-zero importance after swap compaction (light.cpp:524-558).
@@ -1589,24 +1996,16 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     for (uint32_t t0 = 0; t0 < d->n_tris;) {
         uint32_t t1 = t0;
         while (t1 < d->n_tris && d->tri_geom[t1] == d->tri_geom[t0]) ++t1;
-        const ks_material_desc &m = d->materials[d->geom_material[d->tri_geom[t0]]];
+        const uint32_t gm = d->geom_material[d->tri_geom[t0]];
+        const ks_material_desc m = material_of(gm);
         if (m.has_emission) {
-            float lum = 0.212671f * m.emission[0] + 0.715160f * m.emission[1] + 0.072169f * m.emission[2];
+            const DevTexMaterial *tm = (!ht.mat_of.empty() && ht.mat_of[gm] >= 0) ? &ht.mats[ht.mat_of[gm]] : nullptr;
+            const bool em_tex = tm && tm->f[kFieldEmission].tex >= 0;
             uint32_t N = t1 - t0;
             std::vector<uint32_t> ids(N);
-            std::vector<float> imp(N), area(N);
-            for (uint32_t i = 0; i < N; ++i) {
-                const float *a = d->vertices + 3 * d->indices[3 * (t0 + i)];
-                const float *b = d->vertices + 3 * d->indices[3 * (t0 + i) + 1];
-                const float *c = d->vertices + 3 * d->indices[3 * (t0 + i) + 2];
-                float e01[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-                float e12[3] = {c[0] - b[0], c[1] - b[1], c[2] - b[2]};
-                float cx = e01[1] * e12[2] - e01[2] * e12[1], cy = e01[2] * e12[0] - e01[0] * e12[2],
-                      cz = e01[0] * e12[1] - e01[1] * e12[0];
-                area[i] = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
-                imp[i] = two_pi * area[i] * lum;
-                ids[i] = i;
-            }
+            std::vector<float> imp, area;
+            mesh_light_importance(d, t0, t1, m, tm, hctx, imp, area);
+            for (uint32_t i = 0; i < N; ++i) ids[i] = i;
             for (uint32_t i = 0; i < N;) {
                 if (imp[i] == 0.0f) {
                     std::swap(ids[N - 1], ids[i]);
@@ -1622,6 +2021,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
                 l.type = 3;
                 l.prim = t0 + ids[i];
                 l.area = area[i];
+                l.color[3] = em_tex ? 1.0f : 0.0f; // emission read from its texture at each use
                 for (int k = 0; k < 3; ++k) {
                     l.color[k] = m.emission[k];
                     l.vec[k] = d->vertices[3 * d->indices[3 * l.prim] + k];
@@ -1662,6 +2062,15 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
                "BVH4 node array exceeds 4 GiB; build this scene with branching = 2");
     if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
     else KS_HIP(upload(sc->nodes, bvh.nodes, s));
+    if (ht.alpha) { // alpha-tested triangle records carry a non-zero v1.w for the traversal filter
+        for (auto &g : bvh.tris) {
+            const uint32_t m = d->geom_material[d->tri_geom[g.tri]];
+            if (ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity) {
+                const uint32_t one = 1u;
+                std::memcpy(&g.v1[3], &one, 4);
+            }
+        }
+    }
     KS_HIP(upload(sc->tris, bvh.tris, s));
     KS_HIP(upload(sc->verts, verts, s));
     if (d->normals) {
@@ -1675,6 +2084,17 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     KS_HIP(upload(sc->mats, mats, s));
     KS_HIP(upload(sc->lights, lights, s));
     KS_HIP(upload(sc->cdf, cdf, s));
+    if (ht.textured) {
+        KS_HIP(upload(sc->tex_pool, ht.pool, s));
+        std::vector<DevTexture> metas = ht.metas;
+        for (auto &t : metas) t.data = sc->tex_pool.as<uint8_t>() + (t.data - ht.pool.data());
+        KS_HIP(upload(sc->tex_meta, metas, s));
+        KS_HIP(upload(sc->tex_mats, ht.mats, s));
+        KS_HIP(upload(sc->tex_mat_of, ht.mat_of, s));
+        KS_HIP(upload(sc->uvs, ht.uvs, s));
+        std::vector<TexLuts> luts(1, ht.luts);
+        KS_HIP(upload(sc->tex_luts, luts, s));
+    }
     if (have_map) {
         KS_HIP(upload(sc->sky_rgb, host_sky.rgb, s));
         KS_HIP(upload(sc->sky_cond, host_sky.cond, s));
@@ -1699,6 +2119,15 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     ds.n_sky = n_sky;
     for (int k = 0; k < 4; ++k) ds.sky[k] = sky[k];
     ds.sky_map = host_sky.map;
+    ds.textured = ht.textured ? 1 : 0;
+    ds.alpha = ht.alpha ? 1 : 0;
+    if (ht.textured) {
+        ds.uvs = ht.uvs.empty() ? nullptr : sc->uvs.as<float2>();
+        ds.textures = sc->tex_meta.as<DevTexture>();
+        ds.tex_luts = sc->tex_luts.as<TexLuts>();
+        ds.tex_mats = ht.mats.empty() ? nullptr : sc->tex_mats.as<DevTexMaterial>();
+        ds.tex_mat_of = ht.mats.empty() ? nullptr : sc->tex_mat_of.as<int32_t>();
+    }
 
     ks_scene_info &in = sc->info;
     in.n_tris = d->n_tris;
@@ -1711,7 +2140,8 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     // traversal stack bound: every interior node on the path leaves at most width - 1 entries behind
     sc->max_stack = (uint32_t)(width - 1) * in.max_depth + 1;
     in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->verts.bytes + sc->normals.bytes + sc->tri_idx.bytes +
-                      sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes;
+                      sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes + sc->tex_pool.bytes +
+                      sc->tex_mats.bytes + sc->uvs.bytes;
     for (int k = 0; k < 3; ++k) {
         in.bound_lo[k] = bvh.lo[k];
         in.bound_hi[k] = bvh.hi[k];
@@ -2325,14 +2755,14 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
     DevBuf out;
     if (hits) {
         KS_HIP(out.alloc(sizeof(float4) * n));
-        launch_closest(scene->ds.width, dim3(grid), s, scene->ds, (const uint32_t *)nullptr, (uint32_t)n,
+        launch_closest(scene->ds, dim3(grid), s, (const uint32_t *)nullptr, (uint32_t)n,
                        (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<float4>(), dev->ws.ovf.as<uint32_t>(),
                        dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(hits, out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
     } else {
         KS_HIP(out.alloc(n));
-        launch_anyhit<0>(scene->ds.width, dim3(grid), s, scene->ds, (const uint32_t *)nullptr, (uint32_t)n,
+        launch_anyhit<0>(scene->ds, dim3(grid), s, (const uint32_t *)nullptr, (uint32_t)n,
                          (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
                          (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr,
                          dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());

@@ -3,6 +3,7 @@
 #pragma once
 #include "ks_device.h"
 #include "ks_sky.h"
+#include "ks_texture.h"
 
 namespace ksr {
 using namespace ksd;
@@ -42,6 +43,14 @@ struct DevScene {
     int32_t sky[4];
     int32_t width; // BVH node width: 2 (GpuNode, 64 B) or 4 (GpuNode4, 128 B)
     ksd::SkyMap sky_map; // the map of a light of type 4 (environment-map sky), when the scene has one
+    // textured scenes (all nullptr otherwise)
+    const float2 *uvs;                  // per-vertex texture coordinates
+    const ksd::DevTexture *textures;
+    const ksd::TexLuts *tex_luts;
+    const ksd::DevTexMaterial *tex_mats;
+    const int32_t *tex_mat_of;          // per material: index into tex_mats, or -1
+    int32_t textured;                   // 1: some geometry has texcoords or some material has fields
+    int32_t alpha;                      // 1: some triangle is alpha-tested (opacity map)
 };
 
 struct RenderConsts {
@@ -57,6 +66,8 @@ struct RenderConsts {
     int32_t n_segs;
     float p2w[16]; // proj_to_world with the eye folded into rows 0..2 (see camera_ray)
     float cam[3];
+    float cam_dx[3], cam_dy[3]; // world offsets of one pixel step on the film plane (Camera::spawn_ray)
+    float diff_scale;           // 1 / sqrt(spp) with scale_ray_diff, else 1
 };
 
 struct PathBufs {

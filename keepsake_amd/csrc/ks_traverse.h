@@ -379,10 +379,17 @@ KS_DEV uint32_t visit_node(const float4 *__restrict__ nodes, const TravRay &r, f
 // (the persistent speculative while-while loop of Aila and Laine, HPG 2009). Each ray's closest hit is
 // the one traverse_w gives; among triangles hit at exactly the same t, the first one tested wins.
 // load(i, ray) fills the ray and returns its tmax; store(i, hit) takes the result (prim < 0: miss).
-template <bool ANY, int W, typename Load, typename Store>
+// With ALPHA, a triangle whose record carries a non-zero v1.w is an alpha-tested candidate: filter(prim,
+// u, v, ray) decides whether the hit counts (Embree's intersection filter, scene.cpp:175-204). A hit it
+// rejects is skipped and traversal goes on, for closest and any hits alike.
+struct NoFilter {
+    KS_DEV bool operator()(int32_t, float, float, const TravRay &) const { return true; }
+};
+
+template <bool ANY, int W, bool ALPHA = false, typename Load, typename Store, typename Filter = NoFilter>
 KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t i, uint32_t n,
                        uint32_t stride, TravStack &st, Load &&load, Store &&store, uint32_t &n_nodes,
-                       uint32_t &n_tris, uint32_t &n_rays) {
+                       uint32_t &n_tris, uint32_t &n_rays, Filter filter = Filter()) {
     if (i >= n) return;
     TravRay r;
     TravHit hit;
@@ -419,6 +426,7 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
                 float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
                 float t, u, v;
                 if (isect_tri(r, v0, v1, v2, tmax, t, u, v)) {
+                    if (ALPHA && __float_as_uint(v1.w) != 0u && !filter(__float_as_int(v0.w), u, v, r)) continue;
                     hit.prim = __float_as_int(v0.w);
                     if (ANY) {
                         done = true;

@@ -754,12 +754,30 @@ struct Camera {
         for (int i = 0; i < 4; ++i) r[i] = m.a[i][0] * v[0] + m.a[i][1] * v[1] + m.a[i][2] * v[2] + m.a[i][3] * v[3];
         return V3((float)(r[0] / r[3]), (float)(r[1] / r[3]), (float)(r[2] / r[3]));
     }
-    // spawn_ray (camera.cpp:129-170), perspective branch; differentials are not needed (no textures).
-    void spawn_ray(float fx, float fy, V3 &o, V3 &d) const {
+    // spawn_ray (camera.cpp:129-170), perspective branch, with the ray differentials' directions for a
+    // film of res_x x res_y, scaled by 1 / sqrt(spp)
+    void spawn_ray(float fx, float fy, V3 &o, V3 &d, V3 *rxd = nullptr, V3 *ryd = nullptr, int res_x = 1,
+                   int res_y = 1, int spp = 1) const {
         V3 ndc(fx * 2.0f - 1.0f, fy * 2.0f - 1.0f, 1.0f);
         V3 world_pos = point_hdiv(proj_to_world, ndc);
         o = position;
         d = normalized(world_pos - position);
+        if (!rxd) return;
+        V3 mid = point_hdiv(proj_to_camera, V3(0.0f, 0.0f, 1.0f));
+        V3 right = point_hdiv(proj_to_camera, V3(2.0f / res_x, 0.0f, 1.0f));
+        V3 up = point_hdiv(proj_to_camera, V3(0.0f, 2.0f / res_y, 1.0f));
+        auto dir = [&](V3 v) { // camera_to_world.direction
+            const M4 &m = camera_to_world;
+            return V3((float)(m.a[0][0] * v.x + m.a[0][1] * v.y + m.a[0][2] * v.z),
+                      (float)(m.a[1][0] * v.x + m.a[1][1] * v.y + m.a[1][2] * v.z),
+                      (float)(m.a[2][0] * v.x + m.a[2][1] * v.y + m.a[2][2] * v.z));
+        };
+        V3 cdx = dir(right - mid), cdy = dir(up - mid);
+        *rxd = normalized(world_pos + cdx - position);
+        *ryd = normalized(world_pos + cdy - position);
+        float scale = 1.0f / std::sqrt((float)spp);
+        *rxd = d + (*rxd - d) * scale;
+        *ryd = d + (*ryd - d) * scale;
     }
 };
 
@@ -965,7 +983,341 @@ struct SkyMapLight {
     }
 };
 
+// ----------------------------------------------------------------------------------------------------
+// Textures and shader fields (texture.h, texture.cpp:12-1121, shader_field.h:11-86)
+// ----------------------------------------------------------------------------------------------------
+float srgb_to_linear_f(float x) { return x < 0.04045f ? x / 12.92f : std::pow((x + 0.055f) / 1.055f, 2.4f); } // maths.h:631
+float linear_to_srgb_f(float x) { // maths.h:649-659
+    if (x <= 0.0f) return 0.0f;
+    if (x >= 1.0f) return 1.0f;
+    if (x < 0.0031308f) return x * 12.92f;
+    return std::pow(x, 1.0f / 2.4f) * 1.055f - 0.055f;
+}
+
+struct TexTables {
+    float srgb[256]; // srgb_u8_to_linear_f32_lut (texture.cpp:376-401): the sRGB curve at i / 255
+    float ewa[128];  // ewa_lut (texture.cpp:720-856): exp(-2 r2) - exp(-2) at r2 = i / 127
+    TexTables() {
+        for (int i = 0; i < 256; ++i) {
+            double x = (double)i / 255.0;
+            srgb[i] = (float)(x < 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+        }
+        for (int i = 0; i < 128; ++i) ewa[i] = (float)(std::exp(-2.0 * i / 127.0) - std::exp(-2.0));
+    }
+};
+const TexTables g_tex_tables;
+
+struct Texture {
+    int width = 0, height = 0, nc = 0, type = 0; // type 0 u8, 1 u16, 2 f32
+    bool srgb = false;
+    struct Level {
+        int w, h;
+        std::vector<uint8_t> bytes;
+    };
+    std::vector<Level> mips;
+
+    int stride() const { return nc * (type == 0 ? 1 : (type == 1 ? 2 : 4)); }
+    int levels() const { return (int)mips.size(); }
+    // Texture::fetch_as_float (texture.cpp:403-460)
+    void fetch(int x, int y, int l, float *out) const {
+        const uint8_t *b = mips[l].bytes.data() + ((size_t)y * mips[l].w + x) * stride();
+        for (int c = 0; c < nc; ++c) {
+            if (type == 0) {
+                out[c] = srgb ? g_tex_tables.srgb[b[c]] : (float)b[c] / 255.0f;
+            } else if (type == 1) {
+                uint16_t v;
+                std::memcpy(&v, b + 2 * c, 2);
+                out[c] = (float)v / 65535.0f;
+                if (srgb) out[c] = srgb_to_linear_f(out[c]);
+            } else {
+                float v;
+                std::memcpy(&v, b + 4 * c, 4);
+                out[c] = srgb ? srgb_to_linear_f(v) : v;
+            }
+        }
+    }
+    // Texture::set_from_float (texture.cpp:462-518)
+    void store(int x, int y, int l, const float *in) {
+        uint8_t *b = mips[l].bytes.data() + ((size_t)y * mips[l].w + x) * stride();
+        for (int c = 0; c < nc; ++c) {
+            float f = srgb ? linear_to_srgb_f(in[c]) : in[c];
+            if (type == 0) {
+                b[c] = (uint8_t)std::floor(f * 255.0f);
+            } else if (type == 1) {
+                uint16_t v = (uint16_t)std::floor(f * 65535.0f);
+                std::memcpy(b + 2 * c, &v, 2);
+            } else {
+                std::memcpy(b + 4 * c, &f, 4);
+            }
+        }
+    }
+    // One texel of level l as the weighted sum of level l - 1 texels (each term weight * value).
+    void filter_texel(int l, int x, int y, const int *xs, const float *wx, int nx, const int *ys, const float *wy,
+                      int ny, bool wx_only, bool wy_only) {
+        float sum[4] = {0, 0, 0, 0}, v[4];
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                fetch(xs[i], ys[j], l - 1, v);
+                float w = wx_only ? wx[i] : (wy_only ? wy[j] : wx[i] * wy[j]);
+                for (int c = 0; c < nc; ++c) sum[c] += w * v[c];
+            }
+        store(x, y, l, sum);
+    }
+    // Texture(bytes, ..., build_mipmaps) (texture.cpp:21-347): the polyphase NPOT box filter. An odd
+    // size 2n - 1 maps to n with taps 2x - 1, 2x, 2x + 1 weighted x, n, n - x - 1 over 2n - 1; an even
+    // size halves with two taps of 1/2; a size of 1 stays.
+    void build_level(int l) {
+        const int lw = mips[l - 1].w, lh = mips[l - 1].h, w = mips[l].w, h = mips[l].h;
+        const bool odd_x = lw == 2 * w - 1, odd_y = lh == 2 * h - 1;
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                int xs[3], ys[3], nx = 0, ny = 0;
+                float wx[3], wy[3];
+                if (lw == 1) {
+                    xs[nx] = x, wx[nx++] = 1.0f;
+                } else if (odd_x) {
+                    if (2 * x - 1 > 0) xs[nx] = 2 * x - 1, wx[nx++] = (float)x / (float)lw;
+                    xs[nx] = 2 * x, wx[nx++] = (float)w / (float)lw;
+                    if (2 * x + 1 < lw) xs[nx] = 2 * x + 1, wx[nx++] = (float)(w - x - 1) / (float)lw;
+                } else {
+                    xs[nx] = 2 * x, wx[nx++] = 0.5f;
+                    xs[nx] = 2 * x + 1, wx[nx++] = 0.5f;
+                }
+                if (lh == 1) {
+                    ys[ny] = y, wy[ny++] = 1.0f;
+                } else if (odd_y) {
+                    if (2 * y - 1 > 0) ys[ny] = 2 * y - 1, wy[ny++] = (float)y / (float)lh;
+                    ys[ny] = 2 * y, wy[ny++] = (float)h / (float)lh;
+                    if (2 * y + 1 < lh) ys[ny] = 2 * y + 1, wy[ny++] = (float)(h - y - 1) / (float)lh;
+                } else {
+                    ys[ny] = 2 * y, wy[ny++] = 0.5f;
+                    ys[ny] = 2 * y + 1, wy[ny++] = 0.5f;
+                }
+                filter_texel(l, x, y, xs, wx, nx, ys, wy, ny, lh == 1, lw == 1);
+            }
+    }
+    bool init(const ora_texture &d) {
+        if (!d.data || d.width <= 0 || d.height <= 0 || d.channels < 1 || d.channels > 4 || d.data_type < 0 ||
+            d.data_type > 2 || d.levels < 1)
+            return false;
+        width = d.width, height = d.height, nc = d.channels, type = d.data_type, srgb = d.color_space == 1;
+        int full = 1 + (int)std::ceil(std::log2((double)std::max(width, height)));
+        if (d.levels > full) return false;
+        int n = (d.levels == 1 && d.build_mipmaps) ? full : d.levels;
+        const uint8_t *src = (const uint8_t *)d.data;
+        int w = width, h = height;
+        for (int l = 0; l < n; ++l) {
+            Level lv{w, h, std::vector<uint8_t>((size_t)w * h * stride())};
+            if (l < d.levels) {
+                std::memcpy(lv.bytes.data(), src, lv.bytes.size());
+                src += lv.bytes.size();
+            }
+            mips.push_back(std::move(lv));
+            w = std::max(1, (w + 1) / 2);
+            h = std::max(1, (h + 1) / 2);
+        }
+        for (int l = d.levels; l < n; ++l) build_level(l);
+        return true;
+    }
+};
+
+int wrap_index(int x, int dim, int mode) { // texture.cpp:520-531
+    if (mode == 0) return ((x % dim) + dim) % dim;
+    return std::clamp(x, 0, dim - 1);
+}
+
+struct Sampler { // TextureSampler and its kinds (texture.h:68-145)
+    int kind = 1, wrap_u = 0, wrap_v = 0;
+    float ca[4] = {0, 0, 0, 0}, cb[4] = {0, 0, 0, 0};
+    float anisotropy = 8.0f;
+
+    void bilinear(const Texture &t, int l, float su, float sv, float *out) const { // texture.cpp:541-573
+        float u = su * t.mips[l].w - 0.5f, v = sv * t.mips[l].h - 0.5f;
+        int u0 = (int)std::floor(u), v0 = (int)std::floor(v);
+        float du = u - u0, dv = v - v0;
+        u0 = wrap_index(u0, t.mips[l].w, wrap_u);
+        v0 = wrap_index(v0, t.mips[l].h, wrap_v);
+        int u1 = wrap_index(u0 + 1, t.mips[l].w, wrap_u), v1 = wrap_index(v0 + 1, t.mips[l].h, wrap_v);
+        float a[4], b[4], c[4], d[4];
+        t.fetch(u0, v0, l, a);
+        t.fetch(u1, v0, l, b);
+        t.fetch(u0, v1, l, c);
+        t.fetch(u1, v1, l, d);
+        float w00 = (1 - du) * (1 - dv), w10 = du * (1 - dv), w01 = (1 - du) * dv, w11 = du * dv;
+        for (int k = 0; k < t.nc; ++k) out[k] = w00 * a[k] + w10 * b[k] + w01 * c[k] + w11 * d[k];
+    }
+    static float poly(const float *k, float x) { return k[0] * (x * x * x) + k[1] * (x * x) + k[2] * x + k[3]; }
+    void bicubic(const Texture &t, int l, float su, float sv, float *out) const { // texture.cpp:629-677
+        float u = su * t.mips[l].w - 0.5f, v = sv * t.mips[l].h - 0.5f;
+        int u0 = (int)std::floor(u), v0 = (int)std::floor(v);
+        float du = u - u0, dv = v - v0;
+        float rows[4][4];
+        for (int y = 0; y < 4; ++y) {
+            int vy = wrap_index(v0 + y - 1, t.mips[l].h, wrap_v);
+            float c[4][4];
+            for (int x = 0; x < 4; ++x) t.fetch(wrap_index(u0 + x - 1, t.mips[l].w, wrap_u), vy, l, c[x]);
+            // spline (texture.cpp:631-641) with x = du
+            float a0 = poly(cb, (float)(du + 1.0)), a1 = poly(ca, du), a2 = poly(ca, (float)(1.0 - du)),
+                  a3 = poly(cb, (float)(2.0 - du));
+            for (int k = 0; k < t.nc; ++k) rows[y][k] = c[0][k] * a0 + c[1][k] * a1 + c[2][k] * a2 + c[3][k] * a3;
+        }
+        float a0 = poly(cb, (float)(dv + 1.0)), a1 = poly(ca, dv), a2 = poly(ca, (float)(1.0 - dv)),
+              a3 = poly(cb, (float)(2.0 - dv));
+        for (int k = 0; k < t.nc; ++k) out[k] = rows[0][k] * a0 + rows[1][k] * a1 + rows[2][k] * a2 + rows[3][k] * a3;
+    }
+    void ewa(const Texture &t, int l, float u, float v, float maj[2], float mnr[2], float *out) const { // :858-920
+        u = u * t.mips[l].w - 0.5f;
+        v = v * t.mips[l].h - 0.5f;
+        float M0 = maj[0] * t.mips[l].w, M1 = maj[1] * t.mips[l].h;
+        float m0 = mnr[0] * t.mips[l].w, m1 = mnr[1] * t.mips[l].h;
+        float A = M1 * M1 + m1 * m1 + 1, B = -2 * (M0 * M1 + m0 * m1), C = M0 * M0 + m0 * m0 + 1;
+        float invF = 1 / (A * C - B * B * 0.25f);
+        A *= invF, B *= invF, C *= invF;
+        float det = -B * B + 4 * A * C, invDet = 1 / det;
+        float uS = std::sqrt(std::max(0.0f, det * C)), vS = std::sqrt(std::max(0.0f, A * det));
+        float fs0 = std::ceil(u - 2 * invDet * uS), fs1 = std::floor(u + 2 * invDet * uS);
+        float ft0 = std::ceil(v - 2 * invDet * vS), ft1 = std::floor(v + 2 * invDet * vS);
+        if (!(fs1 - fs0 <= 4096.0f && ft1 - ft0 <= 4096.0f && std::fabs(fs0) < 1e9f && std::fabs(ft0) < 1e9f)) {
+            // non-finite footprint (undefined in the reference): the product's guard, a bilinear read
+            bilinear(t, l, (u + 0.5f) / t.mips[l].w, (v + 0.5f) / t.mips[l].h, out);
+            return;
+        }
+        int s0 = (int)fs0, s1 = (int)fs1, t0 = (int)ft0, t1 = (int)ft1;
+        float sum[4] = {0, 0, 0, 0}, wsum = 0, tx[4];
+        for (int it = t0; it <= t1; ++it) {
+            float tt = it - v;
+            for (int is = s0; is <= s1; ++is) {
+                float ss = is - u;
+                float r2 = A * ss * ss + B * ss * tt + C * tt * tt;
+                if (r2 < 1) {
+                    float wgt = g_tex_tables.ewa[std::min<int>(r2 * 128, 127)];
+                    t.fetch(wrap_index(is, t.mips[l].w, wrap_u), wrap_index(it, t.mips[l].h, wrap_v), l, tx);
+                    for (int k = 0; k < t.nc; ++k) sum[k] += wgt * tx[k];
+                    wsum += wgt;
+                }
+            }
+        }
+        for (int k = 0; k < t.nc; ++k) out[k] = wsum > 0 ? sum[k] * (1.0f / wsum) : 0.0f;
+    }
+    void one_level(const Texture &t, int l, float u, float v, float maj[2], float mnr[2], float *out) const {
+        if (kind == 2) bicubic(t, l, u, v, out);
+        else if (kind == 3) ewa(t, l, u, v, maj, mnr, out);
+        else bilinear(t, l, u, v, out);
+    }
+    // operator() of Nearest / Linear / Cubic / EWA samplers (texture.cpp:575-716); d = duvdxy rows
+    void sample(const Texture &t, float u, float v, const float d[4], float *out) const {
+        if (kind == 0) {
+            int x = (int)std::floor(u * t.mips[0].w - 0.5f), y = (int)std::floor(v * t.mips[0].h - 0.5f);
+            t.fetch(wrap_index(x, t.mips[0].w, wrap_u), wrap_index(y, t.mips[0].h, wrap_v), 0, out);
+            return;
+        }
+        float maj[2] = {d[0], d[1]}, mnr[2] = {d[2], d[3]};
+        float width;
+        if (kind == 3) {
+            float lmaj = std::sqrt(maj[0] * maj[0] + maj[1] * maj[1]), lmin = std::sqrt(mnr[0] * mnr[0] + mnr[1] * mnr[1]);
+            if (lmaj < lmin) {
+                std::swap(maj, mnr);
+                std::swap(lmaj, lmin);
+            }
+            if (lmin * anisotropy < lmaj && lmin > 0) {
+                float sc = lmaj / (lmin * anisotropy);
+                mnr[0] *= sc, mnr[1] *= sc;
+                lmin *= sc;
+            }
+            if (lmin == 0) {
+                bilinear(t, 0, u, v, out);
+                return;
+            }
+            width = lmin;
+        } else {
+            width = std::max(std::max(std::fabs(d[0]), std::fabs(d[1])), std::max(std::fabs(d[2]), std::fabs(d[3])));
+        }
+        float level = t.levels() - 1 + std::log2(std::max(width, 1e-8f));
+        if (level < 0 || t.levels() == 1) {
+            one_level(t, 0, u, v, maj, mnr, out);
+        } else if (level >= t.levels() - 1) {
+            t.fetch(0, 0, t.levels() - 1, out);
+        } else {
+            int il = (int)std::floor(level);
+            float delta = level - il, o0[4], o1[4];
+            one_level(t, il, u, v, maj, mnr, o0);
+            one_level(t, il + 1, u, v, maj, mnr, o1);
+            for (int k = 0; k < t.nc; ++k) out[k] = std::lerp(o0[k], o1[k], delta);
+        }
+    }
+};
+
+struct Field { // ConstantField / TextureField<N> (shader_field.h:26-86)
+    int tex = -1;
+    Sampler smp;
+    bool flip_v = true;
+    int swizzle[4] = {0, 1, 2, 3};
+    float uv_scale[2] = {1, 1}, uv_offset[2] = {0, 0}, scale[4] = {1, 1, 1, 1}, value[4] = {0, 0, 0, 0};
+
+    bool init(const ora_field &f, int n, const std::vector<Texture> &texs) {
+        tex = f.texture;
+        for (int k = 0; k < 4; ++k) value[k] = f.value[k];
+        if (tex < 0) return true;
+        if (tex >= (int)texs.size() || f.sampler < 0 || f.sampler > 3 || f.cubic_kernel < 0 || f.cubic_kernel > 2)
+            return false;
+        smp.kind = f.sampler;
+        smp.wrap_u = f.wrap_u;
+        smp.wrap_v = f.wrap_v;
+        smp.anisotropy = std::clamp(f.anisotropy, 1.0f, 16.0f);
+        // CubicSampler kernels (texture.h:92-116)
+        static const float ka[3][4] = {{21, -36, 0, 16}, {3, -6, 0, 4}, {3, -5, 0, 2}};
+        static const float kb[3][4] = {{-7, 36, -60, 32}, {-1, 6, -12, 8}, {-1, 5, -8, 4}};
+        static const float kd[3] = {18, 6, 2};
+        for (int j = 0; j < 4; ++j) {
+            smp.ca[j] = ka[f.cubic_kernel][j] / kd[f.cubic_kernel];
+            smp.cb[j] = kb[f.cubic_kernel][j] / kd[f.cubic_kernel];
+        }
+        flip_v = f.flip_v != 0;
+        for (int k = 0; k < 4; ++k) {
+            swizzle[k] = k < n ? f.swizzle[k] : -1;
+            if (swizzle[k] >= texs[tex].nc) return false;
+            scale[k] = f.scale[k];
+        }
+        uv_scale[0] = f.uv_scale[0], uv_scale[1] = f.uv_scale[1];
+        uv_offset[0] = f.uv_offset[0], uv_offset[1] = f.uv_offset[1];
+        return true;
+    }
+    // TextureField::operator() (shader_field.h:53-77)
+    void eval(const std::vector<Texture> &texs, float u, float v, const float duv[4], int n, float *out) const {
+        if (tex < 0) {
+            for (int k = 0; k < n; ++k) out[k] = value[k];
+            return;
+        }
+        float d[4] = {duv[0], duv[1], duv[2], duv[3]};
+        if (flip_v) {
+            v = 1.0f - v;
+            d[1] *= -1.0f;
+            d[3] *= -1.0f;
+        }
+        u = u * uv_scale[0] + uv_offset[0];
+        v = v * uv_scale[1] + uv_offset[1];
+        d[0] *= uv_scale[0];
+        d[2] *= uv_scale[0];
+        d[1] *= uv_scale[1];
+        d[3] *= uv_scale[1];
+        float s[4] = {0, 0, 0, 0};
+        smp.sample(texs[tex], u, v, d, s);
+        for (int k = 0; k < n; ++k) out[k] = (swizzle[k] >= 0 ? s[swizzle[k]] : 0.0f) * scale[k];
+    }
+};
+
+struct TexMaterial {
+    Field f[9]; // basecolor, roughness, metallic, ior, specular_r0_mul, specular_trans, diffuse_trans,
+                // diffuse_trans_fwd, emission
+    bool has_emission = false, has_normal = false, has_opacity = false;
+    Field normal, opacity;
+    bool normal_world = false, normal_range01 = true;
+    float normal_strength = 1.0f;
+    float normal_m[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}; // map to_world (linear part)
+};
+
 struct Light {
+    bool tex_emission = false; // mesh light whose emission field is a texture
     int type;  // 0 sky ambient, 1 directional, 2 point, 3 mesh triangle, 4 sky map
     V3 color;  // ambient / L / I / emission
     V3 vec;    // dir or pos
@@ -986,11 +1338,49 @@ struct Scene {
     std::vector<float> powers;
     std::vector<uint32_t> sky_lights;
     SkyMapLight sky_map; // texels of the type-4 light, when there is one
+    std::vector<Texture> textures;
+    std::vector<TexMaterial> tmats;
+    std::vector<int> tmat_of;        // per material: index into tmats or -1
+    std::vector<float> uvs;          // 2 per vertex (empty: none)
+    std::vector<uint8_t> tri_alpha;  // per triangle: material has an opacity map
     AABB bound;
     bool has_normals = false;
 
     V3 vtx(uint32_t t, int k) const { return verts[idx[3 * t + k]]; }
+    bool has_uv(uint32_t t) const { return !uvs.empty() && !(geom_flags[tri_geom[t]] & 4); }
+    const TexMaterial *tmat(uint32_t t) const {
+        if (tmat_of.empty()) return nullptr;
+        int k = tmat_of[geom_mat[tri_geom[t]]];
+        return k >= 0 ? &tmats[k] : nullptr;
+    }
+    // rtcInterpolate0 of the texcoords at barycentric (b1, b2) (MeshGeometry::compute_hit_texcoord /
+    // interpolate_texcoord, geometry.cpp:165-190): zero for a mesh without texcoords
+    void texcoord(uint32_t t, float b1, float b2, float &u, float &v) const {
+        u = v = 0.0f;
+        if (!has_uv(t)) return;
+        const float *a = &uvs[2 * idx[3 * t]], *b = &uvs[2 * idx[3 * t + 1]], *c = &uvs[2 * idx[3 * t + 2]];
+        float w = 1.0f - b1 - b2;
+        u = a[0] * w + b[0] * b1 + c[0] * b2;
+        v = a[1] * w + b[1] * b1 + c[1] * b2;
+    }
 };
+
+// OpacityMap::stochastic_test(uv, ro, rd) (opacity_map.cpp:20-31) inside the intersection filter
+// (scene.cpp:175-204): hash_float(ro, rd) = low 32 bits of MurmurHash64A over the 24 bytes, * 2^-32.
+bool opacity_pass(const Scene &s, uint32_t t, float b1, float b2, V3 ro, V3 rd) {
+    const TexMaterial *m = s.tmat(t);
+    float u, v;
+    s.texcoord(t, b1, b2, u, v);
+    const float zero[4] = {0, 0, 0, 0};
+    float o[4];
+    m->opacity.eval(s.textures, u, v, zero, 1, o);
+    float op = clampv(o[0], 0.0f, 1.0f);
+    if (op <= 0.0f) return false;
+    if (op >= 1.0f) return true;
+    float buf[6] = {ro.x, ro.y, ro.z, rd.x, rd.y, rd.z};
+    float r = (float)(uint32_t)murmur64a((const unsigned char *)buf, 24, 0) * 0x1p-32f;
+    return r < op;
+}
 
 uint32_t build_rec(Scene &s, std::vector<uint32_t> &refs, std::vector<AABB> &tb, std::vector<V3> &tc, uint32_t begin,
                    uint32_t end) {
@@ -1167,6 +1557,8 @@ Hit closest(const Scene &s, RayQ r) {
                 V3 coord;
                 float th;
                 if (isect_tri(r, s.vtx(t, 0), s.vtx(t, 1), s.vtx(t, 2), coord, th) && th < r.tmax) {
+                    if (!s.tri_alpha.empty() && s.tri_alpha[t] && !opacity_pass(s, t, coord.y, coord.z, r.o, r.d))
+                        continue;
                     r.tmax = th;
                     h.t = th;
                     h.u = coord.y;
@@ -1204,7 +1596,11 @@ bool occluded(const Scene &s, RayQ r) { // cuda/sbvh.cu:562-603
                 uint32_t t = s.prims[nd.prim_offset + i];
                 V3 coord;
                 float th;
-                if (isect_tri(r, s.vtx(t, 0), s.vtx(t, 1), s.vtx(t, 2), coord, th)) return true;
+                if (isect_tri(r, s.vtx(t, 0), s.vtx(t, 1), s.vtx(t, 2), coord, th)) {
+                    if (!s.tri_alpha.empty() && s.tri_alpha[t] && !opacity_pass(s, t, coord.y, coord.z, r.o, r.d))
+                        continue;
+                    return true;
+                }
             }
             if (sp == 0) break;
             cur = stack[--sp];
@@ -1267,6 +1663,127 @@ float light_probability(const Scene &s, uint32_t i) { // light.cpp:680-683
     return ((s.cdf[i + 1] - s.cdf[i]) * (float)n) / (float)n;
 }
 
+// SummedAreaTable (sat.h:9-61)
+struct SAT {
+    std::vector<double> table;
+    int nx, ny;
+    SAT(const std::vector<float> &v, int nx_, int ny_) : table(v.size()), nx(nx_), ny(ny_) {
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                size_t i = (size_t)y * nx + x;
+                if (x == 0 && y == 0) table[i] = v[i];
+                else if (y == 0) table[i] = v[i] + table[i - 1];
+                else if (x == 0) table[i] = v[i] + table[i - nx];
+                else table[i] = v[i] + table[i - 1] + table[i - nx] - table[i - nx - 1];
+            }
+    }
+    double at(int x, int y) const { return (x == 0 || y == 0) ? 0.0 : table[(size_t)std::min(y - 1, ny - 1) * nx + std::min(x - 1, nx - 1)]; }
+    double lookup(float px, float py) const { // lerp_helper, TickMode::Boundary, WrapMode::Clamp, over (nx + 1, ny + 1)
+        float xs = px * nx, ys = py * ny;
+        int x0 = (int)std::floor(xs), y0 = (int)std::floor(ys);
+        float wx = xs - x0, wy = ys - y0;
+        int x1 = std::clamp(x0 + 1, 0, nx), y1 = std::clamp(y0 + 1, 0, ny);
+        x0 = std::clamp(x0, 0, nx), y0 = std::clamp(y0, 0, ny);
+        return at(x0, y0) * (1.0 - wx) * (1.0 - wy) + at(x1, y0) * (wx) * (1.0 - wy) + at(x0, y1) * (1.0 - wx) * (wy) +
+               at(x1, y1) * (wx) * (wy);
+    }
+    float sum(const float lo[2], const float hi[2]) const {
+        double v = (lookup(hi[0], hi[1]) - lookup(lo[0], hi[1])) + (lookup(lo[0], lo[1]) - lookup(hi[0], lo[1]));
+        return std::max((float)v, 0.0f);
+    }
+};
+
+void mesh_importance(const Scene &s, uint32_t t0, uint32_t t1, V3 Le, const TexMaterial *tm, std::vector<float> &imp,
+                     std::vector<float> &area) {
+    const bool em_tex = tm && tm->f[8].tex >= 0, op = tm && tm->has_opacity, op_tex = op && tm->opacity.tex >= 0;
+    bool fallback = !em_tex && !op_tex;
+    int res[2] = {1, 1};
+    const Field *ref = nullptr;
+    for (const Field *f : {em_tex ? &tm->f[8] : nullptr, op_tex ? &tm->opacity : nullptr}) {
+        if (!f) continue;
+        res[0] = std::max(res[0], s.textures[f->tex].width);
+        res[1] = std::max(res[1], s.textures[f->tex].height);
+        if (!ref) ref = f;
+        else if (ref->smp.wrap_u != f->smp.wrap_u || ref->smp.wrap_v != f->smp.wrap_v || ref->flip_v != f->flip_v ||
+                 ref->uv_scale[0] != f->uv_scale[0] || ref->uv_scale[1] != f->uv_scale[1] ||
+                 ref->uv_offset[0] != f->uv_offset[0] || ref->uv_offset[1] != f->uv_offset[1])
+            fallback = true;
+    }
+    std::vector<float> iuv((size_t)res[0] * res[1]);
+    float total = 0.0f;
+    const float zero[4] = {0, 0, 0, 0};
+    for (int y = 0; y < res[1]; ++y)
+        for (int x = 0; x < res[0]; ++x) {
+            float u = (x + 0.5f) / (float)res[0], v = (y + 0.5f) / (float)res[1];
+            V3 e = Le;
+            if (tm) {
+                float c[4];
+                tm->f[8].eval(s.textures, u, v, zero, 3, c);
+                e = V3(c[0], c[1], c[2]);
+            }
+            float val = luminance(e);
+            if (op) {
+                float o[4];
+                tm->opacity.eval(s.textures, u, v, zero, 1, o);
+                val *= o[0];
+            }
+            iuv[(size_t)y * res[0] + x] = val;
+            total += val;
+        }
+    float avg = total / (float)iuv.size();
+    for (uint32_t i = 0; i < t1 - t0; ++i) {
+        uint32_t t = t0 + i;
+        area[i] = 0.5f * std::sqrt(sqnorm(cross(s.vtx(t, 1) - s.vtx(t, 0), s.vtx(t, 2) - s.vtx(t, 1))));
+        if (fallback) imp[i] = kTwoPi * area[i] * avg;
+    }
+    if (fallback) return;
+    SAT sat(iuv, res[0], res[1]);
+    const float before_one = std::nextafter(1.0f, 0.0f);
+    for (uint32_t i = 0; i < t1 - t0; ++i) {
+        uint32_t t = t0 + i;
+        float lo[2] = {kInf, kInf}, hi[2] = {-kInf, -kInf};
+        for (int k = 0; k < 3; ++k) {
+            float tc[2] = {0.0f, 0.0f};
+            if (s.has_uv(t)) tc[0] = s.uvs[2 * s.idx[3 * t + k]], tc[1] = s.uvs[2 * s.idx[3 * t + k] + 1];
+            if (ref->flip_v) tc[1] = 1.0f - tc[1];
+            for (int a = 0; a < 2; ++a) {
+                float c = tc[a] * ref->uv_scale[a] + ref->uv_offset[a];
+                lo[a] = std::min(lo[a], c);
+                hi[a] = std::max(hi[a], c);
+            }
+        }
+        float value = 0.0f, covered = 0.0f;
+        for (int ty = (int)std::floor(lo[1]); ty < (int)std::ceil(hi[1]); ++ty)
+            for (int tx = (int)std::floor(lo[0]); tx < (int)std::ceil(hi[0]); ++tx) {
+                float blo[2] = {std::max(lo[0], (float)tx), std::max(lo[1], (float)ty)};
+                float bhi[2] = {std::min(hi[0], tx + before_one), std::min(hi[1], ty + before_one)};
+                if (blo[0] > bhi[0] || blo[1] > bhi[1]) continue; // empty intersection: zero area
+                float b_area = (bhi[0] - blo[0]) * (bhi[1] - blo[1]);
+                if (b_area == 0.0f) continue;
+                for (int a = 0; a < 2; ++a) {
+                    int mode = a == 0 ? ref->smp.wrap_u : ref->smp.wrap_v;
+                    if (mode == 0) {
+                        auto m1 = [](float x) { float r = x - std::floor(x / 1.0f) * 1.0f; return r < 0 ? r + 1.0f : r; };
+                        blo[a] = m1(blo[a]);
+                        bhi[a] = m1(bhi[a]);
+                    } else {
+                        blo[a] = clampv(blo[a], 0.0f, 1.0f);
+                        bhi[a] = clampv(bhi[a], 0.0f, 1.0f);
+                    }
+                    if (blo[a] == 0.0f && bhi[a] == 0.0f) bhi[a] = 1.0f / (res[a] + 1);
+                    if (blo[a] == 1.0f && bhi[a] == 1.0f) blo[a] = 1.0f - 1.0f / (res[a] + 1);
+                }
+                bool empty = blo[0] > bhi[0] || blo[1] > bhi[1];
+                float ex = empty ? 0.0f : bhi[0] - blo[0], ey = empty ? 0.0f : bhi[1] - blo[1];
+                float mean = sat.sum(blo, bhi) / ((ex * (float)res[0]) * (ey * (float)res[1]));
+                value += mean * b_area;
+                covered += b_area;
+            }
+        if (covered > 0.0f) value /= covered;
+        imp[i] = value * (kTwoPi * area[i]);
+    }
+}
+
 Scene *create(const ora_scene_desc *d) {
     auto s = std::make_unique<Scene>();
     s->verts.resize(d->n_vertices);
@@ -1284,8 +1801,50 @@ Scene *create(const ora_scene_desc *d) {
     s->geom_flags.assign(d->n_geoms, 0);
     if (d->geom_flags) s->geom_flags.assign(d->geom_flags, d->geom_flags + d->n_geoms);
     s->mats.assign(d->materials, d->materials + d->n_materials);
+    for (uint32_t i = 0; i < d->n_textures; ++i) {
+        Texture t;
+        if (!t.init(d->textures[i])) return nullptr;
+        s->textures.push_back(std::move(t));
+    }
+    if (d->texcoords) s->uvs.assign(d->texcoords, d->texcoords + 2 * (size_t)d->n_vertices);
+    if (d->material_fields) {
+        s->tmat_of.assign(d->n_materials, -1);
+        for (uint32_t m = 0; m < d->n_materials; ++m) {
+            const ora_material_fields *f = d->material_fields[m];
+            if (!f) continue;
+            TexMaterial tm;
+            const ora_field *fs[9] = {&f->basecolor, &f->roughness, &f->metallic, &f->ior, &f->specular_r0_mul,
+                                      &f->specular_trans, &f->diffuse_trans, &f->diffuse_trans_fwd, &f->emission};
+            for (int k = 0; k < 9; ++k)
+                if (!tm.f[k].init(*fs[k], (k == 0 || k == 8) ? 3 : 1, s->textures)) return nullptr;
+            tm.has_emission = f->has_emission != 0;
+            tm.has_normal = f->has_normal_map != 0;
+            if (tm.has_normal) {
+                if (!tm.normal.init(f->normal_map, 3, s->textures)) return nullptr;
+                tm.normal_world = f->normal_space == 1;
+                tm.normal_range01 = f->normal_range01 != 0;
+                tm.normal_strength = f->normal_strength;
+                std::memcpy(tm.normal_m, f->normal_to_world, sizeof(tm.normal_m));
+            }
+            tm.has_opacity = f->has_opacity_map != 0;
+            if (tm.has_opacity && !tm.opacity.init(f->opacity, 1, s->textures)) return nullptr;
+            // the constant record of the material: emission flag and constant emission for the light table
+            ora_material &om = s->mats[m];
+            om = ora_material{};
+            om.has_emission = tm.has_emission;
+            for (int k = 0; k < 3; ++k) om.emission[k] = f->emission.value[k];
+            s->tmat_of[m] = (int)s->tmats.size();
+            s->tmats.push_back(tm);
+        }
+    }
 
     uint32_t nt = d->n_tris;
+    bool any_alpha = false;
+    for (const TexMaterial &tm : s->tmats) any_alpha |= tm.has_opacity;
+    if (any_alpha) {
+        s->tri_alpha.assign(nt, 0);
+        for (uint32_t t = 0; t < nt; ++t) s->tri_alpha[t] = s->tmat(t) && s->tmat(t)->has_opacity;
+    }
     std::vector<AABB> tb(nt);
     std::vector<V3> tc(nt);
     std::vector<uint32_t> refs(nt);
@@ -1341,20 +1900,12 @@ Scene *create(const ora_scene_desc *d) {
         const ora_material &m = s->mats[s->geom_mat[s->tri_geom[t0]]];
         if (m.has_emission) {
             V3 Le(m.emission[0], m.emission[1], m.emission[2]);
-            float lum = luminance(Le);
-            // compute_mesh_light_importance uniform branch (light.cpp:387-409) + swap compaction (:536-545)
+            // compute_mesh_light_importance (light.cpp:307-523) + swap compaction (:536-545)
             uint32_t N = t1 - t0;
             std::vector<uint32_t> ids(N);
             std::vector<float> imp(N), area(N);
-            for (uint32_t i = 0; i < N; ++i) {
-                uint32_t t = t0 + i;
-                V3 e01 = s->vtx(t, 1) - s->vtx(t, 0);
-                V3 e12 = s->vtx(t, 2) - s->vtx(t, 1);
-                float a = 0.5f * std::sqrt(sqnorm(cross(e01, e12)));
-                ids[i] = i;
-                imp[i] = kTwoPi * a * lum;
-                area[i] = a;
-            }
+            mesh_importance(*s, t0, t1, Le, s->tmat(t0), imp, area);
+            for (uint32_t i = 0; i < N; ++i) ids[i] = i;
             for (uint32_t i = 0; i < N;) {
                 if (imp[i] == 0.0f) {
                     std::swap(ids[N - 1], ids[i]);
@@ -1369,6 +1920,7 @@ Scene *create(const ora_scene_desc *d) {
                 Light l;
                 l.type = 3;
                 l.color = Le;
+                l.tex_emission = s->tmat(t0) && s->tmat(t0)->f[8].tex >= 0;
                 l.prim = t0 + ids[i];
                 l.area = area[i];
                 s->tri_light[l.prim] = (int32_t)s->lights.size();
@@ -1389,6 +1941,9 @@ struct SurfHit {
     V3 st, sb, sn;    // shading frame
     float thit;
     uint32_t prim;
+    float u = 0, v = 0;             // Intersection::uv
+    V3 dpdu, dpdv;
+    float duv[4] = {0, 0, 0, 0};    // dudx, dvdx, dudy, dvdy
 };
 
 SurfHit make_hit(const Scene &s, V3 ray_d, const Hit &h) {
@@ -1402,7 +1957,21 @@ SurfHit make_hit(const Scene &s, V3 ray_d, const Hit &h) {
     V3 ng = normalized(cross(v1 - v0, v2 - v0)); // Embree Ng = cross(e2, e1), e1 = v0 - v1, e2 = v2 - v0
     float w = 1.0f - h.u - h.v;
     it.p = v0 * w + v1 * h.u + v2 * h.v;
-    V3 dpdu = v1 - v0;
+    V3 dpdu = v1 - v0, dpdv = v2 - v0; // rtcInterpolate1 derivatives
+    it.u = h.u;
+    it.v = h.v;
+    if (s.has_uv(tr)) { // geometry.cpp:94-116: uv from texcoords, dpdu / dpdv from the uv parameterisation
+        s.texcoord(tr, h.u, h.v, it.u, it.v);
+        const float *t0 = &s.uvs[2 * s.idx[3 * tr]], *t1 = &s.uvs[2 * s.idx[3 * tr + 1]], *t2 = &s.uvs[2 * s.idx[3 * tr + 2]];
+        float duv02[2] = {t0[0] - t2[0], t0[1] - t2[1]}, duv12[2] = {t1[0] - t2[0], t1[1] - t2[1]};
+        V3 dp02 = v0 - v2, dp12 = v1 - v2;
+        float det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+        if (!(std::fabs(det) < 1e-8f)) {
+            float inv = 1.0f / det;
+            dpdu = (dp02 * duv12[1] - dp12 * duv02[1]) * inv;
+            dpdv = (dp02 * -duv12[0] + dp12 * duv02[0]) * inv;
+        }
+    }
     V3 b = normalized(cross(ng, dpdu));
     V3 t = normalized(cross(b, ng));
     const uint8_t flags = s.geom_flags[s.tri_geom[tr]];
@@ -1429,7 +1998,79 @@ SurfHit make_hit(const Scene &s, V3 ray_d, const Hit &h) {
             it.sn = vn;
         }
     }
+    it.dpdu = dpdu;
+    it.dpdv = dpdv;
     return it;
+}
+
+float comp3(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// Intersection::compute_uv_partials (ray.cpp:6-46) for a ray with differentials from origin o.
+void compute_uv_partials(SurfHit &it, V3 o, V3 rxd, V3 ryd) {
+    auto solve = [](float a00, float a01, float a10, float a11, float b0, float b1, float &x0, float &x1) {
+        float det = a00 * a11 - a01 * a10; // solve_linear_system_2x2 (maths.h:559-569)
+        if (std::fabs(det) < 1e-10f) return false;
+        x0 = (a11 * b0 - a01 * b1) / det;
+        x1 = (a00 * b1 - a10 * b0) / det;
+        return !(std::isnan(x0) || std::isnan(x1));
+    };
+    float d = dot(it.n, it.p);
+    float tx = -(dot(it.n, o) - d) / dot(it.n, rxd);
+    float ty = -(dot(it.n, o) - d) / dot(it.n, ryd);
+    for (float &x : it.duv) x = 0.0f;
+    if (std::isinf(tx) || std::isnan(tx) || std::isinf(ty) || std::isnan(ty)) return;
+    V3 px = o + rxd * tx, py = o + ryd * ty;
+    int d0 = 0, d1 = 1;
+    if (std::fabs(it.n.x) > std::fabs(it.n.y) && std::fabs(it.n.x) > std::fabs(it.n.z)) d0 = 1, d1 = 2;
+    else if (std::fabs(it.n.y) > std::fabs(it.n.z)) d0 = 0, d1 = 2;
+    float a00 = comp3(it.dpdu, d0), a01 = comp3(it.dpdv, d0), a10 = comp3(it.dpdu, d1), a11 = comp3(it.dpdv, d1);
+    if (!solve(a00, a01, a10, a11, comp3(px, d0) - comp3(it.p, d0), comp3(px, d1) - comp3(it.p, d1), it.duv[0], it.duv[1]))
+        it.duv[0] = it.duv[1] = 0;
+    if (!solve(a00, a01, a10, a11, comp3(py, d0) - comp3(it.p, d0), comp3(py, d1) - comp3(it.p, d1), it.duv[2], it.duv[3]))
+        it.duv[2] = it.duv[3] = 0;
+}
+
+// NormalMap::sample / apply (normal_map.cpp:8-40)
+void apply_normal_map(const Scene &s, const TexMaterial &m, SurfHit &it) {
+    const float zero[4] = {0, 0, 0, 0};
+    float c[4];
+    m.normal.eval(s.textures, it.u, it.v, zero, 3, c);
+    V3 n(c[0], c[1], c[2]);
+    if (m.normal_range01) n = V3(n.x * 2.0f - 1.0f, n.y * 2.0f - 1.0f, n.z * 2.0f - 1.0f);
+    n = normalized(n);
+    if (!m.normal_world && m.normal_strength < 1.0f) // lerp((0, 0, 1), n, strength)
+        n = normalized(V3(0, 0, 1) * (1.0f - m.normal_strength) + n * m.normal_strength);
+    V3 np;
+    if (m.normal_world) { // Transform::normal: inverse transpose of the map's to_world
+        float inv[9];
+        SkyMapLight::invert3(m.normal_m, inv);
+        np = normalized(V3(inv[0] * n.x + inv[3] * n.y + inv[6] * n.z, inv[1] * n.x + inv[4] * n.y + inv[7] * n.z,
+                           inv[2] * n.x + inv[5] * n.y + inv[8] * n.z));
+    } else {
+        np = it.st * n.x + it.sb * n.y + it.sn * n.z;
+    }
+    float bump = -n.x / n.z;
+    V3 tp = normalized(it.st + it.sn * bump);
+    V3 bp = normalized(cross(np, tp));
+    tp = normalized(cross(bp, np));
+    it.st = tp;
+    it.sb = bp;
+    it.sn = np;
+}
+
+// PrincipledBSDF::eval_closure (principled_bsdf.cpp:246-275) over shader fields
+Closure fields_closure(const Scene &s, const TexMaterial &m, const SurfHit &it) {
+    ora_material c{};
+    float a[4];
+    m.f[0].eval(s.textures, it.u, it.v, it.duv, 3, a);
+    c.basecolor[0] = a[0], c.basecolor[1] = a[1], c.basecolor[2] = a[2];
+    float *dst[7] = {&c.roughness, &c.metallic, &c.ior, &c.specular_r0_mul, &c.specular_trans, &c.diffuse_trans,
+                     &c.diffuse_trans_fwd};
+    for (int k = 0; k < 7; ++k) {
+        m.f[1 + k].eval(s.textures, it.u, it.v, it.duv, 1, a);
+        *dst[k] = a[0];
+    }
+    return make_closure(c);
 }
 
 inline V3 to_local(V3 t, V3 b, V3 n, V3 w) { return V3(dot(t, w), dot(b, w), dot(n, w)); }
@@ -1550,6 +2191,13 @@ V3 light_sample(const Scene &s, const Light &l, V3 p, float u0, float u1, V3 &wi
     }
     pdf *= (d2 / absdot);
     wi_dist = std::max(0.0f, wi_dist - 1e-4f);
+    if (l.tex_emission) { // eval(it) with it.uv = interpolate_texcoord(prim, bary) (light.cpp:576,603-604)
+        float tu, tv, e[4];
+        s.texcoord(l.prim, b0, b1, tu, tv);
+        const float zero[4] = {0, 0, 0, 0};
+        s.tmat(l.prim)->f[8].eval(s.textures, tu, tv, zero, 3, e);
+        return V3(e[0], e[1], e[2]) / pdf;
+    }
     return l.color / pdf;
 }
 
@@ -1572,7 +2220,7 @@ struct PathOut {
 // SmallPT::trace (small_pt.cpp:72-178) with Material::sample_with_nee (material.cpp:101-167) and
 // next_event_estimate (nee.cpp:11-58) inlined. mis_mode 1 adds the light selection probability to the
 // NEE weight so the two strategies' weights sum to one.
-PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob) {
+PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob, V3 rxd = V3(), V3 ryd = V3()) {
     V3 L, beta(1, 1, 1);
     float pdf_wi_bsdf = 0.0f;
     for (int bounce = 0; bounce < p.bounces + 1; ++bounce) {
@@ -1605,6 +2253,9 @@ PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob) {
         }
         const ora_material &m = s.mats[s.geom_mat[s.tri_geom[h.prim]]];
         SurfHit it = make_hit(s, rd, h);
+        const TexMaterial *tm = s.tmat((uint32_t)h.prim);
+        if (bounce == 0 && !is_zero(rxd)) compute_uv_partials(it, ro, rxd, ryd); // camera rays only
+        if (tm && tm->has_normal) apply_normal_map(s, *tm, it); // Scene::intersect1 (scene.cpp:228-233)
         if (m.has_emission) {
             int32_t li = s.tri_light[h.prim];
             if (li >= 0) {
@@ -1612,6 +2263,11 @@ PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob) {
                 float pr = light_probability(s, (uint32_t)li);
                 V3 wi = normalized(rd);
                 V3 Le = l.color;
+                if (l.tex_emission) { // MeshTriLight::eval (light.cpp:560-565) at the hit
+                    float e[4];
+                    tm->f[8].eval(s.textures, it.u, it.v, it.duv, 3, e);
+                    Le = V3(e[0], e[1], e[2]);
+                }
                 if (bounce == 0) {
                     L = L + beta * Le;
                 } else if (pr > 0.0f) {
@@ -1626,7 +2282,7 @@ PathOut trace(const Scene &s, const ora_params &p, V3 ro, V3 rd, ZSobol &sob) {
         }
         if (bounce == p.bounces) break;
         V3 wo = -normalized(rd);
-        Closure c = make_closure(m);
+        Closure c = tm ? fields_closure(s, *tm, it) : make_closure(m);
 
         // NEE (nee.cpp:11-58)
         V3 Ld;
@@ -1720,7 +2376,8 @@ int resolve_threads(int t) {
     return h ? (int)h : 1;
 }
 
-void camera_sample(const Camera &cam, const ora_params &p, int px, int py, ZSobol &sob, V3 &o, V3 &d) {
+void camera_sample(const Camera &cam, const ora_params &p, int px, int py, ZSobol &sob, V3 &o, V3 &d, V3 *rxd = nullptr,
+                   V3 *ryd = nullptr) {
     // small_pt.cpp:98-105 with BoxPixelFilter (render_target.h:108-125)
     float ox = 0.0f, oy = 0.0f;
     if (p.spp != 1) {
@@ -1731,7 +2388,7 @@ void camera_sample(const Camera &cam, const ora_params &p, int px, int py, ZSobo
     }
     float fx = ((float)px + 0.5f + ox) / (float)p.render_width;
     float fy = ((float)py + 0.5f + oy) / (float)p.render_height;
-    cam.spawn_ray(fx, fy, o, d);
+    cam.spawn_ray(fx, fy, o, d, rxd, ryd, p.render_width, p.render_height, p.scale_ray_diff ? p.spp : 1);
 }
 
 } // namespace
@@ -1820,6 +2477,31 @@ int ora_sky_sample(void *scene, const float *u, int64_t n, float *wi, float *le_
     return 0;
 }
 
+int ora_field_eval(void *scene, const ora_field *f, const float *uvd, int64_t n, float *out) {
+    const Scene &s = *(const Scene *)scene;
+    Field fd;
+    if (!fd.init(*f, 4, s.textures)) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        const float *r = uvd + 6 * i;
+        fd.eval(s.textures, r[0], r[1], r + 2, 4, out + 4 * i);
+    }
+    return 0;
+}
+
+int ora_texture_level(void *scene, int tex, int level, int *w, int *h, float *out) {
+    const Scene &s = *(const Scene *)scene;
+    if (tex < 0 || tex >= (int)s.textures.size()) return -1;
+    const Texture &t = s.textures[tex];
+    if (level == -1) return t.nc;
+    if (level < 0 || level >= t.levels()) return -1;
+    *w = t.mips[level].w;
+    *h = t.mips[level].h;
+    if (out)
+        for (int y = 0; y < *h; ++y)
+            for (int x = 0; x < *w; ++x) t.fetch(x, y, level, out + ((size_t)y * *w + x) * t.nc);
+    return 0;
+}
+
 int ora_light_table(void *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf) {
     const Scene &s = *(Scene *)scene;
     int32_t n = (int32_t)s.lights.size();
@@ -1858,9 +2540,9 @@ int ora_render(void *scene, const ora_camera *cam_desc, const ora_params *p, dou
                 size_t pi = (size_t)py * W + px;
                 for (int smp = 0; smp < p->spp; ++smp) {
                     ZSobol sob(p->spp, W, H, p->rng_seed, px, py, smp);
-                    V3 o, d;
-                    camera_sample(cam, *p, px, py, sob, o, d);
-                    PathOut r = trace(s, *p, o, d, sob);
+                    V3 o, d, rxd, ryd;
+                    camera_sample(cam, *p, px, py, sob, o, d, &rxd, &ryd);
+                    PathOut r = trace(s, *p, o, d, sob, rxd, ryd);
                     if (r.hit) { // RenderTarget2::add / add_miss (render_target.cpp:82-107)
                         weights[2 * pi] += 1.0;
                         rgb[3 * pi] += (double)r.L.x;
@@ -1881,9 +2563,9 @@ int ora_trace_samples(void *scene, const ora_camera *cam_desc, const ora_params 
     Camera cam(*cam_desc);
     parallel_for(n, resolve_threads(threads), [&](int64_t i) {
         ZSobol sob(p->spp, p->render_width, p->render_height, p->rng_seed, px[i], py[i], sample[i]);
-        V3 o, d;
-        camera_sample(cam, *p, px[i], py[i], sob, o, d);
-        PathOut r = trace(s, *p, o, d, sob);
+        V3 o, d, rxd, ryd;
+        camera_sample(cam, *p, px[i], py[i], sob, o, d, &rxd, &ryd);
+        PathOut r = trace(s, *p, o, d, sob, rxd, ryd);
         out[4 * i] = r.L.x;
         out[4 * i + 1] = r.L.y;
         out[4 * i + 2] = r.L.z;

@@ -65,6 +65,39 @@ typedef struct ora_sky_map {
     float strength;
 } ora_sky_map;
 
+/* Texture (texture.h:46-66): levels back to back (each max(1, (n + 1) / 2) of the one before), texels of
+ * `channels` values of type 0 u8 / 1 u16 / 2 f32; color_space 0 linear, 1 sRGB; with levels == 1 and
+ * build_mipmaps the chain is built as Texture(bytes, ..., true) does. */
+typedef struct ora_texture {
+    const void *data;
+    int32_t width, height, channels, data_type, color_space, levels, build_mipmaps;
+} ora_texture;
+
+/* ShaderField (shader_field.h:11-86): texture -1 is the constant `value`; sampler 0 nearest, 1 linear,
+ * 2 cubic (kernel 0 Mitchell-Netravali, 1 B-spline, 2 Catmull-Rom), 3 EWA; wrap 0 repeat, 1 clamp. */
+typedef struct ora_field {
+    int32_t texture, sampler, cubic_kernel;
+    float anisotropy;
+    int32_t wrap_u, wrap_v, flip_v;
+    int32_t swizzle[4];
+    float uv_scale[2], uv_offset[2], scale[4], value[4];
+} ora_field;
+
+/* PrincipledBSDF fields + Material::emission / normal_map / opacity_map (material.h:39-43). */
+typedef struct ora_material_fields {
+    ora_field basecolor, roughness, metallic, ior, specular_r0_mul, specular_trans, diffuse_trans,
+        diffuse_trans_fwd;
+    int32_t has_emission;
+    ora_field emission;
+    int32_t has_normal_map;
+    ora_field normal_map;
+    int32_t normal_space, normal_range01; /* space 0 tangent, 1 world */
+    float normal_strength;
+    float normal_to_world[9];
+    int32_t has_opacity_map;
+    ora_field opacity;
+} ora_material_fields;
+
 typedef struct ora_scene_desc {
     const float *vertices; /* 3 * n_vertices */
     uint32_t n_vertices;
@@ -80,6 +113,10 @@ typedef struct ora_scene_desc {
     uint32_t n_lights;
     const uint8_t *geom_flags; /* per geometry: 1 one-sided (MeshData::twosided = false), 2 flat normals; or NULL */
     const ora_sky_map *sky_map; /* texels of the type-3 light, or NULL */
+    const float *texcoords;     /* 2 per vertex, or NULL */
+    const ora_texture *textures;
+    uint32_t n_textures;
+    const ora_material_fields *const *material_fields; /* per material, NULL entries: constant material */
 } ora_scene_desc;
 
 /* create_camera (camera.cpp:172-205): look_at(pos, target, up) + perspective vfov (degrees) / aspect. */
@@ -102,6 +139,7 @@ typedef struct ora_params {
     float backdrop[3];
     float clamp_direct, clamp_indirect;
     int32_t mis_mode;
+    int32_t scale_ray_diff; /* camera ray differentials scaled by 1/sqrt(spp) (camera.cpp:162-167) */
 } ora_params;
 
 void *ora_scene_create(const ora_scene_desc *desc);
@@ -116,6 +154,11 @@ int ora_light_table(void *scene, int32_t *n_lights, int32_t *prim, float *power,
 int ora_sky_eval(void *scene, const float *dirs, int64_t n, float *le, float *pdf);
 /* Light samples from u (n x 2): direction (n x 3), Le / pdf (n x 3), pdf (n). */
 int ora_sky_sample(void *scene, const float *u, int64_t n, float *wi, float *le_over_pdf, float *pdf);
+/* ShaderField evaluation of texture-backed field `f` of the scene at n (u, v, dudx, dvdx, dudy, dvdy)
+ * rows: out n x 4. */
+int ora_field_eval(void *scene, const ora_field *f, const float *uvd, int64_t n, float *out);
+/* Level `level` of texture `tex` as float texels (Texture::fetch_as_float): out w * h * channels. */
+int ora_texture_level(void *scene, int tex, int level, int *w, int *h, float *out);
 /* rgb: W*H*3 FP64 sums, weights: W*H*2 FP64 (hit, miss) -- RenderTarget2 (render_target.h:224-293). */
 int ora_render(void *scene, const ora_camera *cam, const ora_params *p, double *rgb, double *weights,
                int threads);

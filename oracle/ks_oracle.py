@@ -55,7 +55,36 @@ class OraSceneDesc(C.Structure):
         ("n_lights", C.c_uint32),
         ("geom_flags", C.c_void_p),
         ("sky_map", C.c_void_p),
+        ("texcoords", C.c_void_p),
+        ("textures", C.c_void_p),
+        ("n_textures", C.c_uint32),
+        ("material_fields", C.c_void_p),
     ]
+
+
+class OraTexture(C.Structure):
+    _fields_ = [("data", C.c_void_p)] + [(n, C.c_int32) for n in ("width", "height", "channels", "data_type",
+                                                                 "color_space", "levels", "build_mipmaps")]
+
+
+class OraField(C.Structure):
+    _fields_ = [
+        ("texture", C.c_int32),
+        ("sampler", C.c_int32),
+        ("cubic_kernel", C.c_int32),
+        ("anisotropy", C.c_float),
+        ("wrap_u", C.c_int32),
+        ("wrap_v", C.c_int32),
+        ("flip_v", C.c_int32),
+        ("swizzle", C.c_int32 * 4),
+        ("uv_scale", C.c_float * 2),
+        ("uv_offset", C.c_float * 2),
+        ("scale", C.c_float * 4),
+        ("value", C.c_float * 4),
+    ]
+
+
+ORA_MATERIAL_FIELDS_SIZE = 8 * C.sizeof(OraField) + 4 + C.sizeof(OraField) + 4 + C.sizeof(OraField) + 12 + 36 + 4 + C.sizeof(OraField)
 
 
 class OraSkyMap(C.Structure):
@@ -95,6 +124,7 @@ class OraParams(C.Structure):
         ("clamp_direct", C.c_float),
         ("clamp_indirect", C.c_float),
         ("mis_mode", C.c_int32),
+        ("scale_ray_diff", C.c_int32),
     ]
 
 
@@ -108,6 +138,8 @@ _SIGS = {
     "ora_light_table": ([_P, C.POINTER(C.c_int32), _P, _P, _P], C.c_int),
     "ora_sky_eval": ([_P, _P, C.c_int64, _P, _P], C.c_int),
     "ora_sky_sample": ([_P, _P, C.c_int64, _P, _P, _P], C.c_int),
+    "ora_field_eval": ([_P, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_texture_level": ([_P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _P], C.c_int),
     "ora_render": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, C.c_int], C.c_int),
     "ora_trace_samples": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, _P, C.c_int64, _P, C.c_int], C.c_int),
     "ora_sobol": ([C.c_int] * 8 + [_P, _P, _P], C.c_int),
@@ -176,6 +208,7 @@ def params(**kw) -> OraParams:
     p.clamp_direct = kw.get("clamp_direct", 0.0)
     p.clamp_indirect = kw.get("clamp_indirect", 10.0)
     p.mis_mode = kw.get("mis_mode", 0)
+    p.scale_ray_diff = int(kw.get("scale_ray_diff", True))
     return p
 
 
@@ -189,8 +222,11 @@ def material(rec) -> OraMaterial:
 
 class OracleScene:
     def __init__(self, vertices, indices, tri_geom, geom_material, materials, lights, normals=None, geom_flags=None,
-                 sky_map=None):
-        """sky_map: dict(rgb=(H, W, 3), to_world=3x3, y_up=bool, strength=float) for a type-3 light."""
+                 sky_map=None, texcoords=None, textures=None, material_fields=None):
+        """sky_map: dict(rgb=(H, W, 3), to_world=3x3, y_up=bool, strength=float) for a type-3 light.
+        texcoords: (V, 2); textures: dicts with "data" (an (H, W, C) uint8 / uint16 / float32 array, or a
+        list of them, one per level), "srgb", "mipmaps"; material_fields: material index -> a ctypes
+        structure with the ora_material_fields layout."""
         lib = load()
         self.lib = lib
         self._arrs = [
@@ -230,6 +266,36 @@ class OracleScene:
             sm.strength = float(sky_map.get("strength", 1.0))
             self._arrs += [rgb, sm]
             d.sky_map = C.addressof(sm)
+        if texcoords is not None:
+            tc = np.ascontiguousarray(texcoords, np.float32)
+            self._arrs.append(tc)
+            d.texcoords = tc.ctypes.data
+        if textures:
+            types = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}
+            arr = (OraTexture * len(textures))()
+            for i, t in enumerate(textures):
+                levels = [np.asarray(x) for x in (t["data"] if isinstance(t["data"], (list, tuple)) else [t["data"]])]
+                lv0 = levels[0] if levels[0].ndim == 3 else levels[0][..., None]
+                blob = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(x).reshape(-1) for x in levels]))
+                self._arrs.append(blob)
+                arr[i].data = blob.ctypes.data
+                arr[i].height, arr[i].width, arr[i].channels = lv0.shape
+                arr[i].data_type = types[lv0.dtype]
+                arr[i].color_space = 1 if t.get("srgb", False) else 0
+                arr[i].levels = len(levels)
+                arr[i].build_mipmaps = 1 if t.get("mipmaps", True) else 0
+            self._arrs.append(arr)
+            d.textures = C.addressof(arr)
+            d.n_textures = len(textures)
+        if material_fields:
+            ptrs = (C.c_void_p * mats.shape[0])()
+            for m, mf in material_fields.items():
+                assert C.sizeof(mf) == ORA_MATERIAL_FIELDS_SIZE
+                buf = C.create_string_buffer(C.string_at(C.addressof(mf), C.sizeof(mf)), C.sizeof(mf))
+                self._arrs.append(buf)
+                ptrs[m] = C.addressof(buf)
+            self._arrs.append(ptrs)
+            d.material_fields = C.addressof(ptrs)
         self.h = lib.ora_scene_create(C.byref(d))
         if not self.h:
             raise ValueError("ora_scene_create rejected the scene")
@@ -237,7 +303,8 @@ class OracleScene:
     @classmethod
     def from_host_scene(cls, hs) -> "OracleScene":
         return cls(hs.vertices, hs.indices, hs.tri_geom, hs.geom_material, hs.materials, hs.lights, hs.normals,
-                   getattr(hs, "geom_flags", None), getattr(hs, "sky_map", None))
+                   getattr(hs, "geom_flags", None), getattr(hs, "sky_map", None), getattr(hs, "texcoords", None),
+                   getattr(hs, "textures", None), getattr(hs, "material_fields", None))
 
     def close(self):
         if self.h:
@@ -288,6 +355,24 @@ class OracleScene:
         pdf = np.zeros(u.shape[0], np.float32)
         assert self.lib.ora_sky_sample(self.h, _p(u), u.shape[0], _p(wi), _p(b), _p(pdf)) == 0
         return wi, b, pdf
+
+    def field_eval(self, field, uvd):
+        """ShaderField value (n, 4) of `field` (a ctypes structure with the ora_field layout) at rows
+        (u, v, dudx, dvdx, dudy, dvdy)."""
+        uvd = np.ascontiguousarray(uvd, np.float32)
+        out = np.zeros((uvd.shape[0], 4), np.float32)
+        f = OraField.from_buffer_copy(C.string_at(C.addressof(field), C.sizeof(OraField)))
+        assert self.lib.ora_field_eval(self.h, C.byref(f), _p(uvd), uvd.shape[0], _p(out)) == 0
+        return out
+
+    def texture_level(self, tex, level):
+        w, h = C.c_int(0), C.c_int(0)
+        assert self.lib.ora_texture_level(self.h, tex, level, C.byref(w), C.byref(h), None) == 0
+        nc = self.lib.ora_texture_level(self.h, tex, -1, C.byref(w), C.byref(h), None)
+        self.lib.ora_texture_level(self.h, tex, level, C.byref(w), C.byref(h), None)
+        out = np.zeros((h.value, w.value, nc), np.float32)
+        assert self.lib.ora_texture_level(self.h, tex, level, C.byref(w), C.byref(h), _p(out)) == 0
+        return out
 
     def render(self, cam: dict, prm: OraParams, threads=None):
         W, H = prm.render_width, prm.render_height

@@ -7,6 +7,7 @@ oracle's and the GPU library's copies against them:
   - the 24 digit permutations       (sobol.h:171-197)
   - FastOwenScrambler constants     (sobol.h:17-33)
   - mix_bits / murmur constants     (hash.h:140-214)
+  - the sRGB u8 -> linear table     (texture.cpp:376-401) and the EWA weight table (texture.cpp:720-856)
 Run from the repository root: python tests/golden/make_reference_tables.py [/root/reference]
 """
 
@@ -47,8 +48,18 @@ def main(ref: Path) -> None:
     murmur_m = int(re.search(r"const uint64_t m = (0x[0-9a-fA-F]+)", mur).group(1), 16)
     murmur_r = int(re.search(r"const int r = (\d+);", mur).group(1))
 
+    tex_cpp = (ref / "texture.cpp").read_text()
+    lut = tex_cpp[tex_cpp.index("srgb_u8_to_linear_f32_lut[256] = {") :]
+    lut = lut[lut.index("{") + 1 : lut.index("};")]
+    srgb_lut = [float(x) for x in re.findall(r"[0-9]+\.[0-9]+", lut)]
+    assert len(srgb_lut) == 256
+    ewa = tex_cpp[tex_cpp.index("ewa_lut[ewa_lut_size] = {") :]
+    ewa = ewa[ewa.index("{") + 1 : ewa.index("};")]
+    ewa_lut = [float(x.rstrip("f")) for x in re.findall(r"[0-9]+\.[0-9]*(?:e-?[0-9]+)?f", ewa)]
+    assert len(ewa_lut) == 128, len(ewa_lut)
+
     data = {
-        "source": "keepsake reference sources (sobol.cpp, sobol.h, hash.h), read as text",
+        "source": "keepsake reference sources (sobol.cpp, sobol.h, hash.h, texture.cpp), read as text",
         "sobol_matrices_dim01": words,
         "perm4": perms,
         "owen_scrambler_constants": scrambler_consts,
@@ -56,6 +67,8 @@ def main(ref: Path) -> None:
         "mix_bits_shifts": mix_shifts,
         "murmur64a_m": murmur_m,
         "murmur64a_r": murmur_r,
+        "srgb_u8_to_linear": srgb_lut,
+        "ewa_weights": ewa_lut,
     }
     OUT.write_text(json.dumps(data, indent=1) + "\n")
     print(f"wrote {OUT}")
