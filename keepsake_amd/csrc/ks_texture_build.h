@@ -19,16 +19,17 @@ using ksd::DevTexture;
 using ksd::TexLuts;
 
 // sRGB u8 -> linear table (the reference's srgb_u8_to_linear_f32_lut, texture.cpp:376-401) and the EWA
-// weights exp(-2 r^2) - exp(-2) at r^2 = i / 127 (texture.cpp:720-727), both from their closed forms in
-// double, rounded to float.
+// weights exp(-2 r^2) - exp(-2) at r^2 = i / 127 (texture.cpp:720-727). Both tables are the single-
+// precision evaluations of these formulas (srgb_to_linear of i / 255 in float; the EWA generator of
+// texture.cpp:720-726 in float), which reproduce the reference's tables bit for bit.
 inline void fill_tex_luts(TexLuts &l) {
     for (int i = 0; i < 256; ++i) {
-        const double x = i / 255.0;
-        l.srgb_u8[i] = (float)(x < 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+        const float x = (float)i / 255.0f;
+        l.srgb_u8[i] = x < 0.04045f ? x / 12.92f : std::pow((x + 0.055f) / 1.055f, 2.4f);
     }
     for (int i = 0; i < ksd::kEwaLutSize; ++i) {
-        const double r2 = (double)i / (double)(ksd::kEwaLutSize - 1);
-        l.ewa[i] = (float)(std::exp(-2.0 * r2) - std::exp(-2.0));
+        const float alpha = 2.0f, r2 = (float)i / (float)(ksd::kEwaLutSize - 1);
+        l.ewa[i] = std::exp(-alpha * r2) - std::exp(-alpha);
     }
 }
 

@@ -995,14 +995,14 @@ float linear_to_srgb_f(float x) { // maths.h:649-659
 }
 
 struct TexTables {
-    float srgb[256]; // srgb_u8_to_linear_f32_lut (texture.cpp:376-401): the sRGB curve at i / 255
-    float ewa[128];  // ewa_lut (texture.cpp:720-856): exp(-2 r2) - exp(-2) at r2 = i / 127
+    float srgb[256]; // srgb_u8_to_linear_f32_lut (texture.cpp:376-401): srgb_to_linear(i / 255) in float
+    float ewa[128];  // ewa_lut (texture.cpp:720-856): its generator (texture.cpp:720-726) in float
     TexTables() {
-        for (int i = 0; i < 256; ++i) {
-            double x = (double)i / 255.0;
-            srgb[i] = (float)(x < 0.04045 ? x / 12.92 : std::pow((x + 0.055) / 1.055, 2.4));
+        for (int i = 0; i < 256; ++i) srgb[i] = srgb_to_linear_f((float)i / 255.0f);
+        for (int i = 0; i < 128; ++i) {
+            float alpha = 2, r2 = float(i) / float(128 - 1);
+            ewa[i] = std::exp(-alpha * r2) - std::exp(-alpha);
         }
-        for (int i = 0; i < 128; ++i) ewa[i] = (float)(std::exp(-2.0 * i / 127.0) - std::exp(-2.0));
     }
 };
 const TexTables g_tex_tables;
@@ -2480,10 +2480,13 @@ int ora_sky_sample(void *scene, const float *u, int64_t n, float *wi, float *le_
 int ora_field_eval(void *scene, const ora_field *f, const float *uvd, int64_t n, float *out) {
     const Scene &s = *(const Scene *)scene;
     Field fd;
-    if (!fd.init(*f, 4, s.textures)) return -1;
+    // as many components as the texture has channels (a constant field: all four)
+    int nc = f->texture >= 0 && f->texture < (int)s.textures.size() ? s.textures[f->texture].nc : 4;
+    if (!fd.init(*f, nc, s.textures)) return -1;
     for (int64_t i = 0; i < n; ++i) {
         const float *r = uvd + 6 * i;
-        fd.eval(s.textures, r[0], r[1], r + 2, 4, out + 4 * i);
+        std::fill(out + 4 * i, out + 4 * i + 4, 0.0f);
+        fd.eval(s.textures, r[0], r[1], r + 2, nc, out + 4 * i);
     }
     return 0;
 }
@@ -2499,6 +2502,43 @@ int ora_texture_level(void *scene, int tex, int level, int *w, int *h, float *ou
     if (out)
         for (int y = 0; y < *h; ++y)
             for (int x = 0; x < *w; ++x) t.fetch(x, y, level, out + ((size_t)y * *w + x) * t.nc);
+    return 0;
+}
+
+void ora_tex_tables(float *srgb256, float *ewa128) {
+    std::memcpy(srgb256, g_tex_tables.srgb, sizeof(g_tex_tables.srgb));
+    std::memcpy(ewa128, g_tex_tables.ewa, sizeof(g_tex_tables.ewa));
+}
+
+int ora_primary_surface(void *scene, const ora_camera *cam_desc, const ora_params *p, const int32_t *px,
+                        const int32_t *py, const int32_t *sample, int64_t n, float *out) {
+    const Scene &s = *(Scene *)scene;
+    Camera cam(*cam_desc);
+    for (int64_t i = 0; i < n; ++i) {
+        ZSobol sob(p->spp, p->render_width, p->render_height, p->rng_seed, px[i], py[i], sample[i]);
+        V3 o, d, rxd, ryd;
+        camera_sample(cam, *p, px[i], py[i], sob, o, d, &rxd, &ryd);
+        RayQ q;
+        q.o = o;
+        q.d = d;
+        q.tmin = 0.0f;
+        q.tmax = kInf;
+        q.setup();
+        Hit h = closest(s, q);
+        float *r = out + 16 * i;
+        std::fill(r, r + 16, 0.0f);
+        int32_t prim = h.prim;
+        if (h.prim >= 0) {
+            SurfHit it = make_hit(s, d, h);
+            compute_uv_partials(it, o, rxd, ryd);
+            const TexMaterial *tm = s.tmat((uint32_t)h.prim);
+            if (tm && tm->has_normal) apply_normal_map(s, *tm, it);
+            float v[15] = {it.u, it.v, it.duv[0], it.duv[1], it.duv[2], it.duv[3], it.sn.x, it.sn.y, it.sn.z,
+                           it.st.x, it.st.y, it.st.z, it.n.x, it.n.y, it.n.z};
+            std::memcpy(r, v, sizeof(v));
+        }
+        std::memcpy(r + 15, &prim, 4);
+    }
     return 0;
 }
 

@@ -159,6 +159,12 @@ int ora_sky_sample(void *scene, const float *u, int64_t n, float *wi, float *le_
 int ora_field_eval(void *scene, const ora_field *f, const float *uvd, int64_t n, float *out);
 /* Level `level` of texture `tex` as float texels (Texture::fetch_as_float): out w * h * channels. */
 int ora_texture_level(void *scene, int tex, int level, int *w, int *h, float *out);
+/* The oracle's sRGB u8 -> linear table (256) and EWA weights (128). */
+void ora_tex_tables(float *srgb256, float *ewa128);
+/* Surface at the camera ray's first hit for (pixel, sample): out n x 16 = (u, v, dudx, dvdx, dudy, dvdy,
+ * shading n xyz, shading t xyz, geometric n xyz, prim as int bits); prim -1 on a miss. */
+int ora_primary_surface(void *scene, const ora_camera *cam, const ora_params *p, const int32_t *px,
+                        const int32_t *py, const int32_t *sample, int64_t n, float *out);
 /* rgb: W*H*3 FP64 sums, weights: W*H*2 FP64 (hit, miss) -- RenderTarget2 (render_target.h:224-293). */
 int ora_render(void *scene, const ora_camera *cam, const ora_params *p, double *rgb, double *weights,
                int threads);

@@ -139,6 +139,8 @@ _SIGS = {
     "ora_sky_eval": ([_P, _P, C.c_int64, _P, _P], C.c_int),
     "ora_sky_sample": ([_P, _P, C.c_int64, _P, _P, _P], C.c_int),
     "ora_field_eval": ([_P, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_tex_tables": ([_P, _P], None),
+    "ora_primary_surface": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, _P, C.c_int64, _P], C.c_int),
     "ora_texture_level": ([_P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _P], C.c_int),
     "ora_render": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, C.c_int], C.c_int),
     "ora_trace_samples": ([_P, C.POINTER(OraCamera), C.POINTER(OraParams), _P, _P, _P, C.c_int64, _P, C.c_int], C.c_int),
@@ -365,6 +367,17 @@ class OracleScene:
         assert self.lib.ora_field_eval(self.h, C.byref(f), _p(uvd), uvd.shape[0], _p(out)) == 0
         return out
 
+    def primary_surface(self, cam: dict, prm: OraParams, px, py, sample):
+        """(u, v, dudx, dvdx, dudy, dvdy, shading n, shading t, geometric n, prim bits) at the first hit."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        out = np.zeros((px.shape[0], 16), np.float32)
+        c = camera(cam)
+        assert self.lib.ora_primary_surface(self.h, C.byref(c), C.byref(prm), _p(px), _p(py), _p(sample),
+                                            px.shape[0], _p(out)) == 0
+        return out
+
     def texture_level(self, tex, level):
         w, h = C.c_int(0), C.c_int(0)
         assert self.lib.ora_texture_level(self.h, tex, level, C.byref(w), C.byref(h), None) == 0
@@ -442,6 +455,13 @@ def camera_matrices(cam):
     c = camera(cam)
     load().ora_camera_matrices(C.byref(c), _p(p2w), _p(pos))
     return p2w.reshape(4, 4), pos
+
+
+def tex_tables():
+    srgb = np.zeros(256, np.float32)
+    ewa = np.zeros(128, np.float32)
+    load().ora_tex_tables(_p(srgb), _p(ewa))
+    return srgb, ewa
 
 
 def bsdf_eval(mat, wo, wi, variant: str = ""):
