@@ -57,6 +57,13 @@ struct SceneData {
     std::array<float, 9> sky_to_world{1, 0, 0, 0, 1, 0, 0, 0, 1}; // row-major
     bool sky_y_up = true;
     float sky_strength = 1.0f;
+    // textures: per-vertex texture coordinates, texture levels and descriptors (data pointers refreshed
+    // by desc()), and textured materials (field_of[m] indexes `fields`, -1: the constant material)
+    std::vector<float> texcoords;
+    std::vector<std::vector<uint8_t>> texture_bytes;
+    std::vector<ks_texture_desc> textures;
+    std::vector<ks_material_fields> fields;
+    std::vector<int32_t> field_of;
 
     ks_scene_desc desc() const {
         ks_scene_desc d{};
@@ -82,11 +89,23 @@ struct SceneData {
             sky_desc_.strength = sky_strength;
             d.sky_map = &sky_desc_;
         }
+        d.texcoords = texcoords.empty() ? nullptr : texcoords.data();
+        tex_desc_ = textures;
+        for (size_t i = 0; i < tex_desc_.size(); ++i) tex_desc_[i].data = texture_bytes[i].data();
+        d.textures = tex_desc_.empty() ? nullptr : tex_desc_.data();
+        d.n_textures = (uint32_t)tex_desc_.size();
+        field_ptrs_.assign(materials.size(), nullptr);
+        bool any = false;
+        for (size_t m = 0; m < field_of.size() && m < materials.size(); ++m)
+            if (field_of[m] >= 0) field_ptrs_[m] = &fields[(size_t)field_of[m]], any = true;
+        d.material_fields = any ? field_ptrs_.data() : nullptr;
         return d;
     }
 
   private:
     mutable ks_sky_map_desc sky_desc_{}; // points into sky_rgb; refreshed by every desc()
+    mutable std::vector<ks_texture_desc> tex_desc_;
+    mutable std::vector<const ks_material_fields *> field_ptrs_;
 };
 
 // Scene: create_scene + rtcCommitScene + build_mesh_lights + PowerLightSampler::build in one step.

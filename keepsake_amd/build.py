@@ -66,8 +66,8 @@ def build_library(verbose: bool = True) -> Path:
     return LIB
 
 
-HOST_TOOL_SOURCES = ["config.cpp", "mesh_asset.cpp", "gltf_asset.cpp", "image_io.cpp", "small_pt_task.cpp",
-                     "ks_render.cpp"]
+HOST_TOOL_SOURCES = ["config.cpp", "mesh_asset.cpp", "gltf_asset.cpp", "image_io.cpp", "texture_asset.cpp",
+                     "small_pt_task.cpp", "ks_render.cpp"]
 TOOL = LIBDIR / "ks_render"
 
 

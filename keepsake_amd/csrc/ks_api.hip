@@ -1538,7 +1538,7 @@ int build_textures(const ks_scene_desc *d, HostTextures &ht) {
     std::vector<size_t> offs;
     try {
         for (uint32_t i = 0; i < d->n_textures; ++i) {
-            ksa::SourceTexture src = ksa::load_source_texture(d->textures[i], ht.luts);
+            ksa::SourceTexture src = ksa::load_source_texture(d->textures[i], ht.luts.srgb_u8);
             DevTexture meta;
             std::vector<uint8_t> blob = ksa::pack_texture(src, meta);
             offs.push_back(ht.pool.size());

@@ -257,7 +257,9 @@ void write_exr(const fs::path &path, const float *data, int w, int h, int c, boo
     write_all(path, o.b);
 }
 
-std::vector<float> read_exr(const fs::path &path, int &w, int &h) {
+std::vector<float> read_exr(const fs::path &path, int &w, int &h) { return read_exr_channels(path, w, h, 3); }
+
+std::vector<float> read_exr_channels(const fs::path &path, int &w, int &h, int n_out) {
     std::vector<uint8_t> file = read_all(path);
     Cursor cur{file, 0, path.string()};
     if (cur.get<uint32_t>() != 20000630u) throw ConfigError(path.string() + ": not an OpenEXR file");
@@ -313,13 +315,17 @@ std::vector<float> read_exr(const fs::path &path, int &w, int &h) {
             if (chans[i].name == n) return (int)i;
         return -1;
     };
-    int src[3] = {find("R"), find("G"), find("B")};
+    // R, G, B by name (or one channel replicated, alpha included, as LoadEXR does for single-channel
+    // images); A by name, else 1
+    int src[4] = {find("R"), find("G"), find("B"), find("A")};
     if (src[0] < 0 || src[1] < 0 || src[2] < 0) {
         int y = find("Y");
         if (y < 0 && chans.size() == 1) y = 0;
         if (y < 0) throw ConfigError(path.string() + ": no R, G, B (or Y) channels");
         src[0] = src[1] = src[2] = y;
+        if (chans.size() == 1) src[3] = y;
     }
+    if (n_out < 1 || n_out > 4) throw ConfigError(path.string() + ": 1 to 4 channels can be read");
     size_t line_bytes = 0;
     std::vector<size_t> ch_off(chans.size());
     for (size_t i = 0; i < chans.size(); ++i) {
@@ -329,7 +335,9 @@ std::vector<float> read_exr(const fs::path &path, int &w, int &h) {
     const int blocks = (h + lines - 1) / lines;
     std::vector<uint64_t> offsets(blocks);
     for (int k = 0; k < blocks; ++k) offsets[k] = cur.get<uint64_t>();
-    std::vector<float> out((size_t)w * h * 3, 0.0f);
+    std::vector<float> out((size_t)w * h * n_out, 0.0f);
+    if (n_out == 4 && src[3] < 0)
+        for (size_t i = 0; i < (size_t)w * h; ++i) out[4 * i + 3] = 1.0f;
     std::vector<uint8_t> tmp, pix;
     for (int blk = 0; blk < blocks; ++blk) {
         Cursor c2{file, (size_t)offsets[blk], path.string()};
@@ -366,7 +374,8 @@ std::vector<float> read_exr(const fs::path &path, int &w, int &h) {
             exr_unpredict_reorder(tmp, pix);
         }
         for (int l = 0; l < nl; ++l)
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < n_out; ++k) {
+                if (src[k] < 0) continue;
                 const ExrChannel &ch = chans[(size_t)src[k]];
                 const uint8_t *p = pix.data() + (size_t)l * line_bytes + ch_off[(size_t)src[k]];
                 for (int x = 0; x < w; ++x) {
@@ -382,7 +391,7 @@ std::vector<float> read_exr(const fs::path &path, int &w, int &h) {
                         std::memcpy(&u, p + 4 * (size_t)x, 4);
                         v = (float)u;
                     }
-                    out[((size_t)(y0 + l) * w + x) * 3 + k] = v;
+                    out[((size_t)(y0 + l) * w + x) * n_out + k] = v;
                 }
             }
     }
@@ -682,6 +691,238 @@ void tonemap_task(const ConfigArgs &args, const fs::path &task_dir, int task_id)
         }
         write_png(args.load_path("png_path"), png.data(), w, h, 3);
     }
+}
+
+
+// ---- PNG input ----------------------------------------------------------------------------------
+
+namespace {
+
+uint32_t be32(const uint8_t *p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+int paeth(int a, int b, int c) {
+    int p = a + b - c, pa = std::abs(p - a), pb = std::abs(p - b), pc = std::abs(p - c);
+    return (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+}
+
+// Undoes the five PNG row filters of one (sub)image of w x h with `bpp` bytes per complete pixel
+// (at least 1) and `row` bytes per row; src holds h rows of 1 + row bytes.
+void unfilter(const uint8_t *src, size_t avail, int h, size_t row, int bpp, std::vector<uint8_t> &out,
+              const std::string &name) {
+    if (avail < (size_t)h * (row + 1)) throw ConfigError(name + ": truncated PNG image data");
+    out.assign((size_t)h * row, 0);
+    for (int y = 0; y < h; ++y) {
+        const uint8_t f = src[(size_t)y * (row + 1)];
+        const uint8_t *in = src + (size_t)y * (row + 1) + 1;
+        uint8_t *cur = out.data() + (size_t)y * row;
+        const uint8_t *prev = y > 0 ? cur - row : nullptr;
+        for (size_t i = 0; i < row; ++i) {
+            const int a = i >= (size_t)bpp ? cur[i - bpp] : 0, b = prev ? prev[i] : 0,
+                      c = (prev && i >= (size_t)bpp) ? prev[i - bpp] : 0;
+            int v;
+            switch (f) {
+            case 0: v = in[i]; break;
+            case 1: v = in[i] + a; break;
+            case 2: v = in[i] + b; break;
+            case 3: v = in[i] + ((a + b) >> 1); break;
+            case 4: v = in[i] + paeth(a, b, c); break;
+            default: throw ConfigError(name + ": bad PNG filter type");
+            }
+            cur[i] = (uint8_t)v;
+        }
+    }
+}
+
+} // namespace
+
+// stb_image's conversions between channel counts (stbi__convert_format): gray = (77 r + 150 g + 29 b) >> 8
+// (16-bit: (19595 r + 38470 g + 7471 b) >> 16), missing alpha = full scale.
+template <typename T> static std::vector<T> convert_channels(const std::vector<T> &in, size_t n, int from, int to) {
+    if (from == to) return in;
+    const uint32_t full = sizeof(T) == 1 ? 255u : 65535u;
+    auto luma = [&](const T *p) -> T {
+        if (sizeof(T) == 1) return (T)(((uint32_t)p[0] * 77 + (uint32_t)p[1] * 150 + (uint32_t)p[2] * 29) >> 8);
+        return (T)(((uint32_t)p[0] * 19595 + (uint32_t)p[1] * 38470 + (uint32_t)p[2] * 7471) >> 16);
+    };
+    std::vector<T> out(n * (size_t)to);
+    for (size_t i = 0; i < n; ++i) {
+        const T *s = &in[i * from];
+        T *d = &out[i * to];
+        const T g = from >= 3 ? luma(s) : s[0];
+        const T a = (from == 2) ? s[1] : (from == 4 ? s[3] : (T)full);
+        switch (to) {
+        case 1: d[0] = g; break;
+        case 2: d[0] = g, d[1] = a; break;
+        case 3:
+            if (from >= 3) d[0] = s[0], d[1] = s[1], d[2] = s[2];
+            else d[0] = d[1] = d[2] = g;
+            break;
+        default:
+            if (from >= 3) d[0] = s[0], d[1] = s[1], d[2] = s[2];
+            else d[0] = d[1] = d[2] = g;
+            d[3] = a;
+        }
+    }
+    return out;
+}
+
+PngImage decode_png(const uint8_t *b, size_t n, const std::string &name) {
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    if (n < 8 || std::memcmp(b, sig, 8) != 0) throw ConfigError(name + ": not a PNG file");
+    size_t p = 8;
+    uint32_t w = 0, h = 0;
+    int depth = 0, ctype = -1, lace = 0;
+    std::vector<uint8_t> idat, plte, trns;
+    bool end = false;
+    while (!end) {
+        if (p + 12 > n) throw ConfigError(name + ": truncated PNG");
+        const uint32_t len = be32(b + p);
+        const std::string type((const char *)b + p + 4, 4);
+        if (len > n - p - 12) throw ConfigError(name + ": truncated PNG chunk");
+        const uint8_t *d = b + p + 8;
+        if (type == "IHDR") {
+            if (len != 13) throw ConfigError(name + ": bad IHDR");
+            w = be32(d), h = be32(d + 4), depth = d[8], ctype = d[9], lace = d[12];
+            if (d[10] != 0 || d[11] != 0 || lace > 1) throw ConfigError(name + ": unsupported PNG compression / filter / interlace");
+        } else if (type == "PLTE") {
+            plte.assign(d, d + len);
+        } else if (type == "tRNS") {
+            trns.assign(d, d + len);
+        } else if (type == "IDAT") {
+            idat.insert(idat.end(), d, d + len);
+        } else if (type == "IEND") {
+            end = true;
+        }
+        p += 12 + len;
+    }
+    const int chans = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 6 ? 4 : 0;
+    const bool depth_ok = (ctype == 0 && (depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16)) ||
+                          (ctype == 3 && depth <= 8 && (depth & (depth - 1)) == 0) ||
+                          ((ctype == 2 || ctype == 4 || ctype == 6) && (depth == 8 || depth == 16));
+    if (!chans || !depth_ok || w == 0 || h == 0 || w > (1u << 24) || h > (1u << 24))
+        throw ConfigError(name + ": unsupported PNG colour type / bit depth");
+    if (ctype == 3 && (plte.empty() || plte.size() % 3)) throw ConfigError(name + ": palette image without PLTE");
+    // inflate
+    const int bits_pp = chans * depth;
+    auto row_bytes = [&](uint32_t ww) { return ((size_t)ww * bits_pp + 7) / 8; };
+    size_t raw_size = 0;
+    static const int ax0[7] = {0, 4, 0, 2, 0, 1, 0}, ay0[7] = {0, 0, 4, 0, 2, 0, 1};
+    static const int adx[7] = {8, 8, 4, 4, 2, 2, 1}, ady[7] = {8, 8, 8, 4, 4, 2, 2};
+    auto pass_dims = [&](int k, uint32_t &pw, uint32_t &ph) {
+        pw = w > (uint32_t)ax0[k] ? (w - ax0[k] + adx[k] - 1) / adx[k] : 0;
+        ph = h > (uint32_t)ay0[k] ? (h - ay0[k] + ady[k] - 1) / ady[k] : 0;
+    };
+    if (lace) {
+        for (int k = 0; k < 7; ++k) {
+            uint32_t pw, ph;
+            pass_dims(k, pw, ph);
+            if (pw && ph) raw_size += (size_t)ph * (row_bytes(pw) + 1);
+        }
+    } else {
+        raw_size = (size_t)h * (row_bytes(w) + 1);
+    }
+    std::vector<uint8_t> raw(raw_size);
+    uLongf got = (uLongf)raw_size;
+    if (uncompress(raw.data(), &got, idat.data(), (uLong)idat.size()) != Z_OK || got != raw_size)
+        throw ConfigError(name + ": bad PNG image data");
+    // unfilter and unpack to one sample per channel (16-bit samples kept as 16-bit)
+    const int bpp = std::max(1, bits_pp / 8);
+    std::vector<uint16_t> samples((size_t)w * h * chans);
+    auto unpack_into = [&](const std::vector<uint8_t> &rows, uint32_t pw, uint32_t ph, int x0, int y0, int dx, int dy) {
+        const size_t rb = row_bytes(pw);
+        for (uint32_t y = 0; y < ph; ++y) {
+            const uint8_t *r = rows.data() + (size_t)y * rb;
+            for (uint32_t x = 0; x < pw; ++x)
+                for (int c = 0; c < chans; ++c) {
+                    uint32_t v;
+                    const size_t bit = ((size_t)x * chans + c) * depth;
+                    if (depth == 16) v = (uint32_t)r[bit / 8] << 8 | r[bit / 8 + 1];
+                    else if (depth == 8) v = r[bit / 8];
+                    else v = (r[bit / 8] >> (8 - depth - (bit % 8))) & ((1u << depth) - 1);
+                    samples[((size_t)(y0 + y * dy) * w + (x0 + x * dx)) * chans + c] = (uint16_t)v;
+                }
+        }
+    };
+    std::vector<uint8_t> rows;
+    if (lace) {
+        size_t off = 0;
+        for (int k = 0; k < 7; ++k) {
+            uint32_t pw, ph;
+            pass_dims(k, pw, ph);
+            if (!pw || !ph) continue;
+            unfilter(raw.data() + off, raw.size() - off, (int)ph, row_bytes(pw), bpp, rows, name);
+            unpack_into(rows, pw, ph, ax0[k], ay0[k], adx[k], ady[k]);
+            off += (size_t)ph * (row_bytes(pw) + 1);
+        }
+    } else {
+        unfilter(raw.data(), raw.size(), (int)h, row_bytes(w), bpp, rows, name);
+        unpack_into(rows, w, h, 0, 0, 1, 1);
+    }
+    // expand as stb_image does: palette to RGB / RGBA, low-depth gray to 8 bits, tRNS keys to alpha
+    PngImage img;
+    img.w = (int)w, img.h = (int)h;
+    const size_t np = (size_t)w * h;
+    img.bits = depth == 16 ? 16 : 8;
+    std::vector<uint16_t> out;
+    if (ctype == 3) {
+        const bool alpha = !trns.empty();
+        img.comp = alpha ? 4 : 3;
+        out.resize(np * img.comp);
+        const size_t ncol = plte.size() / 3;
+        for (size_t i = 0; i < np; ++i) {
+            const size_t k = samples[i];
+            if (k >= ncol) throw ConfigError(name + ": palette index out of range");
+            for (int c = 0; c < 3; ++c) out[i * img.comp + c] = plte[3 * k + c];
+            if (alpha) out[i * 4 + 3] = k < trns.size() ? trns[k] : 255;
+        }
+    } else {
+        const int scale = depth >= 8 ? 1 : (depth == 4 ? 0x11 : (depth == 2 ? 0x55 : 0xff));
+        const bool key = !trns.empty() && (ctype == 0 || ctype == 2);
+        img.comp = chans + (key ? 1 : 0);
+        out.resize(np * img.comp);
+        uint16_t kv[3] = {0, 0, 0};
+        if (key) {
+            if (trns.size() < (size_t)(2 * chans)) throw ConfigError(name + ": bad tRNS");
+            for (int c = 0; c < chans; ++c) kv[c] = (uint16_t)((trns[2 * c] << 8 | trns[2 * c + 1]) & ((1u << depth) - 1));
+        }
+        const uint16_t full = depth == 16 ? 65535 : 255;
+        for (size_t i = 0; i < np; ++i) {
+            bool match = key;
+            for (int c = 0; c < chans; ++c) {
+                const uint16_t v = samples[i * chans + c];
+                match = match && v == kv[c];
+                out[i * img.comp + c] = (uint16_t)(v * scale);
+            }
+            if (key) out[i * img.comp + chans] = match ? 0 : full;
+        }
+    }
+    if (img.bits == 16) {
+        img.data16 = std::move(out);
+    } else {
+        img.data8.resize(out.size());
+        for (size_t i = 0; i < out.size(); ++i) img.data8[i] = (uint8_t)out[i];
+    }
+    return img;
+}
+
+PngImage read_png(const fs::path &path) {
+    std::vector<uint8_t> f = read_all(path);
+    return decode_png(f.data(), f.size(), path.string());
+}
+
+void PngImage::to_channels(int c) {
+    if (c < 1 || c > 4) throw ConfigError("1 to 4 channels can be requested");
+    if (bits == 16) data16 = convert_channels(data16, (size_t)w * h, comp, c);
+    else data8 = convert_channels(data8, (size_t)w * h, comp, c);
+    comp = c;
+}
+
+void PngImage::to_8bit() {
+    if (bits != 16) return;
+    data8.resize(data16.size());
+    for (size_t i = 0; i < data16.size(); ++i) data8[i] = (uint8_t)(data16[i] >> 8); // stbi__convert_16_to_8
+    data16.clear();
+    bits = 8;
 }
 
 } // namespace ks

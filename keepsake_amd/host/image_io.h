@@ -29,10 +29,29 @@ void write_exr(const fs::path &path, const float *data, int w, int h, int c, boo
 // RGB floats of a scanline EXR (channels R, G, B by name, or Y replicated); PIZ, PXR24, B44 and DWA
 // compressions and tiled or multi-part files are rejected with an error.
 std::vector<float> read_exr(const fs::path &path, int &w, int &h);
+// n_out = 1..4 floats per pixel: R, G, B (a single-channel image replicated into all four, as tinyexr's
+// LoadEXR does) and A (1 when absent) -- the first n_out of them (load_from_exr, image_util.cpp:72-99).
+std::vector<float> read_exr_channels(const fs::path &path, int &w, int &h, int n_out);
 // RGB floats of a Radiance RGBE (.hdr) file (flat or run-length scanlines).
 std::vector<float> read_hdr(const fs::path &path, int &w, int &h);
 // 8-bit PNG, c = 1..4 channels, rows top to bottom.
 void write_png(const fs::path &path, const uint8_t *data, int w, int h, int c);
+
+// A decoded PNG as stb_image delivers it: palette images expanded to RGB (RGBA with tRNS), gray of 1, 2
+// or 4 bits scaled to 8, tRNS colour keys turned into an alpha channel; 8- or 16-bit samples, rows top
+// to bottom. Interlaced files are de-interlaced.
+struct PngImage {
+    int w = 0, h = 0, comp = 0, bits = 8;
+    std::vector<uint8_t> data8;
+    std::vector<uint16_t> data16;
+    // stb_image's channel conversion (stbi__convert_format): luminance (77 r + 150 g + 29 b) >> 8,
+    // opaque alpha where none was stored
+    void to_channels(int c);
+    // 16 -> 8 bits by the high byte (stbi__convert_16_to_8)
+    void to_8bit();
+};
+PngImage decode_png(const uint8_t *bytes, size_t n, const std::string &name);
+PngImage read_png(const fs::path &path);
 
 // tonemap.h: ToneMapper::operator()(color3)
 struct ToneMapper {

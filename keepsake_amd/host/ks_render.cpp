@@ -5,10 +5,13 @@
 // opening a GPU; --dump-scene DIR also writes each resolved scene's flat arrays as raw little-endian
 // files under DIR/task_<id>/ (vertices.f32, normals.f32, indices.u32, tri_geom.u32,
 // geom_material.u32, geom_flags.u8, materials.bin of ks_material_desc records, lights.bin of
-// ks_light_desc records).
+// ks_light_desc records, texcoords.f32, field_of.i32 and fields.bin of ks_material_fields records for
+// textured materials, textures.txt with one "width height channels data_type color_space levels
+// build_mipmaps" line per texture and its levels in texture_<i>.bin).
 #include "image_io.h"
 #include "mesh_asset.h"
 #include "small_pt_task.h"
+#include "texture_asset.h"
 
 #include <cstdio>
 #include <cstring>
@@ -37,6 +40,16 @@ static void dump_scene(const ks::SceneData &s, const ks::fs::path &dir) {
     dump_array(dir / "geom_flags.u8", s.geom_flags);
     dump_array(dir / "materials.bin", s.materials);
     dump_array(dir / "lights.bin", s.lights);
+    dump_array(dir / "texcoords.f32", s.texcoords);
+    dump_array(dir / "field_of.i32", s.field_of);
+    dump_array(dir / "fields.bin", s.fields);
+    std::ofstream t(dir / "textures.txt");
+    for (size_t i = 0; i < s.textures.size(); ++i) {
+        const ks_texture_desc &d = s.textures[i];
+        t << d.width << " " << d.height << " " << d.channels << " " << d.data_type << " " << d.color_space << " "
+          << d.levels << " " << d.build_mipmaps << "\n";
+        dump_array(dir / ("texture_" + std::to_string(i) + ".bin"), s.texture_bytes[i]);
+    }
 }
 
 int main(int argc, char **argv) {
@@ -93,6 +106,7 @@ int main(int argc, char **argv) {
         cfg.register_task("small_pt", ks::small_pt);
         cfg.register_task("convert_mesh_asset", ks::convert_mesh_asset);
         cfg.register_task("tonemap", ks::tonemap_task);
+        cfg.register_task("convert_texture", ks::convert_texture);
         cfg.run_all_tasks();
     } catch (const ks::ConfigError &e) {
         std::fprintf(stderr, "[ks_render] config error: %s\n", e.what());

@@ -19,6 +19,7 @@
 
 #include "gltf_asset.h"
 #include "mesh_asset.h"
+#include "texture_asset.h"
 
 #include <cctype>
 #include <chrono>
@@ -44,49 +45,79 @@ static SceneData scene_from_host(const ks_host_scene *hs, ks_camera_desc *cam) {
     return s;
 }
 
-// A constant shader field (create_constant_shader_field_color, shader_field.h:88-105) of 1 or 3
-// channels; texture fields have no place in the flat material record.
-static std::array<float, 3> load_constant_field(const ConfigArgs &bsdf, const char *key, int channels) {
+// A principled_bsdf field (create_principled_bsdf, principled_bsdf.cpp:707-732): a shader field table.
+static ks_field_desc bsdf_field(const ConfigArgs &bsdf, const char *key, int channels, TextureRegistry &reg) {
     if (!bsdf.contains(key)) throw ConfigError(std::string("principled_bsdf needs '") + key + "'");
-    ConfigArgs f = bsdf[key];
-    std::string type = f.load_string("type");
-    if (type == "texture") throw ConfigError(std::string("texture shader fields are not supported ('") + key + "')");
-    if (type != "constant") throw ConfigError(std::string("unknown shader field type '") + type + "'");
-    if (channels == 1) {
-        float v = f.load_float("value");
-        return {v, v, v};
-    }
-    return f.load_vec3("value");
+    return load_shader_field(bsdf[key], channels, reg);
 }
 
-// [material.X] -> bsdf = "bsdf.Y" of type principled_bsdf (the only BSDF the kernels evaluate).
-static ks_material_desc load_material(const ConfigArgs &mat) {
-    for (const char *k : {"subsurface", "normal_map", "opacity_map"})
-        if (mat.contains(k) && !mat.load_string(k).empty())
-            throw ConfigError(std::string("material key '") + k + "' is not supported");
+// [normal_map.X] (create_normal_map, normal_map.cpp:42-56): map (a 3-channel shader field), space
+// ("tangent" | "world"), range01 (true), strength (1), to_world.
+static void load_normal_map(const ConfigArgs &nm, ks_material_fields &f, TextureRegistry &reg) {
+    f.has_normal_map = 1;
+    f.normal_map = load_shader_field(nm["map"], 3, reg);
+    std::string space = nm.load_string("space", "tangent");
+    if (space != "tangent" && space != "world") throw ConfigError("normal map space must be tangent or world");
+    f.normal_space = space == "world" ? KS_NORMAL_MAP_WORLD : KS_NORMAL_MAP_TANGENT;
+    f.normal_range01 = nm.load_bool("range01", true) ? 1 : 0;
+    f.normal_strength = nm.load_float("strength", 1.0f);
+    std::array<float, 16> m = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    if (nm.contains("to_world")) m = nm.load_transform("to_world");
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) f.normal_to_world[3 * r + c] = m[4 * r + c];
+}
+
+// [material.X] -> bsdf = "bsdf.Y" of type principled_bsdf (the only BSDF the kernels evaluate), and its
+// normal map (create_material, material.cpp:169-177). `fields` receives every closure input as a shader
+// field; `textured` says whether any of them (or a normal map) needs the per-hit evaluation, else the
+// constant record `d` holds the material.
+static ks_material_desc load_material(const ConfigArgs &mat, TextureRegistry &reg, ks_material_fields &fields,
+                                      bool &textured) {
+    if (mat.contains("subsurface") && !mat.load_string("subsurface").empty())
+        throw ConfigError("material key 'subsurface' is not supported");
     ConfigArgs bsdf = mat.asset(mat.load_string("bsdf"));
     std::string type = bsdf.load_string("type");
     if (type != "principled_bsdf") throw ConfigError("bsdf type '" + type + "' is not supported (principled_bsdf only)");
     if (bsdf.load_string("microfacet", "ggx") != "ggx") throw ConfigError("only the ggx microfacet is supported");
-    ks_material_desc d{};
-    auto c = load_constant_field(bsdf, "basecolor", 3);
-    for (int k = 0; k < 3; ++k) d.basecolor[k] = c[k];
-    d.roughness = load_constant_field(bsdf, "roughness", 1)[0];
-    d.metallic = load_constant_field(bsdf, "metallic", 1)[0];
-    d.ior = load_constant_field(bsdf, "ior", 1)[0];
-    d.specular_r0_mul = load_constant_field(bsdf, "specular_r0_mul", 1)[0];
-    d.specular_trans = load_constant_field(bsdf, "specular_trans", 1)[0];
-    d.diffuse_trans = load_constant_field(bsdf, "diffuse_trans", 1)[0];
-    d.diffuse_trans_fwd = load_constant_field(bsdf, "diffuse_trans_fwd", 1)[0];
+    fields = ks_material_fields{};
+    fields.basecolor = bsdf_field(bsdf, "basecolor", 3, reg);
+    fields.roughness = bsdf_field(bsdf, "roughness", 1, reg);
+    fields.metallic = bsdf_field(bsdf, "metallic", 1, reg);
+    fields.ior = bsdf_field(bsdf, "ior", 1, reg);
+    fields.specular_r0_mul = bsdf_field(bsdf, "specular_r0_mul", 1, reg);
+    fields.specular_trans = bsdf_field(bsdf, "specular_trans", 1, reg);
+    fields.diffuse_trans = bsdf_field(bsdf, "diffuse_trans", 1, reg);
+    fields.diffuse_trans_fwd = bsdf_field(bsdf, "diffuse_trans_fwd", 1, reg);
     // the bsdf's `emissive` field is read but, as in the reference's material path, does not make a light
-    if (bsdf.contains("emissive")) (void)load_constant_field(bsdf, "emissive", 3);
-    if (mat.contains("emission")) {
-        std::array<float, 3> e;
-        if (mat["emission"].node->is_table()) e = load_constant_field(mat, "emission", 3);
-        else e = mat.load_vec3("emission");
-        for (int k = 0; k < 3; ++k) d.emission[k] = e[k];
-        d.has_emission = 1;
+    if (bsdf.contains("emissive")) (void)load_shader_field(bsdf["emissive"], 3, reg);
+    fields.emission.texture = -1;
+    if (mat.contains("emission")) { // extension: a shader field table or an RGB triple
+        if (mat["emission"].node->is_table()) {
+            fields.emission = load_shader_field(mat["emission"], 3, reg);
+        } else {
+            auto e = mat.load_vec3("emission");
+            for (int k = 0; k < 3; ++k) fields.emission.value[k] = e[k];
+        }
+        fields.has_emission = 1;
     }
+    const std::string nm = mat.load_string("normal_map", "");
+    if (!nm.empty()) load_normal_map(mat.asset(nm), fields, reg);
+    const ks_field_desc *all[9] = {&fields.basecolor, &fields.roughness, &fields.metallic, &fields.ior,
+                                   &fields.specular_r0_mul, &fields.specular_trans, &fields.diffuse_trans,
+                                   &fields.diffuse_trans_fwd, &fields.emission};
+    textured = fields.has_normal_map != 0;
+    for (const ks_field_desc *f : all) textured |= f->texture >= 0;
+    ks_material_desc d{};
+    for (int k = 0; k < 3; ++k) d.basecolor[k] = fields.basecolor.value[k];
+    d.roughness = fields.roughness.value[0];
+    d.metallic = fields.metallic.value[0];
+    d.ior = fields.ior.value[0];
+    d.specular_r0_mul = fields.specular_r0_mul.value[0];
+    d.specular_trans = fields.specular_trans.value[0];
+    d.diffuse_trans = fields.diffuse_trans.value[0];
+    d.diffuse_trans_fwd = fields.diffuse_trans_fwd.value[0];
+    for (int k = 0; k < 3; ++k) d.emission[k] = fields.emission.value[k];
+    d.has_emission = fields.has_emission;
     return d;
 }
 
@@ -95,7 +126,7 @@ static ks_material_desc load_material(const ConfigArgs &mat) {
 // geometries without smooth normals get zero rows and the KS_GEOM_FLAT flag.
 class FlatSceneBuilder {
   public:
-    explicit FlatSceneBuilder(bool any_normals) : any_normals_(any_normals) {}
+    FlatSceneBuilder(bool any_normals, bool any_texcoords) : any_normals_(any_normals), any_uv_(any_texcoords) {}
     void add(const MeshData &m, uint32_t material) {
         uint32_t base = (uint32_t)(s_.vertices.size() / 3);
         if ((uint64_t)base + m.vertex_count() >= (1ull << 32)) throw ConfigError("scene has too many vertices");
@@ -105,9 +136,15 @@ class FlatSceneBuilder {
             if (smooth) s_.normals.insert(s_.normals.end(), m.vertex_normals.begin(), m.vertex_normals.end());
             else s_.normals.insert(s_.normals.end(), m.vertices.size(), 0.0f);
         }
+        bool uv = !m.texcoords.empty();
+        if (any_uv_) {
+            if (uv) s_.texcoords.insert(s_.texcoords.end(), m.texcoords.begin(), m.texcoords.end());
+            else s_.texcoords.insert(s_.texcoords.end(), 2 * m.vertex_count(), 0.0f);
+        }
         for (uint32_t i : m.indices) s_.indices.push_back(base + i);
         s_.tri_geom.insert(s_.tri_geom.end(), m.tri_count(), (uint32_t)s_.geom_material.size());
-        s_.geom_flags.push_back((uint8_t)((m.twosided ? 0u : KS_GEOM_ONE_SIDED) | (smooth ? 0u : KS_GEOM_FLAT)));
+        s_.geom_flags.push_back((uint8_t)((m.twosided ? 0u : KS_GEOM_ONE_SIDED) | (smooth ? 0u : KS_GEOM_FLAT) |
+                                          (any_uv_ && !uv ? KS_GEOM_NO_TEXCOORD : 0u)));
         s_.geom_material.push_back(material);
     }
     // a [material.*] table by name, read once
@@ -115,27 +152,41 @@ class FlatSceneBuilder {
         if (name->type != TomlValue::Type::String) throw ConfigError("material list entries must be strings");
         auto it = named_.find(name->str);
         if (it == named_.end()) {
-            it = named_.emplace(name->str, (uint32_t)s_.materials.size()).first;
-            s_.materials.push_back(load_material(args.asset(name->str)));
+            ks_material_fields f;
+            bool textured = false;
+            ks_material_desc d = load_material(args.asset(name->str), reg_, f, textured);
+            it = named_.emplace(name->str, add_material(d, textured ? &f : nullptr)).first;
         }
         return it->second;
     }
-    uint32_t add_material(const ks_material_desc &d) {
+    uint32_t add_material(const ks_material_desc &d, const ks_material_fields *fields = nullptr) {
         s_.materials.push_back(d);
+        s_.field_of.push_back(-1);
+        if (fields) {
+            s_.field_of.back() = (int32_t)s_.fields.size();
+            s_.fields.push_back(*fields);
+        }
         return (uint32_t)s_.materials.size() - 1;
     }
+    TextureRegistry &textures() { return reg_; }
     SceneData finish() {
         bool any_flag = false;
         for (uint8_t f : s_.geom_flags) any_flag |= f != 0;
         if (!any_flag) s_.geom_flags.clear();
         if (s_.indices.empty()) throw ConfigError("mesh asset has no triangles");
+        for (TextureData &t : reg_.textures) {
+            s_.textures.push_back(t.desc());
+            s_.texture_bytes.push_back(std::move(t.bytes));
+        }
+        if (s_.fields.empty()) s_.field_of.clear();
         return std::move(s_);
     }
 
   private:
-    bool any_normals_;
+    bool any_normals_, any_uv_;
     SceneData s_;
     std::map<std::string, uint32_t> named_;
+    TextureRegistry reg_;
 };
 
 // The material list of the task: one inner list of [material.*] names per subscene, one name per
@@ -157,9 +208,12 @@ static SceneData scene_from_mesh_asset(const MeshAsset &asset, const ConfigArgs 
     if (names.array_size() != asset.meshes.size())
         throw ConfigError("Material list mismatch dimension: " + std::to_string(names.array_size()) +
                           " materials for " + std::to_string(asset.meshes.size()) + " meshes.");
-    bool any_normals = false;
-    for (const MeshData &m : asset.meshes) any_normals |= m.use_smooth_normal && !m.vertex_normals.empty();
-    FlatSceneBuilder b(any_normals);
+    bool any_normals = false, any_uv = false;
+    for (const MeshData &m : asset.meshes) {
+        any_normals |= m.use_smooth_normal && !m.vertex_normals.empty();
+        any_uv |= !m.texcoords.empty();
+    }
+    FlatSceneBuilder b(any_normals, any_uv);
     for (size_t g = 0; g < asset.meshes.size(); ++g)
         b.add(asset.meshes[g], b.named_material(args, names[(int)g].node));
     return b.finish();
@@ -175,11 +229,13 @@ static SceneData scene_from_compound(const CompoundMeshAsset &asset, const Confi
         throw ConfigError("No materials. Either load materials from file(s) (load_materials = true) or provide a "
                           "material list.");
     std::vector<std::vector<uint32_t>> mat(asset.prototypes.size());
-    bool any_normals = false;
+    bool any_normals = false, any_uv = false;
     for (const auto &inst : asset.instances)
-        for (const MeshData &m : asset.prototypes[inst.first].meshes)
+        for (const MeshData &m : asset.prototypes[inst.first].meshes) {
             any_normals |= m.use_smooth_normal && !m.vertex_normals.empty();
-    FlatSceneBuilder b(any_normals);
+            any_uv |= !m.texcoords.empty();
+        }
+    FlatSceneBuilder b(any_normals, any_uv);
     if (toml_materials) {
         ConfigArgs list = material_list(args, asset.prototypes.size());
         for (size_t p = 0; p < asset.prototypes.size(); ++p) {

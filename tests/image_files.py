@@ -276,3 +276,113 @@ def rgbe_to_float(rgbe):
     e = rgbe[..., 3].astype(np.int32)
     s = np.where(e > 0, np.ldexp(np.float32(1), e - 136), 0).astype(np.float32)
     return rgbe[..., :3].astype(np.float32) * s[..., None]
+
+
+def write_png_any(path, samples, ctype, depth, palette=None, trns=None, interlace=False, seed=0):
+    """PNG writer for every colour type (0 gray, 2 RGB, 3 palette, 4 gray+alpha, 6 RGBA) and bit depth,
+    with a random row filter per row (all five kinds) and optional Adam7 interlacing.
+    samples: (H, W, C) integer array of raw samples (palette indices for type 3)."""
+    rng = np.random.default_rng(seed)
+    h, w, c = samples.shape
+
+    def pack_rows(img):
+        rows = []
+        hh, ww, _ = img.shape
+        for y in range(hh):
+            flat = img[y].reshape(-1).astype(np.int64)
+            if depth == 16:
+                raw = b"".join(struct.pack(">H", int(v)) for v in flat)
+            elif depth == 8:
+                raw = bytes(int(v) for v in flat)
+            else:
+                bits = "".join(format(int(v), f"0{depth}b") for v in flat)
+                bits += "0" * (-len(bits) % 8)
+                raw = bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8))
+            rows.append(raw)
+        return rows
+
+    bpp = max(1, c * depth // 8)
+
+    def filter_rows(rows):
+        out = bytearray()
+        prev = bytes(len(rows[0])) if rows else b""
+        for r in rows:
+            f = int(rng.integers(0, 5))
+            fr = bytearray(len(r))
+            for i in range(len(r)):
+                a = r[i - bpp] if i >= bpp else 0
+                b = prev[i]
+                cc = prev[i - bpp] if i >= bpp else 0
+                if f == 0:
+                    pr = 0
+                elif f == 1:
+                    pr = a
+                elif f == 2:
+                    pr = b
+                elif f == 3:
+                    pr = (a + b) // 2
+                else:
+                    p = a + b - cc
+                    pa, pb, pc = abs(p - a), abs(p - b), abs(p - cc)
+                    pr = a if pa <= pb and pa <= pc else (b if pb <= pc else cc)
+                fr[i] = (r[i] - pr) % 256
+            out += bytes([f]) + bytes(fr)
+            prev = r
+        return bytes(out)
+
+    if interlace:
+        data = b""
+        for x0, y0, dx, dy in [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2), (0, 1, 1, 2)]:
+            sub = samples[y0::dy, x0::dx]
+            if sub.size:
+                data += filter_rows(pack_rows(sub))
+    else:
+        data = filter_rows(pack_rows(samples))
+
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d))
+
+    out = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, ctype, 0, 0, 1 if interlace else 0))
+    if palette is not None:
+        out += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
+    if trns is not None:
+        out += chunk(b"tRNS", bytes(trns))
+    out += chunk(b"IDAT", zlib.compress(data)) + chunk(b"IEND", b"")
+    with open(path, "wb") as f:
+        f.write(out)
+
+
+def lz4_decode(src, size):
+    """LZ4 block decoder from the published block format."""
+    out = bytearray()
+    i = 0
+    while i < len(src):
+        tok = src[i]
+        i += 1
+        lit = tok >> 4
+        if lit == 15:
+            while True:
+                b = src[i]
+                i += 1
+                lit += b
+                if b != 255:
+                    break
+        out += src[i:i + lit]
+        i += lit
+        if i >= len(src):
+            break
+        off = src[i] | src[i + 1] << 8
+        i += 2
+        ml = tok & 15
+        if ml == 15:
+            while True:
+                b = src[i]
+                i += 1
+                ml += b
+                if b != 255:
+                    break
+        ml += 4
+        for _ in range(ml):
+            out.append(out[-off])
+    assert len(out) == size
+    return bytes(out)
