@@ -138,20 +138,23 @@ def test_obj_shapes_remap_and_quads(tmp_path):
     # the square has vertex normals; the others take the geometric normal (flag) and zero rows
     assert np.array_equal(s["normals"][:4], np.tile([0, 0, 1], (4, 1)).astype(np.float32))
     assert not s["normals"][4:].any()
-    # OBJ meshes default to one-sided (twosided = false)
-    assert list(s["geom_flags"]) == [KS_GEOM_ONE_SIDED, KS_GEOM_ONE_SIDED | KS_GEOM_FLAT,
-                                     KS_GEOM_ONE_SIDED | KS_GEOM_FLAT]
+    # OBJ meshes default to one-sided (twosided = false); only the square has texture coordinates
+    no_uv = 4  # KS_GEOM_NO_TEXCOORD
+    assert list(s["geom_flags"]) == [KS_GEOM_ONE_SIDED, KS_GEOM_ONE_SIDED | KS_GEOM_FLAT | no_uv,
+                                     KS_GEOM_ONE_SIDED | KS_GEOM_FLAT | no_uv]
+    tc = np.fromfile(tmp_path / "dump" / "task_0" / "texcoords.f32", np.float32).reshape(-1, 2)
+    assert np.array_equal(tc[:4], np.tile([0.25, 0.75], (4, 1)).astype(np.float32)) and not tc[4:].any()
 
 
 def test_obj_twosided_flat_options(tmp_path):
     (tmp_path / "m.obj").write_text(SHAPES_OBJ)
     text = asset("m", tmp_path / "m.obj", twosided="true", use_smooth_normal="false") + MATS
     info, s = dump(tmp_path, text + task("mesh_asset.m", ["material.grey"] * 3))
-    assert list(s["geom_flags"]) == [KS_GEOM_FLAT] * 3
+    assert list(s["geom_flags"]) == [KS_GEOM_FLAT, KS_GEOM_FLAT | 4, KS_GEOM_FLAT | 4]
     assert s["normals"].size == 0  # no geometry keeps its vertex normals
     text = asset("m", tmp_path / "m.obj", twosided="true") + MATS
     info, s = dump(tmp_path, text + task("mesh_asset.m", ["material.grey"] * 3))
-    assert list(s["geom_flags"]) == [0, KS_GEOM_FLAT, KS_GEOM_FLAT]
+    assert list(s["geom_flags"]) == [0, KS_GEOM_FLAT | 4, KS_GEOM_FLAT | 4]  # 4: KS_GEOM_NO_TEXCOORD
 
 
 def _poly_obj(pts):
@@ -370,9 +373,9 @@ emission = [10.0, 9.0, 8.0]
         (["material.grey", "material.grey", "material.tex"],
          bsdf_table("tex").replace('basecolor = { type = "constant", value = [0.5, 0.5, 0.5] }',
                                    'basecolor = { type = "texture", map = "texture.x" }')
-         + '[material.tex]\nbsdf = "bsdf.tex"\n', "texture shader fields are not supported"),
+         + '[material.tex]\nbsdf = "bsdf.tex"\n', "unknown asset 'texture.x'"),
         (["material.grey", "material.grey", "material.nm"],
-         '[material.nm]\nbsdf = "bsdf.grey"\nnormal_map = "normal_map.x"\n', "material key 'normal_map'"),
+         '[material.nm]\nbsdf = "bsdf.grey"\nnormal_map = "normal_map.x"\n', "unknown asset 'normal_map.x'"),
     ],
 )
 def test_material_errors(tmp_path, mats, extra_tables, msg):
