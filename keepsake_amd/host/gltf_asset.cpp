@@ -1,6 +1,8 @@
 // glTF 2.0 compound mesh assets (mesh_asset.cpp:311-1129) with a small JSON reader of its own.
 #include "gltf_asset.h"
 
+#include "image_io.h"
+
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -508,53 +510,175 @@ bool has_texture(const Json &o, const char *key) {
     return t && t->type == Json::Type::Object && t->get("index");
 }
 
-// A constant-factor principled BSDF for one glTF material (mesh_asset.cpp:650-956); `twosided` is
-// cleared for KHR_materials_transmission.
-ks_material_desc convert_material(const Model &m, const Json *mat, const GltfLoadOptions &opt, bool &twosided) {
+// glTF textures (create_texture_from_gltf, mesh_asset.cpp:370-389): one per image, decoded to RGBA as
+// tinygltf's loader delivers images (4 channels; 16-bit PNGs stay 16-bit), in sRGB where a baseColor or
+// emissive texture uses the image, else linear; every level built. PNG only (no JPEG decoder here).
+struct GltfTextures {
+    const Model *m = nullptr;
+    fs::path dir;
+    std::vector<int> color_space;     // per image
+    std::vector<int> loaded;          // per image: index into out, or -1
+    std::vector<TextureData> *out = nullptr;
+
+    std::vector<uint8_t> image_bytes(size_t img) {
+        const Json &im = m->element("images", (double)img);
+        if (const Json *uri = im.get("uri")) {
+            if (uri->type != Json::Type::String) throw ConfigError(m->src + ": image uri must be a string");
+            const std::string &u = uri->str;
+            if (u.rfind("data:", 0) == 0) {
+                size_t comma = u.find(',');
+                if (comma == std::string::npos || u.substr(0, comma).find(";base64") == std::string::npos)
+                    throw ConfigError(m->src + ": only base64 data URIs are supported");
+                return base64_decode(std::string_view(u).substr(comma + 1), m->src);
+            }
+            return read_file(dir / percent_decode(u));
+        }
+        const Json *bv = im.get("bufferView");
+        if (!bv) throw ConfigError(m->src + ": image without uri or bufferView");
+        const Json &view = m->element("bufferViews", bv->num);
+        const int buf = integer(view, "buffer", -1);
+        if (buf < 0 || (size_t)buf >= m->buffers.size()) throw ConfigError(m->src + ": bad image bufferView");
+        const size_t off = byte_size(view, "byteOffset"), len = byte_size(view, "byteLength");
+        if (off > m->buffers[(size_t)buf].size() || len > m->buffers[(size_t)buf].size() - off)
+            throw ConfigError(m->src + ": image bufferView out of range");
+        const uint8_t *b = m->buffers[(size_t)buf].data() + off;
+        return std::vector<uint8_t>(b, b + len);
+    }
+    int texture(size_t img) {
+        if (loaded[img] >= 0) return loaded[img];
+        std::vector<uint8_t> bytes = image_bytes(img);
+        if (bytes.size() < 8 || bytes[0] != 0x89 || bytes[1] != 'P')
+            throw ConfigError(m->src + ": image " + std::to_string(img) + " is not a PNG (only PNG images are decoded)");
+        PngImage png = decode_png(bytes.data(), bytes.size(), m->src + " image " + std::to_string(img));
+        png.to_channels(4);
+        TextureData t;
+        t.width = png.w, t.height = png.h, t.channels = 4;
+        t.color_space = color_space[img];
+        t.build_mipmaps = 1;
+        if (png.bits == 16) {
+            t.data_type = KS_TEX_U16;
+            t.bytes.resize(png.data16.size() * 2);
+            std::memcpy(t.bytes.data(), png.data16.data(), t.bytes.size());
+        } else {
+            t.data_type = KS_TEX_U8;
+            t.bytes = std::move(png.data8);
+        }
+        out->push_back(std::move(t));
+        return loaded[img] = (int)out->size() - 1;
+    }
+};
+
+// KHR_texture_transform scale and offset of a texture info (get_texture_transform, mesh_asset.cpp:391-444)
+void texture_transform(const Json &info, float scale[2], float offset[2]) {
+    scale[0] = scale[1] = 1.0f;
+    offset[0] = offset[1] = 0.0f;
+    const Json *ext = info.get("extensions");
+    const Json *tt = ext ? ext->get("KHR_texture_transform") : nullptr;
+    if (!tt) return;
+    if (const Json *s = tt->get("scale"); s && s->type == Json::Type::Array)
+        for (size_t i = 0; i < 2 && i < s->arr.size(); ++i)
+            if (s->arr[i].is_number()) scale[i] = (float)s->arr[i].num;
+    if (const Json *o = tt->get("offset"); o && o->type == Json::Type::Array)
+        for (size_t i = 0; i < 2 && i < o->arr.size(); ++i)
+            if (o->arr[i].is_number()) offset[i] = (float)o->arr[i].num;
+}
+
+// create_texture_shader_field (mesh_asset.cpp:445-463): the sampler is EWA when the glTF sampler's
+// minFilter mipmaps, else nearest (a texture without a sampler counts as one without minFilter); wraps
+// repeat and flip_v is off, as the reference builds them.
+ks_field_desc texture_field(const Model &m, GltfTextures &gt, const Json &info, int n, const float *scale,
+                            int swizzle0) {
+    ks_field_desc f{};
+    const Json &tex = m.element("textures", info.get("index")->num);
+    const Json *src = tex.get("source");
+    if (!src) throw ConfigError(m.src + ": texture without source image");
+    (void)m.element("images", src->num); // range check
+    f.texture = gt.texture((size_t)src->num);
+    int min_filter = -1;
+    if (const Json *si = tex.get("sampler")) min_filter = integer(m.element("samplers", si->num), "minFilter", -1);
+    const bool mip = min_filter == 9984 || min_filter == 9985 || min_filter == 9986 || min_filter == 9987;
+    f.sampler = mip ? KS_SAMPLER_EWA : KS_SAMPLER_NEAREST;
+    f.anisotropy = 8.0f;
+    f.wrap_u = f.wrap_v = KS_WRAP_REPEAT;
+    f.flip_v = 0;
+    for (int k = 0; k < 4; ++k) {
+        f.swizzle[k] = k < n ? (swizzle0 >= 0 ? swizzle0 + k : k) : -1;
+        f.scale[k] = k < n ? scale[k] : 1.0f;
+    }
+    texture_transform(info, f.uv_scale, f.uv_offset);
+    return f;
+}
+
+ks_field_desc constant_field(const float *v, int n) {
+    ks_field_desc f{};
+    f.texture = -1;
+    for (int k = 0; k < n; ++k) f.value[k] = v[k];
+    return f;
+}
+
+// One glTF material as a principled BSDF (mesh_asset.cpp:650-956): constant factors in `d`, and with
+// textures every closure input as a field in `fields` (`textured` set); `twosided` is cleared for
+// KHR_materials_transmission. alphaMode MASK or BLEND gives an opacity map (alpha of the base colour
+// texture times its factor, or the factor), tested stochastically.
+ks_material_desc convert_material(const Model &m, const Json *mat, const GltfLoadOptions &opt, bool &twosided,
+                                  GltfTextures &gt, ks_material_fields &fields, bool &textured) {
     static const Json empty{Json::Type::Object};
     const Json &src = mat ? *mat : empty;
     const Json *pbr_p = src.get("pbrMetallicRoughness");
     const Json &pbr = pbr_p ? *pbr_p : empty;
     const Json *ext_p = src.get("extensions");
     const Json &ext = ext_p ? *ext_p : empty;
-    std::string name;
-    if (const Json *n = src.get("name"); n && n->type == Json::Type::String) name = n->str;
-    auto unsupported = [&](const std::string &what) {
-        if (!opt.ignore_textures)
-            throw ConfigError(m.src + ": material '" + name + "' uses " + what +
-                              ", which the flat material record cannot hold (set ignore_textures = true to keep only "
-                              "the constant factors)");
+    const bool tex_ok = !opt.ignore_textures;
+    auto tex = [&](const Json &o, const char *key) -> const Json * {
+        return (tex_ok && has_texture(o, key)) ? o.get(key) : nullptr;
     };
-    if (has_texture(pbr, "baseColorTexture")) unsupported("a base color texture");
-    if (has_texture(pbr, "metallicRoughnessTexture")) unsupported("a metallic-roughness texture");
-    if (has_texture(src, "emissiveTexture")) unsupported("an emissive texture");
-    if (has_texture(src, "normalTexture")) unsupported("a normal map");
+    fields = ks_material_fields{};
+    textured = false;
 
     ks_material_desc d{};
-    double bc[4] = {1, 1, 1, 1};
+    float bc[4] = {1, 1, 1, 1};
     if (const Json *f = pbr.get("baseColorFactor"); f && f->type == Json::Type::Array && f->arr.size() == 4)
-        for (int k = 0; k < 4; ++k) bc[k] = f->arr[(size_t)k].num;
-    for (int k = 0; k < 3; ++k) d.basecolor[k] = (float)bc[k];
+        for (int k = 0; k < 4; ++k) bc[k] = (float)f->arr[(size_t)k].num;
+    for (int k = 0; k < 3; ++k) d.basecolor[k] = bc[k];
+    if (const Json *t = tex(pbr, "baseColorTexture")) fields.basecolor = texture_field(m, gt, *t, 3, bc, -1), textured = true;
+    else fields.basecolor = constant_field(bc, 3);
     const Json *alpha_mode = src.get("alphaMode");
-    if (alpha_mode && alpha_mode->type == Json::Type::String && alpha_mode->str != "OPAQUE" &&
-        ((float)bc[3] != 1.0f || has_texture(pbr, "baseColorTexture")))
-        unsupported("alpha mode " + alpha_mode->str + " with partial opacity");
+    if (alpha_mode && alpha_mode->type == Json::Type::String && alpha_mode->str != "OPAQUE") {
+        fields.has_opacity_map = 1;
+        textured = true;
+        if (const Json *t = tex(pbr, "baseColorTexture")) fields.opacity = texture_field(m, gt, *t, 1, &bc[3], 3);
+        else fields.opacity = constant_field(&bc[3], 1);
+    }
     d.roughness = (float)num(pbr, "roughnessFactor", 1.0);
     d.metallic = (float)num(pbr, "metallicFactor", 1.0);
+    if (const Json *t = tex(pbr, "metallicRoughnessTexture")) { // roughness in G, metallic in B
+        fields.roughness = texture_field(m, gt, *t, 1, &d.roughness, 1);
+        fields.metallic = texture_field(m, gt, *t, 1, &d.metallic, 2);
+        textured = true;
+    } else {
+        fields.roughness = constant_field(&d.roughness, 1);
+        fields.metallic = constant_field(&d.metallic, 1);
+    }
 
     float strength = 1.0f; // KHR_materials_emissive_strength
     if (const Json *e = ext.get("KHR_materials_emissive_strength")) strength = (float)num(*e, "emissiveStrength", 1.0);
     double ef[3] = {0, 0, 0};
     if (const Json *f = src.get("emissiveFactor"); f && f->type == Json::Type::Array && f->arr.size() == 3)
         for (int k = 0; k < 3; ++k) ef[k] = f->arr[(size_t)k].num;
-    for (int k = 0; k < 3; ++k) d.emission[k] = strength * (float)ef[k];
+    float em[3];
+    for (int k = 0; k < 3; ++k) em[k] = strength * (float)ef[k];
+    if (const Json *t = tex(src, "emissiveTexture")) fields.emission = texture_field(m, gt, *t, 3, em, -1), textured = true;
+    else fields.emission = constant_field(em, 3);
     d.has_emission = strength > 0.0f && ((float)ef[0] > 0.0f || (float)ef[1] > 0.0f || (float)ef[2] > 0.0f);
-    if (!d.has_emission) d.emission[0] = d.emission[1] = d.emission[2] = 0.0f;
+    fields.has_emission = d.has_emission;
+    for (int k = 0; k < 3; ++k) d.emission[k] = d.has_emission ? em[k] : 0.0f;
 
     d.ior = 1.5f; // KHR_materials_ior
     if (const Json *e = ext.get("KHR_materials_ior")) d.ior = (float)num(*e, "ior", 1.5);
+    fields.ior = constant_field(&d.ior, 1);
 
     float sf = 0.5f; // KHR_materials_specular: specularFactor times the mean of specularColorFactor
+    const Json *spec_tex = nullptr;
     if (const Json *e = ext.get("KHR_materials_specular")) {
         const Json *f = e->get("specularFactor");
         const Json *cf = e->get("specularColorFactor");
@@ -564,18 +688,35 @@ ks_material_desc convert_material(const Model &m, const Json *mat, const GltfLoa
             if (f && f->is_number()) sf = (float)f->num;
             if (has_cf) sf *= (float)((cf->arr[0].num + cf->arr[1].num + cf->arr[2].num) / 3.0f);
         }
-        if (has_texture(*e, "specularTexture")) unsupported("a specular texture");
+        spec_tex = tex(*e, "specularTexture");
     }
     d.specular_r0_mul = sf;
+    if (spec_tex) fields.specular_r0_mul = texture_field(m, gt, *spec_tex, 1, &sf, 0), textured = true;
+    else fields.specular_r0_mul = constant_field(&sf, 1);
 
     d.specular_trans = 0.0f; // KHR_materials_transmission (single-sided)
+    fields.specular_trans = constant_field(&d.specular_trans, 1);
     if (const Json *e = ext.get("KHR_materials_transmission")) {
         twosided = false;
         d.specular_trans = (float)num(*e, "transmissionFactor", 0.0);
-        if (has_texture(*e, "transmissionTexture")) unsupported("a transmission texture");
+        if (const Json *t = tex(*e, "transmissionTexture"))
+            fields.specular_trans = texture_field(m, gt, *t, 1, &d.specular_trans, 0), textured = true;
+        else fields.specular_trans = constant_field(&d.specular_trans, 1);
     }
     d.diffuse_trans = 0.0f;
     d.diffuse_trans_fwd = 0.0f;
+    fields.diffuse_trans = constant_field(&d.diffuse_trans, 1);
+    fields.diffuse_trans_fwd = constant_field(&d.diffuse_trans_fwd, 1);
+    if (const Json *t = tex(src, "normalTexture")) { // NormalMap with strength = normalTexture.scale
+        const float one[3] = {1, 1, 1};
+        fields.has_normal_map = 1;
+        fields.normal_map = texture_field(m, gt, *t, 3, one, -1);
+        fields.normal_space = KS_NORMAL_MAP_TANGENT;
+        fields.normal_range01 = 1;
+        fields.normal_strength = (float)num(*t, "scale", 1.0);
+        for (int k = 0; k < 9; ++k) fields.normal_to_world[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+        textured = true;
+    }
     return d;
 }
 
@@ -588,6 +729,25 @@ CompoundMeshAsset load_gltf(const fs::path &path, const GltfLoadOptions &opt) {
     const Json &meshes = m.array("meshes");
     out.prototypes.resize(meshes.arr.size());
     out.materials.resize(meshes.arr.size());
+    out.fields.resize(meshes.arr.size());
+    out.textured.resize(meshes.arr.size());
+    // colour spaces of the images: sRGB where a baseColor or emissive texture reads them
+    GltfTextures gt;
+    gt.m = &m;
+    gt.dir = path.parent_path();
+    gt.out = &out.textures;
+    const size_t n_images = m.array("images").arr.size();
+    gt.color_space.assign(n_images, KS_COLORSPACE_LINEAR);
+    gt.loaded.assign(n_images, -1);
+    for (const Json &mat : m.array("materials").arr) {
+        auto mark = [&](const Json *info) {
+            if (!info || info->type != Json::Type::Object || !info->get("index")) return;
+            const Json *srci = m.element("textures", info->get("index")->num).get("source");
+            if (srci && srci->num >= 0 && (size_t)srci->num < n_images) gt.color_space[(size_t)srci->num] = KS_COLORSPACE_SRGB;
+        };
+        if (const Json *pbr = mat.get("pbrMetallicRoughness")) mark(pbr->get("baseColorTexture"));
+        mark(mat.get("emissiveTexture"));
+    }
     for (size_t i = 0; i < meshes.arr.size(); ++i) {
         const Json *prims = meshes.arr[i].get("primitives");
         if (!prims || prims->type != Json::Type::Array) throw ConfigError(m.src + ": mesh without primitives");
@@ -638,8 +798,11 @@ CompoundMeshAsset load_gltf(const fs::path &path, const GltfLoadOptions &opt) {
             if (opt.load_materials) {
                 const Json *mat = nullptr;
                 if (const Json *mi = prim.get("material")) mat = &m.element("materials", mi->num);
-                bool ts = true;
-                out.materials[i].push_back(convert_material(m, mat, opt, ts));
+                bool ts = true, textured = false;
+                ks_material_fields f;
+                out.materials[i].push_back(convert_material(m, mat, opt, ts, gt, f, textured));
+                out.fields[i].push_back(f);
+                out.textured[i].push_back(textured ? 1 : 0);
                 md.twosided = ts;
             }
             out.prototypes[i].meshes.push_back(std::move(md));

@@ -9,6 +9,7 @@
 #pragma once
 #include "../../include/keepsake_amd.h"
 #include "mesh_asset.h"
+#include "texture_asset.h"
 
 #include <array>
 #include <string>
@@ -26,6 +27,11 @@ struct GltfLoadOptions {
 struct CompoundMeshAsset {
     std::vector<MeshAsset> prototypes;                    // per glTF mesh, one MeshData per primitive
     std::vector<std::vector<ks_material_desc>> materials; // per prototype and primitive (load_materials)
+    // textured materials (load_materials): fields per prototype and primitive, used where `textured` is
+    // set; their texture indices point into `textures`, one per glTF image
+    std::vector<std::vector<ks_material_fields>> fields;
+    std::vector<std::vector<uint8_t>> textured;
+    std::vector<TextureData> textures;
     std::vector<std::pair<uint32_t, std::array<float, 16>>> instances; // (prototype, row-major to_world)
     bool has_materials = false;
 };

@@ -254,11 +254,25 @@ static SceneData scene_from_compound(const CompoundMeshAsset &asset, const Confi
                 mp.push_back(b.named_material(args, names[(int)g].node));
         }
     }
+    // the glTF's textures join the scene's after any the TOML materials loaded
+    int tex_base = -1;
+    auto gltf_material = [&](uint32_t p, size_t g) {
+        if (!asset.textured[p][g]) return b.add_material(asset.materials[p][g]);
+        if (tex_base < 0) {
+            tex_base = (int)b.textures().textures.size();
+            for (const TextureData &t : asset.textures) b.textures().textures.push_back(t);
+        }
+        ks_material_fields f = asset.fields[p][g];
+        for (ks_field_desc *fd : {&f.basecolor, &f.roughness, &f.metallic, &f.ior, &f.specular_r0_mul, &f.specular_trans,
+                                  &f.diffuse_trans, &f.diffuse_trans_fwd, &f.emission, &f.normal_map, &f.opacity})
+            if (fd->texture >= 0) fd->texture += tex_base;
+        return b.add_material(asset.materials[p][g], &f);
+    };
     for (const auto &inst : asset.instances) {
         const MeshAsset &proto = asset.prototypes[inst.first];
         auto &mp = mat[inst.first];
         if (mp.empty())
-            for (size_t g = 0; g < proto.meshes.size(); ++g) mp.push_back(b.add_material(asset.materials[inst.first][g]));
+            for (size_t g = 0; g < proto.meshes.size(); ++g) mp.push_back(gltf_material(inst.first, g));
         bool identity = true;
         for (int k = 0; k < 16; ++k) identity &= inst.second[k] == ((k % 5 == 0) ? 1.0f : 0.0f);
         for (size_t g = 0; g < proto.meshes.size(); ++g) {

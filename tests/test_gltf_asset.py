@@ -262,23 +262,100 @@ def test_toml_material_list_overrides_gltf(tmp_path):
     assert list(s["geom_flags"]) == [KS_GEOM_ONE_SIDED | KS_GEOM_FLAT]  # the transmission mesh stays one-sided
 
 
-@pytest.mark.parametrize(
-    "mat,msg",
-    [
-        ({"pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}}, "a base color texture"),
-        ({"normalTexture": {"index": 0}}, "a normal map"),
-        ({"emissiveTexture": {"index": 0}, "emissiveFactor": [1, 1, 1]}, "an emissive texture"),
-        ({"alphaMode": "BLEND", "pbrMetallicRoughness": {"baseColorFactor": [1, 1, 1, 0.5]}}, "partial opacity"),
-    ],
-)
-def test_textures_rejected_unless_ignored(tmp_path, mat, msg):
-    _material_scene(**mat).write_gltf(tmp_path / "m.gltf")
-    r = run(tmp_path, compound("m", tmp_path / "m.gltf", load_materials="true") + task("compound_mesh_asset.m"),
-            "--print-config")
-    assert r.returncode != 0 and msg in r.stderr
+def _textured_gltf(tmp_path):
+    """A triangle with TEXCOORD_0 and a material using every texture slot: images as a file, a data URI
+    and a buffer view (8-bit RGB, 16-bit gray, 8-bit RGBA), samplers with and without mipmap filtering."""
+    import image_files as imf
+
+    g = Gltf()
+    rgb = np.random.default_rng(0).integers(0, 256, (4, 5, 3)).astype(np.uint8)
+    gray16 = np.random.default_rng(1).integers(0, 65536, (3, 3, 1)).astype(np.uint16)
+    rgba = np.random.default_rng(2).integers(0, 256, (2, 2, 4)).astype(np.uint8)
+    imf.write_png_any(tmp_path / "rgb.png", rgb, 2, 8)
+    imf.write_png_any(tmp_path / "g16.png", gray16, 0, 16)
+    imf.write_png_any(tmp_path / "rgba.png", rgba, 6, 8)
+    while len(g.blob) % 4:
+        g.blob.append(0)
+    off = len(g.blob)
+    png = (tmp_path / "rgba.png").read_bytes()
+    g.blob += png
+    g.doc["bufferViews"].append({"buffer": 0, "byteOffset": off, "byteLength": len(png)})
+    data_uri = "data:image/png;base64," + base64.b64encode((tmp_path / "g16.png").read_bytes()).decode()
+    g.doc["images"] = [{"uri": "rgb.png"}, {"uri": data_uri}, {"bufferView": len(g.doc["bufferViews"]) - 1,
+                                                              "mimeType": "image/png"}]
+    g.doc["samplers"] = [{"minFilter": 9987}, {"minFilter": 9729}]
+    g.doc["textures"] = [{"source": 0, "sampler": 0}, {"source": 1, "sampler": 1}, {"source": 2}]
+    tt = {"extensions": {"KHR_texture_transform": {"scale": [2.0, 3.0], "offset": [0.25, 0.5]}}}
+    m = g.material(
+        pbrMetallicRoughness={"baseColorTexture": {"index": 2, **tt}, "baseColorFactor": [0.5, 0.6, 0.7, 0.8],
+                              "metallicRoughnessTexture": {"index": 1}, "metallicFactor": 0.9, "roughnessFactor": 0.4},
+        normalTexture={"index": 0, "scale": 0.7}, emissiveTexture={"index": 0}, emissiveFactor=[1.0, 0.5, 0.25],
+        alphaMode="BLEND",
+        extensions={"KHR_materials_specular": {"specularFactor": 0.8, "specularTexture": {"index": 1}},
+                    "KHR_materials_transmission": {"transmissionFactor": 0.3, "transmissionTexture": {"index": 2}}})
+    p = g.primitive(TRI, [0, 1, 2], material=m)
+    p["attributes"]["TEXCOORD_0"] = g.accessor(np.float32([[0, 0], [1, 0], [0, 1]]), 5126, "VEC2")
+    g.mesh([p])
+    g.node(root=True, mesh=0)
+    g.write_gltf(tmp_path / "m.gltf")
+    return rgb, gray16, rgba
+
+
+def test_gltf_textures(tmp_path):
+    import ctypes as C
+
+    from keepsake_amd import api
+
+    rgb, gray16, rgba = _textured_gltf(tmp_path)
+    info, s = dump(tmp_path, compound("m", tmp_path / "m.gltf", load_materials="true") + task("compound_mesh_asset.m"))
+    d = tmp_path / "dump" / "task_0"
+    assert np.fromfile(d / "field_of.i32", np.int32).tolist() == [0]
+    f = api.MaterialFields.from_buffer_copy((d / "fields.bin").read_bytes())
+    np.testing.assert_array_equal(np.fromfile(d / "texcoords.f32", np.float32).reshape(-1, 2), [[0, 0], [1, 0], [0, 1]])
+    lines = [list(map(int, l.split())) for l in (d / "textures.txt").read_text().split("\n") if l.strip()]
+    # images in first-use order: rgba (base colour), gray16 (metallic-roughness), rgb (normal / emissive)
+    assert lines == [[2, 2, 4, api.KS_TEX_U8, api.KS_COLORSPACE_SRGB, 1, 1],
+                     [3, 3, 4, api.KS_TEX_U16, api.KS_COLORSPACE_LINEAR, 1, 1],
+                     [5, 4, 4, api.KS_TEX_U8, api.KS_COLORSPACE_SRGB, 1, 1]]
+    np.testing.assert_array_equal(np.fromfile(d / "texture_0.bin", np.uint8).reshape(2, 2, 4), rgba)
+    g16 = np.fromfile(d / "texture_1.bin", np.uint16).reshape(3, 3, 4)
+    np.testing.assert_array_equal(g16[..., :3], np.repeat(gray16, 3, -1))
+    assert (g16[..., 3] == 65535).all()
+    t2 = np.fromfile(d / "texture_2.bin", np.uint8).reshape(4, 5, 4)
+    np.testing.assert_array_equal(t2[..., :3], rgb)
+    assert (t2[..., 3] == 255).all()
+    bc = f.basecolor
+    assert bc.texture == 0 and bc.sampler == api.KS_SAMPLER_NEAREST and bc.flip_v == 0  # texture 2 has no sampler
+    np.testing.assert_allclose(list(bc.scale)[:3], [0.5, 0.6, 0.7])
+    assert list(bc.uv_scale) == [2.0, 3.0] and list(bc.uv_offset) == [0.25, 0.5]
+    assert f.has_opacity_map == 1 and f.opacity.texture == 0 and f.opacity.swizzle[0] == 3
+    assert f.opacity.scale[0] == np.float32(0.8)
+    assert (f.roughness.texture, f.roughness.swizzle[0], f.metallic.swizzle[0]) == (1, 1, 2)
+    assert f.roughness.sampler == api.KS_SAMPLER_NEAREST  # minFilter LINEAR: no mipmaps
+    assert f.roughness.scale[0] == np.float32(0.4) and f.metallic.scale[0] == np.float32(0.9)
+    assert f.has_normal_map == 1 and f.normal_map.texture == 2 and f.normal_map.sampler == api.KS_SAMPLER_EWA
+    assert f.normal_strength == np.float32(0.7) and f.normal_range01 == 1
+    assert f.has_emission == 1 and f.emission.texture == 2 and list(f.emission.scale)[:3] == [1.0, 0.5, 0.25]
+    assert f.specular_r0_mul.texture == 1 and f.specular_r0_mul.scale[0] == np.float32(0.8)
+    assert f.specular_trans.texture == 0 and f.specular_trans.scale[0] == np.float32(0.3)
+    assert list(np.fromfile(d / "geom_flags.u8", np.uint8)) == [KS_GEOM_ONE_SIDED | KS_GEOM_FLAT]
+    # ignore_textures keeps the constant factors only
     info, s = dump(tmp_path, compound("m", tmp_path / "m.gltf", load_materials="true", ignore_textures="true")
                    + task("compound_mesh_asset.m"))
-    assert info["n_materials"] == 1
+    assert np.fromfile(tmp_path / "dump" / "task_0" / "field_of.i32", np.int32).tolist() == [0]
+    f2 = api.MaterialFields.from_buffer_copy((tmp_path / "dump" / "task_0" / "fields.bin").read_bytes())
+    assert f2.basecolor.texture == -1 and f2.has_opacity_map == 1 and f2.opacity.texture == -1  # alpha stays
+    assert (tmp_path / "dump" / "task_0" / "textures.txt").read_text() == ""
+
+
+def test_gltf_jpeg_image_is_an_error(tmp_path):
+    g = _material_scene(pbrMetallicRoughness={"baseColorTexture": {"index": 0}})
+    g.doc["images"] = [{"uri": "data:image/jpeg;base64," + base64.b64encode(b"\xff\xd8\xff\xe0 not decoded").decode()}]
+    g.doc["textures"] = [{"source": 0}]
+    g.write_gltf(tmp_path / "m.gltf")
+    r = run(tmp_path, compound("m", tmp_path / "m.gltf", load_materials="true") + task("compound_mesh_asset.m"),
+            "--print-config")
+    assert r.returncode != 0 and "only PNG images" in r.stderr
 
 
 def _broken(kind, tmp_path):
