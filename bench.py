@@ -71,14 +71,51 @@ def cpu_baseline(hs, cam, kw, rows: int, threads: int, budget_s: float) -> dict:
     }
 
 
+BUILDERS = {"host": 0, "device": 1}
+BVH_NAME = {0: "host SBVH", 1: "device-built linear BVH"}
+
+
+def device_bvh_leg(ks, dev, hs, cam, params, args, host_info) -> dict:
+    """Builds the benchmark scene's BVH with the device builder and renders over it: build time against
+    the host SBVH's, and the render rate over each tree."""
+    builds = []
+    for _ in range(3):
+        sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": BUILDERS["device"]})
+        info = sc.info()
+        builds.append(info["bvh_build_ms"])
+        if len(builds) < 3:
+            sc.close()
+    film = ks.Film(dev, args.width, args.height)
+    ks.small_pt_render(sc, cam, params, film)  # warm-up
+    times = []
+    for _ in range(2):
+        film.clear()
+        t0 = time.perf_counter()
+        ks.small_pt_render(sc, cam, params, film)
+        times.append(time.perf_counter() - t0)
+    film.close()
+    sc.close()
+    paths = args.width * args.height * args.spp
+    out = {
+        "build_ms": round(min(builds), 2),
+        "host_sbvh_build_ms": round(host_info["bvh_build_ms"], 1),
+        "nodes": info["n_nodes"],
+        "depth": info["max_depth"],
+        "host_sbvh_nodes": host_info["n_nodes"],
+        "mpaths_per_s": round(paths / min(times) / 1e6, 2),
+    }
+    log(f"device BVH: {out}")
+    return out
+
+
 def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, max_spp=None, pass_spp=None,
-                   methods=(("guided", 0),)) -> dict:
+                   methods=(("guided", 0),), builder=0) -> dict:
     """Wall time for the plain and the guided tracer to bring a scene (16 bounces, consistent MIS, no
     clamping) to relMSE <= target against an FP64 plain reference of ref_spp. methods: (name, fp8) of the
     guided runs."""
     hs = ks.procedural_scene(*scene)
     t0 = time.perf_counter()
-    sc = ks.Scene(dev, hs)
+    sc = ks.Scene(dev, hs, bvh={"builder": builder})
     build_s = time.perf_counter() - t0
     info = sc.info()
     cam = dict(hs.camera)
@@ -201,6 +238,11 @@ def main() -> int:
     ap.add_argument("--bounces", type=int, default=12)
     ap.add_argument("--queue", type=int, default=0)
     ap.add_argument("--branching", type=int, default=4, choices=(2, 4), help="BVH node width on the device")
+    ap.add_argument("--builder", default="host", choices=("host", "device"),
+                    help="BVH of the timed renders: the host SBVH or the device-built linear BVH")
+    ap.add_argument("--no-device-bvh", action="store_true",
+                    help="skip the leg that builds the scene's BVH on the device and renders over it")
+    ap.add_argument("--scale-builder", default="host", choices=("host", "device"))
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -260,10 +302,11 @@ def main() -> int:
     log(f"scene {args.scene}: {hs.n_tris} triangles (generated in {gen_s:.1f} s)")
     dev = ks.Device(0)
     t0 = time.time()
-    sc = ks.Scene(dev, hs, bvh={"branching": args.branching})
+    builder = BUILDERS[args.builder]
+    sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": builder})
     info = sc.info()
-    log(f"SBVH: {info['n_nodes']} BVH{info['branching']} nodes, {info['n_refs']} refs, depth {info['max_depth']}, "
-        f"build {info['build_ms'] / 1e3:.1f} s")
+    log(f"{BVH_NAME[builder]}: {info['n_nodes']} BVH{info['branching']} nodes, {info['n_refs']} refs, depth "
+        f"{info['max_depth']}, BVH build {info['bvh_build_ms'] / 1e3:.2f} s, scene {info['build_ms'] / 1e3:.1f} s")
     cam = hs.camera
     kw = dict(
         bounces=args.bounces,
@@ -321,10 +364,14 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "kernel": f"k_closest (closest-hit SBVH traversal, BVH{info['branching']} nodes)",
+        "kernel": f"k_closest (closest-hit traversal, {BVH_NAME[builder]}, BVH{info['branching']} nodes)",
         "bytes_per_launch": alg_bytes / max(launches, 1),
         "avg_launch_ms": k_ms / max(launches, 1),
     }
+
+    device_bvh = None
+    if not args.no_device_bvh and builder == BUILDERS["host"]:
+        device_bvh = device_bvh_leg(ks, dev, hs, cam, params, args, info)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -350,7 +397,7 @@ def main() -> int:
         try:
             scale = time_to_relmse(ks, dev, args, scene=("room_spheres", 4), size=(args.scale_width, args.scale_height),
                                    ref_spp=args.scale_ref_spp, max_spp=args.scale_max_spp,
-                                   methods=(("guided_fp8", 1),))
+                                   methods=(("guided_fp8", 1),), builder=BUILDERS[args.scale_builder])
         except ks.KsError as e:
             scale = {"error": str(e)}
 
@@ -376,7 +423,7 @@ def main() -> int:
             "spp": args.spp,
             "bounces": args.bounces,
             "queue_size": st["queue_size"],
-            "bvh": f"SBVH collapsed to BVH{info['branching']}",
+            "bvh": f"{BVH_NAME[builder]}, BVH{info['branching']} nodes",
         },
         "mrays_per_s": round(n_rays / (elapsed / args.steps) / 1e6, 1),
         "kernel_ms_per_step": {k: round(v / args.steps, 2) for k, v in totals["kernel_ms"].items()},
@@ -391,6 +438,7 @@ def main() -> int:
         "time_to_relmse": ttr,
         "time_to_relmse_at_scale": scale,
         "batched_views": batch,
+        "device_bvh": device_bvh,
     }
     print(json.dumps(out), flush=True)
     film.close()

@@ -176,7 +176,14 @@ typedef struct ks_bvh_options {
     float alpha;
     int32_t threads; /* host build threads, 0 = all */
     int32_t branching; /* node width on the device: 4 (default; the SBVH collapsed to 128-byte nodes) or 2 */
+    int32_t builder;   /* KS_BVH_BUILD_HOST_SBVH (default) or KS_BVH_BUILD_DEVICE */
 } ks_bvh_options;
+
+/* BVH builders. HOST_SBVH: binned SAH with spatial splits on host threads (the reference's SBVH,
+ * cuda/sbvh.cu:301-440). DEVICE: a linear BVH over Morton codes built by kernels on the scene's device,
+ * with SAH leaf collapsing (no reference counterpart; the reference has no device BVH build). */
+#define KS_BVH_BUILD_HOST_SBVH 0
+#define KS_BVH_BUILD_DEVICE 1
 
 /* create_camera (camera.cpp:172-205) perspective branch: look_at(pos, target, up) with vfov in
  * degrees and aspect; when use_matrix != 0, to_world (row-major 4x4 camera-to-world) replaces look_at. */
@@ -263,8 +270,14 @@ typedef struct ks_scene_info {
     double build_ms;
     float bound_lo[3], bound_hi[3]; /* Scene::bound (scene.cpp:162-169) */
     uint32_t branching;             /* node width of the device BVH (n_nodes and max_depth refer to it) */
+    int32_t builder;                /* the KS_BVH_BUILD_* that built it */
+    double bvh_build_ms;            /* wall time of the BVH build alone (device builds: kernels included) */
 } ks_scene_info;
 int ks_scene_info_get(const ks_scene *scene, ks_scene_info *info);
+/* Copies the device BVH out: info.n_nodes nodes (128-byte BVH4 or 64-byte BVH2 records, layouts in
+ * keepsake_amd/csrc/ks_internal.h) and info.n_refs 48-byte triangle records in leaf order. For
+ * inspection and tests; the reference's Embree BVH is opaque (no counterpart). */
+int ks_scene_bvh_get(const ks_scene *scene, void *nodes, void *refs);
 /* PowerLightSampler tables in flat light order: prim (-1 for non-mesh), power, cdf (n+1). */
 int ks_scene_light_table(const ks_scene *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf);
 

@@ -232,7 +232,12 @@ class BvhOptions(C.Structure):
         ("alpha", C.c_float),
         ("threads", C.c_int32),
         ("branching", C.c_int32),
+        ("builder", C.c_int32),
     ]
+
+
+KS_BVH_BUILD_HOST_SBVH = 0
+KS_BVH_BUILD_DEVICE = 1
 
 
 class CameraDesc(C.Structure):
@@ -311,6 +316,8 @@ class SceneInfo(C.Structure):
         ("bound_lo", C.c_float * 3),
         ("bound_hi", C.c_float * 3),
         ("branching", C.c_uint32),
+        ("builder", C.c_int32),
+        ("bvh_build_ms", C.c_double),
     ]
 
 
@@ -366,6 +373,7 @@ _SIGS = {
     "ks_scene_free": ([_P], C.c_int),
     "ks_scene_info_get": ([_P, C.POINTER(SceneInfo)], C.c_int),
     "ks_scene_light_table": ([_P, C.POINTER(C.c_int32), _P, _P, _P], C.c_int),
+    "ks_scene_bvh_get": ([_P, _P, _P], C.c_int),
     "ks_film_create": ([_P, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(_P)], C.c_int),
     "ks_film_clear": ([_P], C.c_int),
     "ks_film_download": ([_P, _P, _P], C.c_int),
@@ -765,6 +773,15 @@ class Scene:
         d["bound_lo"] = tuple(i.bound_lo)
         d["bound_hi"] = tuple(i.bound_hi)
         return d
+
+    def bvh(self):
+        """The device BVH: (nodes, refs) as uint32 arrays of 32 (BVH4) or 16 (BVH2) words per node and
+        12 words per triangle record."""
+        i = self.info()
+        nodes = np.zeros((i["n_nodes"], 32 if i["branching"] == 4 else 16), np.uint32)
+        refs = np.zeros((i["n_refs"], 12), np.uint32)
+        _check(self.lib.ks_scene_bvh_get(self.h, _ptr(nodes), _ptr(refs)))
+        return nodes, refs
 
     def light_table(self):
         n = C.c_int32(0)

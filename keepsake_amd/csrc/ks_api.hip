@@ -1916,12 +1916,21 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     sc->dev = dev;
     ks_bvh_options o{};
     if (opt) o = *opt;
+    KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE, "unknown BVH builder");
+    const int width = o.branching == 2 ? 2 : 4;
+    // an empty scene gets the host builder's single empty node whichever builder was asked for
+    const bool device_build = o.builder == KS_BVH_BUILD_DEVICE && d->n_tris > 0;
     ksa::BvhResult bvh;
-    try {
-        bvh = ksa::build_sbvh(d->vertices, d->indices, d->n_tris, o);
-    } catch (const std::exception &ex) {
-        ksa::set_error(std::string("BVH build failed: ") + ex.what());
-        return KS_ERR_INVALID;
+    double host_bvh_ms = 0.0;
+    if (!device_build) {
+        try {
+            const auto tb = std::chrono::steady_clock::now();
+            bvh = ksa::build_sbvh(d->vertices, d->indices, d->n_tris, o);
+            host_bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
+        } catch (const std::exception &ex) {
+            ksa::set_error(std::string("BVH build failed: ") + ex.what());
+            return KS_ERR_INVALID;
+        }
     }
 
     HostTextures ht;
@@ -1944,9 +1953,38 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     std::vector<Material> mats(d->n_materials);
     for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(material_of(m));
 
+    hipStream_t s = dev->stream;
+    float scene_lo[3], scene_hi[3];
+    ksa::DeviceBvh dbvh;
+    if (device_build) {
+        KS_HIP(upload(sc->verts, verts, s));
+        KS_HIP(upload(sc->tri_idx, tri_idx, s));
+        DevBuf alpha_of;
+        if (ht.alpha) {
+            std::vector<uint8_t> flags(d->n_materials, 0);
+            for (uint32_t m = 0; m < d->n_materials; ++m)
+                flags[m] = ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity ? 1 : 0;
+            KS_HIP(upload(alpha_of, flags, s));
+        }
+        std::string err;
+        if (ksa::build_bvh_device(sc->verts.p, sc->tri_idx.p, d->n_tris, ht.alpha ? alpha_of.as<uint8_t>() : nullptr,
+                                  o, width, s, dbvh, err) != 0) {
+            ksa::set_error(err);
+            return KS_ERR_HIP;
+        }
+        // the scene takes the two arrays over
+        sc->nodes.p = dbvh.nodes;
+        sc->nodes.bytes = dbvh.node_bytes;
+        sc->tris.p = dbvh.tris;
+        sc->tris.bytes = dbvh.tri_bytes;
+        for (int k = 0; k < 3; ++k) scene_lo[k] = dbvh.lo[k], scene_hi[k] = dbvh.hi[k];
+    } else {
+        for (int k = 0; k < 3; ++k) scene_lo[k] = bvh.lo[k], scene_hi[k] = bvh.hi[k];
+    }
+
     // Light list in PowerLightSampler order (light.cpp:636-677): non-mesh lights, then per emissive
     // geometry the triangles with non-zero importance after swap compaction (light.cpp:524-558).
-    float ext[3] = {bvh.hi[0] - bvh.lo[0], bvh.hi[1] - bvh.lo[1], bvh.hi[2] - bvh.lo[2]};
+    float ext[3] = {scene_hi[0] - scene_lo[0], scene_hi[1] - scene_lo[1], scene_hi[2] - scene_lo[2]};
     float scene_radius = 0.5f * std::sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
     const float pi = 3.14159265359f, two_pi = 6.28318530718f;
     std::vector<Light> lights;
@@ -2055,31 +2093,31 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     sc->light_power = powers;
     sc->light_cdf = cdf;
 
-    hipStream_t s = dev->stream;
-    const int width = o.branching == 2 ? 2 : 4;
     // BVH4 visits address nodes with 32-bit byte offsets
-    KS_REQUIRE(width != 4 || bvh.nodes4.size() < (1u << 25),
-               "BVH4 node array exceeds 4 GiB; build this scene with branching = 2");
-    if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
-    else KS_HIP(upload(sc->nodes, bvh.nodes, s));
-    if (ht.alpha) { // alpha-tested triangle records carry a non-zero v1.w for the traversal filter
-        for (auto &g : bvh.tris) {
-            const uint32_t m = d->geom_material[d->tri_geom[g.tri]];
-            if (ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity) {
-                const uint32_t one = 1u;
-                std::memcpy(&g.v1[3], &one, 4);
+    const size_t n_nodes = device_build ? dbvh.n_nodes : width == 4 ? bvh.nodes4.size() : bvh.nodes.size();
+    KS_REQUIRE(width != 4 || n_nodes < (1u << 25), "BVH4 node array exceeds 4 GiB; build this scene with branching = 2");
+    if (!device_build) {
+        if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
+        else KS_HIP(upload(sc->nodes, bvh.nodes, s));
+        if (ht.alpha) { // alpha-tested triangle records carry a non-zero v1.w for the traversal filter
+            for (auto &g : bvh.tris) {
+                const uint32_t m = d->geom_material[d->tri_geom[g.tri]];
+                if (ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity) {
+                    const uint32_t one = 1u;
+                    std::memcpy(&g.v1[3], &one, 4);
+                }
             }
         }
+        KS_HIP(upload(sc->tris, bvh.tris, s));
+        KS_HIP(upload(sc->verts, verts, s));
     }
-    KS_HIP(upload(sc->tris, bvh.tris, s));
-    KS_HIP(upload(sc->verts, verts, s));
     if (d->normals) {
         std::vector<float4> nrm(d->n_vertices);
         for (uint32_t i = 0; i < d->n_vertices; ++i)
             nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
         KS_HIP(upload(sc->normals, nrm, s));
     }
-    KS_HIP(upload(sc->tri_idx, tri_idx, s));
+    if (!device_build) KS_HIP(upload(sc->tri_idx, tri_idx, s));
     KS_HIP(upload(sc->tri_light, tri_light, s));
     KS_HIP(upload(sc->mats, mats, s));
     KS_HIP(upload(sc->lights, lights, s));
@@ -2132,19 +2170,21 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     ks_scene_info &in = sc->info;
     in.n_tris = d->n_tris;
     in.n_vertices = d->n_vertices;
-    in.n_nodes = (uint32_t)(width == 4 ? bvh.nodes4.size() : bvh.nodes.size());
+    in.n_nodes = (uint32_t)n_nodes;
     in.branching = (uint32_t)width;
-    in.n_refs = (uint32_t)bvh.tris.size();
+    in.n_refs = device_build ? dbvh.n_refs : (uint32_t)bvh.tris.size();
     in.n_lights = (uint32_t)lights.size();
-    in.max_depth = width == 4 ? bvh.max_depth4 : bvh.max_depth;
+    in.max_depth = device_build ? dbvh.max_depth : width == 4 ? bvh.max_depth4 : bvh.max_depth;
+    in.builder = device_build ? KS_BVH_BUILD_DEVICE : KS_BVH_BUILD_HOST_SBVH;
+    in.bvh_build_ms = device_build ? dbvh.build_ms : host_bvh_ms;
     // traversal stack bound: every interior node on the path leaves at most width - 1 entries behind
     sc->max_stack = (uint32_t)(width - 1) * in.max_depth + 1;
     in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->verts.bytes + sc->normals.bytes + sc->tri_idx.bytes +
                       sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes + sc->tex_pool.bytes +
                       sc->tex_mats.bytes + sc->uvs.bytes;
     for (int k = 0; k < 3; ++k) {
-        in.bound_lo[k] = bvh.lo[k];
-        in.bound_hi[k] = bvh.hi[k];
+        in.bound_lo[k] = scene_lo[k];
+        in.bound_hi[k] = scene_hi[k];
     }
     in.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = sc.release();
@@ -2163,6 +2203,16 @@ int ks_scene_free(ks_scene *scene) {
 int ks_scene_info_get(const ks_scene *scene, ks_scene_info *info) {
     KS_REQUIRE(scene && info, "null argument");
     *info = scene->info;
+    return KS_OK;
+}
+
+int ks_scene_bvh_get(const ks_scene *scene, void *nodes, void *refs) {
+    KS_REQUIRE(scene && nodes && refs, "null argument");
+    KS_HIP(hipSetDevice(scene->dev->ordinal));
+    const size_t node_size = scene->info.branching == 4 ? sizeof(ksa::GpuNode4) : sizeof(ksa::GpuNode);
+    KS_HIP(hipStreamSynchronize(scene->dev->stream));
+    KS_HIP(hipMemcpy(nodes, scene->nodes.p, (size_t)scene->info.n_nodes * node_size, hipMemcpyDeviceToHost));
+    KS_HIP(hipMemcpy(refs, scene->tris.p, (size_t)scene->info.n_refs * sizeof(ksa::GpuTri), hipMemcpyDeviceToHost));
     return KS_OK;
 }
 

@@ -61,6 +61,21 @@ BvhResult build_sbvh(const float *vertices, const uint32_t *indices, uint32_t n_
 // opens its interior child of largest surface area until it holds four children.
 std::vector<GpuNode4> collapse_bvh4(const std::vector<GpuNode> &nodes, uint32_t &max_depth4);
 
+// A BVH built on the device (ks_bvh_device.hip): node and triangle-record arrays in the layouts above,
+// owned by the caller after a successful build (release with hipFree).
+struct DeviceBvh {
+    void *nodes = nullptr, *tris = nullptr;
+    size_t node_bytes = 0, tri_bytes = 0;
+    uint32_t n_nodes = 0, n_refs = 0, max_depth = 0;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double build_ms = 0.0;
+};
+// verts: float4 per vertex, tri_idx: uint4 per triangle (vertex ids, material id in the low 24 bits of
+// w), mat_alpha: per-material alpha-test flag or null, stream: the hipStream_t to build on. Returns 0 or
+// -1 with `err` set.
+int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n_tris, const uint8_t *mat_alpha,
+                     const ks_bvh_options &opt, int width, void *stream, DeviceBvh &out, std::string &err);
+
 struct HostScene {
     std::vector<float> vertices, normals;
     std::vector<uint32_t> indices, tri_geom, geom_material;
