@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -30,6 +31,11 @@ HBM_PEAK_GBS = 8000.0
 RAY_IO_BYTES = 48
 NODE_BYTES = {4: 112, 2: 64}
 TRI_BYTES = 48
+# Shading, per path vertex (one per closest-hit ray): read hit 20 + path state 64 + material 64 +
+# triangle 48 + light record 48 + light CDF 4 per binary-search step; write path state 64 + extension
+# ray 32. Per shadow ray: write the ray 32 + its contribution 12.
+SHADE_VERTEX_BYTES = 20 + 64 + 64 + 48 + 48 + 64 + 32
+SHADE_SHADOW_BYTES = 32 + 12
 
 METRIC = "time-to-relMSE 0.01 guided vs plain, Mpaths/s, %HBM peak on 1 MI355X"
 
@@ -373,6 +379,26 @@ def main() -> int:
     if not args.no_device_bvh and builder == BUILDERS["host"]:
         device_bvh = device_bvh_leg(ks, dev, hs, cam, params, args, info)
 
+    shade_launches = totals["launches"]["shade"]
+    shade_ms = totals["kernel_ms"]["shade"]
+    cdf_steps = max(1, math.ceil(math.log2(info["n_lights"] + 1)))
+    shade_bytes = (SHADE_VERTEX_BYTES + 4 * cdf_steps) * totals["rays_closest"] + SHADE_SHADOW_BYTES * totals["rays_shadow"]
+    shade_achieved = shade_bytes / (shade_ms / 1e3) / 1e9 if shade_ms > 0 else 0.0
+    roofline_shade = {
+        "bound": "hbm",
+        "achieved": round(shade_achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(shade_achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "k_shade (path vertex: emission, NEE, BSDF sampling, roulette)",
+        "bytes_per_launch": shade_bytes / max(shade_launches, 1),
+        "avg_launch_ms": shade_ms / max(shade_launches, 1),
+    }
+    tf = ROOT / "profiles" / "traffic_closest_spheres.json"
+    if tf.exists():
+        roofline_shade["traffic"] = json.loads(tf.read_text()).get("shade_hbm_bytes_per_launch")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
@@ -434,6 +460,7 @@ def main() -> int:
             "shadow_rays_per_step": totals["rays_shadow"] // args.steps,
         },
         "roofline": roofline,
+        "roofline_shade": roofline_shade,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
         "time_to_relmse_at_scale": scale,
