@@ -77,39 +77,39 @@ def cpu_baseline(hs, cam, kw, rows: int, threads: int, budget_s: float) -> dict:
     }
 
 
-BUILDERS = {"host": 0, "device": 1}
-BVH_NAME = {0: "host SBVH", 1: "device-built linear BVH"}
+BUILDERS = {"host": 0, "device": 1, "device_fast": 2}
+BVH_NAME = {0: "host SBVH", 1: "device-built BVH (clustering)", 2: "device-built BVH (radix tree)"}
 
 
 def device_bvh_leg(ks, dev, hs, cam, params, args, host_info) -> dict:
-    """Builds the benchmark scene's BVH with the device builder and renders over it: build time against
+    """Builds the benchmark scene's BVH with each device builder and renders over it: build time against
     the host SBVH's, and the render rate over each tree."""
-    builds = []
-    for _ in range(3):
-        sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": BUILDERS["device"]})
-        info = sc.info()
-        builds.append(info["bvh_build_ms"])
-        if len(builds) < 3:
-            sc.close()
-    film = ks.Film(dev, args.width, args.height)
-    ks.small_pt_render(sc, cam, params, film)  # warm-up
-    times = []
-    for _ in range(2):
-        film.clear()
-        t0 = time.perf_counter()
-        ks.small_pt_render(sc, cam, params, film)
-        times.append(time.perf_counter() - t0)
-    film.close()
-    sc.close()
+    out = {"host_sbvh_build_ms": round(host_info["bvh_build_ms"], 1), "host_sbvh_nodes": host_info["n_nodes"]}
     paths = args.width * args.height * args.spp
-    out = {
-        "build_ms": round(min(builds), 2),
-        "host_sbvh_build_ms": round(host_info["bvh_build_ms"], 1),
-        "nodes": info["n_nodes"],
-        "depth": info["max_depth"],
-        "host_sbvh_nodes": host_info["n_nodes"],
-        "mpaths_per_s": round(paths / min(times) / 1e6, 2),
-    }
+    for name in ("device", "device_fast"):
+        builds = []
+        for _ in range(3):
+            sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": BUILDERS[name]})
+            info = sc.info()
+            builds.append(info["bvh_build_ms"])
+            if len(builds) < 3:
+                sc.close()
+        film = ks.Film(dev, args.width, args.height)
+        ks.small_pt_render(sc, cam, params, film)  # warm-up
+        times = []
+        for _ in range(2):
+            film.clear()
+            t0 = time.perf_counter()
+            ks.small_pt_render(sc, cam, params, film)
+            times.append(time.perf_counter() - t0)
+        film.close()
+        sc.close()
+        out[name] = {
+            "build_ms": round(min(builds), 2),
+            "nodes": info["n_nodes"],
+            "depth": info["max_depth"],
+            "mpaths_per_s": round(paths / min(times) / 1e6, 2),
+        }
     log(f"device BVH: {out}")
     return out
 
@@ -244,11 +244,11 @@ def main() -> int:
     ap.add_argument("--bounces", type=int, default=12)
     ap.add_argument("--queue", type=int, default=0)
     ap.add_argument("--branching", type=int, default=4, choices=(2, 4), help="BVH node width on the device")
-    ap.add_argument("--builder", default="host", choices=("host", "device"),
-                    help="BVH of the timed renders: the host SBVH or the device-built linear BVH")
+    ap.add_argument("--builder", default="host", choices=("host", "device", "device_fast"),
+                    help="BVH of the timed renders: the host SBVH or a device-built BVH")
     ap.add_argument("--no-device-bvh", action="store_true",
                     help="skip the leg that builds the scene's BVH on the device and renders over it")
-    ap.add_argument("--scale-builder", default="host", choices=("host", "device"))
+    ap.add_argument("--scale-builder", default="host", choices=("host", "device", "device_fast"))
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)

@@ -1916,10 +1916,12 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     sc->dev = dev;
     ks_bvh_options o{};
     if (opt) o = *opt;
-    KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE, "unknown BVH builder");
+    KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE ||
+                   o.builder == KS_BVH_BUILD_DEVICE_FAST,
+               "unknown BVH builder");
     const int width = o.branching == 2 ? 2 : 4;
     // an empty scene gets the host builder's single empty node whichever builder was asked for
-    const bool device_build = o.builder == KS_BVH_BUILD_DEVICE && d->n_tris > 0;
+    const bool device_build = o.builder != KS_BVH_BUILD_HOST_SBVH && d->n_tris > 0;
     ksa::BvhResult bvh;
     double host_bvh_ms = 0.0;
     if (!device_build) {
@@ -1968,7 +1970,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
         }
         std::string err;
         if (ksa::build_bvh_device(sc->verts.p, sc->tri_idx.p, d->n_tris, ht.alpha ? alpha_of.as<uint8_t>() : nullptr,
-                                  o, width, s, dbvh, err) != 0) {
+                                  o, width, o.builder == KS_BVH_BUILD_DEVICE_FAST, s, dbvh, err) != 0) {
             ksa::set_error(err);
             return KS_ERR_HIP;
         }
@@ -2175,7 +2177,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     in.n_refs = device_build ? dbvh.n_refs : (uint32_t)bvh.tris.size();
     in.n_lights = (uint32_t)lights.size();
     in.max_depth = device_build ? dbvh.max_depth : width == 4 ? bvh.max_depth4 : bvh.max_depth;
-    in.builder = device_build ? KS_BVH_BUILD_DEVICE : KS_BVH_BUILD_HOST_SBVH;
+    in.builder = device_build ? o.builder : KS_BVH_BUILD_HOST_SBVH;
     in.bvh_build_ms = device_build ? dbvh.build_ms : host_bvh_ms;
     // traversal stack bound: every interior node on the path leaves at most width - 1 entries behind
     sc->max_stack = (uint32_t)(width - 1) * in.max_depth + 1;

@@ -1,26 +1,32 @@
-// BVH build on the device: a linear BVH over 63-bit Morton codes of the triangle box centroids
-// (Karras, "Maximizing Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012),
-// a bottom-up pass that fits the boxes and collapses subtrees into leaves where the surface area
-// heuristic prefers a leaf, and a top-down pass per level that lays the tree out in the node formats
-// the traversal reads (GpuNode4 / GpuNode) with the triangle records in leaf order (GpuTri).
+// BVH build on the device. Both builders start from the same Morton order: 63-bit codes of the triangle
+// box centroids, sorted. Then either
+//   - PLOC (KS_BVH_BUILD_DEVICE, the default device build): parallel locally-ordered clustering
+//     (Meister and Bittner, TVCG 2018) merges mutual nearest neighbours (by merged surface area) within
+//     a window of the Morton order until one cluster is left; or
+//   - the radix tree (KS_BVH_BUILD_DEVICE_FAST): Karras, "Maximizing Parallelism in the Construction of
+//     BVHs, Octrees, and k-d Trees", HPG 2012, fitted bottom up.
+// Each new interior node decides by the surface area heuristic whether its subtree becomes one leaf.
+// A top-down pass per level then lays the tree out in the node formats the traversal reads
+// (GpuNode4 / GpuNode) and orders the triangle records (GpuTri) leaf by leaf.
 //
 // The reference builds its SBVH on the host (cuda/sbvh.cu:301-440); its kd-tree builder
 // (cuda/cuda_parallel_kd_tree.cu) is a device pipeline of the same shape: bounds, sort, build,
-// layout. This builder has no spatial splits, so each triangle is referenced exactly once.
+// layout. These builders have no spatial splits, so each triangle is referenced exactly once.
 //
-// Kernels (one launch each unless noted), n triangles:
-//   k_bounds    scene box and centroid box (block reduction, then ordered-integer atomics), per-
-//               triangle boxes (32 B each)
-//   k_morton    63-bit Morton key of each centroid, value = triangle id
-//   radix sort  (rocPRIM) of the (key, id) pairs over bits 0..62
-//   k_hierarchy the n - 1 interior nodes of the radix tree, their ranges and parent links
-//   k_fit       one thread per leaf walks to the root; the second arrival at a node fits its box and
-//               decides leaf or split by SAH (the first arrival ends)
-//   k_layout    one launch per level of the output tree: opens children of largest area until a node
-//               holds `W` children, writes the node, queues its interior children for the next level
-//   k_records   triangle records in leaf order (vertices, triangle id, alpha-test flag)
+// Kernels, n triangles:
+//   k_bounds      scene box and centroid box (block reduction, then ordered-integer atomics), per-
+//                 triangle boxes (32 B each)
+//   k_morton      63-bit Morton key of each centroid, value = triangle id
+//   radix sort    (rocPRIM) of the (key, id) pairs over bits 0..62
+//   k_leaves      leaf nodes in sorted order
+//   PLOC rounds   k_ploc_nn (neighbour search, the window staged in LDS), k_ploc_merge, rocPRIM select
+//   radix tree    k_hierarchy (splits and parent links), k_fit (one thread per leaf climbs; the second
+//                 arrival at a node fits it)
+//   k_layout      one launch per level of the output tree
+//   k_records     triangle records in leaf order (vertices, triangle id, alpha-test flag)
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include "ks_internal.h"
 
@@ -143,6 +149,47 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4 *__restrict__ tr
     ids[t] = t;
 }
 
+// Node ids shared by both tree builders: 0 .. n-1 are the leaves (sorted position p, triangle ids[p]);
+// n .. 2n-2 are interior nodes, whose children are child[id - n] and whose triangle count is
+// count[id - n]. node_box[2 id] = (lo, SAH cost), node_box[2 id + 1] = (hi, 1 if the subtree is laid
+// out as one leaf).
+
+__device__ inline void put_box(float4 *node_box, uint32_t id, float3 lo, float3 hi, float cost, bool terminal) {
+    node_box[2 * (size_t)id] = make_float4(lo.x, lo.y, lo.z, cost);
+    node_box[2 * (size_t)id + 1] = make_float4(hi.x, hi.y, hi.z, terminal ? 1.0f : 0.0f);
+}
+
+// leaf nodes: the triangle's box, SAH cost = its area (intersection cost 1)
+__global__ __launch_bounds__(kBlock) void k_leaves(const float4 *__restrict__ tri_box, const uint32_t *__restrict__ ids,
+                                                   uint32_t n, float4 *__restrict__ node_box,
+                                                   uint32_t *__restrict__ clusters) {
+    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t t = ids[p];
+    const float4 a = tri_box[2 * (size_t)t], b = tri_box[2 * (size_t)t + 1];
+    const float3 lo = make_float3(a.x, a.y, a.z), hi = make_float3(b.x, b.y, b.z);
+    put_box(node_box, p, lo, hi, area_of(lo, hi), true);
+    clusters[p] = p;
+}
+
+// SAH decision for a new interior node over children a and b (their boxes and costs in node_box)
+__device__ inline void fit_node(float4 *node_box, uint32_t *count, uint32_t n, uint32_t id, uint32_t a, uint32_t b,
+                                uint32_t max_leaf, float trav_cost) {
+    const float4 la = node_box[2 * (size_t)a], ha = node_box[2 * (size_t)a + 1];
+    const float4 lb = node_box[2 * (size_t)b], hb = node_box[2 * (size_t)b + 1];
+    const float3 lo = make_float3(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z));
+    const float3 hi = make_float3(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z));
+    const uint32_t c = (a < n ? 1u : count[a - n]) + (b < n ? 1u : count[b - n]);
+    const float area = area_of(lo, hi);
+    const float split = trav_cost * area + la.w + lb.w;
+    const float as_leaf = area * (float)c;
+    const bool leaf = c <= max_leaf && as_leaf <= split;
+    count[id - n] = c;
+    put_box(node_box, id, lo, hi, leaf ? as_leaf : split, leaf);
+}
+
+// ---- radix tree (fast build) ----
+
 // common prefix length of sorted keys i and j; equal keys are told apart by their positions
 __device__ inline int prefix(const uint64_t *__restrict__ k, int64_t n, int64_t i, int64_t j) {
     if (j < 0 || j >= n) return -1;
@@ -151,11 +198,10 @@ __device__ inline int prefix(const uint64_t *__restrict__ k, int64_t n, int64_t 
     return __clzll(a ^ b);
 }
 
-// Interior node i of the radix tree over the n sorted keys: its key range, its split, its children.
+// Interior node i (id n + i) of the radix tree over the n sorted keys: its split and its children.
 __global__ __launch_bounds__(kBlock) void k_hierarchy(const uint64_t *__restrict__ keys, uint32_t n,
-                                                      uint32_t *__restrict__ child, uint2 *__restrict__ range,
-                                                      uint32_t *__restrict__ parent_leaf,
-                                                      uint32_t *__restrict__ parent_node, uint32_t *__restrict__ arrivals) {
+                                                      uint2 *__restrict__ child, uint32_t *__restrict__ parent,
+                                                      uint32_t *__restrict__ arrivals) {
     int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= (int64_t)n - 1) return;
     const int64_t nn = n;
@@ -175,143 +221,220 @@ __global__ __launch_bounds__(kBlock) void k_hierarchy(const uint64_t *__restrict
     }
     const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
     const int64_t first = i < j ? i : j, last = i < j ? j : i;
-    uint32_t left = first == gamma ? kRadixLeaf | (uint32_t)gamma : (uint32_t)gamma;
-    uint32_t right = last == gamma + 1 ? kRadixLeaf | (uint32_t)(gamma + 1) : (uint32_t)(gamma + 1);
-    child[2 * i] = left;
-    child[2 * i + 1] = right;
-    range[i] = make_uint2((uint32_t)first, (uint32_t)last);
+    const uint32_t left = first == gamma ? (uint32_t)gamma : n + (uint32_t)gamma;
+    const uint32_t right = last == gamma + 1 ? (uint32_t)(gamma + 1) : n + (uint32_t)(gamma + 1);
+    child[i] = make_uint2(left, right);
     arrivals[i] = 0u;
-    if (left & kRadixLeaf) parent_leaf[left & ~kRadixLeaf] = (uint32_t)i;
-    else parent_node[left] = (uint32_t)i;
-    if (right & kRadixLeaf) parent_leaf[right & ~kRadixLeaf] = (uint32_t)i;
-    else parent_node[right] = (uint32_t)i;
-    if (i == 0) parent_node[0] = kNone;
-}
-
-struct ChildBox {
-    float3 lo, hi;
-    float cost;
-    bool terminal; // a leaf of the output tree
-};
-
-// node_box[2 i] = (lo, SAH cost), node_box[2 i + 1] = (hi, 1 if the subtree becomes one leaf)
-__device__ inline ChildBox child_box(uint32_t c, const float4 *__restrict__ tri_box, const uint32_t *__restrict__ ids,
-                                     const float4 *node_box, float leaf_cost) {
-    ChildBox r;
-    if (c & kRadixLeaf) {
-        uint32_t t = ids[c & ~kRadixLeaf];
-        float4 a = tri_box[2 * (size_t)t], b = tri_box[2 * (size_t)t + 1];
-        r.lo = make_float3(a.x, a.y, a.z);
-        r.hi = make_float3(b.x, b.y, b.z);
-        r.cost = leaf_cost * area_of(r.lo, r.hi);
-        r.terminal = true;
-    } else {
-        float4 a = node_box[2 * (size_t)c], b = node_box[2 * (size_t)c + 1];
-        r.lo = make_float3(a.x, a.y, a.z);
-        r.hi = make_float3(b.x, b.y, b.z);
-        r.cost = a.w;
-        r.terminal = b.w != 0.0f;
-    }
-    return r;
+    parent[left] = n + (uint32_t)i;
+    parent[right] = n + (uint32_t)i;
+    if (i == 0) parent[n] = kNone;
 }
 
 // One thread per leaf climbs towards the root. The arrival counter of a node lets the second of its two
 // children's threads continue (after both children's boxes are written); the first one stops.
-__global__ __launch_bounds__(kBlock) void k_fit(uint32_t n, const float4 *__restrict__ tri_box,
-                                                const uint32_t *__restrict__ ids, const uint32_t *__restrict__ child,
-                                                const uint2 *__restrict__ range, const uint32_t *__restrict__ parent_leaf,
-                                                const uint32_t *__restrict__ parent_node, uint32_t *arrivals,
-                                                float4 *node_box, uint32_t max_leaf, float trav_cost) {
+__global__ __launch_bounds__(kBlock) void k_fit(uint32_t n, const uint2 *__restrict__ child,
+                                                const uint32_t *__restrict__ parent, uint32_t *arrivals,
+                                                float4 *node_box, uint32_t *count, uint32_t max_leaf,
+                                                float trav_cost) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
-    const float leaf_cost = 1.0f;
-    uint32_t node = parent_leaf[p];
+    uint32_t node = parent[p];
     for (int step = 0; node != kNone && step < kMaxLevels; ++step) { // a radix tree is at most 96 deep
-        uint32_t seen = __hip_atomic_fetch_add(arrivals + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t seen = __hip_atomic_fetch_add(arrivals + (node - n), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (seen == 0u) return; // the sibling subtree is not finished: its thread continues
-        ChildBox a = child_box(child[2 * (size_t)node], tri_box, ids, node_box, leaf_cost);
-        ChildBox b = child_box(child[2 * (size_t)node + 1], tri_box, ids, node_box, leaf_cost);
-        float3 lo = make_float3(fminf(a.lo.x, b.lo.x), fminf(a.lo.y, b.lo.y), fminf(a.lo.z, b.lo.z));
-        float3 hi = make_float3(fmaxf(a.hi.x, b.hi.x), fmaxf(a.hi.y, b.hi.y), fmaxf(a.hi.z, b.hi.z));
-        const float area = area_of(lo, hi);
-        const uint2 rg = range[node];
-        const uint32_t count = rg.y - rg.x + 1;
-        const float split = trav_cost * area + a.cost + b.cost;
-        const float as_leaf = leaf_cost * area * (float)count;
-        const bool leaf = count <= max_leaf && as_leaf <= split;
-        node_box[2 * (size_t)node] = make_float4(lo.x, lo.y, lo.z, leaf ? as_leaf : split);
-        node_box[2 * (size_t)node + 1] = make_float4(hi.x, hi.y, hi.z, leaf ? 1.0f : 0.0f);
-        node = parent_node[node];
+        const uint2 c = child[node - n];
+        fit_node(node_box, count, n, node, c.x, c.y, max_leaf, trav_cost);
+        node = parent[node];
     }
 }
 
-// One output node per queue entry {radix-tree node, output index}. At the root level the entry may be a
-// terminal subtree (or the single leaf of a one-triangle scene): the node then holds that one leaf.
+// ---- PLOC (the default device build) ----
+// Parallel locally-ordered clustering (Meister and Bittner, "Parallel Locally-Ordered Clustering for
+// Bounding Volume Hierarchy Construction", TVCG 2018): the clusters stay in Morton order; each finds
+// the neighbour within kRadius positions whose merged box has the smallest surface area; mutual
+// neighbours merge; the array is compacted; repeat until one cluster is left.
+constexpr int kRadius = 16;
+
+// Neighbour order: smaller merged area first; equal areas by a hash of the pair of positions, then by
+// the pair itself. That is a total order on pairs, the same seen from either end, so the globally
+// smallest pair is mutual and every round merges at least one pair. The hash matters where many boxes
+// are equal (duplicated geometry): ordering ties by position alone would merge one pair per round.
+__device__ inline uint32_t pair_hash(uint32_t lo, uint32_t hi) {
+    uint32_t x = lo * 0x9E3779B1u ^ hi;
+    x ^= x >> 16;
+    x *= 0x85EBCA6Bu;
+    x ^= x >> 13;
+    x *= 0xC2B2AE35u;
+    x ^= x >> 16;
+    return x;
+}
+__device__ inline bool closer(float d, uint32_t i, uint32_t j, float bd, uint32_t bj) {
+    if (d != bd) return d < bd;
+    const uint32_t lo = min(i, j), hi = max(i, j), blo = min(i, bj), bhi = max(i, bj);
+    const uint32_t h = pair_hash(lo, hi), bh = pair_hash(blo, bhi);
+    if (h != bh) return h < bh;
+    return lo != blo ? lo < blo : hi < bhi;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *__restrict__ clusters, uint32_t m,
+                                                    const float4 *__restrict__ node_box, uint32_t *__restrict__ nn) {
+    __shared__ float3 slo[kBlock + 2 * kRadius], shi[kBlock + 2 * kRadius];
+    const int64_t base = (int64_t)blockIdx.x * kBlock - kRadius;
+    for (int k = threadIdx.x; k < kBlock + 2 * kRadius; k += kBlock) {
+        const int64_t idx = base + k;
+        if (idx >= 0 && idx < (int64_t)m) {
+            const uint32_t id = clusters[idx];
+            const float4 a = node_box[2 * (size_t)id], b = node_box[2 * (size_t)id + 1];
+            slo[k] = make_float3(a.x, a.y, a.z);
+            shi[k] = make_float3(b.x, b.y, b.z);
+        }
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const int me = threadIdx.x + kRadius;
+    const float3 lo = slo[me], hi = shi[me];
+    float best = __builtin_inff();
+    uint32_t bj = i;
+    for (int o = -kRadius; o <= kRadius; ++o) {
+        const int64_t j = (int64_t)i + o;
+        if (o == 0 || j < 0 || j >= (int64_t)m) continue;
+        const float3 l2 = slo[me + o], h2 = shi[me + o];
+        const float3 ul = make_float3(fminf(lo.x, l2.x), fminf(lo.y, l2.y), fminf(lo.z, l2.z));
+        const float3 uh = make_float3(fmaxf(hi.x, h2.x), fmaxf(hi.y, h2.y), fmaxf(hi.z, h2.z));
+        const float d = fminf(area_of(ul, uh), 3.0e38f); // non-finite boxes sort last
+        if (bj == i || closer(d, i, (uint32_t)j, best, bj)) best = d, bj = (uint32_t)j;
+    }
+    nn[i] = bj;
+}
+
+// Mutual neighbours i < j merge into a new node that takes position i; position j is dropped.
+__global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restrict__ clusters, uint32_t m,
+                                                       const uint32_t *__restrict__ nn, uint32_t n, uint32_t *next_id,
+                                                       float4 *node_box, uint2 *__restrict__ child, uint32_t *count,
+                                                       uint32_t *__restrict__ merged, uint8_t *__restrict__ keep,
+                                                       uint32_t max_leaf, float trav_cost) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t j = nn[i];
+    const bool mutual = j < m && nn[j] == i;
+    if (!mutual) {
+        merged[i] = clusters[i];
+        keep[i] = 1;
+        return;
+    }
+    if (j < i) {
+        keep[i] = 0;
+        return;
+    }
+    const uint32_t k = atomicAdd(next_id, 1u);
+    if (k >= n - 1) { // cannot happen: n leaves make n - 1 merges; the host sees the counter
+        keep[i] = 0;
+        return;
+    }
+    const uint32_t id = n + k, a = clusters[i], b = clusters[j];
+    child[k] = make_uint2(a, b);
+    fit_node(node_box, count, n, id, a, b, max_leaf, trav_cost);
+    merged[i] = id;
+    keep[i] = 1;
+}
+
+// ---- layout ----
+
+struct Slot {
+    float3 lo, hi;
+    bool terminal;
+};
+
+__device__ inline Slot slot_of(const float4 *__restrict__ node_box, uint32_t id) {
+    const float4 a = node_box[2 * (size_t)id], b = node_box[2 * (size_t)id + 1];
+    return Slot{make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), b.w != 0.0f};
+}
+
+// Queue entry: {tree node, output node index, first triangle record of its subtree}. Each output node
+// opens the interior child of largest surface area until it holds W children (a root that is itself a
+// leaf gets a node with that one leaf). A child laid out as a leaf owns the records [first, first +
+// count), filled here in depth-first order; an interior child is queued for the next level.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_layout(const uint2 *__restrict__ queue, uint32_t count, bool root_level,
-                                                   const float4 *__restrict__ tri_box, const uint32_t *__restrict__ ids,
-                                                   const uint32_t *__restrict__ child, const uint2 *__restrict__ range,
-                                                   const float4 *__restrict__ node_box, void *__restrict__ out_nodes,
-                                                   uint2 *__restrict__ next, uint32_t *next_count, uint32_t next_base,
-                                                   uint32_t cap) {
-    uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= count) return;
-    const uint2 e = queue[qi];
-    if (e.y >= cap) return; // cannot happen (an output node per interior radix-tree node); the host reports it
+__global__ __launch_bounds__(kBlock) void k_layout(const uint4 *__restrict__ queue, uint32_t qn, bool root_level,
+                                                   uint32_t n, const float4 *__restrict__ node_box,
+                                                   const uint2 *__restrict__ child, const uint32_t *__restrict__ count,
+                                                   const uint32_t *__restrict__ ids, void *__restrict__ out_nodes,
+                                                   uint32_t *__restrict__ order, uint4 *__restrict__ next,
+                                                   uint32_t *next_count, uint32_t next_base, uint32_t cap) {
+    const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
+    if (qi >= qn) return;
+    const uint4 e = queue[qi];
+    if (e.y >= cap) return; // cannot happen (an output node per interior tree node); the host reports it
     uint32_t c[W];
-    ChildBox cb[W];
+    Slot cs[W];
     int nc;
-    ChildBox self = child_box(e.x, tri_box, ids, node_box, 1.0f);
+    const Slot self = slot_of(node_box, e.x);
     if (root_level && self.terminal) {
         c[0] = e.x;
-        cb[0] = self;
+        cs[0] = self;
         nc = 1;
     } else {
-        c[0] = child[2 * (size_t)e.x];
-        c[1] = child[2 * (size_t)e.x + 1];
-        cb[0] = child_box(c[0], tri_box, ids, node_box, 1.0f);
-        cb[1] = child_box(c[1], tri_box, ids, node_box, 1.0f);
+        const uint2 ch = child[e.x - n];
+        c[0] = ch.x;
+        c[1] = ch.y;
+        cs[0] = slot_of(node_box, ch.x);
+        cs[1] = slot_of(node_box, ch.y);
         nc = 2;
-        // open the interior child of largest surface area until the node is full
         while (nc < W) {
             int best = -1;
             float best_area = -1.0f;
             uint32_t g = 0;
 #pragma unroll
             for (int k = 0; k < W; ++k)
-                if (k < nc && !cb[k].terminal) {
-                    float a = area_of(cb[k].lo, cb[k].hi);
+                if (k < nc && !cs[k].terminal) {
+                    const float a = area_of(cs[k].lo, cs[k].hi);
                     if (a > best_area) best = k, best_area = a, g = c[k];
                 }
             if (best < 0) break;
-            const uint32_t g0 = child[2 * (size_t)g], g1 = child[2 * (size_t)g + 1];
-            // keep the order of the children: g1 goes right after g0
+            const uint2 gc = child[g - n];
+            // g's two children take its place, in order
 #pragma unroll
             for (int k = W - 1; k > 0; --k)
-                if (k > best + 1 && k <= nc) c[k] = c[k - 1], cb[k] = cb[k - 1];
+                if (k > best + 1 && k <= nc) c[k] = c[k - 1], cs[k] = cs[k - 1];
 #pragma unroll
             for (int k = 0; k < W; ++k) {
-                if (k == best) c[k] = g0, cb[k] = child_box(g0, tri_box, ids, node_box, 1.0f);
-                if (k == best + 1) c[k] = g1, cb[k] = child_box(g1, tri_box, ids, node_box, 1.0f);
+                if (k == best) c[k] = gc.x, cs[k] = slot_of(node_box, gc.x);
+                if (k == best + 1) c[k] = gc.y, cs[k] = slot_of(node_box, gc.y);
             }
             ++nc;
         }
     }
     uint32_t ptr[W];
+    uint32_t first = e.z;
 #pragma unroll
     for (int k = 0; k < W; ++k) {
         ptr[k] = kLeafBit;
         if (k >= nc) continue;
-        if (c[k] & kRadixLeaf) {
-            ptr[k] = kLeafBit | ((c[k] & ~kRadixLeaf) << 4);
-        } else if (cb[k].terminal) {
-            const uint2 rg = range[c[k]];
-            ptr[k] = kLeafBit | (rg.x << 4) | (rg.y - rg.x);
+        const uint32_t cnt = c[k] < n ? 1u : count[c[k] - n];
+        if (cs[k].terminal) {
+            ptr[k] = kLeafBit | (first << 4) | (cnt - 1);
+            // the subtree's leaves in depth-first order (at most kMaxLeafRefs of them)
+            uint32_t stack[kMaxLeafRefs];
+            int sp = 0;
+            stack[sp++] = c[k];
+            uint32_t w = first;
+            while (sp > 0) {
+                const uint32_t x = stack[--sp];
+                if (x < n) {
+                    if (w < first + cnt) order[w++] = ids[x];
+                } else if (sp + 2 <= (int)kMaxLeafRefs) {
+                    const uint2 xc = child[x - n];
+                    stack[sp++] = xc.y;
+                    stack[sp++] = xc.x;
+                }
+            }
         } else {
-            uint32_t q = atomicAdd(next_count, 1u);
-            if (next_base + q < cap) next[q] = make_uint2(c[k], next_base + q);
+            const uint32_t q = atomicAdd(next_count, 1u);
+            if (next_base + q < cap) next[q] = make_uint4(c[k], next_base + q, first, 0u);
             ptr[k] = next_base + q;
         }
+        first += cnt;
     }
     const float inf = __builtin_inff();
     if (W == 4) {
@@ -319,12 +442,12 @@ __global__ __launch_bounds__(kBlock) void k_layout(const uint2 *__restrict__ que
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const bool used = k < nc;
-            nd.lox[k] = used ? cb[k].lo.x : inf;
-            nd.hix[k] = used ? cb[k].hi.x : -inf;
-            nd.loy[k] = used ? cb[k].lo.y : inf;
-            nd.hiy[k] = used ? cb[k].hi.y : -inf;
-            nd.loz[k] = used ? cb[k].lo.z : inf;
-            nd.hiz[k] = used ? cb[k].hi.z : -inf;
+            nd.lox[k] = used ? cs[k].lo.x : inf;
+            nd.hix[k] = used ? cs[k].hi.x : -inf;
+            nd.loy[k] = used ? cs[k].lo.y : inf;
+            nd.hiy[k] = used ? cs[k].hi.y : -inf;
+            nd.loz[k] = used ? cs[k].lo.z : inf;
+            nd.hiz[k] = used ? cs[k].hi.z : -inf;
             nd.child[k] = ptr[k];
             nd.pad[k] = 0u;
         }
@@ -334,12 +457,12 @@ __global__ __launch_bounds__(kBlock) void k_layout(const uint2 *__restrict__ que
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const bool used = k < nc;
-            nd.f[4 * k] = used ? cb[k].lo.x : inf;
-            nd.f[4 * k + 1] = used ? cb[k].hi.x : -inf;
-            nd.f[4 * k + 2] = used ? cb[k].lo.y : inf;
-            nd.f[4 * k + 3] = used ? cb[k].hi.y : -inf;
-            nd.f[8 + 2 * k] = used ? cb[k].lo.z : inf;
-            nd.f[9 + 2 * k] = used ? cb[k].hi.z : -inf;
+            nd.f[4 * k] = used ? cs[k].lo.x : inf;
+            nd.f[4 * k + 1] = used ? cs[k].hi.x : -inf;
+            nd.f[4 * k + 2] = used ? cs[k].lo.y : inf;
+            nd.f[4 * k + 3] = used ? cs[k].hi.y : -inf;
+            nd.f[8 + 2 * k] = used ? cs[k].lo.z : inf;
+            nd.f[9 + 2 * k] = used ? cs[k].hi.z : -inf;
             nd.child[k] = ptr[k];
             nd.pad[k] = 0u;
         }
@@ -347,13 +470,13 @@ __global__ __launch_bounds__(kBlock) void k_layout(const uint2 *__restrict__ que
     }
 }
 
-// triangle records in leaf (sorted) order; v1.w = 1 marks a triangle whose material is alpha tested
+// triangle records in leaf order; v1.w = 1 marks a triangle whose material is alpha tested
 __global__ __launch_bounds__(kBlock) void k_records(const float4 *__restrict__ verts, const uint4 *__restrict__ tri_idx,
-                                                    const uint32_t *__restrict__ ids, uint32_t n,
+                                                    const uint32_t *__restrict__ order, uint32_t n,
                                                     const uint8_t *__restrict__ mat_alpha, float4 *__restrict__ out) {
     uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= n) return;
-    const uint32_t t = ids[p];
+    const uint32_t t = order[p];
     const uint4 ix = tri_idx[t];
     const float4 a = verts[ix.x], b = verts[ix.y], c = verts[ix.z];
     const uint32_t alpha = mat_alpha && mat_alpha[ix.w & 0xffffffu] ? 1u : 0u;
@@ -375,6 +498,7 @@ struct Scratch {
         p = reinterpret_cast<T *>(v);
         return hipSuccess;
     }
+    void release(void *p) { ptrs.erase(std::remove(ptrs.begin(), ptrs.end(), p), ptrs.end()); }
 };
 
 inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
@@ -382,7 +506,8 @@ inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlo
 } // namespace
 
 int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const uint8_t *mat_alpha,
-                     const ks_bvh_options &opt, int width, void *stream, DeviceBvh &out, std::string &err) {
+                     const ks_bvh_options &opt, int width, bool radix_tree, void *stream, DeviceBvh &out,
+                     std::string &err) {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     out = DeviceBvh{};
 #define KS_DB(call)                                                                                            \
@@ -404,14 +529,15 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     const uint32_t max_leaf = std::min<uint32_t>(std::max<uint32_t>(opt.max_prims_in_node ? opt.max_prims_in_node : 4, 1),
                                                  kMaxLeafRefs);
     const float trav_cost = opt.trav_weight > 0.0f ? opt.trav_weight : 1.0f;
-    const uint32_t ni = n > 1 ? n - 1 : 1; // interior radix-tree nodes
+    const uint32_t ni = n > 1 ? n - 1 : 1; // interior tree nodes
     const size_t node_size = width == 4 ? sizeof(GpuNode4) : sizeof(GpuNode);
     Scratch sc;
-    float4 *tri_box;
+    float4 *tri_box, *node_box;
     uint64_t *keys[2];
-    uint32_t *ids[2], *acc, *level_count, *child, *parent_leaf, *parent_node, *arrivals;
-    uint2 *range, *queue[2];
-    float4 *node_box;
+    uint32_t *ids[2], *acc, *level_count, *count, *clusters[2], *aux, *order, *next_id;
+    uint2 *child;
+    uint4 *queue[2];
+    uint8_t *keep;
     KS_DB(sc.get(tri_box, 2 * (size_t)n));
     KS_DB(sc.get(keys[0], n));
     KS_DB(sc.get(keys[1], n));
@@ -419,17 +545,19 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     KS_DB(sc.get(ids[1], n));
     KS_DB(sc.get(acc, 12));
     KS_DB(sc.get(level_count, kMaxLevels));
-    KS_DB(sc.get(child, 2 * (size_t)ni));
-    KS_DB(sc.get(range, ni));
-    KS_DB(sc.get(parent_leaf, n));
-    KS_DB(sc.get(parent_node, ni));
-    KS_DB(sc.get(arrivals, ni));
-    KS_DB(sc.get(node_box, 2 * (size_t)ni));
+    KS_DB(sc.get(node_box, 2 * ((size_t)n + ni)));
+    KS_DB(sc.get(child, ni));
+    KS_DB(sc.get(count, ni));
+    KS_DB(sc.get(clusters[0], n));
+    KS_DB(sc.get(clusters[1], n));
+    KS_DB(sc.get(aux, (size_t)n + ni)); // PLOC: each cluster's neighbour; radix tree: each node's parent
+    KS_DB(sc.get(keep, n));
+    KS_DB(sc.get(next_id, 1));
+    KS_DB(sc.get(order, n));
     KS_DB(sc.get(queue[0], ni));
     KS_DB(sc.get(queue[1], ni));
-    // an output node consumes at least one interior radix-tree node (or is the root)
     void *nodes = nullptr;
-    KS_DB(hipMalloc(&nodes, ni * node_size));
+    KS_DB(hipMalloc(&nodes, ni * node_size)); // an output node per interior tree node at most
     sc.ptrs.push_back(nodes);
 
     const auto t0 = std::chrono::steady_clock::now();
@@ -439,39 +567,80 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
                                                                          tri_box, acc);
     k_morton<<<blocks_for(n), kBlock, 0, s>>>(tri_box, n, acc, keys[0], ids[0]);
     KS_DB(hipGetLastError());
-    size_t temp_bytes = 0;
-    KS_DB(rocprim::radix_sort_pairs(nullptr, temp_bytes, keys[0], keys[1], ids[0], ids[1], n, 0, 63, s));
+    size_t sort_bytes = 0, select_bytes = 0;
+    KS_DB(rocprim::radix_sort_pairs(nullptr, sort_bytes, keys[0], keys[1], ids[0], ids[1], n, 0, 63, s));
+    KS_DB(rocprim::select(nullptr, select_bytes, clusters[1], keep, clusters[0], next_id, (size_t)n, s));
     void *temp = nullptr;
-    KS_DB(hipMalloc(&temp, std::max<size_t>(temp_bytes, 16)));
+    const size_t temp_bytes = std::max<size_t>(std::max(sort_bytes, select_bytes), 16);
+    KS_DB(hipMalloc(&temp, temp_bytes));
     sc.ptrs.push_back(temp);
-    KS_DB(rocprim::radix_sort_pairs(temp, temp_bytes, keys[0], keys[1], ids[0], ids[1], n, 0, 63, s));
+    size_t tb = temp_bytes;
+    KS_DB(rocprim::radix_sort_pairs(temp, tb, keys[0], keys[1], ids[0], ids[1], n, 0, 63, s));
     const uint64_t *skeys = keys[1];
     const uint32_t *sids = ids[1];
-    uint32_t root = 0;
-    if (n > 1) {
-        k_hierarchy<<<blocks_for(n - 1), kBlock, 0, s>>>(skeys, n, child, range, parent_leaf, parent_node, arrivals);
-        k_fit<<<blocks_for(n), kBlock, 0, s>>>(n, tri_box, sids, child, range, parent_leaf, parent_node, arrivals,
-                                                node_box, max_leaf, trav_cost);
+    k_leaves<<<blocks_for(n), kBlock, 0, s>>>(tri_box, sids, n, node_box, clusters[0]);
+    KS_DB(hipGetLastError());
+    uint32_t root = 0; // a one-triangle scene: the leaf itself
+    uint32_t rounds = 0;
+    if (n > 1 && radix_tree) {
+        uint32_t *parent = aux;
+        k_hierarchy<<<blocks_for(n - 1), kBlock, 0, s>>>(skeys, n, child, parent, count);
+        // count doubles as the arrival counters until k_fit overwrites each with the node's count: a
+        // node's counter is read (atomic) before fit_node writes its count, and nothing reads it after
+        k_fit<<<blocks_for(n), kBlock, 0, s>>>(n, child, parent, count, node_box, count, max_leaf, trav_cost);
         KS_DB(hipGetLastError());
-    } else {
-        root = kRadixLeaf; // the one leaf
-    }
-    // root entry {root, output node 0}
-    const uint2 root_entry = make_uint2(root, 0u);
-    KS_DB(hipMemcpyAsync(queue[0], &root_entry, sizeof(uint2), hipMemcpyHostToDevice, s));
-    uint32_t count = 1, base = 1, levels = 0;
-    while (count > 0) {
-        if (levels >= (uint32_t)kMaxLevels) {
-            err = "device BVH build: tree deeper than expected";
+        root = n;
+    } else if (n > 1) {
+        KS_DB(hipMemsetAsync(next_id, 0, 4, s));
+        uint32_t m = n;
+        const int cur = 0; // clusters[0] holds the live clusters, clusters[1] the merged ones before compaction
+        while (m > 1) {
+            if (++rounds > 4096) {
+                err = "device BVH build: clustering did not converge";
+                return -1;
+            }
+            k_ploc_nn<<<blocks_for(m), kBlock, 0, s>>>(clusters[cur], m, node_box, aux);
+            k_ploc_merge<<<blocks_for(m), kBlock, 0, s>>>(clusters[cur], m, aux, n, next_id, node_box, child, count,
+                                                           clusters[cur ^ 1], keep, max_leaf, trav_cost);
+            KS_DB(hipGetLastError());
+            // compaction keeps the order: clusters[1] -> clusters[0], the new count into level_count[0]
+            tb = temp_bytes;
+            KS_DB(rocprim::select(temp, tb, clusters[cur ^ 1], keep, clusters[cur], level_count, (size_t)m, s));
+            uint32_t mm = 0;
+            KS_DB(hipMemcpyAsync(&mm, level_count, 4, hipMemcpyDeviceToHost, s));
+            KS_DB(hipStreamSynchronize(s));
+            if (mm >= m || mm == 0) {
+                err = "device BVH build: a clustering round merged nothing";
+                return -1;
+            }
+            m = mm;
+        }
+        uint32_t made = 0;
+        KS_DB(hipMemcpyAsync(&root, clusters[cur], 4, hipMemcpyDeviceToHost, s));
+        KS_DB(hipMemcpyAsync(&made, next_id, 4, hipMemcpyDeviceToHost, s));
+        KS_DB(hipStreamSynchronize(s));
+        if (made != n - 1 || root != n + made - 1) {
+            err = "device BVH build: clustering made an inconsistent tree";
             return -1;
         }
-        uint2 *cur = queue[levels & 1], *nxt = queue[(levels + 1) & 1];
+        KS_DB(hipMemsetAsync(level_count, 0, kMaxLevels * sizeof(uint32_t), s));
+    }
+    // root entry {root, output node 0, first record 0}
+    const uint4 root_entry = make_uint4(root, 0u, 0u, 0u);
+    KS_DB(hipMemcpyAsync(queue[0], &root_entry, sizeof(uint4), hipMemcpyHostToDevice, s));
+    uint32_t qn = 1, base = 1, levels = 0;
+    while (qn > 0) {
+        if (levels >= (uint32_t)kMaxLevels) {
+            err = "device BVH build: tree deeper than the layout supports";
+            return -1;
+        }
+        uint4 *cur = queue[levels & 1], *nxt = queue[(levels + 1) & 1];
         if (width == 4)
-            k_layout<4><<<blocks_for(count), kBlock, 0, s>>>(cur, count, levels == 0, tri_box, sids, child, range,
-                                                              node_box, nodes, nxt, level_count + levels, base, ni);
+            k_layout<4><<<blocks_for(qn), kBlock, 0, s>>>(cur, qn, levels == 0, n, node_box, child, count, sids, nodes,
+                                                           order, nxt, level_count + levels, base, ni);
         else
-            k_layout<2><<<blocks_for(count), kBlock, 0, s>>>(cur, count, levels == 0, tri_box, sids, child, range,
-                                                              node_box, nodes, nxt, level_count + levels, base, ni);
+            k_layout<2><<<blocks_for(qn), kBlock, 0, s>>>(cur, qn, levels == 0, n, node_box, child, count, sids, nodes,
+                                                           order, nxt, level_count + levels, base, ni);
         KS_DB(hipGetLastError());
         uint32_t next = 0;
         KS_DB(hipMemcpyAsync(&next, level_count + levels, 4, hipMemcpyDeviceToHost, s));
@@ -482,7 +651,7 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
             return -1;
         }
         base += next;
-        count = next;
+        qn = next;
     }
     const uint32_t n_nodes = base;
 
@@ -490,7 +659,7 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     KS_DB(hipMalloc(&tris, (size_t)n * sizeof(GpuTri)));
     sc.ptrs.push_back(tris);
     k_records<<<blocks_for(n), kBlock, 0, s>>>(reinterpret_cast<const float4 *>(verts),
-                                               reinterpret_cast<const uint4 *>(tri_idx), sids, n, mat_alpha,
+                                               reinterpret_cast<const uint4 *>(tri_idx), order, n, mat_alpha,
                                                reinterpret_cast<float4 *>(tris));
     KS_DB(hipGetLastError());
     // the node array shrinks to what the tree uses
@@ -506,9 +675,9 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
         out.lo[k] = unordered(hacc[k]);
         out.hi[k] = unordered(hacc[3 + k]);
     }
-    // hand the two results to the caller; the scratch destructor frees the rest
-    sc.ptrs.erase(std::remove(sc.ptrs.begin(), sc.ptrs.end(), exact), sc.ptrs.end());
-    sc.ptrs.erase(std::remove(sc.ptrs.begin(), sc.ptrs.end(), tris), sc.ptrs.end());
+    // the two results go to the caller; the scratch destructor frees the rest
+    sc.release(exact);
+    sc.release(tris);
     out.nodes = exact;
     out.node_bytes = (size_t)n_nodes * node_size;
     out.n_nodes = n_nodes;
@@ -516,6 +685,7 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     out.tri_bytes = (size_t)n * sizeof(GpuTri);
     out.n_refs = n;
     out.max_depth = levels;
+    out.rounds = rounds;
 #undef KS_DB
     return 0;
 }

@@ -69,12 +69,14 @@ struct DeviceBvh {
     uint32_t n_nodes = 0, n_refs = 0, max_depth = 0;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     double build_ms = 0.0;
+    uint32_t rounds = 0; // clustering rounds (PLOC)
 };
 // verts: float4 per vertex, tri_idx: uint4 per triangle (vertex ids, material id in the low 24 bits of
 // w), mat_alpha: per-material alpha-test flag or null, stream: the hipStream_t to build on. Returns 0 or
-// -1 with `err` set.
+// -1 with `err` set. radix_tree: the fast radix-tree build instead of PLOC.
 int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n_tris, const uint8_t *mat_alpha,
-                     const ks_bvh_options &opt, int width, void *stream, DeviceBvh &out, std::string &err);
+                     const ks_bvh_options &opt, int width, bool radix_tree, void *stream, DeviceBvh &out,
+                     std::string &err);
 
 struct HostScene {
     std::vector<float> vertices, normals;

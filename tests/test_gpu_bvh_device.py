@@ -1,4 +1,5 @@
-"""The device BVH builder (KS_BVH_BUILD_DEVICE): the tree it lays out, and the hits it gives.
+"""The device BVH builders (KS_BVH_BUILD_DEVICE: clustering in Morton order; KS_BVH_BUILD_DEVICE_FAST: the
+radix tree over the same order): the trees they lay out, and the hits they give.
 
 Structure (read back with ks_scene_bvh_get): every triangle is referenced exactly once with its own
 vertices, every child box contains what lies below it, leaves hold at most max_prims_in_node
@@ -39,12 +40,15 @@ def _children(node, width):
     return lo[used], hi[used], ch[used]
 
 
-def check_structure(sc, hs, max_leaf=4):
+BUILDERS = [1, 2]  # KS_BVH_BUILD_DEVICE, KS_BVH_BUILD_DEVICE_FAST
+
+
+def check_structure(sc, hs, builder, max_leaf=4):
     info = sc.info()
     nodes, refs = sc.bvh()
     width = info["branching"]
     n = hs.n_tris
-    assert info["builder"] == 1 and info["n_refs"] == n and info["n_tris"] == n
+    assert info["builder"] == builder and info["n_refs"] == n and info["n_tris"] == n
     tri = refs[:, 3].astype(np.int64)
     assert np.array_equal(np.sort(tri), np.arange(n)), "each triangle once"
     rf = refs.view(np.float32)
@@ -106,13 +110,14 @@ def _random_rays(n, lo, hi, seed):
     return rays
 
 
+@pytest.mark.parametrize("builder", BUILDERS)
 @pytest.mark.parametrize("branching", [4, 2])
 @pytest.mark.parametrize("which", ["cornell", "spheres_small"])
-def test_device_bvh_structure_and_hits(device, ks, ora, cornell, spheres_small, which, branching):
+def test_device_bvh_structure_and_hits(device, ks, ora, cornell, spheres_small, which, branching, builder):
     hs = cornell if which == "cornell" else spheres_small
-    _log("device bvh", which, "branching", branching, "tris", hs.n_tris)
-    sc = ks.Scene(device, hs, bvh={"builder": ks.KS_BVH_BUILD_DEVICE, "branching": branching})
-    info = check_structure(sc, hs)
+    _log("device bvh", which, "branching", branching, "builder", builder, "tris", hs.n_tris)
+    sc = ks.Scene(device, hs, bvh={"builder": builder, "branching": branching})
+    info = check_structure(sc, hs, builder)
     _log("nodes", info["n_nodes"], "depth", info["max_depth"], "build ms", info["bvh_build_ms"])
     host = ks.Scene(device, hs, bvh={"branching": branching})
     assert info["bound_lo"] == host.info()["bound_lo"] and info["bound_hi"] == host.info()["bound_hi"]
@@ -148,7 +153,7 @@ def _soup(ks, tris):
                         np.zeros(t, np.uint32), np.zeros(1, np.uint32), mats, np.zeros(0, ks.LIGHT_DTYPE), cam)
 
 
-@pytest.mark.parametrize("case", ["one", "two", "stacked", "degenerate", "dup_centroids"])
+@pytest.mark.parametrize("case", ["one", "two", "stacked", "degenerate", "dup_centroids", "identical"])
 def test_device_bvh_small_scenes(device, ks, ora, case):
     rng = np.random.default_rng(11)
     base = np.float32([[-1, -1, 0], [1, -1, 0], [0, 1, 0]])
@@ -161,6 +166,8 @@ def test_device_bvh_small_scenes(device, ks, ora, case):
     elif case == "degenerate":  # a line and a point among ordinary triangles
         tris = [base, np.float32([[0, 0, 1], [1, 1, 1], [2, 2, 1]]), np.float32([[0.2, 0.2, 0.5]] * 3),
                 base * 0.5 + np.float32([0, 0, 0.7])]
+    elif case == "identical":  # 6000 copies of one triangle: every box is equal (clustering ties)
+        tris = [base] * 6000
     else:  # 37 triangles whose boxes share one centre: every Morton key is equal
         tris = []
         for _ in range(37):
@@ -169,14 +176,14 @@ def test_device_bvh_small_scenes(device, ks, ora, case):
             tris.append(np.stack([a, -a, b]))
     hs = _soup(ks, tris)
     _log("small scene", case, hs.n_tris)
-    for branching in (4, 2):
-        sc = ks.Scene(device, hs, bvh={"builder": ks.KS_BVH_BUILD_DEVICE, "branching": branching})
-        check_structure(sc, hs)
+    for branching, builder in [(w, b) for w in (4, 2) for b in BUILDERS]:
+        sc = ks.Scene(device, hs, bvh={"builder": builder, "branching": branching})
+        check_structure(sc, hs, builder)
         rays = _random_rays(20000, -3.0, 3.0, 5)
         g = sc.intersect(rays)
         o = ora.OracleScene.from_host_scene(hs).intersect(rays)
         agree, trel = _compare_hits(g, o)
-        assert agree >= 0.9999 and trel <= 1e-5, (case, branching, agree, trel)
+        assert agree >= 0.9999 and trel <= 1e-5, (case, branching, builder, agree, trel)
         sc.close()
 
 
@@ -184,26 +191,28 @@ def test_device_bvh_render_matches_host_sbvh(device, ks, cornell):
     W = H = 128
     p = ks.small_pt_params(bounces=8, render_width=W, render_height=H, spp=16)
     out = []
-    for builder in (ks.KS_BVH_BUILD_HOST_SBVH, ks.KS_BVH_BUILD_DEVICE):
+    for builder in (ks.KS_BVH_BUILD_HOST_SBVH, ks.KS_BVH_BUILD_DEVICE, ks.KS_BVH_BUILD_DEVICE_FAST):
         sc = ks.Scene(device, cornell, bvh={"builder": builder})
         film = ks.Film(device, W, H)
         ks.small_pt_render(sc, cornell.camera, p, film)
         out.append(film.download())
         film.close()
         sc.close()
-    (a, wa), (b, wb) = out
-    assert np.array_equal(wa, wb)
-    same = np.all(a == b, axis=-1).mean()
-    rel = np.abs(a - b).sum() / np.abs(a).sum()
-    _log("pixels bit-equal", same, "relative L1 difference", rel)
-    assert same >= 0.99 and rel < 1e-3
+    a, wa = out[0]
+    for b, wb in out[1:]:
+        assert np.array_equal(wa, wb)
+        same = np.all(a == b, axis=-1).mean()
+        rel = np.abs(a - b).sum() / np.abs(a).sum()
+        _log("pixels bit-equal", same, "relative L1 difference", rel)
+        assert same >= 0.99 and rel < 1e-3
 
 
-def test_device_bvh_alpha_tested(device, ks, ora):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_device_bvh_alpha_tested(device, ks, ora, builder):
     """Alpha-tested triangles keep their flag in the device-built records: the opacity filter gives the
     same hits as over the host SBVH and the oracle."""
     box = textured_box(stochastic_opacity=True)
-    sc = ks.Scene(device, box, bvh={"builder": ks.KS_BVH_BUILD_DEVICE})
+    sc = ks.Scene(device, box, bvh={"builder": builder})
     nodes, refs = sc.bvh()
     alpha_tris = set(refs[refs[:, 7] != 0, 3].tolist())  # v1.w
     assert alpha_tris == {12, 13, 14, 15}, alpha_tris
@@ -226,11 +235,12 @@ def test_device_bvh_alpha_tested(device, ks, ora):
     sc.close()
 
 
-def test_device_bvh_large_scene(device, ks):
+@pytest.mark.parametrize("builder", BUILDERS)
+def test_device_bvh_large_scene(device, ks, builder):
     """The 1.3M-triangle sphere scene: structure-level counts, and the same closest hits as the host
     SBVH for a million random rays."""
     hs = ks.procedural_scene("spheres", 1)
-    sc = ks.Scene(device, hs, bvh={"builder": ks.KS_BVH_BUILD_DEVICE})
+    sc = ks.Scene(device, hs, bvh={"builder": builder})
     info = sc.info()
     _log("spheres device build ms", info["bvh_build_ms"], "nodes", info["n_nodes"], "depth", info["max_depth"])
     nodes, refs = sc.bvh()
