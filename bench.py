@@ -77,16 +77,16 @@ def cpu_baseline(hs, cam, kw, rows: int, threads: int, budget_s: float) -> dict:
     }
 
 
-BUILDERS = {"host": 0, "device": 1, "device_fast": 2}
-BVH_NAME = {0: "host SBVH", 1: "device-built BVH (clustering)", 2: "device-built BVH (radix tree)"}
+BUILDERS = {"host": 0, "device": 1}
+BVH_NAME = {0: "host SBVH", 1: "device-built BVH (PLOC)"}
 
 
 def device_bvh_leg(ks, dev, hs, cam, params, args, host_info) -> dict:
-    """Builds the benchmark scene's BVH with each device builder and renders over it: build time against
-    the host SBVH's, and the render rate over each tree."""
+    """Builds the benchmark scene's BVH on the device and renders over it: build time against the host
+    SBVH's, and the render rate over each tree."""
     out = {"host_sbvh_build_ms": round(host_info["bvh_build_ms"], 1), "host_sbvh_nodes": host_info["n_nodes"]}
     paths = args.width * args.height * args.spp
-    for name in ("device", "device_fast"):
+    for name in ("device",):
         builds = []
         for _ in range(3):
             sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": BUILDERS[name]})
@@ -244,11 +244,11 @@ def main() -> int:
     ap.add_argument("--bounces", type=int, default=12)
     ap.add_argument("--queue", type=int, default=0)
     ap.add_argument("--branching", type=int, default=4, choices=(2, 4), help="BVH node width on the device")
-    ap.add_argument("--builder", default="host", choices=("host", "device", "device_fast"),
+    ap.add_argument("--builder", default="host", choices=("host", "device"),
                     help="BVH of the timed renders: the host SBVH or a device-built BVH")
     ap.add_argument("--no-device-bvh", action="store_true",
                     help="skip the leg that builds the scene's BVH on the device and renders over it")
-    ap.add_argument("--scale-builder", default="host", choices=("host", "device", "device_fast"))
+    ap.add_argument("--scale-builder", default="host", choices=("host", "device"))
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -275,6 +275,8 @@ def main() -> int:
     ap.add_argument("--scale-max-spp", type=int, default=512)
     ap.add_argument("--batch-views", type=int, default=48)
     ap.add_argument("--batch-seed", type=int, default=0)
+    ap.add_argument("--build-only", type=int, default=0, metavar="N",
+                    help="only build the scene's BVH N times with --builder and print the build times")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (calibrates rocprof byte counters)")
     args = ap.parse_args()
@@ -300,6 +302,22 @@ def main() -> int:
         return 2
 
     from keepsake_amd import api as ks
+
+    if args.build_only > 0:
+        # the seeds of the benchmark configurations
+        hs = ks.procedural_scene(args.scene, {"spheres": 1, "spheres_small": 1, "room": 3, "room_spheres": 4}.get(args.scene, 0))
+        dev = ks.Device(0)
+        times = []
+        for i in range(args.build_only):
+            sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": BUILDERS[args.builder]})
+            info = sc.info()
+            times.append(info["bvh_build_ms"])
+            log(f"build {i}: {info['bvh_build_ms']:.2f} ms, {info['n_nodes']} nodes, depth {info['max_depth']}")
+            sc.close()
+        print(json.dumps({"scene": args.scene, "tris": hs.n_tris, "builder": args.builder, "nodes": info["n_nodes"],
+                          "depth": info["max_depth"], "bvh_build_ms": [round(t, 3) for t in times]}), flush=True)
+        dev.close()
+        return 0
 
     seed = 1 if args.scene.startswith("spheres") else 0
     t0 = time.time()

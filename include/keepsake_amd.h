@@ -181,12 +181,10 @@ typedef struct ks_bvh_options {
 
 /* BVH builders. HOST_SBVH: binned SAH with spatial splits on host threads (the reference's SBVH,
  * cuda/sbvh.cu:301-440). DEVICE: built by kernels on the scene's GPU, clustering triangles in Morton
- * order (PLOC). DEVICE_FAST: the radix tree over the same Morton order, faster to build, slower to
- * trace. Both device builders collapse subtrees into leaves by SAH (no reference counterpart; the
- * reference has no device BVH build). */
+ * order (PLOC) and collapsing subtrees into leaves by SAH (no reference counterpart; the reference has
+ * no device BVH build). */
 #define KS_BVH_BUILD_HOST_SBVH 0
 #define KS_BVH_BUILD_DEVICE 1
-#define KS_BVH_BUILD_DEVICE_FAST 2
 
 /* create_camera (camera.cpp:172-205) perspective branch: look_at(pos, target, up) with vfov in
  * degrees and aspect; when use_matrix != 0, to_world (row-major 4x4 camera-to-world) replaces look_at. */

@@ -238,7 +238,6 @@ class BvhOptions(C.Structure):
 
 KS_BVH_BUILD_HOST_SBVH = 0
 KS_BVH_BUILD_DEVICE = 1
-KS_BVH_BUILD_DEVICE_FAST = 2
 
 
 class CameraDesc(C.Structure):

@@ -1916,9 +1916,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
     sc->dev = dev;
     ks_bvh_options o{};
     if (opt) o = *opt;
-    KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE ||
-                   o.builder == KS_BVH_BUILD_DEVICE_FAST,
-               "unknown BVH builder");
+    KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE, "unknown BVH builder");
     const int width = o.branching == 2 ? 2 : 4;
     // an empty scene gets the host builder's single empty node whichever builder was asked for
     const bool device_build = o.builder != KS_BVH_BUILD_HOST_SBVH && d->n_tris > 0;
@@ -1970,7 +1968,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
         }
         std::string err;
         if (ksa::build_bvh_device(sc->verts.p, sc->tri_idx.p, d->n_tris, ht.alpha ? alpha_of.as<uint8_t>() : nullptr,
-                                  o, width, o.builder == KS_BVH_BUILD_DEVICE_FAST, s, dbvh, err) != 0) {
+                                  o, width, s, dbvh, err) != 0) {
             ksa::set_error(err);
             return KS_ERR_HIP;
         }

@@ -1,27 +1,21 @@
-// BVH build on the device. Both builders start from the same Morton order: 63-bit codes of the triangle
-// box centroids, sorted. Then either
-//   - PLOC (KS_BVH_BUILD_DEVICE, the default device build): parallel locally-ordered clustering
-//     (Meister and Bittner, TVCG 2018) merges mutual nearest neighbours (by merged surface area) within
-//     a window of the Morton order until one cluster is left; or
-//   - the radix tree (KS_BVH_BUILD_DEVICE_FAST): Karras, "Maximizing Parallelism in the Construction of
-//     BVHs, Octrees, and k-d Trees", HPG 2012, fitted bottom up.
-// Each new interior node decides by the surface area heuristic whether its subtree becomes one leaf.
-// A top-down pass per level then lays the tree out in the node formats the traversal reads
-// (GpuNode4 / GpuNode) and orders the triangle records (GpuTri) leaf by leaf.
+// BVH build on the device: parallel locally-ordered clustering (PLOC; Meister and Bittner, "Parallel
+// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction", TVCG 2018) over the Morton
+// order of the triangles' box centroids. Each new interior node decides by the surface area heuristic
+// whether its subtree becomes one leaf. A top-down pass per level then lays the tree out in the node
+// formats the traversal reads (GpuNode4 / GpuNode) and orders the triangle records (GpuTri) leaf by
+// leaf.
 //
 // The reference builds its SBVH on the host (cuda/sbvh.cu:301-440); its kd-tree builder
 // (cuda/cuda_parallel_kd_tree.cu) is a device pipeline of the same shape: bounds, sort, build,
-// layout. These builders have no spatial splits, so each triangle is referenced exactly once.
+// layout. This builder has no spatial splits, so each triangle is referenced exactly once.
 //
 // Kernels, n triangles:
 //   k_bounds      scene box and centroid box (block reduction, then ordered-integer atomics), per-
-//                 triangle boxes (32 B each)
+//               triangle boxes (32 B each)
 //   k_morton      63-bit Morton key of each centroid, value = triangle id
 //   radix sort    (rocPRIM) of the (key, id) pairs over bits 0..62
 //   k_leaves      leaf nodes in sorted order
-//   PLOC rounds   k_ploc_nn (neighbour search, the window staged in LDS), k_ploc_merge, rocPRIM select
-//   radix tree    k_hierarchy (splits and parent links), k_fit (one thread per leaf climbs; the second
-//                 arrival at a node fits it)
+//   per round     k_ploc_nn (neighbour search, the window staged in LDS), k_ploc_merge, rocPRIM select
 //   k_layout      one launch per level of the output tree
 //   k_records     triangle records in leaf order (vertices, triangle id, alpha-test flag)
 #include <hip/hip_runtime.h>
@@ -39,10 +33,8 @@
 namespace ksa {
 namespace {
 
-constexpr uint32_t kNone = 0xffffffffu;
-constexpr uint32_t kRadixLeaf = 0x80000000u; // radix-tree child that is a leaf: low bits = sorted position
 constexpr int kBlock = 256;
-constexpr int kMaxLevels = 160; // the radix tree over 63-bit keys + 32-bit index ties is at most 96 deep
+constexpr int kMaxLevels = 256; // levels of the output tree the layout accepts (checked, not assumed)
 
 // order-preserving map of floats onto unsigned integers (for atomic min / max)
 __device__ __host__ inline uint32_t ordered(float f) {
@@ -61,6 +53,33 @@ __device__ inline float area_of(float3 lo, float3 hi) {
     float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
     if (!(ex >= 0.0f && ey >= 0.0f && ez >= 0.0f)) return 0.0f;
     return ex * ey + ey * ez + ez * ex;
+}
+
+// Reserves k slots of a shared counter for each thread of the block with one atomic per block (a
+// counter that every wave bumps serialises on one address: 156K atomics per pass over 10M clusters).
+// Every thread of the block must call it. Returns this thread's first slot.
+__device__ inline uint32_t block_reserve(uint32_t *counter, uint32_t k) {
+    __shared__ uint32_t wave_base[kBlock / 64];
+    __shared__ uint32_t block_base;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = k;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) wave_base[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            const uint32_t v = wave_base[w];
+            wave_base[w] = total;
+            total += v;
+        }
+        block_base = total != 0u ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    return block_base + wave_base[wave] + incl - k;
 }
 
 // acc: [0..2] min of triangle boxes, [3..5] max, [6..8] min of centroids, [9..11] max
@@ -149,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float4 *__restrict__ tr
     ids[t] = t;
 }
 
-// Node ids shared by both tree builders: 0 .. n-1 are the leaves (sorted position p, triangle ids[p]);
+// Node ids: 0 .. n-1 are the leaves (sorted position p, triangle ids[p]);
 // n .. 2n-2 are interior nodes, whose children are child[id - n] and whose triangle count is
 // count[id - n]. node_box[2 id] = (lo, SAH cost), node_box[2 id + 1] = (hi, 1 if the subtree is laid
 // out as one leaf).
@@ -188,67 +207,7 @@ __device__ inline void fit_node(float4 *node_box, uint32_t *count, uint32_t n, u
     put_box(node_box, id, lo, hi, leaf ? as_leaf : split, leaf);
 }
 
-// ---- radix tree (fast build) ----
-
-// common prefix length of sorted keys i and j; equal keys are told apart by their positions
-__device__ inline int prefix(const uint64_t *__restrict__ k, int64_t n, int64_t i, int64_t j) {
-    if (j < 0 || j >= n) return -1;
-    uint64_t a = k[i], b = k[j];
-    if (a == b) return 64 + __clz((uint32_t)i ^ (uint32_t)j);
-    return __clzll(a ^ b);
-}
-
-// Interior node i (id n + i) of the radix tree over the n sorted keys: its split and its children.
-__global__ __launch_bounds__(kBlock) void k_hierarchy(const uint64_t *__restrict__ keys, uint32_t n,
-                                                      uint2 *__restrict__ child, uint32_t *__restrict__ parent,
-                                                      uint32_t *__restrict__ arrivals) {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= (int64_t)n - 1) return;
-    const int64_t nn = n;
-    const int d = prefix(keys, nn, i, i + 1) - prefix(keys, nn, i, i - 1) > 0 ? 1 : -1;
-    const int dmin = prefix(keys, nn, i, i - d);
-    int64_t lmax = 2;
-    while (prefix(keys, nn, i, i + lmax * d) > dmin) lmax *= 2;
-    int64_t l = 0;
-    for (int64_t t = lmax / 2; t >= 1; t /= 2)
-        if (prefix(keys, nn, i, i + (l + t) * d) > dmin) l += t;
-    const int64_t j = i + l * d;
-    const int dnode = prefix(keys, nn, i, j);
-    int64_t s = 0;
-    for (int64_t t = (l + 1) / 2;; t = (t + 1) / 2) {
-        if (prefix(keys, nn, i, i + (s + t) * d) > dnode) s += t;
-        if (t == 1) break;
-    }
-    const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
-    const int64_t first = i < j ? i : j, last = i < j ? j : i;
-    const uint32_t left = first == gamma ? (uint32_t)gamma : n + (uint32_t)gamma;
-    const uint32_t right = last == gamma + 1 ? (uint32_t)(gamma + 1) : n + (uint32_t)(gamma + 1);
-    child[i] = make_uint2(left, right);
-    arrivals[i] = 0u;
-    parent[left] = n + (uint32_t)i;
-    parent[right] = n + (uint32_t)i;
-    if (i == 0) parent[n] = kNone;
-}
-
-// One thread per leaf climbs towards the root. The arrival counter of a node lets the second of its two
-// children's threads continue (after both children's boxes are written); the first one stops.
-__global__ __launch_bounds__(kBlock) void k_fit(uint32_t n, const uint2 *__restrict__ child,
-                                                const uint32_t *__restrict__ parent, uint32_t *arrivals,
-                                                float4 *node_box, uint32_t *count, uint32_t max_leaf,
-                                                float trav_cost) {
-    uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    uint32_t node = parent[p];
-    for (int step = 0; node != kNone && step < kMaxLevels; ++step) { // a radix tree is at most 96 deep
-        uint32_t seen = __hip_atomic_fetch_add(arrivals + (node - n), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (seen == 0u) return; // the sibling subtree is not finished: its thread continues
-        const uint2 c = child[node - n];
-        fit_node(node_box, count, n, node, c.x, c.y, max_leaf, trav_cost);
-        node = parent[node];
-    }
-}
-
-// ---- PLOC (the default device build) ----
+// ---- clustering ----
 // Parallel locally-ordered clustering (Meister and Bittner, "Parallel Locally-Ordered Clustering for
 // Bounding Volume Hierarchy Construction", TVCG 2018): the clusters stay in Morton order; each finds
 // the neighbour within kRadius positions whose merged box has the smallest surface area; mutual
@@ -308,35 +267,53 @@ __global__ __launch_bounds__(kBlock) void k_ploc_nn(const uint32_t *__restrict__
     nn[i] = bj;
 }
 
-// Mutual neighbours i < j merge into a new node that takes position i; position j is dropped.
+// Mutual neighbours i < j merge into a new node that takes position i; position j is dropped. Each
+// thread takes kMergeItems positions, kBlock apart; every thread reaches the reservation.
+constexpr int kMergeItems = 4;
+
 __global__ __launch_bounds__(kBlock) void k_ploc_merge(const uint32_t *__restrict__ clusters, uint32_t m,
                                                        const uint32_t *__restrict__ nn, uint32_t n, uint32_t *next_id,
                                                        float4 *node_box, uint2 *__restrict__ child, uint32_t *count,
                                                        uint32_t *__restrict__ merged, uint8_t *__restrict__ keep,
                                                        uint32_t max_leaf, float trav_cost) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t j = nn[i];
-    const bool mutual = j < m && nn[j] == i;
-    if (!mutual) {
-        merged[i] = clusters[i];
+    uint32_t pos[kMergeItems], partner[kMergeItems];
+    uint32_t makes = 0;
+    bool make[kMergeItems];
+#pragma unroll
+    for (int r = 0; r < kMergeItems; ++r) {
+        const uint32_t i = (blockIdx.x * kMergeItems + r) * kBlock + threadIdx.x;
+        pos[r] = i;
+        make[r] = false;
+        if (i >= m) continue;
+        const uint32_t j = nn[i];
+        partner[r] = j;
+        const bool mutual = j < m && nn[j] == i;
+        if (!mutual) {
+            merged[i] = clusters[i];
+            keep[i] = 1;
+        } else if (j < i) {
+            keep[i] = 0; // the upper end of a pair
+        } else {
+            make[r] = true;
+            ++makes;
+        }
+    }
+    uint32_t k = block_reserve(next_id, makes);
+#pragma unroll
+    for (int r = 0; r < kMergeItems; ++r) {
+        if (!make[r]) continue;
+        const uint32_t i = pos[r];
+        if (k >= n - 1) { // cannot happen (n - 1 merges in all); the host checks the counter
+            keep[i] = 0;
+            continue;
+        }
+        const uint32_t id = n + k, a = clusters[i], b = clusters[partner[r]];
+        child[k] = make_uint2(a, b);
+        fit_node(node_box, count, n, id, a, b, max_leaf, trav_cost);
+        merged[i] = id;
         keep[i] = 1;
-        return;
+        ++k;
     }
-    if (j < i) {
-        keep[i] = 0;
-        return;
-    }
-    const uint32_t k = atomicAdd(next_id, 1u);
-    if (k >= n - 1) { // cannot happen: n leaves make n - 1 merges; the host sees the counter
-        keep[i] = 0;
-        return;
-    }
-    const uint32_t id = n + k, a = clusters[i], b = clusters[j];
-    child[k] = make_uint2(a, b);
-    fit_node(node_box, count, n, id, a, b, max_leaf, trav_cost);
-    merged[i] = id;
-    keep[i] = 1;
 }
 
 // ---- layout ----
@@ -363,14 +340,16 @@ __global__ __launch_bounds__(kBlock) void k_layout(const uint4 *__restrict__ que
                                                    uint32_t *__restrict__ order, uint4 *__restrict__ next,
                                                    uint32_t *next_count, uint32_t next_base, uint32_t cap) {
     const uint32_t qi = blockIdx.x * kBlock + threadIdx.x;
-    if (qi >= qn) return;
-    const uint4 e = queue[qi];
-    if (e.y >= cap) return; // cannot happen (an output node per interior tree node); the host reports it
+    // every thread reaches the reservation below; threads past the queue (or past the node bound, which
+    // cannot happen: an output node per interior tree node) lay out nothing
+    const bool live = qi < qn && queue[qi].y < cap;
+    const uint4 e = live ? queue[qi] : make_uint4(0u, 0u, 0u, 0u);
     uint32_t c[W];
     Slot cs[W];
-    int nc;
-    const Slot self = slot_of(node_box, e.x);
-    if (root_level && self.terminal) {
+    int nc = 0;
+    const Slot self = live ? slot_of(node_box, e.x) : Slot{};
+    if (!live) {
+    } else if (root_level && self.terminal) {
         c[0] = e.x;
         cs[0] = self;
         nc = 1;
@@ -405,6 +384,12 @@ __global__ __launch_bounds__(kBlock) void k_layout(const uint4 *__restrict__ que
             ++nc;
         }
     }
+    uint32_t interior = 0;
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+        if (k < nc && !cs[k].terminal) ++interior;
+    uint32_t q = block_reserve(next_count, interior);
+    if (!live) return;
     uint32_t ptr[W];
     uint32_t first = e.z;
 #pragma unroll
@@ -430,9 +415,9 @@ __global__ __launch_bounds__(kBlock) void k_layout(const uint4 *__restrict__ que
                 }
             }
         } else {
-            const uint32_t q = atomicAdd(next_count, 1u);
             if (next_base + q < cap) next[q] = make_uint4(c[k], next_base + q, first, 0u);
             ptr[k] = next_base + q;
+            ++q;
         }
         first += cnt;
     }
@@ -506,8 +491,7 @@ inline uint32_t blocks_for(size_t n) { return (uint32_t)((n + kBlock - 1) / kBlo
 } // namespace
 
 int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const uint8_t *mat_alpha,
-                     const ks_bvh_options &opt, int width, bool radix_tree, void *stream, DeviceBvh &out,
-                     std::string &err) {
+                     const ks_bvh_options &opt, int width, void *stream, DeviceBvh &out, std::string &err) {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     out = DeviceBvh{};
 #define KS_DB(call)                                                                                            \
@@ -550,7 +534,7 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     KS_DB(sc.get(count, ni));
     KS_DB(sc.get(clusters[0], n));
     KS_DB(sc.get(clusters[1], n));
-    KS_DB(sc.get(aux, (size_t)n + ni)); // PLOC: each cluster's neighbour; radix tree: each node's parent
+    KS_DB(sc.get(aux, n)); // each cluster's neighbour
     KS_DB(sc.get(keep, n));
     KS_DB(sc.get(next_id, 1));
     KS_DB(sc.get(order, n));
@@ -576,21 +560,12 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     sc.ptrs.push_back(temp);
     size_t tb = temp_bytes;
     KS_DB(rocprim::radix_sort_pairs(temp, tb, keys[0], keys[1], ids[0], ids[1], n, 0, 63, s));
-    const uint64_t *skeys = keys[1];
     const uint32_t *sids = ids[1];
     k_leaves<<<blocks_for(n), kBlock, 0, s>>>(tri_box, sids, n, node_box, clusters[0]);
     KS_DB(hipGetLastError());
     uint32_t root = 0; // a one-triangle scene: the leaf itself
     uint32_t rounds = 0;
-    if (n > 1 && radix_tree) {
-        uint32_t *parent = aux;
-        k_hierarchy<<<blocks_for(n - 1), kBlock, 0, s>>>(skeys, n, child, parent, count);
-        // count doubles as the arrival counters until k_fit overwrites each with the node's count: a
-        // node's counter is read (atomic) before fit_node writes its count, and nothing reads it after
-        k_fit<<<blocks_for(n), kBlock, 0, s>>>(n, child, parent, count, node_box, count, max_leaf, trav_cost);
-        KS_DB(hipGetLastError());
-        root = n;
-    } else if (n > 1) {
+    if (n > 1) {
         KS_DB(hipMemsetAsync(next_id, 0, 4, s));
         uint32_t m = n;
         const int cur = 0; // clusters[0] holds the live clusters, clusters[1] the merged ones before compaction
@@ -600,8 +575,9 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
                 return -1;
             }
             k_ploc_nn<<<blocks_for(m), kBlock, 0, s>>>(clusters[cur], m, node_box, aux);
-            k_ploc_merge<<<blocks_for(m), kBlock, 0, s>>>(clusters[cur], m, aux, n, next_id, node_box, child, count,
-                                                           clusters[cur ^ 1], keep, max_leaf, trav_cost);
+            k_ploc_merge<<<blocks_for((m + kMergeItems - 1) / kMergeItems), kBlock, 0, s>>>(
+                clusters[cur], m, aux, n, next_id, node_box, child, count, clusters[cur ^ 1], keep, max_leaf,
+                trav_cost);
             KS_DB(hipGetLastError());
             // compaction keeps the order: clusters[1] -> clusters[0], the new count into level_count[0]
             tb = temp_bytes;

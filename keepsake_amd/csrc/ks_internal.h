@@ -73,10 +73,9 @@ struct DeviceBvh {
 };
 // verts: float4 per vertex, tri_idx: uint4 per triangle (vertex ids, material id in the low 24 bits of
 // w), mat_alpha: per-material alpha-test flag or null, stream: the hipStream_t to build on. Returns 0 or
-// -1 with `err` set. radix_tree: the fast radix-tree build instead of PLOC.
+// -1 with `err` set.
 int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n_tris, const uint8_t *mat_alpha,
-                     const ks_bvh_options &opt, int width, bool radix_tree, void *stream, DeviceBvh &out,
-                     std::string &err);
+                     const ks_bvh_options &opt, int width, void *stream, DeviceBvh &out, std::string &err);
 
 struct HostScene {
     std::vector<float> vertices, normals;

@@ -1,5 +1,5 @@
-"""The device BVH builders (KS_BVH_BUILD_DEVICE: clustering in Morton order; KS_BVH_BUILD_DEVICE_FAST: the
-radix tree over the same order): the trees they lay out, and the hits they give.
+"""The device BVH builder (KS_BVH_BUILD_DEVICE: clustering in Morton order): the trees it lays out, and the
+hits they give.
 
 Structure (read back with ks_scene_bvh_get): every triangle is referenced exactly once with its own
 vertices, every child box contains what lies below it, leaves hold at most max_prims_in_node
@@ -40,7 +40,7 @@ def _children(node, width):
     return lo[used], hi[used], ch[used]
 
 
-BUILDERS = [1, 2]  # KS_BVH_BUILD_DEVICE, KS_BVH_BUILD_DEVICE_FAST
+BUILDERS = [1]  # KS_BVH_BUILD_DEVICE
 
 
 def check_structure(sc, hs, builder, max_leaf=4):
@@ -191,7 +191,7 @@ def test_device_bvh_render_matches_host_sbvh(device, ks, cornell):
     W = H = 128
     p = ks.small_pt_params(bounces=8, render_width=W, render_height=H, spp=16)
     out = []
-    for builder in (ks.KS_BVH_BUILD_HOST_SBVH, ks.KS_BVH_BUILD_DEVICE, ks.KS_BVH_BUILD_DEVICE_FAST):
+    for builder in (ks.KS_BVH_BUILD_HOST_SBVH, ks.KS_BVH_BUILD_DEVICE):
         sc = ks.Scene(device, cornell, bvh={"builder": builder})
         film = ks.Film(device, W, H)
         ks.small_pt_render(sc, cornell.camera, p, film)
