@@ -11,7 +11,8 @@
 // one inner list per glTF mesh. Additional keys of this build: `emission` on a
 // material table (a constant shader field or an RGB triple: makes its triangles mesh lights),
 // output_pfm (also write PFM files beside the EXR / PNG ones), mis_mode ("reference" or
-// "consistent"), queue_size, and a [task.guide] table that turns on the
+// "consistent"), queue_size, bvh_builder ("sbvh", the default, or "device": the BVH built on the GPU),
+// and a [task.guide] table that turns on the
 // learned path guide (keys fraction, train_spp, learning_rate, beta1, beta2, log2_table_size,
 // base_resolution, max_resolution, train_batch, max_steps_per_pass, weight_clamp, seed, inference
 // ("bf16" or "fp8"); it needs mis_mode = "consistent").
@@ -434,6 +435,10 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
     else if (mis == "consistent") in.mis_mode = KS_MIS_CONSISTENT;
     else throw ConfigError("mis_mode must be reference or consistent");
     in.queue_size = args.load_integer("queue_size", 0);
+    const std::string builder = args.load_string("bvh_builder", "sbvh");
+    if (builder == "sbvh") st.bvh_builder = KS_BVH_BUILD_HOST_SBVH;
+    else if (builder == "device") st.bvh_builder = KS_BVH_BUILD_DEVICE;
+    else throw ConfigError("bvh_builder must be \"sbvh\" or \"device\", not \"" + builder + "\"");
     if (args.contains("guide")) {
         ConfigArgs g = args["guide"];
         if (in.mis_mode != KS_MIS_CONSISTENT)
@@ -478,6 +483,7 @@ std::string describe(const SmallPTSetup &st) {
                   "\"render_height\": %d, \"crop\": [%d, %d, %d, %d], \"spp\": %d, \"rng_seed\": %d, "
                   "\"include_background\": %s, \"backdrop\": [%g, %g, %g], \"clamp_direct\": %g, "
                   "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
+                  "\"bvh_builder\": %d, "
                   "\"guided\": %s, \"guide\": {\"fraction\": %g, \"train_spp\": %d, \"learning_rate\": %g, "
                   "\"log2_table_size\": %d, \"inference_fp8\": %d}, "
                   "\"camera\": {\"use_matrix\": %d, \"pos\": [%g, %g, %g], \"target\": [%g, %g, %g], "
@@ -489,7 +495,7 @@ std::string describe(const SmallPTSetup &st) {
                   in.render_width, in.render_height, in.crop_start_x, in.crop_start_y, in.crop_width, in.crop_height,
                   in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
                   in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,
-                  in.queue_size, st.guided ? "true" : "false", st.guide_cfg.guide_fraction, st.guide_cfg.train_spp,
+                  in.queue_size, st.bvh_builder, st.guided ? "true" : "false", st.guide_cfg.guide_fraction, st.guide_cfg.train_spp,
                   st.guide_cfg.learning_rate, st.guide_cfg.log2_table_size, st.guide_cfg.inference_fp8, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
                   c.up[0], c.up[1], c.up[2], c.vfov_deg, c.aspect, c.to_world[0], c.to_world[1], c.to_world[2],
                   c.to_world[3], c.to_world[4], c.to_world[5], c.to_world[6], c.to_world[7], c.to_world[8],
@@ -505,7 +511,9 @@ void small_pt(const ConfigArgs &args, const fs::path &task_dir, int task_id) {
     SmallPTSetup st = resolve_small_pt(args);
     Device dev(g_device_ordinal);
     auto b0 = std::chrono::steady_clock::now();
-    Scene scene(dev, st.scene);
+    ks_bvh_options bvh{};
+    bvh.builder = st.bvh_builder;
+    Scene scene(dev, st.scene, &bvh);
     double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - b0).count();
     RenderTarget2 rt(dev, (uint32_t)st.input.render_width, (uint32_t)st.input.render_height, st.input.backdrop);
     SmallPTInput in = st.input;

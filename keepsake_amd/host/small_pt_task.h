@@ -18,6 +18,7 @@ struct SmallPTSetup {
     ks_guide_config guide_cfg{};  // its keys (zero: library default)
     std::shared_ptr<ToneMapper> tone_mapper; // [task.tone_mapper]: also write tone-mapped PNGs
     bool output_pfm = false;                 // also write PFM files (this build's key)
+    int bvh_builder = KS_BVH_BUILD_HOST_SBVH; // bvh_builder = "sbvh" | "device" (this build's key)
 };
 
 // Parses one [[task]] table of type small_pt without touching the GPU.

@@ -254,3 +254,40 @@ def test_device_bvh_large_scene(device, ks, builder):
     assert np.array_equal(sc.occluded(rays), host.occluded(rays))
     host.close()
     sc.close()
+
+
+def test_ks_render_device_builder(tmp_path, ks, device, cornell):
+    """`bvh_builder = "device"` in a small_pt task renders over the device-built tree: the EXR equals the
+    API render over a device-built scene bit for bit."""
+    import subprocess
+    from pathlib import Path
+
+    from image_files import read_exr_rgb
+
+    tool = Path(__file__).resolve().parents[1] / "keepsake_amd" / "_lib" / "ks_render"
+    cfg = tmp_path / "c.toml"
+    cfg.write_text(
+        f"""
+output_dir = "{tmp_path / 'out'}"
+[[task]]
+type = "small_pt"
+task_dir = "cornell"
+procedural = "cornell"
+bounces = 8
+render_width = 64
+render_height = 64
+spp = 8
+bvh_builder = "device"
+"""
+    )
+    r = subprocess.run([str(tool), "-c", str(cfg)], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1000:], r.stderr[-1000:], flush=True)
+    assert r.returncode == 0
+    exr, _ = read_exr_rgb(tmp_path / "out" / "cornell" / "small_pt_spp8.exr")
+    sc = ks.Scene(device, cornell, bvh={"builder": ks.KS_BVH_BUILD_DEVICE})
+    film = ks.Film(device, 64, 64)
+    ks.small_pt_render(sc, cornell.camera, ks.small_pt_params(bounces=8, render_width=64, render_height=64, spp=8),
+                       film)
+    assert np.array_equal(exr, film.composite())
+    film.close()
+    sc.close()

@@ -43,8 +43,14 @@ def test_defaults_match_small_pt(tmp_path):
     assert t["spp_prog_interval"] == 32 and t["rng_seed"] == 0
     assert t["include_background"] is True and t["crop"] == [0, 0, 0, 0]
     assert t["mis_mode"] == 0
+    assert t["bvh_builder"] == 0  # the host SBVH
     # without a [task.camera] table the procedural scene's own camera is used
     assert t["camera"]["vfov_deg"] == pytest.approx(40.0)
+
+
+def test_bvh_builder_key(tmp_path):
+    (t,) = tasks(tmp_path, BASE + 'bvh_builder = "device"\n')
+    assert t["bvh_builder"] == 1
 
 
 def test_backdrop_disables_background(tmp_path):
@@ -147,6 +153,7 @@ scale_ray_diff = false
         ('camera_animation = "anim"\n', "camera_animation"),
         ('pixel_filter = { type = "gaussian" }\n', "box pixel filter"),
         ('mis_mode = "fancy"\n', "mis_mode"),
+        ('bvh_builder = "gpu"\n', "bvh_builder"),
         ('spp = "many"\n', "duplicate key"),
     ],
 )
