@@ -35,6 +35,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxLevels = 256; // levels of the output tree the layout accepts (checked, not assumed)
+static_assert(kMaxLevels <= kBlock, "k_init_acc clears one level counter per thread of one block");
 
 // order-preserving map of floats onto unsigned integers (for atomic min / max)
 __device__ __host__ inline uint32_t ordered(float f) {
