@@ -163,7 +163,10 @@ KS_DEV void camera_ray_diffs(const RenderConsts &rc, uint32_t x, uint32_t y, uin
 }
 
 // Batched renders: the segment holding work item w (last one whose base is <= w), its view, pixel and
-// sample. The result slot of a batched item is w itself.
+// sample. Within a segment the items are pixel-major (a pixel's samples are consecutive, as in a plain
+// render, so a wave traces neighbouring rays); rc.segs[rc.n_segs] is a sentinel whose base is the
+// chunk's item count, so every segment's sample count is (next base - base) / pixels. The result slot
+// of a batched item is w itself.
 KS_DEV uint32_t locate_batch(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &y, uint32_t &s) {
     int lo = 0, hi = rc.n_segs - 1;
     while (lo < hi) {
@@ -172,9 +175,10 @@ KS_DEV uint32_t locate_batch(const RenderConsts &rc, uint32_t w, uint32_t &x, ui
         else hi = mid - 1;
     }
     uint4 sg = rc.segs[lo];
+    const uint32_t nb = (rc.segs[lo + 1].x - sg.x) / sg.w;
     uint32_t local = w - sg.x;
-    uint32_t sl = local / sg.w;
-    uint32_t p = local - sl * sg.w;
+    uint32_t p = local / nb;
+    uint32_t sl = local - p * nb;
     s = sg.z + sl;
     const RenderConsts &v = rc.views[sg.y];
     uint32_t ry = p / (uint32_t)v.cw;
@@ -785,7 +789,7 @@ __global__ __launch_bounds__(256) void k_film_batch(const RenderConsts *views, c
     double r = rgb[3 * px], g = rgb[3 * px + 1], b = rgb[3 * px + 2];
     double wh = wts[2 * px], wm = wts[2 * px + 1];
     for (uint32_t k = 0; k < nb; ++k) {
-        float4 v4 = samp[(size_t)sg.x + (size_t)k * sg.w + p];
+        float4 v4 = samp[(size_t)sg.x + (size_t)p * nb + k];
         if (v4.w != 0.0f) {
             wh += 1.0;
             r += (double)v4.x;
@@ -2602,11 +2606,13 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
             probe.items += c0.seg_items[k];
         }
         HostToDev<uint4> psegs;
+        const int32_t probe_segs = (int32_t)probe.segs.size();
+        probe.segs.push_back(make_uint4((uint32_t)probe.items, 0u, 0u, 0u)); // sentinel: the item count
         KS_HIP(psegs.put(probe.segs.data(), probe.segs.size(), dev->stream));
         RenderConsts prc = hv[0];
         prc.views = dviews.p();
         prc.segs = psegs.p();
-        prc.n_segs = (int32_t)probe.segs.size();
+        prc.n_segs = probe_segs;
         hipEvent_t ea, eb;
         KS_HIP(hipEventCreate(&ea));
         KS_HIP(hipEventCreate(&eb));
@@ -2651,9 +2657,12 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
                        (int)kStatCount);
     Timer tm{dev};
     for (const BatchChunk &c : chunks) {
+        // declared before the device copies, so it outlives them (their release synchronises)
+        std::vector<uint4> segs = c.segs;
+        segs.push_back(make_uint4((uint32_t)c.items, 0u, 0u, 0u)); // sentinel: the chunk's item count
         HostToDev<uint4> dsegs;
         HostToDev<uint32_t> dtb, dsi;
-        KS_HIP(dsegs.put(c.segs.data(), c.segs.size(), dev->stream));
+        KS_HIP(dsegs.put(segs.data(), segs.size(), dev->stream));
         KS_HIP(dtb.put(c.thread_base.data(), c.thread_base.size(), dev->stream));
         KS_HIP(dsi.put(c.seg_items.data(), c.seg_items.size(), dev->stream));
         RenderConsts rc = hv[0];
