@@ -162,7 +162,8 @@ def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, ma
 
     for name, fp8 in methods:
         cfg = ks.guide_config(seed=1, guide_fraction=args.guide_fraction, train_spp=args.train_spp,
-                              max_steps_per_pass=args.guide_steps, inference_fp8=fp8)
+                              max_steps_per_pass=args.guide_steps, max_guided_bounce=args.guide_max_bounce,
+                              inference_fp8=fp8)
         guide = ks.Guide(dev, info["bound_lo"], info["bound_hi"], cfg)
         film.clear()
         passes.clear()
@@ -171,7 +172,7 @@ def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, ma
         hit_g = next(((s, t, r) for s, t, r in passes if r <= target), None)
         gst = guide.stats()
         res[name] = {"guide_fraction": args.guide_fraction, "train_spp": args.train_spp,
-                     "adam_steps_per_pass": args.guide_steps,
+                     "adam_steps_per_pass": args.guide_steps, "max_guided_bounce": args.guide_max_bounce,
                      "inference": "fp8" if fp8 else "bf16",
                      "seconds": round(hit_g[1], 3) if hit_g else None, "spp": hit_g[0] if hit_g else None,
                      "relmse_at_max_spp": passes[-1][2], "max_spp": passes[-1][0], "adam_steps": gst["steps"],
@@ -267,6 +268,8 @@ def main() -> int:
     ap.add_argument("--train-spp", type=int, default=8,
                     help="guided renders train during their first N spp (1-spp passes), then render frozen")
     ap.add_argument("--guide-steps", type=int, default=8, help="Adam steps after each training pass")
+    ap.add_argument("--guide-max-bounce", type=int, default=1,
+                    help="guide only vertices below this bounce (1: the camera rays' hits; 0: every vertex)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 48-view leg")
     ap.add_argument("--fp8-relmse", action="store_true",
                     help="also time the guided tracer with FP8 inference in the room's time-to-relMSE leg")

@@ -363,6 +363,8 @@ typedef struct ks_guide_config {
                                    this (20) */
     int32_t inference_fp8;      /* 1: sample the guide with FP8 (OCP e4m3) MLP inference, per-tensor power-of-two
                                    scales; training stays BF16 with FP32 master weights (0: BF16) */
+    int32_t max_guided_bounce;  /* vertices at this bounce or deeper sample the BSDF only and skip inference
+                                   (0: every vertex is guided) */
 } ks_guide_config;
 
 int ks_guide_create(ks_device *dev, const ks_guide_config *cfg, const float bound_lo[3], const float bound_hi[3],

@@ -339,6 +339,7 @@ class GuideConfig(C.Structure):
         ("train_spp", C.c_int32),
         ("weight_clamp", C.c_float),
         ("inference_fp8", C.c_int32),
+        ("max_guided_bounce", C.c_int32),
     ]
 
 

@@ -602,7 +602,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                     c.diffuse_trans_fwd = m.diffuse_trans_fwd;
                 }
                 const float4 *mixp = GUIDED && gs.mix ? gs.mix + 10 * (size_t)i : nullptr;
-                const float alpha_m = GUIDED && mixp && c.roughness >= gs.min_roughness ? gs.alpha : 0.0f;
+                const float alpha_m =
+                    GUIDED && mixp && c.roughness >= gs.min_roughness && bounce < gs.max_bounce ? gs.alpha : 0.0f;
 
                 // NEE (nee.cpp:11-58)
                 if (sc.n_lights > 0) {
@@ -1333,8 +1334,8 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         tm.end(e);
         if (gp && gp->shade.mix) {
             e = tm.begin(KS_K_GUIDE);
-            int ge = ksg::launch_infer(gp->guide, sc->ds, &ctrl->n_active, Q, pb[cur].rd, ws.hit.as<float4>(), gp->mix,
-                                       s);
+            int ge = ksg::launch_infer(gp->guide, sc->ds, &ctrl->n_active, Q, pb[cur].rd, ws.hit.as<float4>(),
+                                       pb[cur].state, gp->shade.max_bounce, gp->mix, s);
             tm.end(e);
             if (ge) return ge;
         }
@@ -2444,6 +2445,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
         gp.shade.recs = train ? gw.recs.as<float4>() : nullptr;
         gp.shade.alpha = fraction;
         gp.shade.min_roughness = kGuideMinRoughness;
+        gp.shade.max_bounce = ksg::guide_max_bounce(guide);
         if (gw.stamp == 0xffffffffu) { // stamps wrap: clear the records so no stale stamp can match
             KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
             gw.stamp = 0;

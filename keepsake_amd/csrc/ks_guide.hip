@@ -153,9 +153,12 @@ struct QueryHits { // wavefront paths with a surface hit: position, geometric no
     const uint32_t *n_dev;
     const float4 *rd;
     const float4 *hits;
+    const uint32_t *state; // path state: bounce in bits 16..23
+    int32_t max_bounce;    // paths at this bounce or deeper are not guided: no query
     __device__ uint32_t count() const { return *n_dev; }
     __device__ bool load(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
         if (i >= *n_dev) return false;
+        if ((int32_t)((state[i] >> 16) & 0xffu) >= max_bounce) return false;
         float4 h = hits[i];
         int32_t prim = __float_as_int(h.w);
         if (prim < 0) return false;
@@ -912,19 +915,22 @@ GuideNet net_of(const ks_guide *g) {
 
 ks_device *guide_device(const ks_guide *g) { return g->dev; }
 float guide_fraction(const ks_guide *g) { return g->cfg.guide_fraction; }
+int32_t guide_max_bounce(const ks_guide *g) {
+    return g->cfg.max_guided_bounce > 0 ? std::min(g->cfg.max_guided_bounce, 255) : 255;
+}
 int32_t guide_batch(const ks_guide *g) { return g->cfg.train_batch; }
 int32_t guide_max_steps(const ks_guide *g) { return g->cfg.max_steps_per_pass; }
 int32_t guide_train_spp(const ks_guide *g) { return g->cfg.train_spp; }
 
 int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
-                 const float4 *hits, float4 *mix, hipStream_t s) {
+                 const float4 *hits, const uint32_t *state, int32_t max_bounce, float4 *mix, hipStream_t s) {
     unsigned blocks = (n_max + kInferBlock - 1) / kInferBlock;
     if (g->fp8)
         hipLaunchKernelGGL((k_guide_infer<QueryHits, 2>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
-                           QueryHits{sc, n_dev, rd, hits}, mix, nullptr);
+                           QueryHits{sc, n_dev, rd, hits, state, max_bounce}, mix, nullptr);
     else
         hipLaunchKernelGGL((k_guide_infer<QueryHits, 0>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
-                           QueryHits{sc, n_dev, rd, hits}, mix, nullptr);
+                           QueryHits{sc, n_dev, rd, hits, state, max_bounce}, mix, nullptr);
     KS_HIP(hipGetLastError());
     return KS_OK;
 }

@@ -92,6 +92,7 @@ struct GuideShade {
     float alpha;       // probability of sampling the guide at a guidable vertex
     float min_roughness; // materials below this roughness use the BSDF only
     uint32_t stamp;
+    int32_t max_bounce;  // vertices at this bounce or deeper use the BSDF only (their mixtures are not computed)
 };
 
 // ---------------------------------------------------------------------------------------------------

@@ -14,8 +14,9 @@
 // "consistent"), queue_size, bvh_builder ("sbvh", the default, or "device": the BVH built on the GPU),
 // and a [task.guide] table that turns on the
 // learned path guide (keys fraction, train_spp, learning_rate, beta1, beta2, log2_table_size,
-// base_resolution, max_resolution, train_batch, max_steps_per_pass, weight_clamp, seed, inference
-// ("bf16" or "fp8"); it needs mis_mode = "consistent").
+// base_resolution, max_resolution, train_batch, max_steps_per_pass, weight_clamp, seed, max_bounce
+// (guide only vertices below this bounce; 0: all), inference ("bf16" or "fp8"); it needs mis_mode =
+// "consistent").
 #include "small_pt_task.h"
 
 #include "gltf_asset.h"
@@ -458,6 +459,7 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
         c.max_steps_per_pass = g.load_integer("max_steps_per_pass", 0);
         c.weight_clamp = g.load_float("weight_clamp", 0.0f);
         c.seed = (uint64_t)g.load_integer("seed", 0);
+        c.max_guided_bounce = (int32_t)g.load_integer("max_bounce", 0);
         std::string prec = g.load_string("inference", "bf16");
         if (prec != "bf16" && prec != "fp8") throw ConfigError("guide inference must be \"bf16\" or \"fp8\"");
         c.inference_fp8 = prec == "fp8";
@@ -485,7 +487,7 @@ std::string describe(const SmallPTSetup &st) {
                   "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
                   "\"bvh_builder\": %d, "
                   "\"guided\": %s, \"guide\": {\"fraction\": %g, \"train_spp\": %d, \"learning_rate\": %g, "
-                  "\"log2_table_size\": %d, \"inference_fp8\": %d}, "
+                  "\"log2_table_size\": %d, \"inference_fp8\": %d, \"max_bounce\": %d}, "
                   "\"camera\": {\"use_matrix\": %d, \"pos\": [%g, %g, %g], \"target\": [%g, %g, %g], "
                   "\"up\": [%g, %g, %g], \"vfov_deg\": %.6g, \"aspect\": %.6g, \"to_world\": [%g, %g, %g, %g, %g, %g, "
                   "%g, %g, %g, %g, %g, %g, %g, %g, %g, %g]}}",
@@ -496,7 +498,7 @@ std::string describe(const SmallPTSetup &st) {
                   in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
                   in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,
                   in.queue_size, st.bvh_builder, st.guided ? "true" : "false", st.guide_cfg.guide_fraction, st.guide_cfg.train_spp,
-                  st.guide_cfg.learning_rate, st.guide_cfg.log2_table_size, st.guide_cfg.inference_fp8, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
+                  st.guide_cfg.learning_rate, st.guide_cfg.log2_table_size, st.guide_cfg.inference_fp8, st.guide_cfg.max_guided_bounce, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
                   c.up[0], c.up[1], c.up[2], c.vfov_deg, c.aspect, c.to_world[0], c.to_world[1], c.to_world[2],
                   c.to_world[3], c.to_world[4], c.to_world[5], c.to_world[6], c.to_world[7], c.to_world[8],
                   c.to_world[9], c.to_world[10], c.to_world[11], c.to_world[12], c.to_world[13], c.to_world[14],

@@ -54,15 +54,16 @@ def test_zero_fraction_reproduces_plain(ks, device, room, fp8):
     guided.close()
 
 
-@pytest.mark.parametrize("fp8", [0, 1])
-def test_guided_is_unbiased(ks, device, room, fp8):
+@pytest.mark.parametrize("fp8,max_bounce", [(0, 0), (1, 0), (0, 1)])
+def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
+    """max_bounce 1: only the camera rays' hits are guided; deeper vertices sample the BSDF alone."""
     hs, sc = room
     ref = ks.Film(device, W, H)
     ks.small_pt_render(sc, hs.camera, _params(ks, 8192, 8192), ref)
     ref_rgb, ref_w = ref.download()
     info = sc.info()
     g = ks.Guide(device, info["bound_lo"], info["bound_hi"],
-                 ks.guide_config(seed=3, guide_fraction=0.5, inference_fp8=fp8))
+                 ks.guide_config(seed=3, guide_fraction=0.5, inference_fp8=fp8, max_guided_bounce=max_bounce))
     film = ks.Film(device, W, H)
     sums = []
 

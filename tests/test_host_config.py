@@ -175,10 +175,12 @@ fraction = 0.25
 train_spp = 32
 learning_rate = 5e-3
 log2_table_size = 18
+max_bounce = 2
 """
     (t,) = tasks(tmp_path, text)
     assert t["guided"] is True
     g = t["guide"]
+    assert g["max_bounce"] == 2
     assert g["fraction"] == pytest.approx(0.25) and g["train_spp"] == 32
     assert g["learning_rate"] == pytest.approx(5e-3) and g["log2_table_size"] == 18
     (plain,) = tasks(tmp_path, BASE)
