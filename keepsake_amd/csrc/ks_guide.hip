@@ -264,11 +264,15 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
         float uq[3] = {__shfl(u[0], sl), __shfl(u[1], sl), __shfl(u[2], sl)};
         const float nx = __shfl(Nn.x, sl), ny = __shfl(Nn.y, sl), nz = __shfl(Nn.z, sl);
         const float ox = __shfl(WO.x, sl), oy = __shfl(WO.y, sl), oz = __shfl(WO.z, sl);
-        float f[8];
-        grid_pair(net, 2 * g, uq, f[0], f[1]);
-        grid_pair(net, 2 * g + 1, uq, f[2], f[3]);
-        grid_pair(net, 8 + 2 * g, uq, f[4], f[5]);
-        grid_pair(net, 9 + 2 * g, uq, f[6], f[7]);
+        // lanes whose query is skipped (a miss, or a vertex past the guided bounces) issue no gathers;
+        // their encodings are zeroed below
+        float f[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (vq[t]) {
+            grid_pair(net, 2 * g, uq, f[0], f[1]);
+            grid_pair(net, 2 * g + 1, uq, f[2], f[3]);
+            grid_pair(net, 8 + 2 * g, uq, f[4], f[5]);
+            grid_pair(net, 9 + 2 * g, uq, f[6], f[7]);
+        }
         if (FP8) {
             const float rest[8] = {nx, ny, nz, ox, oy, oz, 1.0f, 0.0f};
             e0[t] = vq[t] ? fp8x8(f, sc[kInvSx]) : 0;
