@@ -2708,4 +2708,116 @@ int ora_bsdf_sample(const ora_material *m, const float *wo, const float *u, int6
     return 0;
 }
 
+// ----------------------------------------------------------------------------------------------------
+// Probes of the oracle's building blocks, for the tests that pin them against closed forms and
+// quadrature. They call the same functions the trace loop calls.
+// ----------------------------------------------------------------------------------------------------
+int ora_ggx_eval(float ax, float ay, int what, const float *a, const float *b, int64_t n, float *out) {
+    GGX g(ax, ay);
+    for (int64_t i = 0; i < n; ++i) {
+        V3 x(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+        if (what == 0) out[i] = g.D(x);
+        else if (what == 1) out[i] = g.lambda(x);
+        else if (what == 2) out[i] = g.G1(x);
+        else if (what == 3) out[i] = g.G2(x, V3(b[3 * i], b[3 * i + 1], b[3 * i + 2]));
+        else return -1;
+    }
+    return 0;
+}
+
+int ora_ggx_sample(float ax, float ay, const float *wo, const float *u, int64_t n, float *wm) {
+    GGX g(ax, ay);
+    for (int64_t i = 0; i < n; ++i) {
+        V3 m = g.sample(V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), u[2 * i], u[2 * i + 1]);
+        wm[3 * i] = m.x;
+        wm[3 * i + 1] = m.y;
+        wm[3 * i + 2] = m.z;
+    }
+    return 0;
+}
+
+int ora_fresnel(int what, const float *c, float eta, int64_t n, float *out) {
+    for (int64_t i = 0; i < n; ++i) {
+        if (what == 0) out[i] = fresnel_dielectric(c[i], eta);
+        else if (what == 1) out[i] = fresnel_schlick(c[i]);
+        else if (what == 2) out[i] = hg_phase(c[i], eta);
+        else return -1;
+    }
+    return 0;
+}
+
+int ora_bsdf_lobes(const ora_material *m, const float *wo, int64_t n, float *w4) {
+    Closure c = make_closure(*m);
+    for (int64_t i = 0; i < n; ++i) lobe_weights(V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), c, w4 + 4 * i);
+    return 0;
+}
+
+// One lobe's cosine-weighted value (0 diffuse, 1 diffuse transmission, 2 metallic specular, 3 dielectric
+// specular), without the hemisphere gating of eval (principled_bsdf.cpp:308-324); lobe -1 is the full
+// bsdf_pdf() without eval's zero-f gating.
+int ora_bsdf_lobe(const ora_material *m, int lobe, const float *wo, const float *wi, int64_t n, float *f) {
+    Closure c = make_closure(*m);
+    for (int64_t i = 0; i < n; ++i) {
+        V3 o(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), w(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        V3 r;
+        if (lobe == -1) {
+            f[3 * i] = f[3 * i + 1] = f[3 * i + 2] = bsdf_pdf(o, w, c);
+            continue;
+        }
+        if (lobe == 0) r = eval_diffuse(o, w, c);
+        else if (lobe == 1) r = eval_diffuse_transmission(o, w, c);
+        else if (lobe == 2) r = eval_metallic_specular(o, w, c);
+        else if (lobe == 3) r = eval_dielectric_specular(o, w, c);
+        else return -1;
+        f[3 * i] = r.x;
+        f[3 * i + 1] = r.y;
+        f[3 * i + 2] = r.z;
+    }
+    return 0;
+}
+
+// Light `light` of the flat table sampled from shading points p (n x 3) with u (n x 2): wi (n x 3),
+// the shadow-ray distance (n), the solid-angle pdf (n) and Le / pdf (n x 3).
+int ora_light_sample(void *scene, int light, const float *p, const float *u, int64_t n, float *wi, float *dist,
+                     float *pdf, float *le_over_pdf) {
+    const Scene &s = *(const Scene *)scene;
+    if (light < 0 || light >= (int)s.lights.size()) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        V3 w;
+        float d = 0.0f, pd = 0.0f;
+        V3 r = light_sample(s, s.lights[light], V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), u[2 * i], u[2 * i + 1], w, d,
+                            pd);
+        wi[3 * i] = w.x;
+        wi[3 * i + 1] = w.y;
+        wi[3 * i + 2] = w.z;
+        dist[i] = d;
+        pdf[i] = pd;
+        le_over_pdf[3 * i] = r.x;
+        le_over_pdf[3 * i + 1] = r.y;
+        le_over_pdf[3 * i + 2] = r.z;
+    }
+    return 0;
+}
+
+// MeshTriLight::pdf (light.cpp:607-628) of mesh light `light` for directions wi at distances dist.
+int ora_light_pdf(void *scene, int light, const float *wi, const float *dist, int64_t n, float *pdf) {
+    const Scene &s = *(const Scene *)scene;
+    if (light < 0 || light >= (int)s.lights.size() || s.lights[light].type != 3) return -1;
+    for (int64_t i = 0; i < n; ++i)
+        pdf[i] = mesh_light_pdf(s, s.lights[light], V3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), dist[i]);
+    return 0;
+}
+
+// PowerLightSampler::sample (light.cpp:677-683 over distrib.cpp:33-41): index and its probability.
+int ora_light_select(void *scene, const float *u, int64_t n, int32_t *idx, float *prob) {
+    const Scene &s = *(const Scene *)scene;
+    if (s.cdf.size() < 2) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        float pr = 0.0f;
+        idx[i] = (int32_t)cdf_sample(s.cdf, u[i], pr);
+        prob[i] = light_probability(s, (uint32_t)idx[i]);
+    }
+    return 0;
+}
+
 } // extern "C"

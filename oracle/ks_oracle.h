@@ -20,11 +20,18 @@
  *                        The split of `dimension` between digit permutation and scramble hash is pinned
  *                        only by the restatement itself (parity partially unpinned).
  *   ora_camera_rays    - closed-form pinhole direction (independent of the matrix inverse path).
- *   ora_intersect      - brute force over all triangles; watertightness on shared edges.
- *   ora_bsdf_*         - FP64 closed forms (Lambert/Disney retro, GGX normalisation), sample weight ==
- *                        eval/pdf, pdf integrates to 1 by quadrature.
- *   ora_render         - furnace/closed-form scenes (uniform emitter enclosure), equal-spp comparison is
- *                        the parity check for the GPU path.
+ *   ora_intersect      - brute force over all triangles; closed meshes never leak rays aimed at their
+ *                        vertices and edges; strict interval ends; the FP64 edge fallback decides a
+ *                        2^-24 case exactly (tests/test_oracle_geometry.py).
+ *   ora_bsdf_*, ora_ggx_*, ora_fresnel
+ *                      - GGX normalisation and projected-area identity by quadrature, angle-form D and
+ *                        Lambda, Beta-function G2, VNDF chi^2; two-index Fresnel, Brewster, TIR; lobes
+ *                        against FP64 closed forms, reciprocity; every lobe sampler by chi^2 against the
+ *                        density derived from it, weight == eval/pdf (tests/test_oracle_bsdf.py).
+ *   ora_light_*        - powers as closed forms of the scene, CDF as normalised prefix sums, upper_bound
+ *                        selection, uniform-area triangle samples with pdf d^2/(A|cos|), ambient sky
+ *                        pdf 1/(2 pi^2 sin theta) (tests/test_oracle_lights.py).
+ *   ora_render         - equal-spp comparison is the parity check for the GPU path.
  */
 #pragma once
 #include <stdint.h>
@@ -193,6 +200,25 @@ int ora_bsdf_eval(const ora_material *m, const float *wo, const float *wi, int64
 /* u: n x 3 (u_lobe, u_wi.x, u_wi.y) -> wi (n x 3), beta (n x 3), pdf (n). */
 int ora_bsdf_sample(const ora_material *m, const float *wo, const float *u, int64_t n, float *wi, float *beta,
                     float *pdf);
+
+/* Probes of building blocks, used by the tests that pin them (closed forms, quadrature, chi^2).
+ * GGX (microfacet.h:20-153) with clamped alphas: what 0 D(a), 1 Lambda(a), 2 G1(a), 3 G2(a, b). */
+int ora_ggx_eval(float ax, float ay, int what, const float *a, const float *b, int64_t n, float *out);
+/* VNDF sample (microfacet.h:98-118): wo (n x 3), u (n x 2) -> microfacet normal (n x 3). */
+int ora_ggx_sample(float ax, float ay, const float *wo, const float *u, int64_t n, float *wm);
+/* what 0 fresnel_dielectric(c, eta), 1 fresnel_schlick(c) (fresnel.h:9-30), 2 HG phase(c, g = eta). */
+int ora_fresnel(int what, const float *c, float eta, int64_t n, float *out);
+/* lobe_sample_weights (principled_bsdf.cpp:518-540): n x 4. */
+int ora_bsdf_lobes(const ora_material *m, const float *wo, int64_t n, float *w4);
+/* One lobe's f*cos (0 diffuse, 1 diffuse transmission, 2 metallic, 3 dielectric), or lobe -1: the
+ * mixture pdf (principled_bsdf.cpp:395-421) in every channel. f: n x 3. */
+int ora_bsdf_lobe(const ora_material *m, int lobe, const float *wo, const float *wi, int64_t n, float *f);
+/* Light `light` of the flat table sampled at points p (n x 3) with u (n x 2). */
+int ora_light_sample(void *scene, int light, const float *p, const float *u, int64_t n, float *wi, float *dist,
+                     float *pdf, float *le_over_pdf);
+int ora_light_pdf(void *scene, int light, const float *wi, const float *dist, int64_t n, float *pdf);
+/* Light selection from u (n): index and probability. */
+int ora_light_select(void *scene, const float *u, int64_t n, int32_t *idx, float *prob);
 
 #ifdef __cplusplus
 }

@@ -153,6 +153,14 @@ _SIGS = {
     "ora_camera_matrices": ([C.POINTER(OraCamera), _P, _P], C.c_int),
     "ora_bsdf_eval": ([C.POINTER(OraMaterial), _P, _P, C.c_int64, _P, _P], C.c_int),
     "ora_bsdf_sample": ([C.POINTER(OraMaterial), _P, _P, C.c_int64, _P, _P, _P], C.c_int),
+    "ora_ggx_eval": ([C.c_float, C.c_float, C.c_int, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_ggx_sample": ([C.c_float, C.c_float, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_fresnel": ([C.c_int, _P, C.c_float, C.c_int64, _P], C.c_int),
+    "ora_bsdf_lobes": ([C.POINTER(OraMaterial), _P, C.c_int64, _P], C.c_int),
+    "ora_bsdf_lobe": ([C.POINTER(OraMaterial), C.c_int, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_light_sample": ([_P, C.c_int, _P, _P, C.c_int64, _P, _P, _P, _P], C.c_int),
+    "ora_light_pdf": ([_P, C.c_int, _P, _P, C.c_int64, _P], C.c_int),
+    "ora_light_select": ([_P, _P, C.c_int64, _P, _P], C.c_int),
 }
 
 _libs: dict = {}
@@ -341,6 +349,30 @@ class OracleScene:
         self.lib.ora_light_table(self.h, C.byref(n), _p(prim), _p(power), _p(cdf))
         return prim, power, cdf
 
+    def light_sample(self, light, p, u):
+        """Light `light` of the flat table sampled at points p (n, 3) with u (n, 2): wi, shadow-ray
+        distance, solid-angle pdf, Le / pdf."""
+        p = np.ascontiguousarray(np.broadcast_to(p, (len(u), 3)), np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        n = u.shape[0]
+        wi, d, pdf, b = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+        assert self.lib.ora_light_sample(self.h, light, _p(p), _p(u), n, _p(wi), _p(d), _p(pdf), _p(b)) == 0
+        return wi, d, pdf, b
+
+    def light_pdf(self, light, wi, dist):
+        wi = np.ascontiguousarray(wi, np.float32)
+        dist = np.ascontiguousarray(dist, np.float32)
+        out = np.zeros(wi.shape[0], np.float32)
+        assert self.lib.ora_light_pdf(self.h, light, _p(wi), _p(dist), wi.shape[0], _p(out)) == 0
+        return out
+
+    def light_select(self, u):
+        u = np.ascontiguousarray(u, np.float32)
+        idx = np.zeros(u.shape[0], np.int32)
+        prob = np.zeros(u.shape[0], np.float32)
+        assert self.lib.ora_light_select(self.h, _p(u), u.shape[0], _p(idx), _p(prob)) == 0
+        return idx, prob
+
     def sky_eval(self, dirs):
         """The sky-map light's radiance (n, 3) and solid-angle pdf (n) at world directions (n, 3)."""
         dirs = np.ascontiguousarray(dirs, np.float32)
@@ -484,3 +516,63 @@ def bsdf_sample(mat, wo, u, variant: str = ""):
     pdf = np.zeros(n, np.float32)
     load(variant).ora_bsdf_sample(C.byref(m), _p(wo), _p(u), n, _p(wi), _p(beta), _p(pdf))
     return wi, beta, pdf
+
+
+GGX_WHAT = {"D": 0, "lambda": 1, "G1": 2, "G2": 3}
+
+
+def ggx(what, ax, ay, a, b=None, variant: str = ""):
+    """GGX building block of the oracle (microfacet.h:20-153): D(a), lambda(a), G1(a) or G2(a, b)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(a if b is None else b, np.float32)
+    out = np.zeros(a.shape[0], np.float32)
+    assert load(variant).ora_ggx_eval(ax, ay, GGX_WHAT[what], _p(a), _p(b), a.shape[0], _p(out)) == 0
+    return out
+
+
+def ggx_sample(ax, ay, wo, u):
+    wo = np.ascontiguousarray(np.broadcast_to(wo, (len(u), 3)), np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    out = np.zeros((u.shape[0], 3), np.float32)
+    assert load().ora_ggx_sample(ax, ay, _p(wo), _p(u), u.shape[0], _p(out)) == 0
+    return out
+
+
+def fresnel_dielectric(c, eta):
+    c = np.ascontiguousarray(c, np.float32)
+    out = np.zeros(c.shape[0], np.float32)
+    assert load().ora_fresnel(0, _p(c), eta, c.shape[0], _p(out)) == 0
+    return out
+
+
+def fresnel_schlick(c):
+    c = np.ascontiguousarray(c, np.float32)
+    out = np.zeros(c.shape[0], np.float32)
+    assert load().ora_fresnel(1, _p(c), 0.0, c.shape[0], _p(out)) == 0
+    return out
+
+
+def hg_phase(c, g):
+    c = np.ascontiguousarray(c, np.float32)
+    out = np.zeros(c.shape[0], np.float32)
+    assert load().ora_fresnel(2, _p(c), g, c.shape[0], _p(out)) == 0
+    return out
+
+
+def bsdf_lobe_weights(mat, wo):
+    m = material(mat)
+    wo = np.ascontiguousarray(wo, np.float32)
+    out = np.zeros((wo.shape[0], 4), np.float32)
+    assert load().ora_bsdf_lobes(C.byref(m), _p(wo), wo.shape[0], _p(out)) == 0
+    return out
+
+
+def bsdf_lobe(mat, lobe, wo, wi):
+    """One lobe's f*cos (0 diffuse, 1 diffuse transmission, 2 metallic, 3 dielectric); lobe -1 gives the
+    mixture pdf without eval's zero-f gating (first column)."""
+    m = material(mat)
+    wo = np.ascontiguousarray(np.broadcast_to(wo, np.shape(wi)), np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    out = np.zeros((wi.shape[0], 3), np.float32)
+    assert load().ora_bsdf_lobe(C.byref(m), lobe, _p(wo), _p(wi), wi.shape[0], _p(out)) == 0
+    return out
