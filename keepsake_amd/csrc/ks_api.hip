@@ -43,7 +43,7 @@ void set_error(const std::string &msg) { g_error = msg; }
         hipError_t e_ = (call);                                                                                    \
         if (e_ != hipSuccess) {                                                                                    \
             ksa::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));                              \
-            return KS_ERR_HIP;                                                                                     \
+            return e_ == hipErrorOutOfMemory ? KS_ERR_OOM : KS_ERR_HIP;                                            \
         }                                                                                                          \
     } while (0)
 
@@ -1471,52 +1471,60 @@ extern "C" {
 const char *ks_last_error(void) { return ksa::g_error.c_str(); }
 
 int ks_device_count(int32_t *count) {
-    KS_REQUIRE(count, "count is null");
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess) n = 0;
-    *count = n;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(count, "count is null");
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess) n = 0;
+        *count = n;
+        return KS_OK;
+    });
 }
 
 int ks_device_open(int32_t ordinal, ks_device **out) {
-    KS_REQUIRE(out, "out is null");
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        ksa::set_error("no HIP device available");
-        return KS_ERR_NO_DEVICE;
-    }
-    KS_REQUIRE(ordinal >= 0 && ordinal < n, "device ordinal out of range");
-    KS_HIP(hipSetDevice(ordinal));
-    auto d = std::make_unique<ks_device>();
-    d->ordinal = ordinal;
-    KS_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
-    hipDeviceProp_t prop;
-    KS_HIP(hipGetDeviceProperties(&prop, ordinal));
-    d->n_cu = prop.multiProcessorCount;
-    KS_HIP(hipHostMalloc((void **)&d->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
-    KS_HIP(hipHostMalloc((void **)&d->host_scratch, 64, hipHostMallocDefault));
-    *out = d.release();
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(out, "out is null");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            ksa::set_error("no HIP device available");
+            return KS_ERR_NO_DEVICE;
+        }
+        KS_REQUIRE(ordinal >= 0 && ordinal < n, "device ordinal out of range");
+        KS_HIP(hipSetDevice(ordinal));
+        auto d = std::make_unique<ks_device>();
+        d->ordinal = ordinal;
+        KS_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+        hipDeviceProp_t prop;
+        KS_HIP(hipGetDeviceProperties(&prop, ordinal));
+        d->n_cu = prop.multiProcessorCount;
+        KS_HIP(hipHostMalloc((void **)&d->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+        KS_HIP(hipHostMalloc((void **)&d->host_scratch, 64, hipHostMallocDefault));
+        *out = d.release();
+        return KS_OK;
+    });
 }
 
 int ks_device_close(ks_device *dev) {
-    if (!dev) return KS_OK;
-    (void)hipSetDevice(dev->ordinal);
-    if (dev->stream) (void)hipStreamSynchronize(dev->stream);
-    if (dev->host_ctrl) (void)hipHostFree(dev->host_ctrl);
-    if (dev->host_scratch) (void)hipHostFree(dev->host_scratch);
-    for (hipEvent_t e : dev->events) (void)hipEventDestroy(e);
-    if (dev->stream) (void)hipStreamDestroy(dev->stream);
-    delete dev;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        if (!dev) return KS_OK;
+        (void)hipSetDevice(dev->ordinal);
+        if (dev->stream) (void)hipStreamSynchronize(dev->stream);
+        if (dev->host_ctrl) (void)hipHostFree(dev->host_ctrl);
+        if (dev->host_scratch) (void)hipHostFree(dev->host_scratch);
+        for (hipEvent_t e : dev->events) (void)hipEventDestroy(e);
+        if (dev->stream) (void)hipStreamDestroy(dev->stream);
+        delete dev;
+        return KS_OK;
+    });
 }
 
 int ks_device_set_timing(ks_device *dev, int32_t enable) {
-    KS_REQUIRE(dev, "device is null");
-    dev->timing = enable ? 1 : 0;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev, "device is null");
+        dev->timing = enable ? 1 : 0;
+        return KS_OK;
+    });
 }
 
 } // extern "C"
@@ -1901,610 +1909,636 @@ int build_sky(const ks_sky_map_desc &d, float scene_radius, HostSky &out) {
 extern "C" {
 
 int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options *opt, ks_scene **out) {
-    KS_REQUIRE(dev && d && out, "null argument");
-    *out = nullptr;
-    KS_REQUIRE(d->n_tris == 0 || (d->vertices && d->indices && d->tri_geom), "scene arrays missing");
-    KS_REQUIRE(d->n_materials > 0 && d->materials, "scene needs at least one material");
-    KS_REQUIRE(d->n_materials < (1u << 24), "at most 2^24 - 1 materials");
-    KS_REQUIRE(d->n_lights <= 64, "at most 64 non-mesh lights");
-    for (uint32_t t = 0; t < d->n_tris; ++t) {
-        for (int k = 0; k < 3; ++k)
-            KS_REQUIRE(d->indices[3 * t + k] < d->n_vertices, "triangle index out of range");
-        KS_REQUIRE(d->tri_geom[t] < d->n_geoms, "triangle geometry id out of range");
-        KS_REQUIRE(t == 0 || d->tri_geom[t] >= d->tri_geom[t - 1], "triangles must be grouped by geometry");
-    }
-    for (uint32_t g = 0; g < d->n_geoms; ++g)
-        KS_REQUIRE(d->geom_material[g] < d->n_materials, "geometry material id out of range");
-    KS_HIP(hipSetDevice(dev->ordinal));
-    auto t0 = std::chrono::steady_clock::now();
-    auto sc = std::make_unique<ks_scene>();
-    sc->dev = dev;
-    ks_bvh_options o{};
-    if (opt) o = *opt;
-    KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE, "unknown BVH builder");
-    const int width = o.branching == 2 ? 2 : 4;
-    // an empty scene gets the host builder's single empty node whichever builder was asked for
-    const bool device_build = o.builder != KS_BVH_BUILD_HOST_SBVH && d->n_tris > 0;
-    ksa::BvhResult bvh;
-    double host_bvh_ms = 0.0;
-    if (!device_build) {
-        try {
-            const auto tb = std::chrono::steady_clock::now();
-            bvh = ksa::build_sbvh(d->vertices, d->indices, d->n_tris, o);
-            host_bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
-        } catch (const std::exception &ex) {
-            ksa::set_error(std::string("BVH build failed: ") + ex.what());
-            return KS_ERR_INVALID;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && d && out, "null argument");
+        *out = nullptr;
+        KS_REQUIRE(d->n_tris == 0 || (d->vertices && d->indices && d->tri_geom), "scene arrays missing");
+        KS_REQUIRE(d->n_materials > 0 && d->materials, "scene needs at least one material");
+        KS_REQUIRE(d->n_materials < (1u << 24), "at most 2^24 - 1 materials");
+        KS_REQUIRE(d->n_lights <= 64, "at most 64 non-mesh lights");
+        for (uint32_t t = 0; t < d->n_tris; ++t) {
+            for (int k = 0; k < 3; ++k)
+                KS_REQUIRE(d->indices[3 * t + k] < d->n_vertices, "triangle index out of range");
+            KS_REQUIRE(d->tri_geom[t] < d->n_geoms, "triangle geometry id out of range");
+            KS_REQUIRE(t == 0 || d->tri_geom[t] >= d->tri_geom[t - 1], "triangles must be grouped by geometry");
         }
-    }
-
-    HostTextures ht;
-    if (int te = build_textures(d, ht)) return te;
-    // the material constants each triangle sees (textured materials: their constant field values)
-    auto material_of = [&](uint32_t m) -> ks_material_desc {
-        if (!ht.mat_of.empty() && ht.mat_of[m] >= 0) return material_constants(ht.mats[ht.mat_of[m]]);
-        return d->materials[m];
-    };
-    const TexContext hctx{ht.metas.data(), &ht.luts};
-
-    std::vector<float4> verts(d->n_vertices);
-    for (uint32_t i = 0; i < d->n_vertices; ++i)
-        verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0.0f);
-    std::vector<uint4> tri_idx(d->n_tris);
-    for (uint32_t t = 0; t < d->n_tris; ++t)
-        tri_idx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2],
-                                d->geom_material[d->tri_geom[t]] |
-                                    ((uint32_t)(d->geom_flags ? d->geom_flags[d->tri_geom[t]] & 7u : 0u) << 24));
-    std::vector<Material> mats(d->n_materials);
-    for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(material_of(m));
-
-    hipStream_t s = dev->stream;
-    float scene_lo[3], scene_hi[3];
-    ksa::DeviceBvh dbvh;
-    if (device_build) {
-        KS_HIP(upload(sc->verts, verts, s));
-        KS_HIP(upload(sc->tri_idx, tri_idx, s));
-        DevBuf alpha_of;
-        if (ht.alpha) {
-            std::vector<uint8_t> flags(d->n_materials, 0);
-            for (uint32_t m = 0; m < d->n_materials; ++m)
-                flags[m] = ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity ? 1 : 0;
-            KS_HIP(upload(alpha_of, flags, s));
-        }
-        std::string err;
-        if (ksa::build_bvh_device(sc->verts.p, sc->tri_idx.p, d->n_tris, ht.alpha ? alpha_of.as<uint8_t>() : nullptr,
-                                  o, width, s, dbvh, err) != 0) {
-            ksa::set_error(err);
-            return KS_ERR_HIP;
-        }
-        // the scene takes the two arrays over
-        sc->nodes.p = dbvh.nodes;
-        sc->nodes.bytes = dbvh.node_bytes;
-        sc->tris.p = dbvh.tris;
-        sc->tris.bytes = dbvh.tri_bytes;
-        for (int k = 0; k < 3; ++k) scene_lo[k] = dbvh.lo[k], scene_hi[k] = dbvh.hi[k];
-    } else {
-        for (int k = 0; k < 3; ++k) scene_lo[k] = bvh.lo[k], scene_hi[k] = bvh.hi[k];
-    }
-
-    // Light list in PowerLightSampler order (light.cpp:636-677): non-mesh lights, then per emissive
-    // geometry the triangles with non-zero importance after swap compaction (light.cpp:524-558).
-    float ext[3] = {scene_hi[0] - scene_lo[0], scene_hi[1] - scene_lo[1], scene_hi[2] - scene_lo[2]};
-    float scene_radius = 0.5f * std::sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
-    const float pi = 3.14159265359f, two_pi = 6.28318530718f;
-    std::vector<Light> lights;
-    std::vector<float> powers;
-    int n_sky = 0;
-    int sky[4] = {0, 0, 0, 0};
-    HostSky host_sky;
-    bool have_map = false;
-    for (uint32_t i = 0; i < d->n_lights; ++i) {
-        const ks_light_desc &L = d->lights[i];
-        KS_REQUIRE(L.type >= 0 && L.type <= 3, "unknown light type");
-        Light l{};
-        l.type = L.type == KS_LIGHT_SKY_MAP ? 4 : L.type;
-        for (int k = 0; k < 3; ++k) {
-            l.color[k] = L.color[k];
-            l.vec[k] = L.vec[k];
-        }
-        float ph[3];
-        if (L.type == KS_LIGHT_SKY_MAP) {
-            KS_REQUIRE(d->sky_map, "a KS_LIGHT_SKY_MAP light needs ks_scene_desc.sky_map");
-            KS_REQUIRE(!have_map, "at most one environment-map sky per scene");
-            KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
-            int e = build_sky(*d->sky_map, scene_radius, host_sky);
-            if (e) return e;
-            have_map = true;
-            sky[n_sky++] = (int)i;
-            for (int k = 0; k < 3; ++k) ph[k] = host_sky.power[k];
-        } else if (L.type == KS_LIGHT_SKY_AMBIENT) { // SkyLight::power (light.cpp:136-158), 1x1 map
-            KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
-            sky[n_sky++] = (int)i;
-            float sin_theta = std::sin(0.5f / 1.0f * pi);
-            for (int k = 0; k < 3; ++k) ph[k] = ((l.color[k] * sin_theta) * ((pi / 1.0f) * (two_pi / 1.0f))) *
-                                                (pi * (scene_radius * scene_radius));
-        } else if (L.type == KS_LIGHT_DIRECTIONAL) {
-            float n = std::sqrt(l.vec[0] * l.vec[0] + l.vec[1] * l.vec[1] + l.vec[2] * l.vec[2]);
-            KS_REQUIRE(n > 0.0f, "directional light without direction");
-            for (int k = 0; k < 3; ++k) l.vec[k] /= n; // load_vec3("dir", true)
-            for (int k = 0; k < 3; ++k) ph[k] = l.color[k] * (pi * (scene_radius * scene_radius));
-        } else {
-            for (int k = 0; k < 3; ++k) ph[k] = l.color[k] * (4.0f * pi);
-        }
-        lights.push_back(l);
-        powers.push_back((ph[0] + ph[1] + ph[2]) / 3.0f);
-        sc->light_prim.push_back(-1);
-    }
-    std::vector<int32_t> tri_light(d->n_tris, -1);
-    for (uint32_t t0 = 0; t0 < d->n_tris;) {
-        uint32_t t1 = t0;
-        while (t1 < d->n_tris && d->tri_geom[t1] == d->tri_geom[t0]) ++t1;
-        const uint32_t gm = d->geom_material[d->tri_geom[t0]];
-        const ks_material_desc m = material_of(gm);
-        if (m.has_emission) {
-            const DevTexMaterial *tm = (!ht.mat_of.empty() && ht.mat_of[gm] >= 0) ? &ht.mats[ht.mat_of[gm]] : nullptr;
-            const bool em_tex = tm && tm->f[kFieldEmission].tex >= 0;
-            uint32_t N = t1 - t0;
-            std::vector<uint32_t> ids(N);
-            std::vector<float> imp, area;
-            mesh_light_importance(d, t0, t1, m, tm, hctx, imp, area);
-            for (uint32_t i = 0; i < N; ++i) ids[i] = i;
-            for (uint32_t i = 0; i < N;) {
-                if (imp[i] == 0.0f) {
-                    std::swap(ids[N - 1], ids[i]);
-                    std::swap(imp[N - 1], imp[i]);
-                    std::swap(area[N - 1], area[i]);
-                    --N;
-                } else {
-                    ++i;
-                }
-            }
-            for (uint32_t i = 0; i < N; ++i) {
-                Light l{};
-                l.type = 3;
-                l.prim = t0 + ids[i];
-                l.area = area[i];
-                l.color[3] = em_tex ? 1.0f : 0.0f; // emission read from its texture at each use
-                for (int k = 0; k < 3; ++k) {
-                    l.color[k] = m.emission[k];
-                    l.vec[k] = d->vertices[3 * d->indices[3 * l.prim] + k];
-                    l.v1[k] = d->vertices[3 * d->indices[3 * l.prim + 1] + k];
-                    l.v2[k] = d->vertices[3 * d->indices[3 * l.prim + 2] + k];
-                }
-                tri_light[l.prim] = (int32_t)lights.size();
-                lights.push_back(l);
-                powers.push_back((imp[i] + imp[i] + imp[i]) / 3.0f);
-                sc->light_prim.push_back((int32_t)l.prim);
+        for (uint32_t g = 0; g < d->n_geoms; ++g)
+            KS_REQUIRE(d->geom_material[g] < d->n_materials, "geometry material id out of range");
+        KS_HIP(hipSetDevice(dev->ordinal));
+        auto t0 = std::chrono::steady_clock::now();
+        auto sc = std::make_unique<ks_scene>();
+        sc->dev = dev;
+        ks_bvh_options o{};
+        if (opt) o = *opt;
+        KS_REQUIRE(o.builder == KS_BVH_BUILD_HOST_SBVH || o.builder == KS_BVH_BUILD_DEVICE, "unknown BVH builder");
+        const int width = o.branching == 2 ? 2 : 4;
+        // an empty scene gets the host builder's single empty node whichever builder was asked for
+        const bool device_build = o.builder != KS_BVH_BUILD_HOST_SBVH && d->n_tris > 0;
+        ksa::BvhResult bvh;
+        double host_bvh_ms = 0.0;
+        if (!device_build) {
+            try {
+                const auto tb = std::chrono::steady_clock::now();
+                bvh = ksa::build_sbvh(d->vertices, d->indices, d->n_tris, o);
+                host_bvh_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tb).count();
+            } catch (const std::exception &ex) {
+                ksa::set_error(std::string("BVH build failed: ") + ex.what());
+                return KS_ERR_INVALID;
             }
         }
-        t0 = t1;
-    }
-    std::vector<float> cdf;
-    if (!powers.empty()) { // DistribTable (distrib.cpp:9-31)
-        uint32_t n = (uint32_t)powers.size();
-        cdf.resize(n + 1);
-        cdf[0] = 0.0f;
-        float invn = 1.0f / (float)n;
-        for (uint32_t i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + powers[i - 1] * invn;
-        float acc = cdf[n];
-        if (acc == 0.0f) {
-            for (uint32_t i = 0; i <= n; ++i) cdf[i] = (float)i / (float)n;
-        } else {
-            float inv = 1.0f / acc;
-            for (uint32_t i = 0; i <= n; ++i) cdf[i] *= inv;
-        }
-        cdf[n] = 1.0f;
-    }
-    sc->light_power = powers;
-    sc->light_cdf = cdf;
 
-    // BVH4 visits address nodes with 32-bit byte offsets
-    const size_t n_nodes = device_build ? dbvh.n_nodes : width == 4 ? bvh.nodes4.size() : bvh.nodes.size();
-    KS_REQUIRE(width != 4 || n_nodes < (1u << 25), "BVH4 node array exceeds 4 GiB; build this scene with branching = 2");
-    if (!device_build) {
-        if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
-        else KS_HIP(upload(sc->nodes, bvh.nodes, s));
-        if (ht.alpha) { // alpha-tested triangle records carry a non-zero v1.w for the traversal filter
-            for (auto &g : bvh.tris) {
-                const uint32_t m = d->geom_material[d->tri_geom[g.tri]];
-                if (ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity) {
-                    const uint32_t one = 1u;
-                    std::memcpy(&g.v1[3], &one, 4);
-                }
-            }
-        }
-        KS_HIP(upload(sc->tris, bvh.tris, s));
-        KS_HIP(upload(sc->verts, verts, s));
-    }
-    if (d->normals) {
-        std::vector<float4> nrm(d->n_vertices);
+        HostTextures ht;
+        if (int te = build_textures(d, ht)) return te;
+        // the material constants each triangle sees (textured materials: their constant field values)
+        auto material_of = [&](uint32_t m) -> ks_material_desc {
+            if (!ht.mat_of.empty() && ht.mat_of[m] >= 0) return material_constants(ht.mats[ht.mat_of[m]]);
+            return d->materials[m];
+        };
+        const TexContext hctx{ht.metas.data(), &ht.luts};
+
+        std::vector<float4> verts(d->n_vertices);
         for (uint32_t i = 0; i < d->n_vertices; ++i)
-            nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
-        KS_HIP(upload(sc->normals, nrm, s));
-    }
-    if (!device_build) KS_HIP(upload(sc->tri_idx, tri_idx, s));
-    KS_HIP(upload(sc->tri_light, tri_light, s));
-    KS_HIP(upload(sc->mats, mats, s));
-    KS_HIP(upload(sc->lights, lights, s));
-    KS_HIP(upload(sc->cdf, cdf, s));
-    if (ht.textured) {
-        KS_HIP(upload(sc->tex_pool, ht.pool, s));
-        std::vector<DevTexture> metas = ht.metas;
-        for (auto &t : metas) t.data = sc->tex_pool.as<uint8_t>() + (t.data - ht.pool.data());
-        KS_HIP(upload(sc->tex_meta, metas, s));
-        KS_HIP(upload(sc->tex_mats, ht.mats, s));
-        KS_HIP(upload(sc->tex_mat_of, ht.mat_of, s));
-        KS_HIP(upload(sc->uvs, ht.uvs, s));
-        std::vector<TexLuts> luts(1, ht.luts);
-        KS_HIP(upload(sc->tex_luts, luts, s));
-    }
-    if (have_map) {
-        KS_HIP(upload(sc->sky_rgb, host_sky.rgb, s));
-        KS_HIP(upload(sc->sky_cond, host_sky.cond, s));
-        KS_HIP(upload(sc->sky_marg, host_sky.marg, s));
-        host_sky.map.rgb = sc->sky_rgb.as<float4>();
-        host_sky.map.cond = sc->sky_cond.as<float>();
-        host_sky.map.marg = sc->sky_marg.as<float>();
-    }
+            verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0.0f);
+        std::vector<uint4> tri_idx(d->n_tris);
+        for (uint32_t t = 0; t < d->n_tris; ++t)
+            tri_idx[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2],
+                                    d->geom_material[d->tri_geom[t]] |
+                                        ((uint32_t)(d->geom_flags ? d->geom_flags[d->tri_geom[t]] & 7u : 0u) << 24));
+        std::vector<Material> mats(d->n_materials);
+        for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(material_of(m));
 
-    DevScene &ds = sc->ds;
-    ds.nodes = sc->nodes.as<float4>();
-    ds.width = width;
-    ds.tris = sc->tris.as<float4>();
-    ds.verts = sc->verts.as<float4>();
-    ds.normals = d->normals ? sc->normals.as<float4>() : nullptr;
-    ds.tri_idx = sc->tri_idx.as<uint4>();
-    ds.tri_light = sc->tri_light.as<int32_t>();
-    ds.mats = sc->mats.as<Material>();
-    ds.lights = sc->lights.as<Light>();
-    ds.cdf = sc->cdf.as<float>();
-    ds.n_lights = (int32_t)lights.size();
-    ds.n_sky = n_sky;
-    for (int k = 0; k < 4; ++k) ds.sky[k] = sky[k];
-    ds.sky_map = host_sky.map;
-    ds.textured = ht.textured ? 1 : 0;
-    ds.alpha = ht.alpha ? 1 : 0;
-    if (ht.textured) {
-        ds.uvs = ht.uvs.empty() ? nullptr : sc->uvs.as<float2>();
-        ds.textures = sc->tex_meta.as<DevTexture>();
-        ds.tex_luts = sc->tex_luts.as<TexLuts>();
-        ds.tex_mats = ht.mats.empty() ? nullptr : sc->tex_mats.as<DevTexMaterial>();
-        ds.tex_mat_of = ht.mats.empty() ? nullptr : sc->tex_mat_of.as<int32_t>();
-    }
+        hipStream_t s = dev->stream;
+        float scene_lo[3], scene_hi[3];
+        ksa::DeviceBvh dbvh;
+        if (device_build) {
+            KS_HIP(upload(sc->verts, verts, s));
+            KS_HIP(upload(sc->tri_idx, tri_idx, s));
+            DevBuf alpha_of;
+            if (ht.alpha) {
+                std::vector<uint8_t> flags(d->n_materials, 0);
+                for (uint32_t m = 0; m < d->n_materials; ++m)
+                    flags[m] = ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity ? 1 : 0;
+                KS_HIP(upload(alpha_of, flags, s));
+            }
+            std::string err;
+            if (ksa::build_bvh_device(sc->verts.p, sc->tri_idx.p, d->n_tris, ht.alpha ? alpha_of.as<uint8_t>() : nullptr,
+                                      o, width, s, dbvh, err) != 0) {
+                ksa::set_error(err);
+                return KS_ERR_HIP;
+            }
+            // the scene takes the two arrays over
+            sc->nodes.p = dbvh.nodes;
+            sc->nodes.bytes = dbvh.node_bytes;
+            sc->tris.p = dbvh.tris;
+            sc->tris.bytes = dbvh.tri_bytes;
+            for (int k = 0; k < 3; ++k) scene_lo[k] = dbvh.lo[k], scene_hi[k] = dbvh.hi[k];
+        } else {
+            for (int k = 0; k < 3; ++k) scene_lo[k] = bvh.lo[k], scene_hi[k] = bvh.hi[k];
+        }
 
-    ks_scene_info &in = sc->info;
-    in.n_tris = d->n_tris;
-    in.n_vertices = d->n_vertices;
-    in.n_nodes = (uint32_t)n_nodes;
-    in.branching = (uint32_t)width;
-    in.n_refs = device_build ? dbvh.n_refs : (uint32_t)bvh.tris.size();
-    in.n_lights = (uint32_t)lights.size();
-    in.max_depth = device_build ? dbvh.max_depth : width == 4 ? bvh.max_depth4 : bvh.max_depth;
-    in.builder = device_build ? o.builder : KS_BVH_BUILD_HOST_SBVH;
-    in.bvh_build_ms = device_build ? dbvh.build_ms : host_bvh_ms;
-    // traversal stack bound: every interior node on the path leaves at most width - 1 entries behind
-    sc->max_stack = (uint32_t)(width - 1) * in.max_depth + 1;
-    in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->verts.bytes + sc->normals.bytes + sc->tri_idx.bytes +
-                      sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes + sc->tex_pool.bytes +
-                      sc->tex_mats.bytes + sc->uvs.bytes;
-    for (int k = 0; k < 3; ++k) {
-        in.bound_lo[k] = scene_lo[k];
-        in.bound_hi[k] = scene_hi[k];
-    }
-    in.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    *out = sc.release();
-    return KS_OK;
+        // Light list in PowerLightSampler order (light.cpp:636-677): non-mesh lights, then per emissive
+        // geometry the triangles with non-zero importance after swap compaction (light.cpp:524-558).
+        float ext[3] = {scene_hi[0] - scene_lo[0], scene_hi[1] - scene_lo[1], scene_hi[2] - scene_lo[2]};
+        float scene_radius = 0.5f * std::sqrt(ext[0] * ext[0] + ext[1] * ext[1] + ext[2] * ext[2]);
+        const float pi = 3.14159265359f, two_pi = 6.28318530718f;
+        std::vector<Light> lights;
+        std::vector<float> powers;
+        int n_sky = 0;
+        int sky[4] = {0, 0, 0, 0};
+        HostSky host_sky;
+        bool have_map = false;
+        for (uint32_t i = 0; i < d->n_lights; ++i) {
+            const ks_light_desc &L = d->lights[i];
+            KS_REQUIRE(L.type >= 0 && L.type <= 3, "unknown light type");
+            Light l{};
+            l.type = L.type == KS_LIGHT_SKY_MAP ? 4 : L.type;
+            for (int k = 0; k < 3; ++k) {
+                l.color[k] = L.color[k];
+                l.vec[k] = L.vec[k];
+            }
+            float ph[3];
+            if (L.type == KS_LIGHT_SKY_MAP) {
+                KS_REQUIRE(d->sky_map, "a KS_LIGHT_SKY_MAP light needs ks_scene_desc.sky_map");
+                KS_REQUIRE(!have_map, "at most one environment-map sky per scene");
+                KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
+                int e = build_sky(*d->sky_map, scene_radius, host_sky);
+                if (e) return e;
+                have_map = true;
+                sky[n_sky++] = (int)i;
+                for (int k = 0; k < 3; ++k) ph[k] = host_sky.power[k];
+            } else if (L.type == KS_LIGHT_SKY_AMBIENT) { // SkyLight::power (light.cpp:136-158), 1x1 map
+                KS_REQUIRE(n_sky < 4, "at most 4 sky lights");
+                sky[n_sky++] = (int)i;
+                float sin_theta = std::sin(0.5f / 1.0f * pi);
+                for (int k = 0; k < 3; ++k) ph[k] = ((l.color[k] * sin_theta) * ((pi / 1.0f) * (two_pi / 1.0f))) *
+                                                    (pi * (scene_radius * scene_radius));
+            } else if (L.type == KS_LIGHT_DIRECTIONAL) {
+                float n = std::sqrt(l.vec[0] * l.vec[0] + l.vec[1] * l.vec[1] + l.vec[2] * l.vec[2]);
+                KS_REQUIRE(n > 0.0f, "directional light without direction");
+                for (int k = 0; k < 3; ++k) l.vec[k] /= n; // load_vec3("dir", true)
+                for (int k = 0; k < 3; ++k) ph[k] = l.color[k] * (pi * (scene_radius * scene_radius));
+            } else {
+                for (int k = 0; k < 3; ++k) ph[k] = l.color[k] * (4.0f * pi);
+            }
+            lights.push_back(l);
+            powers.push_back((ph[0] + ph[1] + ph[2]) / 3.0f);
+            sc->light_prim.push_back(-1);
+        }
+        std::vector<int32_t> tri_light(d->n_tris, -1);
+        for (uint32_t t0 = 0; t0 < d->n_tris;) {
+            uint32_t t1 = t0;
+            while (t1 < d->n_tris && d->tri_geom[t1] == d->tri_geom[t0]) ++t1;
+            const uint32_t gm = d->geom_material[d->tri_geom[t0]];
+            const ks_material_desc m = material_of(gm);
+            if (m.has_emission) {
+                const DevTexMaterial *tm = (!ht.mat_of.empty() && ht.mat_of[gm] >= 0) ? &ht.mats[ht.mat_of[gm]] : nullptr;
+                const bool em_tex = tm && tm->f[kFieldEmission].tex >= 0;
+                uint32_t N = t1 - t0;
+                std::vector<uint32_t> ids(N);
+                std::vector<float> imp, area;
+                mesh_light_importance(d, t0, t1, m, tm, hctx, imp, area);
+                for (uint32_t i = 0; i < N; ++i) ids[i] = i;
+                for (uint32_t i = 0; i < N;) {
+                    if (imp[i] == 0.0f) {
+                        std::swap(ids[N - 1], ids[i]);
+                        std::swap(imp[N - 1], imp[i]);
+                        std::swap(area[N - 1], area[i]);
+                        --N;
+                    } else {
+                        ++i;
+                    }
+                }
+                for (uint32_t i = 0; i < N; ++i) {
+                    Light l{};
+                    l.type = 3;
+                    l.prim = t0 + ids[i];
+                    l.area = area[i];
+                    l.color[3] = em_tex ? 1.0f : 0.0f; // emission read from its texture at each use
+                    for (int k = 0; k < 3; ++k) {
+                        l.color[k] = m.emission[k];
+                        l.vec[k] = d->vertices[3 * d->indices[3 * l.prim] + k];
+                        l.v1[k] = d->vertices[3 * d->indices[3 * l.prim + 1] + k];
+                        l.v2[k] = d->vertices[3 * d->indices[3 * l.prim + 2] + k];
+                    }
+                    tri_light[l.prim] = (int32_t)lights.size();
+                    lights.push_back(l);
+                    powers.push_back((imp[i] + imp[i] + imp[i]) / 3.0f);
+                    sc->light_prim.push_back((int32_t)l.prim);
+                }
+            }
+            t0 = t1;
+        }
+        std::vector<float> cdf;
+        if (!powers.empty()) { // DistribTable (distrib.cpp:9-31)
+            uint32_t n = (uint32_t)powers.size();
+            cdf.resize(n + 1);
+            cdf[0] = 0.0f;
+            float invn = 1.0f / (float)n;
+            for (uint32_t i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + powers[i - 1] * invn;
+            float acc = cdf[n];
+            if (acc == 0.0f) {
+                for (uint32_t i = 0; i <= n; ++i) cdf[i] = (float)i / (float)n;
+            } else {
+                float inv = 1.0f / acc;
+                for (uint32_t i = 0; i <= n; ++i) cdf[i] *= inv;
+            }
+            cdf[n] = 1.0f;
+        }
+        sc->light_power = powers;
+        sc->light_cdf = cdf;
+
+        // BVH4 visits address nodes with 32-bit byte offsets
+        const size_t n_nodes = device_build ? dbvh.n_nodes : width == 4 ? bvh.nodes4.size() : bvh.nodes.size();
+        KS_REQUIRE(width != 4 || n_nodes < (1u << 25), "BVH4 node array exceeds 4 GiB; build this scene with branching = 2");
+        if (!device_build) {
+            if (width == 4) KS_HIP(upload(sc->nodes, bvh.nodes4, s));
+            else KS_HIP(upload(sc->nodes, bvh.nodes, s));
+            if (ht.alpha) { // alpha-tested triangle records carry a non-zero v1.w for the traversal filter
+                for (auto &g : bvh.tris) {
+                    const uint32_t m = d->geom_material[d->tri_geom[g.tri]];
+                    if (ht.mat_of[m] >= 0 && ht.mats[ht.mat_of[m]].has_opacity) {
+                        const uint32_t one = 1u;
+                        std::memcpy(&g.v1[3], &one, 4);
+                    }
+                }
+            }
+            KS_HIP(upload(sc->tris, bvh.tris, s));
+            KS_HIP(upload(sc->verts, verts, s));
+        }
+        if (d->normals) {
+            std::vector<float4> nrm(d->n_vertices);
+            for (uint32_t i = 0; i < d->n_vertices; ++i)
+                nrm[i] = make_float4(d->normals[3 * i], d->normals[3 * i + 1], d->normals[3 * i + 2], 0.0f);
+            KS_HIP(upload(sc->normals, nrm, s));
+        }
+        if (!device_build) KS_HIP(upload(sc->tri_idx, tri_idx, s));
+        KS_HIP(upload(sc->tri_light, tri_light, s));
+        KS_HIP(upload(sc->mats, mats, s));
+        KS_HIP(upload(sc->lights, lights, s));
+        KS_HIP(upload(sc->cdf, cdf, s));
+        if (ht.textured) {
+            KS_HIP(upload(sc->tex_pool, ht.pool, s));
+            std::vector<DevTexture> metas = ht.metas;
+            for (auto &t : metas) t.data = sc->tex_pool.as<uint8_t>() + (t.data - ht.pool.data());
+            KS_HIP(upload(sc->tex_meta, metas, s));
+            KS_HIP(upload(sc->tex_mats, ht.mats, s));
+            KS_HIP(upload(sc->tex_mat_of, ht.mat_of, s));
+            KS_HIP(upload(sc->uvs, ht.uvs, s));
+            std::vector<TexLuts> luts(1, ht.luts);
+            KS_HIP(upload(sc->tex_luts, luts, s));
+        }
+        if (have_map) {
+            KS_HIP(upload(sc->sky_rgb, host_sky.rgb, s));
+            KS_HIP(upload(sc->sky_cond, host_sky.cond, s));
+            KS_HIP(upload(sc->sky_marg, host_sky.marg, s));
+            host_sky.map.rgb = sc->sky_rgb.as<float4>();
+            host_sky.map.cond = sc->sky_cond.as<float>();
+            host_sky.map.marg = sc->sky_marg.as<float>();
+        }
+
+        DevScene &ds = sc->ds;
+        ds.nodes = sc->nodes.as<float4>();
+        ds.width = width;
+        ds.tris = sc->tris.as<float4>();
+        ds.verts = sc->verts.as<float4>();
+        ds.normals = d->normals ? sc->normals.as<float4>() : nullptr;
+        ds.tri_idx = sc->tri_idx.as<uint4>();
+        ds.tri_light = sc->tri_light.as<int32_t>();
+        ds.mats = sc->mats.as<Material>();
+        ds.lights = sc->lights.as<Light>();
+        ds.cdf = sc->cdf.as<float>();
+        ds.n_lights = (int32_t)lights.size();
+        ds.n_sky = n_sky;
+        for (int k = 0; k < 4; ++k) ds.sky[k] = sky[k];
+        ds.sky_map = host_sky.map;
+        ds.textured = ht.textured ? 1 : 0;
+        ds.alpha = ht.alpha ? 1 : 0;
+        if (ht.textured) {
+            ds.uvs = ht.uvs.empty() ? nullptr : sc->uvs.as<float2>();
+            ds.textures = sc->tex_meta.as<DevTexture>();
+            ds.tex_luts = sc->tex_luts.as<TexLuts>();
+            ds.tex_mats = ht.mats.empty() ? nullptr : sc->tex_mats.as<DevTexMaterial>();
+            ds.tex_mat_of = ht.mats.empty() ? nullptr : sc->tex_mat_of.as<int32_t>();
+        }
+
+        ks_scene_info &in = sc->info;
+        in.n_tris = d->n_tris;
+        in.n_vertices = d->n_vertices;
+        in.n_nodes = (uint32_t)n_nodes;
+        in.branching = (uint32_t)width;
+        in.n_refs = device_build ? dbvh.n_refs : (uint32_t)bvh.tris.size();
+        in.n_lights = (uint32_t)lights.size();
+        in.max_depth = device_build ? dbvh.max_depth : width == 4 ? bvh.max_depth4 : bvh.max_depth;
+        in.builder = device_build ? o.builder : KS_BVH_BUILD_HOST_SBVH;
+        in.bvh_build_ms = device_build ? dbvh.build_ms : host_bvh_ms;
+        // traversal stack bound: every interior node on the path leaves at most width - 1 entries behind
+        sc->max_stack = (uint32_t)(width - 1) * in.max_depth + 1;
+        in.device_bytes = sc->nodes.bytes + sc->tris.bytes + sc->verts.bytes + sc->normals.bytes + sc->tri_idx.bytes +
+                          sc->tri_light.bytes + sc->mats.bytes + sc->lights.bytes + sc->cdf.bytes + sc->tex_pool.bytes +
+                          sc->tex_mats.bytes + sc->uvs.bytes;
+        for (int k = 0; k < 3; ++k) {
+            in.bound_lo[k] = scene_lo[k];
+            in.bound_hi[k] = scene_hi[k];
+        }
+        in.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        *out = sc.release();
+        return KS_OK;
+    });
 }
 
 int ks_scene_free(ks_scene *scene) {
-    if (scene) {
-        (void)hipSetDevice(scene->dev->ordinal);
-        (void)hipStreamSynchronize(scene->dev->stream);
-        delete scene;
-    }
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        if (scene) {
+            (void)hipSetDevice(scene->dev->ordinal);
+            (void)hipStreamSynchronize(scene->dev->stream);
+            delete scene;
+        }
+        return KS_OK;
+    });
 }
 
 int ks_scene_info_get(const ks_scene *scene, ks_scene_info *info) {
-    KS_REQUIRE(scene && info, "null argument");
-    *info = scene->info;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && info, "null argument");
+        *info = scene->info;
+        return KS_OK;
+    });
 }
 
 int ks_scene_bvh_get(const ks_scene *scene, void *nodes, void *refs) {
-    KS_REQUIRE(scene && nodes && refs, "null argument");
-    KS_HIP(hipSetDevice(scene->dev->ordinal));
-    const size_t node_size = scene->info.branching == 4 ? sizeof(ksa::GpuNode4) : sizeof(ksa::GpuNode);
-    KS_HIP(hipStreamSynchronize(scene->dev->stream));
-    KS_HIP(hipMemcpy(nodes, scene->nodes.p, (size_t)scene->info.n_nodes * node_size, hipMemcpyDeviceToHost));
-    KS_HIP(hipMemcpy(refs, scene->tris.p, (size_t)scene->info.n_refs * sizeof(ksa::GpuTri), hipMemcpyDeviceToHost));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && nodes && refs, "null argument");
+        KS_HIP(hipSetDevice(scene->dev->ordinal));
+        const size_t node_size = scene->info.branching == 4 ? sizeof(ksa::GpuNode4) : sizeof(ksa::GpuNode);
+        KS_HIP(hipStreamSynchronize(scene->dev->stream));
+        KS_HIP(hipMemcpy(nodes, scene->nodes.p, (size_t)scene->info.n_nodes * node_size, hipMemcpyDeviceToHost));
+        KS_HIP(hipMemcpy(refs, scene->tris.p, (size_t)scene->info.n_refs * sizeof(ksa::GpuTri), hipMemcpyDeviceToHost));
+        return KS_OK;
+    });
 }
 
 int ks_scene_light_table(const ks_scene *scene, int32_t *n_lights, int32_t *prim, float *power, float *cdf) {
-    KS_REQUIRE(scene, "scene is null");
-    int32_t n = (int32_t)scene->light_power.size();
-    if (n_lights) *n_lights = n;
-    for (int32_t i = 0; i < n; ++i) {
-        if (prim) prim[i] = scene->light_prim[i];
-        if (power) power[i] = scene->light_power[i];
-    }
-    if (cdf && n > 0) std::memcpy(cdf, scene->light_cdf.data(), sizeof(float) * (n + 1));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene, "scene is null");
+        int32_t n = (int32_t)scene->light_power.size();
+        if (n_lights) *n_lights = n;
+        for (int32_t i = 0; i < n; ++i) {
+            if (prim) prim[i] = scene->light_prim[i];
+            if (power) power[i] = scene->light_power[i];
+        }
+        if (cdf && n > 0) std::memcpy(cdf, scene->light_cdf.data(), sizeof(float) * (n + 1));
+        return KS_OK;
+    });
 }
 
 int ks_film_create(ks_device *dev, int32_t width, int32_t height, const float backdrop[3], ks_film **out) {
-    KS_REQUIRE(dev && out, "null argument");
-    KS_REQUIRE(width > 0 && height > 0, "film size must be positive");
-    *out = nullptr;
-    KS_HIP(hipSetDevice(dev->ordinal));
-    auto f = std::make_unique<ks_film>();
-    f->dev = dev;
-    f->w = width;
-    f->h = height;
-    if (backdrop)
-        for (int k = 0; k < 3; ++k) f->backdrop[k] = backdrop[k];
-    size_t n = (size_t)width * height;
-    KS_HIP(f->rgb.alloc(sizeof(double) * 3 * n));
-    KS_HIP(f->wts.alloc(sizeof(double) * 2 * n));
-    KS_HIP(hipMemsetAsync(f->rgb.p, 0, f->rgb.bytes, dev->stream));
-    KS_HIP(hipMemsetAsync(f->wts.p, 0, f->wts.bytes, dev->stream));
-    KS_HIP(hipStreamSynchronize(dev->stream));
-    *out = f.release();
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && out, "null argument");
+        KS_REQUIRE(width > 0 && height > 0, "film size must be positive");
+        *out = nullptr;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        auto f = std::make_unique<ks_film>();
+        f->dev = dev;
+        f->w = width;
+        f->h = height;
+        if (backdrop)
+            for (int k = 0; k < 3; ++k) f->backdrop[k] = backdrop[k];
+        size_t n = (size_t)width * height;
+        KS_HIP(f->rgb.alloc(sizeof(double) * 3 * n));
+        KS_HIP(f->wts.alloc(sizeof(double) * 2 * n));
+        KS_HIP(hipMemsetAsync(f->rgb.p, 0, f->rgb.bytes, dev->stream));
+        KS_HIP(hipMemsetAsync(f->wts.p, 0, f->wts.bytes, dev->stream));
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        *out = f.release();
+        return KS_OK;
+    });
 }
 
 int ks_film_clear(ks_film *film) {
-    KS_REQUIRE(film, "film is null");
-    KS_HIP(hipSetDevice(film->dev->ordinal));
-    KS_HIP(hipMemsetAsync(film->rgb.p, 0, film->rgb.bytes, film->dev->stream));
-    KS_HIP(hipMemsetAsync(film->wts.p, 0, film->wts.bytes, film->dev->stream));
-    KS_HIP(hipStreamSynchronize(film->dev->stream));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(film, "film is null");
+        KS_HIP(hipSetDevice(film->dev->ordinal));
+        KS_HIP(hipMemsetAsync(film->rgb.p, 0, film->rgb.bytes, film->dev->stream));
+        KS_HIP(hipMemsetAsync(film->wts.p, 0, film->wts.bytes, film->dev->stream));
+        KS_HIP(hipStreamSynchronize(film->dev->stream));
+        return KS_OK;
+    });
 }
 
 int ks_film_download(const ks_film *film, double *rgb, double *weights) {
-    KS_REQUIRE(film, "film is null");
-    KS_HIP(hipSetDevice(film->dev->ordinal));
-    if (rgb) KS_HIP(hipMemcpyAsync(rgb, film->rgb.p, film->rgb.bytes, hipMemcpyDeviceToHost, film->dev->stream));
-    if (weights)
-        KS_HIP(hipMemcpyAsync(weights, film->wts.p, film->wts.bytes, hipMemcpyDeviceToHost, film->dev->stream));
-    KS_HIP(hipStreamSynchronize(film->dev->stream));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(film, "film is null");
+        KS_HIP(hipSetDevice(film->dev->ordinal));
+        if (rgb) KS_HIP(hipMemcpyAsync(rgb, film->rgb.p, film->rgb.bytes, hipMemcpyDeviceToHost, film->dev->stream));
+        if (weights)
+            KS_HIP(hipMemcpyAsync(weights, film->wts.p, film->wts.bytes, hipMemcpyDeviceToHost, film->dev->stream));
+        KS_HIP(hipStreamSynchronize(film->dev->stream));
+        return KS_OK;
+    });
 }
 
 int ks_film_composite(const ks_film *film, float *out) {
-    KS_REQUIRE(film && out, "null argument");
-    size_t n = (size_t)film->w * film->h;
-    std::vector<double> rgb(3 * n), w(2 * n);
-    int e = ks_film_download(film, rgb.data(), w.data());
-    if (e) return e;
-    for (size_t i = 0; i < n; ++i) { // render_target.cpp:187-192; lerp takes a float t (maths.h:686-690)
-        uint32_t w_sum = (uint32_t)(w[2 * i] + w[2 * i + 1]);
-        float t = (float)(w_sum == 0 ? 0.0 : w[2 * i] / (double)w_sum);
-        for (int k = 0; k < 3; ++k) {
-            double fg = w[2 * i] == 0 ? 0.0 : rgb[3 * i + k] / w[2 * i];
-            out[3 * i + k] = (float)((double)(1.0f - t) * (double)film->backdrop[k] + (double)t * fg);
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(film && out, "null argument");
+        size_t n = (size_t)film->w * film->h;
+        std::vector<double> rgb(3 * n), w(2 * n);
+        int e = ks_film_download(film, rgb.data(), w.data());
+        if (e) return e;
+        for (size_t i = 0; i < n; ++i) { // render_target.cpp:187-192; lerp takes a float t (maths.h:686-690)
+            uint32_t w_sum = (uint32_t)(w[2 * i] + w[2 * i + 1]);
+            float t = (float)(w_sum == 0 ? 0.0 : w[2 * i] / (double)w_sum);
+            for (int k = 0; k < 3; ++k) {
+                double fg = w[2 * i] == 0 ? 0.0 : rgb[3 * i + k] / w[2 * i];
+                out[3 * i + k] = (float)((double)(1.0f - t) * (double)film->backdrop[k] + (double)t * fg);
+            }
         }
-    }
-    return KS_OK;
+        return KS_OK;
+    });
 }
 
 int ks_film_free(ks_film *film) {
-    if (film) {
-        (void)hipSetDevice(film->dev->ordinal);
-        (void)hipStreamSynchronize(film->dev->stream);
-        delete film;
-    }
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        if (film) {
+            (void)hipSetDevice(film->dev->ordinal);
+            (void)hipStreamSynchronize(film->dev->stream);
+            delete film;
+        }
+        return KS_OK;
+    });
 }
 
 int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, ks_film *film,
                        ks_progress_fn progress, void *user, ks_render_stats *stats) {
-    KS_REQUIRE(scene && cam && params && film, "null argument");
-    KS_REQUIRE(film->w == params->render_width && film->h == params->render_height,
-               "film size differs from render_width x render_height");
-    KS_REQUIRE(film->dev == scene->dev, "film and scene belong to different devices");
-    KS_REQUIRE(params->guide_mode == KS_GUIDE_OFF, "guide_mode is not available in this build");
-    ks_device *dev = scene->dev;
-    KS_HIP(hipSetDevice(dev->ordinal));
-    auto t0 = std::chrono::steady_clock::now();
-    RenderConsts rc;
-    int e = fill_consts(scene, cam, params, rc);
-    if (e) return e;
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
-    uint64_t pixels = (uint64_t)rc.cw * rc.ch;
-    // per-sample buffer budget: 1 GiB of float4 results per sub-batch
-    const uint64_t max_samp = (1ull << 30) / sizeof(float4);
-    int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
-    KS_REQUIRE(pixels <= max_samp, "crop window too large for one sub-batch");
-    uint32_t Q = pick_queue(params, pixels * (uint64_t)std::min(spp_chunk, params->spp));
-    e = ensure_workspace(dev, Q, pixels * (uint64_t)std::min(spp_chunk, params->spp), scene->max_stack);
-    if (e) return e;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                       (int)kStatCount);
-    Timer tm{dev};
-    for (int s_start = 0; s_start < params->spp; s_start += interval) {
-        int s_end = std::min(params->spp, s_start + interval);
-        for (int c0 = s_start; c0 < s_end; c0 += spp_chunk) {
-            rc.s0 = c0;
-            rc.nb = std::min(spp_chunk, s_end - c0);
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && cam && params && film, "null argument");
+        KS_REQUIRE(film->w == params->render_width && film->h == params->render_height,
+                   "film size differs from render_width x render_height");
+        KS_REQUIRE(film->dev == scene->dev, "film and scene belong to different devices");
+        KS_REQUIRE(params->guide_mode == KS_GUIDE_OFF, "guide_mode is not available in this build");
+        ks_device *dev = scene->dev;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        auto t0 = std::chrono::steady_clock::now();
+        RenderConsts rc;
+        int e = fill_consts(scene, cam, params, rc);
+        if (e) return e;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
+        uint64_t pixels = (uint64_t)rc.cw * rc.ch;
+        // per-sample buffer budget: 1 GiB of float4 results per sub-batch
+        const uint64_t max_samp = (1ull << 30) / sizeof(float4);
+        int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
+        KS_REQUIRE(pixels <= max_samp, "crop window too large for one sub-batch");
+        uint32_t Q = pick_queue(params, pixels * (uint64_t)std::min(spp_chunk, params->spp));
+        e = ensure_workspace(dev, Q, pixels * (uint64_t)std::min(spp_chunk, params->spp), scene->max_stack);
+        if (e) return e;
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                           (int)kStatCount);
+        Timer tm{dev};
+        for (int s_start = 0; s_start < params->spp; s_start += interval) {
+            int s_end = std::min(params->spp, s_start + interval);
+            for (int c0 = s_start; c0 < s_end; c0 += spp_chunk) {
+                rc.s0 = c0;
+                rc.nb = std::min(spp_chunk, s_end - c0);
+                rc.work_list = nullptr;
+                uint64_t total = pixels * (uint64_t)rc.nb;
+                e = run_wavefront(scene, rc, total, Q, tm, stats);
+                if (e) return e;
+                hipEvent_t ev = tm.begin(KS_K_FILM);
+                hipLaunchKernelGGL(k_film, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, dev->stream, rc,
+                                   dev->ws.samp.as<float4>(), film->rgb.as<double>(), film->wts.as<double>());
+                tm.end(ev);
+                KS_HIP(hipGetLastError());
+            }
+            KS_HIP(hipStreamSynchronize(dev->stream));
+            if (progress) progress(film, s_end, user);
+        }
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        tm.collect(stats);
+        e = read_stats(dev, stats);
+        if (e) return e;
+        if (stats) {
+            stats->paths = pixels * (uint64_t)params->spp;
+            stats->queue_size = (int32_t)Q;
+            stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return KS_OK;
+    });
+}
+
+int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(film && reference && relmse, "null argument");
+        KS_REQUIRE(film->w == reference->w && film->h == reference->h, "films differ in size");
+        KS_REQUIRE(film->dev == reference->dev, "films belong to different devices");
+        ks_device *dev = film->dev;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        size_t n = (size_t)film->w * film->h;
+        unsigned blocks = (unsigned)((n + 255) / 256);
+        DevBuf part;
+        KS_HIP(part.alloc(sizeof(double) * blocks));
+        hipLaunchKernelGGL(k_film_relmse, dim3(blocks), dim3(256), 0, dev->stream, n, film->rgb.as<double>(),
+                           film->wts.as<double>(), make_float3(film->backdrop[0], film->backdrop[1], film->backdrop[2]),
+                           reference->rgb.as<double>(), reference->wts.as<double>(),
+                           make_float3(reference->backdrop[0], reference->backdrop[1], reference->backdrop[2]),
+                           part.as<double>());
+        KS_HIP(hipGetLastError());
+        std::vector<double> h(blocks);
+        KS_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, dev->stream));
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        double sum = 0.0;
+        for (double v : h) sum += v;
+        *relmse = sum / (3.0 * (double)n);
+        return KS_OK;
+    });
+}
+
+int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params,
+                              ks_guide *guide, ks_film *film, ks_progress_fn progress, void *user,
+                              ks_render_stats *stats) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && cam && params && film && guide, "null argument");
+        KS_REQUIRE(film->w == params->render_width && film->h == params->render_height,
+                   "film size differs from render_width x render_height");
+        KS_REQUIRE(film->dev == scene->dev, "film and scene belong to different devices");
+        KS_REQUIRE(ksg::guide_device(guide) == scene->dev, "guide and scene belong to different devices");
+        KS_REQUIRE(params->mis_mode == KS_MIS_CONSISTENT,
+                   "guided rendering needs mis_mode consistent (the reference weights do not sum to one)");
+        ks_device *dev = scene->dev;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        auto t0 = std::chrono::steady_clock::now();
+        RenderConsts rc;
+        int e = fill_consts(scene, cam, params, rc);
+        if (e) return e;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        const int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
+        const uint64_t pixels = (uint64_t)rc.cw * rc.ch;
+        KS_REQUIRE(pixels * kRecVertices < (1ull << 31), "crop window too large for guided rendering");
+        const uint64_t max_samp = (1ull << 30) / sizeof(float4);
+        const int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
+        const uint64_t max_items = pixels * (uint64_t)std::min(spp_chunk, params->spp);
+        uint32_t Q = pick_queue(params, max_items);
+        e = ensure_workspace(dev, Q, max_items, scene->max_stack);
+        if (e) return e;
+        GuideWork &gw = dev->gw;
+        if (gw.mix_q < Q) {
+            KS_HIP(gw.mix.alloc(sizeof(float4) * 10 * (size_t)Q));
+            gw.mix_q = Q;
+        }
+        const float fraction = ksg::guide_fraction(guide);
+        const int32_t max_steps = ksg::guide_max_steps(guide), batch = ksg::guide_batch(guide);
+        const int32_t train_spp = ksg::guide_train_spp(guide);
+        const bool any_training = max_steps > 0;
+        if (any_training && gw.rec_items < pixels) {
+            KS_HIP(gw.recs.alloc(sizeof(float4) * kRecFloat4 * kRecVertices * pixels));
+            KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
+            KS_HIP(gw.train.alloc(ksg::kTrainSampleBytes * kRecVertices * pixels));
+            gw.rec_items = (uint32_t)pixels;
+        }
+        KS_HIP(gw.count.ensure(2 * sizeof(double))); // uint32 count, then the double sum of the weights
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                           (int)kStatCount);
+        const uint32_t cap = (uint32_t)(kRecVertices * pixels);
+        Timer tm{dev};
+        int s = 0;
+        while (s < params->spp) {
+            // training passes hold one sample per pixel; frozen passes are batched like plain renders, without
+            // crossing a progress boundary
+            const bool train = any_training && (train_spp == 0 || s < train_spp);
+            int nb = train ? 1 : std::min(spp_chunk, params->spp - s);
+            nb = std::min(nb, (s / interval + 1) * interval - s);
+            rc.s0 = s;
+            rc.nb = nb;
             rc.work_list = nullptr;
-            uint64_t total = pixels * (uint64_t)rc.nb;
-            e = run_wavefront(scene, rc, total, Q, tm, stats);
+            GuidePass gp;
+            gp.guide = guide;
+            gp.mix = gw.mix.as<float4>();
+            // the guide samples once it has been trained at least once; before that the pass is BSDF-only
+            const bool sample_guide = fraction > 0.0f && ksg::guide_steps(guide) > 0;
+            gp.shade.mix = sample_guide ? gw.mix.as<float4>() : nullptr;
+            gp.shade.recs = train ? gw.recs.as<float4>() : nullptr;
+            gp.shade.alpha = fraction;
+            gp.shade.min_roughness = kGuideMinRoughness;
+            gp.shade.max_bounce = ksg::guide_max_bounce(guide);
+            if (gw.stamp == 0xffffffffu) { // stamps wrap: clear the records so no stale stamp can match
+                KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
+                gw.stamp = 0;
+            }
+            gp.shade.stamp = ++gw.stamp;
+            e = run_wavefront(scene, rc, pixels * (uint64_t)nb, Q, tm, stats, &gp);
             if (e) return e;
             hipEvent_t ev = tm.begin(KS_K_FILM);
             hipLaunchKernelGGL(k_film, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, dev->stream, rc,
                                dev->ws.samp.as<float4>(), film->rgb.as<double>(), film->wts.as<double>());
             tm.end(ev);
             KS_HIP(hipGetLastError());
-        }
-        KS_HIP(hipStreamSynchronize(dev->stream));
-        if (progress) progress(film, s_end, user);
-    }
-    KS_HIP(hipStreamSynchronize(dev->stream));
-    tm.collect(stats);
-    e = read_stats(dev, stats);
-    if (e) return e;
-    if (stats) {
-        stats->paths = pixels * (uint64_t)params->spp;
-        stats->queue_size = (int32_t)Q;
-        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    return KS_OK;
-}
-
-int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse) {
-    KS_REQUIRE(film && reference && relmse, "null argument");
-    KS_REQUIRE(film->w == reference->w && film->h == reference->h, "films differ in size");
-    KS_REQUIRE(film->dev == reference->dev, "films belong to different devices");
-    ks_device *dev = film->dev;
-    KS_HIP(hipSetDevice(dev->ordinal));
-    size_t n = (size_t)film->w * film->h;
-    unsigned blocks = (unsigned)((n + 255) / 256);
-    DevBuf part;
-    KS_HIP(part.alloc(sizeof(double) * blocks));
-    hipLaunchKernelGGL(k_film_relmse, dim3(blocks), dim3(256), 0, dev->stream, n, film->rgb.as<double>(),
-                       film->wts.as<double>(), make_float3(film->backdrop[0], film->backdrop[1], film->backdrop[2]),
-                       reference->rgb.as<double>(), reference->wts.as<double>(),
-                       make_float3(reference->backdrop[0], reference->backdrop[1], reference->backdrop[2]),
-                       part.as<double>());
-    KS_HIP(hipGetLastError());
-    std::vector<double> h(blocks);
-    KS_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, dev->stream));
-    KS_HIP(hipStreamSynchronize(dev->stream));
-    double sum = 0.0;
-    for (double v : h) sum += v;
-    *relmse = sum / (3.0 * (double)n);
-    return KS_OK;
-}
-
-int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params,
-                              ks_guide *guide, ks_film *film, ks_progress_fn progress, void *user,
-                              ks_render_stats *stats) {
-    KS_REQUIRE(scene && cam && params && film && guide, "null argument");
-    KS_REQUIRE(film->w == params->render_width && film->h == params->render_height,
-               "film size differs from render_width x render_height");
-    KS_REQUIRE(film->dev == scene->dev, "film and scene belong to different devices");
-    KS_REQUIRE(ksg::guide_device(guide) == scene->dev, "guide and scene belong to different devices");
-    KS_REQUIRE(params->mis_mode == KS_MIS_CONSISTENT,
-               "guided rendering needs mis_mode consistent (the reference weights do not sum to one)");
-    ks_device *dev = scene->dev;
-    KS_HIP(hipSetDevice(dev->ordinal));
-    auto t0 = std::chrono::steady_clock::now();
-    RenderConsts rc;
-    int e = fill_consts(scene, cam, params, rc);
-    if (e) return e;
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    const int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
-    const uint64_t pixels = (uint64_t)rc.cw * rc.ch;
-    KS_REQUIRE(pixels * kRecVertices < (1ull << 31), "crop window too large for guided rendering");
-    const uint64_t max_samp = (1ull << 30) / sizeof(float4);
-    const int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
-    const uint64_t max_items = pixels * (uint64_t)std::min(spp_chunk, params->spp);
-    uint32_t Q = pick_queue(params, max_items);
-    e = ensure_workspace(dev, Q, max_items, scene->max_stack);
-    if (e) return e;
-    GuideWork &gw = dev->gw;
-    if (gw.mix_q < Q) {
-        KS_HIP(gw.mix.alloc(sizeof(float4) * 10 * (size_t)Q));
-        gw.mix_q = Q;
-    }
-    const float fraction = ksg::guide_fraction(guide);
-    const int32_t max_steps = ksg::guide_max_steps(guide), batch = ksg::guide_batch(guide);
-    const int32_t train_spp = ksg::guide_train_spp(guide);
-    const bool any_training = max_steps > 0;
-    if (any_training && gw.rec_items < pixels) {
-        KS_HIP(gw.recs.alloc(sizeof(float4) * kRecFloat4 * kRecVertices * pixels));
-        KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
-        KS_HIP(gw.train.alloc(ksg::kTrainSampleBytes * kRecVertices * pixels));
-        gw.rec_items = (uint32_t)pixels;
-    }
-    KS_HIP(gw.count.ensure(2 * sizeof(double))); // uint32 count, then the double sum of the weights
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                       (int)kStatCount);
-    const uint32_t cap = (uint32_t)(kRecVertices * pixels);
-    Timer tm{dev};
-    int s = 0;
-    while (s < params->spp) {
-        // training passes hold one sample per pixel; frozen passes are batched like plain renders, without
-        // crossing a progress boundary
-        const bool train = any_training && (train_spp == 0 || s < train_spp);
-        int nb = train ? 1 : std::min(spp_chunk, params->spp - s);
-        nb = std::min(nb, (s / interval + 1) * interval - s);
-        rc.s0 = s;
-        rc.nb = nb;
-        rc.work_list = nullptr;
-        GuidePass gp;
-        gp.guide = guide;
-        gp.mix = gw.mix.as<float4>();
-        // the guide samples once it has been trained at least once; before that the pass is BSDF-only
-        const bool sample_guide = fraction > 0.0f && ksg::guide_steps(guide) > 0;
-        gp.shade.mix = sample_guide ? gw.mix.as<float4>() : nullptr;
-        gp.shade.recs = train ? gw.recs.as<float4>() : nullptr;
-        gp.shade.alpha = fraction;
-        gp.shade.min_roughness = kGuideMinRoughness;
-        gp.shade.max_bounce = ksg::guide_max_bounce(guide);
-        if (gw.stamp == 0xffffffffu) { // stamps wrap: clear the records so no stale stamp can match
-            KS_HIP(hipMemsetAsync(gw.recs.p, 0, gw.recs.bytes, dev->stream));
-            gw.stamp = 0;
-        }
-        gp.shade.stamp = ++gw.stamp;
-        e = run_wavefront(scene, rc, pixels * (uint64_t)nb, Q, tm, stats, &gp);
-        if (e) return e;
-        hipEvent_t ev = tm.begin(KS_K_FILM);
-        hipLaunchKernelGGL(k_film, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, dev->stream, rc,
-                           dev->ws.samp.as<float4>(), film->rgb.as<double>(), film->wts.as<double>());
-        tm.end(ev);
-        KS_HIP(hipGetLastError());
-        if (train) {
-            ev = tm.begin(KS_K_GUIDE);
-            KS_HIP(hipMemsetAsync(gw.count.p, 0, 2 * sizeof(double), dev->stream));
-            e = ksg::launch_extract(gw.recs.as<float4>(), dev->ws.samp.as<float4>(), (uint32_t)pixels,
-                                    gp.shade.stamp, gw.train.p, gw.count.as<uint32_t>(), gw.count.as<double>() + 1,
-                                    cap, dev->stream);
-            tm.end(ev);
-            if (e) return e;
-            KS_HIP(hipMemcpyAsync(dev->host_scratch, gw.count.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                                  dev->stream));
-            KS_HIP(hipStreamSynchronize(dev->stream));
-            uint32_t n_found;
-            double c_total;
-            std::memcpy(&n_found, dev->host_scratch, sizeof(uint32_t));
-            std::memcpy(&c_total, dev->host_scratch + sizeof(double), sizeof(double));
-            const uint32_t n_train = std::min(n_found, cap);
-            const float c_scale = c_total > 0.0 ? (float)((double)n_found / c_total) : 1.0f;
-            // spread the samples of the pass over the steps: step k takes samples k, k + S, k + 2S, ...
-            const int steps = (int)std::min<uint64_t>((uint64_t)max_steps, n_train / 64u);
-            if (steps > 0) {
-                const int per_step = (int)std::min<uint64_t>((uint64_t)batch, n_train / (uint32_t)steps);
+            if (train) {
                 ev = tm.begin(KS_K_GUIDE);
-                for (int k = 0; k < steps; ++k) {
-                    e = ksg::train_on_device(guide, static_cast<uint8_t *>(gw.train.p) + ksg::kTrainSampleBytes * k,
-                                             per_step, steps, c_scale, nullptr, dev->stream);
-                    if (e) return e;
-                }
+                KS_HIP(hipMemsetAsync(gw.count.p, 0, 2 * sizeof(double), dev->stream));
+                e = ksg::launch_extract(gw.recs.as<float4>(), dev->ws.samp.as<float4>(), (uint32_t)pixels,
+                                        gp.shade.stamp, gw.train.p, gw.count.as<uint32_t>(), gw.count.as<double>() + 1,
+                                        cap, dev->stream);
                 tm.end(ev);
-                if (stats) stats->guide_samples += (uint64_t)per_step * steps;
+                if (e) return e;
+                KS_HIP(hipMemcpyAsync(dev->host_scratch, gw.count.p, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                      dev->stream));
+                KS_HIP(hipStreamSynchronize(dev->stream));
+                uint32_t n_found;
+                double c_total;
+                std::memcpy(&n_found, dev->host_scratch, sizeof(uint32_t));
+                std::memcpy(&c_total, dev->host_scratch + sizeof(double), sizeof(double));
+                const uint32_t n_train = std::min(n_found, cap);
+                const float c_scale = c_total > 0.0 ? (float)((double)n_found / c_total) : 1.0f;
+                // spread the samples of the pass over the steps: step k takes samples k, k + S, k + 2S, ...
+                const int steps = (int)std::min<uint64_t>((uint64_t)max_steps, n_train / 64u);
+                if (steps > 0) {
+                    const int per_step = (int)std::min<uint64_t>((uint64_t)batch, n_train / (uint32_t)steps);
+                    ev = tm.begin(KS_K_GUIDE);
+                    for (int k = 0; k < steps; ++k) {
+                        e = ksg::train_on_device(guide, static_cast<uint8_t *>(gw.train.p) + ksg::kTrainSampleBytes * k,
+                                                 per_step, steps, c_scale, nullptr, dev->stream);
+                        if (e) return e;
+                    }
+                    tm.end(ev);
+                    if (stats) stats->guide_samples += (uint64_t)per_step * steps;
+                }
+            }
+            s += nb;
+            if (progress && (s % interval == 0 || s == params->spp)) {
+                KS_HIP(hipStreamSynchronize(dev->stream));
+                progress(film, s, user);
             }
         }
-        s += nb;
-        if (progress && (s % interval == 0 || s == params->spp)) {
-            KS_HIP(hipStreamSynchronize(dev->stream));
-            progress(film, s, user);
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        tm.collect(stats);
+        e = read_stats(dev, stats);
+        if (e) return e;
+        if (stats) {
+            stats->paths = pixels * (uint64_t)params->spp;
+            stats->queue_size = (int32_t)Q;
+            stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
-    }
-    KS_HIP(hipStreamSynchronize(dev->stream));
-    tm.collect(stats);
-    e = read_stats(dev, stats);
-    if (e) return e;
-    if (stats) {
-        stats->paths = pixels * (uint64_t)params->spp;
-        stats->queue_size = (int32_t)Q;
-        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    return KS_OK;
+        return KS_OK;
+    });
 }
 
 } // extern "C"
@@ -2552,246 +2586,254 @@ extern "C" {
 
 int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_view *views, const ks_batch_options *opt,
                              ks_render_stats *stats) {
-    KS_REQUIRE(scene && views && n_views > 0 && n_views <= 65536, "invalid views");
-    ks_device *dev = scene->dev;
-    KS_HIP(hipSetDevice(dev->ordinal));
-    auto t0 = std::chrono::steady_clock::now();
-    ks_batch_options o{};
-    if (opt) o = *opt;
-    std::vector<RenderConsts> hv((size_t)n_views);
-    std::vector<int> spp((size_t)n_views);
-    std::vector<double *> rgbs((size_t)n_views), wtss((size_t)n_views);
-    uint64_t paths = 0, max_pixels = 0;
-    for (int v = 0; v < n_views; ++v) {
-        const ks_batch_view &bv = views[v];
-        KS_REQUIRE(bv.film && bv.film->dev == dev, "every view needs a film on the scene's device");
-        KS_REQUIRE(bv.film->w == bv.params.render_width && bv.film->h == bv.params.render_height,
-                   "film size differs from render_width x render_height");
-        int e = fill_consts(scene, &bv.camera, &bv.params, hv[v]);
-        if (e) return e;
-        spp[v] = bv.params.spp;
-        rgbs[v] = bv.film->rgb.as<double>();
-        wtss[v] = bv.film->wts.as<double>();
-        uint64_t P = (uint64_t)hv[v].cw * hv[v].ch;
-        paths += P * (uint64_t)spp[v];
-        max_pixels = std::max(max_pixels, P);
-    }
-    if (stats) std::memset(stats, 0, sizeof(*stats));
-    const uint64_t budget = (1ull << 30) / sizeof(float4); // per-sample results of one chunk: 1 GiB
-    KS_REQUIRE(max_pixels <= budget, "a view is too large for one chunk");
-    std::vector<BatchChunk> chunks = schedule_batch(hv, spp, budget);
-    uint64_t max_items = 0;
-    for (auto &c : chunks) max_items = std::max(max_items, c.items);
-    HostToDev<RenderConsts> dviews;
-    HostToDev<double *> drgb, dwts;
-    KS_HIP(dviews.put(hv.data(), hv.size(), dev->stream));
-    KS_HIP(drgb.put(rgbs.data(), rgbs.size(), dev->stream));
-    KS_HIP(dwts.put(wtss.data(), wtss.size(), dev->stream));
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && views && n_views > 0 && n_views <= 65536, "invalid views");
+        ks_device *dev = scene->dev;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        auto t0 = std::chrono::steady_clock::now();
+        ks_batch_options o{};
+        if (opt) o = *opt;
+        std::vector<RenderConsts> hv((size_t)n_views);
+        std::vector<int> spp((size_t)n_views);
+        std::vector<double *> rgbs((size_t)n_views), wtss((size_t)n_views);
+        uint64_t paths = 0, max_pixels = 0;
+        for (int v = 0; v < n_views; ++v) {
+            const ks_batch_view &bv = views[v];
+            KS_REQUIRE(bv.film && bv.film->dev == dev, "every view needs a film on the scene's device");
+            KS_REQUIRE(bv.film->w == bv.params.render_width && bv.film->h == bv.params.render_height,
+                       "film size differs from render_width x render_height");
+            int e = fill_consts(scene, &bv.camera, &bv.params, hv[v]);
+            if (e) return e;
+            spp[v] = bv.params.spp;
+            rgbs[v] = bv.film->rgb.as<double>();
+            wtss[v] = bv.film->wts.as<double>();
+            uint64_t P = (uint64_t)hv[v].cw * hv[v].ch;
+            paths += P * (uint64_t)spp[v];
+            max_pixels = std::max(max_pixels, P);
+        }
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        const uint64_t budget = (1ull << 30) / sizeof(float4); // per-sample results of one chunk: 1 GiB
+        KS_REQUIRE(max_pixels <= budget, "a view is too large for one chunk");
+        std::vector<BatchChunk> chunks = schedule_batch(hv, spp, budget);
+        uint64_t max_items = 0;
+        for (auto &c : chunks) max_items = std::max(max_items, c.items);
+        HostToDev<RenderConsts> dviews;
+        HostToDev<double *> drgb, dwts;
+        KS_HIP(dviews.put(hv.data(), hv.size(), dev->stream));
+        KS_HIP(drgb.put(rgbs.data(), rgbs.size(), dev->stream));
+        KS_HIP(dwts.put(wtss.data(), wtss.size(), dev->stream));
 
-    WavefrontOpts wo;
-    wo.batch = true;
-    wo.graph = o.disable_graph == 0;
-    uint32_t Q = 0;
-    // autotuner: queue size 2^18..2^22 x traversal blocks per CU {4, 8}, timed on the first chunk's first
-    // items (at most 2^24), cached per scene and workload size
-    if (o.queue_size > 0) {
-        Q = (uint32_t)std::min<int64_t>(o.queue_size, 1 << 26);
-        wo.blocks_per_cu = o.trace_blocks_per_cu > 0 ? o.trace_blocks_per_cu : kMaxTraceBlocksPerCu;
-    } else if (dev->tune_scene == scene && dev->tune_items == paths) {
-        Q = dev->tune_q;
-        wo.blocks_per_cu = dev->tune_bpc;
-    } else {
-        const BatchChunk &c0 = chunks.front();
-        BatchChunk probe;
-        for (size_t k = 0; k < c0.segs.size() && probe.items < (1ull << 24); ++k) {
-            probe.segs.push_back(c0.segs[k]);
-            probe.items += c0.seg_items[k];
-        }
-        HostToDev<uint4> psegs;
-        const int32_t probe_segs = (int32_t)probe.segs.size();
-        probe.segs.push_back(make_uint4((uint32_t)probe.items, 0u, 0u, 0u)); // sentinel: the item count
-        KS_HIP(psegs.put(probe.segs.data(), probe.segs.size(), dev->stream));
-        RenderConsts prc = hv[0];
-        prc.views = dviews.p();
-        prc.segs = psegs.p();
-        prc.n_segs = probe_segs;
-        hipEvent_t ea, eb;
-        KS_HIP(hipEventCreate(&ea));
-        KS_HIP(hipEventCreate(&eb));
-        float best = 1e30f;
-        const int bpcs[2] = {4, 8};
-        for (int lq = 18; lq <= 22; ++lq) {
-            uint32_t q = (uint32_t)std::min<uint64_t>(1ull << lq, std::max<uint64_t>(probe.items, 256));
-            for (int bpc : bpcs) {
-                int e = ensure_workspace(dev, q, probe.items, scene->max_stack);
-                if (e) return e;
-                WavefrontOpts pw = wo;
-                pw.blocks_per_cu = bpc;
-                Timer ptm{dev};
-                int saved = dev->timing;
-                dev->timing = 0;
-                KS_HIP(hipEventRecord(ea, dev->stream));
-                e = run_wavefront(scene, prc, probe.items, q, ptm, nullptr, nullptr, pw);
-                KS_HIP(hipEventRecord(eb, dev->stream));
-                dev->timing = saved;
-                if (e) return e;
-                KS_HIP(hipEventSynchronize(eb));
-                float ms = 0.0f;
-                KS_HIP(hipEventElapsedTime(&ms, ea, eb));
-                if (ms < best) {
-                    best = ms;
-                    Q = q;
-                    wo.blocks_per_cu = bpc;
-                }
+        WavefrontOpts wo;
+        wo.batch = true;
+        wo.graph = o.disable_graph == 0;
+        uint32_t Q = 0;
+        // autotuner: queue size 2^18..2^22 x traversal blocks per CU {4, 8}, timed on the first chunk's first
+        // items (at most 2^24), cached per scene and workload size
+        if (o.queue_size > 0) {
+            Q = (uint32_t)std::min<int64_t>(o.queue_size, 1 << 26);
+            wo.blocks_per_cu = o.trace_blocks_per_cu > 0 ? o.trace_blocks_per_cu : kMaxTraceBlocksPerCu;
+        } else if (dev->tune_scene == scene && dev->tune_items == paths) {
+            Q = dev->tune_q;
+            wo.blocks_per_cu = dev->tune_bpc;
+        } else {
+            const BatchChunk &c0 = chunks.front();
+            BatchChunk probe;
+            for (size_t k = 0; k < c0.segs.size() && probe.items < (1ull << 24); ++k) {
+                probe.segs.push_back(c0.segs[k]);
+                probe.items += c0.seg_items[k];
             }
-            if ((1ull << lq) >= probe.items) break; // larger queues run the same work
+            HostToDev<uint4> psegs;
+            const int32_t probe_segs = (int32_t)probe.segs.size();
+            probe.segs.push_back(make_uint4((uint32_t)probe.items, 0u, 0u, 0u)); // sentinel: the item count
+            KS_HIP(psegs.put(probe.segs.data(), probe.segs.size(), dev->stream));
+            RenderConsts prc = hv[0];
+            prc.views = dviews.p();
+            prc.segs = psegs.p();
+            prc.n_segs = probe_segs;
+            hipEvent_t ea, eb;
+            KS_HIP(hipEventCreate(&ea));
+            KS_HIP(hipEventCreate(&eb));
+            float best = 1e30f;
+            const int bpcs[2] = {4, 8};
+            for (int lq = 18; lq <= 22; ++lq) {
+                uint32_t q = (uint32_t)std::min<uint64_t>(1ull << lq, std::max<uint64_t>(probe.items, 256));
+                for (int bpc : bpcs) {
+                    int e = ensure_workspace(dev, q, probe.items, scene->max_stack);
+                    if (e) return e;
+                    WavefrontOpts pw = wo;
+                    pw.blocks_per_cu = bpc;
+                    Timer ptm{dev};
+                    int saved = dev->timing;
+                    dev->timing = 0;
+                    KS_HIP(hipEventRecord(ea, dev->stream));
+                    e = run_wavefront(scene, prc, probe.items, q, ptm, nullptr, nullptr, pw);
+                    KS_HIP(hipEventRecord(eb, dev->stream));
+                    dev->timing = saved;
+                    if (e) return e;
+                    KS_HIP(hipEventSynchronize(eb));
+                    float ms = 0.0f;
+                    KS_HIP(hipEventElapsedTime(&ms, ea, eb));
+                    if (ms < best) {
+                        best = ms;
+                        Q = q;
+                        wo.blocks_per_cu = bpc;
+                    }
+                }
+                if ((1ull << lq) >= probe.items) break; // larger queues run the same work
+            }
+            (void)hipEventDestroy(ea);
+            (void)hipEventDestroy(eb);
+            dev->tune_scene = scene;
+            dev->tune_items = paths;
+            dev->tune_q = Q;
+            dev->tune_bpc = wo.blocks_per_cu;
         }
-        (void)hipEventDestroy(ea);
-        (void)hipEventDestroy(eb);
-        dev->tune_scene = scene;
-        dev->tune_items = paths;
-        dev->tune_q = Q;
-        dev->tune_bpc = wo.blocks_per_cu;
-    }
-    int e = ensure_workspace(dev, Q, max_items, scene->max_stack);
-    if (e) return e;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                       (int)kStatCount);
-    Timer tm{dev};
-    for (const BatchChunk &c : chunks) {
-        // declared before the device copies, so it outlives them (their release synchronises)
-        std::vector<uint4> segs = c.segs;
-        segs.push_back(make_uint4((uint32_t)c.items, 0u, 0u, 0u)); // sentinel: the chunk's item count
-        HostToDev<uint4> dsegs;
-        HostToDev<uint32_t> dtb, dsi;
-        KS_HIP(dsegs.put(segs.data(), segs.size(), dev->stream));
-        KS_HIP(dtb.put(c.thread_base.data(), c.thread_base.size(), dev->stream));
-        KS_HIP(dsi.put(c.seg_items.data(), c.seg_items.size(), dev->stream));
-        RenderConsts rc = hv[0];
-        rc.views = dviews.p();
-        rc.segs = dsegs.p();
-        rc.n_segs = (int32_t)c.segs.size();
-        e = run_wavefront(scene, rc, c.items, Q, tm, stats, nullptr, wo);
+        int e = ensure_workspace(dev, Q, max_items, scene->max_stack);
         if (e) return e;
-        hipEvent_t ev = tm.begin(KS_K_FILM);
-        hipLaunchKernelGGL(k_film_batch, dim3((c.threads + 255) / 256), dim3(256), 0, dev->stream, dviews.p(), dsegs.p(),
-                           dtb.p(), dsi.p(), (int)c.segs.size(), c.threads, dev->ws.samp.as<float4>(), drgb.p(),
-                           dwts.p());
-        tm.end(ev);
-        KS_HIP(hipGetLastError());
-        KS_HIP(hipStreamSynchronize(dev->stream)); // segment tables of this chunk are freed next
-    }
-    tm.collect(stats);
-    e = read_stats(dev, stats);
-    if (e) return e;
-    if (stats) {
-        stats->paths = paths;
-        stats->queue_size = (int32_t)Q;
-        stats->trace_blocks_per_cu = wo.blocks_per_cu;
-        stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    }
-    return KS_OK;
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                           (int)kStatCount);
+        Timer tm{dev};
+        for (const BatchChunk &c : chunks) {
+            // declared before the device copies, so it outlives them (their release synchronises)
+            std::vector<uint4> segs = c.segs;
+            segs.push_back(make_uint4((uint32_t)c.items, 0u, 0u, 0u)); // sentinel: the chunk's item count
+            HostToDev<uint4> dsegs;
+            HostToDev<uint32_t> dtb, dsi;
+            KS_HIP(dsegs.put(segs.data(), segs.size(), dev->stream));
+            KS_HIP(dtb.put(c.thread_base.data(), c.thread_base.size(), dev->stream));
+            KS_HIP(dsi.put(c.seg_items.data(), c.seg_items.size(), dev->stream));
+            RenderConsts rc = hv[0];
+            rc.views = dviews.p();
+            rc.segs = dsegs.p();
+            rc.n_segs = (int32_t)c.segs.size();
+            e = run_wavefront(scene, rc, c.items, Q, tm, stats, nullptr, wo);
+            if (e) return e;
+            hipEvent_t ev = tm.begin(KS_K_FILM);
+            hipLaunchKernelGGL(k_film_batch, dim3((c.threads + 255) / 256), dim3(256), 0, dev->stream, dviews.p(), dsegs.p(),
+                               dtb.p(), dsi.p(), (int)c.segs.size(), c.threads, dev->ws.samp.as<float4>(), drgb.p(),
+                               dwts.p());
+            tm.end(ev);
+            KS_HIP(hipGetLastError());
+            KS_HIP(hipStreamSynchronize(dev->stream)); // segment tables of this chunk are freed next
+        }
+        tm.collect(stats);
+        e = read_stats(dev, stats);
+        if (e) return e;
+        if (stats) {
+            stats->paths = paths;
+            stats->queue_size = (int32_t)Q;
+            stats->trace_blocks_per_cu = wo.blocks_per_cu;
+            stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return KS_OK;
+    });
 }
 
 int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
                      const int32_t *px, const int32_t *py, const int32_t *sample, float *out) {
-    KS_REQUIRE(scene && cam && params && px && py && sample && out, "null argument");
-    KS_REQUIRE(n > 0 && n < (1ll << 31), "sample count out of range");
-    ks_device *dev = scene->dev;
-    KS_HIP(hipSetDevice(dev->ordinal));
-    RenderConsts rc;
-    int e = fill_consts(scene, cam, params, rc);
-    if (e) return e;
-    std::vector<int4> wl((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-        KS_REQUIRE(px[i] >= 0 && px[i] < rc.W && py[i] >= 0 && py[i] < rc.H && sample[i] >= 0 && sample[i] < rc.spp,
-                   "pixel or sample index out of range");
-        wl[i] = make_int4(px[i], py[i], sample[i], 0);
-    }
-    HostToDev<int4> dwl;
-    KS_HIP(dwl.put(wl.data(), wl.size(), dev->stream));
-    rc.work_list = dwl.p();
-    rc.s0 = 0;
-    rc.nb = 1;
-    uint32_t Q = pick_queue(params, (uint64_t)n);
-    e = ensure_workspace(dev, Q, (size_t)n, scene->max_stack);
-    if (e) return e;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                       (int)kStatCount);
-    Timer tm{dev};
-    e = run_wavefront(scene, rc, (uint64_t)n, Q, tm, nullptr);
-    if (e) return e;
-    tm.collect(nullptr);
-    std::vector<float4> res((size_t)n);
-    KS_HIP(hipMemcpyAsync(res.data(), dev->ws.samp.p, sizeof(float4) * n, hipMemcpyDeviceToHost, dev->stream));
-    KS_HIP(hipStreamSynchronize(dev->stream));
-    for (int64_t i = 0; i < n; ++i) {
-        out[4 * i] = res[i].x;
-        out[4 * i + 1] = res[i].y;
-        out[4 * i + 2] = res[i].z;
-        out[4 * i + 3] = res[i].w;
-    }
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && cam && params && px && py && sample && out, "null argument");
+        KS_REQUIRE(n > 0 && n < (1ll << 31), "sample count out of range");
+        ks_device *dev = scene->dev;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        RenderConsts rc;
+        int e = fill_consts(scene, cam, params, rc);
+        if (e) return e;
+        std::vector<int4> wl((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            KS_REQUIRE(px[i] >= 0 && px[i] < rc.W && py[i] >= 0 && py[i] < rc.H && sample[i] >= 0 && sample[i] < rc.spp,
+                       "pixel or sample index out of range");
+            wl[i] = make_int4(px[i], py[i], sample[i], 0);
+        }
+        HostToDev<int4> dwl;
+        KS_HIP(dwl.put(wl.data(), wl.size(), dev->stream));
+        rc.work_list = dwl.p();
+        rc.s0 = 0;
+        rc.nb = 1;
+        uint32_t Q = pick_queue(params, (uint64_t)n);
+        e = ensure_workspace(dev, Q, (size_t)n, scene->max_stack);
+        if (e) return e;
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                           (int)kStatCount);
+        Timer tm{dev};
+        e = run_wavefront(scene, rc, (uint64_t)n, Q, tm, nullptr);
+        if (e) return e;
+        tm.collect(nullptr);
+        std::vector<float4> res((size_t)n);
+        KS_HIP(hipMemcpyAsync(res.data(), dev->ws.samp.p, sizeof(float4) * n, hipMemcpyDeviceToHost, dev->stream));
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        for (int64_t i = 0; i < n; ++i) {
+            out[4 * i] = res[i].x;
+            out[4 * i + 1] = res[i].y;
+            out[4 * i + 2] = res[i].z;
+            out[4 * i + 3] = res[i].w;
+        }
+        return KS_OK;
+    });
 }
 
 int ks_sobol_stream(ks_device *dev, int32_t width, int32_t height, int32_t spp, int32_t seed, int64_t n,
                     const int32_t *px, const int32_t *py, const int32_t *sample, int32_t n_calls, const int32_t *kinds,
                     uint32_t *out_u32, float *out_f) {
-    KS_REQUIRE(dev && px && py && sample && kinds && out_u32 && out_f, "null argument");
-    KS_REQUIRE(n > 0 && width > 0 && height > 0 && spp > 0 && n_calls > 0, "invalid sizes");
-    int n_values = 0;
-    for (int c = 0; c < n_calls; ++c) {
-        KS_REQUIRE(kinds[c] == 1 || kinds[c] == 2, "kinds must be 1 or 2");
-        n_values += kinds[c];
-    }
-    KS_HIP(hipSetDevice(dev->ordinal));
-    int log2spp, ndigits;
-    sobol_consts(width, height, spp, log2spp, ndigits);
-    hipStream_t s = dev->stream;
-    HostToDev<int32_t> dpx, dpy, ds, dk;
-    KS_HIP(dpx.put(px, n, s));
-    KS_HIP(dpy.put(py, n, s));
-    KS_HIP(ds.put(sample, n, s));
-    KS_HIP(dk.put(kinds, n_calls, s));
-    DevBuf ou, of;
-    KS_HIP(ou.alloc(sizeof(uint32_t) * n * n_values));
-    KS_HIP(of.alloc(sizeof(float) * n * n_values));
-    hipLaunchKernelGGL(k_sobol_stream, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dpx.p(), dpy.p(),
-                       ds.p(), seed, log2spp, ndigits, n_calls, dk.p(), n_values, ou.as<uint32_t>(), of.as<float>());
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(out_u32, ou.p, sizeof(uint32_t) * n * n_values, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipMemcpyAsync(out_f, of.p, sizeof(float) * n * n_values, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && px && py && sample && kinds && out_u32 && out_f, "null argument");
+        KS_REQUIRE(n > 0 && width > 0 && height > 0 && spp > 0 && n_calls > 0, "invalid sizes");
+        int n_values = 0;
+        for (int c = 0; c < n_calls; ++c) {
+            KS_REQUIRE(kinds[c] == 1 || kinds[c] == 2, "kinds must be 1 or 2");
+            n_values += kinds[c];
+        }
+        KS_HIP(hipSetDevice(dev->ordinal));
+        int log2spp, ndigits;
+        sobol_consts(width, height, spp, log2spp, ndigits);
+        hipStream_t s = dev->stream;
+        HostToDev<int32_t> dpx, dpy, ds, dk;
+        KS_HIP(dpx.put(px, n, s));
+        KS_HIP(dpy.put(py, n, s));
+        KS_HIP(ds.put(sample, n, s));
+        KS_HIP(dk.put(kinds, n_calls, s));
+        DevBuf ou, of;
+        KS_HIP(ou.alloc(sizeof(uint32_t) * n * n_values));
+        KS_HIP(of.alloc(sizeof(float) * n * n_values));
+        hipLaunchKernelGGL(k_sobol_stream, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dpx.p(), dpy.p(),
+                           ds.p(), seed, log2spp, ndigits, n_calls, dk.p(), n_values, ou.as<uint32_t>(), of.as<float>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(out_u32, ou.p, sizeof(uint32_t) * n * n_values, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipMemcpyAsync(out_f, of.p, sizeof(float) * n * n_values, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 int ks_camera_rays(ks_device *dev, const ks_camera_desc *cam, int32_t width, int32_t height, int32_t spp,
                    int32_t seed, int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample, float *rays) {
-    KS_REQUIRE(dev && cam && px && py && sample && rays, "null argument");
-    KS_REQUIRE(n > 0, "n must be positive");
-    KS_HIP(hipSetDevice(dev->ordinal));
-    ks_small_pt_params p{};
-    p.render_width = width;
-    p.render_height = height;
-    p.spp = spp;
-    p.rng_seed = seed;
-    p.bounces = 1;
-    RenderConsts rc;
-    int e = fill_consts(nullptr, cam, &p, rc);
-    if (e) return e;
-    hipStream_t s = dev->stream;
-    HostToDev<int32_t> dpx, dpy, ds;
-    KS_HIP(dpx.put(px, n, s));
-    KS_HIP(dpy.put(py, n, s));
-    KS_HIP(ds.put(sample, n, s));
-    DevBuf o;
-    KS_HIP(o.alloc(sizeof(float) * 6 * n));
-    hipLaunchKernelGGL(k_camera_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rc, n, dpx.p(), dpy.p(),
-                       ds.p(), o.as<float>());
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(rays, o.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && cam && px && py && sample && rays, "null argument");
+        KS_REQUIRE(n > 0, "n must be positive");
+        KS_HIP(hipSetDevice(dev->ordinal));
+        ks_small_pt_params p{};
+        p.render_width = width;
+        p.render_height = height;
+        p.spp = spp;
+        p.rng_seed = seed;
+        p.bounces = 1;
+        RenderConsts rc;
+        int e = fill_consts(nullptr, cam, &p, rc);
+        if (e) return e;
+        hipStream_t s = dev->stream;
+        HostToDev<int32_t> dpx, dpy, ds;
+        KS_HIP(dpx.put(px, n, s));
+        KS_HIP(dpy.put(py, n, s));
+        KS_HIP(ds.put(sample, n, s));
+        DevBuf o;
+        KS_HIP(o.alloc(sizeof(float) * 6 * n));
+        hipLaunchKernelGGL(k_camera_rays, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, rc, n, dpx.p(), dpy.p(),
+                           ds.p(), o.as<float>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(rays, o.p, sizeof(float) * 6 * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits, uint8_t *occ) {
@@ -2835,93 +2877,107 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
 }
 
 int ks_intersect(ks_scene *scene, const float *rays, int64_t n, float *hits) {
-    KS_REQUIRE(hits, "hits is null");
-    return ray_query(scene, rays, n, hits, nullptr);
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(hits, "hits is null");
+        return ray_query(scene, rays, n, hits, nullptr);
+    });
 }
 
 int ks_occluded(ks_scene *scene, const float *rays, int64_t n, uint8_t *occluded) {
-    KS_REQUIRE(occluded, "occluded is null");
-    return ray_query(scene, rays, n, nullptr, occluded);
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(occluded, "occluded is null");
+        return ray_query(scene, rays, n, nullptr, occluded);
+    });
 }
 
 int ks_bsdf_eval(ks_device *dev, const ks_material_desc *m, const float *wo, const float *wi, int64_t n, float *f,
                  float *pdf) {
-    KS_REQUIRE(dev && m && wo && wi && f && pdf && n > 0, "invalid argument");
-    KS_HIP(hipSetDevice(dev->ordinal));
-    hipStream_t s = dev->stream;
-    HostToDev<float> dwo, dwi;
-    KS_HIP(dwo.put(wo, 3 * n, s));
-    KS_HIP(dwi.put(wi, 3 * n, s));
-    DevBuf df, dp;
-    KS_HIP(df.alloc(sizeof(float) * 3 * n));
-    KS_HIP(dp.alloc(sizeof(float) * n));
-    hipLaunchKernelGGL(k_bsdf_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, to_device_material(*m), n,
-                       dwo.p(), dwi.p(), df.as<float>(), dp.as<float>());
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(f, df.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipMemcpyAsync(pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && m && wo && wi && f && pdf && n > 0, "invalid argument");
+        KS_HIP(hipSetDevice(dev->ordinal));
+        hipStream_t s = dev->stream;
+        HostToDev<float> dwo, dwi;
+        KS_HIP(dwo.put(wo, 3 * n, s));
+        KS_HIP(dwi.put(wi, 3 * n, s));
+        DevBuf df, dp;
+        KS_HIP(df.alloc(sizeof(float) * 3 * n));
+        KS_HIP(dp.alloc(sizeof(float) * n));
+        hipLaunchKernelGGL(k_bsdf_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, to_device_material(*m), n,
+                           dwo.p(), dwi.p(), df.as<float>(), dp.as<float>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(f, df.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipMemcpyAsync(pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 int ks_bsdf_sample(ks_device *dev, const ks_material_desc *m, const float *wo, const float *u, int64_t n, float *wi,
                    float *beta, float *pdf) {
-    KS_REQUIRE(dev && m && wo && u && wi && beta && pdf && n > 0, "invalid argument");
-    KS_HIP(hipSetDevice(dev->ordinal));
-    hipStream_t s = dev->stream;
-    HostToDev<float> dwo, du;
-    KS_HIP(dwo.put(wo, 3 * n, s));
-    KS_HIP(du.put(u, 3 * n, s));
-    DevBuf dwi, db, dp;
-    KS_HIP(dwi.alloc(sizeof(float) * 3 * n));
-    KS_HIP(db.alloc(sizeof(float) * 3 * n));
-    KS_HIP(dp.alloc(sizeof(float) * n));
-    hipLaunchKernelGGL(k_bsdf_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, to_device_material(*m), n,
-                       dwo.p(), du.p(), dwi.as<float>(), db.as<float>(), dp.as<float>());
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(wi, dwi.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipMemcpyAsync(beta, db.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipMemcpyAsync(pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && m && wo && u && wi && beta && pdf && n > 0, "invalid argument");
+        KS_HIP(hipSetDevice(dev->ordinal));
+        hipStream_t s = dev->stream;
+        HostToDev<float> dwo, du;
+        KS_HIP(dwo.put(wo, 3 * n, s));
+        KS_HIP(du.put(u, 3 * n, s));
+        DevBuf dwi, db, dp;
+        KS_HIP(dwi.alloc(sizeof(float) * 3 * n));
+        KS_HIP(db.alloc(sizeof(float) * 3 * n));
+        KS_HIP(dp.alloc(sizeof(float) * n));
+        hipLaunchKernelGGL(k_bsdf_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, to_device_material(*m), n,
+                           dwo.p(), du.p(), dwi.as<float>(), db.as<float>(), dp.as<float>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(wi, dwi.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipMemcpyAsync(beta, db.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipMemcpyAsync(pdf, dp.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 int ks_host_scene_create(const char *name, uint64_t seed, ks_host_scene **out) {
-    KS_REQUIRE(name && out, "null argument");
-    auto hs = std::make_unique<ks_host_scene>();
-    if (!ksa::make_procedural_scene(name, seed, hs->s)) {
-        ksa::set_error(std::string("unknown procedural scene: ") + name);
-        return KS_ERR_INVALID;
-    }
-    *out = hs.release();
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(name && out, "null argument");
+        auto hs = std::make_unique<ks_host_scene>();
+        if (!ksa::make_procedural_scene(name, seed, hs->s)) {
+            ksa::set_error(std::string("unknown procedural scene: ") + name);
+            return KS_ERR_INVALID;
+        }
+        *out = hs.release();
+        return KS_OK;
+    });
 }
 
 int ks_host_scene_get(const ks_host_scene *hs, ks_scene_desc *d, ks_camera_desc *cam) {
-    KS_REQUIRE(hs, "host scene is null");
-    const ksa::HostScene &s = hs->s;
-    if (d) {
-        std::memset(d, 0, sizeof(*d));
-        d->vertices = s.vertices.data();
-        d->n_vertices = (uint32_t)(s.vertices.size() / 3);
-        d->normals = s.normals.empty() ? nullptr : s.normals.data();
-        d->indices = s.indices.data();
-        d->n_tris = (uint32_t)(s.indices.size() / 3);
-        d->tri_geom = s.tri_geom.data();
-        d->geom_material = s.geom_material.data();
-        d->n_geoms = (uint32_t)s.geom_material.size();
-        d->materials = s.materials.data();
-        d->n_materials = (uint32_t)s.materials.size();
-        d->lights = s.lights.empty() ? nullptr : s.lights.data();
-        d->n_lights = (uint32_t)s.lights.size();
-    }
-    if (cam) *cam = s.camera;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(hs, "host scene is null");
+        const ksa::HostScene &s = hs->s;
+        if (d) {
+            std::memset(d, 0, sizeof(*d));
+            d->vertices = s.vertices.data();
+            d->n_vertices = (uint32_t)(s.vertices.size() / 3);
+            d->normals = s.normals.empty() ? nullptr : s.normals.data();
+            d->indices = s.indices.data();
+            d->n_tris = (uint32_t)(s.indices.size() / 3);
+            d->tri_geom = s.tri_geom.data();
+            d->geom_material = s.geom_material.data();
+            d->n_geoms = (uint32_t)s.geom_material.size();
+            d->materials = s.materials.data();
+            d->n_materials = (uint32_t)s.materials.size();
+            d->lights = s.lights.empty() ? nullptr : s.lights.data();
+            d->n_lights = (uint32_t)s.lights.size();
+        }
+        if (cam) *cam = s.camera;
+        return KS_OK;
+    });
 }
 
 int ks_host_scene_free(ks_host_scene *hs) {
-    delete hs;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        delete hs;
+        return KS_OK;
+    });
 }
 
 } // extern "C"

@@ -24,7 +24,7 @@ using namespace ksr;
         hipError_t e_ = (call);                                                                                    \
         if (e_ != hipSuccess) {                                                                                    \
             ksa::set_error(std::string(#call) + " failed: " + hipGetErrorString(e_));                              \
-            return KS_ERR_HIP;                                                                                     \
+            return e_ == hipErrorOutOfMemory ? KS_ERR_OOM : KS_ERR_HIP;                                            \
         }                                                                                                          \
     } while (0)
 #define KS_REQUIRE(cond, msg)                                                                                      \
@@ -1005,183 +1005,201 @@ extern "C" {
 
 int ks_guide_create(ks_device *dev, const ks_guide_config *cfg_in, const float bound_lo[3], const float bound_hi[3],
                     ks_guide **out) {
-    KS_REQUIRE(dev && bound_lo && bound_hi && out, "null argument");
-    *out = nullptr;
-    auto g = std::make_unique<ks_guide>();
-    g->dev = dev;
-    g->stream = ksg::device_stream(dev);
-    ks_guide_config c{};
-    if (cfg_in) c = *cfg_in;
-    if (c.log2_table_size == 0) c.log2_table_size = 19;
-    if (c.base_resolution == 0.0f) c.base_resolution = 16.0f;
-    if (c.max_resolution == 0.0f) c.max_resolution = 256.0f;
-    if (c.learning_rate == 0.0f) c.learning_rate = 1e-2f;
-    if (c.beta1 == 0.0f) c.beta1 = 0.9f;
-    if (c.beta2 == 0.0f) c.beta2 = 0.999f;
-    if (c.eps == 0.0f) c.eps = 1e-8f;
-    if (c.guide_fraction == 0.0f) c.guide_fraction = 0.5f;
-    if (c.train_batch == 0) c.train_batch = 1 << 18;
-    if (c.max_steps_per_pass == 0) c.max_steps_per_pass = 4;
-    if (c.weight_clamp == 0.0f) c.weight_clamp = 20.0f;
-    KS_REQUIRE(c.log2_table_size >= 10 && c.log2_table_size <= 24, "log2_table_size must be in [10, 24]");
-    KS_REQUIRE(c.max_resolution >= c.base_resolution && c.base_resolution >= 2.0f, "bad grid resolutions");
-    KS_REQUIRE(c.guide_fraction >= 0.0f && c.guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
-    KS_REQUIRE(c.train_spp >= 0, "train_spp must be non-negative");
-    g->cfg = c;
-    g->log2_T = (uint32_t)c.log2_table_size;
-    // resolutions: floor(base * b^l), b = (max / base)^(1 / (L - 1)), in FP64
-    double growth = std::exp(std::log((double)c.max_resolution / (double)c.base_resolution) / (kGuideLevels - 1));
-    for (int l = 0; l < kGuideLevels; ++l)
-        g->res[l] = (uint32_t)std::floor((double)c.base_resolution * std::pow(growth, (double)l) + 1e-9);
-    for (int k = 0; k < 3; ++k) {
-        float ext = bound_hi[k] - bound_lo[k];
-        KS_REQUIRE(ext >= 0.0f, "bound_hi below bound_lo");
-        float pad = std::max(ext * 0.01f, 1e-4f);
-        g->lo[k] = bound_lo[k] - pad;
-        g->inv_ext[k] = 1.0f / (ext + 2.0f * pad);
-    }
-    g->n_grid = (int64_t)kGuideLevels * ((int64_t)1 << g->log2_T) * kGuideFeatures;
-    int64_t np = g->n_grid + kMlpParams;
-    KS_HIP(hipSetDevice(ksg::device_ordinal(dev)));
-    KS_HIP(g->params.alloc(sizeof(float) * np));
-    KS_HIP(g->grad.alloc(sizeof(float) * np));
-    KS_HIP(g->m1.alloc(sizeof(float) * np));
-    KS_HIP(g->m2.alloc(sizeof(float) * np));
-    KS_HIP(g->grid_h.alloc(sizeof(uint32_t) * g->n_grid / 2));
-    KS_HIP(g->w_bf.alloc(sizeof(uint16_t) * kMlpParams));
-    KS_HIP(g->wt_bf.alloc(sizeof(uint16_t) * kMlpParams));
-    KS_HIP(g->loss.alloc(3 * sizeof(double)));
-    KS_HIP(g->w8.alloc(sizeof(uint64_t) * kFp8Frags));
-    KS_HIP(g->fp8s.alloc(sizeof(float) * kFp8ScaleCount));
-    KS_HIP(g->act_amax.alloc(sizeof(uint32_t) * 3));
-    {
-        const float one = 1.0f; // activation maxima before any training or calibration
-        uint32_t bits[3];
-        std::memcpy(&bits[0], &one, 4);
-        bits[1] = bits[2] = bits[0];
-        KS_HIP(hipMemcpyAsync(g->act_amax.p, bits, sizeof(bits), hipMemcpyHostToDevice, g->stream));
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && bound_lo && bound_hi && out, "null argument");
+        *out = nullptr;
+        auto g = std::make_unique<ks_guide>();
+        g->dev = dev;
+        g->stream = ksg::device_stream(dev);
+        ks_guide_config c{};
+        if (cfg_in) c = *cfg_in;
+        if (c.log2_table_size == 0) c.log2_table_size = 19;
+        if (c.base_resolution == 0.0f) c.base_resolution = 16.0f;
+        if (c.max_resolution == 0.0f) c.max_resolution = 256.0f;
+        if (c.learning_rate == 0.0f) c.learning_rate = 1e-2f;
+        if (c.beta1 == 0.0f) c.beta1 = 0.9f;
+        if (c.beta2 == 0.0f) c.beta2 = 0.999f;
+        if (c.eps == 0.0f) c.eps = 1e-8f;
+        if (c.guide_fraction == 0.0f) c.guide_fraction = 0.5f;
+        if (c.train_batch == 0) c.train_batch = 1 << 18;
+        if (c.max_steps_per_pass == 0) c.max_steps_per_pass = 4;
+        if (c.weight_clamp == 0.0f) c.weight_clamp = 20.0f;
+        KS_REQUIRE(c.log2_table_size >= 10 && c.log2_table_size <= 24, "log2_table_size must be in [10, 24]");
+        KS_REQUIRE(c.max_resolution >= c.base_resolution && c.base_resolution >= 2.0f, "bad grid resolutions");
+        KS_REQUIRE(c.guide_fraction >= 0.0f && c.guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
+        KS_REQUIRE(c.train_spp >= 0, "train_spp must be non-negative");
+        g->cfg = c;
+        g->log2_T = (uint32_t)c.log2_table_size;
+        // resolutions: floor(base * b^l), b = (max / base)^(1 / (L - 1)), in FP64
+        double growth = std::exp(std::log((double)c.max_resolution / (double)c.base_resolution) / (kGuideLevels - 1));
+        for (int l = 0; l < kGuideLevels; ++l)
+            g->res[l] = (uint32_t)std::floor((double)c.base_resolution * std::pow(growth, (double)l) + 1e-9);
+        for (int k = 0; k < 3; ++k) {
+            float ext = bound_hi[k] - bound_lo[k];
+            KS_REQUIRE(ext >= 0.0f, "bound_hi below bound_lo");
+            float pad = std::max(ext * 0.01f, 1e-4f);
+            g->lo[k] = bound_lo[k] - pad;
+            g->inv_ext[k] = 1.0f / (ext + 2.0f * pad);
+        }
+        g->n_grid = (int64_t)kGuideLevels * ((int64_t)1 << g->log2_T) * kGuideFeatures;
+        int64_t np = g->n_grid + kMlpParams;
+        KS_HIP(hipSetDevice(ksg::device_ordinal(dev)));
+        KS_HIP(g->params.alloc(sizeof(float) * np));
+        KS_HIP(g->grad.alloc(sizeof(float) * np));
+        KS_HIP(g->m1.alloc(sizeof(float) * np));
+        KS_HIP(g->m2.alloc(sizeof(float) * np));
+        KS_HIP(g->grid_h.alloc(sizeof(uint32_t) * g->n_grid / 2));
+        KS_HIP(g->w_bf.alloc(sizeof(uint16_t) * kMlpParams));
+        KS_HIP(g->wt_bf.alloc(sizeof(uint16_t) * kMlpParams));
+        KS_HIP(g->loss.alloc(3 * sizeof(double)));
+        KS_HIP(g->w8.alloc(sizeof(uint64_t) * kFp8Frags));
+        KS_HIP(g->fp8s.alloc(sizeof(float) * kFp8ScaleCount));
+        KS_HIP(g->act_amax.alloc(sizeof(uint32_t) * 3));
+        {
+            const float one = 1.0f; // activation maxima before any training or calibration
+            uint32_t bits[3];
+            std::memcpy(&bits[0], &one, 4);
+            bits[1] = bits[2] = bits[0];
+            KS_HIP(hipMemcpyAsync(g->act_amax.p, bits, sizeof(bits), hipMemcpyHostToDevice, g->stream));
+            KS_HIP(hipStreamSynchronize(g->stream));
+        }
+        g->fp8 = c.inference_fp8 != 0;
+        hipLaunchKernelGGL(k_guide_init, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, g->stream,
+                           g->params.as<float>(), g->m1.as<float>(), g->m2.as<float>(), g->grad.as<float>(), g->n_grid,
+                           (uint32_t)(c.seed ^ (c.seed >> 32)));
+        int64_t work = g->n_grid / 2 + kMlpParams;
+        hipLaunchKernelGGL(k_guide_refresh, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, g->stream,
+                           g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
+                           g->wt_bf.as<uint16_t>());
+        KS_HIP(hipGetLastError());
+        {
+            int e = ksg::fp8_pack(g.get(), g->stream);
+            if (e) return e;
+        }
         KS_HIP(hipStreamSynchronize(g->stream));
-    }
-    g->fp8 = c.inference_fp8 != 0;
-    hipLaunchKernelGGL(k_guide_init, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, g->stream,
-                       g->params.as<float>(), g->m1.as<float>(), g->m2.as<float>(), g->grad.as<float>(), g->n_grid,
-                       (uint32_t)(c.seed ^ (c.seed >> 32)));
-    int64_t work = g->n_grid / 2 + kMlpParams;
-    hipLaunchKernelGGL(k_guide_refresh, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, g->stream,
-                       g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
-                       g->wt_bf.as<uint16_t>());
-    KS_HIP(hipGetLastError());
-    {
-        int e = ksg::fp8_pack(g.get(), g->stream);
-        if (e) return e;
-    }
-    KS_HIP(hipStreamSynchronize(g->stream));
-    *out = g.release();
-    return KS_OK;
+        *out = g.release();
+        return KS_OK;
+    });
 }
 
 int ks_guide_free(ks_guide *g) {
-    if (g) {
-        (void)hipStreamSynchronize(g->stream);
-        delete g;
-    }
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        if (g) {
+            (void)hipStreamSynchronize(g->stream);
+            delete g;
+        }
+        return KS_OK;
+    });
 }
 
 int ks_guide_set_mode(ks_guide *g, float guide_fraction, int32_t max_steps_per_pass) {
-    KS_REQUIRE(g, "guide is null");
-    KS_REQUIRE(guide_fraction >= 0.0f && guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
-    g->cfg.guide_fraction = guide_fraction;
-    g->cfg.max_steps_per_pass = max_steps_per_pass;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g, "guide is null");
+        KS_REQUIRE(guide_fraction >= 0.0f && guide_fraction < 1.0f, "guide_fraction must be in [0, 1)");
+        g->cfg.guide_fraction = guide_fraction;
+        g->cfg.max_steps_per_pass = max_steps_per_pass;
+        return KS_OK;
+    });
 }
 
 int ks_guide_get_stats(const ks_guide *g, ks_guide_stats *out) {
-    KS_REQUIRE(g && out, "null argument");
-    double acc[3] = {0, 0, 0};
-    if (g->steps > 0) {
-        KS_HIP(hipMemcpyAsync(acc, g->loss.p, sizeof(acc), hipMemcpyDeviceToHost, g->stream));
-        KS_HIP(hipStreamSynchronize(g->stream));
-    }
-    out->steps = g->steps;
-    out->samples = g->samples_used;
-    out->last_loss = acc[0];
-    out->last_weight = acc[1];
-    out->last_loss_cosine = acc[2];
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && out, "null argument");
+        double acc[3] = {0, 0, 0};
+        if (g->steps > 0) {
+            KS_HIP(hipMemcpyAsync(acc, g->loss.p, sizeof(acc), hipMemcpyDeviceToHost, g->stream));
+            KS_HIP(hipStreamSynchronize(g->stream));
+        }
+        out->steps = g->steps;
+        out->samples = g->samples_used;
+        out->last_loss = acc[0];
+        out->last_weight = acc[1];
+        out->last_loss_cosine = acc[2];
+        return KS_OK;
+    });
 }
 
 int ks_guide_param_count(const ks_guide *g, int64_t *n_grid, int64_t *n_mlp) {
-    KS_REQUIRE(g, "guide is null");
-    if (n_grid) *n_grid = g->n_grid;
-    if (n_mlp) *n_mlp = kMlpParams;
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g, "guide is null");
+        if (n_grid) *n_grid = g->n_grid;
+        if (n_mlp) *n_mlp = kMlpParams;
+        return KS_OK;
+    });
 }
 
 int ks_guide_grid_info(const ks_guide *g, int32_t res[16], float lo[3], float inv_ext[3]) {
-    KS_REQUIRE(g, "guide is null");
-    for (int l = 0; l < kGuideLevels && res; ++l) res[l] = (int32_t)g->res[l];
-    for (int k = 0; k < 3; ++k) {
-        if (lo) lo[k] = g->lo[k];
-        if (inv_ext) inv_ext[k] = g->inv_ext[k];
-    }
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g, "guide is null");
+        for (int l = 0; l < kGuideLevels && res; ++l) res[l] = (int32_t)g->res[l];
+        for (int k = 0; k < 3; ++k) {
+            if (lo) lo[k] = g->lo[k];
+            if (inv_ext) inv_ext[k] = g->inv_ext[k];
+        }
+        return KS_OK;
+    });
 }
 
 int ks_guide_get_params(const ks_guide *g, float *params) {
-    KS_REQUIRE(g && params, "null argument");
-    KS_HIP(hipMemcpyAsync(params, g->params.p, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyDeviceToHost,
-                          g->stream));
-    KS_HIP(hipStreamSynchronize(g->stream));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && params, "null argument");
+        KS_HIP(hipMemcpyAsync(params, g->params.p, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyDeviceToHost,
+                              g->stream));
+        KS_HIP(hipStreamSynchronize(g->stream));
+        return KS_OK;
+    });
 }
 
 int ks_guide_set_params(ks_guide *g, const float *params) {
-    KS_REQUIRE(g && params, "null argument");
-    KS_HIP(hipMemcpyAsync(g->params.p, params, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyHostToDevice,
-                          g->stream));
-    int64_t work = g->n_grid / 2 + kMlpParams;
-    hipLaunchKernelGGL(k_guide_refresh, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, g->stream,
-                       g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
-                       g->wt_bf.as<uint16_t>());
-    KS_HIP(hipGetLastError());
-    int e = ksg::fp8_pack(g, g->stream);
-    if (e) return e;
-    KS_HIP(hipStreamSynchronize(g->stream));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && params, "null argument");
+        KS_HIP(hipMemcpyAsync(g->params.p, params, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyHostToDevice,
+                              g->stream));
+        int64_t work = g->n_grid / 2 + kMlpParams;
+        hipLaunchKernelGGL(k_guide_refresh, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, g->stream,
+                           g->params.as<float>(), g->n_grid, g->grid_h.as<uint32_t>(), g->w_bf.as<uint16_t>(),
+                           g->wt_bf.as<uint16_t>());
+        KS_HIP(hipGetLastError());
+        int e = ksg::fp8_pack(g, g->stream);
+        if (e) return e;
+        KS_HIP(hipStreamSynchronize(g->stream));
+        return KS_OK;
+    });
 }
 
 int ks_guide_forward(ks_guide *g, int64_t n, const float *queries, float *mixtures) {
-    KS_REQUIRE(g && queries && mixtures && n > 0 && n < (1ll << 30), "invalid argument");
-    hipStream_t s = g->stream;
-    DevBuf q, m;
-    KS_HIP(q.alloc(sizeof(float) * 9 * n));
-    KS_HIP(m.alloc(sizeof(float4) * 10 * n));
-    KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
-    unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
-    if (g->fp8)
-        hipLaunchKernelGGL((k_guide_infer<QueryArray, 2>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
-                           QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), nullptr);
-    else
-        hipLaunchKernelGGL((k_guide_infer<QueryArray, 0>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
-                           QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), nullptr);
-    KS_HIP(hipGetLastError());
-    std::vector<float4> h((size_t)n * 10);
-    KS_HIP(hipMemcpyAsync(h.data(), m.p, sizeof(float4) * 10 * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    for (int64_t i = 0; i < n; ++i) { // [w0..w7, (mu xyz, kappa) x 8]
-        const float4 *e = &h[10 * i];
-        float *o = mixtures + 40 * i;
-        o[0] = e[0].x, o[1] = e[0].y, o[2] = e[0].z, o[3] = e[0].w;
-        o[4] = e[1].x, o[5] = e[1].y, o[6] = e[1].z, o[7] = e[1].w;
-        for (int k = 0; k < 8; ++k) {
-            o[8 + 4 * k] = e[2 + k].x;
-            o[9 + 4 * k] = e[2 + k].y;
-            o[10 + 4 * k] = e[2 + k].z;
-            o[11 + 4 * k] = e[2 + k].w;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && queries && mixtures && n > 0 && n < (1ll << 30), "invalid argument");
+        hipStream_t s = g->stream;
+        DevBuf q, m;
+        KS_HIP(q.alloc(sizeof(float) * 9 * n));
+        KS_HIP(m.alloc(sizeof(float4) * 10 * n));
+        KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
+        unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
+        if (g->fp8)
+            hipLaunchKernelGGL((k_guide_infer<QueryArray, 2>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                               QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), nullptr);
+        else
+            hipLaunchKernelGGL((k_guide_infer<QueryArray, 0>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                               QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), nullptr);
+        KS_HIP(hipGetLastError());
+        std::vector<float4> h((size_t)n * 10);
+        KS_HIP(hipMemcpyAsync(h.data(), m.p, sizeof(float4) * 10 * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; ++i) { // [w0..w7, (mu xyz, kappa) x 8]
+            const float4 *e = &h[10 * i];
+            float *o = mixtures + 40 * i;
+            o[0] = e[0].x, o[1] = e[0].y, o[2] = e[0].z, o[3] = e[0].w;
+            o[4] = e[1].x, o[5] = e[1].y, o[6] = e[1].z, o[7] = e[1].w;
+            for (int k = 0; k < 8; ++k) {
+                o[8 + 4 * k] = e[2 + k].x;
+                o[9 + 4 * k] = e[2 + k].y;
+                o[10 + 4 * k] = e[2 + k].z;
+                o[11 + 4 * k] = e[2 + k].w;
+            }
         }
-    }
-    return KS_OK;
+        return KS_OK;
+    });
 }
 
 static int upload_samples(ks_guide *g, int64_t n, const float *queries, const float *wi, const float *weight) {
@@ -1201,106 +1219,120 @@ static int upload_samples(ks_guide *g, int64_t n, const float *queries, const fl
 
 int ks_guide_gradient(ks_guide *g, int64_t n, const float *queries, const float *wi, const float *weight,
                       double *loss, float *grad) {
-    KS_REQUIRE(g && queries && wi && weight && grad && n > 0 && n < (1ll << 30), "invalid argument");
-    int e = upload_samples(g, n, queries, wi, weight);
-    if (e) return e;
-    hipStream_t s = g->stream;
-    int n_chunks = (int)((n + 63) / 64);
-    int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
-    KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
-    KS_HIP(hipMemsetAsync(g->loss.p, 0, 3 * sizeof(double), s));
-    KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * g->n_grid, s));
-    hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, ksg::net_of(g), (int)n, 1,
-                       g->samples.as<TrainSample>(), 1.0f, INFINITY, g->grad.as<float>(), g->partials.as<float>(),
-                       g->loss.as<double>(), nullptr);
-    KS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
-                       g->grad.as<float>() + g->n_grid);
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(grad, g->grad.p, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyDeviceToHost, s));
-    if (loss) KS_HIP(hipMemcpyAsync(loss, g->loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * (g->n_grid + kMlpParams), s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && queries && wi && weight && grad && n > 0 && n < (1ll << 30), "invalid argument");
+        int e = upload_samples(g, n, queries, wi, weight);
+        if (e) return e;
+        hipStream_t s = g->stream;
+        int n_chunks = (int)((n + 63) / 64);
+        int waves = std::min(n_chunks, std::max(1, ksg::device_cu_count(g->dev) * 3));
+        KS_HIP(g->partials.ensure(sizeof(float) * (size_t)waves * kMlpParams));
+        KS_HIP(hipMemsetAsync(g->loss.p, 0, 3 * sizeof(double), s));
+        KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * g->n_grid, s));
+        hipLaunchKernelGGL(k_guide_train, dim3(waves), dim3(64), kTrainWaveBytes, s, ksg::net_of(g), (int)n, 1,
+                           g->samples.as<TrainSample>(), 1.0f, INFINITY, g->grad.as<float>(), g->partials.as<float>(),
+                           g->loss.as<double>(), nullptr);
+        KS_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_guide_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, g->partials.as<float>(), waves,
+                           g->grad.as<float>() + g->n_grid);
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(grad, g->grad.p, sizeof(float) * (g->n_grid + kMlpParams), hipMemcpyDeviceToHost, s));
+        if (loss) KS_HIP(hipMemcpyAsync(loss, g->loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+        KS_HIP(hipMemsetAsync(g->grad.p, 0, sizeof(float) * (g->n_grid + kMlpParams), s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 int ks_guide_train_step(ks_guide *g, int64_t n, const float *queries, const float *wi, const float *weight,
                         double *loss) {
-    KS_REQUIRE(g && queries && wi && weight && n > 0 && n < (1ll << 30), "invalid argument");
-    int e = upload_samples(g, n, queries, wi, weight);
-    if (e) return e;
-    return ksg::train_on_device(g, g->samples.p, (int)n, 1, 0.0f, loss, g->stream);
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && queries && wi && weight && n > 0 && n < (1ll << 30), "invalid argument");
+        int e = upload_samples(g, n, queries, wi, weight);
+        if (e) return e;
+        return ksg::train_on_device(g, g->samples.p, (int)n, 1, 0.0f, loss, g->stream);
+    });
 }
 
 int ks_guide_set_inference(ks_guide *g, int32_t fp8) {
-    KS_REQUIRE(g, "guide is null");
-    g->fp8 = fp8 != 0;
-    g->cfg.inference_fp8 = fp8 != 0;
-    int e = ksg::fp8_pack(g, g->stream);
-    if (e) return e;
-    KS_HIP(hipStreamSynchronize(g->stream));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g, "guide is null");
+        g->fp8 = fp8 != 0;
+        g->cfg.inference_fp8 = fp8 != 0;
+        int e = ksg::fp8_pack(g, g->stream);
+        if (e) return e;
+        KS_HIP(hipStreamSynchronize(g->stream));
+        return KS_OK;
+    });
 }
 
 int ks_guide_calibrate_fp8(ks_guide *g, int64_t n, const float *queries) {
-    KS_REQUIRE(g && queries && n > 0 && n < (1ll << 30), "invalid argument");
-    hipStream_t s = g->stream;
-    DevBuf q, m;
-    KS_HIP(q.alloc(sizeof(float) * 9 * n));
-    KS_HIP(m.alloc(sizeof(float4) * 10 * n));
-    KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
-    KS_HIP(hipMemsetAsync(g->act_amax.p, 0, 3 * sizeof(uint32_t), s));
-    unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
-    hipLaunchKernelGGL((k_guide_infer<QueryArray, 1>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
-                       QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), g->act_amax.as<uint32_t>());
-    KS_HIP(hipGetLastError());
-    int e = ksg::fp8_pack(g, s);
-    if (e) return e;
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && queries && n > 0 && n < (1ll << 30), "invalid argument");
+        hipStream_t s = g->stream;
+        DevBuf q, m;
+        KS_HIP(q.alloc(sizeof(float) * 9 * n));
+        KS_HIP(m.alloc(sizeof(float4) * 10 * n));
+        KS_HIP(hipMemcpyAsync(q.p, queries, sizeof(float) * 9 * n, hipMemcpyHostToDevice, s));
+        KS_HIP(hipMemsetAsync(g->act_amax.p, 0, 3 * sizeof(uint32_t), s));
+        unsigned blocks = (unsigned)((n + kInferBlock - 1) / kInferBlock);
+        hipLaunchKernelGGL((k_guide_infer<QueryArray, 1>), dim3(blocks), dim3(kInferBlock), 0, s, ksg::net_of(g),
+                           QueryArray{q.as<float>(), (uint32_t)n}, m.as<float4>(), g->act_amax.as<uint32_t>());
+        KS_HIP(hipGetLastError());
+        int e = ksg::fp8_pack(g, s);
+        if (e) return e;
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 int ks_guide_fp8_scales(const ks_guide *g, float scales[12]) {
-    KS_REQUIRE(g && scales, "null argument");
-    KS_HIP(hipMemcpyAsync(scales, g->fp8s.p, sizeof(float) * kFp8ScaleCount, hipMemcpyDeviceToHost, g->stream));
-    KS_HIP(hipStreamSynchronize(g->stream));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(g && scales, "null argument");
+        KS_HIP(hipMemcpyAsync(scales, g->fp8s.p, sizeof(float) * kFp8ScaleCount, hipMemcpyDeviceToHost, g->stream));
+        KS_HIP(hipStreamSynchronize(g->stream));
+        return KS_OK;
+    });
 }
 
 int ks_vmm_pdf(ks_device *dev, int64_t n, const float *mixtures, const float *dirs, float *pdf) {
-    KS_REQUIRE(dev && mixtures && dirs && pdf && n > 0, "invalid argument");
-    hipStream_t s = ksg::device_stream(dev);
-    DevBuf m, d, p;
-    KS_HIP(m.alloc(sizeof(float) * 40 * n));
-    KS_HIP(d.alloc(sizeof(float) * 3 * n));
-    KS_HIP(p.alloc(sizeof(float) * n));
-    KS_HIP(hipMemcpyAsync(m.p, mixtures, sizeof(float) * 40 * n, hipMemcpyHostToDevice, s));
-    KS_HIP(hipMemcpyAsync(d.p, dirs, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_vmm_pdf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, m.as<float>(), d.as<float>(),
-                       p.as<float>());
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(pdf, p.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && mixtures && dirs && pdf && n > 0, "invalid argument");
+        hipStream_t s = ksg::device_stream(dev);
+        DevBuf m, d, p;
+        KS_HIP(m.alloc(sizeof(float) * 40 * n));
+        KS_HIP(d.alloc(sizeof(float) * 3 * n));
+        KS_HIP(p.alloc(sizeof(float) * n));
+        KS_HIP(hipMemcpyAsync(m.p, mixtures, sizeof(float) * 40 * n, hipMemcpyHostToDevice, s));
+        KS_HIP(hipMemcpyAsync(d.p, dirs, sizeof(float) * 3 * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_vmm_pdf, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, m.as<float>(), d.as<float>(),
+                           p.as<float>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(pdf, p.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 int ks_vmm_sample(ks_device *dev, int64_t n, const float *mixtures, const float *u, float *dirs, float *pdf) {
-    KS_REQUIRE(dev && mixtures && u && dirs && pdf && n > 0, "invalid argument");
-    hipStream_t s = ksg::device_stream(dev);
-    DevBuf m, uu, d, p;
-    KS_HIP(m.alloc(sizeof(float) * 40 * n));
-    KS_HIP(uu.alloc(sizeof(float) * 2 * n));
-    KS_HIP(d.alloc(sizeof(float) * 3 * n));
-    KS_HIP(p.alloc(sizeof(float) * n));
-    KS_HIP(hipMemcpyAsync(m.p, mixtures, sizeof(float) * 40 * n, hipMemcpyHostToDevice, s));
-    KS_HIP(hipMemcpyAsync(uu.p, u, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_vmm_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, m.as<float>(),
-                       uu.as<float>(), d.as<float>(), p.as<float>());
-    KS_HIP(hipGetLastError());
-    KS_HIP(hipMemcpyAsync(dirs, d.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipMemcpyAsync(pdf, p.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
-    KS_HIP(hipStreamSynchronize(s));
-    return KS_OK;
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev && mixtures && u && dirs && pdf && n > 0, "invalid argument");
+        hipStream_t s = ksg::device_stream(dev);
+        DevBuf m, uu, d, p;
+        KS_HIP(m.alloc(sizeof(float) * 40 * n));
+        KS_HIP(uu.alloc(sizeof(float) * 2 * n));
+        KS_HIP(d.alloc(sizeof(float) * 3 * n));
+        KS_HIP(p.alloc(sizeof(float) * n));
+        KS_HIP(hipMemcpyAsync(m.p, mixtures, sizeof(float) * 40 * n, hipMemcpyHostToDevice, s));
+        KS_HIP(hipMemcpyAsync(uu.p, u, sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_vmm_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, m.as<float>(),
+                           uu.as<float>(), d.as<float>(), p.as<float>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipMemcpyAsync(dirs, d.p, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipMemcpyAsync(pdf, p.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        KS_HIP(hipStreamSynchronize(s));
+        return KS_OK;
+    });
 }
 
 } // extern "C"

@@ -4,6 +4,8 @@
 #include "../../include/keepsake_amd.h"
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -90,5 +92,28 @@ struct HostScene {
 bool make_procedural_scene(const std::string &name, uint64_t seed, HostScene &out);
 
 void set_error(const std::string &msg);
+
+// Runs the body of a C entry point so that no C++ exception crosses the ABI: std::bad_alloc becomes
+// KS_ERR_OOM, any other exception KS_ERR_INVALID, each with its message in ks_last_error().
+template <typename F> int guarded(F &&body) noexcept {
+    auto note = [](const char *what, const char *detail) noexcept {
+        try {
+            set_error(std::string(what) + detail);
+        } catch (...) {
+        }
+    };
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        note("host allocation failed", "");
+        return KS_ERR_OOM;
+    } catch (const std::exception &e) {
+        note("internal error: ", e.what());
+        return KS_ERR_INVALID;
+    } catch (...) {
+        note("internal error", "");
+        return KS_ERR_INVALID;
+    }
+}
 
 } // namespace ksa

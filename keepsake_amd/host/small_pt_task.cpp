@@ -447,7 +447,8 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
         st.guided = true;
         in.guide_mode = KS_GUIDE_TRAIN_AND_SAMPLE;
         ks_guide_config &c = st.guide_cfg;
-        c.guide_fraction = g.load_float("fraction", 0.0f);
+        // ks_guide_config reads 0 as "default 0.5": an explicit fraction = 0 is applied after creation
+        c.guide_fraction = g.load_float("fraction", 0.5f);
         c.train_spp = g.load_integer("train_spp", 0);
         c.learning_rate = g.load_float("learning_rate", 0.0f);
         c.beta1 = g.load_float("beta1", 0.0f);
@@ -522,6 +523,8 @@ void small_pt(const ConfigArgs &args, const fs::path &task_dir, int task_id) {
     std::unique_ptr<Guide> guide;
     if (st.guided) {
         guide = std::make_unique<Guide>(dev, scene, st.guide_cfg);
+        if (st.guide_cfg.guide_fraction == 0.0f)
+            guide->set_mode(0.0f, st.guide_cfg.max_steps_per_pass ? st.guide_cfg.max_steps_per_pass : 4);
         in.guide = guide.get();
     }
     in.scene = &scene;

@@ -72,3 +72,13 @@ backdrop = [0.1, 0.2, 0.3]
     assert diff.max() <= 1 and (diff > 0).mean() < 2e-3
     film.close()
     sc.close()
+
+
+def test_oversized_film_is_a_clean_error(ks, device):
+    """A film larger than the device (2^20 x 2^20 pixels, 40 TiB) fails with KS_ERR_OOM and a message; the
+    device keeps working."""
+    with pytest.raises(ks.KsError) as e:
+        ks.Film(device, 1 << 20, 1 << 20)
+    assert e.value.status == ks.KS_ERR_OOM and "alloc" in str(e.value)
+    f = ks.Film(device, 8, 8)
+    f.close()

@@ -187,6 +187,16 @@ max_bounce = 2
     assert plain["guided"] is False
 
 
+def test_guide_fraction_zero_is_kept(tmp_path):
+    """fraction = 0 (guide trained, never sampled) stays 0; a [task.guide] table without the key gets the
+    library default 0.5."""
+    text = BASE + 'mis_mode = "consistent"\n[task.guide]\n'
+    (t,) = tasks(tmp_path, text + "fraction = 0.0\n")
+    assert t["guide"]["fraction"] == 0.0
+    (t,) = tasks(tmp_path, text + "train_spp = 4\n")
+    assert t["guide"]["fraction"] == pytest.approx(0.5)
+
+
 def test_guide_needs_consistent_mis(tmp_path):
     r = resolve(tmp_path, BASE + "[task.guide]\nfraction = 0.5\n")
     assert r.returncode != 0 and "consistent" in r.stderr

@@ -69,3 +69,16 @@ def test_device_open_fails_loudly_without_gpu(ks):
     with pytest.raises(ks.KsError) as e:
         ks.Device(0)
     assert e.value.status == ks.KS_ERR_NO_DEVICE
+
+
+def test_entry_points_report_bad_arguments(ks):
+    """Errors come back as status codes with a message, never as an exception or abort across the ABI."""
+    import ctypes as C
+
+    lib = ks.load_library()
+    out = C.c_void_p()
+    assert lib.ks_film_create(None, 4, 4, None, C.byref(out)) == ks.KS_ERR_INVALID
+    assert b"null" in lib.ks_last_error()
+    assert lib.ks_scene_create(None, None, None, C.byref(out)) == ks.KS_ERR_INVALID
+    assert lib.ks_guide_set_mode(None, 0.5, 1) == ks.KS_ERR_INVALID
+    assert lib.ks_host_scene_create(b"no-such-scene", 0, C.byref(out)) == ks.KS_ERR_INVALID
