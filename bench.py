@@ -192,20 +192,12 @@ def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, ma
 def batched_views(ks, dev, args) -> dict:
     """48 views of the Cornell box at seed-drawn resolutions and spp (12 bounces) in one batched
     wavefront, against the same views rendered one at a time."""
+    from keepsake_amd.workloads import batch_views
+
     hs = ks.procedural_scene("cornell", 0)
     sc = ks.Scene(dev, hs)
-    rng = np.random.default_rng(args.batch_seed)
-    res_choices = [(64, 64), (128, 128), (256, 144), (512, 512), (1024, 576), (1024, 1024)]
-    views = []
-    for k in range(args.batch_views):
-        w, h = res_choices[rng.integers(len(res_choices))]
-        spp = int(rng.choice([4, 16, 64]))
-        cam = dict(hs.camera)
-        cam["pos"] = (0.5 + rng.uniform(-0.2, 0.2), 0.5 + rng.uniform(-0.2, 0.2), 2.4 + rng.uniform(-0.3, 0.3))
-        cam["aspect"] = w / h
-        p = ks.small_pt_params(bounces=12, render_width=w, render_height=h, spp=spp, spp_prog_interval=spp,
-                               rng_seed=k)
-        views.append((cam, p, ks.Film(dev, w, h)))
+    views = [(cam, p, ks.Film(dev, p.render_width, p.render_height))
+             for cam, p in batch_views(ks, hs.camera, args.batch_views, args.batch_seed)]
     paths = sum(v[1].render_width * v[1].render_height * v[1].spp for v in views)
     st = ks.small_pt_render_batch(sc, views)  # first call autotunes (cached afterwards)
     tune_ms = st["wall_ms"]

@@ -220,7 +220,17 @@ typedef struct ks_small_pt_params {
     int32_t mis_mode;
     int32_t guide_mode;
     int32_t queue_size; /* wavefront path slots; 0 = automatic */
+    /* SmallPTInput::pixel_filter (create_pixel_filter, render_target.cpp:28-44; render_target.h:79-184):
+     * KS_FILTER_BOX draws the jitter uniformly over [-w/2, w/2]^2; the gaussian and Blackman-Harris
+     * filters draw it from their 32-per-unit sample table (PixelFilterSampleTable). */
+    int32_t pixel_filter;
+    float filter_width; /* 0: the type's default (box 1, gaussian 3, blackman_harris 3) */
+    float filter_sigma; /* gaussian only, 0: 0.5 */
 } ks_small_pt_params;
+
+#define KS_FILTER_BOX 0
+#define KS_FILTER_GAUSSIAN 1
+#define KS_FILTER_BLACKMAN_HARRIS 2
 
 /* Kernel classes timed with HIP events on the render stream. */
 #define KS_K_RAYGEN 0
@@ -246,7 +256,7 @@ typedef struct ks_render_stats {
     int32_t queue_size;
     uint64_t guide_samples;             /* training samples consumed by Adam steps (guided renders) */
     int32_t trace_blocks_per_cu;        /* traversal blocks per CU of a batched render */
-    int32_t reserved;
+    int32_t batch_chunks;               /* wavefront chunks of a batched render */
 } ks_render_stats;
 
 typedef void (*ks_progress_fn)(const ks_film *film, int32_t spp_finished, void *user);
@@ -312,6 +322,8 @@ typedef struct ks_batch_options {
     int32_t queue_size;          /* 0: autotune over 2^18..2^22 (cached per scene and workload size) */
     int32_t trace_blocks_per_cu; /* with a fixed queue_size: traversal blocks per CU, 1..8 (8) */
     int32_t disable_graph;       /* 1: launch every iteration eagerly instead of replaying a HIP graph */
+    int32_t max_chunk_items;     /* work items (path samples) per chunk; 0: 2^26, i.e. 1 GiB of per-sample results.
+                                    A view's samples may be split across chunks at sample boundaries */
 } ks_batch_options;
 int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_view *views,
                              const ks_batch_options *opt, ks_render_stats *stats);

@@ -207,9 +207,18 @@ class Guide {
     ks_guide *h_ = nullptr;
 };
 
+// PixelFilter (render_target.h:70-184) as create_pixel_filter builds it from its TOML keys
+// (render_target.cpp:28-44): type KS_FILTER_*, width (0: the type's default), sigma (gaussian).
+struct PixelFilter {
+    int type = KS_FILTER_BOX;
+    float width = 0.0f;
+    float sigma = 0.0f;
+};
+
 struct SmallPTInput {
     const Scene *scene = nullptr;
     const Camera *camera = nullptr;
+    PixelFilter pixel_filter;
     RenderTarget2 *target = nullptr; // the film the run accumulates into (created inside run() in the reference)
     bool include_background = true;
     color3 backdrop = {0.0f, 0.0f, 0.0f};
@@ -256,6 +265,9 @@ struct SmallPTInput {
         p.mis_mode = mis_mode;
         p.guide_mode = KS_GUIDE_OFF; // the guided entry point is selected by run()
         p.queue_size = queue_size;
+        p.pixel_filter = pixel_filter.type;
+        p.filter_width = pixel_filter.width;
+        p.filter_sigma = pixel_filter.sigma;
         return p;
     }
 };

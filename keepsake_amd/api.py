@@ -23,6 +23,10 @@ KS_ERR_OOM = -3
 KS_ERR_UNSUPPORTED = -4
 KS_ERR_NO_DEVICE = -5
 
+KS_FILTER_BOX = 0
+KS_FILTER_GAUSSIAN = 1
+KS_FILTER_BLACKMAN_HARRIS = 2
+
 KS_LIGHT_SKY_AMBIENT = 0
 KS_LIGHT_DIRECTIONAL = 1
 KS_LIGHT_POINT = 2
@@ -272,6 +276,9 @@ class SmallPTParams(C.Structure):
         ("mis_mode", C.c_int32),
         ("guide_mode", C.c_int32),
         ("queue_size", C.c_int32),
+        ("pixel_filter", C.c_int32),
+        ("filter_width", C.c_float),
+        ("filter_sigma", C.c_float),
     ]
 
 
@@ -293,7 +300,7 @@ class RenderStats(C.Structure):
         ("queue_size", C.c_int32),
         ("guide_samples", C.c_uint64),
         ("trace_blocks_per_cu", C.c_int32),
-        ("reserved", C.c_int32),
+        ("batch_chunks", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -358,7 +365,8 @@ class BatchView(C.Structure):
 
 
 class BatchOptions(C.Structure):
-    _fields_ = [("queue_size", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("disable_graph", C.c_int32)]
+    _fields_ = [("queue_size", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("disable_graph", C.c_int32),
+                ("max_chunk_items", C.c_int32)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_void_p)
@@ -662,6 +670,9 @@ def small_pt_params(**kw) -> SmallPTParams:
     p.mis_mode = kw.get("mis_mode", KS_MIS_REFERENCE)
     p.guide_mode = kw.get("guide_mode", KS_GUIDE_OFF)
     p.queue_size = kw.get("queue_size", 0)
+    p.pixel_filter = kw.get("pixel_filter", KS_FILTER_BOX)
+    p.filter_width = kw.get("filter_width", 0.0)
+    p.filter_sigma = kw.get("filter_sigma", 0.0)
     return p
 
 
@@ -886,14 +897,14 @@ def small_pt_render_guided(scene: Scene, cam: dict, params: SmallPTParams, guide
 
 
 def small_pt_render_batch(scene: Scene, views, queue_size: int = 0, trace_blocks_per_cu: int = 0,
-                          graph: bool = True) -> dict:
+                          graph: bool = True, max_chunk_items: int = 0) -> dict:
     """Many views of one scene in one wavefront: views is a list of (camera dict, SmallPTParams, Film)."""
     arr = (BatchView * len(views))()
     for i, (cam, params, film) in enumerate(views):
         arr[i].camera = camera_desc(cam)
         arr[i].params = params
         arr[i].film = film.h
-    o = BatchOptions(queue_size, trace_blocks_per_cu, 0 if graph else 1)
+    o = BatchOptions(queue_size, trace_blocks_per_cu, 0 if graph else 1, max_chunk_items)
     st = RenderStats()
     _check(scene.lib.ks_small_pt_render_batch(scene.h, len(views), C.cast(arr, C.c_void_p), C.byref(o),
                                               C.byref(st)))

@@ -17,6 +17,7 @@
 #include "ks_guide.h"
 #include "ks_guide_host.h"
 #include "ks_internal.h"
+#include "ks_pixel_filter.h"
 #include "ks_render_types.h"
 #include "ks_surface.h"
 #include "ks_texture_build.h"
@@ -99,7 +100,28 @@ KS_DEV void pixel_of(const RenderConsts &rc, uint32_t w, uint32_t &x, uint32_t &
     y = (uint32_t)rc.cy + ry;
 }
 
-// Camera::spawn_ray (camera.cpp:129-170) after the box-filter jitter (small_pt.cpp:98-105); q is
+// DistribTable::sample_linear (distrib.cpp:43-53) over cdf[0..n]: the upper_bound cell of u and the
+// position (index + du) / n inside [0, 1).
+KS_DEV float filter_sample_linear(const float *cdf, int n, float u, uint32_t &index) {
+#pragma clang fp contract(off)
+    u = fminf(fmaxf(u, 0.0f), 0x1.fffffep-1f);
+    uint32_t lo = 0, len = (uint32_t)n + 1; // first entry > u
+    while (len > 0) {
+        uint32_t half = len >> 1;
+        if (cdf[lo + half] <= u) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    index = lo - 1;
+    const float c0 = cdf[index], c1 = cdf[index + 1];
+    const float du = (u - c0) / (c1 - c0);
+    return ((float)index + du) / (float)n;
+}
+
+// Camera::spawn_ray (camera.cpp:129-170) after the pixel-filter jitter (small_pt.cpp:98-105); q is
 // world_pos - eye before normalisation.
 KS_DEV void camera_ray_q(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &sob, f3 &o, f3 &q) {
     // no FMA contraction: the rounding of the projection must not depend on whether the matrix comes
@@ -109,8 +131,17 @@ KS_DEV void camera_ray_q(const RenderConsts &rc, uint32_t x, uint32_t y, Sobol &
     if (rc.spp != 1) {
         float u0, u1;
         sob.next2d(u0, u1);
-        ox = (1.0f - u0) * -0.5f + u0 * 0.5f; // BoxPixelFilter::sample (render_target.h:120)
-        oy = (1.0f - u1) * -0.5f + u1 * 0.5f;
+        if (rc.filter_n > 0) { // PixelFilterSampleTable::sample (render_target.h:96-103)
+            const int n = rc.filter_n;
+            uint32_t row;
+            const float ty = filter_sample_linear(rc.filter_cdf + (size_t)n * (n + 1), n, u1, row);
+            const float tx = filter_sample_linear(rc.filter_cdf + (size_t)row * (n + 1), n, u0, row);
+            u0 = tx;
+            u1 = ty;
+        }
+        // BoxPixelFilter::sample (render_target.h:120) and the table's lerp(-r, r, offset)
+        ox = (1.0f - u0) * -rc.filter_r + u0 * rc.filter_r;
+        oy = (1.0f - u1) * -rc.filter_r + u1 * rc.filter_r;
     }
     float fx = ((float)x + 0.5f + ox) / (float)rc.W;
     float fy = ((float)y + 0.5f + oy) / (float)rc.H;
@@ -1093,6 +1124,15 @@ struct ks_device {
     Ctrl *host_ctrl = nullptr; // pinned
     uint8_t *host_scratch = nullptr; // pinned, 64 bytes
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
+    // pixel-filter sample tables by (type, radius, sigma), kept for the device's lifetime: batched
+    // renders keep pointers to them in their per-view constants
+    struct FilterTable {
+        int type;
+        float r, sigma;
+        int n;
+        DevBuf cdf;
+    };
+    std::vector<std::unique_ptr<FilterTable>> filter_tables;
 };
 
 struct ks_scene {
@@ -1251,7 +1291,37 @@ int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_sta
     return KS_OK;
 }
 
-int fill_consts(const ks_scene *sc, const ks_camera_desc *cam, const ks_small_pt_params *p, RenderConsts &rc) {
+// The filter fields of rc: the box filter's radius, or a tabulated filter's table on the device (built
+// once per filter on the device and kept).
+int filter_consts(ks_device *dev, const ks_small_pt_params *p, RenderConsts &rc) {
+    ksa::PixelFilterSpec f;
+    KS_REQUIRE(ksa::pixel_filter_spec(*p, f), "pixel_filter: unknown type, or width / sigma not positive");
+    rc.filter_r = f.r;
+    rc.filter_n = 0;
+    rc.filter_cdf = nullptr;
+    if (f.type == KS_FILTER_BOX) return KS_OK;
+    for (auto &t : dev->filter_tables)
+        if (t->type == f.type && t->r == f.r && (f.type != KS_FILTER_GAUSSIAN || t->sigma == f.sigma)) {
+            rc.filter_n = t->n;
+            rc.filter_cdf = t->cdf.as<float>();
+            return KS_OK;
+        }
+    auto t = std::make_unique<ks_device::FilterTable>();
+    t->type = f.type;
+    t->r = f.r;
+    t->sigma = f.sigma;
+    std::vector<float> table = ksa::pixel_filter_table(f, t->n);
+    KS_HIP(hipSetDevice(dev->ordinal));
+    KS_HIP(t->cdf.alloc(table.size() * sizeof(float)));
+    KS_HIP(hipMemcpy(t->cdf.p, table.data(), table.size() * sizeof(float), hipMemcpyHostToDevice));
+    rc.filter_n = t->n;
+    rc.filter_cdf = t->cdf.as<float>();
+    dev->filter_tables.push_back(std::move(t));
+    return KS_OK;
+}
+
+int fill_consts(ks_device *dev, const ks_scene *sc, const ks_camera_desc *cam, const ks_small_pt_params *p,
+                RenderConsts &rc) {
     KS_REQUIRE(p->render_width > 0 && p->render_height > 0, "render_width/render_height must be positive");
     KS_REQUIRE(p->spp > 0, "spp must be positive");
     KS_REQUIRE(p->bounces >= 0 && p->bounces < 255, "bounces must be in [0, 254]");
@@ -1282,7 +1352,8 @@ int fill_consts(const ks_scene *sc, const ks_camera_desc *cam, const ks_small_pt
     // Camera::spawn_ray(film_pos, film_res, scale_ray_diff ? spp : 1) scales by 1 / sqrt(that count)
     rc.diff_scale = 1.0f / std::sqrt((float)(p->scale_ray_diff ? p->spp : 1));
     (void)sc;
-    return e;
+    if (e != KS_OK) return e;
+    return filter_consts(dev, p, rc);
 }
 
 // Guide state of one guided wavefront pass: the guide (nullptr: plain), its per-path mixture buffer and
@@ -2339,7 +2410,7 @@ int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_smal
         KS_HIP(hipSetDevice(dev->ordinal));
         auto t0 = std::chrono::steady_clock::now();
         RenderConsts rc;
-        int e = fill_consts(scene, cam, params, rc);
+        int e = fill_consts(scene->dev, scene, cam, params, rc);
         if (e) return e;
         if (stats) std::memset(stats, 0, sizeof(*stats));
         int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
@@ -2427,7 +2498,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
         KS_HIP(hipSetDevice(dev->ordinal));
         auto t0 = std::chrono::steady_clock::now();
         RenderConsts rc;
-        int e = fill_consts(scene, cam, params, rc);
+        int e = fill_consts(scene->dev, scene, cam, params, rc);
         if (e) return e;
         if (stats) std::memset(stats, 0, sizeof(*stats));
         const int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
@@ -2602,7 +2673,7 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
             KS_REQUIRE(bv.film && bv.film->dev == dev, "every view needs a film on the scene's device");
             KS_REQUIRE(bv.film->w == bv.params.render_width && bv.film->h == bv.params.render_height,
                        "film size differs from render_width x render_height");
-            int e = fill_consts(scene, &bv.camera, &bv.params, hv[v]);
+            int e = fill_consts(scene->dev, scene, &bv.camera, &bv.params, hv[v]);
             if (e) return e;
             spp[v] = bv.params.spp;
             rgbs[v] = bv.film->rgb.as<double>();
@@ -2612,7 +2683,9 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
             max_pixels = std::max(max_pixels, P);
         }
         if (stats) std::memset(stats, 0, sizeof(*stats));
-        const uint64_t budget = (1ull << 30) / sizeof(float4); // per-sample results of one chunk: 1 GiB
+        uint64_t budget = (1ull << 30) / sizeof(float4); // per-sample results of one chunk: 1 GiB
+        KS_REQUIRE(o.max_chunk_items >= 0, "max_chunk_items must not be negative");
+        if (o.max_chunk_items > 0) budget = std::min<uint64_t>(budget, (uint64_t)o.max_chunk_items);
         KS_REQUIRE(max_pixels <= budget, "a view is too large for one chunk");
         std::vector<BatchChunk> chunks = schedule_batch(hv, spp, budget);
         uint64_t max_items = 0;
@@ -2723,6 +2796,7 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
             stats->paths = paths;
             stats->queue_size = (int32_t)Q;
             stats->trace_blocks_per_cu = wo.blocks_per_cu;
+            stats->batch_chunks = (int32_t)chunks.size();
             stats->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }
         return KS_OK;
@@ -2737,7 +2811,7 @@ int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_
         ks_device *dev = scene->dev;
         KS_HIP(hipSetDevice(dev->ordinal));
         RenderConsts rc;
-        int e = fill_consts(scene, cam, params, rc);
+        int e = fill_consts(scene->dev, scene, cam, params, rc);
         if (e) return e;
         std::vector<int4> wl((size_t)n);
         for (int64_t i = 0; i < n; ++i) {
@@ -2818,7 +2892,7 @@ int ks_camera_rays(ks_device *dev, const ks_camera_desc *cam, int32_t width, int
         p.rng_seed = seed;
         p.bounces = 1;
         RenderConsts rc;
-        int e = fill_consts(nullptr, cam, &p, rc);
+        int e = fill_consts(dev, nullptr, cam, &p, rc);
         if (e) return e;
         hipStream_t s = dev->stream;
         HostToDev<int32_t> dpx, dpy, ds;

@@ -68,6 +68,11 @@ struct RenderConsts {
     float cam[3];
     float cam_dx[3], cam_dy[3]; // world offsets of one pixel step on the film plane (Camera::spawn_ray)
     float diff_scale;           // 1 / sqrt(spp) with scale_ray_diff, else 1
+    // pixel filter of the jitter: radius, and for a tabulated filter (gaussian, Blackman-Harris) the
+    // table size n and its n rows of n + 1 conditional CDF entries followed by the n + 1 marginal ones
+    float filter_r;
+    int32_t filter_n; // 0: box filter
+    const float *filter_cdf;
 };
 
 struct PathBufs {

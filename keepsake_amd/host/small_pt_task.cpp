@@ -402,10 +402,24 @@ SmallPTSetup resolve_small_pt(const ConfigArgs &args) {
             }
         }
     }
-    if (args.contains("pixel_filter")) {
+    if (args.contains("pixel_filter")) { // create_pixel_filter (render_target.cpp:28-44)
         ConfigArgs pf = args["pixel_filter"];
-        if (pf.load_string("type") != "box" || pf.load_float("width", 1.0f) != 1.0f)
-            throw ConfigError("only the unit-width box pixel filter is supported");
+        std::string type = pf.load_string("type");
+        PixelFilter &f = st.input.pixel_filter;
+        if (type == "box") {
+            f.type = KS_FILTER_BOX;
+            f.width = pf.load_float("width", 1.0f);
+        } else if (type == "gaussian") {
+            f.type = KS_FILTER_GAUSSIAN;
+            f.width = pf.load_float("width", 3.0f);
+            f.sigma = pf.load_float("sigma", 0.5f);
+        } else if (type == "blackman_harris") {
+            f.type = KS_FILTER_BLACKMAN_HARRIS;
+            f.width = pf.load_float("width", 3.0f);
+        } else {
+            throw ConfigError("Invalid pixel filter type [" + type + "]");
+        }
+        if (!(f.width > 0.0f)) throw ConfigError("pixel filter width must be positive");
     }
     if (args.contains("tone_mapper")) st.tone_mapper = create_tone_mapper(args["tone_mapper"]);
     st.output_pfm = args.load_bool("output_pfm", false);
@@ -486,7 +500,7 @@ std::string describe(const SmallPTSetup &st) {
                   "\"render_height\": %d, \"crop\": [%d, %d, %d, %d], \"spp\": %d, \"rng_seed\": %d, "
                   "\"include_background\": %s, \"backdrop\": [%g, %g, %g], \"clamp_direct\": %g, "
                   "\"clamp_indirect\": %g, \"spp_prog_interval\": %d, \"mis_mode\": %d, \"queue_size\": %d, "
-                  "\"bvh_builder\": %d, "
+                  "\"bvh_builder\": %d, \"pixel_filter\": {\"type\": %d, \"width\": %g, \"sigma\": %g}, "
                   "\"guided\": %s, \"guide\": {\"fraction\": %g, \"train_spp\": %d, \"learning_rate\": %g, "
                   "\"log2_table_size\": %d, \"inference_fp8\": %d, \"max_bounce\": %d}, "
                   "\"camera\": {\"use_matrix\": %d, \"pos\": [%g, %g, %g], \"target\": [%g, %g, %g], "
@@ -498,7 +512,8 @@ std::string describe(const SmallPTSetup &st) {
                   in.render_width, in.render_height, in.crop_start_x, in.crop_start_y, in.crop_width, in.crop_height,
                   in.spp, in.rng_seed, in.include_background ? "true" : "false", in.backdrop[0], in.backdrop[1],
                   in.backdrop[2], in.clamp_direct, in.clamp_indirect, in.spp_prog_interval, in.mis_mode,
-                  in.queue_size, st.bvh_builder, st.guided ? "true" : "false", st.guide_cfg.guide_fraction, st.guide_cfg.train_spp,
+                  in.queue_size, st.bvh_builder, in.pixel_filter.type, in.pixel_filter.width, in.pixel_filter.sigma,
+                  st.guided ? "true" : "false", st.guide_cfg.guide_fraction, st.guide_cfg.train_spp,
                   st.guide_cfg.learning_rate, st.guide_cfg.log2_table_size, st.guide_cfg.inference_fp8, st.guide_cfg.max_guided_bounce, c.use_matrix, c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
                   c.up[0], c.up[1], c.up[2], c.vfov_deg, c.aspect, c.to_world[0], c.to_world[1], c.to_world[2],
                   c.to_world[3], c.to_world[4], c.to_world[5], c.to_world[6], c.to_world[7], c.to_world[8],

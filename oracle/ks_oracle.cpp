@@ -13,6 +13,7 @@
 #include <limits>
 #include <memory>
 #include <thread>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -2376,15 +2377,97 @@ int resolve_threads(int t) {
     return h ? (int)h : 1;
 }
 
+// Pixel filters (render_target.h:70-184, create_pixel_filter render_target.cpp:28-44).
+struct PixelFilter {
+    int type = 0;
+    float r = 0.5f, sigma = 0.5f;
+    int n = 0;                             // sample-table size (gaussian, Blackman-Harris)
+    std::vector<std::vector<float>> cond;  // DistribTable per row (cdf, n + 1 entries)
+    std::vector<float> margin;             // DistribTable over rows
+    float eval(float x, float y) const {
+        if (x < -r || x > r || y < -r || y > r) return 0.0f;
+        if (type == 1) { // GaussianPixelFilter::eval (render_target.h:135-143), gaussian (maths.h:1070-1073)
+            auto g = [&](float v) {
+                return 1.0f / std::sqrt(6.28318530718f * sigma * sigma) * std::exp(-sqr(v - 0.0f) / (2.0f * sigma * sigma));
+            };
+            float thr = g(r);
+            return (g(x) - thr) * (g(y) - thr);
+        }
+        // BlackmanHarrisPixelFilter::eval (render_target.h:166-180)
+        constexpr float a0 = 0.35875f, a1 = 0.48829f, a2 = 0.14128f, a3 = 0.01168f, pi = 3.14159265359f;
+        float xx = (x + r) / (2.0f * r);
+        float fx = a0 - a1 * std::cos(2.0f * pi * xx) + a2 * std::cos(4.0f * pi * xx) - a3 * std::cos(6.0f * pi * xx);
+        float yy = (y + r) / (2.0f * r);
+        float fy = a0 - a1 * std::cos(2.0f * pi * yy) + a2 * std::cos(4.0f * pi * yy) - a3 * std::cos(6.0f * pi * yy);
+        return fx * fy;
+    }
+};
+
+// DistribTable::sample_linear (distrib.cpp:43-53)
+float sample_linear(const std::vector<float> &cdf, float u, uint32_t &index) {
+    u = clampv(u, 0.0f, std::nextafter(1.0f, 0.0f));
+    auto it = std::upper_bound(cdf.begin(), cdf.end(), u);
+    index = (uint32_t)std::distance(cdf.begin(), std::prev(it));
+    float n = (float)(cdf.size() - 1);
+    float du = (u - cdf[index]) / (cdf[index + 1] - cdf[index]);
+    return (index + du) / n;
+}
+
+// The filter of the params, with its sample table (PixelFilterSampleTable, render_target.h:79-106) built
+// once per (type, width, sigma).
+const PixelFilter &pixel_filter(const ora_params &p) {
+    static std::mutex mu;
+    static std::vector<std::unique_ptr<PixelFilter>> cache;
+    float width = p.filter_width != 0.0f ? p.filter_width : (p.pixel_filter == 0 ? 1.0f : 3.0f);
+    float sigma = p.filter_sigma != 0.0f ? p.filter_sigma : 0.5f;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto &f : cache)
+        if (f->type == p.pixel_filter && f->r == 0.5f * width && f->sigma == sigma) return *f;
+    auto f = std::make_unique<PixelFilter>();
+    f->type = p.pixel_filter;
+    f->r = 0.5f * width;
+    f->sigma = sigma;
+    if (f->type != 0) {
+        const float r = f->r;
+        const int n = (int)std::ceil(32 * r);
+        f->n = n;
+        std::vector<float> vals((size_t)n * n), acc(n);
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) {
+                float tx = (x + 0.5f) / n, ty = (y + 0.5f) / n;
+                vals[(size_t)y * n + x] = f->eval(std::lerp(-r, r, tx), std::lerp(-r, r, ty));
+            }
+        for (int y = 0; y < n; ++y) {
+            std::vector<float> row(vals.begin() + (size_t)y * n, vals.begin() + (size_t)(y + 1) * n);
+            std::vector<float> cdf(n + 1);
+            cdf[0] = 0.0f;
+            for (int i = 1; i <= n; ++i) cdf[i] = cdf[i - 1] + row[i - 1] * (1.0f / (float)n);
+            acc[y] = cdf[n];
+            f->cond.push_back(make_cdf(row));
+        }
+        f->margin = make_cdf(acc);
+    }
+    cache.push_back(std::move(f));
+    return *cache.back();
+}
+
 void camera_sample(const Camera &cam, const ora_params &p, int px, int py, ZSobol &sob, V3 &o, V3 &d, V3 *rxd = nullptr,
                    V3 *ryd = nullptr) {
-    // small_pt.cpp:98-105 with BoxPixelFilter (render_target.h:108-125)
+    // small_pt.cpp:98-105: the filter's offset from the pixel centre (render_target.h:108-184)
     float ox = 0.0f, oy = 0.0f;
     if (p.spp != 1) {
+        const PixelFilter &f = pixel_filter(p);
         float u0, u1;
         sob.next2d(u0, u1);
-        ox = (1.0f - u0) * -0.5f + u0 * 0.5f;
-        oy = (1.0f - u1) * -0.5f + u1 * 0.5f;
+        if (f.type != 0) { // PixelFilterSampleTable::sample (render_target.h:96-103), DistribTable2D::sample_linear
+            uint32_t iy, ix;
+            float vy = sample_linear(f.margin, u1, iy);
+            float vx = sample_linear(f.cond[iy], u0, ix);
+            u0 = vx;
+            u1 = vy;
+        }
+        ox = (1.0f - u0) * -f.r + u0 * f.r; // lerp(Constant(-r), Constant(r), t) (maths.h:686-690)
+        oy = (1.0f - u1) * -f.r + u1 * f.r;
     }
     float fx = ((float)px + 0.5f + ox) / (float)p.render_width;
     float fy = ((float)py + 0.5f + oy) / (float)p.render_height;
@@ -2818,6 +2901,30 @@ int ora_light_select(void *scene, const float *u, int64_t n, int32_t *idx, float
         prob[i] = light_probability(s, (uint32_t)idx[i]);
     }
     return 0;
+}
+
+// The pixel filter of the params: offsets from u (n x 2) through sample(), and eval() at offsets (n x 2).
+int ora_pixel_filter(const ora_params *p, const float *u, const float *xy, int64_t n, float *offset, float *value) {
+    const PixelFilter &f = pixel_filter(*p);
+    for (int64_t i = 0; i < n; ++i) {
+        if (u && offset) {
+            float u0 = u[2 * i], u1 = u[2 * i + 1];
+            if (f.type != 0) {
+                uint32_t iy, ix;
+                float vy = sample_linear(f.margin, u1, iy);
+                float vx = sample_linear(f.cond[iy], u0, ix);
+                u0 = vx;
+                u1 = vy;
+            }
+            offset[2 * i] = (1.0f - u0) * -f.r + u0 * f.r;
+            offset[2 * i + 1] = (1.0f - u1) * -f.r + u1 * f.r;
+        }
+        if (xy && value) {
+            float x = xy[2 * i], y = xy[2 * i + 1];
+            value[i] = f.type == 0 ? ((x < -f.r || x > f.r || y < -f.r || y > f.r) ? 0.0f : 1.0f) : f.eval(x, y);
+        }
+    }
+    return f.n;
 }
 
 } // extern "C"

@@ -147,6 +147,9 @@ typedef struct ora_params {
     float clamp_direct, clamp_indirect;
     int32_t mis_mode;
     int32_t scale_ray_diff; /* camera ray differentials scaled by 1/sqrt(spp) (camera.cpp:162-167) */
+    int32_t pixel_filter;   /* 0 box, 1 gaussian, 2 Blackman-Harris (create_pixel_filter, render_target.cpp:28-44) */
+    float filter_width;     /* 0: the type's default (1, 3, 3) */
+    float filter_sigma;     /* gaussian, 0: 0.5 */
 } ora_params;
 
 void *ora_scene_create(const ora_scene_desc *desc);
@@ -217,6 +220,9 @@ int ora_bsdf_lobe(const ora_material *m, int lobe, const float *wo, const float 
 int ora_light_sample(void *scene, int light, const float *p, const float *u, int64_t n, float *wi, float *dist,
                      float *pdf, float *le_over_pdf);
 int ora_light_pdf(void *scene, int light, const float *wi, const float *dist, int64_t n, float *pdf);
+/* The params' pixel filter: offsets sampled from u (n x 2) and values at offsets xy (n x 2) (either pair may
+ * be NULL). Returns the sample-table size n (0 for the box filter). */
+int ora_pixel_filter(const ora_params *p, const float *u, const float *xy, int64_t n, float *offset, float *value);
 /* Light selection from u (n): index and probability. */
 int ora_light_select(void *scene, const float *u, int64_t n, int32_t *idx, float *prob);
 

@@ -125,6 +125,9 @@ class OraParams(C.Structure):
         ("clamp_indirect", C.c_float),
         ("mis_mode", C.c_int32),
         ("scale_ray_diff", C.c_int32),
+        ("pixel_filter", C.c_int32),
+        ("filter_width", C.c_float),
+        ("filter_sigma", C.c_float),
     ]
 
 
@@ -161,6 +164,7 @@ _SIGS = {
     "ora_light_sample": ([_P, C.c_int, _P, _P, C.c_int64, _P, _P, _P, _P], C.c_int),
     "ora_light_pdf": ([_P, C.c_int, _P, _P, C.c_int64, _P], C.c_int),
     "ora_light_select": ([_P, _P, C.c_int64, _P, _P], C.c_int),
+    "ora_pixel_filter": ([C.POINTER(OraParams), _P, _P, C.c_int64, _P, _P], C.c_int),
 }
 
 _libs: dict = {}
@@ -219,6 +223,9 @@ def params(**kw) -> OraParams:
     p.clamp_indirect = kw.get("clamp_indirect", 10.0)
     p.mis_mode = kw.get("mis_mode", 0)
     p.scale_ray_diff = int(kw.get("scale_ray_diff", True))
+    p.pixel_filter = kw.get("pixel_filter", 0)
+    p.filter_width = kw.get("filter_width", 0.0)
+    p.filter_sigma = kw.get("filter_sigma", 0.0)
     return p
 
 
@@ -576,3 +583,15 @@ def bsdf_lobe(mat, lobe, wo, wi):
     out = np.zeros((wi.shape[0], 3), np.float32)
     assert load().ora_bsdf_lobe(C.byref(m), lobe, _p(wo), _p(wi), wi.shape[0], _p(out)) == 0
     return out
+
+
+def pixel_filter(prm: OraParams, u=None, xy=None):
+    """The params' pixel filter: (offsets sampled from u, values at xy, sample-table size)."""
+    n = len(u) if u is not None else len(xy)
+    u = None if u is None else np.ascontiguousarray(u, np.float32)
+    xy = None if xy is None else np.ascontiguousarray(xy, np.float32)
+    off = np.zeros((n, 2), np.float32)
+    val = np.zeros(n, np.float32)
+    tn = load().ora_pixel_filter(C.byref(prm), None if u is None else _p(u), None if xy is None else _p(xy), n,
+                                 _p(off), _p(val))
+    return off, val, tn

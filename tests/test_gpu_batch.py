@@ -52,3 +52,35 @@ def test_batch_equals_single_views(ks, device, cornell, mode):
     for f in films:
         f.close()
     sc.close()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_views_split_across_chunks(ks, device, cornell, graph):
+    """A small chunk budget (50 000 items) splits the batch into four chunks, and the spp of the first
+    three views across chunk boundaries (segments starting at s0 > 0, one sentinel per chunk): every film
+    still equals its single render bit for bit. The autotuner's probe runs on the first chunk's partial
+    segment list."""
+    sc = ks.Scene(device, cornell)
+    cam2 = dict(cornell.camera)
+    cam2["pos"] = (0.3, 0.4, 2.2)
+    specs = [(cornell.camera, dict(render_width=64, render_height=64, spp=16, rng_seed=1)),
+             (cam2, dict(render_width=32, render_height=32, spp=64, rng_seed=2)),
+             (cornell.camera, dict(render_width=48, render_height=40, spp=9, rng_seed=3))]
+    views = [(cam, ks.small_pt_params(bounces=12, spp_prog_interval=kw["spp"], **kw), kw) for cam, kw in specs]
+    films = [ks.Film(device, kw["render_width"], kw["render_height"]) for _, _, kw in views]
+    kwargs = dict(max_chunk_items=50_000, graph=graph)
+    if not graph:
+        kwargs.update(queue_size=1 << 13, trace_blocks_per_cu=4)
+    st = ks.small_pt_render_batch(sc, [(c, p, f) for (c, p, _), f in zip(views, films)], **kwargs)
+    print("chunks", st["batch_chunks"], "queue", st["queue_size"], flush=True)
+    assert st["batch_chunks"] == 4
+    for (cam, p, kw), f in zip(views, films):
+        alone = ks.Film(device, kw["render_width"], kw["render_height"])
+        ks.small_pt_render(sc, cam, p, alone)
+        a, wa = f.download()
+        b, wb = alone.download()
+        assert np.array_equal(wa, wb) and np.array_equal(a, b), kw
+        alone.close()
+    for f in films:
+        f.close()
+    sc.close()

@@ -1,0 +1,198 @@
+"""The benchmark configurations at their own sizes, GPU against the CPU oracle.
+
+- Config 2 (64 icospheres, 1,310,728 triangles, 1920x1080, 64 spp, 12 bounces): 20 000 (pixel, sample)
+  paths traced by both, and a 16-row band of the frame rendered by both at 64 spp.
+- Config 3 (48 seed-drawn views of the Cornell box, 12 bounces): every batched view equals its single
+  render bit for bit, and a batched view matches the oracle's image.
+- Config 5 (the room with 30 icospheres, 10,032,258 triangles, device-built BVH, 3840x2160): paths traced by
+  both at the full resolution; guided rendering with FP8 inference stays unbiased on this scene.
+- The device's FP64 relMSE (the metric of every time-to-relMSE figure) equals the host's FP64 formula on
+  the downloaded films.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from conftest import rel_mse
+
+pytestmark = pytest.mark.gpu
+
+
+def _agree(g, o):
+    return np.all(np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o)), axis=1)
+
+
+def _img(rgb, w):
+    return rgb / np.maximum(w[..., :1] + w[..., 1:], 1)
+
+
+@pytest.fixture(scope="module")
+def spheres(ks):
+    return ks.procedural_scene("spheres", 1)
+
+
+def test_config2_paths_match_oracle(ks, device, ora, spheres):
+    sc = ks.Scene(device, spheres)
+    osc = ora.OracleScene.from_host_scene(spheres)
+    W, H, spp = 1920, 1080, 64
+    kw = dict(bounces=12, render_width=W, render_height=H, spp=spp)
+    rng = np.random.default_rng(21)
+    n = 20000
+    px = rng.integers(0, W, n).astype(np.int32)
+    py = rng.integers(0, H, n).astype(np.int32)
+    s = rng.integers(0, spp, n).astype(np.int32)
+    g = sc.trace_samples(spheres.camera, ks.small_pt_params(**kw), px, py, s)
+    o = osc.trace_samples(spheres.camera, ora.params(**kw), px, py, s)
+    a = _agree(g, o)
+    print("config 2 per-sample agreement", a.mean(), "hit flags equal", (g[:, 3] == o[:, 3]).mean(), flush=True)
+    assert a.mean() > 0.999
+    assert (g[:, 3] == o[:, 3]).mean() > 0.9999
+    # a 16-row band through the middle of the frame at the full 64 spp
+    band = dict(kw, crop_start_x=0, crop_start_y=532, crop_width=W, crop_height=16)
+    film = ks.Film(device, W, H)
+    ks.small_pt_render(sc, spheres.camera, ks.small_pt_params(**band), film)
+    grgb, gw = film.download()
+    orgb, ow = osc.render(spheres.camera, ora.params(**band))
+    rows = slice(532, 548)
+    assert np.array_equal(gw[rows], ow[rows])
+    assert not gw[:532].any() and not gw[548:].any()
+    e = rel_mse(_img(grgb[rows], gw[rows]), _img(orgb[rows], ow[rows]))
+    print("config 2 band relMSE gpu vs oracle", e, flush=True)
+    assert e < 1e-4
+    film.close()
+    osc.close()
+    sc.close()
+
+
+def test_config3_batched_views(ks, device, ora, cornell):
+    from keepsake_amd.workloads import batch_views
+
+    sc = ks.Scene(device, cornell)
+    views = batch_views(ks, cornell.camera, 48, 0)
+    films = [ks.Film(device, p.render_width, p.render_height) for _, p in views]
+    st = ks.small_pt_render_batch(sc, [(c, p, f) for (c, p), f in zip(views, films)])
+    print("config 3:", st["paths"], "paths in", st["batch_chunks"], "chunks, queue", st["queue_size"], flush=True)
+    assert st["paths"] == sum(p.render_width * p.render_height * p.spp for _, p in views)
+    for (cam, p), f in zip(views, films):
+        alone = ks.Film(device, p.render_width, p.render_height)
+        ks.small_pt_render(sc, cam, p, alone)
+        a, wa = f.download()
+        b, wb = alone.download()
+        assert np.array_equal(wa, wb) and np.array_equal(a, b), (p.render_width, p.render_height, p.spp)
+        alone.close()
+    # the smallest batched view against the oracle
+    k = min(range(len(views)), key=lambda i: views[i][1].render_width * views[i][1].render_height * views[i][1].spp)
+    cam, p = views[k]
+    grgb, gw = films[k].download()
+    kw = dict(bounces=12, render_width=p.render_width, render_height=p.render_height, spp=p.spp, rng_seed=p.rng_seed)
+    orgb, ow = ora.OracleScene.from_host_scene(cornell).render(cam, ora.params(**kw))
+    assert np.array_equal(gw, ow)
+    e = rel_mse(_img(grgb, gw), _img(orgb, ow))
+    print("config 3 view", k, (p.render_width, p.render_height, p.spp), "relMSE vs oracle", e, flush=True)
+    assert e < 2e-5
+    for f in films:
+        f.close()
+    sc.close()
+
+
+@pytest.fixture(scope="module")
+def room_spheres(ks):
+    return ks.procedural_scene("room_spheres", 4)
+
+
+def test_config5_paths_match_oracle(ks, device, ora, room_spheres):
+    hs = room_spheres
+    assert hs.n_tris == 10032258
+    sc = ks.Scene(device, hs, bvh={"builder": 1})
+    osc = ora.OracleScene.from_host_scene(hs)
+    W, H = 3840, 2160
+    kw = dict(bounces=16, render_width=W, render_height=H, spp=64, mis_mode=ks.KS_MIS_CONSISTENT,
+              clamp_direct=0.0, clamp_indirect=0.0)
+    cam = dict(hs.camera)
+    cam["aspect"] = W / H
+    rng = np.random.default_rng(23)
+    n = 20000
+    px = rng.integers(0, W, n).astype(np.int32)
+    py = rng.integers(0, H, n).astype(np.int32)
+    s = rng.integers(0, 64, n).astype(np.int32)
+    g = sc.trace_samples(cam, ks.small_pt_params(**kw), px, py, s)
+    o = osc.trace_samples(cam, ora.params(**kw), px, py, s)
+    a = _agree(g, o)
+    print("config 5 per-sample agreement", a.mean(), flush=True)
+    assert a.mean() > 0.999
+    osc.close()
+    sc.close()
+
+
+def test_config5_fp8_guided_unbiased(ks, device, room_spheres):
+    """FP8-inference guiding on the 10M-triangle scene at 128x72: block means agree with an FP64 plain
+    reference (Bonferroni bar over the block-channels), and the device relMSE equals the host formula."""
+    hs = room_spheres
+    sc = ks.Scene(device, hs, bvh={"builder": 1})
+    W, H = 128, 72
+    cam = dict(hs.camera)
+    cam["aspect"] = W / H
+
+    def prm(spp, interval):
+        return ks.small_pt_params(bounces=16, render_width=W, render_height=H, spp=spp, spp_prog_interval=interval,
+                                  mis_mode=ks.KS_MIS_CONSISTENT, clamp_direct=0.0, clamp_indirect=0.0)
+
+    ref = ks.Film(device, W, H)
+    ks.small_pt_render(sc, cam, prm(8192, 8192), ref)
+    ref_rgb, ref_w = ref.download()
+    info = sc.info()
+    g = ks.Guide(device, info["bound_lo"], info["bound_hi"],
+                 ks.guide_config(seed=5, guide_fraction=0.5, inference_fp8=1, train_spp=16))
+    film = ks.Film(device, W, H)
+    sums = []
+    P = 144
+    ks.small_pt_render_guided(sc, cam, prm(P, 1), g, film, progress=lambda f, s: sums.append(f.download()[0]))
+    assert len(sums) == P
+    per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)
+    blocks = per_pass[16:].reshape(P - 16, H // 8, 8, W // 8, 8, 3).mean(axis=(2, 4))
+    ref_blocks = _img(ref_rgb, ref_w).reshape(H // 8, 8, W // 8, 8, 3).mean(axis=(1, 3))
+    se = blocks.std(axis=0, ddof=1) / math.sqrt(P - 16)
+    se_ref = se * math.sqrt((P - 16) / 8192)
+    z = (blocks.mean(axis=0) - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
+    live = se > 0
+    print("config 5 guided: max |z|", float(np.abs(z[live]).max()), "over", int(live.sum()), flush=True)
+    assert np.abs(z[live]).max() < 4.3  # Bonferroni over 432 block-channels at 1%
+    img = _img(*film.download())
+    host = rel_mse(img, _img(ref_rgb, ref_w))
+    dev_r = film.relmse(ref)
+    print("relMSE device", dev_r, "host", host, flush=True)
+    assert abs(dev_r - host) <= 1e-12 + 1e-9 * host
+    g.close()
+    film.close()
+    ref.close()
+    sc.close()
+
+
+def test_device_relmse_matches_host(ks, device, cornell):
+    """ks_film_relmse against the host's FP64 formula on the downloaded films, with a backdrop and misses."""
+    sc = ks.Scene(device, cornell)
+    cam = dict(cornell.camera)
+    a = ks.Film(device, 96, 64, backdrop=(0.2, 0.1, 0.05))
+    b = ks.Film(device, 96, 64, backdrop=(0.2, 0.1, 0.05))
+    p = dict(bounces=6, render_width=96, render_height=64, include_background=False, backdrop=(0.2, 0.1, 0.05))
+    ks.small_pt_render(sc, cam, ks.small_pt_params(spp=4, **p), a)
+    ks.small_pt_render(sc, cam, ks.small_pt_params(spp=64, rng_seed=9, **p), b)
+    ra, wa = a.download()
+    rb, wb = b.download()
+
+    def comp(rgb, w, bd):
+        ws = np.floor(w[..., 0] + w[..., 1])
+        t = np.where(ws == 0, 0.0, w[..., 0] / np.where(ws == 0, 1, ws))
+        fg = np.where(w[..., :1] == 0, 0.0, rgb / np.where(w[..., :1] == 0, 1, w[..., :1]))
+        return (1 - t)[..., None] * np.asarray(bd) + t[..., None] * fg
+
+    bd = np.array([0.2, 0.1, 0.05], np.float32).astype(np.float64)
+    host = rel_mse(comp(ra, wa, bd), comp(rb, wb, bd))
+    dev_r = a.relmse(b)
+    print("relMSE device", dev_r, "host", host, flush=True)
+    assert host > 0 and abs(dev_r - host) <= 1e-12 * max(1.0, host) + 1e-9 * host
+    a.close()
+    b.close()
+    sc.close()

@@ -5,8 +5,8 @@
    around the estimator changes nothing.
 2. With the trained guide sampling half of the directions the estimator stays unbiased, with BF16 and
    with FP8 (e4m3) inference of the network: block means of
-   the per-pass images agree with an FP64 plain reference within their standard errors (z-test), and
-   relMSE keeps falling as 1 / spp.
+   the per-pass images agree with an FP64 plain reference within their standard errors (z-test with a
+   Bonferroni bar), and relMSE halves from P/2 to P spp.
 3. Training lowers the cross-entropy of the mixture on the radiance samples below that of a cosine lobe
    (the network learns something the BSDF-only method does not have)."""
 
@@ -83,15 +83,25 @@ def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
     se = blocks[16:].std(axis=0, ddof=1) / math.sqrt(P - 16)
     se_ref = se * math.sqrt((P - 16) / 8192)  # the reference's own error, same variance per sample assumed
     z = (mean - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
-    print("max |z| over", z.size, "block-channels:", float(np.abs(z).max()), flush=True)
-    assert np.abs(z).max() < 5.0
+    n_over3 = int((np.abs(z) > 3.0).sum())
+    print("max |z| over", z.size, "block-channels:", float(np.abs(z).max()), "count |z| > 3:", n_over3, flush=True)
+    # Bonferroni over the 432 block-channels at 1%: two-sided normal quantile of 1 - 0.01 / 864 = 4.23
+    assert np.abs(z).max() < 4.3
+    # |z| > 3 has probability 0.0027 per block-channel under no bias: 1.2 expected, P(>= 7) ~ 1e-3
+    assert n_over3 <= 6
     img_mean = blocks[16:].mean(axis=(1, 2, 3))
     zi = (img_mean.mean() - ref_blocks.mean()) / (img_mean.std(ddof=1) / math.sqrt(P - 16))
     print("whole-image z", float(zi), flush=True)
     assert abs(zi) < 4.0
-    # relMSE of the whole guided render keeps falling (an estimator with a bias would level off)
+    # relMSE keeps falling as 1 / spp: halving it from P/2 to P spp (a bias would level it off). The
+    # reference's own noise (8192 spp) is below 2% of the guided render's.
+    ref_img = ref_rgb / np.maximum(ref_w[..., :1] + ref_w[..., 1:], 1)
+    r_half = float(np.mean((sums[P // 2 - 1] / (P // 2) - ref_img) ** 2 / (ref_img**2 + 0.01)))
+    r_full = float(np.mean((sums[P - 1] / P - ref_img) ** 2 / (ref_img**2 + 0.01)))
     r = film.relmse(ref)
-    print("relMSE after", P, "spp:", r, flush=True)
+    print("relMSE after", P // 2, "and", P, "spp:", r_half, r_full, "device", r, flush=True)
+    assert 1.6 < r_half / r_full < 2.5
+    assert abs(r - r_full) <= 1e-6 + 1e-3 * r_full  # the device's FP64 reduction of the same film
     g.close()
     film.close()
     ref.close()

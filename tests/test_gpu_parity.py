@@ -262,7 +262,8 @@ def test_trace_samples_match(device, ks, ora, cornell):
     close = np.all(np.abs(g - o) <= 1e-3 * np.maximum(1.0, np.abs(o)), axis=1)
     _log("per-sample agreement", close.mean())
     # the same Sobol points drive both; paths only diverge after float-rounding flips a decision
-    assert close.mean() > 0.97
+    # (measured: 99.995% agree)
+    assert close.mean() > 0.999
     assert np.array_equal(g[:, 3], o[:, 3])
     sc.close()
 
@@ -280,7 +281,7 @@ def test_render_cornell_matches_oracle(device, ks, ora, cornell):
     oimg = orgb / np.maximum(ow[..., :1], 1)
     e = rel_mse(gimg, oimg)
     _log("cornell relMSE gpu vs oracle", e, "stats", st["iterations"], st["rays_closest"], st["nonfinite"])
-    assert e < 1e-3
+    assert e < 2e-5  # measured 3.7e-6
     assert st["nonfinite"] == 0
     film.close()
     sc.close()

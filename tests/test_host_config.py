@@ -151,7 +151,7 @@ scale_ray_diff = false
         ('object = "mesh_asset.teapot"\n', "unknown asset 'mesh_asset.teapot'"),
         ('compound_object = "compound_mesh_asset.x"\n', "unknown asset 'compound_mesh_asset.x'"),
         ('camera_animation = "anim"\n', "camera_animation"),
-        ('pixel_filter = { type = "gaussian" }\n', "box pixel filter"),
+        ('pixel_filter = { type = "mitchell" }\n', "Invalid pixel filter type"),
         ('mis_mode = "fancy"\n', "mis_mode"),
         ('bvh_builder = "gpu"\n', "bvh_builder"),
         ('spp = "many"\n', "duplicate key"),
@@ -230,3 +230,22 @@ def test_sky_map(tmp_path):
     r = resolve(tmp_path, BASE + f'sky = {{ map = "{tmp_path}/sky.hdr" }}\n'
                 f'light = [ {{ type = "sky", map = "{tmp_path}/sky.exr" }} ]\n')
     assert r.returncode != 0 and "at most one sky map" in r.stderr
+
+
+@pytest.mark.parametrize("text,want", [
+    ('pixel_filter = { type = "gaussian" }\n', (1, 3.0, 0.5)),
+    ('pixel_filter = { type = "gaussian", width = 2.0, sigma = 0.3 }\n', (1, 2.0, 0.3)),
+    ('pixel_filter = { type = "blackman_harris" }\n', (2, 3.0, 0.0)),
+    ('pixel_filter = { type = "box", width = 2.0 }\n', (0, 2.0, 0.0)),
+    ("", (0, 0.0, 0.0)),
+])
+def test_pixel_filter_key(tmp_path, text, want):
+    """create_pixel_filter's keys and defaults (render_target.cpp:28-44)."""
+    (t,) = tasks(tmp_path, BASE + text)
+    f = t["pixel_filter"]
+    assert (f["type"], f["width"], f["sigma"]) == pytest.approx(want)
+
+
+def test_pixel_filter_unknown_type(tmp_path):
+    r = resolve(tmp_path, BASE + 'pixel_filter = { type = "mitchell" }\n')
+    assert r.returncode != 0 and "Invalid pixel filter type" in r.stderr
