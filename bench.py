@@ -31,6 +31,10 @@ HBM_PEAK_GBS = 8000.0
 RAY_IO_BYTES = 48
 NODE_BYTES = {4: 112, 2: 64}
 TRI_BYTES = 48
+# The implementation-independent form (per closest-hit ray): ray 32 (o, tmin, d, tmax) + hit 20 (t, barycentrics,
+# primitive) + 32 per node fetch of the reference SBVH (its node is 32 B) + 52 per triangle test (4 B reference +
+# 12 B indices + 36 B vertices), with node fetches and triangle tests counted in the reference traversal order.
+REF_RAY_BYTES, REF_HIT_BYTES, REF_NODE_BYTES, REF_TRI_BYTES = 32, 20, 32, 52
 # Shading, per path vertex (one per closest-hit ray): read hit 20 + path state 64 + material 64 +
 # triangle 48 + light record 48 + light CDF 4 per binary-search step; write path state 64 + extension
 # ray 32. Per shadow ray: write the ray 32 + its contribution 12.
@@ -158,6 +162,13 @@ def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, ma
     s_last, _, r_last = passes[-1]
     res["plain"] = {"pass_spp": pass_spp, "seconds": round(hit[1], 3) if hit else None,
                     "spp": hit[0] if hit else None, "relmse_at_max_spp": r_last, "max_spp": s_last}
+    # the reference's own noise: relMSE(s) = v / s + c over the plain passes (least squares in 1/s); the
+    # intercept c is what the ref_spp reference adds to every figure, expected near v / ref_spp
+    inv = np.array([1.0 / s for s, _, _ in passes])
+    rr = np.array([r for _, _, r in passes])
+    v, c = np.linalg.lstsq(np.stack([inv, np.ones_like(inv)], 1), rr, rcond=None)[0]
+    res["reference_noise"] = {"fit": "relMSE(s) = v / s + c over the plain passes", "v": float(v), "c": float(c),
+                              "expected_c": float(v / ref_spp), "c_over_target": float(c / target)}
     log(f"plain: relMSE {target} at {hit}")
 
     for name, fp8 in methods:
@@ -243,7 +254,9 @@ def main() -> int:
                     help="BVH of the timed renders: the host SBVH or a device-built BVH")
     ap.add_argument("--no-device-bvh", action="store_true",
                     help="skip the leg that builds the scene's BVH on the device and renders over it")
-    ap.add_argument("--scale-builder", default="host", choices=("host", "device"))
+    ap.add_argument("--scale-builder", default="device", choices=("host", "device"))
+    ap.add_argument("--count-paths", type=int, default=1 << 16,
+                    help="random paths whose rays are counted in the reference traversal order (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -265,8 +278,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 48-view leg")
     ap.add_argument("--fp8-relmse", action="store_true",
                     help="also time the guided tracer with FP8 inference in the room's time-to-relMSE leg")
-    ap.add_argument("--scale-leg", action="store_true",
-                    help="time-to-relMSE at scale: the room with 30 icospheres (10M triangles) at "
+    ap.add_argument("--no-scale-leg", action="store_true",
+                    help="skip the time-to-relMSE leg at scale: the room with 30 icospheres (10M triangles) at "
                     "--scale-width x --scale-height, plain vs guided with FP8 inference")
     ap.add_argument("--scale-width", type=int, default=3840)
     ap.add_argument("--scale-height", type=int, default=2160)
@@ -368,6 +381,26 @@ def main() -> int:
     paths = args.width * args.height * args.spp
     value = paths * args.steps / elapsed / 1e6
 
+    # Implementation-independent traversal bytes: the reference traversal order (SBVH::intersect /
+    # intersectBool over the BVH2 SBVH) counted on the host over every ray that a random sample of the
+    # workload's paths issues
+    ref_order = None
+    if builder == BUILDERS["host"] and args.count_paths > 0:
+        rng = np.random.default_rng(7)
+        m = args.count_paths
+        px = rng.integers(0, args.width, m).astype(np.int32)
+        py = rng.integers(0, args.height, m).astype(np.int32)
+        ps = rng.integers(0, args.spp, m).astype(np.int32)
+        t0 = time.time()
+        rays_c, rays_s = sc.capture_rays(cam, params, px, py, ps, max_rays=m * (args.bounces + 1))
+        nc, tc = sc.traversal_counts(rays_c)
+        ns, ts = sc.traversal_counts(rays_s, any_hit=True)
+        ref_order = {"paths": m, "closest_rays": len(rays_c), "shadow_rays": len(rays_s),
+                     "nodes_per_closest_ray": nc / len(rays_c), "tris_per_closest_ray": tc / len(rays_c),
+                     "nodes_per_shadow_ray": ns / len(rays_s), "tris_per_shadow_ray": ts / len(rays_s),
+                     "count_s": round(time.time() - t0, 1)}
+        log(f"reference-order traversal counts: {ref_order}")
+
     launches = totals["launches"]["closest"]
     k_ms = totals["kernel_ms"]["closest"]
     alg_bytes = (
@@ -380,17 +413,38 @@ def main() -> int:
     tf = ROOT / "profiles" / "traffic_closest_spheres.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    avg_ms = k_ms / max(launches, 1)
+    kernel_counted = {"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                      "bytes_per_launch": alg_bytes / max(launches, 1),
+                      "bytes_per_ray": f"{RAY_IO_BYTES} + {NODE_BYTES[info['branching']]} per BVH{info['branching']} node "
+                      f"visit + {TRI_BYTES} per triangle test, counted by the kernel"}
     roofline = {
         "bound": "hbm",
-        "achieved": round(achieved, 1),
+        "achieved": kernel_counted["achieved"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "frac": kernel_counted["frac"],
         "traffic": traffic,
         "kernel": f"k_closest (closest-hit traversal, {BVH_NAME[builder]}, BVH{info['branching']} nodes)",
         "bytes_per_launch": alg_bytes / max(launches, 1),
-        "avg_launch_ms": k_ms / max(launches, 1),
+        "avg_launch_ms": avg_ms,
     }
+    if ref_order is not None:
+        # algorithmic bytes per closest ray: ray 32 + hit 20 + 32 per node fetch + 52 per triangle test, with
+        # the counts of the reference traversal order over the same rays (the primary figure)
+        per_ray = REF_RAY_BYTES + REF_HIT_BYTES + REF_NODE_BYTES * ref_order["nodes_per_closest_ray"] + \
+            REF_TRI_BYTES * ref_order["tris_per_closest_ray"]
+        ref_bytes = per_ray * totals["rays_closest"]
+        ref_achieved = ref_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
+        roofline.update({"achieved": round(ref_achieved, 1), "frac": round(ref_achieved / HBM_PEAK_GBS, 4),
+                         "bytes_per_launch": ref_bytes / max(launches, 1),
+                         "bytes_per_ray": round(per_ray, 1),
+                         "basis": "reference traversal order (cuda/sbvh.cu:512-603) over the BVH2 SBVH, counted on the "
+                         f"host for the {ref_order['closest_rays']} closest-hit rays of {ref_order['paths']} random paths"})
+    roofline["kernel_counted"] = kernel_counted
+    if traffic:
+        roofline["counter_hbm"] = {"bytes_per_launch": traffic, "achieved": round(traffic / (avg_ms / 1e3) / 1e9, 1),
+                                   "frac": round(traffic / (avg_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)}
 
     device_bvh = None
     if not args.no_device_bvh and builder == BUILDERS["host"]:
@@ -416,10 +470,25 @@ def main() -> int:
     if tf.exists():
         roofline_shade["traffic"] = json.loads(tf.read_text()).get("shade_hbm_bytes_per_launch")
 
+    roofline_shadow = None
+    if ref_order is not None and totals["kernel_ms"]["shadow"] > 0:
+        # per shadow ray: ray 32 + occlusion flag 4 + the reference-order node fetches and triangle tests
+        per_ray = REF_RAY_BYTES + 4 + REF_NODE_BYTES * ref_order["nodes_per_shadow_ray"] + \
+            REF_TRI_BYTES * ref_order["tris_per_shadow_ray"]
+        sh_ms = totals["kernel_ms"]["shadow"]
+        sh_ach = per_ray * totals["rays_shadow"] / (sh_ms / 1e3) / 1e9
+        roofline_shadow = {"bound": "hbm", "achieved": round(sh_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(sh_ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_ray": round(per_ray, 1),
+                           "kernel": "k_anyhit (NEE shadow rays)",
+                           "avg_launch_ms": sh_ms / max(totals["launches"]["shadow"], 1)}
+
     cpu = None
     if not args.no_cpu_baseline:
+        # the host's CPU share of one GPU on the benchmark machine is 16 threads (the box's limit for worker
+        # pools, whatever os.cpu_count() reports); --cpu-threads overrides
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(hs, cam, kw, args.cpu_rows, threads, args.cpu_seconds)
+        cpu["host_cpus_reported"] = os.cpu_count()
 
     batch = None
     if not args.no_batch:
@@ -436,7 +505,7 @@ def main() -> int:
         except ks.KsError as e:  # reported, not hidden: the plain-throughput line stays valid
             ttr = {"error": str(e)}
     scale = None
-    if args.scale_leg:
+    if not args.no_scale_leg:
         try:
             scale = time_to_relmse(ks, dev, args, scene=("room_spheres", 4), size=(args.scale_width, args.scale_height),
                                    ref_spp=args.scale_ref_spp, max_spp=args.scale_max_spp,
@@ -478,6 +547,8 @@ def main() -> int:
         },
         "roofline": roofline,
         "roofline_shade": roofline_shade,
+        "roofline_shadow": roofline_shadow,
+        "reference_order_counts": ref_order,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
         "time_to_relmse_at_scale": scale,

@@ -328,6 +328,23 @@ typedef struct ks_batch_options {
 int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_view *views,
                              const ks_batch_options *opt, ks_render_stats *stats);
 
+/* ---- measurement ---- */
+/* The rays a render issues: traces the listed (pixel, sample) paths exactly as ks_small_pt_render would
+ * (the same Sobol samples, kernels and params) and returns every closest-hit ray and every NEE shadow ray
+ * they issue, up to max_rays of each, as n x 8 floats (o, tmin, d, tmax). Replaces the reference's
+ * per-ray loop in SmallPT::trace (small_pt.cpp:72-178), which issues them one at a time through
+ * Scene::intersect1 / occlude1 (scene.cpp:206-244). */
+int ks_trace_capture(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
+                     const int32_t *px, const int32_t *py, const int32_t *sample, int64_t max_rays, float *closest_rays,
+                     int64_t *n_closest, float *shadow_rays, int64_t *n_shadow);
+/* The work of the reference traversal order on rays (n x 8): node fetches and triangle tests of
+ * SBVH::intersect (any_hit 0) or SBVH::intersectBool (1) (cuda/sbvh.cu:512-603), with the reference's
+ * slab and watertight triangle tests, over the scene's host-built SBVH in BVH2 form; counted on the host.
+ * t_out (optional, n): the closest hit distance (-1: miss), or 1 / -1 occluded / not. Needs a scene built
+ * with KS_BVH_BUILD_HOST_SBVH (else KS_ERR_UNSUPPORTED). */
+int ks_scene_traversal_counts(const ks_scene *scene, const float *rays, int64_t n, int32_t any_hit,
+                              uint64_t *node_fetches, uint64_t *tri_tests, float *t_out);
+
 /* ---- component entry points (parity tests; same kernels as the render path) ---- */
 /* PTRenderSampler (sobol.h:235-248) streams for n (pixel, sample) pairs; kinds[c] = 1 (next) or
  * 2 (next2d); out holds n * n_values values (uint32 before the float map, and the float). */

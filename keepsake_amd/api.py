@@ -409,6 +409,15 @@ _SIGS = {
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), C.c_int64, _P, _P, _P, _P],
         C.c_int,
     ),
+    "ks_trace_capture": (
+        [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), C.c_int64, _P, _P, _P, C.c_int64, _P,
+         C.POINTER(C.c_int64), _P, C.POINTER(C.c_int64)],
+        C.c_int,
+    ),
+    "ks_scene_traversal_counts": (
+        [_P, _P, C.c_int64, C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), _P],
+        C.c_int,
+    ),
     "ks_small_pt_render_guided": (
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), _P, _P, PROGRESS_FN, _P, C.POINTER(RenderStats)],
         C.c_int,
@@ -828,6 +837,28 @@ class Scene:
             )
         )
         return out
+
+    def capture_rays(self, cam: dict, params: SmallPTParams, px, py, sample, max_rays: int):
+        """Every closest-hit and shadow ray the listed paths issue: two (m, 8) arrays (o, tmin, d, tmax)."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        sample = np.ascontiguousarray(sample, np.int32)
+        rc = np.zeros((max_rays, 8), np.float32)
+        rs = np.zeros((max_rays, 8), np.float32)
+        nc, ns = C.c_int64(), C.c_int64()
+        c = camera_desc(cam)
+        _check(self.lib.ks_trace_capture(self.h, C.byref(c), C.byref(params), px.shape[0], _ptr(px), _ptr(py),
+                                         _ptr(sample), max_rays, _ptr(rc), C.byref(nc), _ptr(rs), C.byref(ns)))
+        return rc[: nc.value], rs[: ns.value]
+
+    def traversal_counts(self, rays, any_hit: bool = False, with_t: bool = False):
+        """Node fetches and triangle tests of the reference traversal order (host SBVH, BVH2) over rays."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        nodes, tris = C.c_uint64(), C.c_uint64()
+        t = np.zeros(rays.shape[0], np.float32) if with_t else None
+        _check(self.lib.ks_scene_traversal_counts(self.h, _ptr(rays), rays.shape[0], int(any_hit), C.byref(nodes),
+                                                  C.byref(tris), _ptr(t) if with_t else None))
+        return (nodes.value, tris.value, t) if with_t else (nodes.value, tris.value)
 
     def close(self) -> None:
         if self.h:

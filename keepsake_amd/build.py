@@ -39,7 +39,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wno-unused-function"]
 
 HIP_SOURCES = ["ks_api.hip", "ks_guide.hip", "ks_bvh_device.hip"]
-HOST_SOURCES = ["sbvh_build.cpp", "procedural.cpp"]
+HOST_SOURCES = ["sbvh_build.cpp", "procedural.cpp", "traversal_count.cpp"]
 HEADERS = ["ks_device.h", "ks_traverse.h", "ks_internal.h"]
 
 

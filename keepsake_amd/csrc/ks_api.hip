@@ -1145,6 +1145,9 @@ struct ks_scene {
     std::vector<int32_t> light_prim;
     std::vector<float> light_power, light_cdf;
     uint32_t max_stack = 1; // deepest traversal stack the tree can need
+    // host SBVH in its BVH2 form (host builder only), for the reference-order traversal count
+    std::vector<ksa::GpuNode> host_nodes2;
+    std::vector<ksa::GpuTri> host_tris;
 };
 
 struct ks_film {
@@ -1370,11 +1373,30 @@ struct WavefrontOpts {
     bool batch = false;
     bool graph = false;
     int blocks_per_cu = kMaxTraceBlocksPerCu;
+    // ray capture (eager launches only): every closest-hit and shadow ray of the run, appended as
+    // (o, tmin), (d, tmax) pairs to cap_c / cap_s up to cap_max each; cap_n[0], cap_n[1] count them
+    float4 *cap_c = nullptr, *cap_s = nullptr;
+    uint32_t *cap_n = nullptr;
+    uint32_t cap_max = 0;
 };
 
 // Runs the wavefront loop over `total` work items (mapped by rc), leaving per-sample results in ws.samp.
 // With opts.graph, two iterations (buffer 0 -> 1 and 1 -> 0) are captured once into a HIP graph and the
 // graph is replayed; the kernels read their counts from the control block, so every replay is valid.
+// Ray capture: append the n rays of a queue after the *count already captured, up to cap rays.
+__global__ __launch_bounds__(256) void k_capture(const uint32_t *n, const float4 *ro, const float4 *rd, float4 *out,
+                                                 const uint32_t *count, uint32_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t base = *count;
+    if (i < *n && base + (uint64_t)i < cap) {
+        out[2 * (size_t)(base + i)] = ro[i];
+        out[2 * (size_t)(base + i) + 1] = rd[i];
+    }
+}
+__global__ void k_capture_done(const uint32_t *n, uint32_t *count, uint32_t cap) {
+    *count = (uint32_t)min((uint64_t)cap, (uint64_t)*count + *n);
+}
+
 int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Timer &tm, ks_render_stats *stats,
                   const GuidePass *gp = nullptr, const WavefrontOpts &wo = WavefrontOpts{}) {
     ks_device *dev = sc->dev;
@@ -1398,6 +1420,12 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         else hipLaunchKernelGGL(k_raygen<false>, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
         hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q);
         tm.end(e);
+        if (wo.cap_n) {
+            hipLaunchKernelGGL(k_capture, dim3(tblocks), dim3(256), 0, s, (const uint32_t *)&ctrl->n_active,
+                               (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, wo.cap_c, wo.cap_n, wo.cap_max);
+            hipLaunchKernelGGL(k_capture_done, dim3(1), dim3(1), 0, s, (const uint32_t *)&ctrl->n_active, wo.cap_n,
+                               wo.cap_max);
+        }
         e = graph ? nullptr : tm.begin(KS_K_CLOSEST);
         launch_closest(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, 0u,
                        (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, ws.hit.as<float4>(),
@@ -1415,6 +1443,13 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                      ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(), ws.sh_tgt.as<uint32_t>(),
                      ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, gp ? gp->shade : GuideShade{});
         tm.end(e);
+        if (wo.cap_n) {
+            hipLaunchKernelGGL(k_capture, dim3(tblocks), dim3(256), 0, s, (const uint32_t *)&ctrl->n_shadow,
+                               (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), wo.cap_s,
+                               wo.cap_n + 1, wo.cap_max);
+            hipLaunchKernelGGL(k_capture_done, dim3(1), dim3(1), 0, s, (const uint32_t *)&ctrl->n_shadow, wo.cap_n + 1,
+                               wo.cap_max);
+        }
         e = graph ? nullptr : tm.begin(KS_K_SHADOW);
         launch_anyhit<1>(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_shadow, 0u,
                          (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), (uint8_t *)nullptr,
@@ -2272,6 +2307,10 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
             in.bound_hi[k] = scene_hi[k];
         }
         in.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (!device_build) {
+            sc->host_nodes2 = std::move(bvh.nodes);
+            sc->host_tris = std::move(bvh.tris);
+        }
         *out = sc.release();
         return KS_OK;
     });
@@ -2842,6 +2881,75 @@ int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_
             out[4 * i + 2] = res[i].z;
             out[4 * i + 3] = res[i].w;
         }
+        return KS_OK;
+    });
+}
+
+int ks_trace_capture(ks_scene *scene, const ks_camera_desc *cam, const ks_small_pt_params *params, int64_t n,
+                     const int32_t *px, const int32_t *py, const int32_t *sample, int64_t max_rays, float *closest_rays,
+                     int64_t *n_closest, float *shadow_rays, int64_t *n_shadow) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && cam && params && px && py && sample && closest_rays && n_closest && shadow_rays && n_shadow,
+                   "null argument");
+        KS_REQUIRE(n > 0 && n < (1ll << 31), "sample count out of range");
+        KS_REQUIRE(max_rays > 0 && max_rays <= (1ll << 28), "max_rays must be in [1, 2^28]");
+        ks_device *dev = scene->dev;
+        KS_HIP(hipSetDevice(dev->ordinal));
+        RenderConsts rc;
+        int e = fill_consts(scene->dev, scene, cam, params, rc);
+        if (e) return e;
+        std::vector<int4> wl((size_t)n);
+        for (int64_t i = 0; i < n; ++i) {
+            KS_REQUIRE(px[i] >= 0 && px[i] < rc.W && py[i] >= 0 && py[i] < rc.H && sample[i] >= 0 && sample[i] < rc.spp,
+                       "pixel or sample index out of range");
+            wl[i] = make_int4(px[i], py[i], sample[i], 0);
+        }
+        HostToDev<int4> dwl;
+        KS_HIP(dwl.put(wl.data(), wl.size(), dev->stream));
+        rc.work_list = dwl.p();
+        rc.s0 = 0;
+        rc.nb = 1;
+        uint32_t Q = pick_queue(params, (uint64_t)n);
+        e = ensure_workspace(dev, Q, (size_t)n, scene->max_stack);
+        if (e) return e;
+        DevBuf cc, cs, cn;
+        KS_HIP(cc.alloc(sizeof(float4) * 2 * (size_t)max_rays));
+        KS_HIP(cs.alloc(sizeof(float4) * 2 * (size_t)max_rays));
+        KS_HIP(cn.alloc(2 * sizeof(uint32_t)));
+        KS_HIP(hipMemsetAsync(cn.p, 0, 2 * sizeof(uint32_t), dev->stream));
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                           (int)kStatCount);
+        WavefrontOpts wo;
+        wo.cap_c = cc.as<float4>();
+        wo.cap_s = cs.as<float4>();
+        wo.cap_n = cn.as<uint32_t>();
+        wo.cap_max = (uint32_t)max_rays;
+        Timer tm{dev};
+        e = run_wavefront(scene, rc, (uint64_t)n, Q, tm, nullptr, nullptr, wo);
+        if (e) return e;
+        tm.collect(nullptr);
+        uint32_t counts[2];
+        KS_HIP(hipMemcpyAsync(counts, cn.p, sizeof(counts), hipMemcpyDeviceToHost, dev->stream));
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        KS_HIP(hipMemcpyAsync(closest_rays, cc.p, sizeof(float4) * 2 * counts[0], hipMemcpyDeviceToHost, dev->stream));
+        KS_HIP(hipMemcpyAsync(shadow_rays, cs.p, sizeof(float4) * 2 * counts[1], hipMemcpyDeviceToHost, dev->stream));
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        *n_closest = counts[0];
+        *n_shadow = counts[1];
+        return KS_OK;
+    });
+}
+
+int ks_scene_traversal_counts(const ks_scene *scene, const float *rays, int64_t n, int32_t any_hit,
+                              uint64_t *node_fetches, uint64_t *tri_tests, float *t_out) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(scene && rays && node_fetches && tri_tests && n >= 0, "null argument");
+        if (scene->host_nodes2.empty()) {
+            ksa::set_error("traversal counts need the host SBVH (ks_bvh_options.builder = KS_BVH_BUILD_HOST_SBVH)");
+            return KS_ERR_UNSUPPORTED;
+        }
+        ksa::count_reference_traversal(scene->host_nodes2, scene->host_tris, scene->info.bound_lo, scene->info.bound_hi,
+                                       rays, n, any_hit != 0, *node_fetches, *tri_tests, t_out);
         return KS_OK;
     });
 }

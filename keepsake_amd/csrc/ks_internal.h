@@ -15,7 +15,8 @@ namespace ksa {
 //   f[0..3]   = c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y
 //   f[4..7]   = c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y
 //   f[8..11]  = c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z
-//   child[2]  = child pointers (see kLeafBit), pad[2]
+//   child[2]  = child pointers (see kLeafBit), pad[2]; the host SBVH builder keeps the node's split axis
+//               in pad[0] (read by the reference-order traversal count, not by the kernels)
 // A child pointer without kLeafBit is the index of an interior node; with kLeafBit it is a leaf:
 // bits 4..30 = first triangle reference, bits 0..3 = count - 1 (1..16 references).
 struct alignas(16) GpuNode {
@@ -59,6 +60,12 @@ struct BvhResult {
 // SAH binned object + spatial split builder (restates cuda/sbvh.cu:301-440) followed by a flatten
 // into GpuNode/GpuTri arrays.
 BvhResult build_sbvh(const float *vertices, const uint32_t *indices, uint32_t n_tris, const ks_bvh_options &opt);
+// Node fetches and triangle tests of the reference SBVH traversal order (cuda/sbvh.cu:512-603) over a
+// host-built BVH2 for rays (n x 8: o, tmin, d, tmax); t_out (optional) gets each closest hit's distance
+// (-1: miss) or the occlusion result (1 / -1).
+void count_reference_traversal(const std::vector<GpuNode> &nodes, const std::vector<GpuTri> &tris, const float lo[3],
+                               const float hi[3], const float *rays, int64_t n, bool any_hit, uint64_t &n_nodes,
+                               uint64_t &n_tris, float *t_out);
 // Collapses a flattened BVH2 into BVH4 nodes (leaves and references unchanged): each node repeatedly
 // opens its interior child of largest surface area until it holds four children.
 std::vector<GpuNode4> collapse_bvh4(const std::vector<GpuNode> &nodes, uint32_t &max_depth4);

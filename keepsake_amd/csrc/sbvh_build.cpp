@@ -444,7 +444,8 @@ struct Flattener {
         put_box(n, 1, b.nodes[bn.child[1]].bound);
         n.child[0] = c0;
         n.child[1] = c1;
-        n.pad[0] = n.pad[1] = 0;
+        n.pad[0] = bn.axis; // SBVHNode::splitAxis (cuda/sbvh.cuh:22), for the reference-order traversal count
+        n.pad[1] = 0;
         return ni;
     }
 };
