@@ -77,8 +77,12 @@ struct DevBuf {
         release();
         if (n == 0) n = 16;
         hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        else p = nullptr;
+        if (e == hipSuccess) {
+            bytes = n;
+        } else {
+            p = nullptr;
+            (void)hipGetLastError(); // the failure is returned here; later launch checks must not see it again
+        }
         return e;
     }
     hipError_t ensure(size_t n) { return n <= bytes ? hipSuccess : alloc(n); }

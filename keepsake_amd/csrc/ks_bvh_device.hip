@@ -479,7 +479,10 @@ struct Scratch {
     template <typename T> hipError_t get(T *&p, size_t count) {
         void *v = nullptr;
         hipError_t e = hipMalloc(&v, std::max<size_t>(count * sizeof(T), 16));
-        if (e != hipSuccess) return e;
+        if (e != hipSuccess) {
+            (void)hipGetLastError(); // reported through the return value only
+            return e;
+        }
         ptrs.push_back(v);
         p = reinterpret_cast<T *>(v);
         return hipSuccess;
@@ -499,6 +502,7 @@ int build_bvh_device(const void *verts, const void *tri_idx, uint32_t n, const u
     do {                                                                                                       \
         hipError_t e_ = (call);                                                                                \
         if (e_ != hipSuccess) {                                                                                \
+            (void)hipGetLastError();                                                                           \
             err = std::string("device BVH build: ") + #call + ": " + hipGetErrorString(e_);                   \
             return -1;                                                                                         \
         }                                                                                                      \

@@ -868,8 +868,12 @@ struct DevBuf {
     hipError_t alloc(size_t n) {
         release();
         hipError_t e = hipMalloc(&p, std::max<size_t>(n, 16));
-        if (e == hipSuccess) bytes = n;
-        else p = nullptr;
+        if (e == hipSuccess) {
+            bytes = n;
+        } else {
+            p = nullptr;
+            (void)hipGetLastError(); // reported through the return value only
+        }
         return e;
     }
     hipError_t ensure(size_t n) { return n <= bytes ? hipSuccess : alloc(n); }
