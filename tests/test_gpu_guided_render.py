@@ -70,8 +70,14 @@ def _block_z(sums, ref_rgb, ref_w, skip):
 # Bonferroni over the 216 block-channels at 1%: two-sided normal quantile of 1 - 0.01 / 432 = 4.07. Blocks of
 # 128 pixels keep the per-pass block means close to normal (per-pass values of single pixels are heavy-tailed).
 Z_MAX = 4.1
-# |z| > 3 has probability 0.0027 per block-channel under no bias: 0.58 expected, P(>= 5) = 0.002
-Z3_MAX = 4
+# Blocks with some channel beyond |z| = 3: the three channels of a block share their paths, so they are counted
+# per block. Each of the 72 independent blocks has one with probability at most 3 x 0.0027 = 0.0081 under no
+# bias: 0.58 expected, P(>= 4) = 0.0025.
+Z3_BLOCKS_MAX = 3
+
+
+def _blocks_over3(z):
+    return int((np.abs(z) > 3).any(axis=-1).sum())
 
 
 @pytest.mark.parametrize("fp8,max_bounce", [(0, 0), (1, 0), (0, 1)])
@@ -102,10 +108,10 @@ def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
     ks.small_pt_render(sc, hs.camera, _params(ks, P, rng_seed=11), plain,
                        progress=lambda f, s: psums.append(f.download()[0].copy()))
     zp = _block_z(psums, ref_rgb, ref_w, 16)
-    print("guided max |z| over", z.size, "block-channels:", float(np.abs(z).max()), "count |z| > 3:",
-          int((np.abs(z) > 3).sum()), "| plain control:", float(np.abs(zp).max()), int((np.abs(zp) > 3).sum()), flush=True)
-    assert np.abs(zp).max() < Z_MAX and (np.abs(zp) > 3).sum() <= Z3_MAX  # the statistic itself is calibrated
-    assert np.abs(z).max() < Z_MAX and (np.abs(z) > 3).sum() <= Z3_MAX
+    print("guided max |z| over", z.size, "block-channels:", float(np.abs(z).max()), "blocks beyond 3:",
+          _blocks_over3(z), "| plain control:", float(np.abs(zp).max()), _blocks_over3(zp), flush=True)
+    assert np.abs(zp).max() < Z_MAX and _blocks_over3(zp) <= Z3_BLOCKS_MAX  # the statistic itself is calibrated
+    assert np.abs(z).max() < Z_MAX and _blocks_over3(z) <= Z3_BLOCKS_MAX
     per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)
     img_mean = per_pass[16:].mean(axis=(1, 2, 3))
     ref_img = ref_rgb / np.maximum(ref_w[..., :1] + ref_w[..., 1:], 1)
