@@ -301,6 +301,13 @@ int ks_film_free(ks_film *film);
 /* relMSE of two films' composites: mean over pixels and channels of (x - r)^2 / (r^2 + 0.01), in FP64 on
  * the device (the error metric of the guided-vs-plain comparison; no reference counterpart) */
 int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse);
+/* ks_film_relmse without any allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory, such as tight progressive loops that check convergence every pass or code
+ * around HIP graph capture. The partial sums live in a static device array of 2049 doubles (16 KiB) that
+ * each GPU holds for the life of the process; every call on one GPU shares it, so calls that overlap on
+ * one GPU race (as they would on the device's stream and pinned host block). Films above 2^32 pixels
+ * are rejected (KS_ERR_INVALID). The result equals ks_film_relmse's up to FP64 summation order. */
+int ks_film_relmse_noalloc(const ks_film *film, const ks_film *reference, double *relmse);
 
 /* ---- the render call: SmallPT::run (small_pt.cpp:15-70). Accumulates into `film` (which must be
  *      render_width x render_height); calls progress after each spp batch of spp_prog_interval on the

@@ -389,6 +389,7 @@ _SIGS = {
     "ks_film_composite": ([_P, _P], C.c_int),
     "ks_film_free": ([_P], C.c_int),
     "ks_film_relmse": ([_P, _P, C.POINTER(C.c_double)], C.c_int),
+    "ks_film_relmse_noalloc": ([_P, _P, C.POINTER(C.c_double)], C.c_int),
     "ks_small_pt_render": (
         [_P, C.POINTER(CameraDesc), C.POINTER(SmallPTParams), _P, PROGRESS_FN, _P, C.POINTER(RenderStats)],
         C.c_int,
@@ -891,6 +892,12 @@ class Film:
         out = np.zeros((self.height, self.width, 3), np.float32)
         _check(self.lib.ks_film_composite(self.h, _ptr(out)))
         return out
+
+    def relmse_noalloc(self, reference: "Film") -> float:
+        """relmse() through the allocation-free entry point (static device scratch)."""
+        v = C.c_double()
+        _check(self.lib.ks_film_relmse_noalloc(self.h, reference.h, C.byref(v)))
+        return v.value
 
     def relmse(self, reference: "Film") -> float:
         """mean over pixels and channels of (x - r)^2 / (r^2 + 0.01) of the composites, FP64 on the device"""

@@ -875,6 +875,50 @@ __global__ __launch_bounds__(256) void k_film_relmse(size_t n, const double *rgb
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
+// Scratch of the allocation-free relMSE (ks_film_relmse_noalloc): the partial sums of at most
+// kRelmseScratchBlocks blocks, then their total. Each device holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress returns on that device
+// (ks_device::relmse_scratch), never through this name.
+constexpr unsigned kRelmseScratchBlocks = 2048;
+static __device__ __align__(256) double scratch[kRelmseScratchBlocks + 1];
+
+// k_film_relmse over a fixed grid: block b sums pixels b * 256 + t + k * gridDim.x * 256.
+__global__ __launch_bounds__(256) void k_film_relmse_strided(size_t n, const double *rgb, const double *wts, float3 bd,
+                                                             const double *rrgb, const double *rwts, float3 rbd,
+                                                             double *partials) {
+    __shared__ double red[256];
+    double v = 0.0;
+    const float b[3] = {bd.x, bd.y, bd.z}, rb[3] = {rbd.x, rbd.y, rbd.z};
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
+        for (int k = 0; k < 3; ++k) {
+            double x = composite_ch(rgb, wts, p, k, b[k]);
+            double r = composite_ch(rrgb, rwts, p, k, rb[k]);
+            v += (x - r) * (x - r) / (r * r + 0.01);
+        }
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// The total of n partial sums, in one block, written to *total.
+__global__ __launch_bounds__(256) void k_sum_partials(const double *partials, unsigned n, double *total) {
+    __shared__ double red[256];
+    double v = 0.0;
+    for (unsigned i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = red[0];
+}
+
 __global__ void k_sobol_stream(int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample, int32_t seed,
                                int32_t log2spp, int32_t ndigits, int32_t n_calls, const int32_t *kinds,
                                int32_t n_values, uint32_t *out_u32, float *out_f) {
@@ -1127,6 +1171,7 @@ struct ks_device {
     int tune_bpc = 0;
     Ctrl *host_ctrl = nullptr; // pinned
     uint8_t *host_scratch = nullptr; // pinned, 64 bytes
+    double *relmse_scratch = nullptr; // this device's address of `scratch` (hipGetSymbolAddress at open)
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
     // pixel-filter sample tables by (type, radius, sigma), kept for the device's lifetime: batched
     // renders keep pointers to them in their per-view constants
@@ -1610,6 +1655,10 @@ int ks_device_open(int32_t ordinal, ks_device **out) {
         d->n_cu = prop.multiProcessorCount;
         KS_HIP(hipHostMalloc((void **)&d->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
         KS_HIP(hipHostMalloc((void **)&d->host_scratch, 64, hipHostMallocDefault));
+        // the allocation-free relMSE's scratch: its address on this device (the current one)
+        void *sp = nullptr;
+        KS_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+        d->relmse_scratch = static_cast<double *>(sp);
         *out = d.release();
         return KS_OK;
     });
@@ -2521,6 +2570,37 @@ int ks_film_relmse(const ks_film *film, const ks_film *reference, double *relmse
         KS_HIP(hipStreamSynchronize(dev->stream));
         double sum = 0.0;
         for (double v : h) sum += v;
+        *relmse = sum / (3.0 * (double)n);
+        return KS_OK;
+    });
+}
+
+int ks_film_relmse_noalloc(const ks_film *film, const ks_film *reference, double *relmse) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(film && reference && relmse, "null argument");
+        KS_REQUIRE(film->w == reference->w && film->h == reference->h, "films differ in size");
+        KS_REQUIRE(film->dev == reference->dev, "films belong to different devices");
+        ks_device *dev = film->dev;
+        KS_REQUIRE(dev->relmse_scratch, "the device has no relMSE scratch (device not opened by ks_device_open)");
+        const size_t n = (size_t)film->w * film->h;
+        KS_REQUIRE(n <= ((size_t)1 << 32), "ks_film_relmse_noalloc: films above 2^32 pixels are not supported");
+        KS_HIP(hipSetDevice(dev->ordinal));
+        // a fixed grid of at most kRelmseScratchBlocks blocks: the partials always fit the scratch, and
+        // every one of them is written before it is summed (nothing to clear between calls)
+        const unsigned blocks = (unsigned)std::min<size_t>(kRelmseScratchBlocks, (n + 255) / 256);
+        double *partials = dev->relmse_scratch, *total = dev->relmse_scratch + kRelmseScratchBlocks;
+        hipLaunchKernelGGL(k_film_relmse_strided, dim3(blocks), dim3(256), 0, dev->stream, n, film->rgb.as<double>(),
+                           film->wts.as<double>(), make_float3(film->backdrop[0], film->backdrop[1], film->backdrop[2]),
+                           reference->rgb.as<double>(), reference->wts.as<double>(),
+                           make_float3(reference->backdrop[0], reference->backdrop[1], reference->backdrop[2]),
+                           partials);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, dev->stream, (const double *)partials, blocks, total);
+        KS_HIP(hipGetLastError());
+        // through the device's pinned host block, allocated when the device was opened
+        KS_HIP(hipMemcpyAsync(dev->host_scratch, total, sizeof(double), hipMemcpyDeviceToHost, dev->stream));
+        KS_HIP(hipStreamSynchronize(dev->stream));
+        double sum;
+        std::memcpy(&sum, dev->host_scratch, sizeof(double));
         *relmse = sum / (3.0 * (double)n);
         return KS_OK;
     });
