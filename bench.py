@@ -200,6 +200,61 @@ def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, ma
     return res
 
 
+def guide_ceiling(ks, dev, args) -> dict:
+    """What a trained guide can gain on the room at equal spp: a guide trained for 64 spp (Adam steps of
+    2^18 samples), then frozen, renders 256 spp at guide fraction 0.5, guiding the first 1 / 2 / all
+    bounces; plain renders the same spp. relMSE is measured against a 4096-spp FP64 plain reference, whose
+    own noise (v / 4096 from the plain run) is subtracted. The ceiling speed-up at equal error is the
+    variance ratio times the ratio of time per sample."""
+    hs = ks.procedural_scene("room", 3)
+    sc = ks.Scene(dev, hs)
+    info = sc.info()
+    W, H = args.room_width, args.room_height
+    cam = dict(hs.camera)
+    cam["aspect"] = W / H
+
+    def prm(spp, seed=0):
+        return ks.small_pt_params(bounces=16, render_width=W, render_height=H, spp=spp, spp_prog_interval=spp,
+                                  mis_mode=ks.KS_MIS_CONSISTENT, clamp_direct=0.0, clamp_indirect=0.0, rng_seed=seed)
+
+    ref = ks.Film(dev, W, H)
+    ks.small_pt_render(sc, cam, prm(4096), ref)
+    n = 256
+    film = ks.Film(dev, W, H)
+    t0 = time.perf_counter()
+    ks.small_pt_render(sc, cam, prm(n, 1), film)
+    t_plain = time.perf_counter() - t0
+    r_plain = film.relmse(ref)
+    c = r_plain * n / 4096 / (1 + n / 4096)  # the reference's share: v/4096 with v from the plain run
+    out = {"spp": n, "reference_spp": 4096, "reference_noise": c, "plain": {"relmse": r_plain, "seconds": t_plain}}
+    for mb in (1, 2, 0):
+        g = ks.Guide(dev, info["bound_lo"], info["bound_hi"],
+                     ks.guide_config(seed=1, guide_fraction=0.5, train_spp=64, max_steps_per_pass=8, max_guided_bounce=mb))
+        tr = ks.Film(dev, W, H)
+        t0 = time.perf_counter()
+        ks.small_pt_render_guided(sc, cam, prm(64, 2), g, tr)
+        t_train = time.perf_counter() - t0
+        st = g.stats()
+        g.set_mode(0.5, 0)  # frozen
+        film.clear()
+        t0 = time.perf_counter()
+        ks.small_pt_render_guided(sc, cam, prm(n, 1), g, film)
+        t_g = time.perf_counter() - t0
+        r_g = film.relmse(ref)
+        vr = (r_plain - c) / max(r_g - c, 1e-12)
+        out[f"guided_max_bounce_{mb or 'all'}"] = {
+            "relmse": r_g, "seconds": t_g, "training_seconds": t_train, "adam_steps": st["steps"],
+            "training_samples": st["samples"], "variance_ratio": round(vr, 3),
+            "time_per_spp_ratio": round(t_g / t_plain, 3), "ceiling_speedup": round(vr * t_plain / t_g, 3)}
+        tr.close()
+        g.close()
+    film.close()
+    ref.close()
+    sc.close()
+    log(f"guide ceiling: {out}")
+    return out
+
+
 def batched_views(ks, dev, args) -> dict:
     """48 views of the Cornell box at seed-drawn resolutions and spp (12 bounces) in one batched
     wavefront, against the same views rendered one at a time."""
@@ -255,6 +310,8 @@ def main() -> int:
     ap.add_argument("--no-device-bvh", action="store_true",
                     help="skip the leg that builds the scene's BVH on the device and renders over it")
     ap.add_argument("--scale-builder", default="device", choices=("host", "device"))
+    ap.add_argument("--sorted-shading", action="store_true",
+                    help="shade each material class with its own kernel instead of one kernel over every path")
     ap.add_argument("--count-paths", type=int, default=1 << 16,
                     help="random paths whose rays are counted in the reference traversal order (0: skip)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU band (0: sized for --cpu-seconds)")
@@ -276,6 +333,8 @@ def main() -> int:
     ap.add_argument("--guide-max-bounce", type=int, default=1,
                     help="guide only vertices below this bounce (1: the camera rays' hits; 0: every vertex)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 48-view leg")
+    ap.add_argument("--guide-ceiling", action="store_true",
+                    help="also measure what a long-trained, frozen guide gains at equal spp on the room")
     ap.add_argument("--fp8-relmse", action="store_true",
                     help="also time the guided tracer with FP8 inference in the room's time-to-relMSE leg")
     ap.add_argument("--no-scale-leg", action="store_true",
@@ -337,6 +396,7 @@ def main() -> int:
     gen_s = time.time() - t0
     log(f"scene {args.scene}: {hs.n_tris} triangles (generated in {gen_s:.1f} s)")
     dev = ks.Device(0)
+    dev.set_shading(args.sorted_shading)
     t0 = time.time()
     builder = BUILDERS[args.builder]
     sc = ks.Scene(dev, hs, bvh={"branching": args.branching, "builder": builder})
@@ -504,6 +564,7 @@ def main() -> int:
             ttr = time_to_relmse(ks, dev, args, methods=methods)
         except ks.KsError as e:  # reported, not hidden: the plain-throughput line stays valid
             ttr = {"error": str(e)}
+    ceiling = guide_ceiling(ks, dev, args) if args.guide_ceiling else None
     scale = None
     if not args.no_scale_leg:
         try:
@@ -552,6 +613,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
         "time_to_relmse_at_scale": scale,
+        "guide_ceiling": ceiling,
         "batched_views": batch,
         "device_bvh": device_bvh,
     }

@@ -268,6 +268,12 @@ int ks_device_open(int32_t ordinal, ks_device **out);
 int ks_device_close(ks_device *dev);
 /* Record HIP-event kernel timings into ks_render_stats (costs one event pair per launch). */
 int ks_device_set_timing(ks_device *dev, int32_t enable);
+/* Shading order of the wavefront (no counterpart in the reference, whose CPU loop shades one path at a time,
+ * Material::sample_with_nee, material.cpp:101-167): 0 (default) shades every path with one kernel; 1 sorts
+ * each bounce's paths by material class (escaped rays, opaque dielectrics, metals, other materials) and
+ * shades each class with a kernel compiled for its BSDF lobes. Results are the same either way; textured
+ * scenes always use one kernel. */
+int ks_device_set_shading(ks_device *dev, int32_t sorted);
 
 /* ---- scene: replaces create_scene + Scene::create_rtc_scene (rtcCommitScene, scene.cpp:144-160),
  *      Scene::build_mesh_lights (scene.cpp:315-336) and PowerLightSampler::build (light.cpp:636-677).

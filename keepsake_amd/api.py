@@ -378,6 +378,7 @@ _SIGS = {
     "ks_device_open": ([C.c_int32, C.POINTER(_P)], C.c_int),
     "ks_device_close": ([_P], C.c_int),
     "ks_device_set_timing": ([_P, C.c_int32], C.c_int),
+    "ks_device_set_shading": ([_P, C.c_int32], C.c_int),
     "ks_scene_create": ([_P, C.POINTER(SceneDesc), C.POINTER(BvhOptions), C.POINTER(_P)], C.c_int),
     "ks_scene_free": ([_P], C.c_int),
     "ks_scene_info_get": ([_P, C.POINTER(SceneInfo)], C.c_int),
@@ -698,6 +699,10 @@ class Device:
 
     def set_timing(self, enable: bool) -> None:
         _check(self.lib.ks_device_set_timing(self.h, int(enable)))
+
+    def set_shading(self, sorted: bool) -> None:
+        """Material-sorted shading, or one shading kernel over every path (the default)."""
+        _check(self.lib.ks_device_set_shading(self.h, int(sorted)))
 
     def close(self) -> None:
         if self.h:

@@ -273,6 +273,7 @@ __global__ void k_gen_done(Ctrl *c, uint32_t Q) {
     c->work_next += n_gen;
     c->n_next = 0;
     c->n_shadow = 0;
+    for (int k = 0; k < kShadeClasses; ++k) c->n_class[k] = 0;
 }
 
 __global__ void k_iter_done(Ctrl *c) {
@@ -480,6 +481,27 @@ KS_DEV float mesh_light_pdf(const DevScene &sc, const Light &l, f3 wi, float wi_
     return (1.0f / l.area) * (sqr(wi_dist) / absdot);
 }
 
+// Material sort of the current buffer: each path's slot goes to the list of its class (a ray that left
+// the scene, or the class of the material it hit), Q entries per class, in the order of the buffer
+// within each wave.
+__global__ __launch_bounds__(256) void k_classify(DevScene sc, const float4 *hits, Ctrl *ctrl, uint32_t *order,
+                                                  uint32_t Q) {
+    __shared__ uint32_t s_app[5];
+    const uint32_t n = ctrl->n_active;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n) return;
+    int cls = -1;
+    if (i < n) {
+        const int32_t prim = __float_as_int(hits[i].w);
+        cls = prim < 0 ? kShadeMiss : (int)sc.mat_class[tri_material(sc.tri_idx[prim])];
+    }
+#pragma unroll
+    for (int k = 0; k < kShadeClasses; ++k) {
+        const uint32_t j = block_append<4>(&ctrl->n_class[k], cls == k, s_app);
+        if (cls == k) order[(size_t)k * Q + j] = i;
+    }
+}
+
 // One bounce of SmallPT::trace (small_pt.cpp:72-178) for every active path.
 // Mixture pdf at w from the 10-float4 record of one path (sum of w_k vmf_k(w), vmf.h:32-39).
 KS_DEV float mix_pdf(const float4 *m, f3 w) {
@@ -501,15 +523,26 @@ KS_DEV float mix_pdf(const float4 *m, f3 w) {
 // MIS weights), and training records of the first kRecVertices vertices of each path.
 // TEX evaluates hits of textured scenes: texture coordinates and uv-based frames, uv partials of camera
 // rays, normal maps, closures and emission from shader fields.
-template <bool GUIDED, bool BATCH, bool TEX>
+// CLASS (material-sorted shading): the kernel shades the slots of one class's list in `order` (Q entries
+// per class) and is compiled for that class only: kShadeMiss holds only the escaped-ray code, the
+// dielectric and metal classes the BSDF lobes their materials can weight. kShadeUnsorted shades the
+// whole buffer with every lobe.
+template <bool GUIDED, bool BATCH, bool TEX, int CLASS = kShadeUnsorted>
 __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, PathBufs cur, PathBufs nxt,
                                                const float4 *hits, Ctrl *ctrl, float4 *sh_o, float4 *sh_d,
                                                uint32_t *sh_tgt, float4 *sh_c, float4 *samp,
-                                               unsigned long long *stats, GuideShade gs) {
-    const uint32_t n = ctrl->n_active;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+                                               unsigned long long *stats, GuideShade gs, const uint32_t *order,
+                                               uint32_t Q) {
+    constexpr unsigned LB = CLASS == kShadeDielectric ? kLobesOpaqueDielectric
+                            : CLASS == kShadeMetal     ? kLobesMetal
+                                                       : kLobesAll;
+    const uint32_t n = CLASS == kShadeUnsorted ? ctrl->n_active : ctrl->n_class[CLASS];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n) return; // whole block idle
-    const bool active = i < n;
+    const bool active = t < n;
+    const uint32_t i = CLASS == kShadeUnsorted ? t : (active ? order[(size_t)CLASS * Q + t] : 0u);
+    (void)Q;
+    (void)order;
 
     bool survive = false, shadow = false;
     uint32_t w = 0, state = 0;
@@ -545,6 +578,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         bool miss_sample = false;
 
         if (prim < 0) {
+          if constexpr (CLASS == kShadeUnsorted || CLASS == kShadeMiss) {
             f3 wi = normalized(rd);
             for (int k = 0; k < sc.n_sky; ++k) { // small_pt.cpp:85-109
                 const Light &l = sc.lights[sc.sky[k]];
@@ -569,8 +603,10 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                 }
             }
             if (bounce == 0 && !vr.include_background) miss_sample = true;
+          }
             done = true;
         } else {
+          if constexpr (CLASS != kShadeMiss) {
             uint4 tri = sc.tri_idx[prim];
             const uint32_t mid = tri_material(tri);
             const Material &m = sc.mats[mid];
@@ -653,8 +689,8 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                     f3 Le_beta = light_sample<TEX>(sc, l, it.p, u0, u1, wi, wi_dist, pdf_l);
                     if (!is_zero(Le_beta) && pdf_l > 0.0f) {
                         f3 wol = sh_to_local(it, wo), wil = sh_to_local(it, wi);
-                        f3 f = bsdf_eval(wol, wil, c);
-                        float pdf_b = is_zero(f) ? 0.0f : bsdf_pdf(wol, wil, c);
+                        f3 f = bsdf_eval<LB>(wol, wil, c);
+                        float pdf_b = is_zero(f) ? 0.0f : bsdf_pdf<LB>(wol, wil, c);
                         if (!is_zero(f) && pdf_b > 0.0f) {
                             if (GUIDED && alpha_m > 0.0f) pdf_b = alpha_m * mix_pdf(mixp, wi) + (1.0f - alpha_m) * pdf_b;
                             float mis = 1.0f;
@@ -701,11 +737,11 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                         lobe.kappa = l4.w;
                         wi_g = vmf_sample(lobe, ur, u1);
                         wil = sh_to_local(it, wi_g);
-                        fcos = bsdf_eval(wol, wil, c);
-                        pb = is_zero(fcos) ? 0.0f : bsdf_pdf(wol, wil, c);
+                        fcos = bsdf_eval<LB>(wol, wil, c);
+                        pb = is_zero(fcos) ? 0.0f : bsdf_pdf<LB>(wol, wil, c);
                     } else {
                         float ub = fminf((u_lobe - alpha_m) / (1.0f - alpha_m), kBeforeOne);
-                        f3 sb = bsdf_sample(wol, wil, c, ub, u0, u1, pb);
+                        f3 sb = bsdf_sample<LB>(wol, wil, c, ub, u0, u1, pb);
                         fcos = sb * pb;
                         wi_g = sh_to_world(it, wil);
                     }
@@ -713,7 +749,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                     pdf_wi = alpha_m * q + (1.0f - alpha_m) * pb;
                     sbeta = pdf_wi > 0.0f ? fcos / pdf_wi : mk(0, 0, 0);
                 } else {
-                    sbeta = bsdf_sample(wol, wil, c, u_lobe, u0, u1, pdf_wi);
+                    sbeta = bsdf_sample<LB>(wol, wil, c, u_lobe, u0, u1, pdf_wi);
                 }
                 if (!all_finite(sbeta)) ++nonfinite;
                 if (maxc(sbeta) == 0.0f || pdf_wi == 0.0f) {
@@ -747,12 +783,14 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                     }
                 }
             }
+          }
         }
         survive = !done;
         if (done) samp[BATCH ? w : samp_index(rc, w)] = make_float4(L.x, L.y, L.z, miss_sample ? 0.0f : 1.0f);
     }
 
-    uint32_t j = wave_append(&ctrl->n_next, survive);
+    __shared__ uint32_t s_app[5];
+    uint32_t j = block_append<4>(&ctrl->n_next, survive, s_app);
     if (survive) {
         nxt.work[j] = w;
         nxt.state[j] = (uint32_t)sob.dim | ((uint32_t)(bounce + 1) << 16);
@@ -761,7 +799,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         nxt.ro[j] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
         nxt.rd[j] = make_float4(new_d.x, new_d.y, new_d.z, kInf);
     }
-    uint32_t k = wave_append(&ctrl->n_shadow, shadow);
+    uint32_t k = block_append<4>(&ctrl->n_shadow, shadow, s_app);
     if (shadow) {
         sh_o[k] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.0f);
         sh_d[k] = make_float4(sh_dir.x, sh_dir.y, sh_dir.z, sh_tmax);
@@ -1143,6 +1181,7 @@ struct Workspace {
     uint32_t Q = 0;
     DevBuf work[2], state[2], beta[2], L[2], ro[2], rd[2];
     DevBuf hit, sh_o, sh_d, sh_tgt, sh_c;
+    DevBuf order; // material-sorted shading: kShadeClasses lists of Q slots
     DevBuf samp;
     DevBuf ctrl, stats;
     DevBuf ovf;
@@ -1152,7 +1191,7 @@ struct Workspace {
 // Buffers of guided renders: per-path mixtures of the current wavefront buffer, per-work-item training
 // records, the extracted training samples and their counter.
 struct GuideWork {
-    DevBuf mix, recs, train, count;
+    DevBuf mix, list, recs, train, count;
     uint32_t mix_q = 0, rec_items = 0;
     uint32_t stamp = 0; // last record stamp handed out on this device
 };
@@ -1172,6 +1211,7 @@ struct ks_device {
     Ctrl *host_ctrl = nullptr; // pinned
     uint8_t *host_scratch = nullptr; // pinned, 64 bytes
     double *relmse_scratch = nullptr; // this device's address of `scratch` (hipGetSymbolAddress at open)
+    int sorted_shading = 0;           // material-sorted shading of untextured scenes (ks_device_set_shading)
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
     // pixel-filter sample tables by (type, radius, sigma), kept for the device's lifetime: batched
     // renders keep pointers to them in their per-view constants
@@ -1186,7 +1226,7 @@ struct ks_device {
 
 struct ks_scene {
     ks_device *dev = nullptr;
-    DevBuf nodes, tris, verts, normals, tri_idx, tri_light, mats, lights, cdf;
+    DevBuf nodes, tris, verts, normals, tri_idx, tri_light, mats, mat_class, lights, cdf;
     DevBuf sky_rgb, sky_cond, sky_marg; // environment map and its sampling tables
     DevBuf tex_pool, tex_meta, tex_mats, tex_mat_of, uvs, tex_luts; // textures and textured materials
     DevScene ds{};
@@ -1289,9 +1329,24 @@ template <int MODE, typename... A> void launch_anyhit(const DevScene &ds, dim3 g
 }
 
 // Shading launch: guided (plain renders only), batched or plain, each with or without texture support.
-template <typename... A> void launch_shade(bool tex, bool guided, bool batch, dim3 g, hipStream_t s, A... args) {
+template <bool GUIDED, bool BATCH, typename... A> void launch_sorted(dim3 g, hipStream_t s, A... args) {
     const dim3 b(256);
-    if (guided) {
+    hipLaunchKernelGGL((k_shade<GUIDED, BATCH, false, kShadeMiss>), g, b, 0, s, args...);
+    hipLaunchKernelGGL((k_shade<GUIDED, BATCH, false, kShadeDielectric>), g, b, 0, s, args...);
+    hipLaunchKernelGGL((k_shade<GUIDED, BATCH, false, kShadeMetal>), g, b, 0, s, args...);
+    hipLaunchKernelGGL((k_shade<GUIDED, BATCH, false, kShadeGeneral>), g, b, 0, s, args...);
+}
+
+// Shading of the current buffer: material-sorted (one kernel per class over the lists k_classify wrote)
+// for untextured scenes, else one kernel over the whole buffer.
+template <typename... A> void launch_shade(bool tex, bool guided, bool batch, bool sorted, dim3 g, hipStream_t s,
+                                           A... args) {
+    const dim3 b(256);
+    if (sorted && !tex) {
+        if (guided) launch_sorted<true, false>(g, s, args...);
+        else if (batch) launch_sorted<false, true>(g, s, args...);
+        else launch_sorted<false, false>(g, s, args...);
+    } else if (guided) {
         if (tex) hipLaunchKernelGGL((k_shade<true, false, true>), g, b, 0, s, args...);
         else hipLaunchKernelGGL((k_shade<true, false, false>), g, b, 0, s, args...);
     } else if (batch) {
@@ -1328,6 +1383,7 @@ int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_sta
         KS_HIP(ws.sh_d.alloc(sizeof(float4) * Q));
         KS_HIP(ws.sh_tgt.alloc(sizeof(uint32_t) * Q));
         KS_HIP(ws.sh_c.alloc(sizeof(float4) * Q));
+        KS_HIP(ws.order.alloc(sizeof(uint32_t) * kShadeClasses * (size_t)Q));
         ws.Q = Q;
     }
     KS_HIP(ws.samp.ensure(sizeof(float4) * std::max<size_t>(n_samp, 1)));
@@ -1413,6 +1469,7 @@ int fill_consts(ks_device *dev, const ks_scene *sc, const ks_camera_desc *cam, c
 struct GuidePass {
     ks_guide *guide = nullptr;
     float4 *mix = nullptr;
+    uint32_t *list = nullptr, *n_list = nullptr; // compacted inference queries (Q slots) and their count
     GuideShade shade{};
 };
 
@@ -1483,14 +1540,19 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         if (gp && gp->shade.mix) {
             e = tm.begin(KS_K_GUIDE);
             int ge = ksg::launch_infer(gp->guide, sc->ds, &ctrl->n_active, Q, pb[cur].rd, ws.hit.as<float4>(),
-                                       pb[cur].state, gp->shade.max_bounce, gp->mix, s);
+                                       pb[cur].state, gp->shade.max_bounce, gp->mix, gp->list, gp->n_list, s);
             tm.end(e);
             if (ge) return ge;
         }
         e = graph ? nullptr : tm.begin(KS_K_SHADE);
-        launch_shade(sc->ds.textured != 0, gp != nullptr, wo.batch, dim3(tblocks), s, sc->ds, rc, pb[cur], pb[1 - cur],
-                     ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(), ws.sh_tgt.as<uint32_t>(),
-                     ws.sh_c.as<float4>(), ws.samp.as<float4>(), st, gp ? gp->shade : GuideShade{});
+        const bool sorted = dev->sorted_shading && !sc->ds.textured;
+        if (sorted)
+            hipLaunchKernelGGL(k_classify, dim3(tblocks), dim3(256), 0, s, sc->ds, (const float4 *)ws.hit.as<float4>(),
+                               ctrl, ws.order.as<uint32_t>(), Q);
+        launch_shade(sc->ds.textured != 0, gp != nullptr, wo.batch, sorted, dim3(tblocks), s, sc->ds, rc, pb[cur],
+                     pb[1 - cur], ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
+                     ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st,
+                     gp ? gp->shade : GuideShade{}, (const uint32_t *)ws.order.as<uint32_t>(), Q);
         tm.end(e);
         if (wo.cap_n) {
             hipLaunchKernelGGL(k_capture, dim3(tblocks), dim3(256), 0, s, (const uint32_t *)&ctrl->n_shadow,
@@ -1682,6 +1744,14 @@ int ks_device_set_timing(ks_device *dev, int32_t enable) {
     return ksa::guarded([&]() -> int {
         KS_REQUIRE(dev, "device is null");
         dev->timing = enable ? 1 : 0;
+        return KS_OK;
+    });
+}
+
+int ks_device_set_shading(ks_device *dev, int32_t sorted) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev, "device is null");
+        dev->sorted_shading = sorted ? 1 : 0;
         return KS_OK;
     });
 }
@@ -2125,6 +2195,19 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
                                         ((uint32_t)(d->geom_flags ? d->geom_flags[d->tri_geom[t]] & 7u : 0u) << 24));
         std::vector<Material> mats(d->n_materials);
         for (uint32_t m = 0; m < d->n_materials; ++m) mats[m] = to_device_material(material_of(m));
+        // material classes of sorted shading, from the clamped closure: metallic 1 weights only the metal
+        // lobe (every other lobe carries a factor 1 - metallic); metallic, specular_trans and
+        // diffuse_trans 0 leave the diffuse lobe and the dielectric reflection; textured materials are general
+        std::vector<uint8_t> mat_class(d->n_materials);
+        for (uint32_t m = 0; m < d->n_materials; ++m) {
+            const Material &dm = mats[m];
+            const bool textured = !ht.mat_of.empty() && ht.mat_of[m] >= 0;
+            mat_class[m] = textured ? kShadeGeneral
+                           : dm.metallic == 1.0f ? kShadeMetal
+                           : dm.metallic == 0.0f && dm.specular_trans == 0.0f && dm.diffuse_trans == 0.0f
+                               ? kShadeDielectric
+                               : kShadeGeneral;
+        }
 
         hipStream_t s = dev->stream;
         float scene_lo[3], scene_hi[3];
@@ -2293,6 +2376,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
         if (!device_build) KS_HIP(upload(sc->tri_idx, tri_idx, s));
         KS_HIP(upload(sc->tri_light, tri_light, s));
         KS_HIP(upload(sc->mats, mats, s));
+        KS_HIP(upload(sc->mat_class, mat_class, s));
         KS_HIP(upload(sc->lights, lights, s));
         KS_HIP(upload(sc->cdf, cdf, s));
         if (ht.textured) {
@@ -2324,6 +2408,7 @@ int ks_scene_create(ks_device *dev, const ks_scene_desc *d, const ks_bvh_options
         ds.tri_idx = sc->tri_idx.as<uint4>();
         ds.tri_light = sc->tri_light.as<int32_t>();
         ds.mats = sc->mats.as<Material>();
+        ds.mat_class = sc->mat_class.as<uint8_t>();
         ds.lights = sc->lights.as<Light>();
         ds.cdf = sc->cdf.as<float>();
         ds.n_lights = (int32_t)lights.size();
@@ -2636,6 +2721,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
         GuideWork &gw = dev->gw;
         if (gw.mix_q < Q) {
             KS_HIP(gw.mix.alloc(sizeof(float4) * 10 * (size_t)Q));
+            KS_HIP(gw.list.alloc(sizeof(uint32_t) * ((size_t)Q + 64)));
             gw.mix_q = Q;
         }
         const float fraction = ksg::guide_fraction(guide);
@@ -2666,6 +2752,8 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
             GuidePass gp;
             gp.guide = guide;
             gp.mix = gw.mix.as<float4>();
+            gp.list = gw.list.as<uint32_t>();
+            gp.n_list = gw.list.as<uint32_t>() + Q; // the count sits after the Q list entries
             // the guide samples once it has been trained at least once; before that the pass is BSDF-only
             const bool sample_guide = fraction > 0.0f && ksg::guide_steps(guide) > 0;
             gp.shade.mix = sample_guide ? gw.mix.as<float4>() : nullptr;
@@ -2702,8 +2790,13 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
                 std::memcpy(&c_total, dev->host_scratch + sizeof(double), sizeof(double));
                 const uint32_t n_train = std::min(n_found, cap);
                 const float c_scale = c_total > 0.0 ? (float)((double)n_found / c_total) : 1.0f;
-                // spread the samples of the pass over the steps: step k takes samples k, k + S, k + 2S, ...
-                const int steps = (int)std::min<uint64_t>((uint64_t)max_steps, n_train / 64u);
+                // Adam steps of `batch` samples (2^18 by default, the configuration's batch): as many whole
+                // batches as the pass holds, at most max_steps; a pass with fewer samples than one batch
+                // makes one step over all of them. The samples are spread over the steps: step k takes
+                // samples k, k + S, k + 2S, ...
+                const int steps = n_train < 64u ? 0
+                                                : (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)max_steps,
+                                                                                                n_train / (uint32_t)batch));
                 if (steps > 0) {
                     const int per_step = (int)std::min<uint64_t>((uint64_t)batch, n_train / (uint32_t)steps);
                     ev = tm.begin(KS_K_GUIDE);

@@ -424,6 +424,16 @@ KS_DEV f3 eval_dielectric_specular(f3 wo, f3 wi, const Closure &c) {
     return mk(f, f, f);
 }
 
+// Lobes of the principled BSDF a shading kernel is compiled for. Material-sorted shading instantiates
+// the BSDF for the lobes its material class can have non-zero weight on; every other lobe then
+// contributes exactly nothing, as its zero weight makes it contribute in the full BSDF.
+constexpr unsigned kLobeDiffuse = 1, kLobeDiffuseTrans = 2, kLobeMetal = 4, kLobeDielRefl = 8, kLobeDielTrans = 16;
+constexpr unsigned kLobesAll = 31;
+// opaque dielectrics (metallic 0, specular_trans 0, diffuse_trans 0) and pure metals (metallic 1)
+constexpr unsigned kLobesOpaqueDielectric = kLobeDiffuse | kLobeDielRefl;
+constexpr unsigned kLobesMetal = kLobeMetal;
+
+template <unsigned L = kLobesAll>
 KS_DEV void lobe_weights(f3 wo, const Closure &c, float w[4]) { // principled_bsdf.cpp:518-540
     float lum = luminance(c.basecolor);
     float wd = wo.z > 0.0f ? (1.0f - c.metallic) * (1.0f - c.specular_trans) * lum * kInvPi : 0.0f;
@@ -431,7 +441,10 @@ KS_DEV void lobe_weights(f3 wo, const Closure &c, float w[4]) { // principled_bs
     w[1] = wd * c.diffuse_trans;
     w[2] = wo.z > 0.0f ? c.metallic * lerpf(lum, 1.0f, fresnel_schlick(wo.z)) : 0.0f;
     float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
-    w[3] = (1.0f - c.metallic) * fresnel_dielectric(fabsf(wo.z), 1.0f / eta);
+    if constexpr ((L & (kLobeDielRefl | kLobeDielTrans)) != 0)
+        w[3] = (1.0f - c.metallic) * fresnel_dielectric(fabsf(wo.z), 1.0f / eta);
+    else
+        w[3] = 0.0f; // metallic 1: (1 - metallic) times a Fresnel term in [0, 1]
     float sum = w[0] + w[1] + w[2] + w[3];
     if (sum == 0.0f) {
         w[0] = w[1] = w[2] = w[3] = 0.0f;
@@ -484,25 +497,41 @@ KS_DEV bool dielectric_pdf_needed(f3 wo, f3 wi, const Closure &c, float w3) {
     return w3 != 0.0f;
 }
 
-KS_DEV f3 bsdf_eval(f3 wo, f3 wi, const Closure &c) {
+template <unsigned L = kLobesAll> KS_DEV f3 bsdf_eval(f3 wo, f3 wi, const Closure &c) {
     if (wo.z == 0.0f || wi.z == 0.0f) return mk(0, 0, 0);
     f3 f = mk(0, 0, 0);
-    if (wo.z > 0.0f && wi.z > 0.0f) f = f + eval_diffuse(wo, wi, c) + eval_metallic_specular(wo, wi, c);
-    if (wo.z > 0.0f && wi.z < 0.0f) f = f + eval_diffuse_transmission(wo, wi, c);
-    return f + eval_dielectric_specular(wo, wi, c);
+    if (wo.z > 0.0f && wi.z > 0.0f) {
+        if constexpr ((L & kLobeDiffuse) != 0) f = f + eval_diffuse(wo, wi, c);
+        if constexpr ((L & kLobeMetal) != 0) f = f + eval_metallic_specular(wo, wi, c);
+    }
+    if constexpr ((L & kLobeDiffuseTrans) != 0)
+        if (wo.z > 0.0f && wi.z < 0.0f) f = f + eval_diffuse_transmission(wo, wi, c);
+    if constexpr ((L & kLobeDielTrans) != 0)
+        return f + eval_dielectric_specular(wo, wi, c);
+    else if constexpr ((L & kLobeDielRefl) != 0)
+        return wo.z * wi.z >= 0.0f ? f + eval_dielectric_specular(wo, wi, c) : f;
+    else
+        return f;
 }
-KS_DEV float bsdf_pdf(f3 wo, f3 wi, const Closure &c) {
+template <unsigned L = kLobesAll> KS_DEV float bsdf_pdf(f3 wo, f3 wi, const Closure &c) {
     if (wo.z == 0.0f || wi.z == 0.0f) return 0.0f;
     float w[4];
-    lobe_weights(wo, c, w);
+    lobe_weights<L>(wo, c, w);
     if (w[0] == 0.0f && w[1] == 0.0f && w[2] == 0.0f && w[3] == 0.0f) return 0.0f;
-    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f;
+    float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, p3 = 0.0f;
     if (wo.z > 0.0f && wi.z > 0.0f) {
         p0 = wi.z * kInvPi;
-        if (w[2] != 0.0f) p2 = pdf_metallic_specular(wo, wi, c);
+        if constexpr ((L & kLobeMetal) != 0)
+            if (w[2] != 0.0f) p2 = pdf_metallic_specular(wo, wi, c);
     }
-    if (wo.z > 0.0f && wi.z < 0.0f && w[1] != 0.0f) p1 = pdf_diffuse_transmission(wo, wi, c);
-    float p3 = dielectric_pdf_needed(wo, wi, c, w[3]) ? pdf_dielectric_specular(wo, wi, c) : 0.0f;
+    if constexpr ((L & kLobeDiffuseTrans) != 0)
+        if (wo.z > 0.0f && wi.z < 0.0f && w[1] != 0.0f) p1 = pdf_diffuse_transmission(wo, wi, c);
+    if constexpr ((L & kLobeDielTrans) != 0) {
+        if (dielectric_pdf_needed(wo, wi, c, w[3])) p3 = pdf_dielectric_specular(wo, wi, c);
+    } else if constexpr ((L & kLobeDielRefl) != 0) {
+        // without transmission the dielectric lobe has no density below the surface
+        if (wo.z * wi.z >= 0.0f && dielectric_pdf_needed(wo, wi, c, w[3])) p3 = pdf_dielectric_specular(wo, wi, c);
+    }
     return p0 * w[0] + p1 * w[1] + p2 * w[2] + p3 * w[3];
 }
 
@@ -525,25 +554,34 @@ KS_DEV int sample_small_distrib4(const float w[4], float u, float *u_remap) { //
     return last_pos;
 }
 
-// Returns f*cos/pdf, sets wi and pdf (principled_bsdf.cpp:326-393)
+// Returns f*cos/pdf, sets wi and pdf (principled_bsdf.cpp:326-393). L: the lobes that can have weight.
+template <unsigned L = kLobesAll>
 KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, float u1, float &pdf) {
     pdf = 0.0f;
     const f3 zero = mk(0, 0, 0);
     if (wo.z == 0.0f) return zero;
     float w[4];
-    lobe_weights(wo, c, w);
+    lobe_weights<L>(wo, c, w);
     if (w[0] == 0.0f && w[1] == 0.0f && w[2] == 0.0f && w[3] == 0.0f) return zero;
     float u_remap = 0.0f;
     int lobe = sample_small_distrib4(w, u_lobe, &u_remap);
     float p[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     GGX g(c.ax, c.ay);
-    if (lobe == 0) {
+    constexpr bool kDiel = (L & (kLobeDielRefl | kLobeDielTrans)) != 0;
+    // the dielectric lobe's density at wi: below the surface only with transmission
+    auto diel_pdf = [&](f3 wi_) -> float {
+        if constexpr ((L & kLobeDielTrans) == 0)
+            if (wo.z * wi_.z < 0.0f) return 0.0f;
+        return dielectric_pdf_needed(wo, wi_, c, w[3]) ? pdf_dielectric_specular(wo, wi_, c) : 0.0f;
+    };
+    if ((L & kLobeDiffuse) != 0 && lobe == 0) {
         wi = sample_cosine_hemisphere(u0, u1);
         p[0] = wi.z * kInvPi;
         if (is_zero(wi) || p[0] == 0.0f) return zero;
-        if (wo.z > 0.0f && wi.z > 0.0f && w[2] != 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
-        if (dielectric_pdf_needed(wo, wi, c, w[3])) p[3] = pdf_dielectric_specular(wo, wi, c);
-    } else if (lobe == 1) {
+        if constexpr ((L & kLobeMetal) != 0)
+            if (wo.z > 0.0f && wi.z > 0.0f && w[2] != 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
+        if constexpr (kDiel) p[3] = diel_pdf(wi);
+    } else if ((L & kLobeDiffuseTrans) != 0 && lobe == 1) {
         if (c.diffuse_trans_fwd > 0.05f) {
             wi = sample_hg(-wo, c.diffuse_trans_fwd, u0, u1, &p[1]);
         } else {
@@ -552,16 +590,16 @@ KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, f
             wi.z = -wi.z;
         }
         if (is_zero(wi) || p[1] == 0.0f) return zero;
-        if (dielectric_pdf_needed(wo, wi, c, w[3])) p[3] = pdf_dielectric_specular(wo, wi, c);
-    } else if (lobe == 2) {
+        if constexpr (kDiel) p[3] = diel_pdf(wi);
+    } else if ((L & kLobeMetal) != 0 && lobe == 2) {
         f3 wh = g.sample(wo * sgnf(wo.z), u0, u1);
         wi = reflect(wo, wh);
         if (wo.z * wi.z < 0.0f) return zero;
         p[2] = g.D(wh) * g.G1(wo) / (4.0f * fabsf(wo.z));
         if (is_zero(wi) || p[2] == 0.0f) return zero;
         if (wo.z > 0.0f && wi.z > 0.0f) p[0] = wi.z * kInvPi;
-        if (dielectric_pdf_needed(wo, wi, c, w[3])) p[3] = pdf_dielectric_specular(wo, wi, c);
-    } else {
+        if constexpr (kDiel) p[3] = diel_pdf(wi);
+    } else if (kDiel && lobe == 3) {
         f3 wh = g.sample(wo * sgnf(wo.z), u0, u1);
         float D = g.D(wh), G1 = g.G1(wo);
         float eta = wo.z >= 0.0f ? c.ior : (1.0f / c.ior);
@@ -574,22 +612,29 @@ KS_DEV f3 bsdf_sample(f3 wo, f3 &wi, const Closure &c, float u_lobe, float u0, f
             wi = reflect(wo, wh);
             if (wo.z * wi.z < 0.0f) return zero;
             p[3] = D * G1 / (4.0f * fabsf(wo.z)) * (Fr_refl / (Fr_refl + Fr_refr));
-        } else {
+        } else if constexpr ((L & kLobeDielTrans) != 0) {
             if (!refract(wo, wh * sgnf(wo.z), 1.0f / eta, wi)) return zero;
             if (wo.z * wi.z > 0.0f) return zero;
             float denom = sqr(dot(wo, wh) + eta * dot(wi, wh));
             float jac = eta * eta * fabsf(dot(wi, wh)) / denom;
             p[3] = D * G1 * fabsf(dot(wo, wh)) / fabsf(wo.z) * jac * (Fr_refr / (Fr_refl + Fr_refr));
+        } else {
+            // specular_trans 0: Fr_refr = 0, so the refraction's pdf carries the factor 0 / (1 + 0)
+            return zero;
         }
         if (is_zero(wi) || p[3] == 0.0f) return zero;
         if (wo.z > 0.0f && wi.z > 0.0f) {
             p[0] = wi.z * kInvPi;
-            if (w[2] != 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
+            if constexpr ((L & kLobeMetal) != 0)
+                if (w[2] != 0.0f) p[2] = pdf_metallic_specular(wo, wi, c);
         }
-        if (wo.z > 0.0f && wi.z < 0.0f && w[1] != 0.0f) p[1] = pdf_diffuse_transmission(wo, wi, c);
+        if constexpr ((L & kLobeDiffuseTrans) != 0)
+            if (wo.z > 0.0f && wi.z < 0.0f && w[1] != 0.0f) p[1] = pdf_diffuse_transmission(wo, wi, c);
+    } else {
+        return zero; // a lobe this instantiation has no code for (its weight is zero for this material)
     }
     pdf = p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3];
-    return bsdf_eval(wo, wi, c) / pdf;
+    return bsdf_eval<L>(wo, wi, c) / pdf;
 }
 
 // ---------------------------------------------------------------------------------------------------

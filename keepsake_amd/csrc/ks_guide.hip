@@ -138,6 +138,7 @@ struct QueryArray { // queries n x 9: position, normal, wo
     const float *q;
     uint32_t n;
     __device__ uint32_t count() const { return n; }
+    __device__ uint32_t slot(uint32_t i) const { return i; }
     __device__ bool load(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
         if (i >= n) return false;
         const float *e = q + 9 * (size_t)i;
@@ -156,9 +157,16 @@ struct QueryHits { // wavefront paths with a surface hit: position, geometric no
     const uint32_t *state; // path state: bounce in bits 16..23
     int32_t max_bounce;    // paths at this bounce or deeper are not guided: no query
     __device__ uint32_t count() const { return *n_dev; }
+    __device__ uint32_t slot(uint32_t i) const { return i; }
+    __device__ bool wanted(uint32_t i) const {
+        return (int32_t)((state[i] >> 16) & 0xffu) < max_bounce && __float_as_int(hits[i].w) >= 0;
+    }
     __device__ bool load(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
         if (i >= *n_dev) return false;
         if ((int32_t)((state[i] >> 16) & 0xffu) >= max_bounce) return false;
+        return load_slot(i, p, nn, wo);
+    }
+    __device__ bool load_slot(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
         float4 h = hits[i];
         int32_t prim = __float_as_int(h.w);
         if (prim < 0) return false;
@@ -170,6 +178,32 @@ struct QueryHits { // wavefront paths with a surface hit: position, geometric no
         return true;
     }
 };
+
+// The compacted queries of a wavefront buffer: the slots of the paths that want a mixture (a hit below the
+// guided bounces), listed by k_guide_compact, so that every wave of the inference works on 64 real
+// queries. Mixtures still go to the paths' own slots.
+struct QueryList {
+    QueryHits h;
+    const uint32_t *list, *n_list;
+    __device__ uint32_t count() const { return *n_list; }
+    __device__ uint32_t slot(uint32_t i) const { return list[i]; }
+    __device__ bool load(uint32_t i, f3 &p, f3 &nn, f3 &wo) const {
+        if (i >= *n_list) return false;
+        return h.load_slot(list[i], p, nn, wo);
+    }
+};
+
+__global__ __launch_bounds__(256) void k_guide_compact(QueryHits h, uint32_t *list, uint32_t *n_list) {
+    __shared__ uint32_t s_app[5];
+    const uint32_t n = *h.n_dev;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n) return;
+    const bool want = i < n && h.wanted(i);
+    const uint32_t j = block_append<4>(n_list, want, s_app);
+    if (want) list[j] = i;
+}
+
+__global__ void k_zero_u32(uint32_t *p) { *p = 0u; }
 
 // Both features of one hash level: FP32 trilinear sum over corners 0..7.
 __device__ __forceinline__ void grid_pair(const GuideNet &net, int level, const float u[3], float &f0, float &f1) {
@@ -397,7 +431,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideNet net, Src s
         sum += __shfl_xor(sum, 16);
         sum += __shfl_xor(sum, 32);
         if (vq[t]) {
-            float4 *dst = mix + 10 * (size_t)(base + 16 * t + q);
+            float4 *dst = mix + 10 * (size_t)src.slot(base + 16 * t + q);
             reinterpret_cast<float2 *>(dst)[g] = make_float2(ex0 / sum, ex1 / sum);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
@@ -931,14 +965,19 @@ int32_t guide_max_steps(const ks_guide *g) { return g->cfg.max_steps_per_pass; }
 int32_t guide_train_spp(const ks_guide *g) { return g->cfg.train_spp; }
 
 int launch_infer(ks_guide *g, const DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
-                 const float4 *hits, const uint32_t *state, int32_t max_bounce, float4 *mix, hipStream_t s) {
+                 const float4 *hits, const uint32_t *state, int32_t max_bounce, float4 *mix, uint32_t *list,
+                 uint32_t *n_list, hipStream_t s) {
     unsigned blocks = (n_max + kInferBlock - 1) / kInferBlock;
+    const QueryHits qh{sc, n_dev, rd, hits, state, max_bounce};
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, n_list);
+    hipLaunchKernelGGL(k_guide_compact, dim3((n_max + 255) / 256), dim3(256), 0, s, qh, list, n_list);
+    const QueryList ql{qh, list, n_list};
     if (g->fp8)
-        hipLaunchKernelGGL((k_guide_infer<QueryHits, 2>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
-                           QueryHits{sc, n_dev, rd, hits, state, max_bounce}, mix, nullptr);
+        hipLaunchKernelGGL((k_guide_infer<QueryList, 2>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g), ql, mix,
+                           nullptr);
     else
-        hipLaunchKernelGGL((k_guide_infer<QueryHits, 0>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g),
-                           QueryHits{sc, n_dev, rd, hits, state, max_bounce}, mix, nullptr);
+        hipLaunchKernelGGL((k_guide_infer<QueryList, 0>), dim3(blocks), dim3(kInferBlock), 0, s, net_of(g), ql, mix,
+                           nullptr);
     KS_HIP(hipGetLastError());
     return KS_OK;
 }

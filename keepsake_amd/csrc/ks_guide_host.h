@@ -26,9 +26,12 @@ int32_t guide_max_steps(const ks_guide *g);
 int32_t guide_train_spp(const ks_guide *g);
 int32_t guide_max_bounce(const ks_guide *g); // first bounce that is not guided (255: none)
 // Mixtures (10 float4 per path slot) for the first *n_dev paths of the current wavefront buffer; paths
-// whose bounce (state bits 16..23) is max_bounce or more get none.
+// whose bounce (state bits 16..23) is max_bounce or more, and misses, get none. The paths that get one
+// are first compacted into `list` (n_max entries, their count in *n_list) so the inference's waves hold
+// only real queries.
 int launch_infer(ks_guide *g, const ksr::DevScene &sc, const uint32_t *n_dev, uint32_t n_max, const float4 *rd,
-                 const float4 *hits, const uint32_t *state, int32_t max_bounce, float4 *mix, hipStream_t s);
+                 const float4 *hits, const uint32_t *state, int32_t max_bounce, float4 *mix, uint32_t *list,
+                 uint32_t *n_list, hipStream_t s);
 // One Adam step over n device-resident training samples samples[0], samples[stride], ... With c_scale > 0
 // the sample weights are multiplied by c_scale and clamped at the guide's weight_clamp (render training:
 // c_scale = 1 / mean weight of the pass); c_scale = 0 takes the weights as given.

@@ -8,12 +8,19 @@
 namespace ksr {
 using namespace ksd;
 
+// Material classes of sorted shading: each class's paths are shaded by a kernel compiled for the BSDF
+// lobes that class can have (kShadeMiss: rays that left the scene).
+constexpr int kShadeMiss = 0, kShadeDielectric = 1, kShadeMetal = 2, kShadeGeneral = 3;
+constexpr int kShadeClasses = 4;
+constexpr int kShadeUnsorted = -1;
+
 struct Ctrl {
     uint32_t n_active; // paths in the current buffer, [0, n_active)
     uint32_t n_next;   // survivors appended by shade into the next buffer
     uint32_t n_shadow; // shadow rays appended by shade
     uint32_t iteration;
     uint64_t work_next, work_end;
+    uint32_t n_class[kShadeClasses]; // paths of the current buffer per material class (sorted shading)
 };
 
 enum StatSlot {
@@ -51,6 +58,7 @@ struct DevScene {
     const int32_t *tex_mat_of;          // per material: index into tex_mats, or -1
     int32_t textured;                   // 1: some geometry has texcoords or some material has fields
     int32_t alpha;                      // 1: some triangle is alpha-tested (opacity map)
+    const uint8_t *mat_class;           // per material: kShadeDielectric, kShadeMetal or kShadeGeneral
 };
 
 struct RenderConsts {
@@ -114,6 +122,31 @@ KS_DEV uint32_t wave_append(uint32_t *counter, bool pred) {
         base = __shfl(base, leader);
     }
     return base + prefix;
+}
+
+// Block-level append: the block's slots come from one atomic per block rather than one per wave (many
+// blocks appending to one counter serialise on its atomic unit). Every thread of the block must call it,
+// as it synchronises the block; s_tmp is NWAVES + 1 words of LDS. Slots follow wave order, then lane
+// order, within the block.
+template <int NWAVES> KS_DEV uint32_t block_append(uint32_t *counter, bool pred, uint32_t *s_tmp) {
+    const uint64_t mask = __ballot(pred);
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t prefix = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) s_tmp[wave] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < NWAVES; ++w) {
+            const uint32_t c = s_tmp[w];
+            s_tmp[w] = total;
+            total += c;
+        }
+        s_tmp[NWAVES] = total ? atomicAdd(counter, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = s_tmp[NWAVES] + s_tmp[wave] + prefix;
+    __syncthreads(); // s_tmp is free again for the next call
+    return slot;
 }
 
 KS_DEV void wave_add_stat(unsigned long long *stats, int slot, uint32_t v) {

@@ -149,14 +149,14 @@ def test_config5_fp8_guided_unbiased(ks, device, room_spheres):
                  ks.guide_config(seed=5, guide_fraction=0.5, inference_fp8=1, train_spp=16))
     film = ks.Film(device, W, H)
     sums = []
-    P = 144
+    P = 256  # a power of two (see the z-Sobol index rule in test_gpu_guided_render.py)
     ks.small_pt_render_guided(sc, cam, prm(P, 1, 7), g, film, progress=lambda f, s: sums.append(f.download()[0]))
     assert len(sums) == P
     def block_z(sums):
-        # per-pass means of 16 x 8 pixel blocks after 16 training passes, against the reference
+        # per-pass means of 16 x 12 pixel blocks after 16 training passes, against the reference
         per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)
-        blocks = per_pass[16:].reshape(P - 16, H // 8, 8, W // 16, 16, 3).mean(axis=(2, 4))
-        ref_blocks = _img(ref_rgb, ref_w).reshape(H // 8, 8, W // 16, 16, 3).mean(axis=(1, 3))
+        blocks = per_pass[16:].reshape(P - 16, H // 12, 12, W // 16, 16, 3).mean(axis=(2, 4))
+        ref_blocks = _img(ref_rgb, ref_w).reshape(H // 12, 12, W // 16, 16, 3).mean(axis=(1, 3))
         se = blocks.std(axis=0, ddof=1) / math.sqrt(P - 16)
         se_ref = se * math.sqrt((P - 16) / 8192)
         z = (blocks.mean(axis=0) - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
@@ -169,8 +169,8 @@ def test_config5_fp8_guided_unbiased(ks, device, room_spheres):
     zp = block_z(psums)
     print("config 5 guided: max |z|", float(np.abs(z).max()), "over", z.size, "| plain control",
           float(np.abs(zp).max()), flush=True)
-    # Bonferroni over the 216 block-channels at 1% (4.07); the plain render is the calibration control
-    assert np.abs(zp).max() < 4.1 and np.abs(z).max() < 4.1
+    # Bonferroni over the 144 block-channels at 1% (3.98); the plain render is the calibration control
+    assert np.abs(zp).max() < 4.0 and np.abs(z).max() < 4.0
     img = _img(*film.download())
     host = rel_mse(img, _img(ref_rgb, ref_w))
     dev_r = film.relmse(ref)

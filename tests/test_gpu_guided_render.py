@@ -54,26 +54,35 @@ def test_zero_fraction_reproduces_plain(ks, device, room, fp8):
     guided.close()
 
 
+BH, BW = 12, 16  # block rows, columns: 6 x 8 blocks of 192 pixels
+
+
 def _block_z(sums, ref_rgb, ref_w, skip):
-    """z of the per-pass 16 x 8 pixel block means (passes after `skip`) against the FP64 reference, per
+    """z of the per-pass 16 x 12 pixel block means (passes after `skip`) against the FP64 reference, per
     block and channel, with the reference's own error added (same per-sample variance assumed)."""
     P = len(sums)
     per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)  # one sample per pixel each
-    blocks = per_pass[skip:].reshape(P - skip, H // 8, 8, W // 16, 16, 3).mean(axis=(2, 4))
-    ref_blocks = (ref_rgb / np.maximum(ref_w[..., :1] + ref_w[..., 1:], 1)).reshape(H // 8, 8, W // 16, 16, 3).mean(
+    blocks = per_pass[skip:].reshape(P - skip, H // BH, BH, W // BW, BW, 3).mean(axis=(2, 4))
+    ref_blocks = (ref_rgb / np.maximum(ref_w[..., :1] + ref_w[..., 1:], 1)).reshape(H // BH, BH, W // BW, BW, 3).mean(
         axis=(1, 3))
     se = blocks.std(axis=0, ddof=1) / math.sqrt(P - skip)
     se_ref = se * math.sqrt((P - skip) / 8192)
-    return (blocks.mean(axis=0) - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
+    z = (blocks.mean(axis=0) - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
+    k = np.unravel_index(np.argmax(np.abs(z)), z.shape)
+    # the largest |z|, with the block's mean, reference, error and the skew of its per-pass values
+    v = blocks[(slice(None),) + k]
+    print("  largest |z| at block", k[:2], "channel", k[2], "z %.2f mean %.5f ref %.5f se %.5f max/mean %.1f"
+          % (z[k], v.mean(), ref_blocks[k], se[k], v.max() / max(v.mean(), 1e-12)), flush=True)
+    return z
 
 
-# Bonferroni over the 216 block-channels at 1%: two-sided normal quantile of 1 - 0.01 / 432 = 4.07. Blocks of
-# 128 pixels keep the per-pass block means close to normal (per-pass values of single pixels are heavy-tailed).
-Z_MAX = 4.1
+# Bonferroni over the 144 block-channels at 1%: two-sided normal quantile of 1 - 0.01 / 288 = 3.98. Blocks of
+# 192 pixels over 496 passes keep the block means close to normal.
+Z_MAX = 4.0
 # Blocks with some channel beyond |z| = 3: the three channels of a block share their paths, so they are counted
-# per block. Each of the 72 independent blocks has one with probability at most 3 x 0.0027 = 0.0081 under no
-# bias: 0.58 expected, P(>= 4) = 0.0025.
-Z3_BLOCKS_MAX = 3
+# per block. Each of the 48 independent blocks has one with probability at most 3 x 0.0027 = 0.0081 under no
+# bias: 0.39 expected, P(>= 3) = 0.008.
+Z3_BLOCKS_MAX = 2
 
 
 def _blocks_over3(z):
@@ -98,7 +107,9 @@ def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
         rgb, _ = f.download()
         sums.append(rgb.copy())
 
-    P = 160
+    # a power of two: with P spp the z-Sobol sample index runs over [0, 2^floor(log2 P)) per pixel, and
+    # indices beyond it would reuse the next pixel's points (sobol.h:143-146), correlating the passes
+    P = 512
     # seeds other than the reference's, so the two sample sets are independent
     ks.small_pt_render_guided(sc, hs.camera, _params(ks, P, rng_seed=7), g, film, progress=on_pass)
     assert len(sums) == P
