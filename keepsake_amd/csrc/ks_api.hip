@@ -230,10 +230,29 @@ __global__ void k_fill_u64(unsigned long long *p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Render statistics: row 0 takes the shading kernels' atomics, row 1 the totals, and rows 2 .. 2 + R
+// the traversal kernels' per-block counts (block_add_stats). One block of 256 threads sums the R
+// partial rows and row 0 into row 1.
+static_assert(256 % kStatCount == 0, "k_stat_reduce splits 256 threads into groups of kStatCount");
+__global__ __launch_bounds__(256) void k_stat_reduce(unsigned long long *stats, int R) {
+    __shared__ unsigned long long s_sum[256];
+    const int k = threadIdx.x % kStatCount, part = threadIdx.x / kStatCount;
+    constexpr int kParts = 256 / kStatCount;
+    unsigned long long x = 0;
+    for (int r = part; r < R; r += kParts) x += stats[(size_t)(2 + r) * kStatCount + k];
+    s_sum[threadIdx.x] = x;
+    __syncthreads();
+    if (threadIdx.x < kStatCount) {
+        unsigned long long t = stats[threadIdx.x];
+        for (int p = 0; p < kParts; ++p) t += s_sum[p * kStatCount + threadIdx.x];
+        stats[kStatCount + threadIdx.x] = t;
+    }
+}
+
 __global__ void k_ctrl_init(Ctrl *c, uint64_t work_end) {
     c->n_active = 0;
-    c->n_next = 0;
-    c->n_shadow = 0;
+    c->n_next.v = 0;
+    c->n_shadow.v = 0;
     c->iteration = 0;
     c->work_next = 0;
     c->work_end = work_end;
@@ -271,23 +290,23 @@ __global__ void k_gen_done(Ctrl *c, uint32_t Q) {
     uint32_t n_gen = (uint32_t)min(room, c->work_end - c->work_next);
     c->n_active += n_gen;
     c->work_next += n_gen;
-    c->n_next = 0;
-    c->n_shadow = 0;
-    for (int k = 0; k < kShadeClasses; ++k) c->n_class[k] = 0;
+    c->n_next.v = 0;
+    c->n_shadow.v = 0;
+    for (int k = 0; k < kShadeClasses; ++k) c->n_class[k].v = 0;
 }
 
 __global__ void k_iter_done(Ctrl *c) {
-    c->n_active = c->n_next;
+    c->n_active = c->n_next.v;
     c->iteration += 1;
 }
 
 // Closest hit for n rays (count read from *n_dev when given). Grid-stride over rays; one LDS stack
-// column per lane.
+// column per lane. Ray, node and triangle counts go to the block's row of stat_rows.
 template <int W, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                          const float4 *ro, const float4 *rd, float4 *hit,
                                                          uint32_t *ovf, uint32_t ovf_cap,
-                                                         unsigned long long *stats) {
+                                                         unsigned long long *stat_rows) {
     __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
     const uint32_t n = n_dev ? *n_dev : n_fixed;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -313,10 +332,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
         },
         n_nodes, n_tris, n_rays,
         [&](int32_t prim, float u, float v, const TravRay &r) { return opacity_pass(sc, prim, u, v, r.o, r.d); });
-    wave_add_stat(stats, kRaysClosest, n_rays);
-    wave_add_stat(stats, kNodesClosest, n_nodes);
-    wave_add_stat(stats, kTrisClosest, n_tris);
-    wave_add_stat(stats, kSpills, st.spills);
+    __shared__ uint32_t s_stat[4 * (kTraceBlock / 64)];
+    constexpr int slots[4] = {kRaysClosest, kNodesClosest, kTrisClosest, kSpills};
+    const uint32_t v[4] = {n_rays, n_nodes, n_tris, st.spills};
+    block_add_stats<kTraceBlock / 64>(stat_rows, slots, v, s_stat);
 }
 
 // Any hit. mode 0: write the occlusion flag; mode 1: add the NEE contribution of unoccluded rays to
@@ -326,7 +345,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
                                                         const uint32_t *target, const float4 *contrib,
                                                         float4 *path_L, float4 *samp, float4 *recs, uint32_t *ovf,
-                                                        uint32_t ovf_cap, unsigned long long *stats) {
+                                                        uint32_t ovf_cap, unsigned long long *stat_rows) {
     __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
     const uint32_t n = n_dev ? *n_dev : n_fixed;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,10 +383,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     trace_rays<true, W, ALPHA>(
         sc.nodes, sc.tris, gid, n, G, st, load, store, n_nodes, n_tris, n_rays,
         [&](int32_t prim, float u, float v, const TravRay &r) { return opacity_pass(sc, prim, u, v, r.o, r.d); });
-    wave_add_stat(stats, kRaysShadow, n_rays);
-    wave_add_stat(stats, kNodesShadow, n_nodes);
-    wave_add_stat(stats, kTrisShadow, n_tris);
-    wave_add_stat(stats, kSpills, st.spills);
+    __shared__ uint32_t s_stat[4 * (kTraceBlock / 64)];
+    constexpr int slots[4] = {kRaysShadow, kNodesShadow, kTrisShadow, kSpills};
+    const uint32_t v[4] = {n_rays, n_nodes, n_tris, st.spills};
+    block_add_stats<kTraceBlock / 64>(stat_rows, slots, v, s_stat);
 }
 
 KS_DEV uint32_t cdf_sample(const float *cdf, int n, float u, float &prob) { // distrib.cpp:33-41
@@ -497,7 +516,7 @@ __global__ __launch_bounds__(256) void k_classify(DevScene sc, const float4 *hit
     }
 #pragma unroll
     for (int k = 0; k < kShadeClasses; ++k) {
-        const uint32_t j = block_append<4>(&ctrl->n_class[k], cls == k, s_app);
+        const uint32_t j = block_append<4>(&ctrl->n_class[k].v, cls == k, s_app);
         if (cls == k) order[(size_t)k * Q + j] = i;
     }
 }
@@ -536,7 +555,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
     constexpr unsigned LB = CLASS == kShadeDielectric ? kLobesOpaqueDielectric
                             : CLASS == kShadeMetal     ? kLobesMetal
                                                        : kLobesAll;
-    const uint32_t n = CLASS == kShadeUnsorted ? ctrl->n_active : ctrl->n_class[CLASS];
+    const uint32_t n = CLASS == kShadeUnsorted ? ctrl->n_active : ctrl->n_class[CLASS].v;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n) return; // whole block idle
     const bool active = t < n;
@@ -790,7 +809,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
     }
 
     __shared__ uint32_t s_app[5];
-    uint32_t j = block_append<4>(&ctrl->n_next, survive, s_app);
+    uint32_t j = block_append<4>(&ctrl->n_next.v, survive, s_app);
     if (survive) {
         nxt.work[j] = w;
         nxt.state[j] = (uint32_t)sob.dim | ((uint32_t)(bounce + 1) << 16);
@@ -799,7 +818,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         nxt.ro[j] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
         nxt.rd[j] = make_float4(new_d.x, new_d.y, new_d.z, kInf);
     }
-    uint32_t k = block_append<4>(&ctrl->n_shadow, shadow, s_app);
+    uint32_t k = block_append<4>(&ctrl->n_shadow.v, shadow, s_app);
     if (shadow) {
         sh_o[k] = make_float4(sh_origin.x, sh_origin.y, sh_origin.z, 0.0f);
         sh_d[k] = make_float4(sh_dir.x, sh_dir.y, sh_dir.z, sh_tmax);
@@ -1367,6 +1386,18 @@ int trace_grid(const ks_device *dev, uint32_t n, int blocks_per_cu = kMaxTraceBl
     return (int)std::max(1u, std::min(max_blocks, need));
 }
 
+// The statistics buffer: kStatCount slots in each of 2 + R rows (k_stat_reduce), R = the largest
+// traversal grid.
+uint32_t stat_partial_rows(const ks_device *dev) {
+    return (uint32_t)std::max(1, dev->n_cu) * (uint32_t)kMaxTraceBlocksPerCu;
+}
+unsigned long long *stat_rows(ks_device *dev) { return dev->ws.stats.as<unsigned long long>() + 2 * kStatCount; }
+void clear_stats(ks_device *dev, hipStream_t s) {
+    const int n = (int)(2 + stat_partial_rows(dev)) * kStatCount;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       dev->ws.stats.as<unsigned long long>(), n);
+}
+
 int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_stack) {
     Workspace &ws = dev->ws;
     if (ws.Q < Q) {
@@ -1388,7 +1419,7 @@ int ensure_workspace(ks_device *dev, uint32_t Q, size_t n_samp, uint32_t max_sta
     }
     KS_HIP(ws.samp.ensure(sizeof(float4) * std::max<size_t>(n_samp, 1)));
     KS_HIP(ws.ctrl.ensure(sizeof(Ctrl)));
-    KS_HIP(ws.stats.ensure(sizeof(unsigned long long) * kStatCount));
+    KS_HIP(ws.stats.ensure(sizeof(unsigned long long) * kStatCount * (2 + (size_t)stat_partial_rows(dev))));
     uint32_t grid = (uint32_t)std::max(1, dev->n_cu) * (uint32_t)kMaxTraceBlocksPerCu * kTraceBlock;
     uint32_t cap = max_stack + 4 > (uint32_t)kStackLds ? max_stack + 4 - kStackLds : 1u;
     if (ws.ovf_grid < grid || ws.ovf_cap < cap) {
@@ -1535,7 +1566,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         e = graph ? nullptr : tm.begin(KS_K_CLOSEST);
         launch_closest(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, 0u,
                        (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, ws.hit.as<float4>(),
-                       ws.ovf.as<uint32_t>(), ws.ovf_cap, st);
+                       ws.ovf.as<uint32_t>(), ws.ovf_cap, stat_rows(dev));
         tm.end(e);
         if (gp && gp->shade.mix) {
             e = tm.begin(KS_K_GUIDE);
@@ -1555,18 +1586,18 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                      gp ? gp->shade : GuideShade{}, (const uint32_t *)ws.order.as<uint32_t>(), Q);
         tm.end(e);
         if (wo.cap_n) {
-            hipLaunchKernelGGL(k_capture, dim3(tblocks), dim3(256), 0, s, (const uint32_t *)&ctrl->n_shadow,
+            hipLaunchKernelGGL(k_capture, dim3(tblocks), dim3(256), 0, s, (const uint32_t *)&ctrl->n_shadow.v,
                                (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), wo.cap_s,
                                wo.cap_n + 1, wo.cap_max);
-            hipLaunchKernelGGL(k_capture_done, dim3(1), dim3(1), 0, s, (const uint32_t *)&ctrl->n_shadow, wo.cap_n + 1,
+            hipLaunchKernelGGL(k_capture_done, dim3(1), dim3(1), 0, s, (const uint32_t *)&ctrl->n_shadow.v, wo.cap_n + 1,
                                wo.cap_max);
         }
         e = graph ? nullptr : tm.begin(KS_K_SHADOW);
-        launch_anyhit<1>(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_shadow, 0u,
+        launch_anyhit<1>(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_shadow.v, 0u,
                          (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), (uint8_t *)nullptr,
                          (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[1 - cur].L,
                          ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(),
-                         ws.ovf_cap, st);
+                         ws.ovf_cap, stat_rows(dev));
         hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
         tm.end(e);
         return KS_OK;
@@ -1640,7 +1671,11 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
 int read_stats(ks_device *dev, ks_render_stats *stats) {
     if (!stats) return KS_OK;
     unsigned long long h[kStatCount];
-    KS_HIP(hipMemcpyAsync(h, dev->ws.stats.p, sizeof(h), hipMemcpyDeviceToHost, dev->stream));
+    hipLaunchKernelGGL(k_stat_reduce, dim3(1), dim3(256), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
+                       (int)stat_partial_rows(dev));
+    KS_HIP(hipGetLastError());
+    KS_HIP(hipMemcpyAsync(h, dev->ws.stats.as<unsigned long long>() + kStatCount, sizeof(h), hipMemcpyDeviceToHost,
+                          dev->stream));
     KS_HIP(hipStreamSynchronize(dev->stream));
     stats->rays_closest += h[kRaysClosest];
     stats->rays_shadow += h[kRaysShadow];
@@ -2599,8 +2634,7 @@ int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_smal
         uint32_t Q = pick_queue(params, pixels * (uint64_t)std::min(spp_chunk, params->spp));
         e = ensure_workspace(dev, Q, pixels * (uint64_t)std::min(spp_chunk, params->spp), scene->max_stack);
         if (e) return e;
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                           (int)kStatCount);
+        clear_stats(dev, dev->stream);
         Timer tm{dev};
         for (int s_start = 0; s_start < params->spp; s_start += interval) {
             int s_end = std::min(params->spp, s_start + interval);
@@ -2735,8 +2769,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
             gw.rec_items = (uint32_t)pixels;
         }
         KS_HIP(gw.count.ensure(2 * sizeof(double))); // uint32 count, then the double sum of the weights
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                           (int)kStatCount);
+        clear_stats(dev, dev->stream);
         const uint32_t cap = (uint32_t)(kRecVertices * pixels);
         Timer tm{dev};
         int s = 0;
@@ -2979,8 +3012,7 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
         }
         int e = ensure_workspace(dev, Q, max_items, scene->max_stack);
         if (e) return e;
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                           (int)kStatCount);
+        clear_stats(dev, dev->stream);
         Timer tm{dev};
         for (const BatchChunk &c : chunks) {
             // declared before the device copies, so it outlives them (their release synchronises)
@@ -3043,8 +3075,7 @@ int ks_trace_samples(ks_scene *scene, const ks_camera_desc *cam, const ks_small_
         uint32_t Q = pick_queue(params, (uint64_t)n);
         e = ensure_workspace(dev, Q, (size_t)n, scene->max_stack);
         if (e) return e;
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                           (int)kStatCount);
+        clear_stats(dev, dev->stream);
         Timer tm{dev};
         e = run_wavefront(scene, rc, (uint64_t)n, Q, tm, nullptr);
         if (e) return e;
@@ -3094,8 +3125,7 @@ int ks_trace_capture(ks_scene *scene, const ks_camera_desc *cam, const ks_small_
         KS_HIP(cs.alloc(sizeof(float4) * 2 * (size_t)max_rays));
         KS_HIP(cn.alloc(2 * sizeof(uint32_t)));
         KS_HIP(hipMemsetAsync(cn.p, 0, 2 * sizeof(uint32_t), dev->stream));
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, dev->stream, dev->ws.stats.as<unsigned long long>(),
-                           (int)kStatCount);
+        clear_stats(dev, dev->stream);
         WavefrontOpts wo;
         wo.cap_c = cc.as<float4>();
         wo.cap_s = cs.as<float4>();
@@ -3212,14 +3242,14 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
     KS_HIP(dD.put(d.data(), d.size(), s));
     int e = ensure_workspace(dev, 256, 1, scene->max_stack);
     if (e) return e;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, dev->ws.stats.as<unsigned long long>(), (int)kStatCount);
+    clear_stats(dev, s);
     int grid = trace_grid(dev, (uint32_t)n);
     DevBuf out;
     if (hits) {
         KS_HIP(out.alloc(sizeof(float4) * n));
         launch_closest(scene->ds, dim3(grid), s, (const uint32_t *)nullptr, (uint32_t)n,
                        (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<float4>(), dev->ws.ovf.as<uint32_t>(),
-                       dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
+                       dev->ws.ovf_cap, stat_rows(dev));
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(hits, out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
     } else {
@@ -3227,7 +3257,7 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
         launch_anyhit<0>(scene->ds, dim3(grid), s, (const uint32_t *)nullptr, (uint32_t)n,
                          (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
                          (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr,
-                         dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap, dev->ws.stats.as<unsigned long long>());
+                         dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap, stat_rows(dev));
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(occ, out.p, n, hipMemcpyDeviceToHost, s));
     }

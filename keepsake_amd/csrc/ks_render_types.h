@@ -14,13 +14,18 @@ constexpr int kShadeMiss = 0, kShadeDielectric = 1, kShadeMetal = 2, kShadeGener
 constexpr int kShadeClasses = 4;
 constexpr int kShadeUnsorted = -1;
 
+// A counter that every block of a launch updates with an atomic sits alone on a 128-byte line.
+struct alignas(128) LineCounter {
+    uint32_t v;
+};
+
 struct Ctrl {
     uint32_t n_active; // paths in the current buffer, [0, n_active)
-    uint32_t n_next;   // survivors appended by shade into the next buffer
-    uint32_t n_shadow; // shadow rays appended by shade
     uint32_t iteration;
     uint64_t work_next, work_end;
-    uint32_t n_class[kShadeClasses]; // paths of the current buffer per material class (sorted shading)
+    LineCounter n_next;   // survivors appended by shade into the next buffer
+    LineCounter n_shadow; // shadow rays appended by shade
+    LineCounter n_class[kShadeClasses]; // paths of the current buffer per material class (sorted shading)
 };
 
 enum StatSlot {
@@ -147,6 +152,33 @@ template <int NWAVES> KS_DEV uint32_t block_append(uint32_t *counter, bool pred,
     const uint32_t slot = s_tmp[NWAVES] + s_tmp[wave] + prefix;
     __syncthreads(); // s_tmp is free again for the next call
     return slot;
+}
+
+// Adds per-lane counts into this block's own row of `rows` (kStatCount slots per row): wave sums by
+// shuffles, then one plain read-modify-write per slot by the block's first thread. A block index owns
+// its row within a launch and launches on one stream run in order, so no atomic is needed. (With one
+// atomic per wave, the thousands of waves of a persistent grid that end together serialise on the few
+// counter addresses.) Every thread of the block calls it, with the same slots; s_tmp holds NWAVES * NS.
+template <int NWAVES, int NS>
+KS_DEV void block_add_stats(unsigned long long *rows, const int (&slot)[NS], const uint32_t (&v)[NS],
+                            uint32_t *s_tmp) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        uint32_t x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (lane == 0) s_tmp[wave * NS + k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *row = rows + (size_t)blockIdx.x * kStatCount;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            unsigned long long sum = 0;
+            for (int w = 0; w < NWAVES; ++w) sum += s_tmp[w * NS + k];
+            if (sum) row[slot[k]] += sum;
+        }
+    }
 }
 
 KS_DEV void wave_add_stat(unsigned long long *stats, int slot, uint32_t v) {
