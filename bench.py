@@ -25,15 +25,17 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(1, str(ROOT / "oracle"))  # the CPU oracle, used only by the cpu_baseline leg
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of one closest-hit query: ray in (2 x 16 B), hit out (16 B), each node fetch
-# (BVH4: 96 B of child boxes + 16 B of child pointers; BVH2: 64 B), each triangle test (48 B: three
-# vertices + id).
+# Algorithmic bytes of one traversal query, as counted by the kernel: ray in (2 x 16 B) and result out (16 B:
+# the hit of a closest-hit ray; for an unoccluded shadow ray, 4 B target + 12 B contribution read, the added
+# radiance written back is not counted), each node fetch (BVH4: 96 B of child boxes + 16 B of child pointers;
+# BVH2: 64 B), each triangle test (48 B: three vertices + id).
 RAY_IO_BYTES = 48
 NODE_BYTES = {4: 112, 2: 64}
 TRI_BYTES = 48
-# The implementation-independent form (per closest-hit ray): ray 32 (o, tmin, d, tmax) + hit 20 (t, barycentrics,
-# primitive) + 32 per node fetch of the reference SBVH (its node is 32 B) + 52 per triangle test (4 B reference +
-# 12 B indices + 36 B vertices), with node fetches and triangle tests counted in the reference traversal order.
+# The implementation-independent form: per closest-hit ray, ray 32 (o, tmin, d, tmax) + hit 20 (t,
+# barycentrics, primitive) + 32 per node fetch of the reference SBVH (its node is 32 B) + 52 per triangle test
+# (4 B reference + 12 B indices + 36 B vertices), with node fetches and triangle tests counted in the
+# reference traversal order; per shadow ray, the same with a 4 B occlusion result in place of the hit.
 REF_RAY_BYTES, REF_HIT_BYTES, REF_NODE_BYTES, REF_TRI_BYTES = 32, 20, 32, 52
 # Shading, per path vertex (one per closest-hit ray): read hit 20 + path state 64 + material 64 +
 # triangle 48 + light record 48 + light CDF 4 per binary-search step; write path state 64 + extension
@@ -461,23 +463,26 @@ def main() -> int:
                      "count_s": round(time.time() - t0, 1)}
         log(f"reference-order traversal counts: {ref_order}")
 
-    launches = totals["launches"]["closest"]
-    k_ms = totals["kernel_ms"]["closest"]
+    # k_trace: one launch per wavefront iteration traces that iteration's closest-hit rays and the previous
+    # shading pass's shadow rays; a last launch after the loop traces the final pass's shadow rays alone
+    launches = totals["launches"]["closest"] + totals["launches"]["shadow"]
+    k_ms = totals["kernel_ms"]["closest"] + totals["kernel_ms"]["shadow"]
+    node_b = NODE_BYTES[info["branching"]]
     alg_bytes = (
-        RAY_IO_BYTES * totals["rays_closest"]
-        + NODE_BYTES[info["branching"]] * totals["node_visits_closest"]
-        + TRI_BYTES * totals["tri_tests_closest"]
+        RAY_IO_BYTES * (totals["rays_closest"] + totals["rays_shadow"])
+        + node_b * (totals["node_visits_closest"] + totals["node_visits_shadow"])
+        + TRI_BYTES * (totals["tri_tests_closest"] + totals["tri_tests_shadow"])
     )
     achieved = alg_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
     traffic = None
-    tf = ROOT / "profiles" / "traffic_closest_spheres.json"
+    tf = ROOT / "profiles" / "traffic_trace_spheres.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
     avg_ms = k_ms / max(launches, 1)
     kernel_counted = {"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
                       "bytes_per_launch": alg_bytes / max(launches, 1),
-                      "bytes_per_ray": f"{RAY_IO_BYTES} + {NODE_BYTES[info['branching']]} per BVH{info['branching']} node "
-                      f"visit + {TRI_BYTES} per triangle test, counted by the kernel"}
+                      "bytes_per_ray": f"{RAY_IO_BYTES} + {node_b} per BVH{info['branching']} node visit + {TRI_BYTES} "
+                      "per triangle test, counted by the kernel"}
     roofline = {
         "bound": "hbm",
         "achieved": kernel_counted["achieved"],
@@ -485,22 +490,28 @@ def main() -> int:
         "unit": "GB/s",
         "frac": kernel_counted["frac"],
         "traffic": traffic,
-        "kernel": f"k_closest (closest-hit traversal, {BVH_NAME[builder]}, BVH{info['branching']} nodes)",
+        "kernel": f"k_trace (closest-hit rays of an iteration + shadow rays of the previous pass, {BVH_NAME[builder]}, "
+        f"BVH{info['branching']} nodes)",
         "bytes_per_launch": alg_bytes / max(launches, 1),
         "avg_launch_ms": avg_ms,
+        "launches_per_step": launches / args.steps,
     }
     if ref_order is not None:
-        # algorithmic bytes per closest ray: ray 32 + hit 20 + 32 per node fetch + 52 per triangle test, with
-        # the counts of the reference traversal order over the same rays (the primary figure)
-        per_ray = REF_RAY_BYTES + REF_HIT_BYTES + REF_NODE_BYTES * ref_order["nodes_per_closest_ray"] + \
+        # algorithmic bytes with the counts of the reference traversal order over the same rays (the primary
+        # figure): per closest-hit ray 32 + 20 + 32 per node fetch + 52 per triangle test; per shadow ray the
+        # same with a 4 B result
+        per_c = REF_RAY_BYTES + REF_HIT_BYTES + REF_NODE_BYTES * ref_order["nodes_per_closest_ray"] + \
             REF_TRI_BYTES * ref_order["tris_per_closest_ray"]
-        ref_bytes = per_ray * totals["rays_closest"]
+        per_s = REF_RAY_BYTES + 4 + REF_NODE_BYTES * ref_order["nodes_per_shadow_ray"] + \
+            REF_TRI_BYTES * ref_order["tris_per_shadow_ray"]
+        ref_bytes = per_c * totals["rays_closest"] + per_s * totals["rays_shadow"]
         ref_achieved = ref_bytes / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
         roofline.update({"achieved": round(ref_achieved, 1), "frac": round(ref_achieved / HBM_PEAK_GBS, 4),
                          "bytes_per_launch": ref_bytes / max(launches, 1),
-                         "bytes_per_ray": round(per_ray, 1),
+                         "bytes_per_closest_ray": round(per_c, 1), "bytes_per_shadow_ray": round(per_s, 1),
                          "basis": "reference traversal order (cuda/sbvh.cu:512-603) over the BVH2 SBVH, counted on the "
-                         f"host for the {ref_order['closest_rays']} closest-hit rays of {ref_order['paths']} random paths"})
+                         f"host for the {ref_order['closest_rays']} closest-hit and {ref_order['shadow_rays']} shadow "
+                         f"rays of {ref_order['paths']} random paths"})
     roofline["kernel_counted"] = kernel_counted
     if traffic:
         roofline["counter_hbm"] = {"bytes_per_launch": traffic, "achieved": round(traffic / (avg_ms / 1e3) / 1e9, 1),
@@ -526,21 +537,9 @@ def main() -> int:
         "bytes_per_launch": shade_bytes / max(shade_launches, 1),
         "avg_launch_ms": shade_ms / max(shade_launches, 1),
     }
-    tf = ROOT / "profiles" / "traffic_closest_spheres.json"
+    tf = ROOT / "profiles" / "traffic_trace_spheres.json"
     if tf.exists():
         roofline_shade["traffic"] = json.loads(tf.read_text()).get("shade_hbm_bytes_per_launch")
-
-    roofline_shadow = None
-    if ref_order is not None and totals["kernel_ms"]["shadow"] > 0:
-        # per shadow ray: ray 32 + occlusion flag 4 + the reference-order node fetches and triangle tests
-        per_ray = REF_RAY_BYTES + 4 + REF_NODE_BYTES * ref_order["nodes_per_shadow_ray"] + \
-            REF_TRI_BYTES * ref_order["tris_per_shadow_ray"]
-        sh_ms = totals["kernel_ms"]["shadow"]
-        sh_ach = per_ray * totals["rays_shadow"] / (sh_ms / 1e3) / 1e9
-        roofline_shadow = {"bound": "hbm", "achieved": round(sh_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(sh_ach / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_ray": round(per_ray, 1),
-                           "kernel": "k_anyhit (NEE shadow rays)",
-                           "avg_launch_ms": sh_ms / max(totals["launches"]["shadow"], 1)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -608,7 +607,6 @@ def main() -> int:
         },
         "roofline": roofline,
         "roofline_shade": roofline_shade,
-        "roofline_shadow": roofline_shadow,
         "reference_order_counts": ref_order,
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,

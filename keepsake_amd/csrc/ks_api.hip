@@ -253,6 +253,7 @@ __global__ void k_ctrl_init(Ctrl *c, uint64_t work_end) {
     c->n_active = 0;
     c->n_next.v = 0;
     c->n_shadow.v = 0;
+    c->n_shadow_trace = 0;
     c->iteration = 0;
     c->work_next = 0;
     c->work_end = work_end;
@@ -285,12 +286,15 @@ __global__ __launch_bounds__(256) void k_raygen(RenderConsts rc, PathBufs cur, c
     cur.rd[slot] = make_float4(d.x, d.y, d.z, kInf);
 }
 
+// Also hands the shadow rays of the previous shading pass to this iteration's traversal (n_shadow_trace)
+// and empties the queue for this iteration's shading pass, which refills it after the traversal.
 __global__ void k_gen_done(Ctrl *c, uint32_t Q) {
     uint64_t room = (uint64_t)(Q - c->n_active);
     uint32_t n_gen = (uint32_t)min(room, c->work_end - c->work_next);
     c->n_active += n_gen;
     c->work_next += n_gen;
     c->n_next.v = 0;
+    c->n_shadow_trace = c->n_shadow.v;
     c->n_shadow.v = 0;
     for (int k = 0; k < kShadeClasses; ++k) c->n_class[k].v = 0;
 }
@@ -338,14 +342,29 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     block_add_stats<kTraceBlock / 64>(stat_rows, slots, v, s_stat);
 }
 
-// Any hit. mode 0: write the occlusion flag; mode 1: add the NEE contribution of unoccluded rays to
-// their target (next path buffer slot, or the per-sample result when bit 31 is set).
-template <int MODE, int W, bool ALPHA>
+// The NEE contribution of unoccluded shadow ray i, added to its target: a slot of the next path buffer, or
+// the per-sample result when bit 31 is set (each target has at most one shadow ray per pass). The
+// contribution's w carries the training record of the vertex that cast the ray, + 1 (0: none).
+KS_DEV void add_nee(uint32_t i, const uint32_t *target, const float4 *contrib, float4 *path_L, float4 *samp,
+                    float4 *recs) {
+    uint32_t tg = target[i];
+    float4 c = contrib[i];
+    float4 *dst = (tg & 0x80000000u) ? &samp[tg & 0x7fffffffu] : &path_L[tg];
+    float4 v = *dst;
+    v.x += c.x;
+    v.y += c.y;
+    v.z += c.z;
+    *dst = v;
+    uint32_t slot = __float_as_uint(c.w);
+    if (recs && slot) recs[(size_t)(slot - 1) * kRecFloat4 + kRecNee] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
+// Any hit: the occlusion flag of each of n rays (ks_occluded).
+template <int W, bool ALPHA>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
-                                                        const uint32_t *target, const float4 *contrib,
-                                                        float4 *path_L, float4 *samp, float4 *recs, uint32_t *ovf,
-                                                        uint32_t ovf_cap, unsigned long long *stat_rows) {
+                                                        uint32_t *ovf, uint32_t ovf_cap,
+                                                        unsigned long long *stat_rows) {
     __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
     const uint32_t n = n_dev ? *n_dev : n_fixed;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -363,23 +382,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
         r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
         return d4.w;
     };
-    auto store = [&](uint32_t i, const TravHit &h) {
-        const bool blocked = h.prim >= 0;
-        if (MODE == 0) {
-            occ[i] = blocked ? 1 : 0;
-        } else if (!blocked) {
-            uint32_t tg = target[i];
-            float4 c = contrib[i];
-            float4 *dst = (tg & 0x80000000u) ? &samp[tg & 0x7fffffffu] : &path_L[tg];
-            float4 v = *dst;
-            v.x += c.x;
-            v.y += c.y;
-            v.z += c.z;
-            *dst = v;
-            uint32_t slot = __float_as_uint(c.w); // training record of the vertex that cast the ray, + 1
-            if (recs && slot) recs[(size_t)(slot - 1) * kRecFloat4 + kRecNee] = make_float4(c.x, c.y, c.z, 0.0f);
-        }
-    };
+    auto store = [&](uint32_t i, const TravHit &h) { occ[i] = h.prim >= 0 ? 1 : 0; };
     trace_rays<true, W, ALPHA>(
         sc.nodes, sc.tris, gid, n, G, st, load, store, n_nodes, n_tris, n_rays,
         [&](int32_t prim, float u, float v, const TravRay &r) { return opacity_pass(sc, prim, u, v, r.o, r.d); });
@@ -387,6 +390,77 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
     constexpr int slots[4] = {kRaysShadow, kNodesShadow, kTrisShadow, kSpills};
     const uint32_t v[4] = {n_rays, n_nodes, n_tris, st.spills};
     block_add_stats<kTraceBlock / 64>(stat_rows, slots, v, s_stat);
+}
+
+// One traversal launch per wavefront iteration: the closest hits of the iteration's nc rays (ro, rd -> hit)
+// and the occlusion of the ns shadow rays of the previous shading pass (sro, srd; an unoccluded ray adds
+// its contribution by add_nee). Indices [0, nc) are closest-hit queries and [nc, nc + ns) shadow rays,
+// one grid-stride sequence: the persistent grid balances both queues, and the long shadow rays of one
+// pass run beside the next pass's closest-hit rays instead of in a launch of their own, whose length
+// the slowest rays set. Node and triangle counts are kept per ray and added to the block's LDS totals of
+// the ray's kind when it ends (no per-kind registers in the loop); block 0 adds the ray counts.
+template <int W, bool ALPHA>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(
+    DevScene sc, const uint32_t *nc_dev, const float4 *ro, const float4 *rd, float4 *hit, const uint32_t *ns_dev,
+    const float4 *sro, const float4 *srd, const uint32_t *target, const float4 *contrib, float4 *path_L, float4 *samp,
+    float4 *recs, uint32_t *ovf, uint32_t ovf_cap, unsigned long long *stat_rows) {
+    __shared__ uint32_t s_stack[kStackLds * kTraceBlock];
+    __shared__ uint32_t s_cnt[5]; // closest nodes, tris; shadow nodes, tris; stack spills
+    const uint32_t nc = *nc_dev, ns = *ns_dev, n = nc + ns;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t G = gridDim.x * blockDim.x;
+    if (blockIdx.x * blockDim.x >= n) return;
+    if (threadIdx.x < 5) s_cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    TravStack st;
+    st.lds = s_stack + threadIdx.x;
+    st.ovf = ovf + gid;
+    st.ovf_stride = G;
+    st.ovf_cap = ovf_cap;
+    st.spills = 0;
+    uint32_t n_nodes = 0, n_tris = 0, n_rays = 0; // nodes and tris of the current ray
+    trace_rays_kind<W, ALPHA>(
+        sc.nodes, sc.tris, gid, n, G, st, [nc](uint32_t i) { return i >= nc; },
+        [&](uint32_t i, TravRay &r) {
+            const float4 *po = ro, *pd = rd;
+            uint32_t j = i;
+            if (i >= nc) {
+                po = sro;
+                pd = srd;
+                j = i - nc;
+            }
+            const float4 o4 = po[j], d4 = pd[j];
+            r = make_trav_ray(ld3(o4), ld3(d4), o4.w);
+            return d4.w;
+        },
+        [&](uint32_t i, const TravHit &h) {
+            const bool sh = i >= nc;
+            if (!sh)
+                hit[i] = h.prim >= 0 ? make_float4(h.t, h.u, h.v, __int_as_float(h.prim))
+                                     : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
+            else if (h.prim < 0)
+                add_nee(i - nc, target, contrib, path_L, samp, recs);
+            atomicAdd(&s_cnt[sh ? 2 : 0], n_nodes);
+            atomicAdd(&s_cnt[sh ? 3 : 1], n_tris);
+            n_nodes = 0;
+            n_tris = 0;
+        },
+        n_nodes, n_tris, n_rays,
+        [&](int32_t prim, float u, float v, const TravRay &r) { return opacity_pass(sc, prim, u, v, r.o, r.d); });
+    if (st.spills) atomicAdd(&s_cnt[4], st.spills);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long *row = stat_rows + (size_t)blockIdx.x * kStatCount;
+        row[kNodesClosest] += s_cnt[0];
+        row[kTrisClosest] += s_cnt[1];
+        row[kNodesShadow] += s_cnt[2];
+        row[kTrisShadow] += s_cnt[3];
+        row[kSpills] += s_cnt[4];
+        if (blockIdx.x == 0) {
+            row[kRaysClosest] += nc;
+            row[kRaysShadow] += ns;
+        }
+    }
 }
 
 KS_DEV uint32_t cdf_sample(const float *cdf, int n, float u, float &prob) { // distrib.cpp:33-41
@@ -1203,7 +1277,7 @@ struct Workspace {
     DevBuf order; // material-sorted shading: kShadeClasses lists of Q slots
     DevBuf samp;
     DevBuf ctrl, stats;
-    DevBuf ovf;
+    DevBuf ovf; // traversal stack spill area, one column per lane of the largest traversal grid
     uint32_t ovf_grid = 0, ovf_cap = 0;
 };
 
@@ -1336,14 +1410,24 @@ template <typename... A> void launch_closest(const DevScene &ds, dim3 g, hipStre
         else hipLaunchKernelGGL((k_closest<2, false>), g, b, 0, s, ds, args...);
     }
 }
-template <int MODE, typename... A> void launch_anyhit(const DevScene &ds, dim3 g, hipStream_t s, A... args) {
+template <typename... A> void launch_trace(const DevScene &ds, dim3 g, hipStream_t s, A... args) {
     const dim3 b(kTraceBlock);
     if (ds.alpha) {
-        if (ds.width == 4) hipLaunchKernelGGL((k_anyhit<MODE, 4, true>), g, b, 0, s, ds, args...);
-        else hipLaunchKernelGGL((k_anyhit<MODE, 2, true>), g, b, 0, s, ds, args...);
+        if (ds.width == 4) hipLaunchKernelGGL((k_trace<4, true>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_trace<2, true>), g, b, 0, s, ds, args...);
     } else {
-        if (ds.width == 4) hipLaunchKernelGGL((k_anyhit<MODE, 4, false>), g, b, 0, s, ds, args...);
-        else hipLaunchKernelGGL((k_anyhit<MODE, 2, false>), g, b, 0, s, ds, args...);
+        if (ds.width == 4) hipLaunchKernelGGL((k_trace<4, false>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_trace<2, false>), g, b, 0, s, ds, args...);
+    }
+}
+template <typename... A> void launch_anyhit(const DevScene &ds, dim3 g, hipStream_t s, A... args) {
+    const dim3 b(kTraceBlock);
+    if (ds.alpha) {
+        if (ds.width == 4) hipLaunchKernelGGL((k_anyhit<4, true>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_anyhit<2, true>), g, b, 0, s, ds, args...);
+    } else {
+        if (ds.width == 4) hipLaunchKernelGGL((k_anyhit<4, false>), g, b, 0, s, ds, args...);
+        else hipLaunchKernelGGL((k_anyhit<2, false>), g, b, 0, s, ds, args...);
     }
 }
 
@@ -1550,7 +1634,10 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
     const int tblocks = (int)((Q + 255) / 256);
     const int tgrid = trace_grid(dev, Q, wo.blocks_per_cu);
     const bool graph = wo.graph && !gp;
-    // one iteration reading buffer cur and writing buffer 1 - cur; no events while a graph is captured
+    // One iteration reading buffer cur and writing buffer 1 - cur; no events while a graph is captured.
+    // Its traversal also traces the previous shading pass's shadow rays (k_trace): their contributions go
+    // to the survivors' slots [0, n_next) of buffer cur (ray generation fills the slots after them) and to
+    // per-sample results, before this iteration's shading reads the slots and refills the shadow queue.
     auto iteration = [&](int cur) -> int {
         hipEvent_t e = graph ? nullptr : tm.begin(KS_K_RAYGEN);
         if (wo.batch) hipLaunchKernelGGL(k_raygen<true>, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
@@ -1564,9 +1651,12 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                                wo.cap_max);
         }
         e = graph ? nullptr : tm.begin(KS_K_CLOSEST);
-        launch_closest(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, 0u,
-                       (const float4 *)pb[cur].ro, (const float4 *)pb[cur].rd, ws.hit.as<float4>(),
-                       ws.ovf.as<uint32_t>(), ws.ovf_cap, stat_rows(dev));
+        launch_trace(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, (const float4 *)pb[cur].ro,
+                     (const float4 *)pb[cur].rd, ws.hit.as<float4>(), (const uint32_t *)&ctrl->n_shadow_trace,
+                     (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(),
+                     (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[cur].L,
+                     ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(), ws.ovf_cap,
+                     stat_rows(dev));
         tm.end(e);
         if (gp && gp->shade.mix) {
             e = tm.begin(KS_K_GUIDE);
@@ -1592,14 +1682,21 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             hipLaunchKernelGGL(k_capture_done, dim3(1), dim3(1), 0, s, (const uint32_t *)&ctrl->n_shadow.v, wo.cap_n + 1,
                                wo.cap_max);
         }
-        e = graph ? nullptr : tm.begin(KS_K_SHADOW);
-        launch_anyhit<1>(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_shadow.v, 0u,
-                         (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(), (uint8_t *)nullptr,
-                         (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[1 - cur].L,
-                         ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(),
-                         ws.ovf_cap, stat_rows(dev));
         hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
+        return KS_OK;
+    };
+    // The last shading pass's shadow rays: a traversal of them alone, once the loop has ended.
+    auto flush_shadow = [&](int cur) -> int {
+        hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q); // no work left: only moves the queue
+        hipEvent_t e = tm.begin(KS_K_SHADOW);
+        launch_trace(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, (const float4 *)pb[cur].ro,
+                     (const float4 *)pb[cur].rd, ws.hit.as<float4>(), (const uint32_t *)&ctrl->n_shadow_trace,
+                     (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(),
+                     (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[cur].L,
+                     ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(), ws.ovf_cap,
+                     stat_rows(dev));
         tm.end(e);
+        KS_HIP(hipGetLastError());
         return KS_OK;
     };
     hipGraphExec_t exec = nullptr;
@@ -1661,6 +1758,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             }
         }
     }
+    if (status == KS_OK) status = flush_shadow(graph ? 0 : cur);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (g) (void)hipGraphDestroy(g);
     if (status) return status;
@@ -3254,10 +3352,9 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
         KS_HIP(hipMemcpyAsync(hits, out.p, sizeof(float4) * n, hipMemcpyDeviceToHost, s));
     } else {
         KS_HIP(out.alloc(n));
-        launch_anyhit<0>(scene->ds, dim3(grid), s, (const uint32_t *)nullptr, (uint32_t)n,
-                         (const float4 *)dO.p(), (const float4 *)dD.p(), out.as<uint8_t>(), (const uint32_t *)nullptr,
-                         (const float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr, (float4 *)nullptr,
-                         dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap, stat_rows(dev));
+        launch_anyhit(scene->ds, dim3(grid), s, (const uint32_t *)nullptr, (uint32_t)n, (const float4 *)dO.p(),
+                      (const float4 *)dD.p(), out.as<uint8_t>(), dev->ws.ovf.as<uint32_t>(), dev->ws.ovf_cap,
+                      stat_rows(dev));
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(occ, out.p, n, hipMemcpyDeviceToHost, s));
     }

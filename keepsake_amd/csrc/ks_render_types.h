@@ -23,6 +23,7 @@ struct Ctrl {
     uint32_t n_active; // paths in the current buffer, [0, n_active)
     uint32_t iteration;
     uint64_t work_next, work_end;
+    uint32_t n_shadow_trace; // shadow rays of the previous shading pass, traced by this iteration's traversal
     LineCounter n_next;   // survivors appended by shade into the next buffer
     LineCounter n_shadow; // shadow rays appended by shade
     LineCounter n_class[kShadeClasses]; // paths of the current buffer per material class (sorted shading)

@@ -15,10 +15,18 @@ constexpr int kTraceBlock = 256; // threads per traversal block
 struct TravRay {
     f3 o, d;
     f3 inv, oinv;
-    bool nx, ny, nz; // direction sign per axis (RayBoundHelper::dir_is_neg, cuda/aabb.cuh:155-169)
-    int kx, ky, kz;  // RayTriHelper (cuda/ray_tri.cuh:8-37)
+    // bits 0-1 kx, 2-3 ky, 4-5 kz (RayTriHelper, cuda/ray_tri.cuh:8-37); bits 6, 7, 8: direction sign of x,
+    // y, z (RayBoundHelper::dir_is_neg, cuda/aabb.cuh:155-169). One register for all six keeps the
+    // traversal loops within 64 VGPRs.
+    uint32_t axes;
     float sx, sy, sz;
     float tmin;
+    KS_DEV int kx() const { return (int)(axes & 3u); }
+    KS_DEV int ky() const { return (int)((axes >> 2) & 3u); }
+    KS_DEV int kz() const { return (int)((axes >> 4) & 3u); }
+    KS_DEV bool nx() const { return (axes >> 6) & 1u; }
+    KS_DEV bool ny() const { return (axes >> 7) & 1u; }
+    KS_DEV bool nz() const { return (axes >> 8) & 1u; }
 };
 
 KS_DEV float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
@@ -36,9 +44,7 @@ KS_DEV TravRay make_trav_ray(f3 o, f3 d, float tmin) {
     float dz = fabsf(d.z) < eps ? copysignf(eps, d.z) : d.z;
     r.inv = mk(1.0f / dx, 1.0f / dy, 1.0f / dz);
     r.oinv = mk(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
-    r.nx = signbit(d.x);
-    r.ny = signbit(d.y);
-    r.nz = signbit(d.z);
+    const uint32_t signs = (signbit(d.x) ? 1u : 0u) | (signbit(d.y) ? 2u : 0u) | (signbit(d.z) ? 4u : 0u);
     float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
     int kz = (ax >= ay && ax >= az) ? 0 : ((ay >= ax && ay >= az) ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
@@ -49,9 +55,7 @@ KS_DEV TravRay make_trav_ray(f3 o, f3 d, float tmin) {
         kx = ky;
         ky = t;
     }
-    r.kx = kx;
-    r.ky = ky;
-    r.kz = kz;
+    r.axes = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4) | (signs << 6);
     r.sx = comp(d, kx) / dkz;
     r.sy = comp(d, ky) / dkz;
     r.sz = 1.0f / dkz;
@@ -68,9 +72,10 @@ KS_DEV bool isect_tri(const TravRay &r, float4 a4, float4 b4, float4 c4, float t
     f3 A0 = mk(a4.x - r.o.x, a4.y - r.o.y, a4.z - r.o.z);
     f3 B0 = mk(b4.x - r.o.x, b4.y - r.o.y, b4.z - r.o.z);
     f3 C0 = mk(c4.x - r.o.x, c4.y - r.o.y, c4.z - r.o.z);
-    float Ax = comp(A0, r.kx), Ay = comp(A0, r.ky), Az = comp(A0, r.kz);
-    float Bx = comp(B0, r.kx), By = comp(B0, r.ky), Bz = comp(B0, r.kz);
-    float Cx = comp(C0, r.kx), Cy = comp(C0, r.ky), Cz = comp(C0, r.kz);
+    const int kx = r.kx(), ky = r.ky(), kz = r.kz();
+    float Ax = comp(A0, kx), Ay = comp(A0, ky), Az = comp(A0, kz);
+    float Bx = comp(B0, kx), By = comp(B0, ky), Bz = comp(B0, kz);
+    float Cx = comp(C0, kx), Cy = comp(C0, ky), Cz = comp(C0, kz);
     float sxA = Ax - r.sx * Az, sxB = Bx - r.sx * Bz, sxC = Cx - r.sx * Cz;
     float syA = Ay - r.sy * Az, syB = By - r.sy * Bz, syC = Cy - r.sy * Cz;
     // bc = cross(sy, sx)
@@ -148,12 +153,12 @@ KS_DEV bool traverse(const float4 *__restrict__ nodes, const float4 *__restrict_
             float4 a = np[0], b = np[1], c = np[2];
             uint2 ch = *reinterpret_cast<const uint2 *>(np + 3);
             // child 0: x (a.x, a.y) y (a.z, a.w) z (c.x, c.y); child 1: x (b.x, b.y) y (b.z, b.w) z (c.z, c.w)
-            float n0x = fmaf(r.nx ? a.y : a.x, r.inv.x, -r.oinv.x), f0x = fmaf(r.nx ? a.x : a.y, r.inv.x, -r.oinv.x);
-            float n0y = fmaf(r.ny ? a.w : a.z, r.inv.y, -r.oinv.y), f0y = fmaf(r.ny ? a.z : a.w, r.inv.y, -r.oinv.y);
-            float n0z = fmaf(r.nz ? c.y : c.x, r.inv.z, -r.oinv.z), f0z = fmaf(r.nz ? c.x : c.y, r.inv.z, -r.oinv.z);
-            float n1x = fmaf(r.nx ? b.y : b.x, r.inv.x, -r.oinv.x), f1x = fmaf(r.nx ? b.x : b.y, r.inv.x, -r.oinv.x);
-            float n1y = fmaf(r.ny ? b.w : b.z, r.inv.y, -r.oinv.y), f1y = fmaf(r.ny ? b.z : b.w, r.inv.y, -r.oinv.y);
-            float n1z = fmaf(r.nz ? c.w : c.z, r.inv.z, -r.oinv.z), f1z = fmaf(r.nz ? c.z : c.w, r.inv.z, -r.oinv.z);
+            float n0x = fmaf(r.nx() ? a.y : a.x, r.inv.x, -r.oinv.x), f0x = fmaf(r.nx() ? a.x : a.y, r.inv.x, -r.oinv.x);
+            float n0y = fmaf(r.ny() ? a.w : a.z, r.inv.y, -r.oinv.y), f0y = fmaf(r.ny() ? a.z : a.w, r.inv.y, -r.oinv.y);
+            float n0z = fmaf(r.nz() ? c.y : c.x, r.inv.z, -r.oinv.z), f0z = fmaf(r.nz() ? c.x : c.y, r.inv.z, -r.oinv.z);
+            float n1x = fmaf(r.nx() ? b.y : b.x, r.inv.x, -r.oinv.x), f1x = fmaf(r.nx() ? b.x : b.y, r.inv.x, -r.oinv.x);
+            float n1y = fmaf(r.ny() ? b.w : b.z, r.inv.y, -r.oinv.y), f1y = fmaf(r.ny() ? b.z : b.w, r.inv.y, -r.oinv.y);
+            float n1z = fmaf(r.nz() ? c.w : c.z, r.inv.z, -r.oinv.z), f1z = fmaf(r.nz() ? c.z : c.w, r.inv.z, -r.oinv.z);
             float t0n = fmaxf(fmaxf(n0x, n0y), fmaxf(n0z, r.tmin));
             float t0f = fminf(fminf(f0x, f0y), fminf(f0z, tmax)) * robust;
             float t1n = fmaxf(fmaxf(n1x, n1y), fmaxf(n1z, r.tmin));
@@ -225,12 +230,12 @@ KS_DEV bool traverse4(const float4 *__restrict__ nodes, const float4 *__restrict
             uint32_t cp[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float nx = fmaf(r.nx ? lane4(hx, i) : lane4(lx, i), r.inv.x, -r.oinv.x);
-                float fx = fmaf(r.nx ? lane4(lx, i) : lane4(hx, i), r.inv.x, -r.oinv.x);
-                float ny = fmaf(r.ny ? lane4(hy, i) : lane4(ly, i), r.inv.y, -r.oinv.y);
-                float fy = fmaf(r.ny ? lane4(ly, i) : lane4(hy, i), r.inv.y, -r.oinv.y);
-                float nz = fmaf(r.nz ? lane4(hz, i) : lane4(lz, i), r.inv.z, -r.oinv.z);
-                float fz = fmaf(r.nz ? lane4(lz, i) : lane4(hz, i), r.inv.z, -r.oinv.z);
+                float nx = fmaf(r.nx() ? lane4(hx, i) : lane4(lx, i), r.inv.x, -r.oinv.x);
+                float fx = fmaf(r.nx() ? lane4(lx, i) : lane4(hx, i), r.inv.x, -r.oinv.x);
+                float ny = fmaf(r.ny() ? lane4(hy, i) : lane4(ly, i), r.inv.y, -r.oinv.y);
+                float fy = fmaf(r.ny() ? lane4(ly, i) : lane4(hy, i), r.inv.y, -r.oinv.y);
+                float nz = fmaf(r.nz() ? lane4(hz, i) : lane4(lz, i), r.inv.z, -r.oinv.z);
+                float fz = fmaf(r.nz() ? lane4(lz, i) : lane4(hz, i), r.inv.z, -r.oinv.z);
                 float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
                 float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * robust;
                 key[i] = tn <= tf ? tn : INFINITY;
@@ -300,12 +305,12 @@ KS_DEV uint32_t visit_node(const float4 *__restrict__ nodes, const TravRay &r, f
         const float4 *np = nodes + 4 * (size_t)node;
         float4 a = np[0], b = np[1], c = np[2];
         uint2 ch = *reinterpret_cast<const uint2 *>(np + 3);
-        float n0x = fmaf(r.nx ? a.y : a.x, r.inv.x, -r.oinv.x), f0x = fmaf(r.nx ? a.x : a.y, r.inv.x, -r.oinv.x);
-        float n0y = fmaf(r.ny ? a.w : a.z, r.inv.y, -r.oinv.y), f0y = fmaf(r.ny ? a.z : a.w, r.inv.y, -r.oinv.y);
-        float n0z = fmaf(r.nz ? c.y : c.x, r.inv.z, -r.oinv.z), f0z = fmaf(r.nz ? c.x : c.y, r.inv.z, -r.oinv.z);
-        float n1x = fmaf(r.nx ? b.y : b.x, r.inv.x, -r.oinv.x), f1x = fmaf(r.nx ? b.x : b.y, r.inv.x, -r.oinv.x);
-        float n1y = fmaf(r.ny ? b.w : b.z, r.inv.y, -r.oinv.y), f1y = fmaf(r.ny ? b.z : b.w, r.inv.y, -r.oinv.y);
-        float n1z = fmaf(r.nz ? c.w : c.z, r.inv.z, -r.oinv.z), f1z = fmaf(r.nz ? c.z : c.w, r.inv.z, -r.oinv.z);
+        float n0x = fmaf(r.nx() ? a.y : a.x, r.inv.x, -r.oinv.x), f0x = fmaf(r.nx() ? a.x : a.y, r.inv.x, -r.oinv.x);
+        float n0y = fmaf(r.ny() ? a.w : a.z, r.inv.y, -r.oinv.y), f0y = fmaf(r.ny() ? a.z : a.w, r.inv.y, -r.oinv.y);
+        float n0z = fmaf(r.nz() ? c.y : c.x, r.inv.z, -r.oinv.z), f0z = fmaf(r.nz() ? c.x : c.y, r.inv.z, -r.oinv.z);
+        float n1x = fmaf(r.nx() ? b.y : b.x, r.inv.x, -r.oinv.x), f1x = fmaf(r.nx() ? b.x : b.y, r.inv.x, -r.oinv.x);
+        float n1y = fmaf(r.ny() ? b.w : b.z, r.inv.y, -r.oinv.y), f1y = fmaf(r.ny() ? b.z : b.w, r.inv.y, -r.oinv.y);
+        float n1z = fmaf(r.nz() ? c.w : c.z, r.inv.z, -r.oinv.z), f1z = fmaf(r.nz() ? c.z : c.w, r.inv.z, -r.oinv.z);
         float t0n = fmaxf(fmaxf(n0x, n0y), fmaxf(n0z, r.tmin));
         float t0f = fminf(fminf(f0x, f0y), fminf(f0z, tmax)) * robust;
         float t1n = fmaxf(fmaxf(n1x, n1y), fmaxf(n1z, r.tmin));
@@ -323,7 +328,7 @@ KS_DEV uint32_t visit_node(const float4 *__restrict__ nodes, const TravRay &r, f
         // adjacent float4s, so the sign selects the load and no per-child select is needed
         const char *base = reinterpret_cast<const char *>(nodes);
         const uint32_t off = node * 128u; // node arrays stay below 4 GiB (2^25 nodes)
-        const uint32_t ox = 16u * (uint32_t)r.nx, oy = 16u * (uint32_t)r.ny, oz = 16u * (uint32_t)r.nz;
+        const uint32_t ox = (r.axes >> 2) & 16u, oy = (r.axes >> 3) & 16u, oz = (r.axes >> 4) & 16u;
         auto ld = [&](uint32_t o) { return *reinterpret_cast<const float4 *>(base + o); };
         const float4 nxv = ld(off + ox), fxv = ld(off + 16u - ox), nyv = ld(off + 32u + oy),
                      fyv = ld(off + 48u - oy), nzv = ld(off + 64u + oz), fzv = ld(off + 80u - oz);
@@ -386,10 +391,16 @@ struct NoFilter {
     KS_DEV bool operator()(int32_t, float, float, const TravRay &) const { return true; }
 };
 
-template <bool ANY, int W, bool ALPHA = false, typename Load, typename Store, typename Filter = NoFilter>
-KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t i, uint32_t n,
-                       uint32_t stride, TravStack &st, Load &&load, Store &&store, uint32_t &n_nodes,
-                       uint32_t &n_tris, uint32_t &n_rays, Filter filter = Filter()) {
+// The query kind of ray i: any_hit(i) true stops at the first accepted hit (occlusion), false finds the
+// closest. trace_rays takes it per ray, so one persistent launch can serve both queues.
+template <bool ANY> struct FixedKind {
+    KS_DEV bool operator()(uint32_t) const { return ANY; }
+};
+
+template <int W, bool ALPHA = false, typename Kind, typename Load, typename Store, typename Filter = NoFilter>
+KS_DEV void trace_rays_kind(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t i, uint32_t n,
+                            uint32_t stride, TravStack &st, Kind any_hit, Load &&load, Store &&store,
+                            uint32_t &n_nodes, uint32_t &n_tris, uint32_t &n_rays, Filter filter = Filter()) {
     if (i >= n) return;
     TravRay r;
     TravHit hit;
@@ -420,6 +431,7 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
             uint32_t first = (leaf >> 4) & 0x7ffffffu;
             uint32_t cnt = (leaf & 15u) + 1u;
             leaf = 0;
+            const bool any = any_hit(i);
             for (uint32_t k = 0; k < cnt; ++k) {
                 ++n_tris;
                 const float4 *tp = tris + 3 * (size_t)(first + k);
@@ -428,7 +440,7 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
                 if (isect_tri(r, v0, v1, v2, tmax, t, u, v)) {
                     if (ALPHA && __float_as_uint(v1.w) != 0u && !filter(__float_as_int(v0.w), u, v, r)) continue;
                     hit.prim = __float_as_int(v0.w);
-                    if (ANY) {
+                    if (any) {
                         done = true;
                         break;
                     }
@@ -452,6 +464,14 @@ KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restric
             exhausted = false;
         }
     }
+}
+
+template <bool ANY, int W, bool ALPHA = false, typename Load, typename Store, typename Filter = NoFilter>
+KS_DEV void trace_rays(const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t i, uint32_t n,
+                       uint32_t stride, TravStack &st, Load &&load, Store &&store, uint32_t &n_nodes,
+                       uint32_t &n_tris, uint32_t &n_rays, Filter filter = Filter()) {
+    trace_rays_kind<W, ALPHA>(nodes, tris, i, n, stride, st, FixedKind<ANY>(), load, store, n_nodes, n_tris, n_rays,
+                              filter);
 }
 
 // Dispatch on the node width of the scene (2 or 4).
