@@ -203,11 +203,12 @@ def time_to_relmse(ks, dev, args, scene=("room", 3), size=None, ref_spp=None, ma
 
 
 def guide_ceiling(ks, dev, args) -> dict:
-    """What a trained guide can gain on the room at equal spp: a guide trained for 64 spp (Adam steps of
-    2^18 samples), then frozen, renders 256 spp at guide fraction 0.5, guiding the first 1 / 2 / all
-    bounces; plain renders the same spp. relMSE is measured against a 4096-spp FP64 plain reference, whose
-    own noise (v / 4096 from the plain run) is subtracted. The ceiling speed-up at equal error is the
-    variance ratio times the ratio of time per sample."""
+    """What a trained guide can gain on the room at equal spp: a guide trained for --ceiling-train-spp
+    one-spp passes (Adam steps of up to 2^18 samples, at most 8 per pass), then frozen, renders 256 spp
+    at guide fraction 0.5, guiding the first 1 / 2 / all bounces; plain renders the same spp. relMSE is
+    measured against a 4096-spp FP64 plain reference, whose own noise (v / 4096 from the plain run) is
+    subtracted. The ceiling speed-up at equal error is the variance ratio times the ratio of time per
+    sample."""
     hs = ks.procedural_scene("room", 3)
     sc = ks.Scene(dev, hs)
     info = sc.info()
@@ -231,10 +232,11 @@ def guide_ceiling(ks, dev, args) -> dict:
     out = {"spp": n, "reference_spp": 4096, "reference_noise": c, "plain": {"relmse": r_plain, "seconds": t_plain}}
     for mb in (1, 2, 0):
         g = ks.Guide(dev, info["bound_lo"], info["bound_hi"],
-                     ks.guide_config(seed=1, guide_fraction=0.5, train_spp=64, max_steps_per_pass=8, max_guided_bounce=mb))
+                     ks.guide_config(seed=1, guide_fraction=0.5, train_spp=args.ceiling_train_spp, max_steps_per_pass=8,
+                                     max_guided_bounce=mb))
         tr = ks.Film(dev, W, H)
         t0 = time.perf_counter()
-        ks.small_pt_render_guided(sc, cam, prm(64, 2), g, tr)
+        ks.small_pt_render_guided(sc, cam, prm(args.ceiling_train_spp, 2), g, tr)
         t_train = time.perf_counter() - t0
         st = g.stats()
         g.set_mode(0.5, 0)  # frozen
@@ -337,6 +339,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 48-view leg")
     ap.add_argument("--guide-ceiling", action="store_true",
                     help="also measure what a long-trained, frozen guide gains at equal spp on the room")
+    ap.add_argument("--ceiling-train-spp", type=int, default=512,
+                    help="training passes (one spp each) of the guide-ceiling leg")
     ap.add_argument("--fp8-relmse", action="store_true",
                     help="also time the guided tracer with FP8 inference in the room's time-to-relMSE leg")
     ap.add_argument("--no-scale-leg", action="store_true",

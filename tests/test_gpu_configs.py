@@ -129,8 +129,9 @@ def test_config5_paths_match_oracle(ks, device, ora, room_spheres):
 
 
 def test_config5_fp8_guided_unbiased(ks, device, room_spheres):
-    """FP8-inference guiding on the 10M-triangle scene at 128x72: block means agree with an FP64 plain
-    reference (Bonferroni bar over the block-channels), and the device relMSE equals the host formula."""
+    """FP8-inference guiding on the 10M-triangle scene at 128x72: block means agree with those of an
+    independent plain render (Bonferroni bar over the block-channels), and the device relMSE against an
+    FP64 plain reference equals the host formula."""
     hs = room_spheres
     sc = ks.Scene(device, hs, bvh={"builder": 1})
     W, H = 128, 72
@@ -152,24 +153,32 @@ def test_config5_fp8_guided_unbiased(ks, device, room_spheres):
     P = 256  # a power of two (see the z-Sobol index rule in test_gpu_guided_render.py)
     ks.small_pt_render_guided(sc, cam, prm(P, 1, 7), g, film, progress=lambda f, s: sums.append(f.download()[0]))
     assert len(sums) == P
-    def block_z(sums):
-        # per-pass means of 16 x 12 pixel blocks after 16 training passes, against the reference
+
+    def block_stats(sums):
+        # per-pass means of 16 x 12 pixel blocks after 16 training passes
         per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)
         blocks = per_pass[16:].reshape(P - 16, H // 12, 12, W // 16, 16, 3).mean(axis=(2, 4))
-        ref_blocks = _img(ref_rgb, ref_w).reshape(H // 12, 12, W // 16, 16, 3).mean(axis=(1, 3))
-        se = blocks.std(axis=0, ddof=1) / math.sqrt(P - 16)
-        se_ref = se * math.sqrt((P - 16) / 8192)
-        z = (blocks.mean(axis=0) - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
-        return z[se > 0]
+        return blocks.mean(axis=0), blocks.std(axis=0, ddof=1) / math.sqrt(P - 16)
 
-    z = block_z(sums)
-    plain = ks.Film(device, W, H)
-    psums = []
-    ks.small_pt_render(sc, cam, prm(P, 1, 11), plain, progress=lambda f, s: psums.append(f.download()[0]))
-    zp = block_z(psums)
+    def block_z(a, b):
+        # two-sample z of two independent renders' block means (as in test_gpu_guided_render.py: the block
+        # means of a room lit through a slit are skewed, a difference of two renders much less so)
+        (ma, sa), (mb, sb) = block_stats(a), block_stats(b)
+        return ((ma - mb) / np.sqrt(sa**2 + sb**2 + 1e-30))[(sa > 0) & (sb > 0)]
+
+    def plain_sums(seed):
+        f = ks.Film(device, W, H)
+        out = []
+        ks.small_pt_render(sc, cam, prm(P, 1, seed), f, progress=lambda ff, s: out.append(ff.download()[0]))
+        f.close()
+        return out
+
+    p11 = plain_sums(11)
+    z = block_z(sums, p11)
+    zp = block_z(plain_sums(21), p11)
     print("config 5 guided: max |z|", float(np.abs(z).max()), "over", z.size, "| plain control",
           float(np.abs(zp).max()), flush=True)
-    # Bonferroni over the 144 block-channels at 1% (3.98); the plain render is the calibration control
+    # Bonferroni over the 144 block-channels at 1% (3.98); two independent plain renders are the control
     assert np.abs(zp).max() < 4.0 and np.abs(z).max() < 4.0
     img = _img(*film.download())
     host = rel_mse(img, _img(ref_rgb, ref_w))
@@ -178,7 +187,6 @@ def test_config5_fp8_guided_unbiased(ks, device, room_spheres):
     assert abs(dev_r - host) <= 1e-12 + 1e-9 * host
     g.close()
     film.close()
-    plain.close()
     ref.close()
     sc.close()
 

@@ -4,9 +4,10 @@
    for bit (same Sobol samples, same estimator), while it still records paths and trains: the plumbing
    around the estimator changes nothing.
 2. With the trained guide sampling half of the directions the estimator stays unbiased, with BF16 and
-   with FP8 (e4m3) inference of the network: block means of
-   the per-pass images agree with an FP64 plain reference within their standard errors (z-test with a
-   Bonferroni bar that a plain render passes as the control), and relMSE halves from P/2 to P spp.
+   with FP8 (e4m3) inference of the network: block means of the per-pass images agree with those of an
+   independent plain render of the same spp within their standard errors (two-sample z-test with a
+   Bonferroni bar, which two independent plain renders pass as the control), the whole image agrees with
+   an FP64 plain reference, and relMSE halves from P/2 to P spp.
 3. Training lowers the cross-entropy of the mixture on the radiance samples below that of a cosine lobe
    (the network learns something the BSDF-only method does not have)."""
 
@@ -57,22 +58,29 @@ def test_zero_fraction_reproduces_plain(ks, device, room, fp8):
 BH, BW = 12, 16  # block rows, columns: 6 x 8 blocks of 192 pixels
 
 
-def _block_z(sums, ref_rgb, ref_w, skip):
-    """z of the per-pass 16 x 12 pixel block means (passes after `skip`) against the FP64 reference, per
-    block and channel, with the reference's own error added (same per-sample variance assumed)."""
+def _block_stats(sums, skip):
+    """Means and standard errors of the per-pass 16 x 12 pixel block means (passes after `skip`), per block
+    and channel."""
     P = len(sums)
     per_pass = np.diff(np.stack([np.zeros_like(sums[0])] + sums), axis=0)  # one sample per pixel each
     blocks = per_pass[skip:].reshape(P - skip, H // BH, BH, W // BW, BW, 3).mean(axis=(2, 4))
-    ref_blocks = (ref_rgb / np.maximum(ref_w[..., :1] + ref_w[..., 1:], 1)).reshape(H // BH, BH, W // BW, BW, 3).mean(
-        axis=(1, 3))
-    se = blocks.std(axis=0, ddof=1) / math.sqrt(P - skip)
-    se_ref = se * math.sqrt((P - skip) / 8192)
-    z = (blocks.mean(axis=0) - ref_blocks) / np.sqrt(se**2 + se_ref**2 + 1e-30)
+    return blocks.mean(axis=0), blocks.std(axis=0, ddof=1) / math.sqrt(P - skip), blocks
+
+
+def _block_z(sums, other, skip):
+    """Two-sample z of the block means of two independent renders of the same spp. Against each other
+    rather than against a high-spp reference: some blocks of this room gather rare bright paths (light
+    through the slit after a glossy bounce), so a 496-pass block mean is skewed, usually a little low and
+    now and then far high, and its z against the exact value has tails beyond the normal's. Two renders
+    of the same kind share that skew, so their difference stays close to normal."""
+    m, se, blocks = _block_stats(sums, skip)
+    mo, seo, _ = _block_stats(other, skip)
+    z = (m - mo) / np.sqrt(se**2 + seo**2 + 1e-30)
     k = np.unravel_index(np.argmax(np.abs(z)), z.shape)
-    # the largest |z|, with the block's mean, reference, error and the skew of its per-pass values
+    # the largest |z|, with the two means, errors and the skew of the block's per-pass values
     v = blocks[(slice(None),) + k]
-    print("  largest |z| at block", k[:2], "channel", k[2], "z %.2f mean %.5f ref %.5f se %.5f max/mean %.1f"
-          % (z[k], v.mean(), ref_blocks[k], se[k], v.max() / max(v.mean(), 1e-12)), flush=True)
+    print("  largest |z| at block", k[:2], "channel", k[2], "z %.2f mean %.5f other %.5f se %.5f %.5f max/mean %.1f"
+          % (z[k], m[k], mo[k], se[k], seo[k], v.max() / max(v.mean(), 1e-12)), flush=True)
     return z
 
 
@@ -89,10 +97,32 @@ def _blocks_over3(z):
     return int((np.abs(z) > 3).any(axis=-1).sum())
 
 
+# a power of two: with P spp the z-Sobol sample index runs over [0, 2^floor(log2 P)) per pixel, and indices
+# beyond it would reuse the next pixel's points (sobol.h:143-146), correlating the passes
+P = 512
+
+
+@pytest.fixture(scope="module")
+def plain_sums(ks, device, room):
+    """Per-pass film sums of two plain renders of P spp with seeds 11 and 21 (independent of each other, of
+    the reference's seed 0 and of the guided renders' seed 7)."""
+    hs, sc = room
+    out = {}
+    for seed in (11, 21):
+        f = ks.Film(device, W, H)
+        sums = []
+        ks.small_pt_render(sc, hs.camera, _params(ks, P, rng_seed=seed), f,
+                           progress=lambda ff, s: sums.append(ff.download()[0].copy()))
+        f.close()
+        out[seed] = sums
+    return out
+
+
 @pytest.mark.parametrize("fp8,max_bounce", [(0, 0), (1, 0), (0, 1)])
-def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
-    """max_bounce 1: only the camera rays' hits are guided; deeper vertices sample the BSDF alone. A plain
-    render through the same statistic is the control: an unbiased estimator that must pass the same bars."""
+def test_guided_is_unbiased(ks, device, room, plain_sums, fp8, max_bounce):
+    """max_bounce 1: only the camera rays' hits are guided; deeper vertices sample the BSDF alone. Two
+    independent plain renders through the same statistic are the control: unbiased estimators that must
+    pass the same bars."""
     hs, sc = room
     ref = ks.Film(device, W, H)
     ks.small_pt_render(sc, hs.camera, _params(ks, 8192, 8192), ref)
@@ -107,18 +137,12 @@ def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
         rgb, _ = f.download()
         sums.append(rgb.copy())
 
-    # a power of two: with P spp the z-Sobol sample index runs over [0, 2^floor(log2 P)) per pixel, and
-    # indices beyond it would reuse the next pixel's points (sobol.h:143-146), correlating the passes
-    P = 512
-    # seeds other than the reference's, so the two sample sets are independent
+    # seeds other than the reference's and the plain renders', so the sample sets are independent
     ks.small_pt_render_guided(sc, hs.camera, _params(ks, P, rng_seed=7), g, film, progress=on_pass)
     assert len(sums) == P
-    z = _block_z(sums, ref_rgb, ref_w, 16)  # passes after the guide has had 16 passes of training
-    plain = ks.Film(device, W, H)
-    psums = []
-    ks.small_pt_render(sc, hs.camera, _params(ks, P, rng_seed=11), plain,
-                       progress=lambda f, s: psums.append(f.download()[0].copy()))
-    zp = _block_z(psums, ref_rgb, ref_w, 16)
+    # passes after the guide has had 16 passes of training, against the plain render
+    z = _block_z(sums, plain_sums[11], 16)
+    zp = _block_z(plain_sums[21], plain_sums[11], 16)
     print("guided max |z| over", z.size, "block-channels:", float(np.abs(z).max()), "blocks beyond 3:",
           _blocks_over3(z), "| plain control:", float(np.abs(zp).max()), _blocks_over3(zp), flush=True)
     assert np.abs(zp).max() < Z_MAX and _blocks_over3(zp) <= Z3_BLOCKS_MAX  # the statistic itself is calibrated
@@ -139,7 +163,6 @@ def test_guided_is_unbiased(ks, device, room, fp8, max_bounce):
     assert abs(r - r_full) <= 1e-6 + 1e-3 * r_full  # the device's FP64 reduction of the same film
     g.close()
     film.close()
-    plain.close()
     ref.close()
 
 
