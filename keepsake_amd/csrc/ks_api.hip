@@ -306,8 +306,11 @@ __global__ void k_iter_done(Ctrl *c) {
 
 // Closest hit for n rays (count read from *n_dev when given). Grid-stride over rays; one LDS stack
 // column per lane. Ray, node and triangle counts go to the block's row of stat_rows.
+// The traversal kernels run at 8 waves per SIMD (64 VGPRs). With ALPHA they call the opacity test, and a
+// call within 64 VGPRs spilled the traversal loop's state to scratch on every node visit; they are held
+// to 4 waves per SIMD (128 VGPRs) instead, with no spills.
 template <int W, bool ALPHA>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(ALPHA ? 4 : 8))) void k_closest(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                          const float4 *ro, const float4 *rd, float4 *hit,
                                                          uint32_t *ovf, uint32_t ovf_cap,
                                                          unsigned long long *stat_rows) {
@@ -361,7 +364,7 @@ KS_DEV void add_nee(uint32_t i, const uint32_t *target, const float4 *contrib, f
 
 // Any hit: the occlusion flag of each of n rays (ks_occluded).
 template <int W, bool ALPHA>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(ALPHA ? 4 : 8))) void k_anyhit(DevScene sc, const uint32_t *n_dev, uint32_t n_fixed,
                                                         const float4 *ro, const float4 *rd, uint8_t *occ,
                                                         uint32_t *ovf, uint32_t ovf_cap,
                                                         unsigned long long *stat_rows) {
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))
 // the slowest rays set. Node and triangle counts are kept per ray and added to the block's LDS totals of
 // the ray's kind when it ends (no per-kind registers in the loop); block 0 adds the ray counts.
 template <int W, bool ALPHA>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_trace(
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(ALPHA ? 4 : 8))) void k_trace(
     DevScene sc, const uint32_t *nc_dev, const float4 *ro, const float4 *rd, float4 *hit, const uint32_t *ns_dev,
     const float4 *sro, const float4 *srd, const uint32_t *target, const float4 *contrib, float4 *path_L, float4 *samp,
     float4 *recs, uint32_t *ovf, uint32_t ovf_cap, unsigned long long *stat_rows) {
@@ -1463,9 +1466,12 @@ template <typename... A> void launch_shade(bool tex, bool guided, bool batch, bo
 
 constexpr int kMaxTraceBlocksPerCu = 8; // the traversal overflow area is sized for this many
 
-int trace_grid(const ks_device *dev, uint32_t n, int blocks_per_cu = kMaxTraceBlocksPerCu) {
-    // 8 blocks of 256 lanes per CU keep every SIMD at 8 waves; the grid-stride loop covers the rest
-    uint32_t max_blocks = (uint32_t)std::max(1, dev->n_cu) * (uint32_t)std::min(blocks_per_cu, kMaxTraceBlocksPerCu);
+int trace_grid(const ks_device *dev, const DevScene &ds, uint32_t n, int blocks_per_cu = kMaxTraceBlocksPerCu) {
+    // 8 blocks of 256 lanes per CU keep every SIMD at 8 waves; the grid-stride loop covers the rest. The
+    // alpha-tested kernels run at 4 waves per SIMD (their opacity call needs 128 VGPRs), so 4 blocks per CU
+    // are resident and the grid stays at that.
+    const int cap = ds.alpha ? kMaxTraceBlocksPerCu / 2 : kMaxTraceBlocksPerCu;
+    uint32_t max_blocks = (uint32_t)std::max(1, dev->n_cu) * (uint32_t)std::max(1, std::min(blocks_per_cu, cap));
     uint32_t need = (n + kTraceBlock - 1) / kTraceBlock;
     return (int)std::max(1u, std::min(max_blocks, need));
 }
@@ -1632,7 +1638,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         pb[b] = PathBufs{ws.work[b].as<uint32_t>(), ws.state[b].as<uint32_t>(), ws.beta[b].as<float4>(),
                          ws.L[b].as<float4>(),     ws.ro[b].as<float4>(),     ws.rd[b].as<float4>()};
     const int tblocks = (int)((Q + 255) / 256);
-    const int tgrid = trace_grid(dev, Q, wo.blocks_per_cu);
+    const int tgrid = trace_grid(dev, sc->ds, Q, wo.blocks_per_cu);
     const bool graph = wo.graph && !gp;
     // One iteration reading buffer cur and writing buffer 1 - cur; no events while a graph is captured.
     // Its traversal also traces the previous shading pass's shadow rays (k_trace): their contributions go
@@ -3341,7 +3347,7 @@ static int ray_query(ks_scene *scene, const float *rays, int64_t n, float *hits,
     int e = ensure_workspace(dev, 256, 1, scene->max_stack);
     if (e) return e;
     clear_stats(dev, s);
-    int grid = trace_grid(dev, (uint32_t)n);
+    int grid = trace_grid(dev, scene->ds, (uint32_t)n);
     DevBuf out;
     if (hits) {
         KS_HIP(out.alloc(sizeof(float4) * n));
