@@ -1466,6 +1466,11 @@ template <typename... A> void launch_shade(bool tex, bool guided, bool batch, bo
 
 constexpr int kMaxTraceBlocksPerCu = 8; // the traversal overflow area is sized for this many
 
+// Per-sample results of one sub-batch (chunk) of a render: 16 B per path. Every chunk ends with a drain of
+// the wavefront (a dozen iterations over the last, longest paths with the GPU mostly idle), so chunks are
+// made as large as this buffer allows: 4 GiB holds a 1920x1080 64-spp render in one chunk.
+constexpr uint64_t kSampleBufferBytes = 4ull << 30;
+
 int trace_grid(const ks_device *dev, const DevScene &ds, uint32_t n, int blocks_per_cu = kMaxTraceBlocksPerCu) {
     // 8 blocks of 256 lanes per CU keep every SIMD at 8 waves; the grid-stride loop covers the rest. The
     // alpha-tested kernels run at 4 waves per SIMD (their opacity call needs 128 VGPRs), so 4 blocks per CU
@@ -2732,7 +2737,7 @@ int ks_small_pt_render(ks_scene *scene, const ks_camera_desc *cam, const ks_smal
         int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
         uint64_t pixels = (uint64_t)rc.cw * rc.ch;
         // per-sample buffer budget: 1 GiB of float4 results per sub-batch
-        const uint64_t max_samp = (1ull << 30) / sizeof(float4);
+        const uint64_t max_samp = kSampleBufferBytes / sizeof(float4);
         int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
         KS_REQUIRE(pixels <= max_samp, "crop window too large for one sub-batch");
         uint32_t Q = pick_queue(params, pixels * (uint64_t)std::min(spp_chunk, params->spp));
@@ -2850,7 +2855,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
         const int interval = params->spp_prog_interval > 0 ? params->spp_prog_interval : 32;
         const uint64_t pixels = (uint64_t)rc.cw * rc.ch;
         KS_REQUIRE(pixels * kRecVertices < (1ull << 31), "crop window too large for guided rendering");
-        const uint64_t max_samp = (1ull << 30) / sizeof(float4);
+        const uint64_t max_samp = kSampleBufferBytes / sizeof(float4);
         const int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
         const uint64_t max_items = pixels * (uint64_t)std::min(spp_chunk, params->spp);
         uint32_t Q = pick_queue(params, max_items);
@@ -3036,7 +3041,7 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
             max_pixels = std::max(max_pixels, P);
         }
         if (stats) std::memset(stats, 0, sizeof(*stats));
-        uint64_t budget = (1ull << 30) / sizeof(float4); // per-sample results of one chunk: 1 GiB
+        uint64_t budget = kSampleBufferBytes / sizeof(float4); // per-sample results of one chunk
         KS_REQUIRE(o.max_chunk_items >= 0, "max_chunk_items must not be negative");
         if (o.max_chunk_items > 0) budget = std::min<uint64_t>(budget, (uint64_t)o.max_chunk_items);
         KS_REQUIRE(max_pixels <= budget, "a view is too large for one chunk");
