@@ -239,15 +239,17 @@ typedef struct ks_small_pt_params {
 #define KS_K_SHADOW 3
 #define KS_K_FILM 4
 #define KS_K_GUIDE 5
-#define KS_K_COUNT 6
+#define KS_K_TAIL 6 /* the last paths of a render, each carried to its end by one thread */
+#define KS_K_COUNT 7
 
 typedef struct ks_render_stats {
     double wall_ms;                     /* host wall time of the render call */
     double kernel_ms[KS_K_COUNT];       /* summed event time per kernel class (0 unless timing on) */
     uint64_t launches[KS_K_COUNT];
     uint64_t paths;                     /* camera samples traced (W*H*spp over the crop) */
-    uint64_t rays_closest;              /* primary + extension rays */
-    uint64_t rays_shadow;               /* NEE shadow rays */
+    uint64_t rays_closest;              /* primary + extension rays of the wavefront's traversal kernel */
+    uint64_t rays_shadow;               /* NEE shadow rays of the wavefront's traversal kernel */
+    uint64_t rays_tail;                 /* closest-hit and shadow rays of the tail kernel (KS_K_TAIL) */
     uint64_t node_visits_closest, tri_tests_closest;
     uint64_t node_visits_shadow, tri_tests_shadow;
     uint64_t nonfinite;                 /* non-finite throughput events (thread_monitor_check, small_pt.cpp:97) */
@@ -274,6 +276,12 @@ int ks_device_set_timing(ks_device *dev, int32_t enable);
  * shades each class with a kernel compiled for its BSDF lobes. Results are the same either way; textured
  * scenes always use one kernel. */
 int ks_device_set_shading(ks_device *dev, int32_t sorted);
+/* The end of a render's wavefront (no counterpart in the reference, whose CPU loop traces each path to its
+ * end): once every work item has started and at most max_paths paths remain, one kernel carries each of
+ * them to its end in one thread instead of one wavefront iteration per bounce. Results are the same.
+ * Applies to plain renders (not guided ones, not HIP-graph batches) of scenes without alpha-tested
+ * triangles; 0 turns it off. Default 2^19. */
+int ks_device_set_tail(ks_device *dev, int32_t max_paths);
 
 /* ---- scene: replaces create_scene + Scene::create_rtc_scene (rtcCommitScene, scene.cpp:144-160),
  *      Scene::build_mesh_lights (scene.cpp:315-336) and PowerLightSampler::build (light.cpp:636-677).

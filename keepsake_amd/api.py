@@ -36,7 +36,7 @@ KS_MIS_CONSISTENT = 1
 KS_GUIDE_OFF = 0
 KS_GUIDE_TRAIN_AND_SAMPLE = 1
 
-KERNEL_CLASSES = ("raygen", "closest", "shade", "shadow", "film", "guide")
+KERNEL_CLASSES = ("raygen", "closest", "shade", "shadow", "film", "guide", "tail")
 
 
 class KsError(RuntimeError):
@@ -285,11 +285,12 @@ class SmallPTParams(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [
         ("wall_ms", C.c_double),
-        ("kernel_ms", C.c_double * 6),
-        ("launches", C.c_uint64 * 6),
+        ("kernel_ms", C.c_double * 7),
+        ("launches", C.c_uint64 * 7),
         ("paths", C.c_uint64),
         ("rays_closest", C.c_uint64),
         ("rays_shadow", C.c_uint64),
+        ("rays_tail", C.c_uint64),
         ("node_visits_closest", C.c_uint64),
         ("tri_tests_closest", C.c_uint64),
         ("node_visits_shadow", C.c_uint64),
@@ -379,6 +380,7 @@ _SIGS = {
     "ks_device_close": ([_P], C.c_int),
     "ks_device_set_timing": ([_P, C.c_int32], C.c_int),
     "ks_device_set_shading": ([_P, C.c_int32], C.c_int),
+    "ks_device_set_tail": ([_P, C.c_int32], C.c_int),
     "ks_scene_create": ([_P, C.POINTER(SceneDesc), C.POINTER(BvhOptions), C.POINTER(_P)], C.c_int),
     "ks_scene_free": ([_P], C.c_int),
     "ks_scene_info_get": ([_P, C.POINTER(SceneInfo)], C.c_int),
@@ -703,6 +705,10 @@ class Device:
     def set_shading(self, sorted: bool) -> None:
         """Material-sorted shading, or one shading kernel over every path (the default)."""
         _check(self.lib.ks_device_set_shading(self.h, int(sorted)))
+
+    def set_tail(self, max_paths: int) -> None:
+        """The render's last max_paths paths (or fewer) go to the tail kernel (0: off)."""
+        _check(self.lib.ks_device_set_tail(self.h, int(max_paths)))
 
     def close(self) -> None:
         if self.h:

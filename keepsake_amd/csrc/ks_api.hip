@@ -233,13 +233,13 @@ __global__ void k_fill_u64(unsigned long long *p, int n) {
 // Render statistics: row 0 takes the shading kernels' atomics, row 1 the totals, and rows 2 .. 2 + R
 // the traversal kernels' per-block counts (block_add_stats). One block of 256 threads sums the R
 // partial rows and row 0 into row 1.
-static_assert(256 % kStatCount == 0, "k_stat_reduce splits 256 threads into groups of kStatCount");
 __global__ __launch_bounds__(256) void k_stat_reduce(unsigned long long *stats, int R) {
     __shared__ unsigned long long s_sum[256];
     const int k = threadIdx.x % kStatCount, part = threadIdx.x / kStatCount;
-    constexpr int kParts = 256 / kStatCount;
+    constexpr int kParts = 256 / kStatCount; // threads beyond kParts * kStatCount add nothing
     unsigned long long x = 0;
-    for (int r = part; r < R; r += kParts) x += stats[(size_t)(2 + r) * kStatCount + k];
+    if (part < kParts)
+        for (int r = part; r < R; r += kParts) x += stats[(size_t)(2 + r) * kStatCount + k];
     s_sum[threadIdx.x] = x;
     __syncthreads();
     if (threadIdx.x < kStatCount) {
@@ -254,6 +254,7 @@ __global__ void k_ctrl_init(Ctrl *c, uint64_t work_end) {
     c->n_next.v = 0;
     c->n_shadow.v = 0;
     c->n_shadow_trace = 0;
+    c->none = 0;
     c->iteration = 0;
     c->work_next = 0;
     c->work_end = work_end;
@@ -613,6 +614,243 @@ KS_DEV float mix_pdf(const float4 *m, f3 w) {
     return p;
 }
 
+// One vertex of SmallPT::trace (small_pt.cpp:79-176) for the path whose state the caller loaded: work
+// item w, state (Sobol dimension, bounce), throughput beta, pdf of the last BSDF sample, radiance L, the
+// ray direction rd that reached the vertex and its hit h (t, u, v, prim; prim < 0: escaped). Adds
+// emission to L, samples NEE (the shadow ray and its contribution, shadow = true) and the continuation
+// (new_o, new_d, updated beta and pdf; survive = true), or ends the path, writing its sample result.
+// i is the path's slot, used for the guide's mixture and training record.
+template <bool GUIDED, bool BATCH, bool TEX, int CLASS>
+KS_DEV void shade_vertex(const DevScene &sc, const RenderConsts &rc, const GuideShade &gs, float4 *samp, uint32_t i,
+                         float4 h, f3 rd, uint32_t w, uint32_t state, f3 &L, f3 &beta, float &pdf_wi_bsdf,
+                         bool &survive, bool &shadow, f3 &new_o, f3 &new_d, f3 &sh_origin, f3 &sh_dir,
+                         f3 &sh_contrib, float &sh_tmax, uint32_t &nonfinite, int &bounce, Sobol &sob,
+                         uint32_t &rec_slot, uint32_t &view) {
+    constexpr unsigned LB = CLASS == kShadeDielectric ? kLobesOpaqueDielectric
+                            : CLASS == kShadeMetal     ? kLobesMetal
+                                                       : kLobesAll;
+    int32_t prim = __float_as_int(h.w);
+    bounce = (int)(state >> 16) & 0xff;
+    uint32_t x, y, s;
+    if (BATCH) view = locate_batch(rc, w, x, y, s);
+    else pixel_of(rc, w, x, y, s);
+    const RenderConsts &vr = BATCH ? rc.views[view] : rc;
+    sobol_start(vr, sob, x, y, s, (int)(state & 0xffffu));
+    bool done = false;
+    bool miss_sample = false;
+
+    if (prim < 0) {
+      if constexpr (CLASS == kShadeUnsorted || CLASS == kShadeMiss) {
+        f3 wi = normalized(rd);
+        for (int k = 0; k < sc.n_sky; ++k) { // small_pt.cpp:85-109
+            const Light &l = sc.lights[sc.sky[k]];
+            const bool map = l.type == 4;
+            f3 Le = mk(l.color[0], l.color[1], l.color[2]);
+            if (map) {
+                float e[3];
+                sky_eval(sc.sky_map, wi.x, wi.y, wi.z, e);
+                Le = mk(e[0], e[1], e[2]);
+            }
+            if (bounce == 0) {
+                if (vr.include_background) L = L + beta * Le;
+            } else {
+                float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)sc.sky[k]);
+                float lpdf = map ? sky_map_pdf(sc.sky_map, wi.x, wi.y, wi.z) : sky_pdf(wi);
+                float mis = power_heur(pdf_wi_bsdf, lpdf * pr);
+                f3 bu = Le * mis;
+                if (!all_finite(bu)) ++nonfinite;
+                if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
+                else if (bounce >= 2 && vr.clamp_indirect > 0.0f) bu = vmin(bu, vr.clamp_indirect);
+                L = L + beta * bu;
+            }
+        }
+        if (bounce == 0 && !vr.include_background) miss_sample = true;
+      }
+        done = true;
+    } else {
+      if constexpr (CLASS != kShadeMiss) {
+        uint4 tri = sc.tri_idx[prim];
+        const uint32_t mid = tri_material(tri);
+        const Material &m = sc.mats[mid];
+        SurfHit it;
+        SurfUV su;
+        const DevTexMaterial *tm = nullptr;
+        if (TEX) {
+            // compute_intersection with uv partials, then the normal map (Scene::intersect1, scene.cpp:222-233)
+            it = make_hit_uv(sc, tri, h.y, h.z, rd, su);
+            const int32_t tmi = sc.tex_mat_of ? sc.tex_mat_of[mid] : -1;
+            if (tmi >= 0) {
+                tm = &sc.tex_mats[tmi];
+                if (bounce == 0) {
+                    f3 co, rxd, ryd;
+                    camera_ray_diffs(vr, x, y, s, co, rxd, ryd);
+                    uv_partials(it, su, co, rxd, ryd);
+                }
+                if (tm->has_normal) apply_normal_map(tex_context(sc), *tm, su.u, su.v, it);
+            }
+        } else {
+            it = make_hit(sc, tri, h.y, h.z, rd);
+        }
+        if (m.has_emission) { // small_pt.cpp:111-138
+            int32_t li = sc.tri_light[prim];
+            if (li >= 0) {
+                const Light &l = sc.lights[li];
+                float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)li);
+                f3 Le = mk(l.color[0], l.color[1], l.color[2]);
+                if (TEX && tm && l.color[3] != 0.0f) { // MeshTriLight::eval at the hit (light.cpp:560-565)
+                    float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                    field_eval(tex_context(sc), tm->f[kFieldEmission], su.u, su.v, su.d, 3, e);
+                    Le = mk(e[0], e[1], e[2]);
+                }
+                if (bounce == 0) {
+                    L = L + beta * Le;
+                } else if (pr > 0.0f) {
+                    float pdf_l = mesh_light_pdf(sc, l, normalized(rd), h.x);
+                    float mis = power_heur(pdf_wi_bsdf, pdf_l * pr);
+                    f3 bu = Le * mis;
+                    if (!all_finite(bu)) ++nonfinite;
+                    if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
+                    else if (bounce >= 2 && vr.clamp_indirect > 0.0f) bu = vmin(bu, vr.clamp_indirect);
+                    L = L + beta * bu;
+                }
+            }
+        }
+        if (bounce == vr.bounces) {
+            done = true;
+        } else {
+            f3 wo = -normalized(rd);
+            Closure c;
+            if (TEX && tm) {
+                c = closure_from_fields(tex_context(sc), *tm, su.u, su.v, su.d);
+            } else {
+                c.basecolor = mk(m.basecolor[0], m.basecolor[1], m.basecolor[2]);
+                c.roughness = m.roughness;
+                c.ax = m.ax;
+                c.ay = m.ay;
+                c.metallic = m.metallic;
+                c.ior = m.ior;
+                c.specular_r0_mul = m.specular_r0_mul;
+                c.specular_trans = m.specular_trans;
+                c.diffuse_trans = m.diffuse_trans;
+                c.diffuse_trans_fwd = m.diffuse_trans_fwd;
+            }
+            const float4 *mixp = GUIDED && gs.mix ? gs.mix + 10 * (size_t)i : nullptr;
+            const float alpha_m =
+                GUIDED && mixp && c.roughness >= gs.min_roughness && bounce < gs.max_bounce ? gs.alpha : 0.0f;
+
+            // NEE (nee.cpp:11-58)
+            if (sc.n_lights > 0) {
+                float pr_light;
+                uint32_t li = cdf_sample(sc.cdf, sc.n_lights, sob.next(), pr_light);
+                const Light &l = sc.lights[li];
+                bool delta_light = l.type == 1 || l.type == 2;
+                float u0, u1;
+                sob.next2d(u0, u1);
+                f3 wi;
+                float wi_dist = 0.0f, pdf_l = 0.0f;
+                f3 Le_beta = light_sample<TEX>(sc, l, it.p, u0, u1, wi, wi_dist, pdf_l);
+                if (!is_zero(Le_beta) && pdf_l > 0.0f) {
+                    f3 wol = sh_to_local(it, wo), wil = sh_to_local(it, wi);
+                    f3 f = bsdf_eval<LB>(wol, wil, c);
+                    float pdf_b = is_zero(f) ? 0.0f : bsdf_pdf<LB>(wol, wil, c);
+                    if (!is_zero(f) && pdf_b > 0.0f) {
+                        if (GUIDED && alpha_m > 0.0f) pdf_b = alpha_m * mix_pdf(mixp, wi) + (1.0f - alpha_m) * pdf_b;
+                        float mis = 1.0f;
+                        if (!delta_light)
+                            mis = vr.mis_mode == KS_MIS_CONSISTENT ? power_heur(pdf_l * pr_light, pdf_b)
+                                                                   : power_heur(pdf_l, pdf_b);
+                        f3 Ld = Le_beta * f * mis / pr_light;
+                        if (!all_finite(Ld)) ++nonfinite;
+                        if (bounce == 0 && vr.clamp_direct > 0.0f) Ld = vmin(Ld, vr.clamp_direct);
+                        else if (bounce >= 1 && vr.clamp_indirect > 0.0f) Ld = vmin(Ld, vr.clamp_indirect);
+                        shadow = true;
+                        sh_origin = spawn_origin(it.p, wi, it.n);
+                        sh_dir = wi;
+                        sh_tmax = wi_dist;
+                        sh_contrib = beta * Ld;
+                    }
+                }
+            } else {
+                (void)sob.next();
+                float a, b;
+                sob.next2d(a, b);
+            }
+            // BSDF sampling (material.cpp:157-166)
+            f3 wol = sh_to_local(it, wo);
+            float u_lobe = sob.next();
+            float u0, u1;
+            sob.next2d(u0, u1);
+            f3 wil;
+            float pdf_wi = 0.0f;
+            f3 sbeta, wi_g = mk(0, 0, 0);
+            const f3 beta_k = beta;
+            if (GUIDED && alpha_m > 0.0f) {
+                // one-sample MIS: u_lobe picks the strategy and is re-used inside it
+                float pb = 0.0f;
+                f3 fcos;
+                if (u_lobe < alpha_m) {
+                    float4 wa = mixp[0], wb = mixp[1];
+                    float wk[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+                    float ur;
+                    int sel = mixture_select(wk, u0, &ur);
+                    float4 l4 = mixp[2 + sel];
+                    Lobe lobe;
+                    lobe.mu = ld3(l4);
+                    lobe.kappa = l4.w;
+                    wi_g = vmf_sample(lobe, ur, u1);
+                    wil = sh_to_local(it, wi_g);
+                    fcos = bsdf_eval<LB>(wol, wil, c);
+                    pb = is_zero(fcos) ? 0.0f : bsdf_pdf<LB>(wol, wil, c);
+                } else {
+                    float ub = fminf((u_lobe - alpha_m) / (1.0f - alpha_m), kBeforeOne);
+                    f3 sb = bsdf_sample<LB>(wol, wil, c, ub, u0, u1, pb);
+                    fcos = sb * pb;
+                    wi_g = sh_to_world(it, wil);
+                }
+                float q = mix_pdf(mixp, wi_g);
+                pdf_wi = alpha_m * q + (1.0f - alpha_m) * pb;
+                sbeta = pdf_wi > 0.0f ? fcos / pdf_wi : mk(0, 0, 0);
+            } else {
+                sbeta = bsdf_sample<LB>(wol, wil, c, u_lobe, u0, u1, pdf_wi);
+            }
+            if (!all_finite(sbeta)) ++nonfinite;
+            if (maxc(sbeta) == 0.0f || pdf_wi == 0.0f) {
+                done = true;
+            } else {
+                f3 wi = GUIDED && alpha_m > 0.0f ? wi_g : sh_to_world(it, wil);
+                beta = beta * sbeta;
+                if (maxc(beta) == 0.0f) {
+                    done = true;
+                } else {
+                    pdf_wi_bsdf = pdf_wi;
+                    new_o = spawn_origin(it.p, wi, it.n);
+                    new_d = wi;
+                    if (maxc(beta) < 1.0f && bounce >= 1) { // Russian roulette (small_pt.cpp:167-174)
+                        float q = fmaxf(0.0f, 1.0f - maxc(beta));
+                        if (sob.next() < q) done = true;
+                        else beta = beta / (1.0f - q);
+                    }
+                    if (GUIDED && gs.recs && !done && bounce < kRecVertices) {
+                        uint32_t slot = w * (uint32_t)kRecVertices + (uint32_t)bounce;
+                        float4 *r = gs.recs + (size_t)slot * kRecFloat4;
+                        r[0] = make_float4(it.p.x, it.p.y, it.p.z, __uint_as_float(gs.stamp));
+                        r[1] = make_float4(it.n.x, it.n.y, it.n.z, 0.0f);
+                        r[2] = make_float4(wo.x, wo.y, wo.z, 0.0f);
+                        r[3] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+                        r[4] = make_float4(L.x, L.y, L.z, 0.0f);
+                        r[5] = make_float4(beta_k.x, beta_k.y, beta_k.z, 0.0f);
+                        r[kRecNee] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        rec_slot = slot + 1;
+                    }
+                }
+            }
+        }
+      }
+    }
+    survive = !done;
+    if (done) samp[BATCH ? w : samp_index(rc, w)] = make_float4(L.x, L.y, L.z, miss_sample ? 0.0f : 1.0f);
+}
+
 // Shading of the current buffer (small_pt.cpp:79-176). GUIDED adds one-sample MIS between the BSDF and
 // the learned mixture (probability gs.alpha of the mixture at vertices whose roughness reaches
 // gs.min_roughness; the continuation pdf is alpha q + (1 - alpha) p_bsdf in the estimator and in both
@@ -629,9 +867,6 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
                                                uint32_t *sh_tgt, float4 *sh_c, float4 *samp,
                                                unsigned long long *stats, GuideShade gs, const uint32_t *order,
                                                uint32_t Q) {
-    constexpr unsigned LB = CLASS == kShadeDielectric ? kLobesOpaqueDielectric
-                            : CLASS == kShadeMetal     ? kLobesMetal
-                                                       : kLobesAll;
     const uint32_t n = CLASS == kShadeUnsorted ? ctrl->n_active : ctrl->n_class[CLASS].v;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x * blockDim.x >= n) return; // whole block idle
@@ -660,229 +895,9 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         beta = ld3(b4);
         pdf_wi_bsdf = b4.w;
         L = ld3(cur.L[i]);
-        float4 rd4 = cur.rd[i];
-        f3 rd = ld3(rd4);
-        float4 h = hits[i];
-        int32_t prim = __float_as_int(h.w);
-        bounce = (int)(state >> 16) & 0xff;
-        uint32_t x, y, s;
-        if (BATCH) view = locate_batch(rc, w, x, y, s);
-        else pixel_of(rc, w, x, y, s);
-        const RenderConsts &vr = BATCH ? rc.views[view] : rc;
-        sobol_start(vr, sob, x, y, s, (int)(state & 0xffffu));
-        bool done = false;
-        bool miss_sample = false;
-
-        if (prim < 0) {
-          if constexpr (CLASS == kShadeUnsorted || CLASS == kShadeMiss) {
-            f3 wi = normalized(rd);
-            for (int k = 0; k < sc.n_sky; ++k) { // small_pt.cpp:85-109
-                const Light &l = sc.lights[sc.sky[k]];
-                const bool map = l.type == 4;
-                f3 Le = mk(l.color[0], l.color[1], l.color[2]);
-                if (map) {
-                    float e[3];
-                    sky_eval(sc.sky_map, wi.x, wi.y, wi.z, e);
-                    Le = mk(e[0], e[1], e[2]);
-                }
-                if (bounce == 0) {
-                    if (vr.include_background) L = L + beta * Le;
-                } else {
-                    float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)sc.sky[k]);
-                    float lpdf = map ? sky_map_pdf(sc.sky_map, wi.x, wi.y, wi.z) : sky_pdf(wi);
-                    float mis = power_heur(pdf_wi_bsdf, lpdf * pr);
-                    f3 bu = Le * mis;
-                    if (!all_finite(bu)) ++nonfinite;
-                    if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
-                    else if (bounce >= 2 && vr.clamp_indirect > 0.0f) bu = vmin(bu, vr.clamp_indirect);
-                    L = L + beta * bu;
-                }
-            }
-            if (bounce == 0 && !vr.include_background) miss_sample = true;
-          }
-            done = true;
-        } else {
-          if constexpr (CLASS != kShadeMiss) {
-            uint4 tri = sc.tri_idx[prim];
-            const uint32_t mid = tri_material(tri);
-            const Material &m = sc.mats[mid];
-            SurfHit it;
-            SurfUV su;
-            const DevTexMaterial *tm = nullptr;
-            if (TEX) {
-                // compute_intersection with uv partials, then the normal map (Scene::intersect1, scene.cpp:222-233)
-                it = make_hit_uv(sc, tri, h.y, h.z, rd, su);
-                const int32_t tmi = sc.tex_mat_of ? sc.tex_mat_of[mid] : -1;
-                if (tmi >= 0) {
-                    tm = &sc.tex_mats[tmi];
-                    if (bounce == 0) {
-                        f3 co, rxd, ryd;
-                        camera_ray_diffs(vr, x, y, s, co, rxd, ryd);
-                        uv_partials(it, su, co, rxd, ryd);
-                    }
-                    if (tm->has_normal) apply_normal_map(tex_context(sc), *tm, su.u, su.v, it);
-                }
-            } else {
-                it = make_hit(sc, tri, h.y, h.z, rd);
-            }
-            if (m.has_emission) { // small_pt.cpp:111-138
-                int32_t li = sc.tri_light[prim];
-                if (li >= 0) {
-                    const Light &l = sc.lights[li];
-                    float pr = light_probability(sc.cdf, sc.n_lights, (uint32_t)li);
-                    f3 Le = mk(l.color[0], l.color[1], l.color[2]);
-                    if (TEX && tm && l.color[3] != 0.0f) { // MeshTriLight::eval at the hit (light.cpp:560-565)
-                        float e[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-                        field_eval(tex_context(sc), tm->f[kFieldEmission], su.u, su.v, su.d, 3, e);
-                        Le = mk(e[0], e[1], e[2]);
-                    }
-                    if (bounce == 0) {
-                        L = L + beta * Le;
-                    } else if (pr > 0.0f) {
-                        float pdf_l = mesh_light_pdf(sc, l, normalized(rd), h.x);
-                        float mis = power_heur(pdf_wi_bsdf, pdf_l * pr);
-                        f3 bu = Le * mis;
-                        if (!all_finite(bu)) ++nonfinite;
-                        if (bounce == 1 && vr.clamp_direct > 0.0f) bu = vmin(bu, vr.clamp_direct);
-                        else if (bounce >= 2 && vr.clamp_indirect > 0.0f) bu = vmin(bu, vr.clamp_indirect);
-                        L = L + beta * bu;
-                    }
-                }
-            }
-            if (bounce == vr.bounces) {
-                done = true;
-            } else {
-                f3 wo = -normalized(rd);
-                Closure c;
-                if (TEX && tm) {
-                    c = closure_from_fields(tex_context(sc), *tm, su.u, su.v, su.d);
-                } else {
-                    c.basecolor = mk(m.basecolor[0], m.basecolor[1], m.basecolor[2]);
-                    c.roughness = m.roughness;
-                    c.ax = m.ax;
-                    c.ay = m.ay;
-                    c.metallic = m.metallic;
-                    c.ior = m.ior;
-                    c.specular_r0_mul = m.specular_r0_mul;
-                    c.specular_trans = m.specular_trans;
-                    c.diffuse_trans = m.diffuse_trans;
-                    c.diffuse_trans_fwd = m.diffuse_trans_fwd;
-                }
-                const float4 *mixp = GUIDED && gs.mix ? gs.mix + 10 * (size_t)i : nullptr;
-                const float alpha_m =
-                    GUIDED && mixp && c.roughness >= gs.min_roughness && bounce < gs.max_bounce ? gs.alpha : 0.0f;
-
-                // NEE (nee.cpp:11-58)
-                if (sc.n_lights > 0) {
-                    float pr_light;
-                    uint32_t li = cdf_sample(sc.cdf, sc.n_lights, sob.next(), pr_light);
-                    const Light &l = sc.lights[li];
-                    bool delta_light = l.type == 1 || l.type == 2;
-                    float u0, u1;
-                    sob.next2d(u0, u1);
-                    f3 wi;
-                    float wi_dist = 0.0f, pdf_l = 0.0f;
-                    f3 Le_beta = light_sample<TEX>(sc, l, it.p, u0, u1, wi, wi_dist, pdf_l);
-                    if (!is_zero(Le_beta) && pdf_l > 0.0f) {
-                        f3 wol = sh_to_local(it, wo), wil = sh_to_local(it, wi);
-                        f3 f = bsdf_eval<LB>(wol, wil, c);
-                        float pdf_b = is_zero(f) ? 0.0f : bsdf_pdf<LB>(wol, wil, c);
-                        if (!is_zero(f) && pdf_b > 0.0f) {
-                            if (GUIDED && alpha_m > 0.0f) pdf_b = alpha_m * mix_pdf(mixp, wi) + (1.0f - alpha_m) * pdf_b;
-                            float mis = 1.0f;
-                            if (!delta_light)
-                                mis = vr.mis_mode == KS_MIS_CONSISTENT ? power_heur(pdf_l * pr_light, pdf_b)
-                                                                       : power_heur(pdf_l, pdf_b);
-                            f3 Ld = Le_beta * f * mis / pr_light;
-                            if (!all_finite(Ld)) ++nonfinite;
-                            if (bounce == 0 && vr.clamp_direct > 0.0f) Ld = vmin(Ld, vr.clamp_direct);
-                            else if (bounce >= 1 && vr.clamp_indirect > 0.0f) Ld = vmin(Ld, vr.clamp_indirect);
-                            shadow = true;
-                            sh_origin = spawn_origin(it.p, wi, it.n);
-                            sh_dir = wi;
-                            sh_tmax = wi_dist;
-                            sh_contrib = beta * Ld;
-                        }
-                    }
-                } else {
-                    (void)sob.next();
-                    float a, b;
-                    sob.next2d(a, b);
-                }
-                // BSDF sampling (material.cpp:157-166)
-                f3 wol = sh_to_local(it, wo);
-                float u_lobe = sob.next();
-                float u0, u1;
-                sob.next2d(u0, u1);
-                f3 wil;
-                float pdf_wi = 0.0f;
-                f3 sbeta, wi_g = mk(0, 0, 0);
-                const f3 beta_k = beta;
-                if (GUIDED && alpha_m > 0.0f) {
-                    // one-sample MIS: u_lobe picks the strategy and is re-used inside it
-                    float pb = 0.0f;
-                    f3 fcos;
-                    if (u_lobe < alpha_m) {
-                        float4 wa = mixp[0], wb = mixp[1];
-                        float wk[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
-                        float ur;
-                        int sel = mixture_select(wk, u0, &ur);
-                        float4 l4 = mixp[2 + sel];
-                        Lobe lobe;
-                        lobe.mu = ld3(l4);
-                        lobe.kappa = l4.w;
-                        wi_g = vmf_sample(lobe, ur, u1);
-                        wil = sh_to_local(it, wi_g);
-                        fcos = bsdf_eval<LB>(wol, wil, c);
-                        pb = is_zero(fcos) ? 0.0f : bsdf_pdf<LB>(wol, wil, c);
-                    } else {
-                        float ub = fminf((u_lobe - alpha_m) / (1.0f - alpha_m), kBeforeOne);
-                        f3 sb = bsdf_sample<LB>(wol, wil, c, ub, u0, u1, pb);
-                        fcos = sb * pb;
-                        wi_g = sh_to_world(it, wil);
-                    }
-                    float q = mix_pdf(mixp, wi_g);
-                    pdf_wi = alpha_m * q + (1.0f - alpha_m) * pb;
-                    sbeta = pdf_wi > 0.0f ? fcos / pdf_wi : mk(0, 0, 0);
-                } else {
-                    sbeta = bsdf_sample<LB>(wol, wil, c, u_lobe, u0, u1, pdf_wi);
-                }
-                if (!all_finite(sbeta)) ++nonfinite;
-                if (maxc(sbeta) == 0.0f || pdf_wi == 0.0f) {
-                    done = true;
-                } else {
-                    f3 wi = GUIDED && alpha_m > 0.0f ? wi_g : sh_to_world(it, wil);
-                    beta = beta * sbeta;
-                    if (maxc(beta) == 0.0f) {
-                        done = true;
-                    } else {
-                        pdf_wi_bsdf = pdf_wi;
-                        new_o = spawn_origin(it.p, wi, it.n);
-                        new_d = wi;
-                        if (maxc(beta) < 1.0f && bounce >= 1) { // Russian roulette (small_pt.cpp:167-174)
-                            float q = fmaxf(0.0f, 1.0f - maxc(beta));
-                            if (sob.next() < q) done = true;
-                            else beta = beta / (1.0f - q);
-                        }
-                        if (GUIDED && gs.recs && !done && bounce < kRecVertices) {
-                            uint32_t slot = w * (uint32_t)kRecVertices + (uint32_t)bounce;
-                            float4 *r = gs.recs + (size_t)slot * kRecFloat4;
-                            r[0] = make_float4(it.p.x, it.p.y, it.p.z, __uint_as_float(gs.stamp));
-                            r[1] = make_float4(it.n.x, it.n.y, it.n.z, 0.0f);
-                            r[2] = make_float4(wo.x, wo.y, wo.z, 0.0f);
-                            r[3] = make_float4(wi.x, wi.y, wi.z, 0.0f);
-                            r[4] = make_float4(L.x, L.y, L.z, 0.0f);
-                            r[5] = make_float4(beta_k.x, beta_k.y, beta_k.z, 0.0f);
-                            r[kRecNee] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                            rec_slot = slot + 1;
-                        }
-                    }
-                }
-            }
-          }
-        }
-        survive = !done;
-        if (done) samp[BATCH ? w : samp_index(rc, w)] = make_float4(L.x, L.y, L.z, miss_sample ? 0.0f : 1.0f);
+        shade_vertex<GUIDED, BATCH, TEX, CLASS>(sc, rc, gs, samp, i, hits[i], ld3(cur.rd[i]), w, state, L, beta,
+                                                pdf_wi_bsdf, survive, shadow, new_o, new_d, sh_origin, sh_dir,
+                                                sh_contrib, sh_tmax, nonfinite, bounce, sob, rec_slot, view);
     }
 
     __shared__ uint32_t s_app[5];
@@ -903,6 +918,91 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
         sh_c[k] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, __uint_as_float(rec_slot));
     }
     wave_add_stat(stats, kNonFinite, nonfinite);
+}
+
+// The last paths of a render, each carried to its end by one thread: closest hit, shading, the NEE shadow
+// ray and the next bounce, in a loop. Once every work item has started and few paths remain, a wavefront
+// iteration costs about the time of its slowest ray, a dozen times over; here a thread waits only for its
+// own rays. The results are the wavefront's: the same traversal hits (traverse_w, the per-ray form of
+// trace_rays_kind's loop), the same shading (shade_vertex), and the NEE contribution added to the path's
+// radiance, or to its sample result once the path has ended, before the next vertex, with the same
+// single addition the wavefront's traversal makes (add_nee). Plain renders of scenes without alpha-tested
+// triangles only; n paths of buffer cur, n no larger than the overflow area's lanes.
+template <int W, bool BATCH, bool TEX>
+__global__ __launch_bounds__(256) void k_tail(DevScene sc, RenderConsts rc, PathBufs cur, const uint32_t *n_dev,
+                                              float4 *samp, uint32_t *ovf, uint32_t ovf_cap,
+                                              unsigned long long *stat_rows) {
+    __shared__ uint32_t s_stack[kStackLds * 256];
+    __shared__ uint32_t s_stat[3 * 4];
+    const uint32_t n = *n_dev;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n) return; // whole block idle
+    uint32_t n_rays = 0, nonfinite = 0, spills = 0;
+    if (i < n) {
+        TravStack st;
+        st.lds = s_stack + threadIdx.x;
+        st.ovf = ovf + i;
+        st.ovf_stride = gridDim.x * blockDim.x;
+        st.ovf_cap = ovf_cap;
+        st.spills = 0;
+        uint32_t n_nodes = 0, n_tris = 0; // the tail's rays are counted, not their nodes
+        const uint32_t w = cur.work[i];
+        uint32_t state = cur.state[i];
+        const float4 b4 = cur.beta[i];
+        f3 beta = ld3(b4), L = ld3(cur.L[i]);
+        float pdf = b4.w;
+        const float4 o4 = cur.ro[i], d4 = cur.rd[i];
+        f3 o = ld3(o4), d = ld3(d4);
+        float tmin = o4.w, tmax = d4.w;
+        while (true) {
+            TravHit th;
+            traverse_w<false, W>(sc.nodes, sc.tris, make_trav_ray(o, d, tmin), tmax, st, th, n_nodes, n_tris);
+            ++n_rays;
+            const float4 h = th.prim >= 0 ? make_float4(th.t, th.u, th.v, __int_as_float(th.prim))
+                                          : make_float4(-1.0f, 0.0f, 0.0f, __int_as_float(-1));
+            bool survive = false, shadow = false;
+            f3 new_o = mk(0, 0, 0), new_d = mk(0, 0, 0);
+            f3 sh_origin = mk(0, 0, 0), sh_dir = mk(0, 0, 0), sh_contrib = mk(0, 0, 0);
+            float sh_tmax = 0.0f;
+            int bounce = 0;
+            Sobol sob;
+            uint32_t rec_slot = 0, view = 0;
+            shade_vertex<false, BATCH, TEX, kShadeUnsorted>(sc, rc, GuideShade{}, samp, i, h, d, w, state, L, beta,
+                                                            pdf, survive, shadow, new_o, new_d, sh_origin, sh_dir,
+                                                            sh_contrib, sh_tmax, nonfinite, bounce, sob, rec_slot,
+                                                            view);
+            if (shadow) {
+                TravHit sh;
+                const bool occluded = traverse_w<true, W>(sc.nodes, sc.tris, make_trav_ray(sh_origin, sh_dir, 0.0f),
+                                                          sh_tmax, st, sh, n_nodes, n_tris);
+                ++n_rays;
+                if (!occluded) {
+                    if (survive) {
+                        L.x += sh_contrib.x;
+                        L.y += sh_contrib.y;
+                        L.z += sh_contrib.z;
+                    } else {
+                        float4 *dst = &samp[BATCH ? w : samp_index(rc, w)];
+                        float4 v = *dst;
+                        v.x += sh_contrib.x;
+                        v.y += sh_contrib.y;
+                        v.z += sh_contrib.z;
+                        *dst = v;
+                    }
+                }
+            }
+            if (!survive) break;
+            state = (uint32_t)sob.dim | ((uint32_t)(bounce + 1) << 16);
+            o = new_o;
+            d = new_d;
+            tmin = 0.0f;
+            tmax = kInf;
+        }
+        spills = st.spills;
+    }
+    constexpr int slots[3] = {kRaysTail, kNonFinite, kSpills};
+    const uint32_t v[3] = {n_rays, nonfinite, spills};
+    block_add_stats<4>(stat_rows, slots, v, s_stat);
 }
 
 // RenderTarget2::add / add_miss (render_target.cpp:82-107) for every sample of the batch, in sample
@@ -1308,6 +1408,7 @@ struct ks_device {
     uint8_t *host_scratch = nullptr; // pinned, 64 bytes
     double *relmse_scratch = nullptr; // this device's address of `scratch` (hipGetSymbolAddress at open)
     int sorted_shading = 0;           // material-sorted shading of untextured scenes (ks_device_set_shading)
+    uint32_t tail_paths = 1u << 19;   // k_tail takes over at or below this many paths (ks_device_set_tail)
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
     // pixel-filter sample tables by (type, radius, sigma), kept for the device's lifetime: batched
     // renders keep pointers to them in their per-view constants
@@ -1421,6 +1522,27 @@ template <typename... A> void launch_trace(const DevScene &ds, dim3 g, hipStream
     } else {
         if (ds.width == 4) hipLaunchKernelGGL((k_trace<4, false>), g, b, 0, s, ds, args...);
         else hipLaunchKernelGGL((k_trace<2, false>), g, b, 0, s, ds, args...);
+    }
+}
+template <typename... A> void launch_tail(const DevScene &ds, bool batch, dim3 g, hipStream_t s, A... args) {
+    const dim3 b(256);
+    const bool tex = ds.textured != 0;
+    if (ds.width == 4) {
+        if (batch) {
+            if (tex) hipLaunchKernelGGL((k_tail<4, true, true>), g, b, 0, s, ds, args...);
+            else hipLaunchKernelGGL((k_tail<4, true, false>), g, b, 0, s, ds, args...);
+        } else {
+            if (tex) hipLaunchKernelGGL((k_tail<4, false, true>), g, b, 0, s, ds, args...);
+            else hipLaunchKernelGGL((k_tail<4, false, false>), g, b, 0, s, ds, args...);
+        }
+    } else {
+        if (batch) {
+            if (tex) hipLaunchKernelGGL((k_tail<2, true, true>), g, b, 0, s, ds, args...);
+            else hipLaunchKernelGGL((k_tail<2, true, false>), g, b, 0, s, ds, args...);
+        } else {
+            if (tex) hipLaunchKernelGGL((k_tail<2, false, true>), g, b, 0, s, ds, args...);
+            else hipLaunchKernelGGL((k_tail<2, false, false>), g, b, 0, s, ds, args...);
+        }
     }
 }
 template <typename... A> void launch_anyhit(const DevScene &ds, dim3 g, hipStream_t s, A... args) {
@@ -1696,11 +1818,12 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         hipLaunchKernelGGL(k_iter_done, dim3(1), dim3(1), 0, s, ctrl);
         return KS_OK;
     };
-    // The last shading pass's shadow rays: a traversal of them alone, once the loop has ended.
+    // The last shading pass's shadow rays: a traversal of them alone, once the loop has ended (or before
+    // the tail kernel takes the remaining paths).
     auto flush_shadow = [&](int cur) -> int {
         hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q); // no work left: only moves the queue
         hipEvent_t e = tm.begin(KS_K_SHADOW);
-        launch_trace(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->n_active, (const float4 *)pb[cur].ro,
+        launch_trace(sc->ds, dim3(tgrid), s, (const uint32_t *)&ctrl->none, (const float4 *)pb[cur].ro,
                      (const float4 *)pb[cur].rd, ws.hit.as<float4>(), (const uint32_t *)&ctrl->n_shadow_trace,
                      (const float4 *)ws.sh_o.as<float4>(), (const float4 *)ws.sh_d.as<float4>(),
                      (const uint32_t *)ws.sh_tgt.as<uint32_t>(), (const float4 *)ws.sh_c.as<float4>(), pb[cur].L,
@@ -1732,6 +1855,13 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
     int iterations = 0;
     const int check_every = 8; // empty trailing iterations cost only launch latency
     int status = KS_OK;
+    // The tail: once every work item has started and at most tail_limit paths remain, one k_tail launch
+    // carries them to their ends (eager plain renders of scenes without alpha-tested triangles; not while
+    // rays are captured). The control block is read after every iteration from the first read that finds
+    // the remaining work items within check_every iterations of generation.
+    const uint32_t tail_limit = std::min<uint32_t>(dev->tail_paths, ws.ovf_grid);
+    const bool tail_ok = !graph && !gp && !wo.cap_n && !sc->ds.alpha && tail_limit > 0;
+    bool check_each = tail_ok && total <= (uint64_t)Q * check_every, tail_ran = false;
     while (true) {
         if (graph) {
             hipError_t le = hipGraphLaunch(exec, s);
@@ -1753,7 +1883,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             cur = 1 - cur;
             ++iterations;
         }
-        if (iterations % check_every == 0) {
+        if (iterations % check_every == 0 || check_each) {
             hipError_t me = hipMemcpyAsync(dev->host_ctrl, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s);
             if (me == hipSuccess) me = hipStreamSynchronize(s);
             if (me != hipSuccess) {
@@ -1761,7 +1891,24 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                 status = KS_ERR_HIP;
                 break;
             }
-            if (dev->host_ctrl->n_active == 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) break;
+            const uint32_t n_left = dev->host_ctrl->n_active;
+            if (n_left == 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) break;
+            if (tail_ok && dev->host_ctrl->work_end - dev->host_ctrl->work_next <= (uint64_t)Q * check_every)
+                check_each = true;
+            if (tail_ok && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) {
+                if (n_left <= tail_limit) {
+                    status = flush_shadow(cur); // the last shading pass's shadow rays add to these paths
+                    if (status) break;
+                    hipEvent_t e = tm.begin(KS_K_TAIL);
+                    launch_tail(sc->ds, wo.batch, dim3((n_left + 255) / 256), s, rc, pb[cur],
+                                (const uint32_t *)&ctrl->n_active, ws.samp.as<float4>(), ws.ovf.as<uint32_t>(),
+                                ws.ovf_cap, stat_rows(dev));
+                    tm.end(e);
+                    KS_HIP(hipGetLastError());
+                    tail_ran = true;
+                    break;
+                }
+            }
             if (iterations > 1000000) {
                 ksa::set_error("wavefront loop did not terminate");
                 status = KS_ERR_INVALID;
@@ -1769,7 +1916,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             }
         }
     }
-    if (status == KS_OK) status = flush_shadow(graph ? 0 : cur);
+    if (status == KS_OK && !tail_ran) status = flush_shadow(graph ? 0 : cur);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (g) (void)hipGraphDestroy(g);
     if (status) return status;
@@ -1794,6 +1941,7 @@ int read_stats(ks_device *dev, ks_render_stats *stats) {
     stats->tri_tests_shadow += h[kTrisShadow];
     stats->nonfinite += h[kNonFinite];
     stats->stack_overflows += h[kSpills];
+    stats->rays_tail += h[kRaysTail];
     return KS_OK;
 }
 
@@ -1888,6 +2036,15 @@ int ks_device_set_timing(ks_device *dev, int32_t enable) {
     return ksa::guarded([&]() -> int {
         KS_REQUIRE(dev, "device is null");
         dev->timing = enable ? 1 : 0;
+        return KS_OK;
+    });
+}
+
+int ks_device_set_tail(ks_device *dev, int32_t max_paths) {
+    return ksa::guarded([&]() -> int {
+        KS_REQUIRE(dev, "device is null");
+        KS_REQUIRE(max_paths >= 0, "max_paths must not be negative");
+        dev->tail_paths = (uint32_t)max_paths;
         return KS_OK;
     });
 }

@@ -24,6 +24,7 @@ struct Ctrl {
     uint32_t iteration;
     uint64_t work_next, work_end;
     uint32_t n_shadow_trace; // shadow rays of the previous shading pass, traced by this iteration's traversal
+    uint32_t none;           // always 0: the closest-hit ray count of a traversal of shadow rays alone
     LineCounter n_next;   // survivors appended by shade into the next buffer
     LineCounter n_shadow; // shadow rays appended by shade
     LineCounter n_class[kShadeClasses]; // paths of the current buffer per material class (sorted shading)
@@ -38,6 +39,7 @@ enum StatSlot {
     kTrisShadow,
     kNonFinite,
     kSpills,
+    kRaysTail,
     kStatCount
 };
 
