@@ -577,7 +577,7 @@ def main() -> int:
         except ks.KsError as e:
             scale = {"error": str(e)}
 
-    n_rays = (totals["rays_closest"] + totals["rays_shadow"]) / args.steps
+    n_rays = (totals["rays_closest"] + totals["rays_shadow"] + totals["rays_tail"]) / args.steps
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -608,6 +608,7 @@ def main() -> int:
             "tris_per_closest_ray": round(totals["tri_tests_closest"] / max(totals["rays_closest"], 1), 2),
             "closest_rays_per_step": totals["rays_closest"] // args.steps,
             "shadow_rays_per_step": totals["rays_shadow"] // args.steps,
+            "tail_kernel_rays_per_step": totals["rays_tail"] // args.steps,
         },
         "roofline": roofline,
         "roofline_shade": roofline_shade,
