@@ -12,9 +12,10 @@ constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr int kStackLds = 16;   // LDS entries per lane (power of two)
 constexpr int kTraceBlock = 256; // threads per traversal block
 // trace_rays_kind: below this many lanes still searching for a leaf, a wave with a finished lane stops
-// its inner-node loop to test leaves and refill finished lanes (measured 8-16 best on the 1080p sphere
-// scene: traversal 35.4 -> 31.8 ms per render against waiting for every lane)
-constexpr int kTraverseBusyMin = 12;
+// its inner-node loop to test leaves and refill finished lanes (on the 1080p sphere scene 8-16 measured
+// best: traversal 35.4 -> 31.8 ms per render against waiting for every lane; with the second postponed
+// leaf, 8 gave 28.5 ms, 12 28.6 and 16 28.8)
+constexpr int kTraverseBusyMin = 8;
 
 struct TravRay {
     f3 o, d;
