@@ -281,8 +281,8 @@ __global__ __launch_bounds__(256) void k_raygen(RenderConsts rc, PathBufs cur, c
     camera_ray(vr, x, y, sob, o, d);
     cur.work[slot] = w;
     cur.state[slot] = (uint32_t)sob.dim;
-    cur.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    cur.L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    // beta (1, 1, 1), pdf 0 and L = 0 of a new path are not written: the shading of bounce 0 takes them
+    // without a read (path_beta_L)
     cur.ro[slot] = make_float4(o.x, o.y, o.z, 0.0f);
     cur.rd[slot] = make_float4(d.x, d.y, d.z, kInf);
 }
@@ -614,6 +614,21 @@ KS_DEV float mix_pdf(const float4 *m, f3 w) {
     return p;
 }
 
+// Throughput, pdf of the last BSDF sample and radiance of path slot i: (1, 1, 1), 0 and 0 for a path that
+// has not been shaded yet (bounce 0, whose values ray generation does not write), else the stored ones.
+KS_DEV void path_beta_L(const PathBufs &cur, uint32_t i, uint32_t state, f3 &beta, float &pdf, f3 &L) {
+    if (((state >> 16) & 0xffu) == 0u) {
+        beta = mk(1.0f, 1.0f, 1.0f);
+        pdf = 0.0f;
+        L = mk(0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 b4 = cur.beta[i];
+    beta = ld3(b4);
+    pdf = b4.w;
+    L = ld3(cur.L[i]);
+}
+
 // One vertex of SmallPT::trace (small_pt.cpp:79-176) for the path whose state the caller loaded: work
 // item w, state (Sobol dimension, bounce), throughput beta, pdf of the last BSDF sample, radiance L, the
 // ray direction rd that reached the vertex and its hit h (t, u, v, prim; prim < 0: escaped). Adds
@@ -891,10 +906,7 @@ __global__ __launch_bounds__(256) void k_shade(DevScene sc, RenderConsts rc, Pat
     if (active) {
         w = cur.work[i];
         state = cur.state[i];
-        float4 b4 = cur.beta[i];
-        beta = ld3(b4);
-        pdf_wi_bsdf = b4.w;
-        L = ld3(cur.L[i]);
+        path_beta_L(cur, i, state, beta, pdf_wi_bsdf, L);
         shade_vertex<GUIDED, BATCH, TEX, CLASS>(sc, rc, gs, samp, i, hits[i], ld3(cur.rd[i]), w, state, L, beta,
                                                 pdf_wi_bsdf, survive, shadow, new_o, new_d, sh_origin, sh_dir,
                                                 sh_contrib, sh_tmax, nonfinite, bounce, sob, rec_slot, view);
@@ -948,9 +960,9 @@ __global__ __launch_bounds__(256) void k_tail(DevScene sc, RenderConsts rc, Path
         uint32_t n_nodes = 0, n_tris = 0; // the tail's rays are counted, not their nodes
         const uint32_t w = cur.work[i];
         uint32_t state = cur.state[i];
-        const float4 b4 = cur.beta[i];
-        f3 beta = ld3(b4), L = ld3(cur.L[i]);
-        float pdf = b4.w;
+        f3 beta, L;
+        float pdf;
+        path_beta_L(cur, i, state, beta, pdf, L);
         const float4 o4 = cur.ro[i], d4 = cur.rd[i];
         f3 o = ld3(o4), d = ld3(d4);
         float tmin = o4.w, tmax = d4.w;
