@@ -39,9 +39,11 @@ TRI_BYTES = 48
 REF_RAY_BYTES, REF_HIT_BYTES, REF_NODE_BYTES, REF_TRI_BYTES = 32, 20, 32, 52
 # Shading, per path vertex (one per closest-hit ray): read hit 20 + path state 64 + material 64 +
 # triangle 48 + light record 48 + light CDF 4 per binary-search step; write path state 64 + extension
-# ray 32. Per shadow ray: write the ray 32 + its contribution 12.
+# ray 32. Per shadow ray: write the ray 32 + its contribution 12. A path's first vertex reads 32 B less:
+# its throughput (1, 1, 1) and radiance 0 are constants, not stored state.
 SHADE_VERTEX_BYTES = 20 + 64 + 64 + 48 + 48 + 64 + 32
 SHADE_SHADOW_BYTES = 32 + 12
+SHADE_FIRST_VERTEX_SAVED = 32
 
 METRIC = "time-to-relMSE 0.01 guided vs plain, Mpaths/s, %HBM peak on 1 MI355X"
 
@@ -528,7 +530,8 @@ def main() -> int:
     shade_launches = totals["launches"]["shade"]
     shade_ms = totals["kernel_ms"]["shade"]
     cdf_steps = max(1, math.ceil(math.log2(info["n_lights"] + 1)))
-    shade_bytes = (SHADE_VERTEX_BYTES + 4 * cdf_steps) * totals["rays_closest"] + SHADE_SHADOW_BYTES * totals["rays_shadow"]
+    shade_bytes = (SHADE_VERTEX_BYTES + 4 * cdf_steps) * totals["rays_closest"] + SHADE_SHADOW_BYTES * totals["rays_shadow"] \
+        - SHADE_FIRST_VERTEX_SAVED * totals["paths"]
     shade_achieved = shade_bytes / (shade_ms / 1e3) / 1e9 if shade_ms > 0 else 0.0
     roofline_shade = {
         "bound": "hbm",
