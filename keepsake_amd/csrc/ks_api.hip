@@ -178,7 +178,7 @@ KS_DEV uint32_t samp_index(const RenderConsts &rc, uint32_t w) {
 }
 
 KS_DEV void sobol_start(const RenderConsts &rc, Sobol &sob, uint32_t x, uint32_t y, uint32_t s, int dim) {
-    sob.start(x, y, s, dim, rc.seed, rc.log2spp, rc.ndigits);
+    sob.start(x, y, s, dim, rc.seed, rc.log2spp, rc.ndigits, rc.perm_tab);
 }
 
 // The camera ray's differential directions (camera.cpp:146-167): one pixel step along x and y on the
@@ -1165,13 +1165,21 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *partials, un
     if (threadIdx.x == 0) *total = red[0];
 }
 
+// The sampler's permutation table: one thread per (dimension, digit prefix) entry, kPermTabDims x
+// kPermTabPrefixes threads in all.
+__global__ __launch_bounds__(256) void k_perm_tab(uint8_t *tab) {
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t dim = e / kPermTabPrefixes, h = e % kPermTabPrefixes;
+    tab[e] = (uint8_t)perm4_byte(perm_index(h ^ (0x55555555u * dim)));
+}
+
 __global__ void k_sobol_stream(int64_t n, const int32_t *px, const int32_t *py, const int32_t *sample, int32_t seed,
                                int32_t log2spp, int32_t ndigits, int32_t n_calls, const int32_t *kinds,
-                               int32_t n_values, uint32_t *out_u32, float *out_f) {
+                               int32_t n_values, const uint8_t *perm_tab, uint32_t *out_u32, float *out_f) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Sobol sob;
-    sob.start((uint32_t)px[i], (uint32_t)py[i], (uint32_t)sample[i], 0, seed, log2spp, ndigits);
+    sob.start((uint32_t)px[i], (uint32_t)py[i], (uint32_t)sample[i], 0, seed, log2spp, ndigits, perm_tab);
     int64_t base = i * n_values;
     int k = 0;
     for (int c = 0; c < n_calls; ++c) {
@@ -1419,6 +1427,7 @@ struct ks_device {
     Ctrl *host_ctrl = nullptr; // pinned
     uint8_t *host_scratch = nullptr; // pinned, 64 bytes
     double *relmse_scratch = nullptr; // this device's address of `scratch` (hipGetSymbolAddress at open)
+    DevBuf perm_tab; // the sampler's permutation bytes (kPermTabDims x kPermTabPrefixes), filled at open
     int sorted_shading = 0;           // material-sorted shading of untextured scenes (ks_device_set_shading)
     uint32_t tail_paths = 1u << 19;   // k_tail takes over at or below this many paths (ks_device_set_tail)
     std::vector<hipEvent_t> events; // timing event pool, reused across renders
@@ -1709,6 +1718,7 @@ int fill_consts(ks_device *dev, const ks_scene *sc, const ks_camera_desc *cam, c
     rc.spp = p->spp;
     rc.seed = p->rng_seed;
     sobol_consts(rc.W, rc.H, rc.spp, rc.log2spp, rc.ndigits);
+    rc.perm_tab = dev->perm_tab.as<uint8_t>();
     KS_REQUIRE(2 * rc.ndigits <= 52 && (uint64_t)1 << rc.log2spp <= (uint64_t)rc.spp,
                "resolution x spp exceeds the sampler index range");
     rc.bounces = p->bounces;
@@ -2025,6 +2035,11 @@ int ks_device_open(int32_t ordinal, ks_device **out) {
         void *sp = nullptr;
         KS_HIP(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
         d->relmse_scratch = static_cast<double *>(sp);
+        KS_HIP(d->perm_tab.alloc((size_t)kPermTabDims * kPermTabPrefixes));
+        hipLaunchKernelGGL(k_perm_tab, dim3(kPermTabDims * kPermTabPrefixes / 256), dim3(256), 0, d->stream,
+                           d->perm_tab.as<uint8_t>());
+        KS_HIP(hipGetLastError());
+        KS_HIP(hipStreamSynchronize(d->stream));
         *out = d.release();
         return KS_OK;
     });
@@ -3463,7 +3478,8 @@ int ks_sobol_stream(ks_device *dev, int32_t width, int32_t height, int32_t spp, 
         KS_HIP(ou.alloc(sizeof(uint32_t) * n * n_values));
         KS_HIP(of.alloc(sizeof(float) * n * n_values));
         hipLaunchKernelGGL(k_sobol_stream, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dpx.p(), dpy.p(),
-                           ds.p(), seed, log2spp, ndigits, n_calls, dk.p(), n_values, ou.as<uint32_t>(), of.as<float>());
+                           ds.p(), seed, log2spp, ndigits, n_calls, dk.p(), n_values, dev->perm_tab.as<uint8_t>(),
+                           ou.as<uint32_t>(), of.as<float>());
         KS_HIP(hipGetLastError());
         KS_HIP(hipMemcpyAsync(out_u32, ou.p, sizeof(uint32_t) * n * n_values, hipMemcpyDeviceToHost, s));
         KS_HIP(hipMemcpyAsync(out_f, of.p, sizeof(float) * n * n_values, hipMemcpyDeviceToHost, s));

@@ -134,20 +134,31 @@ KS_DEV uint32_t perm_index(uint32_t x) {
     return ((uint32_t)(t >> 32) * 16u + (uint32_t)t % 24u) % 24u;
 }
 
+// Permutation bytes by (dimension, digit prefix): entry dim * kPermTabPrefixes + h is
+// perm4_byte(perm_index(h ^ 0x55555555 * dim)). A render's morton index is below 4^n_digits (2^(2 n_digits - 1)
+// for an odd log2 spp), so the prefix (the digits above) of each of the top kPermTabLevels digits has at
+// most 12 bits and is looked up; the lower digits' prefixes are hashed. The table is a constant, built
+// once per device.
+constexpr uint32_t kPermTabDims = 256;
+constexpr uint32_t kPermTabPrefixes = 4096;
+constexpr int kPermTabLevels = 7;
+
 struct Sobol {
     uint32_t morton; // the reference's morton index is built from 32-bit parts (sobol.h:142-146)
     int32_t dim;
     int32_t seed;
     int32_t log2_spp;
     int32_t n_digits;
+    const uint8_t *tab; // the permutation table above, or nullptr: every digit is hashed
 
     // PTRenderSampler ctor: start_pixel with dimension 0 (sobol.h:142-146, 237-244)
     KS_DEV void start(uint32_t px, uint32_t py, uint32_t sample, int32_t dim0, int32_t seed_, int32_t log2spp,
-                      int32_t ndigits) {
+                      int32_t ndigits, const uint8_t *tab_ = nullptr) {
         seed = seed_;
         log2_spp = log2spp;
         n_digits = ndigits;
         dim = dim0;
+        tab = tab_;
         morton = (((part_1_by_1(py) << 1) + part_1_by_1(px)) << log2spp) | sample; // 32-bit as reference
     }
 
@@ -156,7 +167,29 @@ struct Sobol {
         const bool pow2 = log2_spp & 1;
         const int last = pow2 ? 1 : 0;
         const uint32_t dmask = 0x55555555u * (uint32_t)dim;
-        for (int i = n_digits - 1; i >= last; --i) {
+        int i = n_digits - 1;
+        if (tab != nullptr && (uint32_t)dim < kPermTabDims) {
+            // the top digits: their table loads are issued together, then applied
+            const uint8_t *t = tab + (size_t)dim * kPermTabPrefixes;
+            uint32_t pb[kPermTabLevels];
+#pragma unroll
+            for (int k = 0; k < kPermTabLevels; ++k) {
+                const int lv = n_digits - 1 - k;
+                const int shift = 2 * lv - (pow2 ? 1 : 0);
+                const uint32_t higher = lv >= last && shift + 2 < 32 ? morton >> (shift + 2) : 0u;
+                pb[k] = higher < kPermTabPrefixes ? t[higher] : perm4_byte(perm_index(higher ^ dmask));
+            }
+#pragma unroll
+            for (int k = 0; k < kPermTabLevels; ++k) {
+                const int lv = n_digits - 1 - k;
+                if (lv < last) break;
+                const int shift = 2 * lv - (pow2 ? 1 : 0);
+                const uint32_t digit = shift < 32 ? (morton >> shift) & 3u : 0u;
+                si |= (uint64_t)((pb[k] >> (2 * digit)) & 3u) << shift;
+            }
+            i = n_digits - 1 - kPermTabLevels;
+        }
+        for (; i >= last; --i) {
             int shift = 2 * i - (pow2 ? 1 : 0);
             // digits above bit 31 of the 32-bit morton index are zero
             uint32_t digit = shift < 32 ? (morton >> shift) & 3u : 0u;

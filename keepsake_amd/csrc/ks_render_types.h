@@ -89,6 +89,7 @@ struct RenderConsts {
     float filter_r;
     int32_t filter_n; // 0: box filter
     const float *filter_cdf;
+    const uint8_t *perm_tab; // the sampler's permutation bytes by (dimension, digit prefix), or nullptr
 };
 
 struct PathBufs {

@@ -37,6 +37,24 @@ def test_sobol_bit_exact(device, ora, W, H, spp, seed):
         assert np.array_equal(gf[i].view(np.uint32), of.view(np.uint32))
 
 
+@pytest.mark.parametrize("W,H,spp,seed", [(1920, 1080, 64, 0), (640, 480, 8, 2), (3840, 2160, 32, 1)])
+def test_sobol_bit_exact_deep_dimensions(device, ora, W, H, spp, seed):
+    """Streams past dimension 256: the device looks up the top digits' permutations for the first 256
+    dimensions and hashes every digit beyond; both must match the oracle bit for bit."""
+    _log("sobol deep", W, H, spp, seed)
+    rng = np.random.default_rng(7 * W + spp)
+    n = 96
+    px = rng.integers(0, W, n).astype(np.int32)
+    py = rng.integers(0, H, n).astype(np.int32)
+    s = rng.integers(0, spp, n).astype(np.int32)
+    kinds = np.array([2, 1, 1] * 70, np.int32)  # 280 dimensions
+    gu, gf = device.sobol_stream(W, H, spp, seed, px, py, s, kinds)
+    for i in range(n):
+        ou, of = ora.sobol(W, H, spp, seed, int(px[i]), int(py[i]), int(s[i]), kinds)
+        assert np.array_equal(gu[i], ou), (i, px[i], py[i], s[i])
+        assert np.array_equal(gf[i].view(np.uint32), of.view(np.uint32))
+
+
 def pinhole_dirs(ora, cam, W, H, spp, seed, px, py, s):
     """Closed-form pinhole direction in FP64: R (tan(v/2) a x_ndc, -tan(v/2) y_ndc, -1), R = look_at
     basis, with the box-filter jitter of the (bit-exact) sampler."""
