@@ -266,6 +266,8 @@ typedef void (*ks_progress_fn)(const ks_film *film, int32_t spp_finished, void *
 /* ---- device ---------------------------------------------------------------------------------- */
 const char *ks_last_error(void);
 int ks_device_count(int32_t *count);
+/* Opens a device handle: its stream, pinned control blocks and the sampler's constant permutation
+   table (128 MiB of device memory, filled once here and freed by ks_device_close). */
 int ks_device_open(int32_t ordinal, ks_device **out);
 int ks_device_close(ks_device *dev);
 /* Record HIP-event kernel timings into ks_render_stats (costs one event pair per launch). */

@@ -137,11 +137,12 @@ KS_DEV uint32_t perm_index(uint32_t x) {
 // Permutation bytes by (dimension, digit prefix): entry dim * kPermTabPrefixes + h is
 // perm4_byte(perm_index(h ^ 0x55555555 * dim)). A render's morton index is below 4^n_digits (2^(2 n_digits - 1)
 // for an odd log2 spp), so the prefix (the digits above) of each of the top kPermTabLevels digits has at
-// most 12 bits and is looked up; the lower digits' prefixes are hashed. The table is a constant, built
-// once per device.
-constexpr uint32_t kPermTabDims = 256;
-constexpr uint32_t kPermTabPrefixes = 4096;
-constexpr int kPermTabLevels = 7;
+// most 20 bits and is looked up; the lower digits' prefixes, and every digit of dimensions from
+// kPermTabDims on (paths of more than about 17 bounces), are hashed. The table is a constant of 128 MiB,
+// built once per device.
+constexpr uint32_t kPermTabDims = 128;
+constexpr uint32_t kPermTabPrefixes = 1u << 20;
+constexpr int kPermTabLevels = 11;
 
 struct Sobol {
     uint32_t morton; // the reference's morton index is built from 32-bit parts (sobol.h:142-146)

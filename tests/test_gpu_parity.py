@@ -39,7 +39,7 @@ def test_sobol_bit_exact(device, ora, W, H, spp, seed):
 
 @pytest.mark.parametrize("W,H,spp,seed", [(1920, 1080, 64, 0), (640, 480, 8, 2), (3840, 2160, 32, 1)])
 def test_sobol_bit_exact_deep_dimensions(device, ora, W, H, spp, seed):
-    """Streams past dimension 256: the device looks up the top digits' permutations for the first 256
+    """Streams of 280 dimensions: the device looks up the top digits' permutations for the first 128
     dimensions and hashes every digit beyond; both must match the oracle bit for bit."""
     _log("sobol deep", W, H, spp, seed)
     rng = np.random.default_rng(7 * W + spp)
