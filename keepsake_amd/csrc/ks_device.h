@@ -89,25 +89,20 @@ KS_DEV uint32_t owen_scramble(uint32_t v, uint32_t seed) { // FastOwenScrambler,
 
 // Sobol generator matrices of dimensions 0 and 1 applied to index a (sobol.h:35-49). Dimension 0 is
 // the van der Corput matrix (bit reversal of the low 32 index bits; words 32..51 of its column are
-// zero). Dimension 1 has v_i = v_{i-1} ^ (v_{i-1} >> 1) with period 32 in the 52-word column.
+// zero). Dimension 1 has v_i = v_{i-1} ^ (v_{i-1} >> 1) with period 32 in the 52-word column, so index
+// bits 32..51 act like bits 0..19 and the product is linear in lo ^ hi. Its column i is (I + S)^i of the
+// top bit (S: shift right by one), whose bit 31 - k is C(i, k) mod 2: by Lucas' theorem, odd exactly when
+// the bits of k are a subset of those of i. Output bit 31 - k is therefore the XOR of the index bits at
+// the supersets i of k: five butterfly steps over the word, then a bit reversal.
 KS_DEV uint32_t sobol_dim0(uint64_t a) { return __builtin_bitreverse32((uint32_t)a); }
 KS_DEV uint32_t sobol_dim1(uint64_t a) {
-    uint32_t v = 0, col = 0x80000000u;
-    uint32_t lo = (uint32_t)a;
-    for (int i = 0; i < 32; ++i) {
-        v ^= (lo & 1u) ? col : 0u;
-        lo >>= 1;
-        col ^= col >> 1;
-    }
-    // index bits 32..51 reuse columns 0..19
-    uint32_t hi = (uint32_t)(a >> 32) & 0xfffffu;
-    col = 0x80000000u;
-    for (int i = 0; hi != 0; ++i) {
-        v ^= (hi & 1u) ? col : 0u;
-        hi >>= 1;
-        col ^= col >> 1;
-    }
-    return v;
+    uint32_t v = (uint32_t)a ^ ((uint32_t)(a >> 32) & 0xfffffu);
+    v ^= (v >> 1) & 0x55555555u;
+    v ^= (v >> 2) & 0x33333333u;
+    v ^= (v >> 4) & 0x0f0f0f0fu;
+    v ^= (v >> 8) & 0x00ff00ffu;
+    v ^= (v >> 16) & 0x0000ffffu;
+    return __builtin_bitreverse32(v);
 }
 KS_DEV float u32_to_unit(uint32_t v) { return fminf((float)v * 0x1p-32f, kBeforeOne); }
 

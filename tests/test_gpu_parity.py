@@ -20,7 +20,10 @@ def _log(*a):
 
 @pytest.mark.parametrize(
     "W,H,spp,seed",
-    [(256, 256, 16, 0), (1920, 1080, 64, 0), (1280, 720, 8, 3), (64, 48, 1, 5), (100, 37, 2, 9), (3840, 2160, 32, 1)],
+    [(256, 256, 16, 0), (1920, 1080, 64, 0), (1280, 720, 8, 3), (64, 48, 1, 5), (100, 37, 2, 9), (3840, 2160, 32, 1),
+     # 17 and 22 base-4 digits: sample indices above 2^32, where dimension 1's index bits 32 and up
+     # reuse the low columns
+     (4096, 4096, 1024, 2), (65536, 4, 4096, 1)],
 )
 def test_sobol_bit_exact(device, ora, W, H, spp, seed):
     _log("sobol", W, H, spp, seed)
