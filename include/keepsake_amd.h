@@ -219,7 +219,8 @@ typedef struct ks_small_pt_params {
     int32_t spp_prog_interval;
     int32_t mis_mode;
     int32_t guide_mode;
-    int32_t queue_size; /* wavefront path slots; 0 = automatic */
+    int32_t queue_size; /* wavefront path slots; 0 = automatic: the pass's paths, at most 2^26 (2^23 for
+                           guided renders) */
     /* SmallPTInput::pixel_filter (create_pixel_filter, render_target.cpp:28-44; render_target.h:79-184):
      * KS_FILTER_BOX draws the jitter uniformly over [-w/2, w/2]^2; the gaussian and Blackman-Harris
      * filters draw it from their 32-per-unit sample table (PixelFilterSampleTable). */
@@ -282,7 +283,8 @@ int ks_device_set_shading(ks_device *dev, int32_t sorted);
  * end): once every work item has started and at most max_paths paths remain, one kernel carries each of
  * them to its end in one thread instead of one wavefront iteration per bounce. Results are the same.
  * Applies to plain renders (not guided ones, not HIP-graph batches) of scenes without alpha-tested
- * triangles; 0 turns it off. Default 2^19. */
+ * triangles; 0 turns it off. Default 0: with the default 2^26-path queue the wavefront's last iterations
+ * cost less than the tail kernel (40.5 against 42.0 ms for a 1080p 64-spp render). */
 int ks_device_set_tail(ks_device *dev, int32_t max_paths);
 
 /* ---- scene: replaces create_scene + Scene::create_rtc_scene (rtcCommitScene, scene.cpp:144-160),

@@ -31,7 +31,7 @@ def test_tail_equals_wavefront(ks, device, name, seed, bounces, extra):
     # every remaining path once all work items have started (the control block is read after every
     # iteration near the end of generation, so paths are still in flight)
     (b, wb), sb = _render(ks, device, sc, cam, p, W, H, 1 << 30)
-    device.set_tail(1 << 19)  # the default
+    device.set_tail(0)  # the default
     print(name, "tail rays", sb["rays_tail"], "wavefront rays", sa["rays_closest"] + sa["rays_shadow"],
           "iterations", sa["iterations"], sb["iterations"], flush=True)
     assert sa["rays_tail"] == 0 and sb["rays_tail"] > 0
@@ -71,7 +71,7 @@ def test_tail_textured_and_batched(ks, device, cornell):
         out.append(([f.download() for f in films], st))
         for f in films:
             f.close()
-    device.set_tail(1 << 19)
+    device.set_tail(0)
     print("batched tail rays", out[1][1]["rays_tail"], flush=True)
     assert out[0][1]["rays_tail"] == 0 and out[1][1]["rays_tail"] > 0
     for (x, wx), (y, wy) in zip(out[0][0], out[1][0]):
