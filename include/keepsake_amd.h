@@ -219,8 +219,7 @@ typedef struct ks_small_pt_params {
     int32_t spp_prog_interval;
     int32_t mis_mode;
     int32_t guide_mode;
-    int32_t queue_size; /* wavefront path slots; 0 = automatic: the pass's paths, at most 2^26 (2^23 for
-                           guided renders) */
+    int32_t queue_size; /* wavefront path slots; 0 = automatic: the pass's paths, at most 2^26 */
     /* SmallPTInput::pixel_filter (create_pixel_filter, render_target.cpp:28-44; render_target.h:79-184):
      * KS_FILTER_BOX draws the jitter uniformly over [-w/2, w/2]^2; the gaussian and Blackman-Harris
      * filters draw it from their 32-per-unit sample table (PixelFilterSampleTable). */

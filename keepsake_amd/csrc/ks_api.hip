@@ -1967,12 +1967,12 @@ int read_stats(ks_device *dev, ks_render_stats *stats) {
     return KS_OK;
 }
 
-uint32_t pick_queue(const ks_small_pt_params *p, uint64_t total, uint64_t dflt = 1ull << 26) {
+uint32_t pick_queue(const ks_small_pt_params *p, uint64_t total) {
     // 2^26 in-flight paths (about 14 GB of path state): each persistent traversal launch then drains
     // once per 2^26 rays rather than per 2^23, and a 1080p 64-spp render takes 16 iterations instead of
     // 32. Measured on the 1080p 64-spp sphere scene: 43.6 ms per render at 2^23, 42.0 at 2^24, 41.2 at
     // 2^25, 40.5 at 2^26 (2^21 was 1.34x slower than 2^23).
-    uint64_t q = p->queue_size > 0 ? (uint64_t)p->queue_size : dflt;
+    uint64_t q = p->queue_size > 0 ? (uint64_t)p->queue_size : (1ull << 26);
     q = std::min<uint64_t>(q, std::max<uint64_t>(total, 256));
     q = std::max<uint64_t>(q, 256);
     return (uint32_t)std::min<uint64_t>(q, 1ull << 26);
@@ -3044,8 +3044,7 @@ int ks_small_pt_render_guided(ks_scene *scene, const ks_camera_desc *cam, const 
         const uint64_t max_samp = kSampleBufferBytes / sizeof(float4);
         const int spp_chunk = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)interval, max_samp / pixels));
         const uint64_t max_items = pixels * (uint64_t)std::min(spp_chunk, params->spp);
-        // guided renders keep 2^23 slots by default: each slot also holds a 160 B guide mixture
-        uint32_t Q = pick_queue(params, max_items, 1ull << 23);
+        uint32_t Q = pick_queue(params, max_items); // each slot also holds a 160 B guide mixture
         e = ensure_workspace(dev, Q, max_items, scene->max_stack);
         if (e) return e;
         GuideWork &gw = dev->gw;
