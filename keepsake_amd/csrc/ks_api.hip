@@ -1787,6 +1787,10 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         pb[b] = PathBufs{ws.work[b].as<uint32_t>(), ws.state[b].as<uint32_t>(), ws.beta[b].as<float4>(),
                          ws.L[b].as<float4>(),     ws.ro[b].as<float4>(),     ws.rd[b].as<float4>()};
     const int tblocks = (int)((Q + 255) / 256);
+    // Blocks of ray generation and shading: one thread per queue slot until a control read bounds the
+    // work left. From a read on, an iteration holds at most the paths then active plus the work items not
+    // yet started, and generates at most the latter, so the late iterations' launches stay small.
+    int gen_blocks = tblocks, shade_blocks = tblocks;
     const int tgrid = trace_grid(dev, sc->ds, Q, wo.blocks_per_cu);
     const bool graph = wo.graph && !gp;
     // One iteration reading buffer cur and writing buffer 1 - cur; no events while a graph is captured.
@@ -1795,8 +1799,8 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
     // per-sample results, before this iteration's shading reads the slots and refills the shadow queue.
     auto iteration = [&](int cur) -> int {
         hipEvent_t e = graph ? nullptr : tm.begin(KS_K_RAYGEN);
-        if (wo.batch) hipLaunchKernelGGL(k_raygen<true>, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
-        else hipLaunchKernelGGL(k_raygen<false>, dim3(tblocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
+        if (wo.batch) hipLaunchKernelGGL(k_raygen<true>, dim3(gen_blocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
+        else hipLaunchKernelGGL(k_raygen<false>, dim3(gen_blocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
         hipLaunchKernelGGL(k_gen_done, dim3(1), dim3(1), 0, s, ctrl, Q);
         tm.end(e);
         if (wo.cap_n) {
@@ -1825,7 +1829,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         if (sorted)
             hipLaunchKernelGGL(k_classify, dim3(tblocks), dim3(256), 0, s, sc->ds, (const float4 *)ws.hit.as<float4>(),
                                ctrl, ws.order.as<uint32_t>(), Q);
-        launch_shade(sc->ds.textured != 0, gp != nullptr, wo.batch, sorted, dim3(tblocks), s, sc->ds, rc, pb[cur],
+        launch_shade(sc->ds.textured != 0, gp != nullptr, wo.batch, sorted, dim3(shade_blocks), s, sc->ds, rc, pb[cur],
                      pb[1 - cur], ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
                      ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st,
                      gp ? gp->shade : GuideShade{}, (const uint32_t *)ws.order.as<uint32_t>(), Q);
@@ -1905,7 +1909,9 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             cur = 1 - cur;
             ++iterations;
         }
-        if (iterations % check_every == 0 || check_each) {
+        // the control block is also read after iterations 1, 2 and 4, so that the launch bounds tighten early
+        const bool early_read = !graph && (iterations == 1 || iterations == 2 || iterations == 4);
+        if (iterations % check_every == 0 || check_each || early_read) {
             hipError_t me = hipMemcpyAsync(dev->host_ctrl, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, s);
             if (me == hipSuccess) me = hipStreamSynchronize(s);
             if (me != hipSuccess) {
@@ -1915,6 +1921,11 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             }
             const uint32_t n_left = dev->host_ctrl->n_active;
             if (n_left == 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) break;
+            if (!graph) {
+                const uint64_t todo = dev->host_ctrl->work_end - dev->host_ctrl->work_next;
+                gen_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(tblocks, (todo + 255) / 256));
+                shade_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(tblocks, (n_left + todo + 255) / 256));
+            }
             if (tail_ok && dev->host_ctrl->work_end - dev->host_ctrl->work_next <= (uint64_t)Q * check_every)
                 check_each = true;
             if (tail_ok && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) {
