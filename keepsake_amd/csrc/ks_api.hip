@@ -1819,7 +1819,9 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         tm.end(e);
         if (gp && gp->shade.mix) {
             e = tm.begin(KS_K_GUIDE);
-            int ge = ksg::launch_infer(gp->guide, sc->ds, &ctrl->n_active, Q, pb[cur].rd, ws.hit.as<float4>(),
+            // the guided queries are among the paths shading takes: the same launch bound holds
+            const uint32_t n_max = (uint32_t)std::min<uint64_t>(Q, (uint64_t)shade_blocks * 256u);
+            int ge = ksg::launch_infer(gp->guide, sc->ds, &ctrl->n_active, n_max, pb[cur].rd, ws.hit.as<float4>(),
                                        pb[cur].state, gp->shade.max_bounce, gp->mix, gp->list, gp->n_list, s);
             tm.end(e);
             if (ge) return ge;
