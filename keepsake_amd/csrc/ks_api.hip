@@ -1797,6 +1797,12 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
     // Its traversal also traces the previous shading pass's shadow rays (k_trace): their contributions go
     // to the survivors' slots [0, n_next) of buffer cur (ray generation fills the slots after them) and to
     // per-sample results, before this iteration's shading reads the slots and refills the shadow queue.
+    // Guide inference serves only vertices below the guide's max bounce. A path generated in iteration g
+    // is at bounce i - g in iteration i, so once every work item has started by iteration last_gen (known
+    // from the start when the queue holds the whole pass, else from a control read), iterations at or
+    // beyond last_gen + max bounce have no guided vertex and skip the inference.
+    int iter_index = 0;
+    int last_gen = total <= (uint64_t)Q ? 0 : -1; // -1: not known yet
     auto iteration = [&](int cur) -> int {
         hipEvent_t e = graph ? nullptr : tm.begin(KS_K_RAYGEN);
         if (wo.batch) hipLaunchKernelGGL(k_raygen<true>, dim3(gen_blocks), dim3(256), 0, s, rc, pb[cur], ctrl, Q);
@@ -1817,7 +1823,8 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                      ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(), ws.ovf_cap,
                      stat_rows(dev));
         tm.end(e);
-        if (gp && gp->shade.mix) {
+        const bool guided_vertices = last_gen < 0 || iter_index - last_gen < (gp ? gp->shade.max_bounce : 0);
+        if (gp && gp->shade.mix && guided_vertices) {
             e = tm.begin(KS_K_GUIDE);
             // the guided queries are among the paths shading takes: the same launch bound holds
             const uint32_t n_max = (uint32_t)std::min<uint64_t>(Q, (uint64_t)shade_blocks * 256u);
@@ -1910,6 +1917,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             }
             cur = 1 - cur;
             ++iterations;
+            iter_index = iterations;
         }
         // the control block is also read after iterations 1, 2 and 4, so that the launch bounds tighten early
         const bool early_read = !graph && (iterations == 1 || iterations == 2 || iterations == 4);
@@ -1923,6 +1931,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
             }
             const uint32_t n_left = dev->host_ctrl->n_active;
             if (n_left == 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) break;
+            if (last_gen < 0 && dev->host_ctrl->work_next >= dev->host_ctrl->work_end) last_gen = iterations - 1;
             if (!graph) {
                 const uint64_t todo = dev->host_ctrl->work_end - dev->host_ctrl->work_next;
                 gen_blocks = (int)std::max<uint64_t>(1, std::min<uint64_t>(tblocks, (todo + 255) / 256));
