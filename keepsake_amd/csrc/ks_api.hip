@@ -1823,7 +1823,11 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
                      ws.samp.as<float4>(), gp ? gp->shade.recs : (float4 *)nullptr, ws.ovf.as<uint32_t>(), ws.ovf_cap,
                      stat_rows(dev));
         tm.end(e);
-        const bool guided_vertices = last_gen < 0 || iter_index - last_gen < (gp ? gp->shade.max_bounce : 0);
+        auto young = [&](int b) { return last_gen < 0 || iter_index - last_gen < b; };
+        const bool guided_vertices = gp && young(gp->shade.max_bounce);
+        // without guided vertices and training records (the first kRecVertices vertices of a path) the
+        // guided shading is the plain one, whose kernel holds fewer registers
+        const bool guided_shade = gp && (guided_vertices || (gp->shade.recs && young(kRecVertices)));
         if (gp && gp->shade.mix && guided_vertices) {
             e = tm.begin(KS_K_GUIDE);
             // the guided queries are among the paths shading takes: the same launch bound holds
@@ -1838,7 +1842,7 @@ int run_wavefront(ks_scene *sc, RenderConsts rc, uint64_t total, uint32_t Q, Tim
         if (sorted)
             hipLaunchKernelGGL(k_classify, dim3(tblocks), dim3(256), 0, s, sc->ds, (const float4 *)ws.hit.as<float4>(),
                                ctrl, ws.order.as<uint32_t>(), Q);
-        launch_shade(sc->ds.textured != 0, gp != nullptr, wo.batch, sorted, dim3(shade_blocks), s, sc->ds, rc, pb[cur],
+        launch_shade(sc->ds.textured != 0, guided_shade, wo.batch, sorted, dim3(shade_blocks), s, sc->ds, rc, pb[cur],
                      pb[1 - cur], ws.hit.as<float4>(), ctrl, ws.sh_o.as<float4>(), ws.sh_d.as<float4>(),
                      ws.sh_tgt.as<uint32_t>(), ws.sh_c.as<float4>(), ws.samp.as<float4>(), st,
                      gp ? gp->shade : GuideShade{}, (const uint32_t *)ws.order.as<uint32_t>(), Q);
