@@ -343,7 +343,7 @@ typedef struct ks_batch_view {
     ks_film *film;
 } ks_batch_view;
 typedef struct ks_batch_options {
-    int32_t queue_size;          /* 0: autotune over 2^18..2^22 (cached per scene and workload size) */
+    int32_t queue_size;          /* 0: autotune over 2^18..2^25 (cached per scene and workload size) */
     int32_t trace_blocks_per_cu; /* with a fixed queue_size: traversal blocks per CU, 1..8 (8) */
     int32_t disable_graph;       /* 1: launch every iteration eagerly instead of replaying a HIP graph */
     int32_t max_chunk_items;     /* work items (path samples) per chunk; 0: 2^28, i.e. 4 GiB of per-sample results.

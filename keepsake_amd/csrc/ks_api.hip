@@ -3270,8 +3270,8 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
         wo.batch = true;
         wo.graph = o.disable_graph == 0;
         uint32_t Q = 0;
-        // autotuner: queue size 2^18..2^22 x traversal blocks per CU {4, 8}, timed on the first chunk's first
-        // items (at most 2^24), cached per scene and workload size
+        // autotuner: queue size 2^18..2^25 x traversal blocks per CU {4, 8}, timed on the first chunk's first
+        // items (at most 2^25), cached per scene and workload size
         if (o.queue_size > 0) {
             Q = (uint32_t)std::min<int64_t>(o.queue_size, 1 << 26);
             wo.blocks_per_cu = o.trace_blocks_per_cu > 0 ? o.trace_blocks_per_cu : kMaxTraceBlocksPerCu;
@@ -3281,7 +3281,7 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
         } else {
             const BatchChunk &c0 = chunks.front();
             BatchChunk probe;
-            for (size_t k = 0; k < c0.segs.size() && probe.items < (1ull << 24); ++k) {
+            for (size_t k = 0; k < c0.segs.size() && probe.items < (1ull << 25); ++k) {
                 probe.segs.push_back(c0.segs[k]);
                 probe.items += c0.seg_items[k];
             }
@@ -3298,7 +3298,7 @@ int ks_small_pt_render_batch(ks_scene *scene, int32_t n_views, const ks_batch_vi
             KS_HIP(hipEventCreate(&eb));
             float best = 1e30f;
             const int bpcs[2] = {4, 8};
-            for (int lq = 18; lq <= 22; ++lq) {
+            for (int lq = 18; lq <= 25; ++lq) {
                 uint32_t q = (uint32_t)std::min<uint64_t>(1ull << lq, std::max<uint64_t>(probe.items, 256));
                 for (int bpc : bpcs) {
                     int e = ensure_workspace(dev, q, probe.items, scene->max_stack);
