@@ -448,6 +448,15 @@ def main() -> int:
     elapsed = sum(t_steps)
     paths = args.width * args.height * args.spp
     value = paths * args.steps / elapsed / 1e6
+    # the same renders again without the per-launch HIP events that the timed steps record for the
+    # roofline: their cost is in `value`, not in this figure
+    t_untimed = []
+    for i in range(args.steps):
+        film.clear()
+        t0 = time.perf_counter()
+        ks.small_pt_render(sc, cam, params, film)
+        t_untimed.append(time.perf_counter() - t0)
+    untimed = paths * args.steps / sum(t_untimed) / 1e6
 
     # Implementation-independent traversal bytes: the reference traversal order (SBVH::intersect /
     # intersectBool over the BVH2 SBVH) counted on the host over every ray that a random sample of the
@@ -605,6 +614,7 @@ def main() -> int:
             "bvh": f"{BVH_NAME[builder]}, BVH{info['branching']} nodes",
         },
         "mrays_per_s": round(n_rays / (elapsed / args.steps) / 1e6, 1),
+        "mpaths_per_s_without_timing_events": round(untimed, 1),
         "kernel_ms_per_step": {k: round(v / args.steps, 2) for k, v in totals["kernel_ms"].items()},
         "traversal": {
             "nodes_per_closest_ray": round(totals["node_visits_closest"] / max(totals["rays_closest"], 1), 2),
